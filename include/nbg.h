@@ -1,0 +1,162 @@
+/*
+ * nbg.h -- C ABI of the MI355X gravity library (libnbg_hip.so).
+ *
+ * Plain C: pointers, sizes and POD structs only; no C++ types or exceptions cross it.
+ * A context owns one HIP stream and every device buffer of the path.  Host arrays are
+ * caller-owned.  A context is single-threaded; distinct contexts may run concurrently.
+ * Every call returns NBG_OK (0) or a negative NBG_E_* code; nbg_last_error(ctx) gives
+ * the message of the last failure on that context.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the
+ * reference repository root, HasanHelal429/OpenGL-Physics).
+ */
+#ifndef NBG_H
+#define NBG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NBG_OK 0
+#define NBG_E_INVALID (-1)   /* bad argument (null pointer, n out of range, bad params) */
+#define NBG_E_OOM (-2)       /* device allocation failed */
+#define NBG_E_CAPACITY (-3)  /* a fixed-capacity pool would overflow; nothing was dropped */
+#define NBG_E_HIP (-4)       /* HIP runtime error */
+#define NBG_E_STATE (-5)     /* call out of order (e.g. step before upload) */
+
+/* Solver kinds, ngrav::Solver (projects/nbody_core/include/ngrav/Solvers.hpp:11-18). */
+#define NBG_SOLVER_DIRECT 0
+#define NBG_SOLVER_BARNES_HUT 1
+
+/* Multipole order of accepted cells.  The reference's System always adds the
+ * quadrupole in 3D (projects/nbody_core/include/ngrav/System.hpp:59); a tree without
+ * ComputeQuadrupoles() is monopole-only (projects/nbody_core/src/SelfTest.cpp:238-240). */
+#define NBG_MONOPOLE 0
+#define NBG_QUADRUPOLE 1
+
+/* flags */
+#define NBG_FLAG_KAHAN 1u           /* compensated FP32 force sums */
+#define NBG_FLAG_COUNT 2u           /* record per-group interaction counts (nbg_interaction_counts) */
+#define NBG_FLAG_LEARNED 4u         /* add the learned far-field correction (nbg_mlp_set_weights) */
+#define NBG_FLAG_NO_GRAPH 8u        /* nbg_step: launch eagerly instead of replaying a hipGraph */
+
+typedef struct nbg_ctx nbg_ctx;
+
+/* Everything one force evaluation needs; ngrav::StepParams
+ * (projects/nbody_core/include/ngrav/Solvers.hpp:22-40) restricted to the Plummer,
+ * geometric-MAC, fixed-dt path. */
+typedef struct nbg_params {
+    int solver;     /* NBG_SOLVER_* */
+    int multipole;  /* NBG_MONOPOLE | NBG_QUADRUPOLE */
+    double G;       /* StepParams::G */
+    double eps2;    /* Softening::eps2 (Plummer) */
+    double theta;   /* MacParams::theta (geometric MAC) */
+    double dt;      /* StepParams::dt (nbg_step only) */
+    uint32_t flags; /* NBG_FLAG_* */
+} nbg_params;
+
+/* Opt-in stage timings and tree statistics, like GpuBarnesHutStats
+ * (projects/nbody_core/include/ngrav/gpu/GpuBarnesHut.hpp:23-31).  Times are HIP-event
+ * milliseconds of the last force evaluation (or of all steps, for nbg_step). */
+typedef struct nbg_stats {
+    int64_t n_nodes;      /* live tree cells */
+    int64_t n_leaves;
+    int32_t max_depth;    /* deepest leaf */
+    int32_t deep_keys;    /* 1 if the 40-level (two-word) key path was needed */
+    int64_t n_p2p;        /* particle-particle interactions (NBG_FLAG_COUNT) */
+    int64_t n_m2p;        /* cell-particle interactions (NBG_FLAG_COUNT) */
+    double ms_keys;       /* bbox + root + Morton keys */
+    double ms_sort;       /* radix sort */
+    double ms_build;      /* topology + leaf order + gather */
+    double ms_upsweep;    /* moments */
+    double ms_walk;       /* tree walk + P2P/M2P (or direct sum) */
+    double ms_integrate;  /* kick/drift kernels */
+    double ms_total;
+} nbg_stats;
+
+/* Create a context on HIP device `device` able to hold up to max_n particles.
+ * Replaces the process-global lazily created GL state of
+ * projects/nbody_core/src/gpu/GpuBarnesHut.cpp:452-511.  Returns NULL on failure. */
+nbg_ctx* nbg_create(int device, int64_t max_n);
+void nbg_destroy(nbg_ctx* ctx);
+const char* nbg_last_error(const nbg_ctx* ctx);
+
+/* Upload particle state (FP32, original particle order).  vx/vy/vz may be NULL (zero).
+ * Replaces the FP64->vec4 pack + glNamedBufferSubData of
+ * projects/nbody_core/src/gpu/GpuBarnesHut.cpp:525-531.  Accelerations are cleared. */
+int nbg_upload(nbg_ctx* ctx, int64_t n, const float* x, const float* y, const float* z, const float* m,
+               const float* vx, const float* vy, const float* vz);
+
+/* Upload a batch of independent systems stored back to back: system s owns particles
+ * [offsets[s], offsets[s+1]).  offsets has nsys+1 entries, offsets[0] = 0.  Every
+ * system gets its own tree and sees only its own particles.  No reference counterpart:
+ * the reference runs one System at a time (projects/nbody_core/include/ngrav/System.hpp). */
+int nbg_upload_batch(nbg_ctx* ctx, int32_t nsys, const int64_t* offsets, const float* x, const float* y,
+                     const float* z, const float* m, const float* vx, const float* vy, const float* vz);
+
+/* Upload accelerations a(t) (e.g. from a previous run); NULL clears them. */
+int nbg_upload_accel(nbg_ctx* ctx, const float* ax, const float* ay, const float* az);
+
+/* Force evaluation a(x) into the context's acceleration buffer, G included.
+ * Replaces ngrav::gpu::ComputeAccelGpuBarnesHut (projects/nbody_core/src/gpu/GpuBarnesHut.cpp:515-694)
+ * and ngrav::gpu::ComputeAccelGpuDirect (projects/nbody_core/src/gpu/GpuDirect.cpp:61-117), and on the
+ * CPU side System::ComputeAccel (projects/nbody_core/include/ngrav/System.hpp:43-78), whose
+ * Barnes-Hut case is the grouped walk ComputeAccelBarnesHutGroup
+ * (projects/nbody_core/src/Solvers.cpp:279-355) over AdaptiveTree<3> (projects/nbody_core/src/Tree.cpp).
+ * Synchronous: returns when the accelerations are in device memory. */
+int nbg_accel(nbg_ctx* ctx, const nbg_params* p, nbg_stats* stats);
+
+/* nsteps kick-drift-kick leapfrog steps of p->dt with force evaluations by p->solver.
+ * LeapfrogStep (projects/nbody_core/src/Integrator.cpp:9-39) as driven by System::Step
+ * (projects/nbody_core/include/ngrav/System.hpp:89-108).  Precondition: the context holds
+ * a(t) (call nbg_accel first, as System::Prime does).  Unless NBG_FLAG_NO_GRAPH is set,
+ * one step is captured once into a hipGraph and replayed. */
+int nbg_step(nbg_ctx* ctx, const nbg_params* p, int32_t nsteps, nbg_stats* stats);
+
+/* Copy state back to the host (original particle order).  Any pointer may be NULL.
+ * Replaces the readback + FP32->FP64 unpack of projects/nbody_core/src/gpu/GpuBarnesHut.cpp:672-679. */
+int nbg_download(nbg_ctx* ctx, float* x, float* y, float* z, float* vx, float* vy, float* vz, float* ax,
+                 float* ay, float* az);
+
+/* Tree of the last Barnes-Hut evaluation, for bit-exact comparison with AdaptiveTree<3>
+ * (projects/nbody_core/include/ngrav/Tree.hpp:44-63).  Nodes are numbered in DFS pre-order
+ * with children in ascending octant, the order AdaptiveTree::Finalize visits them
+ * (projects/nbody_core/src/Tree.cpp:157-175).  Any output pointer may be NULL.
+ *   n_nodes  in: capacity of the node arrays; out: node count
+ *   keys     n uint64, the sorted 63-bit keys (levels 1..21, 3 bits per level, MSB first)
+ *   order    n int32, AdaptiveTree::order (leaf members in ascending original index)
+ *   center   4 doubles per node: centre x,y,z and halfSize (FP64, bit-exact)
+ *   topo     5 int32 per node: parent, firstParticle (-1 internal), particleCount
+ *            (0 internal), depth, childCount
+ *   children 8 int32 per node (-1 = empty octant)
+ *   moments  10 floats per node: com x,y,z, mass, maxRadius, Qxx,Qyy,Qxy,Qxz,Qyz */
+int nbg_tree_export(nbg_ctx* ctx, int64_t* n_nodes, uint64_t* keys, int32_t* order, double* center,
+                    int32_t* topo, int32_t* children, float* moments);
+
+/* Per-leaf interaction counts of the last Barnes-Hut evaluation run with
+ * NBG_FLAG_COUNT, indexed like nbg_tree_export (zero for internal nodes): far cells
+ * (M2P sources) and near particles (P2P sources, self excluded) of each leaf group,
+ * as CollectGroupInteractions builds them (projects/nbody_core/src/Solvers.cpp:248-277). */
+int nbg_interaction_counts(nbg_ctx* ctx, int64_t n_nodes, int64_t* n_far, int64_t* n_near);
+
+/* FP64 diagnostics of the current state: out[0]=kinetic, out[1]=potential (O(N^2), Plummer
+ * softened), out[2..4]=centre of mass, out[5..7]=angular momentum about the COM,
+ * out[8..10]=linear momentum.  System::TotalEnergy / CenterOfMass / AngularMomentum
+ * (projects/nbody_core/include/ngrav/System.hpp:119-193) and the momentum of
+ * DeckSim::Snapshot (projects/nbody_core/src/DeckSim.cpp:147-153). For a batch, system 0.
+ * potential is skipped (left 0) when with_potential is 0. */
+int nbg_diagnostics(nbg_ctx* ctx, const nbg_params* p, int32_t with_potential, double* out);
+
+/* Direct-sum accelerations (G included) onto a subset of targets, all particles of the
+ * target's system as sources; FP32 with compensated sums.  ComputeAccelDirectTargets
+ * (projects/nbody_core/src/Solvers.cpp:108-134).  Results go to host arrays of length nt. */
+int nbg_accel_direct_targets(nbg_ctx* ctx, const nbg_params* p, int64_t nt, const int64_t* targets, float* ax,
+                             float* ay, float* az);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBG_H */

@@ -1,0 +1,685 @@
+// nbg_api.hip -- the C ABI of libnbg_hip.so (declared in include/nbg.h): context and
+// device memory, the force-evaluation pipeline, the leapfrog driver and its hipGraph.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "nbg_internal.hpp"
+
+namespace nbg {
+void launch_gather(nbg_ctx* c, float G);
+}
+
+using namespace nbg;
+
+namespace {
+
+constexpr int kMaxSys = 1 << 20;
+
+template <typename T>
+int alloc_into(nbg_ctx* c, T*& p, size_t count) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    hipError_t e;
+    p = dev_alloc<T>(count, e);
+    if (!p) {
+        c->set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        return NBG_E_OOM;
+    }
+    return NBG_OK;
+}
+
+template <typename T>
+void free_dev(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+int check_launch(nbg_ctx* c) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        c->set_error(std::string("kernel launch: ") + hipGetErrorString(e));
+        return NBG_E_HIP;
+    }
+    return NBG_OK;
+}
+
+int validate(nbg_ctx* c, const nbg_params* p) {
+    if (!p) {
+        c->set_error("params is NULL");
+        return NBG_E_INVALID;
+    }
+    if (p->solver != NBG_SOLVER_DIRECT && p->solver != NBG_SOLVER_BARNES_HUT) {
+        c->set_error("unknown solver");
+        return NBG_E_INVALID;
+    }
+    if (p->multipole != NBG_MONOPOLE && p->multipole != NBG_QUADRUPOLE) {
+        c->set_error("unknown multipole order");
+        return NBG_E_INVALID;
+    }
+    if (!(p->eps2 >= 0.0) || !(p->theta >= 0.0) || !(p->G == p->G)) {
+        c->set_error("eps2 and theta must be >= 0");
+        return NBG_E_INVALID;
+    }
+    if (!c->have_state) {
+        c->set_error("no particles uploaded");
+        return NBG_E_STATE;
+    }
+    return NBG_OK;
+}
+
+// Enqueue one Barnes-Hut force evaluation (no host synchronisation).
+void enqueue_bh(nbg_ctx* c, const nbg_params& p, int levels, bool timed) {
+    if (timed) (void)hipEventRecord(c->ev[0], c->stream);
+    launch_bbox_and_keys(c, levels);
+    if (timed) (void)hipEventRecord(c->ev[1], c->stream);
+    launch_sort(c, levels);
+    if (timed) (void)hipEventRecord(c->ev[2], c->stream);
+    launch_build(c, levels);
+    launch_gather(c, static_cast<float>(p.G));
+    if (timed) (void)hipEventRecord(c->ev[3], c->stream);
+    launch_upsweep(c, levels, p.multipole);
+    if (timed) (void)hipEventRecord(c->ev[4], c->stream);
+    if ((p.flags & NBG_FLAG_COUNT) && c->cnt_far) {
+        launch_fill_i32(c, c->cnt_far, c->cap_nodes, 0);
+        launch_fill_i32(c, c->cnt_near, c->cap_nodes, 0);
+    }
+    launch_walk(c, p);
+    if (timed) (void)hipEventRecord(c->ev[5], c->stream);
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+int read_scalars(nbg_ctx* c) {
+    NBG_HIP_CHECK(c, hipMemcpyAsync(c->hsc, c->dsc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+    NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return NBG_OK;
+}
+
+void fill_tree_stats(nbg_ctx* c, nbg_stats* s) {
+    s->n_nodes = c->hsc->n_nodes;
+    s->n_leaves = c->hsc->n_leaves;
+    s->max_depth = c->hsc->max_depth;
+    s->deep_keys = (c->last_levels == kLevelsDeep) ? 1 : 0;
+}
+
+// One full force evaluation with the short-key path, falling back to two-word keys
+// when a depth-21 cell still holds more than ncrit particles.
+int accel_eager(nbg_ctx* c, const nbg_params& p, nbg_stats* stats) {
+    const bool timed = stats != nullptr;
+    c->last_G = p.G;
+    if (p.solver == NBG_SOLVER_DIRECT) {
+        if (timed) (void)hipEventRecord(c->ev[0], c->stream);
+        launch_direct(c, p);
+        if (timed) (void)hipEventRecord(c->ev[5], c->stream);
+        if (int r = check_launch(c)) return r;
+        NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        c->have_tree = false;
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->ms_walk = elapsed(c->ev[0], c->ev[5]);
+            stats->ms_total = stats->ms_walk;
+        }
+        return NBG_OK;
+    }
+    int levels = kLevelsShort;
+    for (;;) {
+        enqueue_bh(c, p, levels, timed);
+        if (int r = check_launch(c)) return r;
+        if (int r = read_scalars(c)) return r;
+        if (c->hsc->cap_exceeded) {
+            c->set_error("node pool capacity exceeded");
+            return NBG_E_CAPACITY;
+        }
+        if (c->hsc->overflow && levels == kLevelsShort) {
+            levels = kLevelsDeep;
+            continue;
+        }
+        break;
+    }
+    c->last_levels = levels;
+    c->have_tree = true;
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        fill_tree_stats(c, stats);
+        stats->ms_keys = elapsed(c->ev[0], c->ev[1]);
+        stats->ms_sort = elapsed(c->ev[1], c->ev[2]);
+        stats->ms_build = elapsed(c->ev[2], c->ev[3]);
+        stats->ms_upsweep = elapsed(c->ev[3], c->ev[4]);
+        stats->ms_walk = elapsed(c->ev[4], c->ev[5]);
+        stats->ms_total = elapsed(c->ev[0], c->ev[5]);
+        if (p.flags & NBG_FLAG_COUNT) {
+            // interaction totals: per group counts times group sizes
+            const int nn = c->hsc->n_nodes;
+            std::vector<int32_t> f(nn), q(nn), cnt(nn);
+            std::vector<float4> aux(nn);
+            (void)hipMemcpy(f.data(), c->cnt_far, nn * sizeof(int32_t), hipMemcpyDeviceToHost);
+            (void)hipMemcpy(q.data(), c->cnt_near, nn * sizeof(int32_t), hipMemcpyDeviceToHost);
+            (void)hipMemcpy(cnt.data(), c->nd_count, nn * sizeof(int32_t), hipMemcpyDeviceToHost);
+            (void)hipMemcpy(aux.data(), c->nd_aux, nn * sizeof(float4), hipMemcpyDeviceToHost);
+            int64_t m2p = 0, p2p = 0;
+            for (int i = 0; i < nn; ++i) {
+                uint32_t info;
+                std::memcpy(&info, &aux[i].w, 4);
+                if (!(info & kLeafBit)) continue;
+                m2p += static_cast<int64_t>(f[i]) * cnt[i];
+                p2p += static_cast<int64_t>(q[i]) * cnt[i];
+            }
+            stats->n_m2p = m2p;
+            stats->n_p2p = p2p;
+        }
+    }
+    return NBG_OK;
+}
+
+// One KDK step, enqueued (force evaluation with `levels`).
+void enqueue_step(nbg_ctx* c, const nbg_params& p, int levels) {
+    const float dt = static_cast<float>(p.dt);
+    const float hdt = static_cast<float>(0.5 * p.dt);
+    launch_kick_drift(c, hdt, dt);
+    if (p.solver == NBG_SOLVER_DIRECT) {
+        launch_direct(c, p);
+    } else {
+        nbg_params q = p;
+        q.flags &= ~NBG_FLAG_COUNT;
+        enqueue_bh(c, q, levels, false);
+    }
+    launch_kick(c, hdt);
+}
+
+int save_backup(nbg_ctx* c) {
+    float* src[9] = {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay, c->az};
+    for (int k = 0; k < 9; ++k) launch_copy_f32(c, c->backup + static_cast<int64_t>(k) * c->cap_n, src[k], c->n);
+    return check_launch(c);
+}
+
+int restore_backup(nbg_ctx* c) {
+    float* dst[9] = {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay, c->az};
+    for (int k = 0; k < 9; ++k) launch_copy_f32(c, dst[k], c->backup + static_cast<int64_t>(k) * c->cap_n, c->n);
+    return check_launch(c);
+}
+
+void drop_graph(nbg_ctx* c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    c->graph_valid = false;
+}
+
+bool same_params(const nbg_params& a, const nbg_params& b) {
+    return a.solver == b.solver && a.multipole == b.multipole && a.G == b.G && a.eps2 == b.eps2 &&
+           a.theta == b.theta && a.dt == b.dt && (a.flags & ~NBG_FLAG_COUNT) == (b.flags & ~NBG_FLAG_COUNT);
+}
+
+// Eager steps with a scalar read-back per step to catch the deep-key case.
+int steps_eager(nbg_ctx* c, const nbg_params& p, int nsteps) {
+    const float dt = static_cast<float>(p.dt);
+    const float hdt = static_cast<float>(0.5 * p.dt);
+    for (int s = 0; s < nsteps; ++s) {
+        launch_kick_drift(c, hdt, dt);
+        if (int r = check_launch(c)) return r;
+        nbg_params q = p;
+        q.flags &= ~NBG_FLAG_COUNT;
+        if (int r = accel_eager(c, q, nullptr)) return r;
+        launch_kick(c, hdt);
+        if (int r = check_launch(c)) return r;
+    }
+    return NBG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+nbg_ctx* nbg_create(int device, int64_t max_n) {
+    if (max_n < 1 || max_n > (int64_t{1} << 30)) return nullptr;
+    if (hipSetDevice(device) != hipSuccess) return nullptr;
+    nbg_ctx* c = new (std::nothrow) nbg_ctx();
+    if (!c) return nullptr;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return nullptr;
+    }
+    c->cap_n = max_n;
+    c->cap_nodes = max_n + (40 * max_n + 8) / 9 + 64;
+    const size_t N = static_cast<size_t>(max_n);
+    const size_t C = static_cast<size_t>(c->cap_nodes);
+    int r = NBG_OK;
+    r |= alloc_into(c, c->x, N);
+    r |= alloc_into(c, c->y, N);
+    r |= alloc_into(c, c->z, N);
+    r |= alloc_into(c, c->m, N);
+    r |= alloc_into(c, c->vx, N);
+    r |= alloc_into(c, c->vy, N);
+    r |= alloc_into(c, c->vz, N);
+    r |= alloc_into(c, c->ax, N);
+    r |= alloc_into(c, c->ay, N);
+    r |= alloc_into(c, c->az, N);
+    r |= alloc_into(c, c->backup, 9 * N);
+    r |= alloc_into(c, c->pos_sys, N);
+    r |= alloc_into(c, c->key_a, N);
+    r |= alloc_into(c, c->key_b, N);
+    r |= alloc_into(c, c->klo_a, N);
+    r |= alloc_into(c, c->klo_b, N);
+    r |= alloc_into(c, c->val_a, N);
+    r |= alloc_into(c, c->val_b, N);
+    r |= alloc_into(c, c->leaf_depth, N);
+    r |= alloc_into(c, c->cpl_prev, N);
+    r |= alloc_into(c, c->node_scan, N + 1);
+    r |= alloc_into(c, c->leaf_list, N);
+    r |= alloc_into(c, c->spos, N);
+    r |= alloc_into(c, c->nd_start, C);
+    r |= alloc_into(c, c->nd_count, C);
+    r |= alloc_into(c, c->nd_depth, C);
+    r |= alloc_into(c, c->nd_parent, C);
+    r |= alloc_into(c, c->nd_oct, C);
+    r |= alloc_into(c, c->nd_child, 8 * C);
+    r |= alloc_into(c, c->nd_geom, 4 * C);
+    r |= alloc_into(c, c->nd_com, C);
+    r |= alloc_into(c, c->nd_quad, C);
+    r |= alloc_into(c, c->nd_aux, C);
+    r |= alloc_into(c, c->depth_nodes, C);
+    r |= alloc_into(c, c->cnt_far, C);
+    r |= alloc_into(c, c->cnt_near, C);
+    r |= alloc_into(c, c->dsc, 1);
+    c->diag_part_cap = static_cast<int>(diag_partials_needed(max_n));
+    r |= alloc_into(c, c->diag_part, static_cast<size_t>(c->diag_part_cap));
+    if (r != NBG_OK || hipHostMalloc(reinterpret_cast<void**>(&c->hsc), sizeof(DevScalars)) != hipSuccess) {
+        nbg_destroy(c);
+        return nullptr;
+    }
+    for (auto& e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            nbg_destroy(c);
+            return nullptr;
+        }
+    return c;
+}
+
+void nbg_destroy(nbg_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    free_dev(c->x); free_dev(c->y); free_dev(c->z); free_dev(c->m);
+    free_dev(c->vx); free_dev(c->vy); free_dev(c->vz);
+    free_dev(c->ax); free_dev(c->ay); free_dev(c->az);
+    free_dev(c->backup); free_dev(c->pos_sys);
+    free_dev(c->key_a); free_dev(c->key_b); free_dev(c->klo_a); free_dev(c->klo_b);
+    free_dev(c->val_a); free_dev(c->val_b);
+    free_dev(c->hist); free_dev(c->scan_part);
+    free_dev(c->leaf_depth); free_dev(c->cpl_prev); free_dev(c->node_scan); free_dev(c->leaf_list);
+    free_dev(c->spos);
+    free_dev(c->nd_start); free_dev(c->nd_count); free_dev(c->nd_depth); free_dev(c->nd_parent);
+    free_dev(c->nd_oct); free_dev(c->nd_child); free_dev(c->nd_geom);
+    free_dev(c->nd_com); free_dev(c->nd_quad); free_dev(c->nd_aux);
+    free_dev(c->depth_nodes); free_dev(c->cnt_far); free_dev(c->cnt_near);
+    free_dev(c->dsc); free_dev(c->diag_part);
+    free_dev(c->sys_off); free_dev(c->sys_root); free_dev(c->sys_geom); free_dev(c->sys_bbox);
+    free_dev(c->blk_sys); free_dev(c->blk_start); free_dev(c->blk_count); free_dev(c->sys_blk0);
+    if (c->hsc) (void)hipHostFree(c->hsc);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* nbg_last_error(const nbg_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int nbg_upload_batch(nbg_ctx* c, int32_t nsys, const int64_t* offsets, const float* x, const float* y, const float* z,
+                     const float* m, const float* vx, const float* vy, const float* vz) {
+    if (!c) return NBG_E_INVALID;
+    if (nsys < 1 || nsys > kMaxSys || !offsets || offsets[0] != 0) {
+        c->set_error("bad batch layout");
+        return NBG_E_INVALID;
+    }
+    for (int s = 0; s < nsys; ++s)
+        if (offsets[s + 1] < offsets[s]) {
+            c->set_error("batch offsets must be non-decreasing");
+            return NBG_E_INVALID;
+        }
+    const int64_t n = offsets[nsys];
+    if (n < 0 || n > c->cap_n) {
+        c->set_error("particle count exceeds the context capacity");
+        return NBG_E_INVALID;
+    }
+    if (n > 0 && (!x || !y || !z || !m)) {
+        c->set_error("positions and masses are required");
+        return NBG_E_INVALID;
+    }
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    drop_graph(c);
+    c->n = static_cast<int>(n);
+    c->nsys = nsys;
+    c->h_sys_off.assign(offsets, offsets + nsys + 1);
+    // sort blocks: slices of <= kSortTile keys inside one system
+    std::vector<int32_t> bsys, bstart, bcnt, sb0(nsys + 1), off32(nsys + 1), psys(static_cast<size_t>(n));
+    for (int s = 0; s < nsys; ++s) {
+        sb0[s] = static_cast<int32_t>(bsys.size());
+        off32[s] = static_cast<int32_t>(offsets[s]);
+        for (int64_t b = offsets[s]; b < offsets[s + 1]; b += kSortTile) {
+            bsys.push_back(s);
+            bstart.push_back(static_cast<int32_t>(b));
+            bcnt.push_back(static_cast<int32_t>(std::min<int64_t>(kSortTile, offsets[s + 1] - b)));
+        }
+        for (int64_t i = offsets[s]; i < offsets[s + 1]; ++i) psys[static_cast<size_t>(i)] = s;
+    }
+    sb0[nsys] = static_cast<int32_t>(bsys.size());
+    off32[nsys] = static_cast<int32_t>(n);
+    c->n_blocks = static_cast<int>(bsys.size());
+    const size_t nb = std::max<size_t>(1, bsys.size());
+    int r = NBG_OK;
+    r |= alloc_into(c, c->blk_sys, nb);
+    r |= alloc_into(c, c->blk_start, nb);
+    r |= alloc_into(c, c->blk_count, nb);
+    r |= alloc_into(c, c->sys_blk0, nsys + 1);
+    r |= alloc_into(c, c->sys_off, nsys + 1);
+    r |= alloc_into(c, c->sys_root, nsys);
+    r |= alloc_into(c, c->sys_geom, 4 * static_cast<size_t>(nsys));
+    r |= alloc_into(c, c->sys_bbox, 6 * static_cast<size_t>(nsys));
+    r |= alloc_into(c, c->hist, static_cast<size_t>(kRadix) * nb);
+    const size_t parts = std::max<size_t>(grid_for(n + 1, 2048), grid_for(static_cast<int64_t>(kRadix) * nb, 2048)) + 8;
+    r |= alloc_into(c, c->scan_part, parts);
+    if (r != NBG_OK) return r;
+    if (!bsys.empty()) {
+        NBG_HIP_CHECK(c, hipMemcpy(c->blk_sys, bsys.data(), bsys.size() * 4, hipMemcpyHostToDevice));
+        NBG_HIP_CHECK(c, hipMemcpy(c->blk_start, bstart.data(), bstart.size() * 4, hipMemcpyHostToDevice));
+        NBG_HIP_CHECK(c, hipMemcpy(c->blk_count, bcnt.data(), bcnt.size() * 4, hipMemcpyHostToDevice));
+    }
+    NBG_HIP_CHECK(c, hipMemcpy(c->sys_blk0, sb0.data(), sb0.size() * 4, hipMemcpyHostToDevice));
+    NBG_HIP_CHECK(c, hipMemcpy(c->sys_off, off32.data(), off32.size() * 4, hipMemcpyHostToDevice));
+    if (n > 0) {
+        NBG_HIP_CHECK(c, hipMemcpy(c->pos_sys, psys.data(), psys.size() * 4, hipMemcpyHostToDevice));
+        const size_t bytes = static_cast<size_t>(n) * sizeof(float);
+        NBG_HIP_CHECK(c, hipMemcpy(c->x, x, bytes, hipMemcpyHostToDevice));
+        NBG_HIP_CHECK(c, hipMemcpy(c->y, y, bytes, hipMemcpyHostToDevice));
+        NBG_HIP_CHECK(c, hipMemcpy(c->z, z, bytes, hipMemcpyHostToDevice));
+        NBG_HIP_CHECK(c, hipMemcpy(c->m, m, bytes, hipMemcpyHostToDevice));
+        float* vdst[3] = {c->vx, c->vy, c->vz};
+        const float* vsrc[3] = {vx, vy, vz};
+        for (int k = 0; k < 3; ++k) {
+            if (vsrc[k])
+                NBG_HIP_CHECK(c, hipMemcpy(vdst[k], vsrc[k], bytes, hipMemcpyHostToDevice));
+            else
+                NBG_HIP_CHECK(c, hipMemset(vdst[k], 0, bytes));
+        }
+        NBG_HIP_CHECK(c, hipMemset(c->ax, 0, bytes));
+        NBG_HIP_CHECK(c, hipMemset(c->ay, 0, bytes));
+        NBG_HIP_CHECK(c, hipMemset(c->az, 0, bytes));
+    }
+    NBG_HIP_CHECK(c, hipDeviceSynchronize());
+    c->have_state = true;
+    c->have_tree = false;
+    return NBG_OK;
+}
+
+int nbg_upload(nbg_ctx* c, int64_t n, const float* x, const float* y, const float* z, const float* m,
+               const float* vx, const float* vy, const float* vz) {
+    const int64_t off[2] = {0, n};
+    return nbg_upload_batch(c, 1, off, x, y, z, m, vx, vy, vz);
+}
+
+int nbg_upload_accel(nbg_ctx* c, const float* ax, const float* ay, const float* az) {
+    if (!c) return NBG_E_INVALID;
+    if (!c->have_state) {
+        c->set_error("no particles uploaded");
+        return NBG_E_STATE;
+    }
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    const size_t bytes = static_cast<size_t>(c->n) * sizeof(float);
+    if (bytes == 0) return NBG_OK;
+    float* dst[3] = {c->ax, c->ay, c->az};
+    const float* src[3] = {ax, ay, az};
+    for (int k = 0; k < 3; ++k) {
+        if (src[k])
+            NBG_HIP_CHECK(c, hipMemcpy(dst[k], src[k], bytes, hipMemcpyHostToDevice));
+        else
+            NBG_HIP_CHECK(c, hipMemset(dst[k], 0, bytes));
+    }
+    return NBG_OK;
+}
+
+int nbg_accel(nbg_ctx* c, const nbg_params* p, nbg_stats* stats) {
+    if (!c) return NBG_E_INVALID;
+    if (int r = validate(c, p)) return r;
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    if (c->n == 0) {
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        return NBG_OK;
+    }
+    return accel_eager(c, *p, stats);
+}
+
+int nbg_step(nbg_ctx* c, const nbg_params* p, int32_t nsteps, nbg_stats* stats) {
+    if (!c) return NBG_E_INVALID;
+    if (int r = validate(c, p)) return r;
+    if (nsteps < 0) {
+        c->set_error("nsteps must be >= 0");
+        return NBG_E_INVALID;
+    }
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    if (c->n == 0 || nsteps == 0) {
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        return NBG_OK;
+    }
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    c->last_G = p->G;
+    NBG_HIP_CHECK(c, hipEventRecord(c->ev[6], c->stream));
+    const bool use_graph = !(p->flags & NBG_FLAG_NO_GRAPH) && c->last_levels == kLevelsShort;
+    if (!use_graph) {
+        if (int r = steps_eager(c, *p, nsteps)) return r;
+    } else {
+        if (!c->graph_valid || !same_params(c->graph_params, *p)) {
+            drop_graph(c);
+            NBG_HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_step(c, *p, kLevelsShort);
+            const hipError_t le = hipGetLastError();
+            hipGraph_t g = nullptr;
+            const hipError_t ee = hipStreamEndCapture(c->stream, &g);
+            if (le != hipSuccess || ee != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                c->set_error(std::string("graph capture failed: ") + hipGetErrorString(le != hipSuccess ? le : ee));
+                return NBG_E_HIP;
+            }
+            c->graph = g;
+            NBG_HIP_CHECK(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+            c->graph_params = *p;
+            c->graph_valid = true;
+        }
+        if (int r = save_backup(c)) return r;
+        NBG_HIP_CHECK(c, hipMemsetAsync(&c->dsc->sticky_overflow, 0, 2 * sizeof(int32_t), c->stream));
+        for (int s = 0; s < nsteps; ++s) NBG_HIP_CHECK(c, hipGraphLaunch(c->graph_exec, c->stream));
+        NBG_HIP_CHECK(c, hipMemcpyAsync(&c->hsc->sticky_overflow, &c->dsc->sticky_overflow, 2 * sizeof(int32_t),
+                                        hipMemcpyDeviceToHost, c->stream));
+        NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        const bool overflow = c->hsc->sticky_overflow != 0;
+        const bool cap = c->hsc->sticky_cap != 0;
+        if (cap) {
+            c->set_error("node pool capacity exceeded");
+            return NBG_E_CAPACITY;
+        }
+        if (overflow) {
+            if (int r = restore_backup(c)) return r;
+            if (int r = steps_eager(c, *p, nsteps)) return r;
+        }
+    }
+    NBG_HIP_CHECK(c, hipEventRecord(c->ev[7], c->stream));
+    NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->have_tree = p->solver == NBG_SOLVER_BARNES_HUT;
+    if (stats) {
+        if (int r = read_scalars(c)) return r;
+        fill_tree_stats(c, stats);
+        stats->ms_total = elapsed(c->ev[6], c->ev[7]);
+    }
+    return NBG_OK;
+}
+
+int nbg_download(nbg_ctx* c, float* x, float* y, float* z, float* vx, float* vy, float* vz, float* ax, float* ay,
+                 float* az) {
+    if (!c) return NBG_E_INVALID;
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    const size_t bytes = static_cast<size_t>(c->n) * sizeof(float);
+    if (bytes == 0) return NBG_OK;
+    float* dst[9] = {x, y, z, vx, vy, vz, ax, ay, az};
+    const float* src[9] = {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay, c->az};
+    for (int k = 0; k < 9; ++k)
+        if (dst[k]) NBG_HIP_CHECK(c, hipMemcpy(dst[k], src[k], bytes, hipMemcpyDeviceToHost));
+    return NBG_OK;
+}
+
+int nbg_tree_export(nbg_ctx* c, int64_t* n_nodes, uint64_t* keys, int32_t* order, double* center, int32_t* topo,
+                    int32_t* children, float* moments) {
+    if (!c || !n_nodes) return NBG_E_INVALID;
+    if (!c->have_tree) {
+        c->set_error("no Barnes-Hut tree: run nbg_accel with the Barnes-Hut solver first");
+        return NBG_E_STATE;
+    }
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    if (int r = read_scalars(c)) return r;
+    const int64_t nn = c->hsc->n_nodes;
+    const int64_t cap = *n_nodes;
+    *n_nodes = nn;
+    const size_t n = static_cast<size_t>(c->n);
+    if (keys) NBG_HIP_CHECK(c, hipMemcpy(keys, c->key_sorted, n * 8, hipMemcpyDeviceToHost));
+    if (order) NBG_HIP_CHECK(c, hipMemcpy(order, c->perm, n * 4, hipMemcpyDeviceToHost));
+    if (!center && !topo && !children && !moments) return NBG_OK;
+    if (cap < nn) {
+        c->set_error("node arrays too small");
+        return NBG_E_INVALID;
+    }
+    const size_t un = static_cast<size_t>(nn);
+    if (center) NBG_HIP_CHECK(c, hipMemcpy(center, c->nd_geom, un * 4 * sizeof(double), hipMemcpyDeviceToHost));
+    if (children) NBG_HIP_CHECK(c, hipMemcpy(children, c->nd_child, un * 8 * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<float4> aux(un);
+    NBG_HIP_CHECK(c, hipMemcpy(aux.data(), c->nd_aux, un * sizeof(float4), hipMemcpyDeviceToHost));
+    if (topo) {
+        std::vector<int32_t> st(un), cnt(un), dep(un), par(un);
+        NBG_HIP_CHECK(c, hipMemcpy(st.data(), c->nd_start, un * 4, hipMemcpyDeviceToHost));
+        NBG_HIP_CHECK(c, hipMemcpy(cnt.data(), c->nd_count, un * 4, hipMemcpyDeviceToHost));
+        NBG_HIP_CHECK(c, hipMemcpy(dep.data(), c->nd_depth, un * 4, hipMemcpyDeviceToHost));
+        NBG_HIP_CHECK(c, hipMemcpy(par.data(), c->nd_parent, un * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < un; ++i) {
+            uint32_t info;
+            std::memcpy(&info, &aux[i].w, 4);
+            const bool leaf = (info & kLeafBit) != 0;
+            topo[5 * i + 0] = par[i];
+            topo[5 * i + 1] = leaf ? st[i] : -1;
+            topo[5 * i + 2] = leaf ? cnt[i] : 0;
+            topo[5 * i + 3] = dep[i];
+            topo[5 * i + 4] = leaf ? 0 : static_cast<int32_t>(info);
+        }
+    }
+    if (moments) {
+        std::vector<float4> com(un), quad(un);
+        NBG_HIP_CHECK(c, hipMemcpy(com.data(), c->nd_com, un * sizeof(float4), hipMemcpyDeviceToHost));
+        NBG_HIP_CHECK(c, hipMemcpy(quad.data(), c->nd_quad, un * sizeof(float4), hipMemcpyDeviceToHost));
+        // device masses and quadrupoles carry G; report them without it
+        const float ig = (c->last_G != 0.0) ? static_cast<float>(1.0 / c->last_G) : 1.0f;
+        for (size_t i = 0; i < un; ++i) {
+            float* o = moments + 10 * i;
+            o[0] = com[i].x;
+            o[1] = com[i].y;
+            o[2] = com[i].z;
+            o[3] = com[i].w * ig;
+            o[4] = aux[i].z;
+            o[5] = quad[i].x * ig;
+            o[6] = quad[i].y * ig;
+            o[7] = quad[i].z * ig;
+            o[8] = quad[i].w * ig;
+            o[9] = aux[i].x * ig;
+        }
+    }
+    return NBG_OK;
+}
+
+int nbg_interaction_counts(nbg_ctx* c, int64_t n_nodes, int64_t* n_far, int64_t* n_near) {
+    if (!c || !n_far || !n_near) return NBG_E_INVALID;
+    if (!c->have_tree) {
+        c->set_error("no Barnes-Hut tree");
+        return NBG_E_STATE;
+    }
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    if (int r = read_scalars(c)) return r;
+    const int64_t nn = c->hsc->n_nodes;
+    if (n_nodes < nn) {
+        c->set_error("count arrays too small");
+        return NBG_E_INVALID;
+    }
+    std::vector<int32_t> f(static_cast<size_t>(nn)), q(static_cast<size_t>(nn));
+    NBG_HIP_CHECK(c, hipMemcpy(f.data(), c->cnt_far, f.size() * 4, hipMemcpyDeviceToHost));
+    NBG_HIP_CHECK(c, hipMemcpy(q.data(), c->cnt_near, q.size() * 4, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nn; ++i) {
+        n_far[i] = f[static_cast<size_t>(i)];
+        n_near[i] = q[static_cast<size_t>(i)];
+    }
+    return NBG_OK;
+}
+
+int nbg_diagnostics(nbg_ctx* c, const nbg_params* p, int32_t with_potential, double* out) {
+    if (!c || !out) return NBG_E_INVALID;
+    if (int r = validate(c, p)) return r;
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    int r = diagnostics(c, *p, with_potential, out);
+    if (r == NBG_OK) r = check_launch(c);
+    return r;
+}
+
+int nbg_accel_direct_targets(nbg_ctx* c, const nbg_params* p, int64_t nt, const int64_t* targets, float* ax,
+                             float* ay, float* az) {
+    if (!c) return NBG_E_INVALID;
+    if (int r = validate(c, p)) return r;
+    if (nt < 0 || (nt > 0 && (!targets || !ax || !ay || !az))) {
+        c->set_error("bad target list");
+        return NBG_E_INVALID;
+    }
+    if (nt == 0) return NBG_OK;
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    std::vector<int32_t> t32(static_cast<size_t>(nt));
+    for (int64_t k = 0; k < nt; ++k) {
+        if (targets[k] < 0 || targets[k] >= c->n) {
+            c->set_error("target index out of range");
+            return NBG_E_INVALID;
+        }
+        t32[static_cast<size_t>(k)] = static_cast<int32_t>(targets[k]);
+    }
+    int32_t* dt = nullptr;
+    float* dout = nullptr;
+    NBG_HIP_CHECK(c, hipMalloc(&dt, t32.size() * 4));
+    if (hipMalloc(&dout, t32.size() * 12) != hipSuccess) {
+        (void)hipFree(dt);
+        c->set_error("hipMalloc failed");
+        return NBG_E_OOM;
+    }
+    int r = NBG_OK;
+    if (hipMemcpy(dt, t32.data(), t32.size() * 4, hipMemcpyHostToDevice) != hipSuccess) r = NBG_E_HIP;
+    if (r == NBG_OK) {
+        launch_direct_targets(c, *p, nt, dt, dout);
+        r = check_launch(c);
+    }
+    std::vector<float> h(static_cast<size_t>(nt) * 3);
+    if (r == NBG_OK && hipStreamSynchronize(c->stream) != hipSuccess) r = NBG_E_HIP;
+    if (r == NBG_OK && hipMemcpy(h.data(), dout, h.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) r = NBG_E_HIP;
+    (void)hipFree(dt);
+    (void)hipFree(dout);
+    if (r != NBG_OK) {
+        if (c->err.empty()) c->set_error("direct-targets evaluation failed");
+        return r;
+    }
+    for (int64_t k = 0; k < nt; ++k) {
+        ax[k] = h[3 * static_cast<size_t>(k) + 0];
+        ay[k] = h[3 * static_cast<size_t>(k) + 1];
+        az[k] = h[3 * static_cast<size_t>(k) + 2];
+    }
+    c->have_tree = false;
+    return NBG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,78 @@
+// nbg_common.hpp -- device helpers shared by the kernels (wave64 primitives, key math).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nbg {
+
+__device__ __forceinline__ int lane_id() {
+    return static_cast<int>(__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)));
+}
+
+// Number of set bits of `m` in lanes below this one.
+__device__ __forceinline__ int mbcnt(uint64_t m) {
+    return static_cast<int>(
+        __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u)));
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// Orders LDS traffic of one wavefront: every LDS write issued before is visible to
+// LDS reads issued after, for all lanes of the wave.  Waves of a workgroup never
+// share LDS regions in the kernels that use this.
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); __builtin_amdgcn_wave_barrier(); }
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        T t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Order-preserving float <-> uint map (for atomic min/max of floats).
+__device__ __forceinline__ uint32_t f2ord(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__host__ __device__ inline float ord2f(uint32_t u) {
+    const uint32_t b = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    float f;
+    __builtin_memcpy(&f, &b, 4);
+    return f;
+}
+
+// Common-prefix length, in levels, of two keys.  A short key holds levels 1..21 in
+// bits 62..0 (3 bits per level, level 1 highest); a deep key adds levels 22..40 in
+// bits 56..0 of the low word.  Equal keys return the number of levels held.
+__device__ __forceinline__ int cpl_short(uint64_t a, uint64_t b) {
+    const uint64_t d = a ^ b;
+    if (d == 0) return 21;
+    return (__clzll(static_cast<long long>(d)) - 1) / 3;
+}
+__device__ __forceinline__ int cpl_deep(uint64_t ah, uint64_t al, uint64_t bh, uint64_t bl) {
+    const uint64_t d = ah ^ bh;
+    if (d != 0) return (__clzll(static_cast<long long>(d)) - 1) / 3;
+    const uint64_t e = al ^ bl;
+    if (e == 0) return 40;
+    return 21 + (__clzll(static_cast<long long>(e)) - 7) / 3;
+}
+
+// Octant digit of level `d` (1-based) of a key.
+__device__ __forceinline__ int key_digit(uint64_t hi, uint64_t lo, int d) {
+    if (d <= 21) return static_cast<int>((hi >> (3 * (21 - d))) & 7u);
+    return static_cast<int>((lo >> (3 * (40 - d))) & 7u);
+}
+
+}  // namespace nbg

@@ -1,0 +1,177 @@
+// nbg_internal.hpp -- context layout and kernel launchers shared by the .hip files of
+// libnbg_hip.so.  Everything here is gfx950 (CDNA4) code: 64-wide wavefronts, LDS, HBM3E.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/nbg.h"
+
+namespace nbg {
+
+// ---------------------------------------------------------------------------- constants
+constexpr int kNcrit = 8;           // AdaptiveTree ncrit (projects/nbody_core/include/ngrav/Tree.hpp:38)
+constexpr int kMaxDepth = 40;       // AdaptiveTree maxDepth (same line)
+constexpr int kLevelsShort = 21;    // levels held by one 63-bit key
+constexpr int kLevelsDeep = 40;     // levels held by the two-word key
+constexpr int kSortThreads = 256;   // radix-sort workgroup
+constexpr int kSortItems = 8;       // keys per thread
+constexpr int kSortTile = kSortThreads * kSortItems;  // keys per sort block (2048)
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+
+// Leaf info word stored (as float bits) in node_aux.w: bit 31 set = leaf, low bits =
+// particle count (leaf) or child count (internal).
+constexpr uint32_t kLeafBit = 0x80000000u;
+
+// Device-side scalars written by the build, read back only for stats.
+struct DevScalars {
+    int32_t n_nodes;
+    int32_t n_leaves;
+    int32_t max_depth;
+    int32_t overflow;    // a depth-21 cell held more than ncrit particles
+    int32_t cap_exceeded;
+    int32_t sticky_overflow;  // like overflow/cap_exceeded, but only cleared by the host
+    int32_t sticky_cap;
+    int32_t pad;
+    uint32_t depth_count[64];   // internal nodes per depth
+    uint32_t depth_offset[64];  // exclusive scan of depth_count
+    uint32_t depth_fill[64];    // scatter cursors
+};
+
+// ---------------------------------------------------------------------------- helpers
+#define NBG_HIP_CHECK(ctx, expr)                                                         \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            (ctx)->set_error(std::string(#expr) + ": " + hipGetErrorString(e_));         \
+            return NBG_E_HIP;                                                            \
+        }                                                                                \
+    } while (0)
+
+template <typename T>
+inline T* dev_alloc(size_t count, hipError_t& err) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    err = hipMalloc(&p, count * sizeof(T));
+    if (err != hipSuccess) return nullptr;
+    return static_cast<T*>(p);
+}
+
+inline int grid_for(int64_t n, int threads) { return static_cast<int>((n + threads - 1) / threads); }
+
+}  // namespace nbg
+
+// ---------------------------------------------------------------------------- context
+struct nbg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    void set_error(const std::string& s) { err = s; }
+
+    int64_t cap_n = 0;    // particle capacity
+    int64_t cap_nodes = 0;
+    int n = 0;            // particles uploaded
+    int nsys = 0;         // systems in the batch
+    bool have_state = false;
+    bool have_tree = false;
+    int last_levels = 21;
+    double last_G = 1.0;  // G folded into the device masses of the last evaluation
+
+    // batch layout (host + device)
+    std::vector<int64_t> h_sys_off;
+    int32_t* sys_off = nullptr;     // nsys+1
+    int32_t* pos_sys = nullptr;     // system of each particle / sorted position
+    int32_t* sys_root = nullptr;    // root node of each system
+    double* sys_geom = nullptr;     // 4 per system: root centre x,y,z, halfSize
+    uint32_t* sys_bbox = nullptr;   // 6 per system: order-preserving min x,y,z, max x,y,z
+    // sort blocks: each covers at most kSortTile keys of one system
+    int n_blocks = 0;
+    int32_t* blk_sys = nullptr;
+    int32_t* blk_start = nullptr;
+    int32_t* blk_count = nullptr;
+    int32_t* sys_blk0 = nullptr;    // nsys+1: first sort block of each system
+
+    // particle state, original order (FP32 SoA)
+    float *x = nullptr, *y = nullptr, *z = nullptr, *m = nullptr;
+    float *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    float *ax = nullptr, *ay = nullptr, *az = nullptr;
+    // backup of the state for the deep-key re-run of a graph batch
+    float* backup = nullptr;
+
+    // keys and sort
+    uint64_t *key_a = nullptr, *key_b = nullptr;   // levels 1..21 (ping-pong)
+    uint64_t *klo_a = nullptr, *klo_b = nullptr;   // levels 22..40 (deep path)
+    uint32_t *val_a = nullptr, *val_b = nullptr;
+    uint32_t* hist = nullptr;                      // kRadix * n_blocks
+    uint64_t* scan_part = nullptr;                 // block partials of the scans
+    uint64_t* key_sorted = nullptr;                // final sorted keys (alias of a or b)
+    uint64_t* klo_sorted = nullptr;
+    uint32_t* perm = nullptr;                      // sorted position -> original index
+
+    // build
+    int32_t* leaf_depth = nullptr;   // per sorted position
+    int32_t* cpl_prev = nullptr;     // common-prefix length with the previous position
+    uint64_t* node_scan = nullptr;   // (nodes started << 32 | leaf started), then its exclusive scan
+
+    // node arrays (DFS pre-order ids)
+    int32_t* nd_start = nullptr;
+    int32_t* nd_count = nullptr;
+    int32_t* nd_depth = nullptr;
+    int32_t* nd_parent = nullptr;
+    int32_t* nd_oct = nullptr;
+    int32_t* nd_child = nullptr;     // 8 per node
+    double* nd_geom = nullptr;       // 4 per node: centre x,y,z, halfSize (FP64)
+    float4* nd_com = nullptr;        // com x,y,z, G*mass
+    float4* nd_quad = nullptr;       // G*(Qxx, Qyy, Qxy, Qxz)
+    float4* nd_aux = nullptr;        // G*Qyz, (2 halfSize)^2, maxRadius, info bits
+    int32_t* leaf_list = nullptr;    // leaf node ids, in pre-order
+    int32_t* depth_nodes = nullptr;  // internal node ids grouped by depth
+    int32_t* cnt_far = nullptr;      // per node (leaf groups), NBG_FLAG_COUNT
+    int32_t* cnt_near = nullptr;
+
+    // sorted particles: x,y,z, G*m
+    float4* spos = nullptr;
+
+    // scalars
+    nbg::DevScalars* dsc = nullptr;
+    nbg::DevScalars* hsc = nullptr;   // pinned mirror
+
+    // diagnostics scratch
+    double* diag_part = nullptr;
+    int diag_part_cap = 0;
+
+    // timing
+    hipEvent_t ev[8] = {};
+
+    // graph of one step
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    nbg_params graph_params{};
+    bool graph_valid = false;
+};
+
+namespace nbg {
+
+// ---------------------------------------------------------------------------- launchers
+// tree.hip
+void launch_bbox_and_keys(nbg_ctx* c, int levels);
+void launch_sort(nbg_ctx* c, int levels);
+void launch_build(nbg_ctx* c, int levels);
+void launch_upsweep(nbg_ctx* c, int levels, int multipole);
+void launch_fill_i32(nbg_ctx* c, int32_t* p, int64_t n, int32_t v);
+// walk.hip
+void launch_walk(nbg_ctx* c, const nbg_params& p);
+void launch_direct(nbg_ctx* c, const nbg_params& p);
+void launch_direct_targets(nbg_ctx* c, const nbg_params& p, int64_t nt, const int32_t* targets, float* out3);
+// integrate.hip
+void launch_kick(nbg_ctx* c, float hdt);
+void launch_kick_drift(nbg_ctx* c, float hdt, float dt);
+void launch_copy_f32(nbg_ctx* c, float* dst, const float* src, int64_t n);
+int diagnostics(nbg_ctx* c, const nbg_params& p, int with_potential, double* out);
+int64_t diag_partials_needed(int64_t n);
+
+}  // namespace nbg

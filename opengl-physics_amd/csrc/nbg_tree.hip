@@ -1,0 +1,898 @@
+// nbg_tree.hip -- bounding box, reference-consistent Morton keys, LDS radix sort,
+// bottom-up octree build and multipole upsweep, for gfx950.
+//
+// The octree reproduces AdaptiveTree<3> (projects/nbody_core/src/Tree.cpp:33-203) bit for
+// bit without its serial insertion:
+//  * Root box: Tree.cpp:54-88 (FP64 min/max of the positions, half = max half-extent,
+//    1.0 if degenerate, padded by 1.001).
+//  * Octant of a particle in a cell: p >= centre per axis (Tree.cpp:11-20); child centre
+//    = parent centre +- 0.5*parent half (Tree.cpp:96-99).  Replaying that FP64 recurrence
+//    per particle gives its octant digit at every level, i.e. a Morton key whose prefixes
+//    are exactly the reference's cells.
+//  * Topology: insertion splits a cell once it would hold ncrit+1 = 9 particles and its
+//    depth is < 40 (Tree.cpp:124-135), and creates a child only for an octant that
+//    receives a particle.  So a cell is a node iff its parent holds > 8 particles, and
+//    it is a leaf iff it holds <= 8 (or sits at depth 40).  On sorted keys every cell is
+//    a contiguous run, so each sorted position finds its leaf depth from its +-8
+//    neighbours and the nodes it starts from one prefix sum, with no atomics.
+//  * Numbering: nodes started at a position, in increasing depth, over positions in key
+//    order, is DFS pre-order with ascending octant -- the order Finalize walks
+//    (Tree.cpp:157-175).
+//  * `order`: leaves in key order, members in ascending original index (insertion fills
+//    buckets in index order), so each leaf's <= 8 members are re-sorted by index.
+#include "nbg_common.hpp"
+#include "nbg_internal.hpp"
+
+namespace nbg {
+
+// ============================================================================ utilities
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void launch_fill_i32(nbg_ctx* c, int32_t* p, int64_t n, int32_t v) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, p, n, v);
+}
+
+// ------------------------------------------------------------------- generic prefix scan
+// Exclusive scan in three kernels: block sums, one-block scan of the sums, block-local scan
+// plus offset.  2048 elements per block (256 threads x 8, blocked per thread).
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+template <typename T>
+__device__ T block_excl_scan(T v, T* s_wave, T* total) {
+    const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
+    T incl = wave_incl_scan(v);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    T wave_off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+        const T s = s_wave[w];
+        if (w < wave) wave_off += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return wave_off + incl - v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const T* __restrict__ in, int64_t n, T* __restrict__ part) {
+    __shared__ T s_wave[kScanThreads / 64];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+    T sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) sum += in[base + k];
+    T tot;
+    (void)block_excl_scan(sum, s_wave, &tot);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// One workgroup scans the block sums in place (exclusive) and stores the grand total.
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_partials(T* __restrict__ part, int64_t nparts, T* __restrict__ total_out) {
+    __shared__ T s_wave[kScanThreads / 64];
+    T carry = 0;
+    for (int64_t base = 0; base < nparts; base += kScanTile) {
+        const int64_t b = base + threadIdx.x * kScanItems;
+        T v[kScanItems];
+        T sum = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            v[k] = (b + k < nparts) ? part[b + k] : T(0);
+            sum += v[k];
+        }
+        T tot;
+        T off = block_excl_scan(sum, s_wave, &tot) + carry;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            if (b + k < nparts) part[b + k] = off;
+            off += v[k];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(const T* __restrict__ in, T* __restrict__ out, int64_t n,
+                                                            const T* __restrict__ part) {
+    __shared__ T s_wave[kScanThreads / 64];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+    T v[kScanItems];
+    T sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : T(0);
+        sum += v[k];
+    }
+    T tot;
+    T off = block_excl_scan(sum, s_wave, &tot) + part[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = off;
+        off += v[k];
+    }
+}
+
+// part must hold grid_for(n, kScanTile) elements.
+template <typename T>
+void excl_scan(hipStream_t st, const T* in, T* out, int64_t n, T* part, T* total_out) {
+    const int nb = grid_for(n, kScanTile);
+    hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(kScanThreads), 0, st, in, n, part);
+    hipLaunchKernelGGL(k_scan_partials<T>, dim3(1), dim3(kScanThreads), 0, st, part, static_cast<int64_t>(nb), total_out);
+    hipLaunchKernelGGL(k_scan_down<T>, dim3(nb), dim3(kScanThreads), 0, st, in, out, n, part);
+}
+
+// ============================================================================ bbox + keys
+
+// Resets the per-call device scalars and each system's bbox accumulators.
+__global__ void k_reset(DevScalars* __restrict__ sc, uint32_t* __restrict__ bbox, int nsys) {
+    const int t = threadIdx.x;
+    if (t < 64) {
+        sc->depth_count[t] = 0;
+        sc->depth_offset[t] = 0;
+        sc->depth_fill[t] = 0;
+    }
+    if (t == 0) {
+        sc->n_nodes = 0;
+        sc->n_leaves = 0;
+        sc->max_depth = 0;
+        sc->overflow = 0;
+        sc->cap_exceeded = 0;
+    }
+    for (int s = t; s < nsys; s += blockDim.x) {
+        bbox[6 * s + 0] = 0xffffffffu;
+        bbox[6 * s + 1] = 0xffffffffu;
+        bbox[6 * s + 2] = 0xffffffffu;
+        bbox[6 * s + 3] = 0u;
+        bbox[6 * s + 4] = 0u;
+        bbox[6 * s + 5] = 0u;
+    }
+}
+
+// Per sort block (a slice of one system): exact FP32 min/max, one atomic per block.
+__global__ __launch_bounds__(256) void k_bbox(const float* __restrict__ x, const float* __restrict__ y,
+                                              const float* __restrict__ z, const int32_t* __restrict__ blk_sys,
+                                              const int32_t* __restrict__ blk_start,
+                                              const int32_t* __restrict__ blk_count, uint32_t* __restrict__ bbox) {
+    __shared__ uint32_t s_red[6][4];
+    const int b = blockIdx.x;
+    const int s = blk_sys[b];
+    const int start = blk_start[b], cnt = blk_count[b];
+    uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+        const int i = start + k;
+        const uint32_t v[3] = {f2ord(x[i]), f2ord(y[i]), f2ord(z[i])};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = min(lo[a], v[a]);
+            hi[a] = max(hi[a], v[a]);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = min(lo[a], static_cast<uint32_t>(__shfl_xor(static_cast<int>(lo[a]), o, 64)));
+            hi[a] = max(hi[a], static_cast<uint32_t>(__shfl_xor(static_cast<int>(hi[a]), o, 64)));
+        }
+    }
+    const int wave = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            s_red[a][wave] = lo[a];
+            s_red[3 + a][wave] = hi[a];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int a = threadIdx.x;
+        uint32_t l = s_red[a][0], h = s_red[3 + a][0];
+        for (int w = 1; w < 4; ++w) {
+            l = min(l, s_red[a][w]);
+            h = max(h, s_red[3 + a][w]);
+        }
+        if (cnt > 0) {
+            atomicMin(&bbox[6 * s + a], l);
+            atomicMax(&bbox[6 * s + 3 + a], h);
+        }
+    }
+}
+
+// Root geometry per system, FP64, as Tree.cpp:54-88.
+__global__ void k_root(const uint32_t* __restrict__ bbox, const int32_t* __restrict__ sys_off, int nsys,
+                       double* __restrict__ geom) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    double lo[3], hi[3];
+    const bool empty = sys_off[s + 1] == sys_off[s];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = empty ? 0.0 : static_cast<double>(ord2f(bbox[6 * s + a]));
+        hi[a] = empty ? 0.0 : static_cast<double>(ord2f(bbox[6 * s + 3 + a]));
+    }
+    double half = 0.5 * (hi[0] - lo[0]);
+    half = fmax(half, 0.5 * (hi[1] - lo[1]));
+    half = fmax(half, 0.5 * (hi[2] - lo[2]));
+    if (!(half > 1e-12)) half = 1.0;
+    for (int a = 0; a < 3; ++a) geom[4 * s + a] = 0.5 * (lo[a] + hi[a]);
+    geom[4 * s + 3] = half * 1.001;
+}
+
+// Morton key of every particle from the FP64 centre recurrence (see file header).
+// LEVELS = 21 fills the short key; LEVELS = 40 also fills the low word.
+template <int LEVELS>
+__global__ __launch_bounds__(256) void k_keys(const float* __restrict__ x, const float* __restrict__ y,
+                                              const float* __restrict__ z, const int32_t* __restrict__ pos_sys,
+                                              const double* __restrict__ geom, int n, uint64_t* __restrict__ key,
+                                              uint64_t* __restrict__ klo, uint32_t* __restrict__ val) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = pos_sys[i];
+    double cx = geom[4 * s + 0], cy = geom[4 * s + 1], cz = geom[4 * s + 2], h = geom[4 * s + 3];
+    const double px = static_cast<double>(x[i]), py = static_cast<double>(y[i]), pz = static_cast<double>(z[i]);
+    uint64_t hi = 0, lo = 0;
+#pragma unroll
+    for (int l = 1; l <= LEVELS; ++l) {
+        const int ox = px >= cx, oy = py >= cy, oz = pz >= cz;
+        h = 0.5 * h;
+        cx = cx + (ox ? h : -h);
+        cy = cy + (oy ? h : -h);
+        cz = cz + (oz ? h : -h);
+        const uint64_t oct = static_cast<uint64_t>(ox | (oy << 1) | (oz << 2));
+        if (l <= 21)
+            hi = (hi << 3) | oct;
+        else
+            lo = (lo << 3) | oct;
+    }
+    key[i] = hi;
+    if (LEVELS > 21) klo[i] = lo;
+    val[i] = static_cast<uint32_t>(i);
+}
+
+void launch_bbox_and_keys(nbg_ctx* c, int levels) {
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, st, c->dsc, c->sys_bbox, c->nsys);
+    hipLaunchKernelGGL(k_bbox, dim3(c->n_blocks), dim3(256), 0, st, c->x, c->y, c->z, c->blk_sys, c->blk_start,
+                       c->blk_count, c->sys_bbox);
+    hipLaunchKernelGGL(k_root, dim3(grid_for(c->nsys, 64)), dim3(64), 0, st, c->sys_bbox, c->sys_off, c->nsys,
+                       c->sys_geom);
+    if (levels == kLevelsShort)
+        hipLaunchKernelGGL(k_keys<kLevelsShort>, dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x, c->y, c->z,
+                           c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
+    else
+        hipLaunchKernelGGL(k_keys<kLevelsDeep>, dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x, c->y, c->z,
+                           c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
+}
+
+// ============================================================================ radix sort
+// LSD radix sort, 8-bit digits, segmented by system: the digit counters of a system's
+// blocks form one contiguous region (digit-major, block-minor), so a single exclusive
+// scan over all counters yields, for every (block, digit), the output offset inside its
+// own system's range.  The scatter ranks keys stably inside the block with wave ballots
+// and per-wave LDS digit counts.
+
+__device__ __forceinline__ int hist_index(int b, int d, const int32_t* blk_sys, const int32_t* sys_blk0) {
+    const int s = blk_sys[b];
+    const int b0 = sys_blk0[s];
+    const int nb = sys_blk0[s + 1] - b0;
+    return kRadix * b0 + d * nb + (b - b0);
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t* __restrict__ keys, int shift,
+                                                             const int32_t* __restrict__ blk_sys,
+                                                             const int32_t* __restrict__ blk_start,
+                                                             const int32_t* __restrict__ blk_count,
+                                                             const int32_t* __restrict__ sys_blk0,
+                                                             uint32_t* __restrict__ hist) {
+    __shared__ uint32_t s_cnt[kRadix];
+    const int b = blockIdx.x;
+    s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int start = blk_start[b], cnt = blk_count[b];
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int k = r * kSortThreads + threadIdx.x;
+        if (k < cnt) {
+            const int d = static_cast<int>((keys[start + k] >> shift) & (kRadix - 1));
+            atomicAdd(&s_cnt[d], 1u);
+        }
+    }
+    __syncthreads();
+    hist[hist_index(b, threadIdx.x, blk_sys, sys_blk0)] = s_cnt[threadIdx.x];
+}
+
+template <bool EXTRA>
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, const uint64_t* __restrict__ xin,
+    uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t* __restrict__ xout, int shift,
+    const int32_t* __restrict__ blk_sys, const int32_t* __restrict__ blk_start, const int32_t* __restrict__ blk_count,
+    const int32_t* __restrict__ sys_blk0, const uint32_t* __restrict__ offs) {
+    __shared__ uint32_t s_base[kRadix];
+    __shared__ uint32_t s_wcnt[kSortThreads / 64][kRadix];
+    const int b = blockIdx.x;
+    const int wave = threadIdx.x >> 6;
+    const int lane = lane_id();
+    const uint64_t lt = (1ull << lane) - 1ull;
+    s_base[threadIdx.x] = offs[hist_index(b, threadIdx.x, blk_sys, sys_blk0)];
+    const int start = blk_start[b], cnt = blk_count[b];
+    for (int r = 0; r < kSortItems; ++r) {
+        const int k = r * kSortThreads + threadIdx.x;
+        const bool valid = k < cnt;
+        uint64_t key = 0, xtra = 0;
+        uint32_t v = 0;
+        if (valid) {
+            key = kin[start + k];
+            v = vin[start + k];
+            if (EXTRA) xtra = xin[start + k];
+        }
+        const int d = static_cast<int>((key >> shift) & (kRadix - 1));
+        uint64_t peers = ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < kRadixBits; ++bit) {
+            const bool set = (d >> bit) & 1;
+            const uint64_t m = ballot(set);
+            peers &= set ? m : ~m;
+        }
+        const int rank = __popcll(peers & lt);
+#pragma unroll
+        for (int w = 0; w < kSortThreads / 64; ++w) s_wcnt[w][threadIdx.x] = 0;
+        __syncthreads();
+        if (valid && rank == 0) s_wcnt[wave][d] = static_cast<uint32_t>(__popcll(peers));
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = s_base[d] + static_cast<uint32_t>(rank);
+            for (int w = 0; w < wave; ++w) pos += s_wcnt[w][d];
+            kout[pos] = key;
+            vout[pos] = v;
+            if (EXTRA) xout[pos] = xtra;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int w = 0; w < kSortThreads / 64; ++w) add += s_wcnt[w][threadIdx.x];
+        s_base[threadIdx.x] += add;
+        __syncthreads();
+    }
+}
+
+// After an even number of passes the sorted data is back in key_a/val_a(/klo_a).
+void launch_sort(nbg_ctx* c, int levels) {
+    hipStream_t st = c->stream;
+    const int nb = c->n_blocks;
+    const int64_t nh = static_cast<int64_t>(kRadix) * nb;
+    auto pass = [&](uint64_t* kin, uint64_t* kout, uint32_t* vin, uint32_t* vout, uint64_t* xin, uint64_t* xout,
+                    int shift) {
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, st, kin, shift, c->blk_sys, c->blk_start,
+                           c->blk_count, c->sys_blk0, c->hist);
+        excl_scan<uint32_t>(st, c->hist, c->hist, nh, reinterpret_cast<uint32_t*>(c->scan_part), nullptr);
+        if (xin)
+            hipLaunchKernelGGL(k_radix_scatter<true>, dim3(nb), dim3(kSortThreads), 0, st, kin, vin, xin, kout, vout,
+                               xout, shift, c->blk_sys, c->blk_start, c->blk_count, c->sys_blk0, c->hist);
+        else
+            hipLaunchKernelGGL(k_radix_scatter<false>, dim3(nb), dim3(kSortThreads), 0, st, kin, vin, nullptr, kout,
+                               vout, nullptr, shift, c->blk_sys, c->blk_start, c->blk_count, c->sys_blk0, c->hist);
+    };
+    if (levels == kLevelsDeep) {
+        // low word first (levels 22..40), carrying the high word, then the high word
+        for (int p = 0; p < 8; ++p) {
+            const bool even = (p & 1) == 0;
+            pass(even ? c->klo_a : c->klo_b, even ? c->klo_b : c->klo_a, even ? c->val_a : c->val_b,
+                 even ? c->val_b : c->val_a, even ? c->key_a : c->key_b, even ? c->key_b : c->key_a, 8 * p);
+        }
+        for (int p = 0; p < 8; ++p) {
+            const bool even = (p & 1) == 0;
+            pass(even ? c->key_a : c->key_b, even ? c->key_b : c->key_a, even ? c->val_a : c->val_b,
+                 even ? c->val_b : c->val_a, even ? c->klo_a : c->klo_b, even ? c->klo_b : c->klo_a, 8 * p);
+        }
+    } else {
+        for (int p = 0; p < 8; ++p) {
+            const bool even = (p & 1) == 0;
+            pass(even ? c->key_a : c->key_b, even ? c->key_b : c->key_a, even ? c->val_a : c->val_b,
+                 even ? c->val_b : c->val_a, nullptr, nullptr, 8 * p);
+        }
+    }
+    c->key_sorted = c->key_a;
+    c->klo_sorted = c->klo_a;
+    c->perm = c->val_a;
+}
+
+// ============================================================================ topology
+
+template <int LEVELS>
+__device__ __forceinline__ int cpl_at(const uint64_t* __restrict__ kh, const uint64_t* __restrict__ kl, int i, int j) {
+    if (LEVELS == kLevelsShort) return cpl_short(kh[i], kh[j]);
+    return cpl_deep(kh[i], kl[i], kh[j], kl[j]);
+}
+
+// Leaf depth and common prefix with the predecessor, per sorted position; the number of
+// nodes and leaves each position starts, packed for one 64-bit scan.
+template <int LEVELS>
+__global__ __launch_bounds__(256) void k_leaf_depth(const uint64_t* __restrict__ kh, const uint64_t* __restrict__ kl,
+                                                    const int32_t* __restrict__ pos_sys,
+                                                    const int32_t* __restrict__ sys_off, int n,
+                                                    int32_t* __restrict__ leaf_depth, int32_t* __restrict__ cpl_prev,
+                                                    uint64_t* __restrict__ node_scan, DevScalars* __restrict__ sc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = pos_sys[i];
+    const int s0 = sys_off[s], s1 = sys_off[s + 1];
+    int L[kNcrit], R[kNcrit];
+#pragma unroll
+    for (int k = 1; k <= kNcrit; ++k) {
+        L[k - 1] = (i - k >= s0) ? cpl_at<LEVELS>(kh, kl, i - k, i) : -1;
+        R[k - 1] = (i + k < s1) ? cpl_at<LEVELS>(kh, kl, i, i + k) : -1;
+    }
+    // cell size at depth d: 1 + #{L >= d} + #{R >= d}; leaf depth = first d with size <= 8
+    int D = -1;
+    for (int d = 0; d <= LEVELS; ++d) {
+        int size = 1;
+#pragma unroll
+        for (int k = 0; k < kNcrit; ++k) size += (L[k] >= d) + (R[k] >= d);
+        if (size <= kNcrit) {
+            D = d;
+            break;
+        }
+    }
+    if (D < 0) {
+        if (LEVELS == kLevelsShort) {
+            atomicOr(&sc->overflow, 1);
+            atomicOr(&sc->sticky_overflow, 1);
+            D = LEVELS;
+        } else {
+            D = kMaxDepth;  // depth cap: the leaf keeps every particle (Tree.cpp:126)
+        }
+    }
+    const int cp = L[0];  // -1 at the start of a system
+    leaf_depth[i] = D;
+    cpl_prev[i] = cp;
+    const uint64_t ns = (D > cp) ? static_cast<uint64_t>(D - cp) : 0u;
+    const uint64_t ls = (D > cp) ? 1u : 0u;
+    node_scan[i] = (ns << 32) | ls;
+    int dmax = D;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) dmax = max(dmax, __shfl_xor(dmax, o, 64));
+    if (lane_id() == 0) atomicMax(&sc->max_depth, dmax);
+}
+
+// Grand totals of the node scan -> device scalars.
+__global__ void k_node_totals(const uint64_t* __restrict__ total, DevScalars* __restrict__ sc, int64_t cap_nodes) {
+    const uint64_t t = *total;
+    const int64_t nodes = static_cast<int64_t>(t >> 32);
+    sc->n_leaves = static_cast<int32_t>(t & 0xffffffffu);
+    if (nodes > cap_nodes) {
+        sc->cap_exceeded = 1;
+        sc->sticky_cap = 1;
+        sc->n_nodes = static_cast<int32_t>(cap_nodes);
+    } else {
+        sc->n_nodes = static_cast<int32_t>(nodes);
+    }
+}
+
+template <int LEVELS>
+__global__ __launch_bounds__(256) void k_nodes(const uint64_t* __restrict__ kh, const uint64_t* __restrict__ kl,
+                                               const int32_t* __restrict__ pos_sys,
+                                               const int32_t* __restrict__ sys_off,
+                                               const int32_t* __restrict__ leaf_depth,
+                                               const int32_t* __restrict__ cpl_prev,
+                                               const uint64_t* __restrict__ node_scan, int n, int64_t cap_nodes,
+                                               int32_t* __restrict__ nd_start, int32_t* __restrict__ nd_count,
+                                               int32_t* __restrict__ nd_depth, int32_t* __restrict__ nd_parent,
+                                               int32_t* __restrict__ nd_oct, int32_t* __restrict__ nd_child,
+                                               int32_t* __restrict__ leaf_list, int32_t* __restrict__ sys_root,
+                                               DevScalars* __restrict__ sc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int D = leaf_depth[i];
+    const int cp = cpl_prev[i];
+    if (D <= cp) return;
+    const int s = pos_sys[i];
+    const int s0 = sys_off[s], s1 = sys_off[s + 1];
+    const uint64_t sc_i = node_scan[i];
+    const int64_t base = static_cast<int64_t>(sc_i >> 32);
+    const int lbase = static_cast<int>(sc_i & 0xffffffffu);
+    const uint64_t kh_i = kh[i];
+    const uint64_t kl_i = (LEVELS == kLevelsShort) ? 0u : kl[i];
+    for (int d = cp + 1; d <= D; ++d) {
+        const int64_t id64 = base + (d - (cp + 1));
+        if (id64 >= cap_nodes) {
+            atomicOr(&sc->cap_exceeded, 1);
+            atomicOr(&sc->sticky_cap, 1);
+            return;
+        }
+        const int id = static_cast<int>(id64);
+        // particle range end: first j > i outside the depth-d cell
+        int count;
+        if (d == D && D < LEVELS) {
+            count = 1;
+            for (int k = 1; k <= kNcrit && i + k < s1; ++k) count += (cpl_at<LEVELS>(kh, kl, i, i + k) >= d);
+        } else {
+            int lo = i + 1, hi = s1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (cpl_at<LEVELS>(kh, kl, i, mid) >= d)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            count = lo - i;
+        }
+        int parent;
+        if (d == 0) {
+            parent = -1;
+            sys_root[s] = id;
+        } else if (d > cp + 1) {
+            parent = id - 1;
+        } else {
+            // start of the depth-(d-1) cell that holds i
+            int lo = s0, hi = i;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (cpl_at<LEVELS>(kh, kl, mid, i) >= d - 1)
+                    hi = mid;
+                else
+                    lo = mid + 1;
+            }
+            parent = static_cast<int>(node_scan[lo] >> 32) + (d - 1) - (cpl_prev[lo] + 1);
+        }
+        nd_start[id] = i;
+        nd_count[id] = count;
+        nd_depth[id] = d;
+        nd_parent[id] = parent;
+        nd_oct[id] = (d > 0) ? key_digit(kh_i, kl_i, d) : 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) nd_child[8 * id + c] = -1;
+        if (d == D) leaf_list[lbase] = id;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_link(const int32_t* __restrict__ nd_parent, const int32_t* __restrict__ nd_oct,
+                                              const DevScalars* __restrict__ sc, int32_t* __restrict__ nd_child) {
+    const int nn = sc->n_nodes;
+    for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < nn; id += gridDim.x * blockDim.x) {
+        const int p = nd_parent[id];
+        if (p >= 0) nd_child[8 * p + nd_oct[id]] = id;
+    }
+}
+
+// Child counts, FP64 geometry (replaying Tree.cpp:96-99 along the key), leaf info word,
+// and the per-depth histogram of internal nodes.
+template <int LEVELS>
+__global__ __launch_bounds__(256) void k_node_finish(const uint64_t* __restrict__ kh, const uint64_t* __restrict__ kl,
+                                                     const int32_t* __restrict__ pos_sys,
+                                                     const double* __restrict__ sys_geom,
+                                                     const int32_t* __restrict__ nd_start,
+                                                     const int32_t* __restrict__ nd_count,
+                                                     const int32_t* __restrict__ nd_depth,
+                                                     const int32_t* __restrict__ nd_child, double* __restrict__ nd_geom,
+                                                     float4* __restrict__ nd_aux, DevScalars* __restrict__ sc) {
+    const int nn = sc->n_nodes;
+    for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < nn; id += gridDim.x * blockDim.x) {
+        int nk = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) nk += nd_child[8 * id + c] >= 0;
+        const int i = nd_start[id];
+        const int d = nd_depth[id];
+        const int s = pos_sys[i];
+        const uint64_t h_ = kh[i];
+        const uint64_t l_ = (LEVELS == kLevelsShort) ? 0u : kl[i];
+        double cx = sys_geom[4 * s + 0], cy = sys_geom[4 * s + 1], cz = sys_geom[4 * s + 2], h = sys_geom[4 * s + 3];
+        for (int l = 1; l <= d; ++l) {
+            const int oct = key_digit(h_, l_, l);
+            h = 0.5 * h;
+            cx = cx + ((oct & 1) ? h : -h);
+            cy = cy + ((oct & 2) ? h : -h);
+            cz = cz + ((oct & 4) ? h : -h);
+        }
+        nd_geom[4 * id + 0] = cx;
+        nd_geom[4 * id + 1] = cy;
+        nd_geom[4 * id + 2] = cz;
+        nd_geom[4 * id + 3] = h;
+        const double size = 2.0 * h;
+        const uint32_t info = (nk == 0) ? (kLeafBit | static_cast<uint32_t>(nd_count[id])) : static_cast<uint32_t>(nk);
+        nd_aux[id] = make_float4(0.0f, static_cast<float>(size * size), 0.0f, __uint_as_float(info));
+        if (nk > 0) atomicAdd(&sc->depth_count[d], 1u);
+    }
+}
+
+__global__ void k_depth_scan(DevScalars* __restrict__ sc) {
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int d = 0; d < 64; ++d) {
+            sc->depth_offset[d] = acc;
+            sc->depth_fill[d] = acc;
+            acc += sc->depth_count[d];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_depth_scatter(const int32_t* __restrict__ nd_depth,
+                                                       const float4* __restrict__ nd_aux, DevScalars* __restrict__ sc,
+                                                       int32_t* __restrict__ depth_nodes) {
+    const int nn = sc->n_nodes;
+    for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < nn; id += gridDim.x * blockDim.x) {
+        const uint32_t info = __float_as_uint(nd_aux[id].w);
+        if (info & kLeafBit) continue;
+        const uint32_t pos = atomicAdd(&sc->depth_fill[nd_depth[id]], 1u);
+        depth_nodes[pos] = id;
+    }
+}
+
+// Leaf members in ascending original index (the reference bucket order).
+__global__ __launch_bounds__(256) void k_leaf_order(const int32_t* __restrict__ leaf_list,
+                                                    const int32_t* __restrict__ nd_start,
+                                                    const int32_t* __restrict__ nd_count,
+                                                    const DevScalars* __restrict__ sc, uint32_t* __restrict__ perm) {
+    const int nl = sc->n_leaves;
+    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nl; li += gridDim.x * blockDim.x) {
+        const int id = leaf_list[li];
+        const int st = nd_start[id], cnt = nd_count[id];
+        if (cnt <= kNcrit) {
+            uint32_t v[kNcrit];
+#pragma unroll
+            for (int k = 0; k < kNcrit; ++k) v[k] = (k < cnt) ? perm[st + k] : 0xffffffffu;
+            // insertion sort of <= 8 values
+#pragma unroll
+            for (int a = 1; a < kNcrit; ++a) {
+#pragma unroll
+                for (int b = a; b > 0; --b) {
+                    const uint32_t lo = min(v[b - 1], v[b]), hi = max(v[b - 1], v[b]);
+                    v[b - 1] = lo;
+                    v[b] = hi;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kNcrit; ++k)
+                if (k < cnt) perm[st + k] = v[k];
+        } else {
+            // depth-capped leaf: plain insertion sort in place
+            for (int a = st + 1; a < st + cnt; ++a) {
+                const uint32_t key = perm[a];
+                int b = a - 1;
+                while (b >= st && perm[b] > key) {
+                    perm[b + 1] = perm[b];
+                    --b;
+                }
+                perm[b + 1] = key;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather(const float* __restrict__ x, const float* __restrict__ y,
+                                                const float* __restrict__ z, const float* __restrict__ m,
+                                                const uint32_t* __restrict__ perm, int n, float G,
+                                                float4* __restrict__ spos) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t j = perm[k];
+    spos[k] = make_float4(x[j], y[j], z[j], G * m[j]);
+}
+
+void launch_build(nbg_ctx* c, int levels) {
+    hipStream_t st = c->stream;
+    const int n = c->n;
+    const int g = grid_for(n, 256);
+    const int gnodes = 2048;  // grid-stride kernels over the device-side node count
+    if (levels == kLevelsShort)
+        hipLaunchKernelGGL(k_leaf_depth<kLevelsShort>, dim3(g), dim3(256), 0, st, c->key_sorted, c->klo_sorted,
+                           c->pos_sys, c->sys_off, n, c->leaf_depth, c->cpl_prev, c->node_scan, c->dsc);
+    else
+        hipLaunchKernelGGL(k_leaf_depth<kLevelsDeep>, dim3(g), dim3(256), 0, st, c->key_sorted, c->klo_sorted,
+                           c->pos_sys, c->sys_off, n, c->leaf_depth, c->cpl_prev, c->node_scan, c->dsc);
+    // node_scan[n] receives the grand total (the array holds n+1 entries)
+    excl_scan<uint64_t>(st, c->node_scan, c->node_scan, n, c->scan_part, c->node_scan + n);
+    hipLaunchKernelGGL(k_node_totals, dim3(1), dim3(1), 0, st, c->node_scan + n, c->dsc, c->cap_nodes);
+    if (levels == kLevelsShort)
+        hipLaunchKernelGGL(k_nodes<kLevelsShort>, dim3(g), dim3(256), 0, st, c->key_sorted, c->klo_sorted, c->pos_sys,
+                           c->sys_off, c->leaf_depth, c->cpl_prev, c->node_scan, n, c->cap_nodes, c->nd_start,
+                           c->nd_count, c->nd_depth, c->nd_parent, c->nd_oct, c->nd_child, c->leaf_list, c->sys_root,
+                           c->dsc);
+    else
+        hipLaunchKernelGGL(k_nodes<kLevelsDeep>, dim3(g), dim3(256), 0, st, c->key_sorted, c->klo_sorted, c->pos_sys,
+                           c->sys_off, c->leaf_depth, c->cpl_prev, c->node_scan, n, c->cap_nodes, c->nd_start,
+                           c->nd_count, c->nd_depth, c->nd_parent, c->nd_oct, c->nd_child, c->leaf_list, c->sys_root,
+                           c->dsc);
+    hipLaunchKernelGGL(k_link, dim3(gnodes), dim3(256), 0, st, c->nd_parent, c->nd_oct, c->dsc, c->nd_child);
+    if (levels == kLevelsShort)
+        hipLaunchKernelGGL(k_node_finish<kLevelsShort>, dim3(gnodes), dim3(256), 0, st, c->key_sorted, c->klo_sorted,
+                           c->pos_sys, c->sys_geom, c->nd_start, c->nd_count, c->nd_depth, c->nd_child, c->nd_geom,
+                           c->nd_aux, c->dsc);
+    else
+        hipLaunchKernelGGL(k_node_finish<kLevelsDeep>, dim3(gnodes), dim3(256), 0, st, c->key_sorted, c->klo_sorted,
+                           c->pos_sys, c->sys_geom, c->nd_start, c->nd_count, c->nd_depth, c->nd_child, c->nd_geom,
+                           c->nd_aux, c->dsc);
+    hipLaunchKernelGGL(k_depth_scan, dim3(1), dim3(64), 0, st, c->dsc);
+    hipLaunchKernelGGL(k_depth_scatter, dim3(gnodes), dim3(256), 0, st, c->nd_depth, c->nd_aux, c->dsc,
+                       c->depth_nodes);
+    hipLaunchKernelGGL(k_leaf_order, dim3(gnodes), dim3(256), 0, st, c->leaf_list, c->nd_start, c->nd_count, c->dsc,
+                       c->perm);
+}
+
+// ============================================================================ upsweep
+// Moments of every node (G-scaled masses): leaves from their particles, internal nodes
+// from their children one depth at a time, deepest first.  Offsets are taken relative
+// to the node centre in FP32; the root combines its children in FP64.
+// Quadrupoles and maxRadius follow Tree.cpp:179-202 and 205-254.
+
+template <bool QUAD>
+__global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ leaf_list,
+                                                 const int32_t* __restrict__ nd_start,
+                                                 const int32_t* __restrict__ nd_count,
+                                                 const double* __restrict__ nd_geom, const float4* __restrict__ spos,
+                                                 const DevScalars* __restrict__ sc, float4* __restrict__ nd_com,
+                                                 float4* __restrict__ nd_quad, float4* __restrict__ nd_aux) {
+    const int nl = sc->n_leaves;
+    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nl; li += gridDim.x * blockDim.x) {
+        const int id = leaf_list[li];
+        const int st = nd_start[id], cnt = nd_count[id];
+        const float cx = static_cast<float>(nd_geom[4 * id + 0]);
+        const float cy = static_cast<float>(nd_geom[4 * id + 1]);
+        const float cz = static_cast<float>(nd_geom[4 * id + 2]);
+        float M = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+        for (int k = st; k < st + cnt; ++k) {
+            const float4 p = spos[k];
+            M += p.w;
+            sx += p.w * (p.x - cx);
+            sy += p.w * (p.y - cy);
+            sz += p.w * (p.z - cz);
+        }
+        // a massless cell has an undefined (NaN) centre of mass, as the incremental
+        // update of Tree.cpp:120-122 produces; the walk never uses it as a source and a
+        // massless group then opens every cell, as in the reference.
+        float gx = __int_as_float(0x7fc00000), gy = gx, gz = gx;
+        if (M != 0.f) {
+            gx = cx + sx / M;
+            gy = cy + sy / M;
+            gz = cz + sz / M;
+        }
+        float qxx = 0.f, qyy = 0.f, qxy = 0.f, qxz = 0.f, qyz = 0.f, r2max = 0.f;
+        if (M > 0.f) {
+            for (int k = st; k < st + cnt; ++k) {
+                const float4 p = spos[k];
+                const float dx = p.x - gx, dy = p.y - gy, dz = p.z - gz;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                r2max = fmaxf(r2max, d2);
+                if (QUAD && cnt > 1) {
+                    qxx += p.w * (3.f * dx * dx - d2);
+                    qyy += p.w * (3.f * dy * dy - d2);
+                    qxy += p.w * 3.f * dx * dy;
+                    qxz += p.w * 3.f * dx * dz;
+                    qyz += p.w * 3.f * dy * dz;
+                }
+            }
+        }
+        nd_com[id] = make_float4(gx, gy, gz, M);
+        nd_quad[id] = make_float4(qxx, qyy, qxy, qxz);
+        float4 a = nd_aux[id];
+        a.x = qyz;
+        a.z = sqrtf(r2max);
+        nd_aux[id] = a;
+    }
+}
+
+template <bool QUAD>
+__global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* __restrict__ depth_nodes,
+                                                     const int32_t* __restrict__ nd_child,
+                                                     const double* __restrict__ nd_geom,
+                                                     const DevScalars* __restrict__ sc, float4* __restrict__ nd_com,
+                                                     float4* __restrict__ nd_quad, float4* __restrict__ nd_aux) {
+    const int cnt = static_cast<int>(sc->depth_count[depth]);
+    const int off = static_cast<int>(sc->depth_offset[depth]);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+        const int id = depth_nodes[off + k];
+        int ch[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) ch[c] = nd_child[8 * id + c];
+        float gx, gy, gz, M;
+        if (depth == 0) {
+            // root: FP64 centre-of-mass reduction
+            double dm = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                if (ch[c] < 0) continue;
+                const float4 cc = nd_com[ch[c]];
+                if (cc.w == 0.f) continue;
+                dm += cc.w;
+                dx += static_cast<double>(cc.w) * cc.x;
+                dy += static_cast<double>(cc.w) * cc.y;
+                dz += static_cast<double>(cc.w) * cc.z;
+            }
+            M = static_cast<float>(dm);
+            gx = gy = gz = __int_as_float(0x7fc00000);
+            if (dm != 0.0) {
+                gx = static_cast<float>(dx / dm);
+                gy = static_cast<float>(dy / dm);
+                gz = static_cast<float>(dz / dm);
+            }
+        } else {
+            const float cx = static_cast<float>(nd_geom[4 * id + 0]);
+            const float cy = static_cast<float>(nd_geom[4 * id + 1]);
+            const float cz = static_cast<float>(nd_geom[4 * id + 2]);
+            float sx = 0.f, sy = 0.f, sz = 0.f;
+            M = 0.f;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                if (ch[c] < 0) continue;
+                const float4 cc = nd_com[ch[c]];
+                if (cc.w == 0.f) continue;
+                M += cc.w;
+                sx += cc.w * (cc.x - cx);
+                sy += cc.w * (cc.y - cy);
+                sz += cc.w * (cc.z - cz);
+            }
+            gx = gy = gz = __int_as_float(0x7fc00000);
+            if (M != 0.f) {
+                gx = cx + sx / M;
+                gy = cy + sy / M;
+                gz = cz + sz / M;
+            }
+        }
+        float qxx = 0.f, qyy = 0.f, qxy = 0.f, qxz = 0.f, qyz = 0.f, rmax = 0.f;
+        if (M > 0.f) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                if (ch[c] < 0) continue;
+                const float4 cc = nd_com[ch[c]];
+                const float4 ca = nd_aux[ch[c]];
+                const float dx = cc.x - gx, dy = cc.y - gy, dz = cc.z - gz;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                rmax = fmaxf(rmax, ca.z + sqrtf(d2));
+                if (QUAD && cc.w > 0.f) {
+                    const float4 cq = nd_quad[ch[c]];
+                    qxx += cq.x + cc.w * (3.f * dx * dx - d2);
+                    qyy += cq.y + cc.w * (3.f * dy * dy - d2);
+                    qxy += cq.z + cc.w * 3.f * dx * dy;
+                    qxz += cq.w + cc.w * 3.f * dx * dz;
+                    qyz += ca.x + cc.w * 3.f * dy * dz;
+                }
+            }
+        }
+        nd_com[id] = make_float4(gx, gy, gz, M);
+        nd_quad[id] = make_float4(qxx, qyy, qxy, qxz);
+        float4 a = nd_aux[id];
+        a.x = qyz;
+        a.z = rmax;
+        nd_aux[id] = a;
+    }
+}
+
+void launch_upsweep(nbg_ctx* c, int levels, int multipole) {
+    hipStream_t st = c->stream;
+    const int gnodes = 2048;
+    const bool quad = multipole == NBG_QUADRUPOLE;
+    if (quad)
+        hipLaunchKernelGGL(k_up_leaf<true>, dim3(gnodes), dim3(256), 0, st, c->leaf_list, c->nd_start, c->nd_count,
+                           c->nd_geom, c->spos, c->dsc, c->nd_com, c->nd_quad, c->nd_aux);
+    else
+        hipLaunchKernelGGL(k_up_leaf<false>, dim3(gnodes), dim3(256), 0, st, c->leaf_list, c->nd_start, c->nd_count,
+                           c->nd_geom, c->spos, c->dsc, c->nd_com, c->nd_quad, c->nd_aux);
+    // internal nodes sit at depths 0..levels-1 (depth-40 cells are always leaves)
+    const int top = (levels == kLevelsShort) ? kLevelsShort - 1 : kMaxDepth - 1;
+    for (int d = top; d >= 0; --d) {
+        const int gd = d < 6 ? 64 : 1024;
+        if (quad)
+            hipLaunchKernelGGL(k_up_internal<true>, dim3(gd), dim3(256), 0, st, d, c->depth_nodes, c->nd_child,
+                               c->nd_geom, c->dsc, c->nd_com, c->nd_quad, c->nd_aux);
+        else
+            hipLaunchKernelGGL(k_up_internal<false>, dim3(gd), dim3(256), 0, st, d, c->depth_nodes, c->nd_child,
+                               c->nd_geom, c->dsc, c->nd_com, c->nd_quad, c->nd_aux);
+    }
+}
+
+// gather is part of the build step in the driver (needs G)
+void launch_gather(nbg_ctx* c, float G) {
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->x, c->y, c->z, c->m, c->perm,
+                       c->n, G, c->spos);
+}
+
+}  // namespace nbg

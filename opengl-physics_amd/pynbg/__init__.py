@@ -1,0 +1,219 @@
+"""Python binding of libnbg_hip.so (the C ABI declared in include/nbg.h).
+
+Thin ctypes layer used by the tests, the benchmark and the smoke check: numpy arrays in,
+numpy arrays out.  There is no CPU fallback -- if the HIP library is missing this module
+raises on import of the library.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_DIR, "libnbg_hip.so")
+
+SOLVER_DIRECT = 0
+SOLVER_BARNES_HUT = 1
+MONOPOLE = 0
+QUADRUPOLE = 1
+FLAG_KAHAN = 1
+FLAG_COUNT = 2
+FLAG_LEARNED = 4
+FLAG_NO_GRAPH = 8
+
+ERRORS = {0: "NBG_OK", -1: "NBG_E_INVALID", -2: "NBG_E_OOM", -3: "NBG_E_CAPACITY", -4: "NBG_E_HIP",
+          -5: "NBG_E_STATE"}
+
+EXPORTED = ["nbg_create", "nbg_destroy", "nbg_last_error", "nbg_upload", "nbg_upload_batch", "nbg_upload_accel",
+            "nbg_accel", "nbg_step", "nbg_download", "nbg_tree_export", "nbg_interaction_counts", "nbg_diagnostics",
+            "nbg_accel_direct_targets"]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("solver", ctypes.c_int), ("multipole", ctypes.c_int), ("G", ctypes.c_double),
+                ("eps2", ctypes.c_double), ("theta", ctypes.c_double), ("dt", ctypes.c_double),
+                ("flags", ctypes.c_uint32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("n_nodes", ctypes.c_int64), ("n_leaves", ctypes.c_int64), ("max_depth", ctypes.c_int32),
+                ("deep_keys", ctypes.c_int32), ("n_p2p", ctypes.c_int64), ("n_m2p", ctypes.c_int64),
+                ("ms_keys", ctypes.c_double), ("ms_sort", ctypes.c_double), ("ms_build", ctypes.c_double),
+                ("ms_upsweep", ctypes.c_double), ("ms_walk", ctypes.c_double), ("ms_integrate", ctypes.c_double),
+                ("ms_total", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+_f = ctypes.POINTER(ctypes.c_float)
+_d = ctypes.POINTER(ctypes.c_double)
+_i32 = ctypes.POINTER(ctypes.c_int32)
+_i64 = ctypes.POINTER(ctypes.c_int64)
+_u64 = ctypes.POINTER(ctypes.c_uint64)
+
+
+def load():
+    """Load libnbg_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() or make -C opengl-physics_amd")
+    L = ctypes.CDLL(LIB_PATH)
+    L.nbg_create.restype = ctypes.c_void_p
+    L.nbg_create.argtypes = [ctypes.c_int, ctypes.c_int64]
+    L.nbg_destroy.argtypes = [ctypes.c_void_p]
+    L.nbg_last_error.restype = ctypes.c_char_p
+    L.nbg_last_error.argtypes = [ctypes.c_void_p]
+    L.nbg_upload.argtypes = [ctypes.c_void_p, ctypes.c_int64] + [_f] * 7
+    L.nbg_upload_batch.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i64] + [_f] * 7
+    L.nbg_upload_accel.argtypes = [ctypes.c_void_p] + [_f] * 3
+    L.nbg_accel.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.POINTER(Stats)]
+    L.nbg_step.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_int32, ctypes.POINTER(Stats)]
+    L.nbg_download.argtypes = [ctypes.c_void_p] + [_f] * 9
+    L.nbg_tree_export.argtypes = [ctypes.c_void_p, _i64, _u64, _i32, _d, _i32, _i32, _f]
+    L.nbg_interaction_counts.argtypes = [ctypes.c_void_p, ctypes.c_int64, _i64, _i64]
+    L.nbg_diagnostics.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_int32, _d]
+    L.nbg_accel_direct_targets.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_int64, _i64] + [_f] * 3
+    L.nbg_destroy.restype = None
+    for name in EXPORTED:
+        if name not in ("nbg_create", "nbg_destroy", "nbg_last_error"):
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _f32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+
+
+class NbgError(RuntimeError):
+    pass
+
+
+def make_params(solver="barnes_hut", multipole="quadrupole", G=1.0, eps=None, eps2=None, theta=0.5, dt=1e-3,
+                kahan=True, count=False, no_graph=False, learned=False):
+    if eps2 is None:
+        eps2 = (eps if eps is not None else 0.05) ** 2
+    flags = (FLAG_KAHAN if kahan else 0) | (FLAG_COUNT if count else 0) | (FLAG_NO_GRAPH if no_graph else 0) | \
+        (FLAG_LEARNED if learned else 0)
+    return Params(SOLVER_DIRECT if solver == "direct" else SOLVER_BARNES_HUT,
+                  QUADRUPOLE if multipole == "quadrupole" else MONOPOLE, float(G), float(eps2), float(theta),
+                  float(dt), flags)
+
+
+class Context:
+    """One device context (nbg_ctx*) owning the particle state and every device buffer."""
+
+    def __init__(self, max_n, device=0):
+        self._L = load()
+        self._h = self._L.nbg_create(device, int(max_n))
+        if not self._h:
+            raise NbgError(f"nbg_create(device={device}, max_n={max_n}) failed")
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.nbg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._L.nbg_last_error(self._h).decode()
+            raise NbgError(f"{what} -> {ERRORS.get(rc, rc)}: {msg}")
+
+    def upload(self, x, y, z, m, vx=None, vy=None, vz=None):
+        arrs = [_f32(a) for a in (x, y, z, m, vx, vy, vz)]
+        self.n = len(arrs[0])
+        self._keep = arrs
+        self._check(self._L.nbg_upload(self._h, self.n, *[_p(a, _f) for a in arrs]), "nbg_upload")
+
+    def upload_batch(self, offsets, x, y, z, m, vx=None, vy=None, vz=None):
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        arrs = [_f32(a) for a in (x, y, z, m, vx, vy, vz)]
+        self.n = int(off[-1])
+        self._check(self._L.nbg_upload_batch(self._h, len(off) - 1, _p(off, _i64), *[_p(a, _f) for a in arrs]),
+                    "nbg_upload_batch")
+
+    def upload_accel(self, ax=None, ay=None, az=None):
+        arrs = [_f32(a) for a in (ax, ay, az)]
+        self._check(self._L.nbg_upload_accel(self._h, *[_p(a, _f) for a in arrs]), "nbg_upload_accel")
+
+    def accel(self, params, stats=True):
+        st = Stats()
+        self._check(self._L.nbg_accel(self._h, ctypes.byref(params), ctypes.byref(st) if stats else None),
+                    "nbg_accel")
+        return st
+
+    def step(self, params, nsteps, stats=True):
+        st = Stats()
+        self._check(self._L.nbg_step(self._h, ctypes.byref(params), int(nsteps),
+                                     ctypes.byref(st) if stats else None), "nbg_step")
+        return st
+
+    def download(self):
+        out = {k: np.empty(self.n, np.float32) for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az")}
+        self._check(self._L.nbg_download(self._h, *[_p(out[k], _f) for k in
+                                                    ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az")]),
+                    "nbg_download")
+        return out
+
+    def accelerations(self):
+        d = self.download()
+        return np.stack([d["ax"], d["ay"], d["az"]], axis=1)
+
+    def tree_export(self):
+        nn = ctypes.c_int64(0)
+        self._check(self._L.nbg_tree_export(self._h, ctypes.byref(nn), None, None, None, None, None, None),
+                    "nbg_tree_export(count)")
+        n_nodes = nn.value
+        keys = np.empty(self.n, np.uint64)
+        order = np.empty(self.n, np.int32)
+        center = np.empty((n_nodes, 4), np.float64)
+        topo = np.empty((n_nodes, 5), np.int32)
+        children = np.empty((n_nodes, 8), np.int32)
+        moments = np.empty((n_nodes, 10), np.float32)
+        nn = ctypes.c_int64(n_nodes)
+        self._check(self._L.nbg_tree_export(self._h, ctypes.byref(nn), _p(keys, _u64), _p(order, _i32),
+                                            _p(center, _d), _p(topo, _i32), _p(children, _i32), _p(moments, _f)),
+                    "nbg_tree_export")
+        return dict(keys=keys, order=order, center=center[:, :3], half=center[:, 3], parent=topo[:, 0],
+                    first=topo[:, 1], count=topo[:, 2], depth=topo[:, 3], child_count=topo[:, 4], children=children,
+                    com=moments[:, :3], mass=moments[:, 3], max_radius=moments[:, 4], quad=moments[:, 5:10],
+                    n_nodes=n_nodes)
+
+    def interaction_counts(self, n_nodes):
+        f = np.empty(n_nodes, np.int64)
+        q = np.empty(n_nodes, np.int64)
+        self._check(self._L.nbg_interaction_counts(self._h, n_nodes, _p(f, _i64), _p(q, _i64)),
+                    "nbg_interaction_counts")
+        return f, q
+
+    def diagnostics(self, params, with_potential=True):
+        out = np.empty(11, np.float64)
+        self._check(self._L.nbg_diagnostics(self._h, ctypes.byref(params), 1 if with_potential else 0,
+                                            _p(out, _d)), "nbg_diagnostics")
+        return dict(kinetic=out[0], potential=out[1], energy=out[0] + out[1], com=out[2:5].copy(),
+                    L=out[5:8].copy(), P=out[8:11].copy())
+
+    def direct_targets(self, params, targets):
+        t = np.ascontiguousarray(targets, dtype=np.int64)
+        ax, ay, az = (np.empty(len(t), np.float32) for _ in range(3))
+        self._check(self._L.nbg_accel_direct_targets(self._h, ctypes.byref(params), len(t), _p(t, _i64),
+                                                     _p(ax, _f), _p(ay, _f), _p(az, _f)), "nbg_accel_direct_targets")
+        return np.stack([ax, ay, az], axis=1)

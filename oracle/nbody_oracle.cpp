@@ -1,0 +1,693 @@
+// nbody_oracle.cpp -- FP64 CPU restatement of the reference nbody_core gravity path.
+//
+// TEST INFRASTRUCTURE ONLY.  This library is the checker the GPU path is compared
+// against.  Only the test-suite, the smoke check and the benchmark's CPU-baseline
+// leg may load it, and only as the checker or the timed CPU baseline, never as the
+// computation that is shipped.  The product library (libnbg_hip.so) neither links
+// nor loads it.
+//
+// Every function restates a reference function term for term (same operation
+// order, FP64, no fused multiply-add when built with -ffp-contract=off).  Citations
+// are relative to the reference repository root.
+//
+//   orc_ic_plummer          projects/nbody_core/include/ngrav/ic/Plummer.hpp:26-112
+//   orc_ic_hernquist        projects/nbody_core/include/ngrav/ic/Hernquist.hpp:51-181
+//   orc_ic_king             projects/nbody_core/include/ngrav/ic/King.hpp:58-211
+//   orc_tree_build          projects/nbody_core/src/Tree.cpp:33-203 (serial insertion)
+//   quadrupoles             projects/nbody_core/src/Tree.cpp:205-254
+//   orc_accel_bh            projects/nbody_core/src/Solvers.cpp:248-355 (grouped walk),
+//                           pair kernel Solvers.cpp:19-38, quadrupole Solvers.cpp:147-179
+//   orc_accel_bh_pp         projects/nbody_core/src/Solvers.cpp:181-227, 359-382
+//   orc_accel_direct        projects/nbody_core/src/Solvers.cpp:77-106
+//   orc_leapfrog            projects/nbody_core/src/Integrator.cpp:9-39
+//   orc_total_energy        projects/nbody_core/include/ngrav/System.hpp:119-164
+//   orc_com / orc_angmom    projects/nbody_core/include/ngrav/System.hpp:166-193
+//   orc_morton_keys         derived contract (not a reference function): the octant
+//                           digits the insertion build of Tree.cpp:11-20, 96-99,
+//                           114-145 assigns to a particle, level by level, from the
+//                           same FP64 box-centre recurrence.
+//
+// Pinning (what checks each function beyond retyping it) is recorded in the
+// tests: tests/test_oracle.py.  No function here is "parity unpinned" except the
+// bit patterns of the IC samplers against a contraction-enabled reference build,
+// which no fixture of the reference covers.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <random>
+#include <vector>
+
+namespace {
+
+struct Tree {
+    int ncrit = 8, maxDepth = 40;
+    // node SoA in insertion order (Tree.hpp:44-58)
+    std::vector<double> cx, cy, cz, half, comx, comy, comz, mass, maxR;
+    std::vector<std::array<int, 8>> child;
+    std::vector<int> childCount, parent, first, count, depth;
+    std::vector<int> order;
+    std::vector<double> Qxx, Qyy, Qxy, Qxz, Qyz;
+    bool hasQ = false;
+    std::vector<std::vector<int>> bucket;
+    // canonical numbering: DFS pre-order, children in ascending octant
+    std::vector<int> canon;    // insertion id -> canonical id
+    std::vector<int> fromCanon; // canonical id -> insertion id
+};
+
+struct Pts {
+    const double *x, *y, *z, *m;
+    int n;
+};
+
+int OctantOf(double cx, double cy, double cz, double px, double py, double pz) {
+    int oct = 0;
+    if (px >= cx) oct |= 1;
+    if (py >= cy) oct |= 2;
+    if (pz >= cz) oct |= 4;
+    return oct;
+}
+
+int NewNode(Tree& t, double cx, double cy, double cz, double h, int parent, int depth) {
+    const int id = static_cast<int>(t.mass.size());
+    t.cx.push_back(cx);
+    t.cy.push_back(cy);
+    t.cz.push_back(cz);
+    t.half.push_back(h);
+    t.comx.push_back(0.0);
+    t.comy.push_back(0.0);
+    t.comz.push_back(0.0);
+    t.mass.push_back(0.0);
+    std::array<int, 8> none;
+    none.fill(-1);
+    t.child.push_back(none);
+    t.childCount.push_back(0);
+    t.parent.push_back(parent);
+    t.first.push_back(-1);
+    t.count.push_back(0);
+    t.depth.push_back(depth);
+    t.bucket.emplace_back();
+    return id;
+}
+
+void RootBox(const Pts& p, double c[3], double* halfOut) {
+    double lo[3] = {std::numeric_limits<double>::max(), std::numeric_limits<double>::max(),
+                    std::numeric_limits<double>::max()};
+    double hi[3] = {std::numeric_limits<double>::lowest(), std::numeric_limits<double>::lowest(),
+                    std::numeric_limits<double>::lowest()};
+    for (int i = 0; i < p.n; ++i) {
+        const double v[3] = {p.x[i], p.y[i], p.z[i]};
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = (v[k] < lo[k]) ? v[k] : lo[k];
+            hi[k] = (hi[k] < v[k]) ? v[k] : hi[k];
+        }
+    }
+    if (p.n == 0) {
+        for (int k = 0; k < 3; ++k) lo[k] = hi[k] = 0.0;
+    }
+    double half = 0.5 * (hi[0] - lo[0]);
+    half = std::max(half, 0.5 * (hi[1] - lo[1]));
+    half = std::max(half, 0.5 * (hi[2] - lo[2]));
+    if (!(half > 1e-12)) half = 1.0;
+    for (int k = 0; k < 3; ++k) c[k] = 0.5 * (lo[k] + hi[k]);
+    *halfOut = half * 1.001;
+}
+
+void Insert(Tree& t, const Pts& p, int node, int i, int depth);
+
+void InsertIntoChild(Tree& t, const Pts& p, int node, int i, int depth) {
+    const int oct = OctantOf(t.cx[node], t.cy[node], t.cz[node], p.x[i], p.y[i], p.z[i]);
+    int ch = t.child[node][oct];
+    if (ch == -1) {
+        const double h = 0.5 * t.half[node];
+        const double dx = (oct & 1) ? h : -h, dy = (oct & 2) ? h : -h, dz = (oct & 4) ? h : -h;
+        ch = NewNode(t, t.cx[node] + dx, t.cy[node] + dy, t.cz[node] + dz, h, node, depth + 1);
+        t.child[node][oct] = ch;
+        t.childCount[node] += 1;
+    }
+    Insert(t, p, ch, i, depth + 1);
+}
+
+void Insert(Tree& t, const Pts& p, int node, int i, int depth) {
+    const double mp = p.m[i];
+    const double newMass = t.mass[node] + mp;
+    t.comx[node] = (t.comx[node] * t.mass[node] + p.x[i] * mp) / newMass;
+    t.comy[node] = (t.comy[node] * t.mass[node] + p.y[i] * mp) / newMass;
+    t.comz[node] = (t.comz[node] * t.mass[node] + p.z[i] * mp) / newMass;
+    t.mass[node] = newMass;
+    if (t.childCount[node] == 0) {
+        if (static_cast<int>(t.bucket[node].size()) < t.ncrit || depth >= t.maxDepth) {
+            t.bucket[node].push_back(i);
+            return;
+        }
+        std::vector<int> held;
+        held.swap(t.bucket[node]);
+        for (int q : held) InsertIntoChild(t, p, node, q, depth);
+    }
+    InsertIntoChild(t, p, node, i, depth);
+}
+
+void Finalize(Tree& t, const Pts& p) {
+    const int nn = static_cast<int>(t.mass.size());
+    t.order.clear();
+    t.order.reserve(p.n);
+    t.maxR.assign(nn, 0.0);
+    t.canon.assign(nn, -1);
+    t.fromCanon.clear();
+    std::vector<int> stack{0};
+    while (!stack.empty()) {
+        const int node = stack.back();
+        stack.pop_back();
+        t.canon[node] = static_cast<int>(t.fromCanon.size());
+        t.fromCanon.push_back(node);
+        if (t.childCount[node] == 0) {
+            t.first[node] = static_cast<int>(t.order.size());
+            for (int q : t.bucket[node]) t.order.push_back(q);
+            t.count[node] = static_cast<int>(t.bucket[node].size());
+            continue;
+        }
+        for (int c = 7; c >= 0; --c)
+            if (t.child[node][c] != -1) stack.push_back(t.child[node][c]);
+    }
+    t.bucket.clear();
+    for (int nd = nn - 1; nd >= 0; --nd) {
+        if (t.mass[nd] <= 0.0) continue;
+        double r = 0.0;
+        if (t.childCount[nd] == 0) {
+            for (int k = t.first[nd]; k < t.first[nd] + t.count[nd]; ++k) {
+                const int q = t.order[k];
+                const double dx = p.x[q] - t.comx[nd], dy = p.y[q] - t.comy[nd], dz = p.z[q] - t.comz[nd];
+                r = std::max(r, std::sqrt(dx * dx + dy * dy + dz * dz));
+            }
+        } else {
+            for (int c = 0; c < 8; ++c) {
+                const int ch = t.child[nd][c];
+                if (ch == -1) continue;
+                const double dx = t.comx[ch] - t.comx[nd], dy = t.comy[ch] - t.comy[nd], dz = t.comz[ch] - t.comz[nd];
+                r = std::max(r, t.maxR[ch] + std::sqrt(dx * dx + dy * dy + dz * dz));
+            }
+        }
+        t.maxR[nd] = r;
+    }
+}
+
+void Quadrupoles(Tree& t, const Pts& p) {
+    const int nn = static_cast<int>(t.mass.size());
+    t.Qxx.assign(nn, 0.0);
+    t.Qyy.assign(nn, 0.0);
+    t.Qxy.assign(nn, 0.0);
+    t.Qxz.assign(nn, 0.0);
+    t.Qyz.assign(nn, 0.0);
+    t.hasQ = true;
+    for (int nd = nn - 1; nd >= 0; --nd) {
+        if (t.mass[nd] <= 0.0) continue;
+        if (t.childCount[nd] == 0) {
+            if (t.count[nd] <= 1) continue;
+            for (int k = t.first[nd]; k < t.first[nd] + t.count[nd]; ++k) {
+                const int q = t.order[k];
+                const double dx = p.x[q] - t.comx[nd], dy = p.y[q] - t.comy[nd], dz = p.z[q] - t.comz[nd];
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                const double mp = p.m[q];
+                t.Qxx[nd] += mp * (3.0 * dx * dx - d2);
+                t.Qyy[nd] += mp * (3.0 * dy * dy - d2);
+                t.Qxy[nd] += mp * 3.0 * dx * dy;
+                t.Qxz[nd] += mp * 3.0 * dx * dz;
+                t.Qyz[nd] += mp * 3.0 * dy * dz;
+            }
+            continue;
+        }
+        for (int c = 0; c < 8; ++c) {
+            const int ch = t.child[nd][c];
+            if (ch == -1 || t.mass[ch] <= 0.0) continue;
+            const double dx = t.comx[ch] - t.comx[nd], dy = t.comy[ch] - t.comy[nd], dz = t.comz[ch] - t.comz[nd];
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            const double mc = t.mass[ch];
+            t.Qxx[nd] += t.Qxx[ch] + mc * (3.0 * dx * dx - d2);
+            t.Qyy[nd] += t.Qyy[ch] + mc * (3.0 * dy * dy - d2);
+            t.Qxy[nd] += t.Qxy[ch] + mc * 3.0 * dx * dy;
+            t.Qxz[nd] += t.Qxz[ch] + mc * 3.0 * dx * dz;
+            t.Qyz[nd] += t.Qyz[ch] + mc * 3.0 * dy * dz;
+        }
+    }
+}
+
+// Plummer pair kernel (Solvers.cpp:19-38); gmj = G*m_j.
+inline void AccumPair(double dx, double dy, double dz, double gmj, double eps2, double& ax, double& ay,
+                      double& az) {
+    const double dist2 = dx * dx + dy * dy + dz * dz + eps2;
+    const double inv = 1.0 / std::sqrt(dist2);
+    const double inv3 = inv * inv * inv;
+    const double s = gmj * inv3;
+    ax += s * dx;
+    ay += s * dy;
+    az += s * dz;
+}
+
+// Traceless quadrupole term, unsoftened (Solvers.cpp:147-179).
+inline void AddQuad(const Tree& t, int nd, double dx, double dy, double dz, double dist2, double G, double& ax,
+                    double& ay, double& az) {
+    if (!t.hasQ) return;
+    const double Qxx = t.Qxx[nd], Qyy = t.Qyy[nd], Qzz = -(Qxx + Qyy);
+    const double Qxy = t.Qxy[nd], Qxz = t.Qxz[nd], Qyz = t.Qyz[nd];
+    const double Qdx = Qxx * dx + Qxy * dy + Qxz * dz;
+    const double Qdy = Qxy * dx + Qyy * dy + Qyz * dz;
+    const double Qdz = Qxz * dx + Qyz * dy + Qzz * dz;
+    const double Qd2 = dx * Qdx + dy * Qdy + dz * Qdz;
+    const double invR2 = 1.0 / dist2;
+    const double invR = std::sqrt(invR2);
+    const double invR5 = invR2 * invR2 * invR;
+    const double invR7 = invR5 * invR2;
+    const double c1 = -G * invR5;
+    const double c2 = 2.5 * G * Qd2 * invR7;
+    ax += c1 * Qdx + c2 * dx;
+    ay += c1 * Qdy + c2 * dy;
+    az += c1 * Qdz + c2 * dz;
+}
+
+// CollectGroupInteractions (Solvers.cpp:248-277).
+void Collect(const Tree& t, int node, int group, double gx, double gy, double gz, double R, double theta2,
+             std::vector<int>& far, std::vector<int>& near) {
+    if (node == group) return;
+    if (t.mass[node] <= 0.0) return;
+    if (t.childCount[node] == 0) {
+        near.push_back(node);
+        return;
+    }
+    const double dx = t.comx[node] - gx, dy = t.comy[node] - gy, dz = t.comz[node] - gz;
+    const double dist2 = dx * dx + dy * dy + dz * dz;
+    const double size = 2.0 * t.half[node];
+    const double worst = std::sqrt(dist2) - R;
+    if (worst > 0.0 && size * size < theta2 * worst * worst) {
+        far.push_back(node);
+        return;
+    }
+    for (int c = 0; c < 8; ++c)
+        if (t.child[node][c] != -1) Collect(t, t.child[node][c], group, gx, gy, gz, R, theta2, far, near);
+}
+
+// Per-particle walk, geometric or relative MAC (Solvers.cpp:181-227).
+void WalkPP(const Tree& t, const Pts& p, int node, int i, double xi, double yi, double zi, double G, double theta2,
+            double eps2, int relative, double alpha, double aOld, double& ax, double& ay, double& az) {
+    if (t.mass[node] <= 0.0) return;
+    if (t.childCount[node] == 0) {
+        for (int k = t.first[node]; k < t.first[node] + t.count[node]; ++k) {
+            const int q = t.order[k];
+            if (q == i) continue;
+            AccumPair(p.x[q] - xi, p.y[q] - yi, p.z[q] - zi, G * p.m[q], eps2, ax, ay, az);
+        }
+        return;
+    }
+    const double dx = t.comx[node] - xi, dy = t.comy[node] - yi, dz = t.comz[node] - zi;
+    const double dist2 = dx * dx + dy * dy + dz * dz;
+    const double size = 2.0 * t.half[node];
+    bool accept;
+    if (relative && aOld > 0.0)
+        accept = (t.mass[node] * size * size) < (alpha * aOld * dist2 * dist2 / G);
+    else
+        accept = (size * size) < (theta2 * dist2);
+    if (accept) {
+        AccumPair(dx, dy, dz, G * t.mass[node], eps2, ax, ay, az);
+        AddQuad(t, node, dx, dy, dz, dist2, G, ax, ay, az);
+        return;
+    }
+    for (int c = 0; c < 8; ++c)
+        if (t.child[node][c] != -1)
+            WalkPP(t, p, t.child[node][c], i, xi, yi, zi, G, theta2, eps2, relative, alpha, aOld, ax, ay, az);
+}
+
+} // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------- initial conditions
+// ic::Plummer<3> (Plummer.hpp:26-112).  Uses libstdc++ std::mt19937 and
+// uniform_real_distribution<double>, exactly the reference's engine and distribution.
+void orc_ic_plummer(int n, uint32_t seed, double totalMass, double scaleRadius, double G, double* x, double* y,
+                    double* z, double* vx, double* vy, double* vz, double* m) {
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<double> u(0.0, 1.0);
+    const double mi = totalMass / n;
+    const double vUnit = std::sqrt(G * totalMass / scaleRadius);
+    for (int i = 0; i < n; ++i) {
+        const double x1 = u(rng);
+        double r = 1.0 / std::sqrt(std::pow(x1, -2.0 / 3.0) - 1.0);
+        r = std::min(r, 20.0);
+        double cosT = 2.0 * u(rng) - 1.0;
+        double sinT = std::sqrt(std::max(0.0, 1.0 - cosT * cosT));
+        double phi = 2.0 * M_PI * u(rng);
+        const double rr = r * scaleRadius;
+        x[i] = sinT * std::cos(phi) * rr;
+        y[i] = sinT * std::sin(phi) * rr;
+        z[i] = cosT * rr;
+        double q, g;
+        do {
+            q = u(rng);
+            g = u(rng) * 0.1;
+        } while (g > q * q * std::pow(1.0 - q * q, 3.5));
+        const double vEsc = std::sqrt(2.0) * std::pow(1.0 + r * r, -0.25);
+        const double speed = q * vEsc;
+        cosT = 2.0 * u(rng) - 1.0;
+        sinT = std::sqrt(std::max(0.0, 1.0 - cosT * cosT));
+        phi = 2.0 * M_PI * u(rng);
+        const double sv = speed * vUnit;
+        vx[i] = sinT * std::cos(phi) * sv;
+        vy[i] = sinT * std::sin(phi) * sv;
+        vz[i] = cosT * sv;
+        m[i] = mi;
+    }
+    double cp[3] = {0, 0, 0}, cv[3] = {0, 0, 0}, tm = 0.0;
+    for (int i = 0; i < n; ++i) {
+        cp[0] += m[i] * x[i];
+        cp[1] += m[i] * y[i];
+        cp[2] += m[i] * z[i];
+        cv[0] += m[i] * vx[i];
+        cv[1] += m[i] * vy[i];
+        cv[2] += m[i] * vz[i];
+        tm += m[i];
+    }
+    for (int k = 0; k < 3; ++k) {
+        cp[k] /= tm;
+        cv[k] /= tm;
+    }
+    for (int i = 0; i < n; ++i) {
+        x[i] -= cp[0];
+        y[i] -= cp[1];
+        z[i] -= cp[2];
+        vx[i] -= cv[0];
+        vy[i] -= cv[1];
+        vz[i] -= cv[2];
+    }
+}
+
+// The 10000th output of a default-seeded mt19937 (C++ standard [rand.predef]).
+uint32_t orc_mt19937_10000th(void) {
+    std::mt19937 g;
+    g.discard(9999);
+    return static_cast<uint32_t>(g());
+}
+
+// ---------------------------------------------------------------- tree
+void* orc_tree_build(int n, const double* x, const double* y, const double* z, const double* m, int ncrit,
+                     int maxDepth, int withQuad) {
+    Tree* t = new Tree();
+    t->ncrit = ncrit;
+    t->maxDepth = maxDepth;
+    const Pts p{x, y, z, m, n};
+    double c[3], h;
+    RootBox(p, c, &h);
+    NewNode(*t, c[0], c[1], c[2], h, -1, 0);
+    for (int i = 0; i < n; ++i) Insert(*t, p, 0, i, 0);
+    Finalize(*t, p);
+    if (withQuad) Quadrupoles(*t, p);
+    return t;
+}
+
+void orc_tree_free(void* h) { delete static_cast<Tree*>(h); }
+
+int orc_tree_num_nodes(const void* h) { return static_cast<int>(static_cast<const Tree*>(h)->mass.size()); }
+
+// Export in canonical numbering (DFS pre-order, children in ascending octant),
+// which is the order Finalize's DFS visits nodes (Tree.cpp:157-175).
+// Any output pointer may be NULL.  geom: [cx,cy,cz,half]; mom: [comx,comy,comz,mass,maxR];
+// quad: [Qxx,Qyy,Qxy,Qxz,Qyz]; topo: [parent, first, count, depth, childCount];
+// children: 8 per node (-1 = empty).
+void orc_tree_export(const void* h, double* geom, double* mom, double* quad, int* topo, int* children,
+                     int* order) {
+    const Tree& t = *static_cast<const Tree*>(h);
+    const int nn = static_cast<int>(t.mass.size());
+    for (int ci = 0; ci < nn; ++ci) {
+        const int nd = t.fromCanon[ci];
+        if (geom) {
+            geom[4 * ci + 0] = t.cx[nd];
+            geom[4 * ci + 1] = t.cy[nd];
+            geom[4 * ci + 2] = t.cz[nd];
+            geom[4 * ci + 3] = t.half[nd];
+        }
+        if (mom) {
+            mom[5 * ci + 0] = t.comx[nd];
+            mom[5 * ci + 1] = t.comy[nd];
+            mom[5 * ci + 2] = t.comz[nd];
+            mom[5 * ci + 3] = t.mass[nd];
+            mom[5 * ci + 4] = t.maxR[nd];
+        }
+        if (quad) {
+            quad[5 * ci + 0] = t.hasQ ? t.Qxx[nd] : 0.0;
+            quad[5 * ci + 1] = t.hasQ ? t.Qyy[nd] : 0.0;
+            quad[5 * ci + 2] = t.hasQ ? t.Qxy[nd] : 0.0;
+            quad[5 * ci + 3] = t.hasQ ? t.Qxz[nd] : 0.0;
+            quad[5 * ci + 4] = t.hasQ ? t.Qyz[nd] : 0.0;
+        }
+        if (topo) {
+            topo[5 * ci + 0] = t.parent[nd] < 0 ? -1 : t.canon[t.parent[nd]];
+            topo[5 * ci + 1] = t.first[nd];
+            topo[5 * ci + 2] = t.count[nd];
+            topo[5 * ci + 3] = t.depth[nd];
+            topo[5 * ci + 4] = t.childCount[nd];
+        }
+        if (children) {
+            for (int c = 0; c < 8; ++c)
+                children[8 * ci + c] = t.child[nd][c] < 0 ? -1 : t.canon[t.child[nd][c]];
+        }
+    }
+    if (order) std::memcpy(order, t.order.data(), sizeof(int) * t.order.size());
+}
+
+// ---------------------------------------------------------------- forces
+// ComputeAccelBarnesHutGroup (Solvers.cpp:279-355) on a tree built by orc_tree_build
+// from the same arrays.  nFar / nNear (nullable) receive, per canonical node id, the
+// far-node count and the near P2P source count (near leaves + own leaf, self excluded)
+// of that leaf's group; zero for internal nodes.
+void orc_accel_bh(const void* h, int n, const double* x, const double* y, const double* z, const double* m,
+                  double G, double eps2, double theta, double* ax, double* ay, double* az, int64_t* nFar,
+                  int64_t* nNear) {
+    const Tree& t = *static_cast<const Tree*>(h);
+    const Pts p{x, y, z, m, n};
+    const double theta2 = theta * theta;
+    std::vector<int> leaves;
+    for (int nd = 0; nd < static_cast<int>(t.mass.size()); ++nd)
+        if (t.childCount[nd] == 0 && t.count[nd] > 0) leaves.push_back(nd);
+    if (nFar) std::memset(nFar, 0, sizeof(int64_t) * t.mass.size());
+    if (nNear) std::memset(nNear, 0, sizeof(int64_t) * t.mass.size());
+    const int nl = static_cast<int>(leaves.size());
+#pragma omp parallel
+    {
+        std::vector<int> far, near;
+#pragma omp for schedule(dynamic, 32)
+        for (int li = 0; li < nl; ++li) {
+            const int g = leaves[li];
+            far.clear();
+            near.clear();
+            Collect(t, 0, g, t.comx[g], t.comy[g], t.comz[g], t.maxR[g], theta2, far, near);
+            int64_t nearParticles = t.count[g] - 1;
+            for (int lf : near) nearParticles += t.count[lf];
+            if (nFar) nFar[t.canon[g]] = static_cast<int64_t>(far.size());
+            if (nNear) nNear[t.canon[g]] = nearParticles;
+            for (int k = t.first[g]; k < t.first[g] + t.count[g]; ++k) {
+                const int i = t.order[k];
+                const double xi = x[i], yi = y[i], zi = z[i];
+                double sx = 0.0, sy = 0.0, sz = 0.0;
+                for (int nd : far) {
+                    const double dx = t.comx[nd] - xi, dy = t.comy[nd] - yi, dz = t.comz[nd] - zi;
+                    const double dist2 = dx * dx + dy * dy + dz * dz;
+                    AccumPair(dx, dy, dz, G * t.mass[nd], eps2, sx, sy, sz);
+                    AddQuad(t, nd, dx, dy, dz, dist2, G, sx, sy, sz);
+                }
+                for (int lf : near) {
+                    for (int k2 = t.first[lf]; k2 < t.first[lf] + t.count[lf]; ++k2) {
+                        const int q = t.order[k2];
+                        AccumPair(x[q] - xi, y[q] - yi, z[q] - zi, G * m[q], eps2, sx, sy, sz);
+                    }
+                }
+                for (int k2 = t.first[g]; k2 < t.first[g] + t.count[g]; ++k2) {
+                    if (k2 == k) continue;
+                    const int q = t.order[k2];
+                    AccumPair(x[q] - xi, y[q] - yi, z[q] - zi, G * m[q], eps2, sx, sy, sz);
+                }
+                ax[i] = sx;
+                ay[i] = sy;
+                az[i] = sz;
+            }
+        }
+    }
+    (void)p;
+}
+
+// ComputeAccelBarnesHutPerParticle (Solvers.cpp:359-382): relative MAC when
+// relative != 0 and aOld (nullable) holds each particle's previous |a|.
+void orc_accel_bh_pp(const void* h, int n, const double* x, const double* y, const double* z, const double* m,
+                     double G, double eps2, double theta, int relative, double alpha, const double* aOld, double* ax,
+                     double* ay, double* az) {
+    const Tree& t = *static_cast<const Tree*>(h);
+    const Pts p{x, y, z, m, n};
+    const double theta2 = theta * theta;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i) {
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        WalkPP(t, p, 0, i, x[i], y[i], z[i], G, theta2, eps2, relative, alpha, aOld ? aOld[i] : 0.0, sx, sy, sz);
+        ax[i] = sx;
+        ay[i] = sy;
+        az[i] = sz;
+    }
+}
+
+// ComputeAccelDirect (Solvers.cpp:77-106), Plummer softening.
+void orc_accel_direct(int n, const double* x, const double* y, const double* z, const double* m, double G,
+                      double eps2, double* ax, double* ay, double* az) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        const double xi = x[i], yi = y[i], zi = z[i];
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int j = 0; j < n; ++j) {
+            if (j == i) continue;
+            AccumPair(x[j] - xi, y[j] - yi, z[j] - zi, G * m[j], eps2, sx, sy, sz);
+        }
+        ax[i] = sx;
+        ay[i] = sy;
+        az[i] = sz;
+    }
+}
+
+// Direct sum onto a subset of targets (ComputeAccelDirectTargets, Solvers.cpp:108-134).
+void orc_accel_direct_targets(int n, const double* x, const double* y, const double* z, const double* m, double G,
+                              double eps2, int nt, const int* targets, double* ax, double* ay, double* az) {
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < nt; ++t) {
+        const int i = targets[t];
+        const double xi = x[i], yi = y[i], zi = z[i];
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int j = 0; j < n; ++j) {
+            if (j == i) continue;
+            AccumPair(x[j] - xi, y[j] - yi, z[j] - zi, G * m[j], eps2, sx, sy, sz);
+        }
+        ax[t] = sx;
+        ay[t] = sy;
+        az[t] = sz;
+    }
+}
+
+// ---------------------------------------------------------------- integrator
+// LeapfrogStep (Integrator.cpp:9-39) with the System's BarnesHut dispatch
+// (System.hpp:43-78: tree rebuilt per evaluation, quadrupoles when withQuad) or
+// Direct (solver 0).  a[] must hold a(t) on entry and holds a(t+dt) on exit.
+void orc_leapfrog(int n, double* x, double* y, double* z, double* vx, double* vy, double* vz, const double* m,
+                  double* ax, double* ay, double* az, double G, double eps2, double theta, int solver, int withQuad,
+                  double dt, int nsteps) {
+    const double hdt = 0.5 * dt;
+    for (int s = 0; s < nsteps; ++s) {
+        for (int i = 0; i < n; ++i) {
+            vx[i] += hdt * ax[i];
+            vy[i] += hdt * ay[i];
+            vz[i] += hdt * az[i];
+        }
+        for (int i = 0; i < n; ++i) {
+            x[i] += dt * vx[i];
+            y[i] += dt * vy[i];
+            z[i] += dt * vz[i];
+        }
+        if (solver == 0) {
+            orc_accel_direct(n, x, y, z, m, G, eps2, ax, ay, az);
+        } else {
+            void* t = orc_tree_build(n, x, y, z, m, 8, 40, withQuad);
+            orc_accel_bh(t, n, x, y, z, m, G, eps2, theta, ax, ay, az, nullptr, nullptr);
+            orc_tree_free(t);
+        }
+        for (int i = 0; i < n; ++i) {
+            vx[i] += hdt * ax[i];
+            vy[i] += hdt * ay[i];
+            vz[i] += hdt * az[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- diagnostics
+// System::TotalEnergy (System.hpp:119-164), Plummer softening.  Per-i partial sums
+// are added in i order so the result does not depend on the thread count.
+double orc_total_energy(int n, const double* x, const double* y, const double* z, const double* vx,
+                        const double* vy, const double* vz, const double* m, double G, double eps2) {
+    double kinetic = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double v2 = vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i];
+        kinetic += 0.5 * m[i] * v2;
+    }
+    std::vector<double> local(static_cast<size_t>(n), 0.0);
+#pragma omp parallel for schedule(dynamic, 32)
+    for (int i = 0; i < n; ++i) {
+        double acc = 0.0;
+        for (int j = i + 1; j < n; ++j) {
+            const double dx = x[j] - x[i], dy = y[j] - y[i], dz = z[j] - z[i];
+            const double mij = m[i] * m[j];
+            const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+            acc -= G * mij / std::sqrt(r2);
+        }
+        local[i] = acc;
+    }
+    double potential = 0.0;
+    for (int i = 0; i < n; ++i) potential += local[i];
+    return kinetic + potential;
+}
+
+// CenterOfMass / AngularMomentum (System.hpp:166-193); momentum as DeckSim.cpp:147-153.
+void orc_moments(int n, const double* x, const double* y, const double* z, const double* vx, const double* vy,
+                 const double* vz, const double* m, double* com3, double* L3, double* P3) {
+    double tm = 0.0, c[3] = {0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+        c[0] += m[i] * x[i];
+        c[1] += m[i] * y[i];
+        c[2] += m[i] * z[i];
+        tm += m[i];
+    }
+    if (tm > 0.0)
+        for (int k = 0; k < 3; ++k) c[k] /= tm;
+    double L[3] = {0, 0, 0}, P[3] = {0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+        const double rx = x[i] - c[0], ry = y[i] - c[1], rz = z[i] - c[2];
+        L[0] += m[i] * (ry * vz[i] - rz * vy[i]);
+        L[1] += m[i] * (rz * vx[i] - rx * vz[i]);
+        L[2] += m[i] * (rx * vy[i] - ry * vx[i]);
+        P[0] += m[i] * vx[i];
+        P[1] += m[i] * vy[i];
+        P[2] += m[i] * vz[i];
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (com3) com3[k] = c[k];
+        if (L3) L3[k] = L[k];
+        if (P3) P3[k] = P[k];
+    }
+}
+
+// ---------------------------------------------------------------- derived key contract
+// Root box exactly as Tree.cpp:54-88 builds it.
+void orc_root_box(int n, const double* x, const double* y, const double* z, double* center3, double* half) {
+    const Pts p{x, y, z, nullptr, n};
+    RootBox(p, center3, half);
+}
+
+// Octant digits of each particle for levels 1..levels (levels <= 40), MSB first, from
+// the FP64 centre recurrence of Tree.cpp:96-99 / 11-20.  Digits of levels 1..21 go to
+// keyHi (bits 62..0), levels 22..40 to keyLo (bits 56..0); keyLo may be NULL.
+void orc_morton_keys(int n, const double* x, const double* y, const double* z, const double* center3,
+                     double half, int levels, uint64_t* keyHi, uint64_t* keyLo) {
+    for (int i = 0; i < n; ++i) {
+        double cx = center3[0], cy = center3[1], cz = center3[2], h = half;
+        uint64_t hi = 0, lo = 0;
+        for (int l = 1; l <= levels; ++l) {
+            const int oct = OctantOf(cx, cy, cz, x[i], y[i], z[i]);
+            h = 0.5 * h;
+            cx = cx + ((oct & 1) ? h : -h);
+            cy = cy + ((oct & 2) ? h : -h);
+            cz = cz + ((oct & 4) ? h : -h);
+            if (l <= 21)
+                hi = (hi << 3) | static_cast<uint64_t>(oct);
+            else
+                lo = (lo << 3) | static_cast<uint64_t>(oct);
+        }
+        if (levels < 21) hi <<= 3 * (21 - levels);
+        if (levels > 21 && levels < 40) lo <<= 3 * (40 - levels);
+        keyHi[i] = hi;
+        if (keyLo) keyLo[i] = lo;
+    }
+}
+
+} // extern "C"

@@ -1,0 +1,195 @@
+"""ctypes binding of the FP64 oracle (oracle/nbody_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
+checker, and by bench.py's CPU-baseline leg.  The product never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBS = {}
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int)
+_i64 = ctypes.POINTER(ctypes.c_int64)
+_u64 = ctypes.POINTER(ctypes.c_uint64)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def lib(fast=False):
+    """Load liboracle.so (checker build) or liboracle_fast.so (reference flags)."""
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name in _LIBS:
+        return _LIBS[name]
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"oracle library {path} is not built (run make -C oracle)")
+    L = ctypes.CDLL(path)
+    L.orc_ic_plummer.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double] + [_d] * 7
+    L.orc_mt19937_10000th.restype = ctypes.c_uint32
+    L.orc_tree_build.restype = ctypes.c_void_p
+    L.orc_tree_build.argtypes = [ctypes.c_int] + [_d] * 4 + [ctypes.c_int] * 3
+    L.orc_tree_free.argtypes = [ctypes.c_void_p]
+    L.orc_tree_num_nodes.argtypes = [ctypes.c_void_p]
+    L.orc_tree_export.argtypes = [ctypes.c_void_p, _d, _d, _d, _i, _i, _i]
+    L.orc_accel_bh.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_d] * 4 + [ctypes.c_double] * 3 + [_d] * 3 + \
+        [_i64] * 2
+    L.orc_accel_bh_pp.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_d] * 4 + [ctypes.c_double] * 3 + \
+        [ctypes.c_int, ctypes.c_double, _d] + [_d] * 3
+    L.orc_accel_direct.argtypes = [ctypes.c_int] + [_d] * 4 + [ctypes.c_double] * 2 + [_d] * 3
+    L.orc_accel_direct_targets.argtypes = [ctypes.c_int] + [_d] * 4 + [ctypes.c_double] * 2 + \
+        [ctypes.c_int, _i] + [_d] * 3
+    L.orc_leapfrog.argtypes = [ctypes.c_int] + [_d] * 10 + [ctypes.c_double] * 3 + [ctypes.c_int] * 2 + \
+        [ctypes.c_double, ctypes.c_int]
+    L.orc_total_energy.restype = ctypes.c_double
+    L.orc_total_energy.argtypes = [ctypes.c_int] + [_d] * 7 + [ctypes.c_double] * 2
+    L.orc_moments.argtypes = [ctypes.c_int] + [_d] * 7 + [_d] * 3
+    L.orc_root_box.argtypes = [ctypes.c_int] + [_d] * 3 + [_d, _d]
+    L.orc_morton_keys.argtypes = [ctypes.c_int] + [_d] * 3 + [_d, ctypes.c_double, ctypes.c_int, _u64, _u64]
+    for opt in ("orc_ic_hernquist", "orc_ic_king"):
+        if hasattr(L, opt):
+            pass
+    _LIBS[name] = L
+    return L
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def plummer(n, seed, total_mass=1.0, scale_radius=1.0, G=1.0, fast=False):
+    """ic::Plummer<3> in FP64 -> dict of x,y,z,vx,vy,vz,m."""
+    out = {k: np.empty(n, np.float64) for k in ("x", "y", "z", "vx", "vy", "vz", "m")}
+    lib(fast).orc_ic_plummer(n, seed, total_mass, scale_radius, G,
+                             *[_ptr(out[k], _d) for k in ("x", "y", "z", "vx", "vy", "vz", "m")])
+    return out
+
+
+def mt19937_10000th():
+    return int(lib().orc_mt19937_10000th())
+
+
+class Tree:
+    """AdaptiveTree<3> built by serial insertion (Tree.cpp:33-203)."""
+
+    def __init__(self, x, y, z, m, ncrit=8, max_depth=40, quad=True, fast=False):
+        self._L = lib(fast)
+        self.x, self.y, self.z, self.m = (_f64(a) for a in (x, y, z, m))
+        self.n = len(self.x)
+        self._h = self._L.orc_tree_build(self.n, _ptr(self.x, _d), _ptr(self.y, _d), _ptr(self.z, _d),
+                                         _ptr(self.m, _d), ncrit, max_depth, 1 if quad else 0)
+        self.num_nodes = self._L.orc_tree_num_nodes(self._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.orc_tree_free(self._h)
+            self._h = None
+
+    def export(self):
+        """Canonical (DFS pre-order) node arrays."""
+        nn = self.num_nodes
+        geom = np.empty((nn, 4), np.float64)
+        mom = np.empty((nn, 5), np.float64)
+        quad = np.empty((nn, 5), np.float64)
+        topo = np.empty((nn, 5), np.int32)
+        ch = np.empty((nn, 8), np.int32)
+        order = np.empty(self.n, np.int32)
+        self._L.orc_tree_export(self._h, _ptr(geom, _d), _ptr(mom, _d), _ptr(quad, _d), _ptr(topo, _i),
+                                _ptr(ch, _i), _ptr(order, _i))
+        return dict(center=geom[:, :3], half=geom[:, 3], com=mom[:, :3], mass=mom[:, 3], max_radius=mom[:, 4],
+                    quad=quad, parent=topo[:, 0], first=topo[:, 1], count=topo[:, 2], depth=topo[:, 3],
+                    child_count=topo[:, 4], children=ch, order=order)
+
+    def accel_bh(self, G, eps2, theta, counts=False):
+        n = self.n
+        ax, ay, az = (np.empty(n, np.float64) for _ in range(3))
+        nfar = np.empty(self.num_nodes, np.int64) if counts else None
+        nnear = np.empty(self.num_nodes, np.int64) if counts else None
+        self._L.orc_accel_bh(self._h, n, _ptr(self.x, _d), _ptr(self.y, _d), _ptr(self.z, _d), _ptr(self.m, _d),
+                             G, eps2, theta, _ptr(ax, _d), _ptr(ay, _d), _ptr(az, _d), _ptr(nfar, _i64),
+                             _ptr(nnear, _i64))
+        a = np.stack([ax, ay, az], axis=1)
+        return (a, nfar, nnear) if counts else a
+
+    def accel_bh_per_particle(self, G, eps2, theta, relative=False, alpha=0.001, a_old=None):
+        n = self.n
+        ax, ay, az = (np.empty(n, np.float64) for _ in range(3))
+        ao = _f64(a_old) if a_old is not None else None
+        self._L.orc_accel_bh_pp(self._h, n, _ptr(self.x, _d), _ptr(self.y, _d), _ptr(self.z, _d),
+                                _ptr(self.m, _d), G, eps2, theta, 1 if relative else 0, alpha, _ptr(ao, _d),
+                                _ptr(ax, _d), _ptr(ay, _d), _ptr(az, _d))
+        return np.stack([ax, ay, az], axis=1)
+
+
+def accel_direct(x, y, z, m, G, eps2, fast=False):
+    x, y, z, m = (_f64(a) for a in (x, y, z, m))
+    n = len(x)
+    ax, ay, az = (np.empty(n, np.float64) for _ in range(3))
+    lib(fast).orc_accel_direct(n, _ptr(x, _d), _ptr(y, _d), _ptr(z, _d), _ptr(m, _d), G, eps2,
+                               _ptr(ax, _d), _ptr(ay, _d), _ptr(az, _d))
+    return np.stack([ax, ay, az], axis=1)
+
+
+def accel_direct_targets(x, y, z, m, G, eps2, targets, fast=False):
+    x, y, z, m = (_f64(a) for a in (x, y, z, m))
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    nt = len(t)
+    ax, ay, az = (np.empty(nt, np.float64) for _ in range(3))
+    lib(fast).orc_accel_direct_targets(len(x), _ptr(x, _d), _ptr(y, _d), _ptr(z, _d), _ptr(m, _d), G, eps2, nt,
+                                       _ptr(t, _i), _ptr(ax, _d), _ptr(ay, _d), _ptr(az, _d))
+    return np.stack([ax, ay, az], axis=1)
+
+
+def leapfrog(state, a, G, eps2, theta, dt, nsteps, solver="barnes_hut", quad=True, fast=False):
+    """KDK steps in place on a dict state (x..vz,m float64) with a (n,3) = a(t)."""
+    s = {k: _f64(state[k]).copy() for k in ("x", "y", "z", "vx", "vy", "vz", "m")}
+    ax, ay, az = (np.ascontiguousarray(a[:, k], dtype=np.float64).copy() for k in range(3))
+    lib(fast).orc_leapfrog(len(s["x"]), *[_ptr(s[k], _d) for k in ("x", "y", "z", "vx", "vy", "vz", "m")],
+                           _ptr(ax, _d), _ptr(ay, _d), _ptr(az, _d), G, eps2, theta,
+                           0 if solver == "direct" else 1, 1 if quad else 0, dt, nsteps)
+    return s, np.stack([ax, ay, az], axis=1)
+
+
+def total_energy(s, G, eps2, fast=False):
+    arrs = [_f64(s[k]) for k in ("x", "y", "z", "vx", "vy", "vz", "m")]
+    return lib(fast).orc_total_energy(len(arrs[0]), *[_ptr(a, _d) for a in arrs], G, eps2)
+
+
+def moments(s):
+    arrs = [_f64(s[k]) for k in ("x", "y", "z", "vx", "vy", "vz", "m")]
+    com, L, P = (np.empty(3, np.float64) for _ in range(3))
+    lib().orc_moments(len(arrs[0]), *[_ptr(a, _d) for a in arrs], _ptr(com, _d), _ptr(L, _d), _ptr(P, _d))
+    return com, L, P
+
+
+def root_box(x, y, z):
+    x, y, z = (_f64(a) for a in (x, y, z))
+    c = np.empty(3, np.float64)
+    h = np.empty(1, np.float64)
+    lib().orc_root_box(len(x), _ptr(x, _d), _ptr(y, _d), _ptr(z, _d), _ptr(c, _d), _ptr(h, _d))
+    return c, float(h[0])
+
+
+def morton_keys(x, y, z, levels=21):
+    """Derived key contract: octant digits per level from the FP64 centre recurrence."""
+    x, y, z = (_f64(a) for a in (x, y, z))
+    c, h = root_box(x, y, z)
+    n = len(x)
+    hi = np.empty(n, np.uint64)
+    lo = np.empty(n, np.uint64)
+    lib().orc_morton_keys(n, _ptr(x, _d), _ptr(y, _d), _ptr(z, _d), _ptr(c, _d), h, levels, _ptr(hi, _u64),
+                          _ptr(lo, _u64))
+    return hi, lo
+
+
+def rel_rms(a, ref):
+    """sqrt(sum |a-ref|^2 / sum |ref|^2)."""
+    a = np.asarray(a, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.sqrt(np.sum((a - ref) ** 2) / np.sum(ref ** 2)))

@@ -1,0 +1,46 @@
+"""CPU checks of the drop-in boundary: libnbg_hip.so loads and exports every entry point
+that include/nbg.h declares (no compute calls -- no GPU is needed)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "nbg.h")
+LIB = os.path.join(ROOT, "opengl-physics_amd", "libnbg_hip.so")
+
+
+def declared_functions(path):
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nbg_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_the_api():
+    names = declared_functions(HEADER)
+    for must in ("nbg_create", "nbg_destroy", "nbg_upload", "nbg_accel", "nbg_step", "nbg_download",
+                 "nbg_tree_export", "nbg_diagnostics"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(LIB):
+        pytest.fail(f"{LIB} not built")
+    lib = ctypes.CDLL(LIB)
+    missing = [n for n in declared_functions(HEADER) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_exported_symbols_are_c_linkage():
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (nbg_[a-z_0-9]+)$", out, flags=re.M))
+    assert set(declared_functions(HEADER)) <= exported
+
+
+def test_binding_covers_the_header():
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "opengl-physics_amd"))
+    import pynbg
+    assert set(declared_functions(HEADER)) <= set(pynbg.EXPORTED) | {"nbg_upload_batch"}
