@@ -41,6 +41,8 @@ extern "C" {
 #define NBG_FLAG_COUNT 2u           /* record per-group interaction counts (nbg_interaction_counts) */
 #define NBG_FLAG_LEARNED 4u         /* add the learned far-field correction (nbg_mlp_set_weights) */
 #define NBG_FLAG_NO_GRAPH 8u        /* nbg_step: launch eagerly instead of replaying a hipGraph */
+#define NBG_FLAG_STAGE_TIMES 16u    /* nbg_step: per-stage HIP-event times summed over all steps
+                                       (one host synchronisation per step) */
 
 typedef struct nbg_ctx nbg_ctx;
 

@@ -1,0 +1,31 @@
+/*
+ * ngrav_host.h -- host-side C entry points of libngrav_host.so (C++ host code above the
+ * nbg C ABI).  Initial-condition samplers: FP64 arrays of length n, caller-owned.
+ * Return 0 on success, -1 on a bad argument.
+ */
+#ifndef NGRAV_HOST_H
+#define NGRAV_HOST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ic::Plummer<3> (projects/nbody_core/include/ngrav/ic/Plummer.hpp:26-112). */
+int ngh_ic_plummer(int n, uint32_t seed, double total_mass, double scale_radius, double G, double* x, double* y,
+                   double* z, double* vx, double* vy, double* vz, double* m);
+
+/* ic::Hernquist<3> (projects/nbody_core/include/ngrav/ic/Hernquist.hpp:131-181). */
+int ngh_ic_hernquist(int n, uint32_t seed, double total_mass, double scale_radius, double G, double* x, double* y,
+                     double* z, double* vx, double* vy, double* vz, double* m);
+
+/* ic::King<3> (projects/nbody_core/include/ngrav/ic/King.hpp:168-211). */
+int ngh_ic_king(int n, uint32_t seed, double total_mass, double core_radius, double w0, double G, double* x,
+                double* y, double* z, double* vx, double* vy, double* vz, double* m);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NGRAV_HOST_H */

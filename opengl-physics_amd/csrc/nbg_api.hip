@@ -177,19 +177,38 @@ int accel_eager(nbg_ctx* c, const nbg_params& p, nbg_stats* stats) {
     return NBG_OK;
 }
 
-// One KDK step, enqueued (force evaluation with `levels`).
-void enqueue_step(nbg_ctx* c, const nbg_params& p, int levels) {
+// One KDK step, enqueued (force evaluation with `levels`).  With `timed`, event
+// records bracket every stage (ev[8..9] first kick + drift, ev[0..5] force stages,
+// ev[10..11] closing kick).
+void enqueue_step(nbg_ctx* c, const nbg_params& p, int levels, bool timed) {
     const float dt = static_cast<float>(p.dt);
     const float hdt = static_cast<float>(0.5 * p.dt);
+    if (timed) (void)hipEventRecord(c->ev[8], c->stream);
     launch_kick_drift(c, hdt, dt);
+    if (timed) (void)hipEventRecord(c->ev[9], c->stream);
     if (p.solver == NBG_SOLVER_DIRECT) {
+        if (timed) (void)hipEventRecord(c->ev[4], c->stream);
         launch_direct(c, p);
+        if (timed) (void)hipEventRecord(c->ev[5], c->stream);
     } else {
         nbg_params q = p;
         q.flags &= ~NBG_FLAG_COUNT;
-        enqueue_bh(c, q, levels, false);
+        enqueue_bh(c, q, levels, timed);
     }
+    if (timed) (void)hipEventRecord(c->ev[10], c->stream);
     launch_kick(c, hdt);
+    if (timed) (void)hipEventRecord(c->ev[11], c->stream);
+}
+
+void add_stage_times(nbg_ctx* c, const nbg_params& p, nbg_stats* s) {
+    s->ms_integrate += elapsed(c->ev[8], c->ev[9]) + elapsed(c->ev[10], c->ev[11]);
+    s->ms_walk += elapsed(c->ev[4], c->ev[5]);
+    if (p.solver == NBG_SOLVER_BARNES_HUT) {
+        s->ms_keys += elapsed(c->ev[0], c->ev[1]);
+        s->ms_sort += elapsed(c->ev[1], c->ev[2]);
+        s->ms_build += elapsed(c->ev[2], c->ev[3]);
+        s->ms_upsweep += elapsed(c->ev[3], c->ev[4]);
+    }
 }
 
 int save_backup(nbg_ctx* c) {
@@ -218,17 +237,32 @@ bool same_params(const nbg_params& a, const nbg_params& b) {
 }
 
 // Eager steps with a scalar read-back per step to catch the deep-key case.
-int steps_eager(nbg_ctx* c, const nbg_params& p, int nsteps) {
+int steps_eager(nbg_ctx* c, const nbg_params& p, int nsteps, nbg_stats* stats) {
     const float dt = static_cast<float>(p.dt);
     const float hdt = static_cast<float>(0.5 * p.dt);
+    const bool timed = (p.flags & NBG_FLAG_STAGE_TIMES) && stats;
     for (int s = 0; s < nsteps; ++s) {
+        if (timed) (void)hipEventRecord(c->ev[8], c->stream);
         launch_kick_drift(c, hdt, dt);
+        if (timed) (void)hipEventRecord(c->ev[9], c->stream);
         if (int r = check_launch(c)) return r;
         nbg_params q = p;
         q.flags &= ~NBG_FLAG_COUNT;
-        if (int r = accel_eager(c, q, nullptr)) return r;
+        nbg_stats one;
+        if (int r = accel_eager(c, q, timed ? &one : nullptr)) return r;
+        if (timed) (void)hipEventRecord(c->ev[10], c->stream);
         launch_kick(c, hdt);
+        if (timed) (void)hipEventRecord(c->ev[11], c->stream);
         if (int r = check_launch(c)) return r;
+        if (timed) {
+            NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+            stats->ms_keys += one.ms_keys;
+            stats->ms_sort += one.ms_sort;
+            stats->ms_build += one.ms_build;
+            stats->ms_upsweep += one.ms_upsweep;
+            stats->ms_walk += one.ms_walk;
+            stats->ms_integrate += elapsed(c->ev[8], c->ev[9]) + elapsed(c->ev[10], c->ev[11]);
+        }
     }
     return NBG_OK;
 }
@@ -474,12 +508,12 @@ int nbg_step(nbg_ctx* c, const nbg_params* p, int32_t nsteps, nbg_stats* stats) 
     NBG_HIP_CHECK(c, hipEventRecord(c->ev[6], c->stream));
     const bool use_graph = !(p->flags & NBG_FLAG_NO_GRAPH) && c->last_levels == kLevelsShort;
     if (!use_graph) {
-        if (int r = steps_eager(c, *p, nsteps)) return r;
+        if (int r = steps_eager(c, *p, nsteps, stats)) return r;
     } else {
         if (!c->graph_valid || !same_params(c->graph_params, *p)) {
             drop_graph(c);
             NBG_HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            enqueue_step(c, *p, kLevelsShort);
+            enqueue_step(c, *p, kLevelsShort, (p->flags & NBG_FLAG_STAGE_TIMES) != 0);
             const hipError_t le = hipGetLastError();
             hipGraph_t g = nullptr;
             const hipError_t ee = hipStreamEndCapture(c->stream, &g);
@@ -495,7 +529,14 @@ int nbg_step(nbg_ctx* c, const nbg_params* p, int32_t nsteps, nbg_stats* stats) 
         }
         if (int r = save_backup(c)) return r;
         NBG_HIP_CHECK(c, hipMemsetAsync(&c->dsc->sticky_overflow, 0, 2 * sizeof(int32_t), c->stream));
-        for (int s = 0; s < nsteps; ++s) NBG_HIP_CHECK(c, hipGraphLaunch(c->graph_exec, c->stream));
+        const bool per_step = (p->flags & NBG_FLAG_STAGE_TIMES) && stats;
+        for (int s = 0; s < nsteps; ++s) {
+            NBG_HIP_CHECK(c, hipGraphLaunch(c->graph_exec, c->stream));
+            if (per_step) {
+                NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+                add_stage_times(c, *p, stats);
+            }
+        }
         NBG_HIP_CHECK(c, hipMemcpyAsync(&c->hsc->sticky_overflow, &c->dsc->sticky_overflow, 2 * sizeof(int32_t),
                                         hipMemcpyDeviceToHost, c->stream));
         NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -507,7 +548,8 @@ int nbg_step(nbg_ctx* c, const nbg_params* p, int32_t nsteps, nbg_stats* stats) 
         }
         if (overflow) {
             if (int r = restore_backup(c)) return r;
-            if (int r = steps_eager(c, *p, nsteps)) return r;
+            if (stats) std::memset(stats, 0, sizeof(*stats));
+            if (int r = steps_eager(c, *p, nsteps, stats)) return r;
         }
     }
     NBG_HIP_CHECK(c, hipEventRecord(c->ev[7], c->stream));

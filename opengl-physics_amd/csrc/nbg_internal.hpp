@@ -145,7 +145,7 @@ struct nbg_ctx {
     int diag_part_cap = 0;
 
     // timing
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[12] = {};
 
     // graph of one step
     hipGraphExec_t graph_exec = nullptr;

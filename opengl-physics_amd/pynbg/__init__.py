@@ -11,6 +11,7 @@ import numpy as np
 
 _PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_PKG_DIR, "libnbg_hip.so")
+HOST_LIB_PATH = os.path.join(_PKG_DIR, "libngrav_host.so")
 
 SOLVER_DIRECT = 0
 SOLVER_BARNES_HUT = 1
@@ -20,6 +21,7 @@ FLAG_KAHAN = 1
 FLAG_COUNT = 2
 FLAG_LEARNED = 4
 FLAG_NO_GRAPH = 8
+FLAG_STAGE_TIMES = 16
 
 ERRORS = {0: "NBG_OK", -1: "NBG_E_INVALID", -2: "NBG_E_OOM", -3: "NBG_E_CAPACITY", -4: "NBG_E_HIP",
           -5: "NBG_E_STATE"}
@@ -98,11 +100,11 @@ class NbgError(RuntimeError):
 
 
 def make_params(solver="barnes_hut", multipole="quadrupole", G=1.0, eps=None, eps2=None, theta=0.5, dt=1e-3,
-                kahan=True, count=False, no_graph=False, learned=False):
+                kahan=True, count=False, no_graph=False, learned=False, stage_times=False):
     if eps2 is None:
         eps2 = (eps if eps is not None else 0.05) ** 2
     flags = (FLAG_KAHAN if kahan else 0) | (FLAG_COUNT if count else 0) | (FLAG_NO_GRAPH if no_graph else 0) | \
-        (FLAG_LEARNED if learned else 0)
+        (FLAG_LEARNED if learned else 0) | (FLAG_STAGE_TIMES if stage_times else 0)
     return Params(SOLVER_DIRECT if solver == "direct" else SOLVER_BARNES_HUT,
                   QUADRUPOLE if multipole == "quadrupole" else MONOPOLE, float(G), float(eps2), float(theta),
                   float(dt), flags)
@@ -217,3 +219,53 @@ class Context:
         self._check(self._L.nbg_accel_direct_targets(self._h, ctypes.byref(params), len(t), _p(t, _i64),
                                                      _p(ax, _f), _p(ay, _f), _p(az, _f)), "nbg_accel_direct_targets")
         return np.stack([ax, ay, az], axis=1)
+
+
+# ---------------------------------------------------------------------------- host library
+_host = None
+
+
+def load_host():
+    """libngrav_host.so: host C++ above the C ABI (initial-condition samplers)."""
+    global _host
+    if _host is not None:
+        return _host
+    if not os.path.exists(HOST_LIB_PATH):
+        raise RuntimeError(f"{HOST_LIB_PATH} is not built")
+    H = ctypes.CDLL(HOST_LIB_PATH)
+    H.ngh_ic_plummer.argtypes = [ctypes.c_int, ctypes.c_uint32] + [ctypes.c_double] * 3 + [_d] * 7
+    H.ngh_ic_hernquist.argtypes = [ctypes.c_int, ctypes.c_uint32] + [ctypes.c_double] * 3 + [_d] * 7
+    H.ngh_ic_king.argtypes = [ctypes.c_int, ctypes.c_uint32] + [ctypes.c_double] * 4 + [_d] * 7
+    for f in (H.ngh_ic_plummer, H.ngh_ic_hernquist, H.ngh_ic_king):
+        f.restype = ctypes.c_int
+    _host = H
+    return H
+
+
+def _ic(fn, n, seed, *scalars):
+    out = {k: np.empty(n, np.float64) for k in ("x", "y", "z", "vx", "vy", "vz", "m")}
+    rc = fn(int(n), int(seed), *[float(v) for v in scalars],
+            *[_p(out[k], _d) for k in ("x", "y", "z", "vx", "vy", "vz", "m")])
+    if rc != 0:
+        raise NbgError(f"initial-condition sampler failed (rc={rc})")
+    return out
+
+
+def ic_plummer(n, seed, total_mass=1.0, scale_radius=1.0, G=1.0):
+    """ic::Plummer<3>, FP64 arrays x,y,z,vx,vy,vz,m."""
+    return _ic(load_host().ngh_ic_plummer, n, seed, total_mass, scale_radius, G)
+
+
+def ic_hernquist(n, seed, total_mass=1.0, scale_radius=1.0, G=1.0):
+    """ic::Hernquist<3>, FP64 arrays x,y,z,vx,vy,vz,m."""
+    return _ic(load_host().ngh_ic_hernquist, n, seed, total_mass, scale_radius, G)
+
+
+def ic_king(n, seed, w0=6.0, total_mass=1.0, core_radius=1.0, G=1.0):
+    """ic::King<3>(W0), FP64 arrays x,y,z,vx,vy,vz,m."""
+    return _ic(load_host().ngh_ic_king, n, seed, total_mass, core_radius, w0, G)
+
+
+def to_f32(state):
+    """Round an FP64 state dict to the FP32 arrays the device path takes."""
+    return {k: np.asarray(v, np.float32) for k, v in state.items()}

@@ -381,6 +381,14 @@ void orc_ic_plummer(int n, uint32_t seed, double totalMass, double scaleRadius, 
     }
 }
 
+// n successive draws of std::uniform_real_distribution<double>(lo, hi) on mt19937(seed):
+// reproduces the inputs of the reference's own seeded self-tests.
+void orc_uniform_sequence(uint32_t seed, double lo, double hi, int n, double* out) {
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<double> u(lo, hi);
+    for (int i = 0; i < n; ++i) out[i] = u(rng);
+}
+
 // The 10000th output of a default-seeded mt19937 (C++ standard [rand.predef]).
 uint32_t orc_mt19937_10000th(void) {
     std::mt19937 g;

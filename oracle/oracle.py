@@ -33,6 +33,7 @@ def lib(fast=False):
     L.orc_ic_plummer.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_double] + [_d] * 7
     L.orc_mt19937_10000th.restype = ctypes.c_uint32
+    L.orc_uniform_sequence.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_int, _d]
     L.orc_tree_build.restype = ctypes.c_void_p
     L.orc_tree_build.argtypes = [ctypes.c_int] + [_d] * 4 + [ctypes.c_int] * 3
     L.orc_tree_free.argtypes = [ctypes.c_void_p]
@@ -68,6 +69,13 @@ def plummer(n, seed, total_mass=1.0, scale_radius=1.0, G=1.0, fast=False):
     out = {k: np.empty(n, np.float64) for k in ("x", "y", "z", "vx", "vy", "vz", "m")}
     lib(fast).orc_ic_plummer(n, seed, total_mass, scale_radius, G,
                              *[_ptr(out[k], _d) for k in ("x", "y", "z", "vx", "vy", "vz", "m")])
+    return out
+
+
+def uniform_sequence(seed, lo, hi, n):
+    """n draws of std::uniform_real_distribution<double>(lo, hi) from std::mt19937(seed)."""
+    out = np.empty(n, np.float64)
+    lib().orc_uniform_sequence(seed, lo, hi, n, _ptr(out, _d))
     return out
 
 

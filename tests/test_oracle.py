@@ -1,0 +1,350 @@
+"""Pins the FP64 oracle (oracle/nbody_oracle.cpp) against known answers, closed forms,
+invariants and an independent brute-force construction, so that a plausible mistake in
+the restatement (a dropped term, a wrong sign, index or octant rule) fails here.
+
+The reference ships no golden vectors: its own tests regenerate inputs from seeds and
+check tolerances (projects/nbody_core/src/SelfTest.cpp).  Those checks are reproduced
+here on the oracle; the reference build itself is unavailable (glm is absent).
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+# ---------------------------------------------------------------------------- RNG / ICs
+
+
+def test_mt19937_standard_value():
+    # C++ standard [rand.predef]: 10000th output of default-constructed mt19937
+    assert oracle.mt19937_10000th() == 4123659995
+
+
+def test_product_plummer_sampler_matches_oracle_bitwise():
+    import pynbg
+    for n, seed in ((20000, 0), (3001, 3)):
+        a = pynbg.ic_plummer(n, seed)
+        b = oracle.plummer(n, seed)
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k])
+
+
+def _half_mass_radius(s):
+    r = np.sqrt(s["x"] ** 2 + s["y"] ** 2 + s["z"] ** 2)
+    order = np.argsort(r)
+    cm = np.cumsum(s["m"][order])
+    return r[order][np.searchsorted(cm, 0.5 * cm[-1])]
+
+
+def _virial_ratio(s, eps=0.0025):
+    T = 0.5 * np.sum(s["m"] * (s["vx"] ** 2 + s["vy"] ** 2 + s["vz"] ** 2))
+    E = oracle.total_energy(s, 1.0, eps * eps)
+    W = E - T
+    return 2.0 * T / abs(W)
+
+
+def _density_median_err(s, rho):
+    r = np.sqrt(s["x"] ** 2 + s["y"] ** 2 + s["z"] ** 2)
+    edges = np.quantile(r, np.linspace(0.02, 0.9, 16))
+    errs = []
+    for a, b in zip(edges[:-1], edges[1:]):
+        msk = (r >= a) & (r < b)
+        vol = 4.0 / 3.0 * np.pi * (b ** 3 - a ** 3)
+        meas = s["m"][msk].sum() / vol
+        ana = rho(0.5 * (a + b))
+        errs.append(abs(meas - ana) / ana)
+    return float(np.median(errs))
+
+
+def test_plummer_statistics():
+    """SelfTest.cpp:729-760: density profile, virial ratio, half-mass radius ~1.305."""
+    s = oracle.plummer(20000, 3)
+    assert abs(_half_mass_radius(s) - 1.305) < 0.05
+    assert _density_median_err(s, lambda r: 3.0 / (4.0 * np.pi) * (1 + r * r) ** -2.5) < 0.10
+    assert abs(_virial_ratio(s) - 1.0) < 0.12
+    # recentred: zero COM position and momentum
+    com, L, P = oracle.moments(s)
+    assert np.abs(com).max() < 1e-12 and np.abs(P).max() < 1e-12
+
+
+def test_hernquist_and_king_statistics():
+    """SelfTest.cpp:762-810 on the product samplers (Hernquist r_half ~2.414; King compact)."""
+    import pynbg
+    h = pynbg.ic_hernquist(20000, 7)
+    assert abs(_half_mass_radius(h) - 2.414) < 0.15
+    assert _density_median_err(h, lambda r: 1.0 / (2 * np.pi) / (r * (1 + r) ** 3)) < 0.10
+    assert abs(_virial_ratio(h) - 1.0) < 0.20
+    k = pynbg.ic_king(20000, 11, w0=6.0)
+    rmax = np.sqrt(k["x"] ** 2 + k["y"] ** 2 + k["z"] ** 2).max()
+    assert rmax < 20.0
+    assert abs(_virial_ratio(k) - 1.0) < 0.25
+
+
+# ---------------------------------------------------------------------------- tree
+
+
+def _brute_tree(x, y, z, ncrit=8, max_depth=40):
+    """Independent recursive construction from the count rule: a cell is split iff it
+    holds more than ncrit particles and depth < max_depth; children exist for non-empty
+    octants; octant bit k set iff coordinate >= centre (FP64).  Returns canonical
+    (pre-order) records (depth, count, parent, sorted member tuple)."""
+    pts = np.stack([x, y, z], 1)
+    lo, hi = pts.min(0), pts.max(0)
+    half = max(0.5 * (hi[0] - lo[0]), 0.5 * (hi[1] - lo[1]), 0.5 * (hi[2] - lo[2]))
+    if not half > 1e-12:
+        half = 1.0
+    out = []
+
+    def rec(members, c, h, depth, parent):
+        me = len(out)
+        out.append([depth, len(members), parent, tuple(sorted(members)), c.copy(), h])
+        if len(members) <= ncrit or depth >= max_depth:
+            return
+        octs = {}
+        for i in members:
+            o = int(pts[i, 0] >= c[0]) | (int(pts[i, 1] >= c[1]) << 1) | (int(pts[i, 2] >= c[2]) << 2)
+            octs.setdefault(o, []).append(i)
+        for o in range(8):
+            if o in octs:
+                hc = 0.5 * h
+                cc = c + np.array([hc if o & 1 else -hc, hc if o & 2 else -hc, hc if o & 4 else -hc])
+                rec(octs[o], cc, hc, depth + 1, me)
+
+    rec(list(range(len(x))), 0.5 * (lo + hi), half * 1.001, 0, -1)
+    return out
+
+
+@pytest.mark.parametrize("case", ["uniform", "lattice", "coincident"])
+def test_insertion_tree_equals_brute_force(case):
+    rng = np.random.default_rng(5)
+    if case == "uniform":
+        p = rng.uniform(-1, 1, (400, 3))
+    elif case == "lattice":
+        g = np.linspace(-1, 1, 5)
+        p = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    else:
+        p = rng.uniform(-1, 1, (60, 3))
+        p[10:25] = p[3]
+    m = np.full(len(p), 1.0 / len(p))
+    T = oracle.Tree(p[:, 0], p[:, 1], p[:, 2], m, quad=False)
+    e = T.export()
+    bt = _brute_tree(p[:, 0], p[:, 1], p[:, 2])
+    assert len(bt) == T.num_nodes
+    for cid, (depth, cnt, parent, members, c, h) in enumerate(bt):
+        assert e["depth"][cid] == depth
+        assert e["parent"][cid] == parent
+        np.testing.assert_array_equal(e["center"][cid], c)
+        assert e["half"][cid] == h
+        if e["child_count"][cid] == 0:
+            assert e["count"][cid] == cnt
+            got = e["order"][e["first"][cid]:e["first"][cid] + cnt]
+            assert tuple(got) == members  # leaf members in ascending index
+
+
+def test_tree_invariants_plummer():
+    s = oracle.plummer(8192, 1)
+    T = oracle.Tree(s["x"], s["y"], s["z"], s["m"], quad=True)
+    e = T.export()
+    n = 8192
+    assert sorted(e["order"]) == list(range(n))
+    leaf = e["child_count"] == 0
+    assert np.all(e["count"][leaf] <= 8) and np.all(e["count"][leaf] >= 1)
+    # leaves in pre-order own consecutive ranges of `order`
+    firsts = e["first"][leaf]
+    counts = e["count"][leaf]
+    assert firsts[0] == 0 and np.all(firsts[1:] == firsts[:-1] + counts[:-1])
+    for nd in range(T.num_nodes):
+        ch = e["children"][nd]
+        for o in range(8):
+            c = ch[o]
+            if c < 0:
+                continue
+            assert e["parent"][c] == nd and e["depth"][c] == e["depth"][nd] + 1
+            h = e["half"][nd] * 0.5
+            assert e["half"][c] == h
+            sgn = np.array([1 if o & 1 else -1, 1 if o & 2 else -1, 1 if o & 4 else -1], float)
+            np.testing.assert_array_equal(e["center"][c], e["center"][nd] + sgn * h)
+    # maxRadius of a leaf is the largest member distance from its com; Q about the com
+    for nd in np.nonzero(leaf)[0][:500]:
+        mem = e["order"][e["first"][nd]:e["first"][nd] + e["count"][nd]]
+        d = np.stack([s["x"][mem], s["y"][mem], s["z"][mem]], 1) - e["com"][nd]
+        assert abs(np.sqrt((d ** 2).sum(1)).max() - e["max_radius"][nd]) < 1e-13
+        if len(mem) > 1:
+            mm = s["m"][mem]
+            qxx = np.sum(mm * (3 * d[:, 0] ** 2 - (d ** 2).sum(1)))
+            qxy = np.sum(mm * 3 * d[:, 0] * d[:, 1])
+            assert abs(qxx - e["quad"][nd, 0]) < 1e-14 and abs(qxy - e["quad"][nd, 2]) < 1e-14
+    # internal Q by parallel axis equals the direct particle sum about the node com
+    root_d = np.stack([s["x"], s["y"], s["z"]], 1) - e["com"][0]
+    qyz = np.sum(s["m"] * 3 * root_d[:, 1] * root_d[:, 2])
+    assert abs(qyz - e["quad"][0, 4]) < 1e-12
+
+
+def test_key_contract_reproduces_insertion_topology():
+    """Sorting by the FP64-recurrence keys and applying the count rule to key prefixes
+    gives the insertion tree: nodes = prefixes whose parent prefix holds > 8 particles."""
+    rng = np.random.default_rng(9)
+    p = np.concatenate([rng.standard_normal((3000, 3)), np.round(rng.uniform(-2, 2, (500, 3)) * 4) / 4])
+    x, y, z = p[:, 0], p[:, 1], p[:, 2]
+    hi, _ = oracle.morton_keys(x, y, z, 21)
+    T = oracle.Tree(x, y, z, np.ones(len(x)), quad=False)
+    e = T.export()
+    order = np.lexsort((np.arange(len(x)), hi))
+    ks = hi[order]
+    nodes = []  # (depth, prefix) in pre-order
+
+    def rec(lo, hi_, depth):
+        nodes.append((depth, lo, hi_))
+        if hi_ - lo <= 8 or depth >= 21:
+            return
+        shift = 3 * (21 - depth - 1)
+        digits = (ks[lo:hi_] >> np.uint64(shift)) & np.uint64(7)
+        for d in range(8):
+            idx = np.nonzero(digits == d)[0]
+            if len(idx):
+                rec(lo + idx[0], lo + idx[-1] + 1, depth + 1)
+
+    rec(0, len(x), 0)
+    assert len(nodes) == T.num_nodes
+    np.testing.assert_array_equal([d for d, _, _ in nodes], e["depth"])
+    leaf_ranges = [(a, b) for (d, a, b), cc in zip(nodes, e["child_count"]) if cc == 0]
+    perm = np.concatenate([np.sort(order[a:b]) for a, b in leaf_ranges])
+    np.testing.assert_array_equal(perm, e["order"])
+
+
+# ---------------------------------------------------------------------------- forces
+
+
+def test_two_body_closed_form():
+    x = np.array([-0.5, 0.5])
+    z0 = np.zeros(2)
+    m = np.array([1.0, 3.0])
+    a = oracle.accel_direct(x, z0, z0, m, 2.0, 0.01)
+    f = 1.0 / (1.0 + 0.01) ** 1.5
+    np.testing.assert_allclose(a[0], [6.0 * f, 0, 0], rtol=1e-15)
+    np.testing.assert_allclose(a[1], [-2.0 * f, 0, 0], rtol=1e-15)
+
+
+def test_direct_conserves_momentum():
+    s = oracle.plummer(2000, 4)
+    a = oracle.accel_direct(s["x"], s["y"], s["z"], s["m"], 1.0, 1e-4)
+    net = (s["m"][:, None] * a).sum(0)
+    assert np.abs(net).max() < 1e-13 * np.abs(s["m"][:, None] * a).sum()
+
+
+def test_bh_theta0_equals_direct():
+    """SelfTest.cpp:43-74, same inputs (mt19937(12345), uniform(-1,1) drawn x,y,z,m per
+    particle): theta=0 opens every cell -> identical to direct (<= 1e-12)."""
+    u = oracle.uniform_sequence(12345, -1.0, 1.0, 4000).reshape(1000, 4)
+    p = u[:, :3]
+    m = 0.5 + 0.5 * (u[:, 3] + 1.0)
+    T = oracle.Tree(p[:, 0], p[:, 1], p[:, 2], m, quad=True)
+    ab = T.accel_bh(1.0, 0.02 ** 2, 0.0)
+    ad = oracle.accel_direct(p[:, 0], p[:, 1], p[:, 2], m, 1.0, 0.02 ** 2)
+    rel = np.linalg.norm(ab - ad, axis=1) / np.linalg.norm(ad, axis=1)
+    assert rel.max() <= 1e-12
+
+
+def quad_gain_inputs():
+    """SelfTest.cpp:212-235 inputs: mt19937(2024), uniform(0,1) drawn r, cos, phi per body."""
+    n = 4000
+    u = oracle.uniform_sequence(2024, 0.0, 1.0, 3 * n).reshape(n, 3)
+    r = np.cbrt(u[:, 0])
+    ct = 2.0 * u[:, 1] - 1.0
+    st = np.sqrt(np.maximum(0.0, 1.0 - ct * ct))
+    ph = 2.0 * np.pi * u[:, 2]
+    return r * st * np.cos(ph), r * st * np.sin(ph), r * ct, np.full(n, 1.0 / n)
+
+
+def test_quadrupole_gain():
+    """SelfTest.cpp:212-265 on its exact inputs: quadrupole gives >= 3x lower max force
+    error at theta=0.5.  The reference's documented measurements on this case -- grouped
+    walk monopole 3.74e-2 (projects/02_nbody_gravity/N_Body_Gravity_Plan.md:228-232) and
+    quadrupole 8.1e-3 (docs/SIMULATION.md:122-124) -- are reproduced to their printed digits."""
+    x, y, z, m = quad_gain_inputs()
+    ad = oracle.accel_direct(x, y, z, m, 1.0, 1e-4)
+
+    def maxerr(a):
+        return (np.linalg.norm(a - ad, axis=1) / np.linalg.norm(ad, axis=1)).max()
+    mono = maxerr(oracle.Tree(x, y, z, m, quad=False).accel_bh(1.0, 1e-4, 0.5))
+    quad = maxerr(oracle.Tree(x, y, z, m, quad=True).accel_bh(1.0, 1e-4, 0.5))
+    print(f"max err mono={mono:.4e} quad={quad:.4e} gain={mono / quad:.2f}")
+    assert mono / quad >= 3.0
+    assert round(mono, 4) == 0.0374
+    assert round(quad, 4) == 0.0081
+
+
+def test_quadrupole_far_field_is_exact_to_third_order():
+    """A compact cluster seen from far: the monopole+quadrupole error falls as r^-4,
+    two orders faster than the monopole-only error (r^-2 relative ~ r^-4 absolute)."""
+    rng = np.random.default_rng(3)
+    c = rng.uniform(-0.05, 0.05, (9, 3))  # 9 bodies -> the root splits, cluster stays one cell
+    m = rng.uniform(0.5, 1.5, 9)
+    errs = {}
+    for R in (4.0, 8.0):
+        p = np.concatenate([c, [[R, 0.3 * R, -0.2 * R]]])
+        mm = np.concatenate([m, [1e-9]])
+        ad = oracle.accel_direct(p[:, 0], p[:, 1], p[:, 2], mm, 1.0, 0.0)[-1]
+        errs[R] = []
+        for quad in (False, True):
+            a = oracle.Tree(p[:, 0], p[:, 1], p[:, 2], mm, quad=quad).accel_bh_per_particle(1.0, 0.0, 0.9)[-1]
+            errs[R].append(np.linalg.norm(a - ad) / np.linalg.norm(ad))
+    mono_ratio = errs[4.0][0] / errs[8.0][0]
+    quad_ratio = errs[4.0][1] / errs[8.0][1]
+    assert errs[8.0][1] < errs[8.0][0]
+    assert 3.0 < mono_ratio < 5.5    # relative monopole error ~ (s/r)^2 (dipole vanishes about com)
+    assert quad_ratio > 6.0          # relative quadrupole error ~ (s/r)^3
+
+
+def test_group_walk_matches_per_particle_walk_closely():
+    """The grouped walk (conservative MAC) is at least as accurate as the per-particle walk."""
+    s = oracle.plummer(4000, 8)
+    T = oracle.Tree(s["x"], s["y"], s["z"], s["m"], quad=True)
+    ad = oracle.accel_direct(s["x"], s["y"], s["z"], s["m"], 1.0, 1e-4)
+    eg = oracle.rel_rms(T.accel_bh(1.0, 1e-4, 0.6), ad)
+    ep = oracle.rel_rms(T.accel_bh_per_particle(1.0, 1e-4, 0.6), ad)
+    assert eg <= ep * 1.05
+
+
+# ---------------------------------------------------------------------------- integrator
+
+
+def test_kepler_orbit():
+    """SelfTest.cpp:83-117: e=0.36 two-body orbit over 4 periods with Direct, eps=1e-6."""
+    G, m, sep = 1.0, 1.0, 1.0
+    v = 0.8 * np.sqrt(G * 2 * m / sep)
+    period = 2 * np.pi * np.sqrt(sep ** 3 / (G * 2 * m))
+    st = dict(x=np.array([-0.5, 0.5]), y=np.zeros(2), z=np.zeros(2), vx=np.zeros(2), vy=np.array([-v / 2, v / 2]),
+              vz=np.zeros(2), m=np.array([m, m]))
+    eps2 = 1e-12
+    a = oracle.accel_direct(st["x"], st["y"], st["z"], st["m"], G, eps2)
+    e0 = oracle.total_energy(st, G, eps2)
+    rmin, rmax, emin, emax = 1e30, 0.0, e0, e0
+    for _ in range(4 * 40):
+        st, a = oracle.leapfrog(st, a, G, eps2, 0.5, period / 2000, 50, solver="direct")
+        r = np.hypot(st["x"][1] - st["x"][0], st["y"][1] - st["y"][0])
+        rmin, rmax = min(rmin, r), max(rmax, r)
+        e = oracle.total_energy(st, G, eps2)
+        emin, emax = min(emin, e), max(emax, e)
+    semi = 1.0 / 1.36
+    assert abs(rmin - semi * 0.64) / (semi * 0.64) < 2e-3  # sampled every 50 steps
+    assert abs(rmax - 1.0) < 1e-4
+    assert (emax - emin) / abs(0.5 * (emax + emin)) < 1e-4
+
+
+def test_leapfrog_time_reversible():
+    s = oracle.plummer(300, 6)
+    a = oracle.accel_direct(s["x"], s["y"], s["z"], s["m"], 1.0, 1e-3)
+    s1, a1 = oracle.leapfrog(s, a, 1.0, 1e-3, 0.5, 1e-3, 50, solver="direct")
+    back = dict(s1)
+    for k in ("vx", "vy", "vz"):
+        back[k] = -s1[k]
+    s2, _ = oracle.leapfrog(back, a1, 1.0, 1e-3, 0.5, 1e-3, 50, solver="direct")
+    for k in ("x", "y", "z"):
+        np.testing.assert_allclose(s2[k], s[k], atol=1e-11)
+
+
+def test_total_energy_two_body():
+    st = dict(x=np.array([0.0, 2.0]), y=np.zeros(2), z=np.zeros(2), vx=np.array([1.0, 0.0]), vy=np.zeros(2),
+              vz=np.zeros(2), m=np.array([2.0, 3.0]))
+    e = oracle.total_energy(st, 1.5, 0.25)
+    assert abs(e - (0.5 * 2.0 * 1.0 - 1.5 * 6.0 / np.sqrt(4.25))) < 1e-15
