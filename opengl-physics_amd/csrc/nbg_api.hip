@@ -69,24 +69,32 @@ int validate(nbg_ctx* c, const nbg_params* p) {
     return NBG_OK;
 }
 
+// Record a stage event; while capturing a graph it becomes an event-record node.
+inline void rec(nbg_ctx* c, int i) {
+    if (c->capturing)
+        (void)hipEventRecordWithFlags(c->ev[i], c->stream, hipEventRecordExternal);
+    else
+        (void)hipEventRecord(c->ev[i], c->stream);
+}
+
 // Enqueue one Barnes-Hut force evaluation (no host synchronisation).
 void enqueue_bh(nbg_ctx* c, const nbg_params& p, int levels, bool timed) {
-    if (timed) (void)hipEventRecord(c->ev[0], c->stream);
+    if (timed) rec(c, 0);
     launch_bbox_and_keys(c, levels);
-    if (timed) (void)hipEventRecord(c->ev[1], c->stream);
+    if (timed) rec(c, 1);
     launch_sort(c, levels);
-    if (timed) (void)hipEventRecord(c->ev[2], c->stream);
+    if (timed) rec(c, 2);
     launch_build(c, levels);
     launch_gather(c, static_cast<float>(p.G));
-    if (timed) (void)hipEventRecord(c->ev[3], c->stream);
-    launch_upsweep(c, levels, p.multipole);
-    if (timed) (void)hipEventRecord(c->ev[4], c->stream);
+    if (timed) rec(c, 3);
+    launch_upsweep(c, levels, p.multipole, p.G);
+    if (timed) rec(c, 4);
     if ((p.flags & NBG_FLAG_COUNT) && c->cnt_far) {
         launch_fill_i32(c, c->cnt_far, c->cap_nodes, 0);
         launch_fill_i32(c, c->cnt_near, c->cap_nodes, 0);
     }
     launch_walk(c, p);
-    if (timed) (void)hipEventRecord(c->ev[5], c->stream);
+    if (timed) rec(c, 5);
 }
 
 float elapsed(hipEvent_t a, hipEvent_t b) {
@@ -114,9 +122,9 @@ int accel_eager(nbg_ctx* c, const nbg_params& p, nbg_stats* stats) {
     const bool timed = stats != nullptr;
     c->last_G = p.G;
     if (p.solver == NBG_SOLVER_DIRECT) {
-        if (timed) (void)hipEventRecord(c->ev[0], c->stream);
+        if (timed) rec(c, 0);
         launch_direct(c, p);
-        if (timed) (void)hipEventRecord(c->ev[5], c->stream);
+        if (timed) rec(c, 5);
         if (int r = check_launch(c)) return r;
         NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
         c->have_tree = false;
@@ -183,21 +191,21 @@ int accel_eager(nbg_ctx* c, const nbg_params& p, nbg_stats* stats) {
 void enqueue_step(nbg_ctx* c, const nbg_params& p, int levels, bool timed) {
     const float dt = static_cast<float>(p.dt);
     const float hdt = static_cast<float>(0.5 * p.dt);
-    if (timed) (void)hipEventRecord(c->ev[8], c->stream);
+    if (timed) rec(c, 8);
     launch_kick_drift(c, hdt, dt);
-    if (timed) (void)hipEventRecord(c->ev[9], c->stream);
+    if (timed) rec(c, 9);
     if (p.solver == NBG_SOLVER_DIRECT) {
-        if (timed) (void)hipEventRecord(c->ev[4], c->stream);
+        if (timed) rec(c, 4);
         launch_direct(c, p);
-        if (timed) (void)hipEventRecord(c->ev[5], c->stream);
+        if (timed) rec(c, 5);
     } else {
         nbg_params q = p;
         q.flags &= ~NBG_FLAG_COUNT;
         enqueue_bh(c, q, levels, timed);
     }
-    if (timed) (void)hipEventRecord(c->ev[10], c->stream);
+    if (timed) rec(c, 10);
     launch_kick(c, hdt);
-    if (timed) (void)hipEventRecord(c->ev[11], c->stream);
+    if (timed) rec(c, 11);
 }
 
 void add_stage_times(nbg_ctx* c, const nbg_params& p, nbg_stats* s) {
@@ -242,17 +250,17 @@ int steps_eager(nbg_ctx* c, const nbg_params& p, int nsteps, nbg_stats* stats) {
     const float hdt = static_cast<float>(0.5 * p.dt);
     const bool timed = (p.flags & NBG_FLAG_STAGE_TIMES) && stats;
     for (int s = 0; s < nsteps; ++s) {
-        if (timed) (void)hipEventRecord(c->ev[8], c->stream);
+        if (timed) rec(c, 8);
         launch_kick_drift(c, hdt, dt);
-        if (timed) (void)hipEventRecord(c->ev[9], c->stream);
+        if (timed) rec(c, 9);
         if (int r = check_launch(c)) return r;
         nbg_params q = p;
         q.flags &= ~NBG_FLAG_COUNT;
         nbg_stats one;
         if (int r = accel_eager(c, q, timed ? &one : nullptr)) return r;
-        if (timed) (void)hipEventRecord(c->ev[10], c->stream);
+        if (timed) rec(c, 10);
         launch_kick(c, hdt);
-        if (timed) (void)hipEventRecord(c->ev[11], c->stream);
+        if (timed) rec(c, 11);
         if (int r = check_launch(c)) return r;
         if (timed) {
             NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -319,6 +327,8 @@ nbg_ctx* nbg_create(int device, int64_t max_n) {
     r |= alloc_into(c, c->nd_com, C);
     r |= alloc_into(c, c->nd_quad, C);
     r |= alloc_into(c, c->nd_aux, C);
+    r |= alloc_into(c, c->nd_com64, C);
+    r |= alloc_into(c, c->nd_rmax64, C);
     r |= alloc_into(c, c->depth_nodes, C);
     r |= alloc_into(c, c->cnt_far, C);
     r |= alloc_into(c, c->cnt_near, C);
@@ -353,7 +363,7 @@ void nbg_destroy(nbg_ctx* c) {
     free_dev(c->spos);
     free_dev(c->nd_start); free_dev(c->nd_count); free_dev(c->nd_depth); free_dev(c->nd_parent);
     free_dev(c->nd_oct); free_dev(c->nd_child); free_dev(c->nd_geom);
-    free_dev(c->nd_com); free_dev(c->nd_quad); free_dev(c->nd_aux);
+    free_dev(c->nd_com); free_dev(c->nd_quad); free_dev(c->nd_aux); free_dev(c->nd_com64); free_dev(c->nd_rmax64);
     free_dev(c->depth_nodes); free_dev(c->cnt_far); free_dev(c->cnt_near);
     free_dev(c->dsc); free_dev(c->diag_part);
     free_dev(c->sys_off); free_dev(c->sys_root); free_dev(c->sys_geom); free_dev(c->sys_bbox);
@@ -512,8 +522,11 @@ int nbg_step(nbg_ctx* c, const nbg_params* p, int32_t nsteps, nbg_stats* stats) 
     } else {
         if (!c->graph_valid || !same_params(c->graph_params, *p)) {
             drop_graph(c);
+            (void)hipGetLastError();
             NBG_HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            c->capturing = true;
             enqueue_step(c, *p, kLevelsShort, (p->flags & NBG_FLAG_STAGE_TIMES) != 0);
+            c->capturing = false;
             const hipError_t le = hipGetLastError();
             hipGraph_t g = nullptr;
             const hipError_t ee = hipStreamEndCapture(c->stream, &g);
@@ -621,6 +634,8 @@ int nbg_tree_export(nbg_ctx* c, int64_t* n_nodes, uint64_t* keys, int32_t* order
     }
     if (moments) {
         std::vector<float4> com(un), quad(un);
+        std::vector<double4> com64(un);
+        NBG_HIP_CHECK(c, hipMemcpy(com64.data(), c->nd_com64, un * sizeof(double4), hipMemcpyDeviceToHost));
         NBG_HIP_CHECK(c, hipMemcpy(com.data(), c->nd_com, un * sizeof(float4), hipMemcpyDeviceToHost));
         NBG_HIP_CHECK(c, hipMemcpy(quad.data(), c->nd_quad, un * sizeof(float4), hipMemcpyDeviceToHost));
         // device masses and quadrupoles carry G; report them without it
@@ -630,7 +645,7 @@ int nbg_tree_export(nbg_ctx* c, int64_t* n_nodes, uint64_t* keys, int32_t* order
             o[0] = com[i].x;
             o[1] = com[i].y;
             o[2] = com[i].z;
-            o[3] = com[i].w * ig;
+            o[3] = static_cast<float>(com64[i].w);
             o[4] = aux[i].z;
             o[5] = quad[i].x * ig;
             o[6] = quad[i].y * ig;

@@ -128,6 +128,8 @@ struct nbg_ctx {
     float4* nd_com = nullptr;        // com x,y,z, G*mass
     float4* nd_quad = nullptr;       // G*(Qxx, Qyy, Qxy, Qxz)
     float4* nd_aux = nullptr;        // G*Qyz, (2 halfSize)^2, maxRadius, info bits
+    double4* nd_com64 = nullptr;     // FP64 com x,y,z and mass (without G): exact MAC decisions
+    double* nd_rmax64 = nullptr;     // FP64 maxRadius of leaves (the group radius of the MAC)
     int32_t* leaf_list = nullptr;    // leaf node ids, in pre-order
     int32_t* depth_nodes = nullptr;  // internal node ids grouped by depth
     int32_t* cnt_far = nullptr;      // per node (leaf groups), NBG_FLAG_COUNT
@@ -152,6 +154,7 @@ struct nbg_ctx {
     hipGraph_t graph = nullptr;
     nbg_params graph_params{};
     bool graph_valid = false;
+    bool capturing = false;  // inside hipStreamBeginCapture: events become graph nodes
 };
 
 namespace nbg {
@@ -161,7 +164,7 @@ namespace nbg {
 void launch_bbox_and_keys(nbg_ctx* c, int levels);
 void launch_sort(nbg_ctx* c, int levels);
 void launch_build(nbg_ctx* c, int levels);
-void launch_upsweep(nbg_ctx* c, int levels, int multipole);
+void launch_upsweep(nbg_ctx* c, int levels, int multipole, double G);
 void launch_fill_i32(nbg_ctx* c, int32_t* p, int64_t n, int32_t v);
 // walk.hip
 void launch_walk(nbg_ctx* c, const nbg_params& p);

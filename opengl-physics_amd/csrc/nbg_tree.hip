@@ -720,49 +720,53 @@ void launch_build(nbg_ctx* c, int levels) {
 
 // ============================================================================ upsweep
 // Moments of every node (G-scaled masses): leaves from their particles, internal nodes
-// from their children one depth at a time, deepest first.  Offsets are taken relative
-// to the node centre in FP32; the root combines its children in FP64.
-// Quadrupoles and maxRadius follow Tree.cpp:179-202 and 205-254.
+// from their children one depth at a time, deepest first (Tree.cpp:179-202, 205-254).
+// Centres of mass are reduced in FP64 (exact MAC decisions need them to FP64 accuracy;
+// the walk's M2P uses their FP32 rounding); quadrupoles are accumulated in FP32 from
+// offsets relative to the node's centre of mass.  A massless cell gets an undefined
+// (NaN) centre of mass, as the incremental update of Tree.cpp:120-122 produces: the walk
+// never uses it as a source, and a massless group then opens every cell, as in the
+// reference.
 
 template <bool QUAD>
 __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ leaf_list,
                                                  const int32_t* __restrict__ nd_start,
                                                  const int32_t* __restrict__ nd_count,
-                                                 const double* __restrict__ nd_geom, const float4* __restrict__ spos,
+                                                 const float4* __restrict__ spos, const float* __restrict__ mass,
+                                                 const uint32_t* __restrict__ perm, double G,
                                                  const DevScalars* __restrict__ sc, float4* __restrict__ nd_com,
-                                                 float4* __restrict__ nd_quad, float4* __restrict__ nd_aux) {
+                                                 float4* __restrict__ nd_quad, float4* __restrict__ nd_aux,
+                                                 double4* __restrict__ nd_com64, double* __restrict__ nd_rmax64) {
     const int nl = sc->n_leaves;
     for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nl; li += gridDim.x * blockDim.x) {
         const int id = leaf_list[li];
         const int st = nd_start[id], cnt = nd_count[id];
-        const float cx = static_cast<float>(nd_geom[4 * id + 0]);
-        const float cy = static_cast<float>(nd_geom[4 * id + 1]);
-        const float cz = static_cast<float>(nd_geom[4 * id + 2]);
-        float M = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+        double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
         for (int k = st; k < st + cnt; ++k) {
             const float4 p = spos[k];
-            M += p.w;
-            sx += p.w * (p.x - cx);
-            sy += p.w * (p.y - cy);
-            sz += p.w * (p.z - cz);
+            const double w = mass[perm[k]];
+            M += w;
+            sx += w * p.x;
+            sy += w * p.y;
+            sz += w * p.z;
         }
-        // a massless cell has an undefined (NaN) centre of mass, as the incremental
-        // update of Tree.cpp:120-122 produces; the walk never uses it as a source and a
-        // massless group then opens every cell, as in the reference.
-        float gx = __int_as_float(0x7fc00000), gy = gx, gz = gx;
-        if (M != 0.f) {
-            gx = cx + sx / M;
-            gy = cy + sy / M;
-            gz = cz + sz / M;
+        double gx = __longlong_as_double(0x7ff8000000000000ll), gy = gx, gz = gx;
+        if (M != 0.0) {
+            gx = sx / M;
+            gy = sy / M;
+            gz = sz / M;
         }
-        float qxx = 0.f, qyy = 0.f, qxy = 0.f, qxz = 0.f, qyz = 0.f, r2max = 0.f;
-        if (M > 0.f) {
+        const float fx = static_cast<float>(gx), fy = static_cast<float>(gy), fz = static_cast<float>(gz);
+        float qxx = 0.f, qyy = 0.f, qxy = 0.f, qxz = 0.f, qyz = 0.f;
+        double rmax = 0.0;
+        if (M > 0.0) {
             for (int k = st; k < st + cnt; ++k) {
                 const float4 p = spos[k];
-                const float dx = p.x - gx, dy = p.y - gy, dz = p.z - gz;
-                const float d2 = dx * dx + dy * dy + dz * dz;
-                r2max = fmaxf(r2max, d2);
+                const double ex = p.x - gx, ey = p.y - gy, ez = p.z - gz;
+                rmax = fmax(rmax, sqrt(ex * ex + ey * ey + ez * ez));
                 if (QUAD && cnt > 1) {
+                    const float dx = p.x - fx, dy = p.y - fy, dz = p.z - fz;
+                    const float d2 = dx * dx + dy * dy + dz * dz;
                     qxx += p.w * (3.f * dx * dx - d2);
                     qyy += p.w * (3.f * dy * dy - d2);
                     qxy += p.w * 3.f * dx * dy;
@@ -771,11 +775,13 @@ __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ lea
                 }
             }
         }
-        nd_com[id] = make_float4(gx, gy, gz, M);
+        nd_com64[id] = make_double4(gx, gy, gz, M);
+        nd_rmax64[id] = rmax;
+        nd_com[id] = make_float4(fx, fy, fz, static_cast<float>(G * M));
         nd_quad[id] = make_float4(qxx, qyy, qxy, qxz);
         float4 a = nd_aux[id];
         a.x = qyz;
-        a.z = sqrtf(r2max);
+        a.z = static_cast<float>(rmax);
         nd_aux[id] = a;
     }
 }
@@ -783,9 +789,9 @@ __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ lea
 template <bool QUAD>
 __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* __restrict__ depth_nodes,
                                                      const int32_t* __restrict__ nd_child,
-                                                     const double* __restrict__ nd_geom,
-                                                     const DevScalars* __restrict__ sc, float4* __restrict__ nd_com,
-                                                     float4* __restrict__ nd_quad, float4* __restrict__ nd_aux) {
+                                                     double G, const DevScalars* __restrict__ sc,
+                                                     float4* __restrict__ nd_com, float4* __restrict__ nd_quad,
+                                                     float4* __restrict__ nd_aux, double4* __restrict__ nd_com64) {
     const int cnt = static_cast<int>(sc->depth_count[depth]);
     const int off = static_cast<int>(sc->depth_offset[depth]);
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
@@ -793,58 +799,33 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
         int ch[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) ch[c] = nd_child[8 * id + c];
-        float gx, gy, gz, M;
-        if (depth == 0) {
-            // root: FP64 centre-of-mass reduction
-            double dm = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
+        double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                if (ch[c] < 0) continue;
-                const float4 cc = nd_com[ch[c]];
-                if (cc.w == 0.f) continue;
-                dm += cc.w;
-                dx += static_cast<double>(cc.w) * cc.x;
-                dy += static_cast<double>(cc.w) * cc.y;
-                dz += static_cast<double>(cc.w) * cc.z;
-            }
-            M = static_cast<float>(dm);
-            gx = gy = gz = __int_as_float(0x7fc00000);
-            if (dm != 0.0) {
-                gx = static_cast<float>(dx / dm);
-                gy = static_cast<float>(dy / dm);
-                gz = static_cast<float>(dz / dm);
-            }
-        } else {
-            const float cx = static_cast<float>(nd_geom[4 * id + 0]);
-            const float cy = static_cast<float>(nd_geom[4 * id + 1]);
-            const float cz = static_cast<float>(nd_geom[4 * id + 2]);
-            float sx = 0.f, sy = 0.f, sz = 0.f;
-            M = 0.f;
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                if (ch[c] < 0) continue;
-                const float4 cc = nd_com[ch[c]];
-                if (cc.w == 0.f) continue;
-                M += cc.w;
-                sx += cc.w * (cc.x - cx);
-                sy += cc.w * (cc.y - cy);
-                sz += cc.w * (cc.z - cz);
-            }
-            gx = gy = gz = __int_as_float(0x7fc00000);
-            if (M != 0.f) {
-                gx = cx + sx / M;
-                gy = cy + sy / M;
-                gz = cz + sz / M;
-            }
+        for (int c = 0; c < 8; ++c) {
+            if (ch[c] < 0) continue;
+            const double4 cc = nd_com64[ch[c]];
+            const double w = cc.w;
+            if (w == 0.0) continue;
+            M += w;
+            sx += w * cc.x;
+            sy += w * cc.y;
+            sz += w * cc.z;
         }
+        double gx = __longlong_as_double(0x7ff8000000000000ll), gy = gx, gz = gx;
+        if (M != 0.0) {
+            gx = sx / M;
+            gy = sy / M;
+            gz = sz / M;
+        }
+        const float fx = static_cast<float>(gx), fy = static_cast<float>(gy), fz = static_cast<float>(gz);
         float qxx = 0.f, qyy = 0.f, qxy = 0.f, qxz = 0.f, qyz = 0.f, rmax = 0.f;
-        if (M > 0.f) {
+        if (M > 0.0) {
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
                 if (ch[c] < 0) continue;
                 const float4 cc = nd_com[ch[c]];
                 const float4 ca = nd_aux[ch[c]];
-                const float dx = cc.x - gx, dy = cc.y - gy, dz = cc.z - gz;
+                const float dx = cc.x - fx, dy = cc.y - fy, dz = cc.z - fz;
                 const float d2 = dx * dx + dy * dy + dz * dz;
                 rmax = fmaxf(rmax, ca.z + sqrtf(d2));
                 if (QUAD && cc.w > 0.f) {
@@ -857,7 +838,8 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
                 }
             }
         }
-        nd_com[id] = make_float4(gx, gy, gz, M);
+        nd_com64[id] = make_double4(gx, gy, gz, M);
+        nd_com[id] = make_float4(fx, fy, fz, static_cast<float>(G * M));
         nd_quad[id] = make_float4(qxx, qyy, qxy, qxz);
         float4 a = nd_aux[id];
         a.x = qyz;
@@ -866,26 +848,28 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
     }
 }
 
-void launch_upsweep(nbg_ctx* c, int levels, int multipole) {
+void launch_upsweep(nbg_ctx* c, int levels, int multipole, double G) {
     hipStream_t st = c->stream;
     const int gnodes = 2048;
     const bool quad = multipole == NBG_QUADRUPOLE;
     if (quad)
         hipLaunchKernelGGL(k_up_leaf<true>, dim3(gnodes), dim3(256), 0, st, c->leaf_list, c->nd_start, c->nd_count,
-                           c->nd_geom, c->spos, c->dsc, c->nd_com, c->nd_quad, c->nd_aux);
+                           c->spos, c->m, c->perm, G, c->dsc, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64,
+                           c->nd_rmax64);
     else
         hipLaunchKernelGGL(k_up_leaf<false>, dim3(gnodes), dim3(256), 0, st, c->leaf_list, c->nd_start, c->nd_count,
-                           c->nd_geom, c->spos, c->dsc, c->nd_com, c->nd_quad, c->nd_aux);
+                           c->spos, c->m, c->perm, G, c->dsc, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64,
+                           c->nd_rmax64);
     // internal nodes sit at depths 0..levels-1 (depth-40 cells are always leaves)
     const int top = (levels == kLevelsShort) ? kLevelsShort - 1 : kMaxDepth - 1;
     for (int d = top; d >= 0; --d) {
         const int gd = d < 6 ? 64 : 1024;
         if (quad)
-            hipLaunchKernelGGL(k_up_internal<true>, dim3(gd), dim3(256), 0, st, d, c->depth_nodes, c->nd_child,
-                               c->nd_geom, c->dsc, c->nd_com, c->nd_quad, c->nd_aux);
+            hipLaunchKernelGGL(k_up_internal<true>, dim3(gd), dim3(256), 0, st, d, c->depth_nodes, c->nd_child, G,
+                               c->dsc, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64);
         else
-            hipLaunchKernelGGL(k_up_internal<false>, dim3(gd), dim3(256), 0, st, d, c->depth_nodes, c->nd_child,
-                               c->nd_geom, c->dsc, c->nd_com, c->nd_quad, c->nd_aux);
+            hipLaunchKernelGGL(k_up_internal<false>, dim3(gd), dim3(256), 0, st, d, c->depth_nodes, c->nd_child, G,
+                               c->dsc, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64);
     }
 }
 

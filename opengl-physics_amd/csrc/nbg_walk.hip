@@ -111,6 +111,9 @@ struct WalkArgs {
     const float4* nd_com;
     const float4* nd_quad;
     const float4* nd_aux;
+    const double4* nd_com64;
+    const double* nd_rmax64;
+    const double* nd_geom;
     const int32_t* sys_root;
     const int32_t* pos_sys;
     const float4* spos;
@@ -122,7 +125,9 @@ struct WalkArgs {
     int32_t* cnt_far;
     int32_t* cnt_near;
     float eps2;
+    float theta;
     float theta2;
+    double theta2_64;
 };
 
 template <bool QUAD, bool KAHAN, bool COUNT>
@@ -136,7 +141,7 @@ __global__ __launch_bounds__(64 * kWalkWaves) void k_walk(WalkArgs A) {
     int* far = s_far[wave];
     int* near = s_near[wave];
     const int nl = A.sc->n_leaves;
-    const float eps2 = A.eps2, theta2 = A.theta2;
+    const float eps2 = A.eps2, theta = A.theta, theta2 = A.theta2;
 
     for (int li = blockIdx.x * kWalkWaves + wave; li < nl; li += gridDim.x * kWalkWaves) {
         const int g = A.leaf_list[li];
@@ -144,6 +149,9 @@ __global__ __launch_bounds__(64 * kWalkWaves) void k_walk(WalkArgs A) {
         const int gcnt = A.nd_count[g];
         const float4 gc = A.nd_com[g];
         const float gR = A.nd_aux[g].z;
+        const double4 g64 = A.nd_com64[g];
+        const double gR64 = A.nd_rmax64[g];
+        const float gscale = fabsf(gc.x) + fabsf(gc.y) + fabsf(gc.z) + gR;
         const int root = A.sys_root[A.pos_sys[gst]];
 
         for (int t0 = 0; t0 < gcnt; t0 += kMaxT) {
@@ -185,7 +193,25 @@ __global__ __launch_bounds__(64 * kWalkWaves) void k_walk(WalkArgs A) {
                             const float dx = cm.x - gc.x, dy = cm.y - gc.y, dz = cm.z - gc.z;
                             const float dist2 = dx * dx + dy * dy + dz * dz;
                             const float worst = sqrtf(dist2) - gR;
-                            if (worst > 0.f && ax.y < theta2 * worst * worst)
+                            bool accept = worst > 0.f && ax.y < theta2 * worst * worst;
+                            // Within FP32 reach of the boundary the decision is redone in
+                            // FP64 exactly as CollectGroupInteractions writes it
+                            // (Solvers.cpp:261-269), so the interaction lists are the
+                            // reference's.  The slack bounds the FP32 error of worst and size.
+                            const float sz = sqrtf(ax.y);
+                            const float slack = 1e-4f * sz + 1e-6f * (gscale + fabsf(cm.x) + fabsf(cm.y) + fabsf(cm.z));
+                            if (fabsf(theta * worst - sz) <= slack) {
+                                const double4 c64 = A.nd_com64[node];
+                                const double ex = __dsub_rn(c64.x, g64.x), ey = __dsub_rn(c64.y, g64.y),
+                                             ez = __dsub_rn(c64.z, g64.z);
+                                const double d2 =
+                                    __dadd_rn(__dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey)), __dmul_rn(ez, ez));
+                                const double size = 2.0 * A.nd_geom[4 * node + 3];
+                                const double w64 = __dsub_rn(__dsqrt_rn(d2), gR64);
+                                accept = (w64 > 0.0) &&
+                                         (__dmul_rn(size, size) < __dmul_rn(__dmul_rn(A.theta2_64, w64), w64));
+                            }
+                            if (accept)
                                 is_far = true;
                             else
                                 is_open = true;
@@ -324,8 +350,13 @@ void launch_walk(nbg_ctx* c, const nbg_params& p) {
     A.az = c->az;
     A.cnt_far = c->cnt_far;
     A.cnt_near = c->cnt_near;
+    A.nd_com64 = c->nd_com64;
+    A.nd_rmax64 = c->nd_rmax64;
+    A.nd_geom = c->nd_geom;
     A.eps2 = static_cast<float>(p.eps2);
+    A.theta = static_cast<float>(p.theta);
     A.theta2 = static_cast<float>(p.theta * p.theta);
+    A.theta2_64 = p.theta * p.theta;
     const bool quad = p.multipole == NBG_QUADRUPOLE;
     const bool kahan = (p.flags & NBG_FLAG_KAHAN) != 0;
     const bool count = (p.flags & NBG_FLAG_COUNT) != 0 && c->cnt_far != nullptr;

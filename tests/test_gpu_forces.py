@@ -120,12 +120,15 @@ def test_graph_replay_equals_eager_and_is_deterministic(gpu_ctx_factory):
     import pynbg
     s = plummer_f32(20000, 5)
     res = []
-    for no_graph in (False, True, False):
-        p = pynbg.make_params("barnes_hut", "quadrupole", G=1.0, eps=0.01, theta=0.6, dt=1e-3, no_graph=no_graph)
+    for no_graph, timed in ((False, False), (True, False), (False, True)):
+        p = pynbg.make_params("barnes_hut", "quadrupole", G=1.0, eps=0.01, theta=0.6, dt=1e-3, no_graph=no_graph,
+                              stage_times=timed)
         ctx = gpu_ctx_factory(20000)
         ctx.upload(s["x"], s["y"], s["z"], s["m"], s["vx"], s["vy"], s["vz"])
         ctx.accel(p)
-        ctx.step(p, 5)
+        st = ctx.step(p, 5)
+        if timed:
+            assert st.ms_walk > 0 and st.ms_sort > 0 and st.ms_integrate > 0
         d = ctx.download()
         res.append(np.stack([d[k] for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az")]))
     np.testing.assert_array_equal(res[0], res[1])
