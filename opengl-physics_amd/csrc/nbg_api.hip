@@ -1,6 +1,7 @@
 // nbg_api.hip -- the C ABI of libnbg_hip.so (declared in include/nbg.h): context and
 // device memory, the force-evaluation pipeline, the leapfrog driver and its hipGraph.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -290,6 +291,7 @@ nbg_ctx* nbg_create(int device, int64_t max_n) {
         return nullptr;
     }
     c->cap_n = max_n;
+    if (const char* w = std::getenv("NBG_WALK_KERNEL")) c->walk_kernel = (std::string(w) == "leaf") ? 1 : 0;
     c->cap_nodes = max_n + (40 * max_n + 8) / 9 + 64;
     const size_t N = static_cast<size_t>(max_n);
     const size_t C = static_cast<size_t>(c->cap_nodes);
@@ -316,7 +318,8 @@ nbg_ctx* nbg_create(int device, int64_t max_n) {
     r |= alloc_into(c, c->cpl_prev, N);
     r |= alloc_into(c, c->node_scan, N + 1);
     r |= alloc_into(c, c->leaf_list, N);
-    r |= alloc_into(c, c->spos, N);
+    r |= alloc_into(c, c->leaf_of, N);
+    r |= alloc_into(c, c->spos, N + kSposPad);
     r |= alloc_into(c, c->nd_start, C);
     r |= alloc_into(c, c->nd_count, C);
     r |= alloc_into(c, c->nd_depth, C);
@@ -336,6 +339,11 @@ nbg_ctx* nbg_create(int device, int64_t max_n) {
     c->diag_part_cap = static_cast<int>(diag_partials_needed(max_n));
     r |= alloc_into(c, c->diag_part, static_cast<size_t>(c->diag_part_cap));
     if (r != NBG_OK || hipHostMalloc(reinterpret_cast<void**>(&c->hsc), sizeof(DevScalars)) != hipSuccess) {
+        nbg_destroy(c);
+        return nullptr;
+    }
+    // the walk reads up to 8 particles past a leaf start; the pad stays zero (zero mass)
+    if (hipMemset(c->spos, 0, (N + kSposPad) * sizeof(float4)) != hipSuccess) {
         nbg_destroy(c);
         return nullptr;
     }
@@ -359,7 +367,7 @@ void nbg_destroy(nbg_ctx* c) {
     free_dev(c->key_a); free_dev(c->key_b); free_dev(c->klo_a); free_dev(c->klo_b);
     free_dev(c->val_a); free_dev(c->val_b);
     free_dev(c->hist); free_dev(c->scan_part);
-    free_dev(c->leaf_depth); free_dev(c->cpl_prev); free_dev(c->node_scan); free_dev(c->leaf_list);
+    free_dev(c->leaf_depth); free_dev(c->cpl_prev); free_dev(c->node_scan); free_dev(c->leaf_list); free_dev(c->leaf_of);
     free_dev(c->spos);
     free_dev(c->nd_start); free_dev(c->nd_count); free_dev(c->nd_depth); free_dev(c->nd_parent);
     free_dev(c->nd_oct); free_dev(c->nd_child); free_dev(c->nd_geom);
@@ -367,7 +375,7 @@ void nbg_destroy(nbg_ctx* c) {
     free_dev(c->depth_nodes); free_dev(c->cnt_far); free_dev(c->cnt_near);
     free_dev(c->dsc); free_dev(c->diag_part);
     free_dev(c->sys_off); free_dev(c->sys_root); free_dev(c->sys_geom); free_dev(c->sys_bbox);
-    free_dev(c->blk_sys); free_dev(c->blk_start); free_dev(c->blk_count); free_dev(c->sys_blk0);
+    free_dev(c->blk_sys); free_dev(c->blk_start); free_dev(c->blk_count); free_dev(c->sys_blk0); free_dev(c->bin_start); free_dev(c->bin_count);
     if (c->hsc) (void)hipHostFree(c->hsc);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -405,7 +413,12 @@ int nbg_upload_batch(nbg_ctx* c, int32_t nsys, const int64_t* offsets, const flo
     c->h_sys_off.assign(offsets, offsets + nsys + 1);
     // sort blocks: slices of <= kSortTile keys inside one system
     std::vector<int32_t> bsys, bstart, bcnt, sb0(nsys + 1), off32(nsys + 1), psys(static_cast<size_t>(n));
+    std::vector<int32_t> wstart, wcnt;
     for (int s = 0; s < nsys; ++s) {
+        for (int64_t b = offsets[s]; b < offsets[s + 1]; b += 64) {
+            wstart.push_back(static_cast<int32_t>(b));
+            wcnt.push_back(static_cast<int32_t>(std::min<int64_t>(64, offsets[s + 1] - b)));
+        }
         sb0[s] = static_cast<int32_t>(bsys.size());
         off32[s] = static_cast<int32_t>(offsets[s]);
         for (int64_t b = offsets[s]; b < offsets[s + 1]; b += kSortTile) {
@@ -429,6 +442,9 @@ int nbg_upload_batch(nbg_ctx* c, int32_t nsys, const int64_t* offsets, const flo
     r |= alloc_into(c, c->sys_geom, 4 * static_cast<size_t>(nsys));
     r |= alloc_into(c, c->sys_bbox, 6 * static_cast<size_t>(nsys));
     r |= alloc_into(c, c->hist, static_cast<size_t>(kRadix) * nb);
+    c->n_bins = static_cast<int>(wstart.size());
+    r |= alloc_into(c, c->bin_start, std::max<size_t>(1, wstart.size()));
+    r |= alloc_into(c, c->bin_count, std::max<size_t>(1, wcnt.size()));
     const size_t parts = std::max<size_t>(grid_for(n + 1, 2048), grid_for(static_cast<int64_t>(kRadix) * nb, 2048)) + 8;
     r |= alloc_into(c, c->scan_part, parts);
     if (r != NBG_OK) return r;
@@ -438,6 +454,10 @@ int nbg_upload_batch(nbg_ctx* c, int32_t nsys, const int64_t* offsets, const flo
         NBG_HIP_CHECK(c, hipMemcpy(c->blk_count, bcnt.data(), bcnt.size() * 4, hipMemcpyHostToDevice));
     }
     NBG_HIP_CHECK(c, hipMemcpy(c->sys_blk0, sb0.data(), sb0.size() * 4, hipMemcpyHostToDevice));
+    if (!wstart.empty()) {
+        NBG_HIP_CHECK(c, hipMemcpy(c->bin_start, wstart.data(), wstart.size() * 4, hipMemcpyHostToDevice));
+        NBG_HIP_CHECK(c, hipMemcpy(c->bin_count, wcnt.data(), wcnt.size() * 4, hipMemcpyHostToDevice));
+    }
     NBG_HIP_CHECK(c, hipMemcpy(c->sys_off, off32.data(), off32.size() * 4, hipMemcpyHostToDevice));
     if (n > 0) {
         NBG_HIP_CHECK(c, hipMemcpy(c->pos_sys, psys.data(), psys.size() * 4, hipMemcpyHostToDevice));

@@ -27,6 +27,10 @@ constexpr int kRadix = 1 << kRadixBits;
 // particle count (leaf) or child count (internal).
 constexpr uint32_t kLeafBit = 0x80000000u;
 
+// Zero float4 entries after the sorted particles: the walk loads 8 particles at a time
+// from a leaf start without a bound check.
+constexpr int kSposPad = 8;
+
 // Device-side scalars written by the build, read back only for stats.
 struct DevScalars {
     int32_t n_nodes;
@@ -40,6 +44,7 @@ struct DevScalars {
     uint32_t depth_count[64];   // internal nodes per depth
     uint32_t depth_offset[64];  // exclusive scan of depth_count
     uint32_t depth_fill[64];    // scatter cursors
+    uint32_t work[8];           // per-XCD work counters of the walk
 };
 
 // ---------------------------------------------------------------------------- helpers
@@ -94,6 +99,10 @@ struct nbg_ctx {
     int32_t* blk_start = nullptr;
     int32_t* blk_count = nullptr;
     int32_t* sys_blk0 = nullptr;    // nsys+1: first sort block of each system
+    // walk bins: runs of <= 64 consecutive sorted positions inside one system
+    int n_bins = 0;
+    int32_t* bin_start = nullptr;
+    int32_t* bin_count = nullptr;
 
     // particle state, original order (FP32 SoA)
     float *x = nullptr, *y = nullptr, *z = nullptr, *m = nullptr;
@@ -131,6 +140,7 @@ struct nbg_ctx {
     double4* nd_com64 = nullptr;     // FP64 com x,y,z and mass (without G): exact MAC decisions
     double* nd_rmax64 = nullptr;     // FP64 maxRadius of leaves (the group radius of the MAC)
     int32_t* leaf_list = nullptr;    // leaf node ids, in pre-order
+    int32_t* leaf_of = nullptr;      // sorted position -> its leaf node id
     int32_t* depth_nodes = nullptr;  // internal node ids grouped by depth
     int32_t* cnt_far = nullptr;      // per node (leaf groups), NBG_FLAG_COUNT
     int32_t* cnt_near = nullptr;
@@ -155,6 +165,7 @@ struct nbg_ctx {
     nbg_params graph_params{};
     bool graph_valid = false;
     bool capturing = false;  // inside hipStreamBeginCapture: events become graph nodes
+    int walk_kernel = 0;     // 0: bin walk (one wave per 64 targets), 1: one wave per leaf group
 };
 
 namespace nbg {

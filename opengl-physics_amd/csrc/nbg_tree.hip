@@ -149,6 +149,7 @@ __global__ void k_reset(DevScalars* __restrict__ sc, uint32_t* __restrict__ bbox
         sc->overflow = 0;
         sc->cap_exceeded = 0;
     }
+    if (t < 8) sc->work[t] = 0;
     for (int s = t; s < nsys; s += blockDim.x) {
         bbox[6 * s + 0] = 0xffffffffu;
         bbox[6 * s + 1] = 0xffffffffu;
@@ -565,7 +566,8 @@ __global__ __launch_bounds__(256) void k_link(const int32_t* __restrict__ nd_par
 }
 
 // Child counts, FP64 geometry (replaying Tree.cpp:96-99 along the key), leaf info word,
-// and the per-depth histogram of internal nodes.
+// the per-depth histogram of internal nodes (privatised in LDS, one global atomic per
+// block and depth) and the sorted-position -> leaf map.
 template <int LEVELS>
 __global__ __launch_bounds__(256) void k_node_finish(const uint64_t* __restrict__ kh, const uint64_t* __restrict__ kl,
                                                      const int32_t* __restrict__ pos_sys,
@@ -574,9 +576,15 @@ __global__ __launch_bounds__(256) void k_node_finish(const uint64_t* __restrict_
                                                      const int32_t* __restrict__ nd_count,
                                                      const int32_t* __restrict__ nd_depth,
                                                      const int32_t* __restrict__ nd_child, double* __restrict__ nd_geom,
-                                                     float4* __restrict__ nd_aux, DevScalars* __restrict__ sc) {
+                                                     float4* __restrict__ nd_aux, int32_t* __restrict__ leaf_of,
+                                                     DevScalars* __restrict__ sc) {
+    __shared__ uint32_t s_hist[64];
+    if (threadIdx.x < 64) s_hist[threadIdx.x] = 0;
+    __syncthreads();
     const int nn = sc->n_nodes;
-    for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < nn; id += gridDim.x * blockDim.x) {
+    for (int base = blockIdx.x * blockDim.x; base < nn; base += gridDim.x * blockDim.x) {
+        const int id = base + threadIdx.x;
+        if (id >= nn) break;
         int nk = 0;
 #pragma unroll
         for (int c = 0; c < 8; ++c) nk += nd_child[8 * id + c] >= 0;
@@ -598,10 +606,17 @@ __global__ __launch_bounds__(256) void k_node_finish(const uint64_t* __restrict_
         nd_geom[4 * id + 2] = cz;
         nd_geom[4 * id + 3] = h;
         const double size = 2.0 * h;
-        const uint32_t info = (nk == 0) ? (kLeafBit | static_cast<uint32_t>(nd_count[id])) : static_cast<uint32_t>(nk);
+        const int cnt = nd_count[id];
+        const uint32_t info = (nk == 0) ? (kLeafBit | static_cast<uint32_t>(cnt)) : static_cast<uint32_t>(nk);
         nd_aux[id] = make_float4(0.0f, static_cast<float>(size * size), 0.0f, __uint_as_float(info));
-        if (nk > 0) atomicAdd(&sc->depth_count[d], 1u);
+        if (nk > 0) {
+            atomicAdd(&s_hist[d], 1u);
+        } else {
+            for (int k = i; k < i + cnt; ++k) leaf_of[k] = id;
+        }
     }
+    __syncthreads();
+    if (threadIdx.x < 64 && s_hist[threadIdx.x]) atomicAdd(&sc->depth_count[threadIdx.x], s_hist[threadIdx.x]);
 }
 
 __global__ void k_depth_scan(DevScalars* __restrict__ sc) {
@@ -615,15 +630,29 @@ __global__ void k_depth_scan(DevScalars* __restrict__ sc) {
     }
 }
 
+// Internal nodes grouped by depth: each block ranks its nodes per depth in LDS, then
+// reserves one range per depth with a single global atomic.
 __global__ __launch_bounds__(256) void k_depth_scatter(const int32_t* __restrict__ nd_depth,
                                                        const float4* __restrict__ nd_aux, DevScalars* __restrict__ sc,
                                                        int32_t* __restrict__ depth_nodes) {
+    __shared__ uint32_t s_cnt[64];
+    __shared__ uint32_t s_base[64];
     const int nn = sc->n_nodes;
-    for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < nn; id += gridDim.x * blockDim.x) {
-        const uint32_t info = __float_as_uint(nd_aux[id].w);
-        if (info & kLeafBit) continue;
-        const uint32_t pos = atomicAdd(&sc->depth_fill[nd_depth[id]], 1u);
-        depth_nodes[pos] = id;
+    for (int base = blockIdx.x * blockDim.x; base < nn; base += gridDim.x * blockDim.x) {
+        if (threadIdx.x < 64) s_cnt[threadIdx.x] = 0;
+        __syncthreads();
+        const int id = base + threadIdx.x;
+        int d = -1;
+        uint32_t rank = 0;
+        if (id < nn && !(__float_as_uint(nd_aux[id].w) & kLeafBit)) {
+            d = nd_depth[id];
+            rank = atomicAdd(&s_cnt[d], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < 64 && s_cnt[threadIdx.x]) s_base[threadIdx.x] = atomicAdd(&sc->depth_fill[threadIdx.x], s_cnt[threadIdx.x]);
+        __syncthreads();
+        if (d >= 0) depth_nodes[s_base[d] + rank] = id;
+        __syncthreads();
     }
 }
 
@@ -706,11 +735,11 @@ void launch_build(nbg_ctx* c, int levels) {
     if (levels == kLevelsShort)
         hipLaunchKernelGGL(k_node_finish<kLevelsShort>, dim3(gnodes), dim3(256), 0, st, c->key_sorted, c->klo_sorted,
                            c->pos_sys, c->sys_geom, c->nd_start, c->nd_count, c->nd_depth, c->nd_child, c->nd_geom,
-                           c->nd_aux, c->dsc);
+                           c->nd_aux, c->leaf_of, c->dsc);
     else
         hipLaunchKernelGGL(k_node_finish<kLevelsDeep>, dim3(gnodes), dim3(256), 0, st, c->key_sorted, c->klo_sorted,
                            c->pos_sys, c->sys_geom, c->nd_start, c->nd_count, c->nd_depth, c->nd_child, c->nd_geom,
-                           c->nd_aux, c->dsc);
+                           c->nd_aux, c->leaf_of, c->dsc);
     hipLaunchKernelGGL(k_depth_scan, dim3(1), dim3(64), 0, st, c->dsc);
     hipLaunchKernelGGL(k_depth_scatter, dim3(gnodes), dim3(256), 0, st, c->nd_depth, c->nd_aux, c->dsc,
                        c->depth_nodes);

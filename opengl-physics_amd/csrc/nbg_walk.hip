@@ -331,7 +331,431 @@ __global__ __launch_bounds__(64 * kWalkWaves) void k_walk(WalkArgs A) {
     }
 }
 
-void launch_walk(nbg_ctx* c, const nbg_params& p) {
+// ============================================================================ bin walk
+// One wavefront per bin: <= 64 consecutive sorted particles of one system (one target
+// per lane).  The bin's particles belong to <= 64 leaves; leaf j of the bin is bit j of a
+// 64-bit mask.  The tree is walked once for the whole bin: every stack entry carries the
+// mask of bin leaves still descending through it, and each visited node is classified for
+// every active leaf with that leaf's own group MAC, exactly as CollectGroupInteractions
+// classifies it for that leaf alone.  So each leaf's far and near lists are the
+// reference's; a far cell or near leaf reached by several leaves of the bin is staged once
+// and evaluated by all 64 lanes, each lane keeping the contribution only if its own leaf's
+// bit is in the entry's mask.  Sources are read with wave-uniform (scalar) loads.
+// The walk over the tree visits up to 64 stacked nodes per step, one per lane, each lane
+// looping over the bin's leaves (leaf data broadcast from LDS).
+constexpr int kBinWaves = 4;
+
+// Wave-uniform reads of data no kernel writes while the walk runs (tree and sorted
+// particles): going through the constant address space lets the compiler issue them as
+// scalar (SMEM) loads into SGPRs, one per wave instead of one per lane.
+typedef __attribute__((address_space(4))) const float cfloat;
+typedef __attribute__((address_space(4))) const int32_t cint32;
+typedef __attribute__((address_space(4))) const uint32_t cuint32;
+__device__ __forceinline__ float4 cld(const float4* p, int i) {
+    cfloat* q = (cfloat*)(p + i);
+    return make_float4(q[0], q[1], q[2], q[3]);
+}
+__device__ __forceinline__ int32_t cld(const int32_t* p, int i) { return ((cint32*)p)[i]; }
+__device__ __forceinline__ uint32_t cld_u32(const void* p, int i) { return ((cuint32*)p)[i]; }
+constexpr int kBinStack = 640;
+constexpr int kBinRing = 128;  // power of two; far and near rings
+
+struct BinArgs {
+    int n_pad;  // index of the first zero pad entry after the sorted particles
+    const int32_t* bin_start;
+    const int32_t* bin_count;
+    int n_bins;
+    const int32_t* leaf_of;
+    const int32_t* nd_start;
+    const int32_t* nd_child;
+    const float4* nd_com;
+    const float4* nd_quad;
+    const float4* nd_aux;
+    const double4* nd_com64;
+    const double* nd_rmax64;
+    const double* nd_geom;
+    const int32_t* sys_root;
+    const int32_t* pos_sys;
+    const float4* spos;
+    const uint32_t* perm;
+    DevScalars* sc;
+    float* ax;
+    float* ay;
+    float* az;
+    int32_t* cnt_far;
+    int32_t* cnt_near;
+    float eps2;
+    float inv_theta;
+    double theta2_64;
+};
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+    const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(v)), src, 64));
+    const uint32_t hi = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(v >> 32)), src, 64));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Exclusive prefix over the wave of small non-negative counts (< 16) via bit-sliced ballots.
+__device__ __forceinline__ int wave_excl_small(int v, int* total) {
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const uint64_t m = ballot((v >> b) & 1);
+        off += mbcnt(m) << b;
+        tot += __popcll(m) << b;
+    }
+    *total = tot;
+    return off;
+}
+
+template <bool QUAD, bool KAHAN, bool COUNT>
+__global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
+    __shared__ int s_snode[kBinWaves][kBinStack];
+    __shared__ uint64_t s_smask[kBinWaves][kBinStack];
+    __shared__ int s_fnode[kBinWaves][kBinRing];
+    __shared__ uint64_t s_fmask[kBinWaves][kBinRing];
+    __shared__ int s_qstart[kBinWaves][kBinRing];
+    __shared__ int s_qcnt[kBinWaves][kBinRing];
+    __shared__ uint64_t s_qmask[kBinWaves][kBinRing];
+    __shared__ float4 s_lcom[kBinWaves][64];
+    __shared__ int s_lid[kBinWaves][64];
+    __shared__ int s_next[kBinWaves];
+
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = lane_id();
+    int* snode = s_snode[wave];
+    uint64_t* smask = s_smask[wave];
+    int* fnode = s_fnode[wave];
+    uint64_t* fmask = s_fmask[wave];
+    int* qstart = s_qstart[wave];
+    int* qcnt = s_qcnt[wave];
+    uint64_t* qmask = s_qmask[wave];
+    float4* lcom = s_lcom[wave];
+    int* lid = s_lid[wave];
+    const float eps2 = A.eps2, inv_theta = A.inv_theta;
+
+    // XCD-aware work distribution: workgroups are dispatched round-robin over the 8 XCDs,
+    // so XCD x takes the contiguous bin range [x*per, (x+1)*per) and its waves pull bins
+    // from it in order through one counter per XCD (neighbouring bins share tree nodes in
+    // that XCD's L2).
+    const int xcd = blockIdx.x & 7;
+    const int per = (A.n_bins + 7) >> 3;
+    // once its own range is drained a wave helps the other XCDs' ranges (tail balance)
+    int victim = 0;
+    // stack headroom: a depth-first descent pushes at most 7 siblings per level
+    const int reserve = 8 * (A.sc->max_depth + 1);
+
+    for (;;) {
+        int b = -1;
+        while (victim < 8) {
+            const int r = (xcd + victim) & 7;
+            if (lane == 0) s_next[wave] = static_cast<int>(atomicAdd(&A.sc->work[r], 1u));
+            wave_sync();
+            const int k = __builtin_amdgcn_readfirstlane(s_next[wave]);
+            const int lo = r * per;
+            if (lo + k < min(A.n_bins, lo + per)) {
+                b = lo + k;
+                break;
+            }
+            ++victim;
+        }
+        if (b < 0) break;
+        const int start = cld(A.bin_start, b);
+        const int cnt = cld(A.bin_count, b);
+        const bool tv = lane < cnt;
+        const int tpos = start + (tv ? lane : 0);
+        const float4 tp = A.spos[tpos];
+        const int mylf = tv ? A.leaf_of[tpos] : -1;
+        const int prevlf = (tv && lane > 0) ? A.leaf_of[tpos - 1] : -2;
+        const uint64_t first = ballot(tv && mylf != prevlf);
+        const int nl = __popcll(first);
+        const int j = tv ? (__popcll(first & ((2ull << lane) - 1ull)) - 1) : 63;
+        const uint64_t mybit = tv ? (1ull << j) : 0ull;
+        const uint64_t all = (nl >= 64) ? ~0ull : ((1ull << nl) - 1ull);
+        if (tv && mylf != prevlf) {
+            const float4 c = A.nd_com[mylf];
+            lcom[j] = make_float4(c.x, c.y, c.z, A.nd_aux[mylf].z);
+            lid[j] = mylf;
+            // the leaf's own members (self excluded when evaluated)
+            qstart[j] = A.nd_start[mylf];
+            qcnt[j] = static_cast<int>(__float_as_uint(A.nd_aux[mylf].w) & ~kLeafBit);
+            qmask[j] = 1ull << j;
+        }
+        const int root = cld(A.sys_root, cld(A.pos_sys, start));
+        if (lane == 0) {
+            snode[0] = root;
+            smask[0] = all;
+        }
+        wave_sync();
+
+        float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+        int n_far = 0, n_near = 0;
+        int sp = 1;
+        int f_head = 0, f_tail = 0, q_head = 0, q_tail = nl;
+
+        // M2P of one far cell onto this lane's target
+        auto m2p_one = [&](float4 cm, float4 q, float qyz, float& ex, float& ey, float& ez) {
+            const float dx = cm.x - tp.x, dy = cm.y - tp.y, dz = cm.z - tp.z;
+            const float dist2 = dx * dx + dy * dy + dz * dz;
+            const float inv = __builtin_amdgcn_rsqf(dist2 + eps2);
+            const float f = cm.w * (inv * inv * inv);
+            ex = f * dx;
+            ey = f * dy;
+            ez = f * dz;
+            if (QUAD) {
+                const float qxx = q.x, qyy = q.y, qxy = q.z, qxz = q.w, qzz = -(qxx + qyy);
+                const float qdx = qxx * dx + qxy * dy + qxz * dz;
+                const float qdy = qxy * dx + qyy * dy + qyz * dz;
+                const float qdz = qxz * dx + qyz * dy + qzz * dz;
+                const float qd2 = dx * qdx + dy * qdy + dz * qdz;
+                const float ir = __builtin_amdgcn_rsqf(dist2);
+                const float ir2 = ir * ir;
+                const float ir5 = ir2 * ir2 * ir;
+                const float c2 = 2.5f * qd2 * (ir5 * ir2);
+                ex += c2 * dx - ir5 * qdx;
+                ey += c2 * dy - ir5 * qdy;
+                ez += c2 * dz - ir5 * qdz;
+            }
+        };
+        // evaluates `n` staged far cells starting at ring position `h`, four at a time
+        // (their scalar loads issued before any arithmetic); every lane computes every
+        // cell and keeps it only if its leaf's bit is in the cell's mask
+        auto eval_far = [&](int h, int n) {
+            float sx = 0.f, sy = 0.f, sz = 0.f;
+            for (int e0 = 0; e0 < n; e0 += 4) {
+                int nd[4];
+                float um[4];
+                float4 cm[4], q[4];
+                float qyz[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int slot = (h + min(e0 + u, n - 1)) & (kBinRing - 1);
+                    nd[u] = __builtin_amdgcn_readfirstlane(fnode[slot]);
+                    um[u] = (e0 + u < n && (fmask[slot] & mybit)) ? 1.f : 0.f;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    cm[u] = cld(A.nd_com, nd[u]);
+                    q[u] = cld(A.nd_quad, nd[u]);
+                    qyz[u] = __uint_as_float(cld_u32(A.nd_aux, 4 * nd[u]));
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    float ex, ey, ez;
+                    m2p_one(cm[u], q[u], qyz[u], ex, ey, ez);
+                    sx += (um[u] > 0.f) ? ex : 0.f;
+                    sy += (um[u] > 0.f) ? ey : 0.f;
+                    sz += (um[u] > 0.f) ? ez : 0.f;
+                    if (COUNT) n_far += um[u] > 0.f;
+                }
+            }
+            kadd<KAHAN>(ax, cx, sx);
+            kadd<KAHAN>(ay, cy, sy);
+            kadd<KAHAN>(az, cz, sz);
+        };
+        // evaluates the particles of `n` staged near leaves starting at ring position `h`,
+        // packed eight at a time across leaf boundaries (a scalar cursor walks the leaves;
+        // the eight scalar loads of a group are issued before any arithmetic).  Unused
+        // slots of the last group read the zero pad after the sorted particles.
+        auto eval_near = [&](int h, int n) {
+            float sx = 0.f, sy = 0.f, sz = 0.f;
+            int e = 0, k = 0, st = 0, c = 0;
+            uint64_t em = 0;
+            auto load_entry = [&](int ee) {
+                const int slot = (h + ee) & (kBinRing - 1);
+                st = __builtin_amdgcn_readfirstlane(qstart[slot]);
+                c = __builtin_amdgcn_readfirstlane(qcnt[slot]);
+                em = qmask[slot];
+            };
+            if (n > 0) load_entry(0);
+            int groups = 0;
+            while (e < n) {
+                int idx[8];
+                float us[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (e < n) {
+                        idx[u] = st + k;
+                        us[u] = (em & mybit) ? 1.f : 0.f;
+                        if (COUNT) n_near += (em & mybit) && (st + k != tpos);
+                        if (++k == c) {
+                            k = 0;
+                            if (++e < n) load_entry(e);
+                        }
+                    } else {
+                        idx[u] = A.n_pad;
+                        us[u] = 0.f;
+                    }
+                }
+                float4 sp8[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) sp8[u] = cld(A.spos, idx[u]);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const float4 q = sp8[u];
+                    const float dx = q.x - tp.x, dy = q.y - tp.y, dz = q.z - tp.z;
+                    const float r2 = dx * dx + dy * dy + dz * dz + eps2;
+                    const float inv = __builtin_amdgcn_rsqf(r2);
+                    float f = q.w * (inv * inv * inv);
+                    f = (us[u] > 0.f && idx[u] != tpos) ? f : 0.f;
+                    sx += f * dx;
+                    sy += f * dy;
+                    sz += f * dz;
+                }
+                if ((++groups & 7) == 0) {
+                    kadd<KAHAN>(ax, cx, sx);
+                    kadd<KAHAN>(ay, cy, sy);
+                    kadd<KAHAN>(az, cz, sz);
+                    sx = sy = sz = 0.f;
+                }
+            }
+            kadd<KAHAN>(ax, cx, sx);
+            kadd<KAHAN>(ay, cy, sy);
+            kadd<KAHAN>(az, cz, sz);
+        };
+
+        while (sp > 0) {
+            const int room = kBinStack - sp - reserve;
+            const int k = min(min(64, sp), max(1, room / 7));
+            int node = -1;
+            uint64_t act = 0;
+            if (lane < k) {
+                node = snode[sp - k + lane];
+                act = smask[sp - k + lane];
+            }
+            sp -= k;
+            wave_sync();
+            uint64_t far_m = 0, near_m = 0, open_m = 0;
+            int lstart = 0, lcount = 0;
+            if (node >= 0) {
+                const float4 cm = A.nd_com[node];
+                const float4 au = A.nd_aux[node];
+                const uint32_t info = __float_as_uint(au.w);
+                if (info & kLeafBit) {
+                    // a bin leaf never takes itself as a near leaf
+                    uint64_t excl = 0;
+                    for (int jj = 0; jj < nl; ++jj) excl |= (lid[jj] == node) ? (1ull << jj) : 0ull;
+                    act &= ~excl;
+                    lstart = A.nd_start[node];
+                    lcount = static_cast<int>(info & ~kLeafBit);
+                }
+                if (cm.w > 0.f && act) {
+                    if (info & kLeafBit) {
+                        near_m = act;
+                    } else {
+                        const float sth = sqrtf(au.y) * inv_theta;  // size / theta
+                        const float cscale = fabsf(cm.x) + fabsf(cm.y) + fabsf(cm.z);
+                        for (uint64_t todo = act; todo; todo &= todo - 1) {
+                            const int jj = __ffsll(static_cast<long long>(todo)) - 1;
+                            const uint64_t bit = 1ull << jj;
+                            const float4 lc = lcom[jj];
+                            const float dx = cm.x - lc.x, dy = cm.y - lc.y, dz = cm.z - lc.z;
+                            const float dist2 = dx * dx + dy * dy + dz * dz;
+                            // worst > 0 && size^2 < theta^2 worst^2  <=>  dist > R + size/theta
+                            const float q = lc.w + sth;
+                            bool accept = dist2 > q * q;
+                            const float slack =
+                                2.f * q * (1e-4f * q + 2e-6f * (cscale + fabsf(lc.x) + fabsf(lc.y) + fabsf(lc.z) + lc.w));
+                            if (fabsf(dist2 - q * q) <= slack) {
+                                // FP64 decision exactly as CollectGroupInteractions writes it
+                                // (Solvers.cpp:261-269)
+                                const double4 c64 = A.nd_com64[node];
+                                const double4 g64 = A.nd_com64[lid[jj]];
+                                const double gR64 = A.nd_rmax64[lid[jj]];
+                                const double ex = __dsub_rn(c64.x, g64.x), ey = __dsub_rn(c64.y, g64.y),
+                                             ez = __dsub_rn(c64.z, g64.z);
+                                const double d2 = __dadd_rn(__dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey)),
+                                                            __dmul_rn(ez, ez));
+                                const double size = 2.0 * A.nd_geom[4 * node + 3];
+                                const double w64 = __dsub_rn(__dsqrt_rn(d2), gR64);
+                                accept = (w64 > 0.0) &&
+                                         (__dmul_rn(size, size) < __dmul_rn(__dmul_rn(A.theta2_64, w64), w64));
+                            }
+                            if (accept)
+                                far_m |= bit;
+                            else
+                                open_m |= bit;
+                        }
+                    }
+                }
+            }
+            // far cells -> ring
+            {
+                const bool has = far_m != 0;
+                const uint64_t bm = ballot(has);
+                if (has) {
+                    const int slot = (f_tail + mbcnt(bm)) & (kBinRing - 1);
+                    fnode[slot] = node;
+                    fmask[slot] = far_m;
+                }
+                f_tail += __popcll(bm);
+            }
+            // near leaves -> ring
+            {
+                const bool has = near_m != 0;
+                const uint64_t bm = ballot(has);
+                if (has) {
+                    const int slot = (q_tail + mbcnt(bm)) & (kBinRing - 1);
+                    qstart[slot] = lstart;
+                    qcnt[slot] = lcount;
+                    qmask[slot] = near_m;
+                }
+                q_tail += __popcll(bm);
+            }
+            // opened cells -> push children with the opening leaves' mask
+            {
+                int kids[8];
+                int nk = 0;
+                if (open_m) {
+                    const int4 c0 = reinterpret_cast<const int4*>(A.nd_child)[2 * node];
+                    const int4 c1 = reinterpret_cast<const int4*>(A.nd_child)[2 * node + 1];
+                    kids[0] = c0.x; kids[1] = c0.y; kids[2] = c0.z; kids[3] = c0.w;
+                    kids[4] = c1.x; kids[5] = c1.y; kids[6] = c1.z; kids[7] = c1.w;
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) nk += kids[c] >= 0;
+                }
+                int ktot;
+                const int koff = wave_excl_small(nk, &ktot);
+                if (open_m) {
+                    int w = sp + koff;
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        if (kids[c] >= 0) {
+                            snode[w] = kids[c];
+                            smask[w] = open_m;
+                            ++w;
+                        }
+                    }
+                }
+                sp += ktot;
+            }
+            wave_sync();
+            while (f_tail - f_head >= 64) {
+                eval_far(f_head, 64);
+                f_head += 64;
+            }
+            while (q_tail - q_head >= 64) {
+                eval_near(q_head, 64);
+                q_head += 64;
+            }
+        }
+        eval_far(f_head, f_tail - f_head);
+        eval_near(q_head, q_tail - q_head);
+        if (tv) {
+            const uint32_t o = A.perm[tpos];
+            A.ax[o] = KAHAN ? ax - cx : ax;
+            A.ay[o] = KAHAN ? ay - cy : ay;
+            A.az[o] = KAHAN ? az - cz : az;
+            if (COUNT) {
+                A.cnt_far[mylf] = n_far;
+                A.cnt_near[mylf] = n_near;
+            }
+        }
+        wave_sync();
+    }
+}
+
+void launch_walk_leaf(nbg_ctx* c, const nbg_params& p) {
     WalkArgs A;
     A.leaf_list = c->leaf_list;
     A.nd_start = c->nd_start;
@@ -360,7 +784,6 @@ void launch_walk(nbg_ctx* c, const nbg_params& p) {
     const bool quad = p.multipole == NBG_QUADRUPOLE;
     const bool kahan = (p.flags & NBG_FLAG_KAHAN) != 0;
     const bool count = (p.flags & NBG_FLAG_COUNT) != 0 && c->cnt_far != nullptr;
-    // enough waves to fill 256 CUs several times over; the loop strides over leaves
     const int blocks = 256 * 8;
     const dim3 grid(blocks), block(64 * kWalkWaves);
 #define NBG_WALK(Q, K, C) hipLaunchKernelGGL((k_walk<Q, K, C>), grid, block, 0, c->stream, A)
@@ -378,6 +801,63 @@ void launch_walk(nbg_ctx* c, const nbg_params& p) {
         }
     }
 #undef NBG_WALK
+}
+
+void launch_walk_bins(nbg_ctx* c, const nbg_params& p) {
+    BinArgs A;
+    A.n_pad = c->n;
+    A.bin_start = c->bin_start;
+    A.bin_count = c->bin_count;
+    A.n_bins = c->n_bins;
+    A.leaf_of = c->leaf_of;
+    A.nd_start = c->nd_start;
+    A.nd_child = c->nd_child;
+    A.nd_com = c->nd_com;
+    A.nd_quad = c->nd_quad;
+    A.nd_aux = c->nd_aux;
+    A.nd_com64 = c->nd_com64;
+    A.nd_rmax64 = c->nd_rmax64;
+    A.nd_geom = c->nd_geom;
+    A.sys_root = c->sys_root;
+    A.pos_sys = c->pos_sys;
+    A.spos = c->spos;
+    A.perm = c->perm;
+    A.sc = c->dsc;
+    A.ax = c->ax;
+    A.ay = c->ay;
+    A.az = c->az;
+    A.cnt_far = c->cnt_far;
+    A.cnt_near = c->cnt_near;
+    A.eps2 = static_cast<float>(p.eps2);
+    A.inv_theta = (p.theta > 0.0) ? static_cast<float>(1.0 / p.theta) : __builtin_inff();
+    A.theta2_64 = p.theta * p.theta;
+    const bool quad = p.multipole == NBG_QUADRUPOLE;
+    const bool kahan = (p.flags & NBG_FLAG_KAHAN) != 0;
+    const bool count = (p.flags & NBG_FLAG_COUNT) != 0 && c->cnt_far != nullptr;
+    // 8 XCDs x 32 CUs x 3 resident blocks
+    const dim3 grid(8 * 32 * 3), block(64 * kBinWaves);
+#define NBG_WALKB(Q, K, C) hipLaunchKernelGGL((k_walk_bins<Q, K, C>), grid, block, 0, c->stream, A)
+    if (quad) {
+        if (kahan) {
+            if (count) NBG_WALKB(true, true, true); else NBG_WALKB(true, true, false);
+        } else {
+            if (count) NBG_WALKB(true, false, true); else NBG_WALKB(true, false, false);
+        }
+    } else {
+        if (kahan) {
+            if (count) NBG_WALKB(false, true, true); else NBG_WALKB(false, true, false);
+        } else {
+            if (count) NBG_WALKB(false, false, true); else NBG_WALKB(false, false, false);
+        }
+    }
+#undef NBG_WALKB
+}
+
+void launch_walk(nbg_ctx* c, const nbg_params& p) {
+    if (c->walk_kernel == 1)
+        launch_walk_leaf(c, p);
+    else
+        launch_walk_bins(c, p);
 }
 
 // ============================================================================ direct sum

@@ -522,6 +522,39 @@ void orc_accel_bh(const void* h, int n, const double* x, const double* y, const 
     (void)p;
 }
 
+// The interaction lists CollectGroupInteractions (Solvers.cpp:248-277) builds for every
+// leaf, as CSR arrays over canonical ids: leaf k (in pre-order among leaves) has far
+// cells far_idx[far_ptr[k] .. far_ptr[k+1]) and near leaves near_idx[...] in walk order.
+// Call with far_idx == NULL to get the totals in far_ptr[nleaves], near_ptr[nleaves].
+void orc_group_lists(const void* h, double theta, int64_t* far_ptr, int32_t* far_idx, int64_t* near_ptr,
+                     int32_t* near_idx) {
+    const Tree& t = *static_cast<const Tree*>(h);
+    const double theta2 = theta * theta;
+    std::vector<int> far, near;
+    int64_t nf = 0, nq = 0;
+    int k = 0;
+    for (int ci = 0; ci < static_cast<int>(t.fromCanon.size()); ++ci) {
+        const int g = t.fromCanon[ci];
+        if (t.childCount[g] != 0 || t.count[g] == 0) continue;
+        far.clear();
+        near.clear();
+        Collect(t, 0, g, t.comx[g], t.comy[g], t.comz[g], t.maxR[g], theta2, far, near);
+        far_ptr[k] = nf;
+        near_ptr[k] = nq;
+        if (far_idx)
+            for (int nd : far) far_idx[nf++] = t.canon[nd];
+        else
+            nf += static_cast<int64_t>(far.size());
+        if (near_idx)
+            for (int nd : near) near_idx[nq++] = t.canon[nd];
+        else
+            nq += static_cast<int64_t>(near.size());
+        ++k;
+    }
+    far_ptr[k] = nf;
+    near_ptr[k] = nq;
+}
+
 // ComputeAccelBarnesHutPerParticle (Solvers.cpp:359-382): relative MAC when
 // relative != 0 and aOld (nullable) holds each particle's previous |a|.
 void orc_accel_bh_pp(const void* h, int n, const double* x, const double* y, const double* z, const double* m,

@@ -43,6 +43,7 @@ def lib(fast=False):
         [_i64] * 2
     L.orc_accel_bh_pp.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_d] * 4 + [ctypes.c_double] * 3 + \
         [ctypes.c_int, ctypes.c_double, _d] + [_d] * 3
+    L.orc_group_lists.argtypes = [ctypes.c_void_p, ctypes.c_double, _i64, _i, _i64, _i]
     L.orc_accel_direct.argtypes = [ctypes.c_int] + [_d] * 4 + [ctypes.c_double] * 2 + [_d] * 3
     L.orc_accel_direct_targets.argtypes = [ctypes.c_int] + [_d] * 4 + [ctypes.c_double] * 2 + \
         [ctypes.c_int, _i] + [_d] * 3
@@ -124,6 +125,19 @@ class Tree:
                              _ptr(nnear, _i64))
         a = np.stack([ax, ay, az], axis=1)
         return (a, nfar, nnear) if counts else a
+
+    def group_lists(self, theta):
+        """Per-leaf CSR interaction lists (canonical ids, leaves in pre-order):
+        (far_ptr, far_idx, near_ptr, near_idx)."""
+        e = self.export()
+        nl = int(np.sum((e["child_count"] == 0) & (e["count"] > 0)))
+        fp = np.zeros(nl + 1, np.int64)
+        qp = np.zeros(nl + 1, np.int64)
+        self._L.orc_group_lists(self._h, theta, _ptr(fp, _i64), None, _ptr(qp, _i64), None)
+        fi = np.empty(int(fp[-1]), np.int32)
+        qi = np.empty(int(qp[-1]), np.int32)
+        self._L.orc_group_lists(self._h, theta, _ptr(fp, _i64), _ptr(fi, _i), _ptr(qp, _i64), _ptr(qi, _i))
+        return fp, fi, qp, qi
 
     def accel_bh_per_particle(self, G, eps2, theta, relative=False, alpha=0.001, a_old=None):
         n = self.n
