@@ -30,6 +30,17 @@ FLOP_M2P_QUAD = 65.0
 FLOP_M2P_MONO = 20.0
 
 
+def walk_traffic(n, theta):
+    """HBM bytes per walk launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes of
+    this workload (scaled by a copy kernel of known size), or None for other workloads."""
+    path = os.path.join(ROOT, "profiles", "walk_traffic_plummer_1m_theta0.5.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d["bytes_per_launch"] if (d["n"] == n and d["theta"] == theta) else None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -124,8 +135,9 @@ def main():
                    f"plummer_bh_theta{args.theta}", "n": n, "theta": args.theta, "eps": args.eps, "dt": args.dt,
                    "multipole": args.multipole, "kahan": kahan, "variant": "plain"},
         "roofline": {"bound": "mfma", "pipe": "fp32-valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                     "kernel": "k_walk", "kernel_ms": round(walk_ms, 4),
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": walk_traffic(n, args.theta),
+                     "kernel": "k_walk_bins", "kernel_ms": round(walk_ms, 4),
                      "flop_per_launch": flops},
         "interactions_per_s": round((n_p2p + n_m2p) / (walk_ms * 1e-3), 1),
         "interactions_per_step": {"p2p": n_p2p, "m2p": n_m2p},
