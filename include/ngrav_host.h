@@ -24,6 +24,12 @@ int ngh_ic_hernquist(int n, uint32_t seed, double total_mass, double scale_radiu
 int ngh_ic_king(int n, uint32_t seed, double total_mass, double core_radius, double w0, double G, double* x,
                 double* y, double* z, double* vx, double* vy, double* vz, double* m);
 
+/* Deck-driven batch run on the GPU backend: DeckSim::Configure + fw::RunHeadless +
+ * fw::OutputWriter (projects/nbody_core/src/DeckSim.cpp:26-169,
+ * framework/src/HeadlessRunner.cpp:11-41, framework/src/OutputWriter.cpp:38-169).
+ * frames <= 0 uses the deck's time.frames.  Returns 0, or -1 with a message in err. */
+int ngh_run_deck(const char* deck_path, const char* out_dir, int frames, int quiet, char* err, int err_len);
+
 #ifdef __cplusplus
 }
 #endif

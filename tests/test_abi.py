@@ -44,3 +44,22 @@ def test_binding_covers_the_header():
     sys.path.insert(0, os.path.join(ROOT, "opengl-physics_amd"))
     import pynbg
     assert set(declared_functions(HEADER)) <= set(pynbg.EXPORTED) | {"nbg_upload_batch"}
+
+
+HOST_HEADER = os.path.join(ROOT, "include", "ngrav_host.h")
+HOST_LIB = os.path.join(ROOT, "opengl-physics_amd", "libngrav_host.so")
+
+
+def test_host_library_exports_its_header():
+    names = [n for n in re.findall(r"\b(ngh_[a-z_0-9]+)\s*\(", re.sub(r"/\*.*?\*/", "", open(HOST_HEADER).read(),
+                                                                    flags=re.S))]
+    lib = ctypes.CDLL(HOST_LIB)
+    assert names and all(hasattr(lib, n) for n in names)
+
+
+def test_deck_runner_rejects_unsupported_features(tmp_path):
+    """Error path on the CPU: the deck is validated before any device work."""
+    exe = os.path.join(ROOT, "opengl-physics_amd", "nbg_deck")
+    r = subprocess.run([exe, "--deck", os.path.join(ROOT, "tests", "decks", "spline_unsupported.toml"), "--out",
+                        str(tmp_path), "--quiet"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "spline" in r.stderr
