@@ -39,7 +39,7 @@ extern "C" {
 /* flags */
 #define NBG_FLAG_KAHAN 1u           /* compensated FP32 force sums */
 #define NBG_FLAG_COUNT 2u           /* record per-group interaction counts (nbg_interaction_counts) */
-#define NBG_FLAG_LEARNED 4u         /* add the learned far-field correction (nbg_mlp_set_weights) */
+#define NBG_FLAG_LEARNED 4u         /* add the learned far-field correction (nbg_learn_set_weights) */
 #define NBG_FLAG_NO_GRAPH 8u        /* nbg_step: launch eagerly instead of replaying a hipGraph */
 #define NBG_FLAG_STAGE_TIMES 16u    /* nbg_step: per-stage HIP-event times summed over all steps
                                        (one host synchronisation per step) */
@@ -57,6 +57,9 @@ typedef struct nbg_params {
     double theta;   /* MacParams::theta (geometric MAC) */
     double dt;      /* StepParams::dt (nbg_step only) */
     uint32_t flags; /* NBG_FLAG_* */
+    double learn_band; /* NBG_FLAG_LEARNED: an accepted cell gets the learned correction when
+                          size / (distance - group radius) exceeds learn_band (0 = every
+                          accepted cell).  No reference counterpart. */
 } nbg_params;
 
 /* Opt-in stage timings and tree statistics, like GpuBarnesHutStats
@@ -76,6 +79,8 @@ typedef struct nbg_stats {
     double ms_walk;       /* tree walk + P2P/M2P (or direct sum) */
     double ms_integrate;  /* kick/drift kernels */
     double ms_total;
+    int64_t n_learned;    /* (leaf, cell) pairs given the learned correction (last evaluation) */
+    double ms_learned;    /* learned-correction kernels (inside ms_walk's evaluation) */
 } nbg_stats;
 
 /* Create a context on HIP device `device` able to hold up to max_n particles.
@@ -156,6 +161,54 @@ int nbg_diagnostics(nbg_ctx* ctx, const nbg_params* p, int32_t with_potential, d
  * (projects/nbody_core/src/Solvers.cpp:108-134).  Results go to host arrays of length nt. */
 int nbg_accel_direct_targets(nbg_ctx* ctx, const nbg_params* p, int64_t nt, const int64_t* targets, float* ax,
                              float* ay, float* az);
+
+/* ---------------------------------------------------------------------------------
+ * Learned far-field correction.  No reference counterpart: the reference has no learned
+ * component.  A multilayer perceptron (13 inputs, three hidden layers of 64 with SiLU,
+ * 3 outputs) maps the geometry and moments of one accepted (leaf group, cell) pair to the
+ * residual of that cell's multipole force against direct summation, averaged over the
+ * group's members; the walk adds the prediction to every member of the group.  Inference
+ * and training run in BF16 on the matrix cores with FP32 accumulation.
+ *
+ * Model blob (NBG_LEARN_NPARAM floats): feature mean[16], feature 1/std[16], then
+ * W1[13][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64][3] b4[3] (y = x W + b).
+ * ------------------------------------------------------------------------------- */
+#define NBG_LEARN_FEATURES 13
+#define NBG_LEARN_HIDDEN 64
+#define NBG_LEARN_NTRAIN 9411 /* trainable parameters */
+#define NBG_LEARN_NPARAM (32 + NBG_LEARN_NTRAIN)
+
+/* Install a model blob (FP32 master copy; BF16 copy derived).  Required before
+ * NBG_FLAG_LEARNED evaluations and before nbg_learn_train. */
+int nbg_learn_set_weights(nbg_ctx* ctx, const float* blob, int64_t n);
+/* Copy the current model blob (after training) to the host. */
+int nbg_learn_get_weights(nbg_ctx* ctx, float* blob, int64_t n);
+
+/* Append training samples from the uploaded snapshot to the context's sample pool:
+ * direct-sum accelerations, then a Barnes-Hut walk with p->theta and p->learn_band that
+ * lists its accepted (leaf, cell) pairs, then for each pair (up to max_new) its 13
+ * features, the exact residual (FP64 direct sum over the cell minus the walk's multipole
+ * force, averaged over the leaf's members, divided by the pair's scale) and a loss weight.
+ * *n_pool receives the pool size. */
+int nbg_learn_collect(nbg_ctx* ctx, const nbg_params* p, int64_t max_new, int64_t* n_pool);
+int nbg_learn_clear(nbg_ctx* ctx);
+/* Download pool samples [offset, offset+count): feats 16 floats per sample (0..12 the
+ * features, 13 and 14 the leaf and cell node ids as int32 bits, numbered as
+ * nbg_tree_export numbers them), targets 4 floats (residual x,y,z and loss weight). */
+int nbg_learn_pool(nbg_ctx* ctx, int64_t offset, int64_t count, float* feats, float* targets);
+/* Set the feature standardisation of the installed model from the pool's mean and
+ * standard deviation (first `count` samples, all if <= 0). */
+int nbg_learn_standardize(nbg_ctx* ctx, int64_t count);
+/* Adam on the pool: `steps` steps of `batch` samples (multiple of 64) drawn with
+ * replacement by a counter-based hash of `seed`.  loss (nullable, `steps` floats)
+ * receives the weighted mean-square loss of each step's batch before its update. */
+int nbg_learn_train(nbg_ctx* ctx, int32_t steps, int32_t batch, float lr, uint64_t seed, float* loss);
+/* Forward + backward of the installed model on n caller samples (feats 16 floats,
+ * targets 4 floats each), through the same kernels as training, without an update:
+ * out (n x 3) predictions, *loss = sum_i w_i |y_i - t_i|^2 / n, grad (NBG_LEARN_NTRAIN)
+ * d loss / d parameters in blob order (without the standardisation entries). */
+int nbg_learn_grad(nbg_ctx* ctx, int64_t n, const float* feats, const float* targets, float* out, float* loss,
+                   float* grad);
 
 #ifdef __cplusplus
 }

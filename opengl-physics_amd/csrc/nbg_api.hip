@@ -1,6 +1,7 @@
 // nbg_api.hip -- the C ABI of libnbg_hip.so (declared in include/nbg.h): context and
 // device memory, the force-evaluation pipeline, the leapfrog driver and its hipGraph.
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -67,6 +68,21 @@ int validate(nbg_ctx* c, const nbg_params* p) {
         c->set_error("no particles uploaded");
         return NBG_E_STATE;
     }
+    if (p->flags & NBG_FLAG_LEARNED) {
+        if (p->solver != NBG_SOLVER_BARNES_HUT || p->multipole != NBG_QUADRUPOLE) {
+            c->set_error("the learned correction needs the Barnes-Hut solver with quadrupoles");
+            return NBG_E_INVALID;
+        }
+        if (!c->have_weights) {
+            c->set_error("no learned model installed (nbg_learn_set_weights)");
+            return NBG_E_STATE;
+        }
+        if (!(p->learn_band >= 0.0)) {
+            c->set_error("learn_band must be >= 0");
+            return NBG_E_INVALID;
+        }
+        if (int r = learn_alloc(c)) return r;
+    }
     return NBG_OK;
 }
 
@@ -78,8 +94,11 @@ inline void rec(nbg_ctx* c, int i) {
         (void)hipEventRecord(c->ev[i], c->stream);
 }
 
+void drop_graph(nbg_ctx* c);
+
 // Enqueue one Barnes-Hut force evaluation (no host synchronisation).
-void enqueue_bh(nbg_ctx* c, const nbg_params& p, int levels, bool timed) {
+void enqueue_bh(nbg_ctx* c, const nbg_params& p, int levels, bool timed, bool list_only = false) {
+    const bool learn = (p.flags & NBG_FLAG_LEARNED) != 0;
     if (timed) rec(c, 0);
     launch_bbox_and_keys(c, levels);
     if (timed) rec(c, 1);
@@ -89,6 +108,7 @@ void enqueue_bh(nbg_ctx* c, const nbg_params& p, int levels, bool timed) {
     launch_gather(c, static_cast<float>(p.G));
     if (timed) rec(c, 3);
     launch_upsweep(c, levels, p.multipole, p.G);
+    if (learn) launch_learn_moments(c, levels);
     if (timed) rec(c, 4);
     if ((p.flags & NBG_FLAG_COUNT) && c->cnt_far) {
         launch_fill_i32(c, c->cnt_far, c->cap_nodes, 0);
@@ -96,6 +116,12 @@ void enqueue_bh(nbg_ctx* c, const nbg_params& p, int levels, bool timed) {
     }
     launch_walk(c, p);
     if (timed) rec(c, 5);
+    if (learn && !list_only) {
+        if (timed) rec(c, 12);
+        launch_learn_zero(c);
+        launch_learn_correct(c, p);
+        if (timed) rec(c, 13);
+    }
 }
 
 float elapsed(hipEvent_t a, hipEvent_t b) {
@@ -119,8 +145,9 @@ void fill_tree_stats(nbg_ctx* c, nbg_stats* s) {
 
 // One full force evaluation with the short-key path, falling back to two-word keys
 // when a depth-21 cell still holds more than ncrit particles.
-int accel_eager(nbg_ctx* c, const nbg_params& p, nbg_stats* stats) {
+int accel_eager(nbg_ctx* c, const nbg_params& p, nbg_stats* stats, bool list_only = false) {
     const bool timed = stats != nullptr;
+    const bool learn = (p.flags & NBG_FLAG_LEARNED) != 0;
     c->last_G = p.G;
     if (p.solver == NBG_SOLVER_DIRECT) {
         if (timed) rec(c, 0);
@@ -138,7 +165,7 @@ int accel_eager(nbg_ctx* c, const nbg_params& p, nbg_stats* stats) {
     }
     int levels = kLevelsShort;
     for (;;) {
-        enqueue_bh(c, p, levels, timed);
+        enqueue_bh(c, p, levels, timed, list_only);
         if (int r = check_launch(c)) return r;
         if (int r = read_scalars(c)) return r;
         if (c->hsc->cap_exceeded) {
@@ -147,6 +174,12 @@ int accel_eager(nbg_ctx* c, const nbg_params& p, nbg_stats* stats) {
         }
         if (c->hsc->overflow && levels == kLevelsShort) {
             levels = kLevelsDeep;
+            continue;
+        }
+        if (learn && static_cast<int64_t>(c->hsc->learn_pairs) > c->lpairs_cap) {
+            // the pair list was too short: grow it and evaluate again (the state is unchanged)
+            drop_graph(c);
+            if (int r = learn_grow_pairs(c, c->hsc->learn_pairs)) return r;
             continue;
         }
         break;
@@ -162,6 +195,13 @@ int accel_eager(nbg_ctx* c, const nbg_params& p, nbg_stats* stats) {
         stats->ms_upsweep = elapsed(c->ev[3], c->ev[4]);
         stats->ms_walk = elapsed(c->ev[4], c->ev[5]);
         stats->ms_total = elapsed(c->ev[0], c->ev[5]);
+        if (learn) {
+            stats->n_learned = c->hsc->learn_pairs;
+            if (!list_only) {
+                stats->ms_learned = elapsed(c->ev[12], c->ev[13]);
+                stats->ms_total = elapsed(c->ev[0], c->ev[13]);
+            }
+        }
         if (p.flags & NBG_FLAG_COUNT) {
             // interaction totals: per group counts times group sizes
             const int nn = c->hsc->n_nodes;
@@ -217,6 +257,7 @@ void add_stage_times(nbg_ctx* c, const nbg_params& p, nbg_stats* s) {
         s->ms_sort += elapsed(c->ev[1], c->ev[2]);
         s->ms_build += elapsed(c->ev[2], c->ev[3]);
         s->ms_upsweep += elapsed(c->ev[3], c->ev[4]);
+        if (p.flags & NBG_FLAG_LEARNED) s->ms_learned += elapsed(c->ev[12], c->ev[13]);
     }
 }
 
@@ -242,7 +283,8 @@ void drop_graph(nbg_ctx* c) {
 
 bool same_params(const nbg_params& a, const nbg_params& b) {
     return a.solver == b.solver && a.multipole == b.multipole && a.G == b.G && a.eps2 == b.eps2 &&
-           a.theta == b.theta && a.dt == b.dt && (a.flags & ~NBG_FLAG_COUNT) == (b.flags & ~NBG_FLAG_COUNT);
+           a.theta == b.theta && a.dt == b.dt && (a.flags & ~NBG_FLAG_COUNT) == (b.flags & ~NBG_FLAG_COUNT) &&
+           a.learn_band == b.learn_band;
 }
 
 // Eager steps with a scalar read-back per step to catch the deep-key case.
@@ -270,6 +312,7 @@ int steps_eager(nbg_ctx* c, const nbg_params& p, int nsteps, nbg_stats* stats) {
             stats->ms_build += one.ms_build;
             stats->ms_upsweep += one.ms_upsweep;
             stats->ms_walk += one.ms_walk;
+            stats->ms_learned += one.ms_learned;
             stats->ms_integrate += elapsed(c->ev[8], c->ev[9]) + elapsed(c->ev[10], c->ev[11]);
         }
     }
@@ -374,6 +417,9 @@ void nbg_destroy(nbg_ctx* c) {
     free_dev(c->nd_com); free_dev(c->nd_quad); free_dev(c->nd_aux); free_dev(c->nd_com64); free_dev(c->nd_rmax64);
     free_dev(c->depth_nodes); free_dev(c->cnt_far); free_dev(c->cnt_near);
     free_dev(c->dsc); free_dev(c->diag_part);
+    free_dev(c->nd_mom); free_dev(c->lcorr); free_dev(c->lpairs); free_dev(c->lw32); free_dev(c->lw16);
+    free_dev(c->adam_m); free_dev(c->adam_v); free_dev(c->pool_feat); free_dev(c->pool_tgt); free_dev(c->amag2);
+    free_dev(c->grad_part); free_dev(c->loss_part); free_dev(c->loss_dev);
     free_dev(c->sys_off); free_dev(c->sys_root); free_dev(c->sys_geom); free_dev(c->sys_bbox);
     free_dev(c->blk_sys); free_dev(c->blk_start); free_dev(c->blk_count); free_dev(c->sys_blk0); free_dev(c->bin_start); free_dev(c->bin_count);
     if (c->hsc) (void)hipHostFree(c->hsc);
@@ -562,6 +608,7 @@ int nbg_step(nbg_ctx* c, const nbg_params* p, int32_t nsteps, nbg_stats* stats) 
         }
         if (int r = save_backup(c)) return r;
         NBG_HIP_CHECK(c, hipMemsetAsync(&c->dsc->sticky_overflow, 0, 2 * sizeof(int32_t), c->stream));
+        NBG_HIP_CHECK(c, hipMemsetAsync(&c->dsc->sticky_learn, 0, sizeof(int32_t), c->stream));
         const bool per_step = (p->flags & NBG_FLAG_STAGE_TIMES) && stats;
         for (int s = 0; s < nsteps; ++s) {
             NBG_HIP_CHECK(c, hipGraphLaunch(c->graph_exec, c->stream));
@@ -572,8 +619,10 @@ int nbg_step(nbg_ctx* c, const nbg_params* p, int32_t nsteps, nbg_stats* stats) 
         }
         NBG_HIP_CHECK(c, hipMemcpyAsync(&c->hsc->sticky_overflow, &c->dsc->sticky_overflow, 2 * sizeof(int32_t),
                                         hipMemcpyDeviceToHost, c->stream));
+        NBG_HIP_CHECK(c, hipMemcpyAsync(&c->hsc->sticky_learn, &c->dsc->sticky_learn, sizeof(int32_t),
+                                        hipMemcpyDeviceToHost, c->stream));
         NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-        const bool overflow = c->hsc->sticky_overflow != 0;
+        const bool overflow = c->hsc->sticky_overflow != 0 || c->hsc->sticky_learn != 0;
         const bool cap = c->hsc->sticky_cap != 0;
         if (cap) {
             c->set_error("node pool capacity exceeded");
@@ -756,6 +805,300 @@ int nbg_accel_direct_targets(nbg_ctx* c, const nbg_params* p, int64_t nt, const 
         az[k] = h[3 * static_cast<size_t>(k) + 2];
     }
     c->have_tree = false;
+    return NBG_OK;
+}
+
+// ---------------------------------------------------------------------------- learned correction
+
+extern "C++" {
+namespace {
+uint16_t host_bf16(float f) {
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    b += 0x7fffu + ((b >> 16) & 1u);
+    return static_cast<uint16_t>(b >> 16);
+}
+
+template <typename T>
+int grow_copy(nbg_ctx* c, T*& p, size_t old_count, size_t new_count) {
+    T* q = nullptr;
+    hipError_t e;
+    q = dev_alloc<T>(new_count, e);
+    if (!q) {
+        c->set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        return NBG_E_OOM;
+    }
+    if (p && old_count) NBG_HIP_CHECK(c, hipMemcpy(q, p, old_count * sizeof(T), hipMemcpyDeviceToDevice));
+    if (p) (void)hipFree(p);
+    p = q;
+    return NBG_OK;
+}
+}  // namespace
+}  // extern "C++"
+
+int nbg_learn_set_weights(nbg_ctx* c, const float* blob, int64_t n) {
+    if (!c) return NBG_E_INVALID;
+    if (!blob || n != NBG_LEARN_NPARAM) {
+        c->set_error("model blob must hold NBG_LEARN_NPARAM floats");
+        return NBG_E_INVALID;
+    }
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(blob[i])) {
+            c->set_error("model blob holds a non-finite value");
+            return NBG_E_INVALID;
+        }
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    int r = NBG_OK;
+    if (!c->lw32) {
+        r |= alloc_into(c, c->lw32, NBG_LEARN_NPARAM);
+        r |= alloc_into(c, c->lw16, NBG_LEARN_NTRAIN + 8);
+        r |= alloc_into(c, c->adam_m, NBG_LEARN_NTRAIN);
+        r |= alloc_into(c, c->adam_v, NBG_LEARN_NTRAIN);
+        if (r != NBG_OK) return r;
+    }
+    std::vector<uint16_t> h16(NBG_LEARN_NTRAIN + 8, 0);
+    for (int i = 0; i < NBG_LEARN_NTRAIN; ++i) h16[static_cast<size_t>(i)] = host_bf16(blob[32 + i]);
+    NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    NBG_HIP_CHECK(c, hipMemcpy(c->lw32, blob, NBG_LEARN_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+    NBG_HIP_CHECK(c, hipMemcpy(c->lw16, h16.data(), h16.size() * 2, hipMemcpyHostToDevice));
+    NBG_HIP_CHECK(c, hipMemset(c->adam_m, 0, NBG_LEARN_NTRAIN * sizeof(float)));
+    NBG_HIP_CHECK(c, hipMemset(c->adam_v, 0, NBG_LEARN_NTRAIN * sizeof(float)));
+    c->adam_t = 0;
+    c->have_weights = true;
+    return NBG_OK;
+}
+
+int nbg_learn_get_weights(nbg_ctx* c, float* blob, int64_t n) {
+    if (!c) return NBG_E_INVALID;
+    if (!blob || n < NBG_LEARN_NPARAM) {
+        c->set_error("blob must hold NBG_LEARN_NPARAM floats");
+        return NBG_E_INVALID;
+    }
+    if (!c->have_weights) {
+        c->set_error("no learned model installed");
+        return NBG_E_STATE;
+    }
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    NBG_HIP_CHECK(c, hipMemcpy(blob, c->lw32, NBG_LEARN_NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+    return NBG_OK;
+}
+
+int nbg_learn_clear(nbg_ctx* c) {
+    if (!c) return NBG_E_INVALID;
+    c->pool_n = 0;
+    return NBG_OK;
+}
+
+int nbg_learn_collect(nbg_ctx* c, const nbg_params* p, int64_t max_new, int64_t* n_pool) {
+    if (!c) return NBG_E_INVALID;
+    if (!p) {
+        c->set_error("params is NULL");
+        return NBG_E_INVALID;
+    }
+    nbg_params q = *p;
+    q.solver = NBG_SOLVER_BARNES_HUT;
+    q.multipole = NBG_QUADRUPOLE;
+    q.flags = NBG_FLAG_KAHAN;
+    if (int r = validate(c, &q)) return r;
+    if (!(q.learn_band >= 0.0)) {
+        c->set_error("learn_band must be >= 0");
+        return NBG_E_INVALID;
+    }
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    if (int r = learn_alloc(c)) return r;
+    if (c->n == 0) {
+        if (n_pool) *n_pool = c->pool_n;
+        return NBG_OK;
+    }
+    // direct-sum accelerations of the snapshot: the loss weights use |a|
+    nbg_params d = q;
+    d.solver = NBG_SOLVER_DIRECT;
+    c->last_G = q.G;
+    launch_direct(c, d);
+    launch_learn_amag2(c);
+    if (int r = check_launch(c)) return r;
+    // the walk with the pair list (no correction applied)
+    q.flags |= NBG_FLAG_LEARNED;
+    if (int r = accel_eager(c, q, nullptr, true)) return r;
+    const int64_t npairs = c->hsc->learn_pairs;
+    const int64_t nnew = (max_new > 0) ? std::min(max_new, npairs) : npairs;
+    if (c->pool_n + nnew > c->pool_cap) {
+        const int64_t cap = std::max<int64_t>(c->pool_n + nnew, c->pool_cap + c->pool_cap / 2);
+        if (int r = grow_copy(c, c->pool_feat, static_cast<size_t>(16 * c->pool_n), static_cast<size_t>(16 * cap)))
+            return r;
+        if (int r = grow_copy(c, c->pool_tgt, static_cast<size_t>(4 * c->pool_n), static_cast<size_t>(4 * cap)))
+            return r;
+        c->pool_cap = cap;
+    }
+    if (nnew > 0) {
+        launch_learn_samples(c, q, npairs, nnew, c->pool_n);
+        if (int r = check_launch(c)) return r;
+        NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    }
+    c->pool_n += nnew;
+    c->have_tree = true;
+    if (n_pool) *n_pool = c->pool_n;
+    return NBG_OK;
+}
+
+int nbg_learn_pool(nbg_ctx* c, int64_t offset, int64_t count, float* feats, float* targets) {
+    if (!c) return NBG_E_INVALID;
+    if (offset < 0 || count < 0 || offset + count > c->pool_n) {
+        c->set_error("pool range out of bounds");
+        return NBG_E_INVALID;
+    }
+    if (count == 0) return NBG_OK;
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if (feats)
+        NBG_HIP_CHECK(c, hipMemcpy(feats, c->pool_feat + 16 * offset, static_cast<size_t>(count) * 64,
+                                   hipMemcpyDeviceToHost));
+    if (targets)
+        NBG_HIP_CHECK(c, hipMemcpy(targets, c->pool_tgt + 4 * offset, static_cast<size_t>(count) * 16,
+                                   hipMemcpyDeviceToHost));
+    return NBG_OK;
+}
+
+int nbg_learn_standardize(nbg_ctx* c, int64_t count) {
+    if (!c) return NBG_E_INVALID;
+    if (!c->have_weights) {
+        c->set_error("no learned model installed");
+        return NBG_E_STATE;
+    }
+    const int64_t n = (count <= 0 || count > c->pool_n) ? c->pool_n : count;
+    if (n < 2) {
+        c->set_error("the pool holds fewer than two samples");
+        return NBG_E_STATE;
+    }
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    double* dsum = nullptr;
+    NBG_HIP_CHECK(c, hipMalloc(&dsum, 33 * sizeof(double)));
+    int r = NBG_OK;
+    if (hipMemsetAsync(dsum, 0, 33 * sizeof(double), c->stream) != hipSuccess) r = NBG_E_HIP;
+    double h[33] = {};
+    if (r == NBG_OK) {
+        launch_learn_stats(c, n, dsum);
+        r = check_launch(c);
+    }
+    if (r == NBG_OK && hipStreamSynchronize(c->stream) != hipSuccess) r = NBG_E_HIP;
+    if (r == NBG_OK && hipMemcpy(h, dsum, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) r = NBG_E_HIP;
+    (void)hipFree(dsum);
+    if (r != NBG_OK) {
+        c->set_error("pool statistics failed");
+        return r;
+    }
+    float norm[32];
+    for (int k = 0; k < 16; ++k) {
+        const double mu = h[k] / static_cast<double>(n);
+        const double var = std::max(0.0, h[16 + k] / static_cast<double>(n) - mu * mu);
+        const bool used = k < NBG_LEARN_FEATURES && var > 1e-20;
+        norm[k] = used ? static_cast<float>(mu) : 0.f;
+        norm[16 + k] = used ? static_cast<float>(1.0 / std::sqrt(var)) : 1.f;
+    }
+    const double wmean = h[32] / static_cast<double>(n);
+    c->pool_inv_wmean = (wmean > 0.0) ? static_cast<float>(1.0 / wmean) : 1.f;
+    NBG_HIP_CHECK(c, hipMemcpy(c->lw32, norm, sizeof(norm), hipMemcpyHostToDevice));
+    return NBG_OK;
+}
+
+extern "C++" {
+namespace {
+int train_buffers(nbg_ctx* c, int nblocks, int nloss) {
+    if (nblocks > c->grad_blocks_cap) {
+        if (int r = alloc_into(c, c->grad_part, static_cast<size_t>(nblocks) * NBG_LEARN_NTRAIN)) return r;
+        if (int r = alloc_into(c, c->loss_part, static_cast<size_t>(nblocks))) return r;
+        c->grad_blocks_cap = nblocks;
+    }
+    if (nloss > c->loss_cap) {
+        if (int r = alloc_into(c, c->loss_dev, static_cast<size_t>(nloss))) return r;
+        c->loss_cap = nloss;
+    }
+    return NBG_OK;
+}
+}  // namespace
+}  // extern "C++"
+
+int nbg_learn_train(nbg_ctx* c, int32_t steps, int32_t batch, float lr, uint64_t seed, float* loss) {
+    if (!c) return NBG_E_INVALID;
+    if (!c->have_weights) {
+        c->set_error("no learned model installed");
+        return NBG_E_STATE;
+    }
+    if (steps < 0 || batch < 64 || batch % 64 != 0 || batch > (1 << 22) || !(lr > 0.f)) {
+        c->set_error("steps >= 0, batch a multiple of 64 (<= 4M) and lr > 0 are required");
+        return NBG_E_INVALID;
+    }
+    if (c->pool_n < 1) {
+        c->set_error("the sample pool is empty (nbg_learn_collect)");
+        return NBG_E_STATE;
+    }
+    if (steps == 0) return NBG_OK;
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    const int nblocks = batch / 64;
+    if (int r = train_buffers(c, nblocks, steps)) return r;
+    for (int s = 0; s < steps; ++s) {
+        launch_learn_fwdbwd(c, c->pool_feat, c->pool_tgt, c->pool_n, nblocks, seed, static_cast<int>(c->adam_t), 1,
+                            c->pool_inv_wmean, 1.f / static_cast<float>(batch), nullptr);
+        ++c->adam_t;
+        launch_learn_adam(c, nblocks, lr, c->adam_t, c->loss_dev + s);
+        if (int r = check_launch(c)) return r;
+    }
+    NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if (loss) {
+        NBG_HIP_CHECK(c, hipMemcpy(loss, c->loss_dev, static_cast<size_t>(steps) * 4, hipMemcpyDeviceToHost));
+        for (int s = 0; s < steps; ++s) loss[s] /= static_cast<float>(batch);
+    }
+    return NBG_OK;
+}
+
+int nbg_learn_grad(nbg_ctx* c, int64_t n, const float* feats, const float* targets, float* out, float* loss,
+                   float* grad) {
+    if (!c) return NBG_E_INVALID;
+    if (!c->have_weights) {
+        c->set_error("no learned model installed");
+        return NBG_E_STATE;
+    }
+    if (n < 1 || n > (1 << 22) || !feats || !targets) {
+        c->set_error("1 <= n <= 4M samples with features and targets are required");
+        return NBG_E_INVALID;
+    }
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    const int nblocks = static_cast<int>((n + 63) / 64);
+    if (int r = train_buffers(c, nblocks, 1)) return r;
+    float *df = nullptr, *dt = nullptr, *dout = nullptr, *dgrad = nullptr;
+    int r = NBG_OK;
+    if (hipMalloc(&df, static_cast<size_t>(n) * 64) != hipSuccess) r = NBG_E_OOM;
+    if (r == NBG_OK && hipMalloc(&dt, static_cast<size_t>(n) * 16) != hipSuccess) r = NBG_E_OOM;
+    if (r == NBG_OK && hipMalloc(&dout, static_cast<size_t>(n) * 12) != hipSuccess) r = NBG_E_OOM;
+    if (r == NBG_OK && hipMalloc(&dgrad, static_cast<size_t>(NBG_LEARN_NTRAIN + 1) * 4) != hipSuccess) r = NBG_E_OOM;
+    if (r == NBG_OK && hipMemcpy(df, feats, static_cast<size_t>(n) * 64, hipMemcpyHostToDevice) != hipSuccess)
+        r = NBG_E_HIP;
+    if (r == NBG_OK && hipMemcpy(dt, targets, static_cast<size_t>(n) * 16, hipMemcpyHostToDevice) != hipSuccess)
+        r = NBG_E_HIP;
+    if (r == NBG_OK) {
+        launch_learn_fwdbwd(c, df, dt, n, nblocks, 0, 0, 0, 1.f, 1.f / static_cast<float>(n), dout);
+        launch_learn_reduce(c, nblocks, dgrad, dgrad + NBG_LEARN_NTRAIN);
+        r = check_launch(c);
+    }
+    if (r == NBG_OK && hipStreamSynchronize(c->stream) != hipSuccess) r = NBG_E_HIP;
+    float lsum = 0.f;
+    if (r == NBG_OK && out && hipMemcpy(out, dout, static_cast<size_t>(n) * 12, hipMemcpyDeviceToHost) != hipSuccess)
+        r = NBG_E_HIP;
+    if (r == NBG_OK && grad &&
+        hipMemcpy(grad, dgrad, NBG_LEARN_NTRAIN * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        r = NBG_E_HIP;
+    if (r == NBG_OK && hipMemcpy(&lsum, dgrad + NBG_LEARN_NTRAIN, 4, hipMemcpyDeviceToHost) != hipSuccess)
+        r = NBG_E_HIP;
+    (void)hipFree(df);
+    (void)hipFree(dt);
+    (void)hipFree(dout);
+    (void)hipFree(dgrad);
+    if (r != NBG_OK) {
+        if (c->err.empty()) c->set_error("learned forward/backward failed");
+        return r;
+    }
+    if (loss) *loss = lsum / static_cast<float>(n);
     return NBG_OK;
 }
 

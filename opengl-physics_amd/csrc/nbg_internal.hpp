@@ -45,6 +45,9 @@ struct DevScalars {
     uint32_t depth_offset[64];  // exclusive scan of depth_count
     uint32_t depth_fill[64];    // scatter cursors
     uint32_t work[8];           // per-XCD work counters of the walk
+    uint32_t learn_pairs;       // (leaf, cell) pairs listed by the walk for the learned correction
+    int32_t sticky_learn;       // the pair list overflowed during a graph batch (host clears)
+    int32_t pad2[2];
 };
 
 // ---------------------------------------------------------------------------- helpers
@@ -157,7 +160,7 @@ struct nbg_ctx {
     int diag_part_cap = 0;
 
     // timing
-    hipEvent_t ev[12] = {};
+    hipEvent_t ev[14] = {};
 
     // graph of one step
     hipGraphExec_t graph_exec = nullptr;
@@ -166,6 +169,28 @@ struct nbg_ctx {
     bool graph_valid = false;
     bool capturing = false;  // inside hipStreamBeginCapture: events become graph nodes
     int walk_kernel = 0;     // 0: bin walk (one wave per 64 targets), 1: one wave per leaf group
+
+    // learned far-field correction (allocated on first use)
+    float4* nd_mom = nullptr;       // 4 per node: second (6) and third (10) moments about the FP32 com, G*m weights
+    float4* lcorr = nullptr;        // per node: summed correction of a leaf group
+    int2* lpairs = nullptr;         // (leaf, cell) pairs listed by the walk
+    int64_t lpairs_cap = 0;
+    float* lw32 = nullptr;          // FP32 model blob (NBG_LEARN_NPARAM)
+    uint16_t* lw16 = nullptr;       // BF16 copy of the trainable part (blob order)
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    int64_t adam_t = 0;
+    bool have_weights = false;
+    float* pool_feat = nullptr;     // 16 per sample
+    float* pool_tgt = nullptr;      // 4 per sample: residual x,y,z, weight
+    int64_t pool_n = 0, pool_cap = 0;
+    float pool_inv_wmean = 1.f;
+    float* amag2 = nullptr;         // |a_direct|^2 per particle (sample collection)
+    float* grad_part = nullptr;     // per training block partial gradients
+    float* loss_part = nullptr;
+    float* loss_dev = nullptr;
+    int grad_blocks_cap = 0;
+    int loss_cap = 0;
 };
 
 namespace nbg {
@@ -187,5 +212,19 @@ void launch_kick_drift(nbg_ctx* c, float hdt, float dt);
 void launch_copy_f32(nbg_ctx* c, float* dst, const float* src, int64_t n);
 int diagnostics(nbg_ctx* c, const nbg_params& p, int with_potential, double* out);
 int64_t diag_partials_needed(int64_t n);
+// learn.hip
+int learn_alloc(nbg_ctx* c);
+int learn_grow_pairs(nbg_ctx* c, int64_t need);
+void launch_learn_moments(nbg_ctx* c, int levels);
+void launch_learn_zero(nbg_ctx* c);
+void launch_learn_correct(nbg_ctx* c, const nbg_params& p);
+void launch_learn_samples(nbg_ctx* c, const nbg_params& p, int64_t npairs, int64_t nnew, int64_t dst);
+void launch_learn_amag2(nbg_ctx* c);
+void launch_learn_stats(nbg_ctx* c, int64_t count, double* out /* device, 16+16+1 */);
+void launch_learn_fwdbwd(nbg_ctx* c, const float* feat, const float* tgt, int64_t n_src, int nblocks, uint64_t seed,
+                         int step, int use_hash, float inv_wmean, float inv_n, float* out);
+void launch_learn_adam(nbg_ctx* c, int nblocks, float lr, int64_t t, float* loss_slot);
+void launch_learn_reduce(nbg_ctx* c, int nblocks, float* grad_out, float* loss_out);
+// walk.hip: learned-mode walk parameters live in nbg_params (learn_band)
 
 }  // namespace nbg

@@ -1,0 +1,974 @@
+// nbg_learn.hip -- learned far-field correction: moments, features, BF16 MFMA inference
+// and training kernels, sample generation from direct-sum snapshots.
+//
+// The model is a perceptron 13 -> 64 -> 64 -> 64 -> 3 with SiLU.  One evaluation unit is
+// an accepted (leaf group, cell) pair listed by the walk.  Its features describe the cell
+// as seen from the group's centre of mass, in units of the cell:
+//   u (3)        unit vector from the cell's com to the group's com
+//   s/r          cell size over distance
+//   a3 (3)       octupole force shape: 7.5 S.uu - 17.5 (S:uuu) u - 1.5 t + 7.5 (t.u) u,
+//                S = third moments / (M s^3), t_i = S_ijj
+//   Q.u (3)      second moments / (M s^2), contracted with u
+//   R_g/r        group radius over distance
+//   eps/r        softening over distance
+//   tr Q         trace of the second moments / (M s^2)
+// and its output, times the scale G M s^3 / r^5, is the residual of the cell's monopole +
+// quadrupole force against direct summation, averaged over the group's members.
+//
+// Matrix-core layout (v_mfma_f32_16x16x16_bf16, 64 lanes, g = lane / 16, c = lane % 16):
+//   A operand  lane holds A[row c][k 4g .. 4g+3]
+//   B operand  lane holds B[k 4g .. 4g+3][col c]
+//   C/D        lane holds D[row 4g .. 4g+3][col c]
+// Layers are computed transposed (hidden x pairs), so the D registers of one layer are
+// the B operand of the next with no data movement: D rows 4g+j of row block mb are the
+// B rows 4g+j of K block mb.
+#include <cstring>
+#include <vector>
+
+#include "nbg_common.hpp"
+#include "nbg_internal.hpp"
+
+namespace nbg {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kF = NBG_LEARN_FEATURES;  // 13
+constexpr int kH = NBG_LEARN_HIDDEN;    // 64
+// trainable blob offsets (after the 32 standardisation entries)
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kF * kH;   // 832
+constexpr int kOffW2 = kOffB1 + kH;        // 896
+constexpr int kOffB2 = kOffW2 + kH * kH;   // 4992
+constexpr int kOffW3 = kOffB2 + kH;        // 5056
+constexpr int kOffB3 = kOffW3 + kH * kH;   // 9152
+constexpr int kOffW4 = kOffB3 + kH;        // 9216
+constexpr int kOffB4 = kOffW4 + kH * 3;    // 9408
+constexpr int kNTrain = kOffB4 + 3;        // 9411
+static_assert(kNTrain == NBG_LEARN_NTRAIN, "blob layout");
+constexpr int kNorm = 32;                  // mean[16], 1/std[16]
+
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+    uint32_t b = __float_as_uint(f);
+    b += 0x7fffu + ((b >> 16) & 1u);
+    return static_cast<uint16_t>(b >> 16);
+}
+__device__ __forceinline__ float bf16_f32(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+__device__ __forceinline__ float sigmoid_f(float z) {
+    return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+}
+__device__ __forceinline__ float silu_f(float z) { return z * sigmoid_f(z); }
+__device__ __forceinline__ float dsilu_f(float z) {
+    const float s = sigmoid_f(z);
+    return s * (1.f + z * (1.f - s));
+}
+
+__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------- features
+// tgt: group com (xyz) and radius (w); cm: cell com (xyz) and G*M (w); size2 = (2h)^2;
+// mom: the cell's four moment words.  Writes the 13 raw features and the output scale.
+__device__ __forceinline__ void pair_features(float4 tgt, float4 cm, float size2, const float4* mom, float eps,
+                                              float f[kF], float& scale) {
+    const float Rx = tgt.x - cm.x, Ry = tgt.y - cm.y, Rz = tgt.z - cm.z;
+    const float r2 = Rx * Rx + Ry * Ry + Rz * Rz;
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float ux = Rx * ir, uy = Ry * ir, uz = Rz * ir;
+    const float s = sqrtf(size2);
+    const float M = cm.w;
+    const float i2 = 1.f / (M * size2);
+    const float i3 = i2 / s;
+    const float4 m0 = mom[0], m1 = mom[1], m2 = mom[2], m3 = mom[3];
+    const float sxx = m0.x, syy = m0.y, szz = m0.z, sxy = m0.w, sxz = m1.x, syz = m1.y;
+    const float xxx = m1.z, yyy = m1.w, zzz = m2.x, xxy = m2.y, xxz = m2.z, xyy = m2.w, yyz = m3.x, xzz = m3.y,
+                yzz = m3.z, xyz = m3.w;
+    const float uxx = ux * ux, uyy = uy * uy, uzz = uz * uz, uxy = 2.f * ux * uy, uxz = 2.f * ux * uz,
+                uyz = 2.f * uy * uz;
+    const float Sx = (xxx * uxx + xyy * uyy + xzz * uzz + xxy * uxy + xxz * uxz + xyz * uyz) * i3;
+    const float Sy = (xxy * uxx + yyy * uyy + yzz * uzz + xyy * uxy + xyz * uxz + yyz * uyz) * i3;
+    const float Sz = (xxz * uxx + yyz * uyy + zzz * uzz + xyz * uxy + xzz * uxz + yzz * uyz) * i3;
+    const float tx = (xxx + xyy + xzz) * i3, ty = (xxy + yyy + yzz) * i3, tz = (xxz + yyz + zzz) * i3;
+    const float Suuu = Sx * ux + Sy * uy + Sz * uz;
+    const float tu = tx * ux + ty * uy + tz * uz;
+    const float cu = 7.5f * tu - 17.5f * Suuu;
+    f[0] = ux;
+    f[1] = uy;
+    f[2] = uz;
+    f[3] = s * ir;
+    f[4] = 7.5f * Sx - 1.5f * tx + cu * ux;
+    f[5] = 7.5f * Sy - 1.5f * ty + cu * uy;
+    f[6] = 7.5f * Sz - 1.5f * tz + cu * uz;
+    f[7] = (sxx * ux + sxy * uy + sxz * uz) * i2;
+    f[8] = (sxy * ux + syy * uy + syz * uz) * i2;
+    f[9] = (sxz * ux + syz * uy + szz * uz) * i2;
+    f[10] = tgt.w * ir;
+    f[11] = eps * ir;
+    f[12] = (sxx + syy + szz) * i2;
+    const float ir2 = ir * ir;
+    scale = M * (s * size2) * (ir2 * ir2 * ir);
+}
+
+// ---------------------------------------------------------------------------- moments
+// Second and third moments about the FP32 com with G*m weights, packed as
+//   [xx yy zz xy] [xz yz xxx yyy] [zzz xxy xxz xyy] [yyz xzz yzz xyz]
+__global__ __launch_bounds__(256) void k_mom_leaf(const int32_t* __restrict__ leaf_list,
+                                                  const int32_t* __restrict__ nd_start,
+                                                  const int32_t* __restrict__ nd_count,
+                                                  const float4* __restrict__ spos, const float4* __restrict__ nd_com,
+                                                  const DevScalars* __restrict__ sc, float4* __restrict__ nd_mom) {
+    const int nl = sc->n_leaves;
+    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nl; li += gridDim.x * blockDim.x) {
+        const int id = leaf_list[li];
+        const int st = nd_start[id], cnt = nd_count[id];
+        const float4 c = nd_com[id];
+        float a[16] = {};
+        if (c.w > 0.f) {
+            for (int k = st; k < st + cnt; ++k) {
+                const float4 p = spos[k];
+                const float dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z, w = p.w;
+                a[0] += w * dx * dx;
+                a[1] += w * dy * dy;
+                a[2] += w * dz * dz;
+                a[3] += w * dx * dy;
+                a[4] += w * dx * dz;
+                a[5] += w * dy * dz;
+                a[6] += w * dx * dx * dx;
+                a[7] += w * dy * dy * dy;
+                a[8] += w * dz * dz * dz;
+                a[9] += w * dx * dx * dy;
+                a[10] += w * dx * dx * dz;
+                a[11] += w * dx * dy * dy;
+                a[12] += w * dy * dy * dz;
+                a[13] += w * dx * dz * dz;
+                a[14] += w * dy * dz * dz;
+                a[15] += w * dx * dy * dz;
+            }
+        }
+        nd_mom[4 * id + 0] = make_float4(a[0], a[1], a[2], a[3]);
+        nd_mom[4 * id + 1] = make_float4(a[4], a[5], a[6], a[7]);
+        nd_mom[4 * id + 2] = make_float4(a[8], a[9], a[10], a[11]);
+        nd_mom[4 * id + 3] = make_float4(a[12], a[13], a[14], a[15]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mom_internal(int depth, const int32_t* __restrict__ depth_nodes,
+                                                      const int32_t* __restrict__ nd_child,
+                                                      const float4* __restrict__ nd_com,
+                                                      const DevScalars* __restrict__ sc, float4* __restrict__ nd_mom) {
+    const int cnt = static_cast<int>(sc->depth_count[depth]);
+    const int off = static_cast<int>(sc->depth_offset[depth]);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+        const int id = depth_nodes[off + k];
+        const float4 p = nd_com[id];
+        float a[16] = {};
+        if (p.w > 0.f) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int ch = nd_child[8 * id + c];
+                if (ch < 0) continue;
+                const float4 cc = nd_com[ch];
+                if (!(cc.w > 0.f)) continue;
+                const float4 q0 = nd_mom[4 * ch + 0], q1 = nd_mom[4 * ch + 1], q2 = nd_mom[4 * ch + 2],
+                             q3 = nd_mom[4 * ch + 3];
+                const float M = cc.w, dx = cc.x - p.x, dy = cc.y - p.y, dz = cc.z - p.z;
+                const float sxx = q0.x, syy = q0.y, szz = q0.z, sxy = q0.w, sxz = q1.x, syz = q1.y;
+                a[0] += sxx + M * dx * dx;
+                a[1] += syy + M * dy * dy;
+                a[2] += szz + M * dz * dz;
+                a[3] += sxy + M * dx * dy;
+                a[4] += sxz + M * dx * dz;
+                a[5] += syz + M * dy * dz;
+                a[6] += q1.z + 3.f * sxx * dx + M * dx * dx * dx;
+                a[7] += q1.w + 3.f * syy * dy + M * dy * dy * dy;
+                a[8] += q2.x + 3.f * szz * dz + M * dz * dz * dz;
+                a[9] += q2.y + sxx * dy + 2.f * sxy * dx + M * dx * dx * dy;
+                a[10] += q2.z + sxx * dz + 2.f * sxz * dx + M * dx * dx * dz;
+                a[11] += q2.w + syy * dx + 2.f * sxy * dy + M * dx * dy * dy;
+                a[12] += q3.x + syy * dz + 2.f * syz * dy + M * dy * dy * dz;
+                a[13] += q3.y + szz * dx + 2.f * sxz * dz + M * dx * dz * dz;
+                a[14] += q3.z + szz * dy + 2.f * syz * dz + M * dy * dz * dz;
+                a[15] += q3.w + sxy * dz + sxz * dy + syz * dx + M * dx * dy * dz;
+            }
+        }
+        nd_mom[4 * id + 0] = make_float4(a[0], a[1], a[2], a[3]);
+        nd_mom[4 * id + 1] = make_float4(a[4], a[5], a[6], a[7]);
+        nd_mom[4 * id + 2] = make_float4(a[8], a[9], a[10], a[11]);
+        nd_mom[4 * id + 3] = make_float4(a[12], a[13], a[14], a[15]);
+    }
+}
+
+// ---------------------------------------------------------------------------- weights
+// The per-lane weight fragments of the whole network (A operands) and its biases (as
+// C initial values), held in registers for the life of a wave.
+struct Frags {
+    bf16x4 a1[4], a2[4][4], a3[4][4], a4[4];
+    f32x4 b1[4], b2[4], b3[4], b4;
+};
+
+__device__ __forceinline__ bf16x4 ld4_bf16(const uint16_t* w, int i0, int stride, bool valid) {
+    bf16x4 v;
+    v[0] = valid ? static_cast<short>(w[i0]) : 0;
+    v[1] = valid ? static_cast<short>(w[i0 + stride]) : 0;
+    v[2] = valid ? static_cast<short>(w[i0 + 2 * stride]) : 0;
+    v[3] = valid ? static_cast<short>(w[i0 + 3 * stride]) : 0;
+    return v;
+}
+
+__device__ __forceinline__ void load_frags(Frags& F, const uint16_t* __restrict__ w16, const float* __restrict__ b32,
+                                           int lane) {
+    const int g = lane >> 4, c = lane & 15;
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        // A1[h][f] = W1[f][h]: h = 16 mb + c, f = 4g + j (< 13)
+        bf16x4 v;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int f = 4 * g + j;
+            v[j] = (f < kF) ? static_cast<short>(w16[kOffW1 + f * kH + 16 * mb + c]) : 0;
+        }
+        F.a1[mb] = v;
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            // A[h][i] = W[i][h]: h = 16 mb + c, i = 16 kb + 4g + j
+            F.a2[mb][kb] = ld4_bf16(w16, kOffW2 + (16 * kb + 4 * g) * kH + 16 * mb + c, kH, true);
+            F.a3[mb][kb] = ld4_bf16(w16, kOffW3 + (16 * kb + 4 * g) * kH + 16 * mb + c, kH, true);
+        }
+        // A4[o][i] = W4[i][o]: o = c (< 3), i = 16 mb + 4g + j
+        F.a4[mb] = ld4_bf16(w16, kOffW4 + (16 * mb + 4 * g) * 3 + (c < 3 ? c : 0), 3, c < 3);
+        f32x4 b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = b32[kOffB1 + 16 * mb + 4 * g + j];
+        F.b1[mb] = b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = b32[kOffB2 + 16 * mb + 4 * g + j];
+        F.b2[mb] = b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = b32[kOffB3 + 16 * mb + 4 * g + j];
+        F.b3[mb] = b;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) F.b4[j] = (4 * g + j < 3) ? b32[kOffB4 + 4 * g + j] : 0.f;
+}
+
+__device__ __forceinline__ bf16x4 silu_pack(f32x4 z) {
+    bf16x4 h;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) h[j] = static_cast<short>(bf16_rne(silu_f(z[j])));
+    return h;
+}
+
+// One 16-pair tile through the network.  x: this lane's B fragment of the standardised
+// features (rows 4g..4g+3 of column c).  Returns D of the output layer (lanes 0..15 hold
+// outputs 0..2 of pair c).  With Z != nullptr the pre-activations of the three hidden
+// layers are stored as Z[layer][pair][hidden] (pair = col0 + c).
+template <bool STORE>
+__device__ __forceinline__ f32x4 mlp_tile(const Frags& F, bf16x4 x, float* Z, int col0, int lane) {
+    const int g = lane >> 4, c = lane & 15;
+    bf16x4 h[4];
+    f32x4 z[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) z[mb] = mfma16(F.a1[mb], x, F.b1[mb]);
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        if (STORE) *reinterpret_cast<f32x4*>(Z + (col0 + c) * kH + 16 * mb + 4 * g) = z[mb];
+        h[mb] = silu_pack(z[mb]);
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        f32x4 acc = F.b2[mb];
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) acc = mfma16(F.a2[mb][kb], h[kb], acc);
+        z[mb] = acc;
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        if (STORE) *reinterpret_cast<f32x4*>(Z + 64 * kH + (col0 + c) * kH + 16 * mb + 4 * g) = z[mb];
+        h[mb] = silu_pack(z[mb]);
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        f32x4 acc = F.b3[mb];
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) acc = mfma16(F.a3[mb][kb], h[kb], acc);
+        z[mb] = acc;
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        if (STORE) *reinterpret_cast<f32x4*>(Z + 2 * 64 * kH + (col0 + c) * kH + 16 * mb + 4 * g) = z[mb];
+        h[mb] = silu_pack(z[mb]);
+    }
+    f32x4 y = F.b4;
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) y = mfma16(F.a4[kb], h[kb], y);
+    return y;
+}
+
+// ---------------------------------------------------------------------------- inference
+struct InferArgs {
+    const int2* pairs;
+    const DevScalars* sc;
+    uint32_t cap;
+    const float4* nd_com;
+    const float4* nd_aux;
+    const float4* nd_mom;
+    const uint16_t* w16;
+    const float* w32;  // full blob (standardisation first)
+    float eps;
+    float4* corr;
+    DevScalars* scw;
+};
+
+constexpr int kInferWaves = 4;
+
+__global__ __launch_bounds__(64 * kInferWaves) void k_learn_infer(InferArgs A) {
+    __shared__ __attribute__((aligned(16))) uint16_t s_x[kInferWaves][64][16];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = lane_id();
+    const int g = lane >> 4, c = lane & 15;
+    const uint32_t listed = A.sc->learn_pairs;
+    const uint32_t np = min(listed, A.cap);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && listed > A.cap) atomicOr(&A.scw->sticky_learn, 1);
+    const int64_t nchunks = (static_cast<int64_t>(np) + 63) / 64;
+    const int64_t wid = static_cast<int64_t>(blockIdx.x) * kInferWaves + wave;
+    const int64_t nw = static_cast<int64_t>(gridDim.x) * kInferWaves;
+    if (wid >= nchunks) return;
+    Frags F;
+    load_frags(F, A.w16, A.w32 + kNorm, lane);
+    const float* mu = A.w32;
+    const float* isd = A.w32 + 16;
+    uint16_t(*xs)[16] = s_x[wave];
+    for (int64_t ch = wid; ch < nchunks; ch += nw) {
+        const int64_t q = ch * 64 + lane;
+        const bool valid = q < np;
+        int leaf = 0;
+        float scale = 0.f;
+        float f[kF];
+        if (valid) {
+            const int2 pr = A.pairs[q];
+            leaf = pr.x;
+            const float4 lc = A.nd_com[pr.x];
+            const float4 tgt = make_float4(lc.x, lc.y, lc.z, A.nd_aux[pr.x].z);
+            const float4 cm = A.nd_com[pr.y];
+            const float size2 = A.nd_aux[pr.y].y;
+            pair_features(tgt, cm, size2, A.nd_mom + 4 * pr.y, A.eps, f, scale);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kF; ++k) f[k] = 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) xs[lane][k] = (k < kF) ? bf16_rne((f[k] - mu[k]) * isd[k]) : 0;
+        wave_sync();
+        float o0 = 0.f, o1 = 0.f, o2 = 0.f;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const bf16x4 x = *reinterpret_cast<const bf16x4*>(&xs[16 * t + c][4 * g]);
+            const f32x4 y = mlp_tile<false>(F, x, nullptr, 0, lane);
+            // lanes 0..15 hold the outputs of pair 16 t + c; move them to lane 16 t + c
+            const float y0 = __shfl(y[0], lane & 15, 64), y1 = __shfl(y[1], lane & 15, 64),
+                        y2 = __shfl(y[2], lane & 15, 64);
+            if (g == t) {
+                o0 = y0;
+                o1 = y1;
+                o2 = y2;
+            }
+        }
+        if (valid) {
+            float* dst = reinterpret_cast<float*>(A.corr + leaf);
+            atomicAdd(dst + 0, scale * o0);
+            atomicAdd(dst + 1, scale * o1);
+            atomicAdd(dst + 2, scale * o2);
+        }
+        wave_sync();
+    }
+}
+
+__global__ void k_learn_zero(float4* __restrict__ corr, const DevScalars* __restrict__ sc) {
+    const int nn = sc->n_nodes;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x)
+        corr[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ void k_learn_apply(const float4* __restrict__ corr, const int32_t* __restrict__ leaf_of,
+                              const uint32_t* __restrict__ perm, int n, float* __restrict__ ax,
+                              float* __restrict__ ay, float* __restrict__ az) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const float4 cr = corr[leaf_of[p]];
+        const uint32_t o = perm[p];
+        ax[o] += cr.x;
+        ay[o] += cr.y;
+        az[o] += cr.z;
+    }
+}
+
+// ---------------------------------------------------------------------------- samples
+__device__ __forceinline__ double shfl_xor_f64(double v, int m) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(__shfl_xor(hi, m, 64), __shfl_xor(lo, m, 64));
+}
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += shfl_xor_f64(v, o);
+    return v;
+}
+
+struct SampleArgs {
+    const int2* pairs;
+    int64_t npairs;
+    int64_t nnew;
+    const int32_t* nd_start;
+    const int32_t* nd_count;
+    const float4* nd_com;
+    const float4* nd_quad;
+    const float4* nd_aux;
+    const float4* nd_mom;
+    const float4* spos;
+    const uint32_t* perm;
+    const float* amag2;
+    double eps2;
+    float* feat;  // 16 per sample, at the destination offset
+    float* tgt;   // 4 per sample
+};
+
+// One wave per sample: pair k*npairs/nnew of the listed pairs.  The exact field of the
+// cell on each member of the group is an FP64 sum over the cell's particles (self
+// excluded); the multipole field is the walk's monopole + quadrupole formula evaluated
+// in FP64 on the same FP32 moments.
+__global__ __launch_bounds__(256) void k_learn_samples(SampleArgs A) {
+    const int lane = lane_id();
+    const int64_t wid = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = (static_cast<int64_t>(gridDim.x) * blockDim.x) >> 6;
+    for (int64_t k = wid; k < A.nnew; k += nw) {
+        const int64_t q = (A.nnew >= A.npairs) ? k : (k * A.npairs) / A.nnew;
+        const int2 pr = A.pairs[q];
+        const int gst = A.nd_start[pr.x], gcn = A.nd_count[pr.x];
+        const int vst = A.nd_start[pr.y], vcn = A.nd_count[pr.y];
+        const float4 cm = A.nd_com[pr.y];
+        const float4 qd = A.nd_quad[pr.y];
+        const float4 au = A.nd_aux[pr.y];
+        double rx = 0.0, ry = 0.0, rz = 0.0, wa = 0.0;
+        for (int mI = 0; mI < gcn; ++mI) {
+            const float4 pm = A.spos[gst + mI];
+            double ex = 0.0, ey = 0.0, ez = 0.0;
+            for (int j = vst + lane; j < vst + vcn; j += 64) {
+                if (j == gst + mI) continue;
+                const float4 ps = A.spos[j];
+                const double dx = static_cast<double>(ps.x) - pm.x, dy = static_cast<double>(ps.y) - pm.y,
+                             dz = static_cast<double>(ps.z) - pm.z;
+                const double r2 = dx * dx + dy * dy + dz * dz + A.eps2;
+                const double inv = 1.0 / (r2 * sqrt(r2));
+                const double f = static_cast<double>(ps.w) * inv;
+                ex += f * dx;
+                ey += f * dy;
+                ez += f * dz;
+            }
+            ex = wave_sum_f64(ex);
+            ey = wave_sum_f64(ey);
+            ez = wave_sum_f64(ez);
+            // monopole (softened) + quadrupole, as the walk evaluates an accepted cell
+            const double dx = static_cast<double>(cm.x) - pm.x, dy = static_cast<double>(cm.y) - pm.y,
+                         dz = static_cast<double>(cm.z) - pm.z;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            const double inv = 1.0 / sqrt(d2 + A.eps2);
+            const double f = static_cast<double>(cm.w) * inv * inv * inv;
+            const double qxx = qd.x, qyy = qd.y, qxy = qd.z, qxz = qd.w, qyz = au.x, qzz = -(qxx + qyy);
+            const double qdx = qxx * dx + qxy * dy + qxz * dz;
+            const double qdy = qxy * dx + qyy * dy + qyz * dz;
+            const double qdz = qxz * dx + qyz * dy + qzz * dz;
+            const double qd2 = dx * qdx + dy * qdy + dz * qdz;
+            const double ir2 = 1.0 / d2, ir5 = ir2 * ir2 * sqrt(ir2);
+            const double c2 = 2.5 * qd2 * ir5 * ir2;
+            const double mx = f * dx + c2 * dx - ir5 * qdx;
+            const double my = f * dy + c2 * dy - ir5 * qdy;
+            const double mz = f * dz + c2 * dz - ir5 * qdz;
+            rx += ex - mx;
+            ry += ey - my;
+            rz += ez - mz;
+            const double a2 = A.amag2[A.perm[gst + mI]];
+            wa += (a2 > 0.0) ? 1.0 / a2 : 0.0;
+        }
+        const float4 lc = A.nd_com[pr.x];
+        const float4 tg = make_float4(lc.x, lc.y, lc.z, A.nd_aux[pr.x].z);
+        float f[kF], scale;
+        pair_features(tg, cm, au.y, A.nd_mom + 4 * pr.y, static_cast<float>(sqrt(A.eps2)), f, scale);
+        const double inv_n = gcn > 0 ? 1.0 / gcn : 0.0;
+        // entries 13 and 14 carry the pair's node ids (int32 bits) for inspection
+        if (lane < 16)
+            A.feat[16 * k + lane] = (lane < kF)     ? f[lane]
+                                    : (lane == 13) ? __int_as_float(pr.x)
+                                    : (lane == 14) ? __int_as_float(pr.y)
+                                                   : 0.f;
+        if (lane == 0) {
+            const double is = 1.0 / scale;
+            A.tgt[4 * k + 0] = static_cast<float>(rx * inv_n * is);
+            A.tgt[4 * k + 1] = static_cast<float>(ry * inv_n * is);
+            A.tgt[4 * k + 2] = static_cast<float>(rz * inv_n * is);
+            A.tgt[4 * k + 3] = static_cast<float>(scale * sqrt(wa));
+        }
+    }
+}
+
+__global__ void k_learn_amag2(const float* __restrict__ ax, const float* __restrict__ ay,
+                              const float* __restrict__ az, int n, float* __restrict__ out) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i];
+}
+
+// Pool statistics: per feature sum and sum of squares (FP64), and the weight sum.
+__global__ __launch_bounds__(256) void k_learn_stats(const float* __restrict__ feat, const float* __restrict__ tgt,
+                                                     int64_t n, double* __restrict__ out) {
+    __shared__ double s[256][2];
+    const int k = threadIdx.x & 15;   // feature of this thread
+    const int r = threadIdx.x >> 4;   // row offset
+    double a = 0.0, b = 0.0, w = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 16 + r; i < n; i += static_cast<int64_t>(gridDim.x) * 16) {
+        const double v = feat[16 * i + k];
+        a += v;
+        b += v * v;
+        if (k == 0) w += tgt[4 * i + 3];
+    }
+    s[threadIdx.x][0] = a;
+    s[threadIdx.x][1] = b;
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        double sa = 0.0, sb = 0.0;
+        for (int j = threadIdx.x; j < 256; j += 16) {
+            sa += s[j][0];
+            sb += s[j][1];
+        }
+        atomicAdd(out + threadIdx.x, sa);
+        atomicAdd(out + 16 + threadIdx.x, sb);
+    }
+    __syncthreads();
+    s[threadIdx.x][0] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sw = 0.0;
+        for (int j = 0; j < 256; ++j) sw += s[j][0];
+        atomicAdd(out + 32, sw);
+    }
+}
+
+// ---------------------------------------------------------------------------- training
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+struct TrainArgs {
+    const float* feat;  // 16 per sample
+    const float* tgt;   // 4 per sample
+    int64_t n_src;      // samples available
+    int use_hash;       // 1: sample i of block b = hash(seed, step, b*64+i) mod n_src; 0: row b*64+i
+    uint64_t seed;
+    int step;
+    float inv_wmean;    // loss weight multiplier
+    float inv_n;        // 1 / samples in the batch
+    const uint16_t* w16;
+    const float* w32;   // full blob
+    float* grad_part;   // per block, kNTrain
+    float* loss_part;   // per block
+    float* out;         // optional predictions (n x 3, direct rows only)
+};
+
+// One wave per block, 64 samples: forward with the pre-activations kept in LDS, then the
+// backward pass; weight gradients are matrix-core products over the block's samples.
+__global__ __launch_bounds__(64) void k_learn_fwdbwd(TrainArgs A) {
+    __shared__ __attribute__((aligned(16))) uint16_t s_x[64][16];
+    __shared__ __attribute__((aligned(16))) float s_z[3][64][kH];
+    __shared__ __attribute__((aligned(16))) uint16_t s_d[64][kH];
+    __shared__ __attribute__((aligned(16))) float s_y[64][4];
+    const int lane = lane_id();
+    const int g = lane >> 4, c = lane & 15;
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * 64 + lane;
+    int64_t src = row;
+    bool valid = true;
+    if (A.use_hash) {
+        src = static_cast<int64_t>(mix64(A.seed ^ mix64(static_cast<uint64_t>(A.step) * 0x100000001b3ull +
+                                                        static_cast<uint64_t>(row))) %
+                                   static_cast<uint64_t>(A.n_src));
+    } else {
+        valid = row < A.n_src;
+        if (!valid) src = 0;
+    }
+    const float* mu = A.w32;
+    const float* isd = A.w32 + 16;
+    const float* b32 = A.w32 + kNorm;
+    // inputs
+    const float4* fp = reinterpret_cast<const float4*>(A.feat + 16 * src);
+    float fin[16];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float4 v = fp[k];
+        fin[4 * k] = v.x;
+        fin[4 * k + 1] = v.y;
+        fin[4 * k + 2] = v.z;
+        fin[4 * k + 3] = v.w;
+    }
+    const float4 tg = reinterpret_cast<const float4*>(A.tgt)[src];
+    const float wgt = valid ? tg.w * A.inv_wmean : 0.f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s_x[lane][k] = (k < kF) ? bf16_rne((fin[k] - mu[k]) * isd[k]) : 0;
+    __syncthreads();
+    // forward
+    {
+        Frags F;
+        load_frags(F, A.w16, b32, lane);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const bf16x4 x = *reinterpret_cast<const bf16x4*>(&s_x[16 * t + c][4 * g]);
+            const f32x4 y = mlp_tile<true>(F, x, &s_z[0][0][0], 16 * t, lane);
+            if (g == 0) {
+                s_y[16 * t + c][0] = y[0];
+                s_y[16 * t + c][1] = y[1];
+                s_y[16 * t + c][2] = y[2];
+            }
+        }
+    }
+    __syncthreads();
+    // loss and dL/dy (lane = sample)
+    float dy0, dy1, dy2;
+    {
+        const float e0 = s_y[lane][0] - tg.x, e1 = s_y[lane][1] - tg.y, e2 = s_y[lane][2] - tg.z;
+        const float l = wgt * (e0 * e0 + e1 * e1 + e2 * e2);
+        const float ls = wave_sum(l);
+        if (lane == 0) A.loss_part[blockIdx.x] = ls;
+        const float k2 = 2.f * wgt * A.inv_n;
+        dy0 = k2 * e0;
+        dy1 = k2 * e1;
+        dy2 = k2 * e2;
+        if (A.out && valid) {
+            A.out[3 * row + 0] = s_y[lane][0];
+            A.out[3 * row + 1] = s_y[lane][1];
+            A.out[3 * row + 2] = s_y[lane][2];
+        }
+    }
+    __syncthreads();
+    s_y[lane][0] = dy0;
+    s_y[lane][1] = dy1;
+    s_y[lane][2] = dy2;
+    __syncthreads();
+    float* gp = A.grad_part + static_cast<int64_t>(blockIdx.x) * kNTrain;
+    // output layer: dW4[i][o] = sum_n h3[n][i] dy[n][o] (lane = i), db4, then dZ3 (lane = n)
+    {
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+        for (int n = 0; n < 64; ++n) {
+            const float h = bf16_f32(bf16_rne(silu_f(s_z[2][n][lane])));
+            a0 += h * s_y[n][0];
+            a1 += h * s_y[n][1];
+            a2 += h * s_y[n][2];
+        }
+        gp[kOffW4 + 3 * lane + 0] = a0;
+        gp[kOffW4 + 3 * lane + 1] = a1;
+        gp[kOffW4 + 3 * lane + 2] = a2;
+        if (lane < 3) {
+            float s = 0.f;
+            for (int n = 0; n < 64; ++n) s += s_y[n][lane];
+            gp[kOffB4 + lane] = s;
+        }
+        const uint16_t* w4 = A.w16 + kOffW4;  // the BF16 W4[i][o] the forward used
+        for (int i = 0; i < kH; ++i) {
+            const float dh = dy0 * bf16_f32(w4[3 * i]) + dy1 * bf16_f32(w4[3 * i + 1]) + dy2 * bf16_f32(w4[3 * i + 2]);
+            s_d[lane][i] = bf16_rne(dh * dsilu_f(s_z[2][lane][i]));
+        }
+    }
+    __syncthreads();
+    // hidden layers, L = 3, 2: dW_L from (h_{L-1}, dZ_L), then dZ_{L-1}
+    // dZ3 lives in s_d; dZ2 is written over z3 (s_z[2], dead); dZ1 over s_d.
+    uint16_t* d3 = &s_d[0][0];
+    uint16_t* d2 = reinterpret_cast<uint16_t*>(&s_z[2][0][0]);
+#pragma unroll
+    for (int L = 3; L >= 2; --L) {
+        const uint16_t* dcur = (L == 3) ? d3 : d2;
+        uint16_t* dnext = (L == 3) ? d2 : d3;
+        const float* zin = &s_z[L - 2][0][0];  // pre-activations of the layer's input
+        const int offW = (L == 3) ? kOffW3 : kOffW2;
+        const int offB = (L == 3) ? kOffB3 : kOffB2;
+        // dW[i][h] = sum_n h_in[n][i] dZ[n][h]: D rows i, cols h, K = samples
+        {
+            bf16x4 Ain[4][4];  // [ib][nb]: A[i = 16 ib + c][n = 16 nb + 4g + j]
+            bf16x4 Bd[4][4];   // [nb][hb]: B[n = 16 nb + 4g + j][h = 16 hb + c]
+#pragma unroll
+            for (int ib = 0; ib < 4; ++ib)
+#pragma unroll
+                for (int nb = 0; nb < 4; ++nb) {
+                    bf16x4 v;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        v[j] = static_cast<short>(bf16_rne(silu_f(zin[(16 * nb + 4 * g + j) * kH + 16 * ib + c])));
+                    Ain[ib][nb] = v;
+                }
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+                for (int hb = 0; hb < 4; ++hb) {
+                    bf16x4 v;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) v[j] = static_cast<short>(dcur[(16 * nb + 4 * g + j) * kH + 16 * hb + c]);
+                    Bd[nb][hb] = v;
+                }
+#pragma unroll
+            for (int ib = 0; ib < 4; ++ib)
+#pragma unroll
+                for (int hb = 0; hb < 4; ++hb) {
+                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int nb = 0; nb < 4; ++nb) acc = mfma16(Ain[ib][nb], Bd[nb][hb], acc);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) gp[offW + (16 * ib + 4 * g + j) * kH + 16 * hb + c] = acc[j];
+                }
+        }
+        // db[h] = sum_n dZ[n][h] (lane = h)
+        {
+            float s = 0.f;
+            for (int n = 0; n < 64; ++n) s += bf16_f32(dcur[n * kH + lane]);
+            gp[offB + lane] = s;
+        }
+        // dH_in^T[i][n] = sum_h W[i][h] dZ^T[h][n]; dZ_in = dH_in * silu'(z_in)
+        {
+            f32x4 acc[4][4];  // [ib][nb]
+#pragma unroll
+            for (int ib = 0; ib < 4; ++ib)
+#pragma unroll
+                for (int nb = 0; nb < 4; ++nb) acc[ib][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) {
+                bf16x4 Bt[4];
+#pragma unroll
+                for (int nb = 0; nb < 4; ++nb)
+                    Bt[nb] = *reinterpret_cast<const bf16x4*>(dcur + (16 * nb + c) * kH + 16 * kb + 4 * g);
+#pragma unroll
+                for (int ib = 0; ib < 4; ++ib) {
+                    const bf16x4 a =
+                        *reinterpret_cast<const bf16x4*>(A.w16 + offW + (16 * ib + c) * kH + 16 * kb + 4 * g);
+#pragma unroll
+                    for (int nb = 0; nb < 4; ++nb) acc[ib][nb] = mfma16(a, Bt[nb], acc[ib][nb]);
+                }
+            }
+            __syncthreads();  // dcur and z_out fully read before dnext (may alias z_out) is written
+#pragma unroll
+            for (int ib = 0; ib < 4; ++ib)
+#pragma unroll
+                for (int nb = 0; nb < 4; ++nb) {
+                    const int n = 16 * nb + c;
+                    bf16x4 v;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int i = 16 * ib + 4 * g + j;
+                        v[j] = static_cast<short>(bf16_rne(acc[ib][nb][j] * dsilu_f(zin[n * kH + i])));
+                    }
+                    *reinterpret_cast<bf16x4*>(dnext + n * kH + 16 * ib + 4 * g) = v;
+                }
+        }
+        __syncthreads();
+    }
+    // first layer: dW1[f][h] = sum_n x[n][f] dZ1[n][h]; db1
+    {
+        const uint16_t* d1 = d3;
+        bf16x4 Ax[4];  // [nb]: A[f = c][n = 16 nb + 4g + j]
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            bf16x4 v;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[j] = static_cast<short>(s_x[16 * nb + 4 * g + j][c]);
+            Ax[nb] = v;
+        }
+#pragma unroll
+        for (int hb = 0; hb < 4; ++hb) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                bf16x4 v;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) v[j] = static_cast<short>(d1[(16 * nb + 4 * g + j) * kH + 16 * hb + c]);
+                acc = mfma16(Ax[nb], v, acc);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int f = 4 * g + j;
+                if (f < kF) gp[kOffW1 + f * kH + 16 * hb + c] = acc[j];
+            }
+        }
+        float s = 0.f;
+        for (int n = 0; n < 64; ++n) s += bf16_f32(d1[n * kH + lane]);
+        gp[kOffB1 + lane] = s;
+    }
+}
+
+// Sum of the block partials; Adam update of the FP32 master and its BF16 copy.
+__global__ __launch_bounds__(256) void k_learn_adam(const float* __restrict__ grad_part, int nblocks,
+                                                    const float* __restrict__ loss_part, float* __restrict__ loss_slot,
+                                                    float* __restrict__ w32, uint16_t* __restrict__ w16,
+                                                    float* __restrict__ m, float* __restrict__ v, float lr, float bc1,
+                                                    float bc2) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0 && loss_slot) {
+        float s = 0.f;
+        for (int b = 0; b < nblocks; ++b) s += loss_part[b];
+        *loss_slot = s;
+    }
+    if (p >= kNTrain) return;
+    float gsum = 0.f;
+    for (int b = 0; b < nblocks; ++b) gsum += grad_part[static_cast<int64_t>(b) * kNTrain + p];
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float mm = b1 * m[p] + (1.f - b1) * gsum;
+    const float vv = b2 * v[p] + (1.f - b2) * gsum * gsum;
+    m[p] = mm;
+    v[p] = vv;
+    const float w = w32[kNorm + p] - lr * (mm * bc1) / (sqrtf(vv * bc2) + 1e-8f);
+    w32[kNorm + p] = w;
+    w16[p] = bf16_rne(w);
+}
+
+__global__ __launch_bounds__(256) void k_learn_reduce(const float* __restrict__ grad_part, int nblocks,
+                                                      const float* __restrict__ loss_part, float* __restrict__ grad,
+                                                      float* __restrict__ loss) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) {
+        float s = 0.f;
+        for (int b = 0; b < nblocks; ++b) s += loss_part[b];
+        *loss = s;
+    }
+    if (p >= kNTrain) return;
+    float gsum = 0.f;
+    for (int b = 0; b < nblocks; ++b) gsum += grad_part[static_cast<int64_t>(b) * kNTrain + p];
+    grad[p] = gsum;
+}
+
+// ---------------------------------------------------------------------------- launchers
+namespace {
+template <typename T>
+int lalloc(nbg_ctx* c, T*& p, size_t count) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    hipError_t e;
+    p = dev_alloc<T>(count, e);
+    if (!p) {
+        c->set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        return NBG_E_OOM;
+    }
+    return NBG_OK;
+}
+}  // namespace
+
+int learn_alloc(nbg_ctx* c) {
+    if (c->nd_mom) return NBG_OK;
+    const size_t C = static_cast<size_t>(c->cap_nodes);
+    int r = NBG_OK;
+    r |= lalloc(c, c->nd_mom, 4 * C);
+    r |= lalloc(c, c->lcorr, C);
+    r |= lalloc(c, c->amag2, static_cast<size_t>(c->cap_n));
+    if (r != NBG_OK) return r;
+    if (!c->lpairs) return learn_grow_pairs(c, std::max<int64_t>(int64_t{1} << 20, 32 * c->cap_n));
+    return NBG_OK;
+}
+
+int learn_grow_pairs(nbg_ctx* c, int64_t need) {
+    if (need <= c->lpairs_cap && c->lpairs) return NBG_OK;
+    const int64_t cap = std::min<int64_t>(need + need / 4 + 1024, int64_t{0xffffffffll});
+    if (int r = lalloc(c, c->lpairs, static_cast<size_t>(cap))) {
+        c->lpairs_cap = 0;
+        return r;
+    }
+    c->lpairs_cap = cap;
+    return NBG_OK;
+}
+
+void launch_learn_moments(nbg_ctx* c, int levels) {
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(k_mom_leaf, dim3(2048), dim3(256), 0, st, c->leaf_list, c->nd_start, c->nd_count, c->spos,
+                       c->nd_com, c->dsc, c->nd_mom);
+    const int top = (levels == kLevelsShort) ? kLevelsShort - 1 : kMaxDepth - 1;
+    for (int d = top; d >= 0; --d)
+        hipLaunchKernelGGL(k_mom_internal, dim3(d < 6 ? 64 : 1024), dim3(256), 0, st, d, c->depth_nodes,
+                           c->nd_child, c->nd_com, c->dsc, c->nd_mom);
+}
+
+void launch_learn_zero(nbg_ctx* c) {
+    hipLaunchKernelGGL(k_learn_zero, dim3(1024), dim3(256), 0, c->stream, c->lcorr, c->dsc);
+}
+
+void launch_learn_correct(nbg_ctx* c, const nbg_params& p) {
+    InferArgs A;
+    A.pairs = c->lpairs;
+    A.sc = c->dsc;
+    A.cap = static_cast<uint32_t>(c->lpairs_cap);
+    A.nd_com = c->nd_com;
+    A.nd_aux = c->nd_aux;
+    A.nd_mom = c->nd_mom;
+    A.w16 = c->lw16;
+    A.w32 = c->lw32;
+    A.eps = static_cast<float>(std::sqrt(p.eps2));
+    A.corr = c->lcorr;
+    A.scw = c->dsc;
+    hipLaunchKernelGGL(k_learn_infer, dim3(256 * 8), dim3(64 * kInferWaves), 0, c->stream, A);
+    hipLaunchKernelGGL(k_learn_apply, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->lcorr, c->leaf_of,
+                       c->perm, c->n, c->ax, c->ay, c->az);
+}
+
+void launch_learn_amag2(nbg_ctx* c) {
+    hipLaunchKernelGGL(k_learn_amag2, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->ax, c->ay, c->az, c->n,
+                       c->amag2);
+}
+
+void launch_learn_samples(nbg_ctx* c, const nbg_params& p, int64_t npairs, int64_t nnew, int64_t dst) {
+    SampleArgs A;
+    A.pairs = c->lpairs;
+    A.npairs = npairs;
+    A.nnew = nnew;
+    A.nd_start = c->nd_start;
+    A.nd_count = c->nd_count;
+    A.nd_com = c->nd_com;
+    A.nd_quad = c->nd_quad;
+    A.nd_aux = c->nd_aux;
+    A.nd_mom = c->nd_mom;
+    A.spos = c->spos;
+    A.perm = c->perm;
+    A.amag2 = c->amag2;
+    A.eps2 = p.eps2;
+    A.feat = c->pool_feat + 16 * dst;
+    A.tgt = c->pool_tgt + 4 * dst;
+    hipLaunchKernelGGL(k_learn_samples, dim3(4096), dim3(256), 0, c->stream, A);
+}
+
+void launch_learn_stats(nbg_ctx* c, int64_t count, double* out) {
+    hipLaunchKernelGGL(k_learn_stats, dim3(1024), dim3(256), 0, c->stream, c->pool_feat, c->pool_tgt, count, out);
+}
+
+void launch_learn_fwdbwd(nbg_ctx* c, const float* feat, const float* tgt, int64_t n_src, int nblocks, uint64_t seed,
+                         int step, int use_hash, float inv_wmean, float inv_n, float* out) {
+    TrainArgs A;
+    A.feat = feat;
+    A.tgt = tgt;
+    A.n_src = n_src;
+    A.use_hash = use_hash;
+    A.seed = seed;
+    A.step = step;
+    A.inv_wmean = inv_wmean;
+    A.inv_n = inv_n;
+    A.w16 = c->lw16;
+    A.w32 = c->lw32;
+    A.grad_part = c->grad_part;
+    A.loss_part = c->loss_part;
+    A.out = out;
+    hipLaunchKernelGGL(k_learn_fwdbwd, dim3(nblocks), dim3(64), 0, c->stream, A);
+}
+
+void launch_learn_adam(nbg_ctx* c, int nblocks, float lr, int64_t t, float* loss_slot) {
+    const double b1 = 0.9, b2 = 0.999;
+    const float bc1 = static_cast<float>(1.0 / (1.0 - std::pow(b1, static_cast<double>(t))));
+    const float bc2 = static_cast<float>(1.0 / (1.0 - std::pow(b2, static_cast<double>(t))));
+    hipLaunchKernelGGL(k_learn_adam, dim3(grid_for(kNTrain, 256)), dim3(256), 0, c->stream, c->grad_part, nblocks,
+                       c->loss_part, loss_slot, c->lw32, c->lw16, c->adam_m, c->adam_v, lr, bc1, bc2);
+}
+
+void launch_learn_reduce(nbg_ctx* c, int nblocks, float* grad_out, float* loss_out) {
+    hipLaunchKernelGGL(k_learn_reduce, dim3(grid_for(kNTrain, 256)), dim3(256), 0, c->stream, c->grad_part, nblocks,
+                       c->loss_part, grad_out, loss_out);
+}
+
+}  // namespace nbg

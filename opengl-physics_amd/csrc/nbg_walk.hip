@@ -387,6 +387,10 @@ struct BinArgs {
     float eps2;
     float inv_theta;
     double theta2_64;
+    // learned correction: accepted pairs with size / (dist - R) > band are listed
+    float inv_band;
+    int2* lpairs;
+    uint32_t lcap;
 };
 
 __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
@@ -408,7 +412,7 @@ __device__ __forceinline__ int wave_excl_small(int v, int* total) {
     return off;
 }
 
-template <bool QUAD, bool KAHAN, bool COUNT>
+template <bool QUAD, bool KAHAN, bool COUNT, bool LEARN>
 __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     __shared__ int s_snode[kBinWaves][kBinStack];
     __shared__ uint64_t s_smask[kBinWaves][kBinStack];
@@ -487,6 +491,10 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             smask[0] = all;
         }
         wave_sync();
+        // leaves whose first particle lies in this bin: a leaf that starts in the previous
+        // bin gets its pairs listed there (its MAC decisions are the same in both bins)
+        uint64_t own_m = all;
+        if (LEARN && nl > 0 && A.nd_start[lid[0]] < start) own_m &= ~1ull;
 
         float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
         int n_far = 0, n_near = 0;
@@ -625,7 +633,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             }
             sp -= k;
             wave_sync();
-            uint64_t far_m = 0, near_m = 0, open_m = 0;
+            uint64_t far_m = 0, near_m = 0, open_m = 0, band_m = 0;
             int lstart = 0, lcount = 0;
             if (node >= 0) {
                 const float4 cm = A.nd_com[node];
@@ -644,6 +652,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                         near_m = act;
                     } else {
                         const float sth = sqrtf(au.y) * inv_theta;  // size / theta
+                        const float sbd = LEARN ? sqrtf(au.y) * A.inv_band : 0.f;
                         const float cscale = fabsf(cm.x) + fabsf(cm.y) + fabsf(cm.z);
                         for (uint64_t todo = act; todo; todo &= todo - 1) {
                             const int jj = __ffsll(static_cast<long long>(todo)) - 1;
@@ -671,10 +680,15 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                                 accept = (w64 > 0.0) &&
                                          (__dmul_rn(size, size) < __dmul_rn(__dmul_rn(A.theta2_64, w64), w64));
                             }
-                            if (accept)
+                            if (accept) {
                                 far_m |= bit;
-                            else
+                                if (LEARN) {
+                                    const float qb = lc.w + sbd;
+                                    if (dist2 <= qb * qb) band_m |= bit;
+                                }
+                            } else {
                                 open_m |= bit;
+                            }
                         }
                     }
                 }
@@ -689,6 +703,24 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                     fmask[slot] = far_m;
                 }
                 f_tail += __popcll(bm);
+            }
+            // learned correction: list (leaf, cell) for the band pairs of leaves owned here
+            if (LEARN) {
+                const uint64_t em = band_m & own_m;
+                const int ne = __popcll(em);
+                const int incl = wave_incl_scan(ne);
+                const int tot = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+                if (tot > 0) {
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(&A.sc->learn_pairs, static_cast<uint32_t>(tot));
+                    base = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(base)));
+                    uint32_t w = base + static_cast<uint32_t>(incl - ne);
+                    for (uint64_t t = em; t; t &= t - 1) {
+                        const int jj = __ffsll(static_cast<long long>(t)) - 1;
+                        if (w < A.lcap) A.lpairs[w] = make_int2(lid[jj], node);
+                        ++w;
+                    }
+                }
             }
             // near leaves -> ring
             {
@@ -834,20 +866,31 @@ void launch_walk_bins(nbg_ctx* c, const nbg_params& p) {
     const bool quad = p.multipole == NBG_QUADRUPOLE;
     const bool kahan = (p.flags & NBG_FLAG_KAHAN) != 0;
     const bool count = (p.flags & NBG_FLAG_COUNT) != 0 && c->cnt_far != nullptr;
+    const bool learn = quad && (p.flags & NBG_FLAG_LEARNED) != 0 && c->lpairs != nullptr;
+    A.inv_band = (p.learn_band > 0.0) ? static_cast<float>(1.0 / p.learn_band) : 1e30f;
+    A.lpairs = c->lpairs;
+    A.lcap = static_cast<uint32_t>(c->lpairs_cap);
     // 8 XCDs x 32 CUs x 3 resident blocks
     const dim3 grid(8 * 32 * 3), block(64 * kBinWaves);
-#define NBG_WALKB(Q, K, C) hipLaunchKernelGGL((k_walk_bins<Q, K, C>), grid, block, 0, c->stream, A)
-    if (quad) {
+#define NBG_WALKB(Q, K, C, L) hipLaunchKernelGGL((k_walk_bins<Q, K, C, L>), grid, block, 0, c->stream, A)
+    if (learn) {
+        // the learned correction is defined against the quadrupole expansion
         if (kahan) {
-            if (count) NBG_WALKB(true, true, true); else NBG_WALKB(true, true, false);
+            if (count) NBG_WALKB(true, true, true, true); else NBG_WALKB(true, true, false, true);
         } else {
-            if (count) NBG_WALKB(true, false, true); else NBG_WALKB(true, false, false);
+            if (count) NBG_WALKB(true, false, true, true); else NBG_WALKB(true, false, false, true);
+        }
+    } else if (quad) {
+        if (kahan) {
+            if (count) NBG_WALKB(true, true, true, false); else NBG_WALKB(true, true, false, false);
+        } else {
+            if (count) NBG_WALKB(true, false, true, false); else NBG_WALKB(true, false, false, false);
         }
     } else {
         if (kahan) {
-            if (count) NBG_WALKB(false, true, true); else NBG_WALKB(false, true, false);
+            if (count) NBG_WALKB(false, true, true, false); else NBG_WALKB(false, true, false, false);
         } else {
-            if (count) NBG_WALKB(false, false, true); else NBG_WALKB(false, false, false);
+            if (count) NBG_WALKB(false, false, true, false); else NBG_WALKB(false, false, false, false);
         }
     }
 #undef NBG_WALKB

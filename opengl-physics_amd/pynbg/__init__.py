@@ -28,13 +28,19 @@ ERRORS = {0: "NBG_OK", -1: "NBG_E_INVALID", -2: "NBG_E_OOM", -3: "NBG_E_CAPACITY
 
 EXPORTED = ["nbg_create", "nbg_destroy", "nbg_last_error", "nbg_upload", "nbg_upload_batch", "nbg_upload_accel",
             "nbg_accel", "nbg_step", "nbg_download", "nbg_tree_export", "nbg_interaction_counts", "nbg_diagnostics",
-            "nbg_accel_direct_targets"]
+            "nbg_accel_direct_targets", "nbg_learn_set_weights", "nbg_learn_get_weights", "nbg_learn_collect",
+            "nbg_learn_clear", "nbg_learn_pool", "nbg_learn_standardize", "nbg_learn_train", "nbg_learn_grad"]
+
+LEARN_FEATURES = 13
+LEARN_HIDDEN = 64
+LEARN_NTRAIN = 9411
+LEARN_NPARAM = 32 + LEARN_NTRAIN
 
 
 class Params(ctypes.Structure):
     _fields_ = [("solver", ctypes.c_int), ("multipole", ctypes.c_int), ("G", ctypes.c_double),
                 ("eps2", ctypes.c_double), ("theta", ctypes.c_double), ("dt", ctypes.c_double),
-                ("flags", ctypes.c_uint32)]
+                ("flags", ctypes.c_uint32), ("learn_band", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -42,7 +48,7 @@ class Stats(ctypes.Structure):
                 ("deep_keys", ctypes.c_int32), ("n_p2p", ctypes.c_int64), ("n_m2p", ctypes.c_int64),
                 ("ms_keys", ctypes.c_double), ("ms_sort", ctypes.c_double), ("ms_build", ctypes.c_double),
                 ("ms_upsweep", ctypes.c_double), ("ms_walk", ctypes.c_double), ("ms_integrate", ctypes.c_double),
-                ("ms_total", ctypes.c_double)]
+                ("ms_total", ctypes.c_double), ("n_learned", ctypes.c_int64), ("ms_learned", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -79,6 +85,14 @@ def load():
     L.nbg_interaction_counts.argtypes = [ctypes.c_void_p, ctypes.c_int64, _i64, _i64]
     L.nbg_diagnostics.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_int32, _d]
     L.nbg_accel_direct_targets.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_int64, _i64] + [_f] * 3
+    L.nbg_learn_set_weights.argtypes = [ctypes.c_void_p, _f, ctypes.c_int64]
+    L.nbg_learn_get_weights.argtypes = [ctypes.c_void_p, _f, ctypes.c_int64]
+    L.nbg_learn_collect.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_int64, _i64]
+    L.nbg_learn_clear.argtypes = [ctypes.c_void_p]
+    L.nbg_learn_pool.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, _f, _f]
+    L.nbg_learn_standardize.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+    L.nbg_learn_train.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_uint64, _f]
+    L.nbg_learn_grad.argtypes = [ctypes.c_void_p, ctypes.c_int64, _f, _f, _f, _f, _f]
     L.nbg_destroy.restype = None
     for name in EXPORTED:
         if name not in ("nbg_create", "nbg_destroy", "nbg_last_error"):
@@ -100,14 +114,14 @@ class NbgError(RuntimeError):
 
 
 def make_params(solver="barnes_hut", multipole="quadrupole", G=1.0, eps=None, eps2=None, theta=0.5, dt=1e-3,
-                kahan=True, count=False, no_graph=False, learned=False, stage_times=False):
+                kahan=True, count=False, no_graph=False, learned=False, stage_times=False, learn_band=0.0):
     if eps2 is None:
         eps2 = (eps if eps is not None else 0.05) ** 2
     flags = (FLAG_KAHAN if kahan else 0) | (FLAG_COUNT if count else 0) | (FLAG_NO_GRAPH if no_graph else 0) | \
         (FLAG_LEARNED if learned else 0) | (FLAG_STAGE_TIMES if stage_times else 0)
     return Params(SOLVER_DIRECT if solver == "direct" else SOLVER_BARNES_HUT,
                   QUADRUPOLE if multipole == "quadrupole" else MONOPOLE, float(G), float(eps2), float(theta),
-                  float(dt), flags)
+                  float(dt), flags, float(learn_band))
 
 
 class Context:
@@ -219,6 +233,52 @@ class Context:
         self._check(self._L.nbg_accel_direct_targets(self._h, ctypes.byref(params), len(t), _p(t, _i64),
                                                      _p(ax, _f), _p(ay, _f), _p(az, _f)), "nbg_accel_direct_targets")
         return np.stack([ax, ay, az], axis=1)
+
+    # ------------------------------------------------------------------ learned correction
+    def learn_set_weights(self, blob):
+        b = np.ascontiguousarray(blob, dtype=np.float32)
+        self._check(self._L.nbg_learn_set_weights(self._h, _p(b, _f), len(b)), "nbg_learn_set_weights")
+
+    def learn_get_weights(self):
+        b = np.empty(LEARN_NPARAM, np.float32)
+        self._check(self._L.nbg_learn_get_weights(self._h, _p(b, _f), len(b)), "nbg_learn_get_weights")
+        return b
+
+    def learn_collect(self, params, max_new=0):
+        n = ctypes.c_int64(0)
+        self._check(self._L.nbg_learn_collect(self._h, ctypes.byref(params), int(max_new), ctypes.byref(n)),
+                    "nbg_learn_collect")
+        return n.value
+
+    def learn_clear(self):
+        self._check(self._L.nbg_learn_clear(self._h), "nbg_learn_clear")
+
+    def learn_pool(self, offset, count):
+        f = np.empty((count, 16), np.float32)
+        t = np.empty((count, 4), np.float32)
+        self._check(self._L.nbg_learn_pool(self._h, int(offset), int(count), _p(f, _f), _p(t, _f)),
+                    "nbg_learn_pool")
+        return f, t
+
+    def learn_standardize(self, count=0):
+        self._check(self._L.nbg_learn_standardize(self._h, int(count)), "nbg_learn_standardize")
+
+    def learn_train(self, steps, batch, lr, seed=0):
+        loss = np.empty(max(1, steps), np.float32)
+        self._check(self._L.nbg_learn_train(self._h, int(steps), int(batch), float(lr), int(seed), _p(loss, _f)),
+                    "nbg_learn_train")
+        return loss[:steps]
+
+    def learn_grad(self, feats, targets):
+        f = np.ascontiguousarray(feats, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        n = len(f)
+        out = np.empty((n, 3), np.float32)
+        grad = np.empty(LEARN_NTRAIN, np.float32)
+        loss = ctypes.c_float(0)
+        self._check(self._L.nbg_learn_grad(self._h, n, _p(f, _f), _p(t, _f), _p(out, _f), ctypes.byref(loss),
+                                           _p(grad, _f)), "nbg_learn_grad")
+        return out, float(loss.value), grad
 
 
 # ---------------------------------------------------------------------------- host library

@@ -1,0 +1,271 @@
+"""GPU checks of the learned far-field correction against the oracle.
+
+* the walk's (leaf, cell) pair list equals the oracle's per-leaf far lists
+  (CollectGroupInteractions, projects/nbody_core/src/Solvers.cpp:248-277) -- bit-exact;
+* sample generation (moments, features, FP64 residuals) against oracle/learned.py;
+* forward/backward of the BF16 matrix-core kernels against the FP64 oracle MLP;
+* training lowers the loss; the trained model lowers the force error of theta = 0.8;
+* graph-replayed and eager learned steps agree.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import f64, plummer_f32
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+WEIGHTS = os.path.join(ROOT, "opengl-physics_amd", "learned", "farfield_mlp.npy")
+
+
+def _ids(feats):
+    return feats[:, 13].view(np.int32).copy(), feats[:, 14].view(np.int32).copy()
+
+
+def _subtree_ranges(e):
+    """Sorted-order particle range [first, first+count) of every node (pre-order ids)."""
+    nn = len(e["parent"])
+    leaf = e["child_count"] == 0
+    first = np.zeros(nn, np.int64)
+    cnt = np.zeros(nn, np.int64)
+    cur = -1
+    for v in range(nn - 1, -1, -1):
+        if leaf[v]:
+            cnt[v] += e["count"][v]
+            cur = v
+        first[v] = e["first"][cur]
+        if e["parent"][v] >= 0:
+            cnt[e["parent"][v]] += cnt[v]
+    return first, cnt
+
+
+def _blob_for_lists():
+    from pynbg import learned
+    return learned.init_blob(0)
+
+
+def test_pair_list_equals_oracle_far_lists(gpu_ctx_factory):
+    import oracle
+    import pynbg
+    s = plummer_f32(4096, 2)
+    x, y, z, m = f64(s)
+    ctx = gpu_ctx_factory(4096)
+    ctx.upload(s["x"], s["y"], s["z"], s["m"])
+    p = pynbg.make_params("barnes_hut", "quadrupole", G=1.0, eps=0.005, theta=0.8, learn_band=0.0)
+    ctx.learn_clear()
+    n = ctx.learn_collect(p)
+    feats, _ = ctx.learn_pool(0, n)
+    gl, gc = _ids(feats)
+    T = oracle.Tree(x, y, z, m, quad=True)
+    e = T.export()
+    fp, fi, _, _ = T.group_lists(0.8)
+    leaves = np.nonzero((e["child_count"] == 0) & (e["count"] > 0))[0]
+    ol = np.repeat(leaves, np.diff(fp))
+    print(f"pairs gpu {n} oracle {len(fi)}")
+    got = np.sort(gl.astype(np.int64) << 32 | gc.astype(np.int64))
+    want = np.sort(ol.astype(np.int64) << 32 | fi.astype(np.int64))
+    np.testing.assert_array_equal(got, want)
+
+
+def test_band_selects_the_marginal_pairs(gpu_ctx_factory):
+    import oracle
+    import pynbg
+    s = plummer_f32(4096, 2)
+    x, y, z, m = f64(s)
+    ctx = gpu_ctx_factory(4096)
+    ctx.upload(s["x"], s["y"], s["z"], s["m"])
+    ctx.learn_clear()
+    band = 0.6
+    n = ctx.learn_collect(pynbg.make_params("barnes_hut", G=1.0, eps=0.005, theta=0.8, learn_band=band))
+    feats, _ = ctx.learn_pool(0, n)
+    gl, gc = _ids(feats)
+    T = oracle.Tree(x, y, z, m, quad=True)
+    e = T.export()
+    fp, fi, _, _ = T.group_lists(0.8)
+    leaves = np.nonzero((e["child_count"] == 0) & (e["count"] > 0))[0]
+    ol = np.repeat(leaves, np.diff(fp))
+    worst = np.linalg.norm(e["com"][fi] - e["com"][ol], axis=1) - e["max_radius"][ol]
+    ratio = 2.0 * e["half"][fi] / worst
+    sel = ratio > band
+    want = set(zip(ol[sel].tolist(), fi[sel].tolist()))
+    got = set(zip(gl.tolist(), gc.tolist()))
+    # the device tests the band in FP32: only pairs within a rounding of the edge may differ
+    diff = want ^ got
+    edge = np.abs(ratio - band) < 1e-5 * band
+    assert len(diff) <= int(edge.sum()), (len(diff), int(edge.sum()))
+    assert len(got) == n and 0.1 < n / len(fi) < 0.5
+
+
+def test_samples_match_oracle(gpu_ctx_factory):
+    import learned as LO
+    import oracle
+    import pynbg
+    s = plummer_f32(4096, 5)
+    x, y, z, m = f64(s)
+    eps = 0.005
+    ctx = gpu_ctx_factory(4096)
+    ctx.upload(s["x"], s["y"], s["z"], s["m"])
+    ctx.learn_clear()
+    n = ctx.learn_collect(pynbg.make_params("barnes_hut", G=1.0, eps=eps, theta=0.8, learn_band=0.6))
+    feats, tg = ctx.learn_pool(0, n)
+    g = ctx.tree_export()
+    first, cnt = _subtree_ranges(g)
+    P = np.stack([s["x"], s["y"], s["z"]], 1).astype(np.float64)
+    w = s["m"].astype(np.float64)
+    order = g["order"]
+    # accelerations of the snapshot (direct, FP64) for the loss weights
+    ad = oracle.accel_direct(x, y, z, m, 1.0, eps * eps)
+    a2 = np.sum(ad * ad, axis=1)
+    rng = np.random.default_rng(0)
+    pick = rng.choice(n, 150, replace=False)
+    gl, gc = _ids(feats)
+    F_or, Y_or, W_or = [], [], []
+    for k in pick:
+        lf, cl = gl[k], gc[k]
+        mem = order[first[lf]:first[lf] + cnt[lf]]
+        src = order[first[cl]:first[cl] + cnt[cl]]
+        com = g["com"][cl].astype(np.float64)
+        M = float(np.float32(g["mass"][cl]))
+        s2, s3 = LO.cell_moments(P[src], w[src], com)
+        skip = [np.nonzero(src == i)[0] for i in mem]
+        f, yv, scale = LO.pair_sample(P[mem], P[src], w[src], com, M, g["quad"][cl].astype(np.float64), s2, s3,
+                                      2.0 * g["half"][cl], g["com"][lf].astype(np.float64),
+                                      float(g["max_radius"][lf]), eps * eps,
+                                      member_in_cell=[sk if len(sk) else None for sk in skip])
+        F_or.append(f)
+        Y_or.append(yv)
+        W_or.append(scale * np.sqrt(np.sum(1.0 / a2[mem])))
+    F_or, Y_or, W_or = np.array(F_or), np.array(Y_or), np.array(W_or)
+    Fg, Yg, Wg = feats[pick, :13].astype(np.float64), tg[pick, :3].astype(np.float64), tg[pick, 3]
+    # geometry and second-moment features: FP32 rounding
+    np.testing.assert_allclose(Fg[:, [0, 1, 2, 3, 10, 11]], F_or[:, [0, 1, 2, 3, 10, 11]], rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(Fg[:, [7, 8, 9, 12]], F_or[:, [7, 8, 9, 12]], rtol=1e-3, atol=1e-4)
+    # octupole features: FP32 parallel-axis sums with cancellation, relative to their spread
+    err3 = np.abs(Fg[:, 4:7] - F_or[:, 4:7]).max()
+    print(f"octupole feature max abs err {err3:.2e} (spread {np.abs(F_or[:, 4:7]).max():.2e})")
+    assert err3 < 2e-3 * max(1.0, np.abs(F_or[:, 4:7]).max())
+    # residual targets: FP64 sums on both sides over the same FP32 inputs
+    errY = np.abs(Yg - Y_or).max() / np.abs(Y_or).max()
+    print(f"target max rel err {errY:.2e}")
+    assert errY < 1e-3
+    np.testing.assert_allclose(Wg, W_or, rtol=2e-3)
+
+
+def test_forward_backward_match_oracle(gpu_ctx_factory):
+    import learned as LO
+    import pynbg
+    rng = np.random.default_rng(11)
+    blob = LO.init_blob(11)
+    blob[:13] = rng.normal(size=13) * 0.2
+    blob[16:29] = 0.5 + rng.random(13)
+    t = blob[32:]
+    for k in ("b1", "b2", "b3", "b4"):
+        o, sz = LO.OFF[k]
+        t[o:o + sz] = rng.normal(size=sz) * 0.1
+    n = 200  # not a multiple of 64: the last block is partly empty
+    X = np.zeros((n, 16), np.float32)
+    X[:, :13] = rng.normal(size=(n, 13)).astype(np.float32)
+    T = np.concatenate([rng.normal(size=(n, 3)), 0.5 + rng.random((n, 1))], 1).astype(np.float32)
+    ctx = gpu_ctx_factory(64)
+    ctx.learn_set_weights(blob)
+    y, loss, grad = ctx.learn_grad(X, T)
+    g_or, y_or = LO.mlp_backward(blob.astype(np.float64), X, T, emulate_bf16=True)
+    loss_or = float(np.sum(T[:, 3] * np.sum((y_or - T[:, :3]) ** 2, 1)) / n)
+    ey = np.abs(y - y_or).max() / np.abs(y_or).max()
+    eg = np.linalg.norm(grad - g_or) / np.linalg.norm(g_or)
+    cos = float(grad @ g_or / (np.linalg.norm(grad) * np.linalg.norm(g_or)))
+    print(f"forward max rel {ey:.2e}  loss {loss:.6f} vs {loss_or:.6f}  grad rel L2 {eg:.2e} cos {cos:.6f}")
+    # BF16 operands: inputs, weights and activations round identically on both sides;
+    # the device also rounds the backward deltas to BF16 (2^-8 relative each)
+    assert ey < 5e-3
+    assert abs(loss - loss_or) <= 1e-2 * loss_or
+    assert eg < 3e-2 and cos > 0.999
+    for k in ("W1", "b1", "W2", "b2", "W3", "b3", "W4", "b4"):
+        o, sz = LO.OFF[k]
+        e = np.linalg.norm(grad[o:o + sz] - g_or[o:o + sz]) / np.linalg.norm(g_or[o:o + sz])
+        assert e < 5e-2, (k, e)
+
+
+def test_training_lowers_the_loss(gpu_ctx_factory):
+    import pynbg
+    from pynbg import learned
+    s = plummer_f32(16384, 7)
+    ctx = gpu_ctx_factory(16384)
+    ctx.upload(s["x"], s["y"], s["z"], s["m"])
+    ctx.learn_set_weights(learned.init_blob(1))
+    ctx.learn_clear()
+    n = ctx.learn_collect(pynbg.make_params("barnes_hut", G=1.0, eps=0.005, theta=0.8, learn_band=0.6))
+    ctx.learn_standardize()
+    loss = ctx.learn_train(600, 4096, 2e-3, seed=3)
+    first, last = float(np.mean(loss[:20])), float(np.mean(loss[-50:]))
+    print(f"pool {n} loss {first:.4f} -> {last:.4f}")
+    assert np.all(np.isfinite(loss))
+    assert last < 0.6 * first
+
+
+def _rel_rms(a, ref):
+    return float(np.sqrt(np.sum((a - ref) ** 2) / np.sum(ref ** 2)))
+
+
+def test_trained_model_gain_theta08(gpu_ctx_factory):
+    """The committed model on an unseen Plummer draw (N=65,536, seed 11, eps=0.005):
+    learned theta=0.8 against plain theta=0.8, both judged by the FP32 direct sum."""
+    import pynbg
+    from pynbg import learned
+    if not os.path.exists(WEIGHTS):
+        pytest.skip("no trained model committed")
+    s = plummer_f32(65536, 11)
+    ctx = gpu_ctx_factory(65536)
+    ctx.upload(s["x"], s["y"], s["z"], s["m"])
+    ctx.accel(pynbg.make_params("direct", G=1.0, eps=0.005))
+    ref = ctx.accelerations().astype(np.float64)
+    ctx.learn_set_weights(learned.load_weights(WEIGHTS))
+    res = {}
+    for name, kw in (("plain 0.8", dict(theta=0.8)), ("plain 0.5", dict(theta=0.5)),
+                     ("learned 0.8", dict(theta=0.8, learned=True, learn_band=0.6))):
+        st = ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=0.005, **kw))
+        res[name] = _rel_rms(ctx.accelerations().astype(np.float64), ref)
+        if "learned" in name:
+            assert st.n_learned > 0
+    print({k: f"{v:.3e}" for k, v in res.items()})
+    assert res["learned 0.8"] < 0.8 * res["plain 0.8"]
+
+
+def test_learned_graph_matches_eager(gpu_ctx_factory):
+    import pynbg
+    from pynbg import learned
+    s = plummer_f32(8192, 4)
+    out = []
+    for no_graph in (True, False):
+        ctx = gpu_ctx_factory(8192)
+        ctx.upload(s["x"], s["y"], s["z"], s["m"], s["vx"], s["vy"], s["vz"])
+        ctx.learn_set_weights(learned.init_blob(2))
+        p = pynbg.make_params("barnes_hut", G=1.0, eps=0.01, theta=0.8, dt=1e-3, learned=True, learn_band=0.5,
+                              no_graph=no_graph)
+        ctx.accel(p)
+        ctx.step(p, 3)
+        d = ctx.download()
+        out.append(np.stack([d["x"], d["y"], d["z"], d["ax"], d["ay"], d["az"]], 1).astype(np.float64))
+    # the correction sums per group with float atomics: equal up to summation order
+    err = np.abs(out[0] - out[1]).max() / np.abs(out[0]).max()
+    print(f"graph vs eager max rel {err:.2e}")
+    assert err < 1e-5
+
+
+def test_learned_argument_errors(gpu_ctx_factory):
+    import pynbg
+    s = plummer_f32(512, 1)
+    ctx = gpu_ctx_factory(512)
+    ctx.upload(s["x"], s["y"], s["z"], s["m"])
+    with pytest.raises(pynbg.NbgError, match="no learned model"):
+        ctx.accel(pynbg.make_params("barnes_hut", theta=0.8, learned=True))
+    with pytest.raises(pynbg.NbgError, match="NBG_LEARN_NPARAM"):
+        ctx.learn_set_weights(np.zeros(10, np.float32))
+    from pynbg import learned
+    ctx.learn_set_weights(learned.init_blob(0))
+    with pytest.raises(pynbg.NbgError, match="quadrupoles"):
+        ctx.accel(pynbg.make_params("barnes_hut", "monopole", theta=0.8, learned=True))
+    with pytest.raises(pynbg.NbgError, match="multiple of 64"):
+        ctx.learn_train(1, 100, 1e-3)
