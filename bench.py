@@ -3,15 +3,22 @@
 
 Workload (the configuration the headline metric is quoted on): Plummer sphere
 N=1,048,576 (ic::Plummer<3>, seed 0, M=1, a=1, G=1), FP32 positions/velocities, Plummer
-softening eps=0.005, Barnes-Hut theta=0.5 with quadrupoles, kick-drift-kick leapfrog
-dt=5e-4.  A step = one KDK step: tree rebuilt from scratch (keys, sort, build, upsweep),
-group walk, two kicks and a drift, replayed from a captured hipGraph.
+softening eps=0.005, quadrupoles, kick-drift-kick leapfrog dt=5e-4.  A step = one KDK
+step: tree rebuilt from scratch (keys, sort, build, upsweep), group walk, two kicks and a
+drift, replayed from a captured hipGraph.  Two variants are timed on the same state:
+  plain    Barnes-Hut theta=0.5
+  learned  Barnes-Hut theta=0.8 plus the learned far-field correction on the accepted
+           (leaf, cell) pairs with size/(distance - group radius) > 0.6
+Each variant's force error is its relative RMS against the direct sum on 16,384 random
+targets of the initial state.
 
-Prints one JSON line.  `value` is seconds per step (lower is better); `roofline` is the
-walk kernel's algorithmic FP32 rate (20 flop per P2P, 65 per quadrupole M2P, counted from
-the actual interaction lists) over its HIP-event time inside the timed steps, against the
-MI355X FP32 peak; `cpu_baseline` times the FP64 oracle (the reference CPU path restated,
-OpenMP) on a bounded sample of the same workload on this host's cores.
+Prints one JSON line.  `value` is seconds per step of the learned variant (lower is
+better); `comparison` holds both variants' s/step and errors and the speedup.  `roofline`
+is the dominant kernel's algorithmic rate over its HIP-event time inside the timed steps:
+the walk against the FP32 peak (20 flop per P2P, 65 per quadrupole M2P, counted from the
+actual interaction lists) or the MLP against the dense BF16 MFMA peak (18,432 flop per
+pair).  `cpu_baseline` times the FP64 oracle (the reference CPU path restated, OpenMP,
+plain theta=0.5) on a bounded sample of the same workload on this host's cores.
 """
 import argparse
 import json
@@ -25,9 +32,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "opengl-physics_amd"))
 
 PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0   # dense MFMA
 FLOP_P2P = 20.0
 FLOP_M2P_QUAD = 65.0
 FLOP_M2P_MONO = 20.0
+FLOP_MLP_PAIR = 2.0 * (13 * 64 + 64 * 64 + 64 * 64 + 64 * 3)   # 18,432 multiply-add flop per pair
 
 
 def walk_traffic(n, theta):
@@ -39,6 +48,16 @@ def walk_traffic(n, theta):
     with open(path) as f:
         d = json.load(f)
     return d["bytes_per_launch"] if (d["n"] == n and d["theta"] == theta) else None
+
+
+def mlp_traffic(n, theta, band):
+    """HBM bytes per learned-correction launch from the committed rocprofv3 passes, or None."""
+    path = os.path.join(ROOT, "profiles", "learn_traffic_plummer_1m_theta0.8.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d["bytes_per_launch"] if (d["n"] == n and d["theta"] == theta and d["band"] == band) else None
 
 
 def log(*a):
@@ -66,17 +85,51 @@ def cpu_baseline(state32, eps, theta, dt, multipole):
                       f"evaluation; built -O3 -march=x86-64-v3 -funroll-loops -fopenmp"}
 
 
+def run_variant(ctx, pynbg, ic, base, steps, warmup):
+    """Prime (with interaction counting), warm up, then time `steps` graph-replayed KDK steps.
+    Returns wall seconds, the stage stats of the timed steps and the interaction counts."""
+    kw = dict(base)
+    pc = pynbg.make_params("barnes_hut", count=True, **kw)
+    p = pynbg.make_params("barnes_hut", stage_times=True, **kw)
+    ctx.upload(ic["x"], ic["y"], ic["z"], ic["m"], ic["vx"], ic["vy"], ic["vz"])
+    st0 = ctx.accel(pc)  # System::Prime, with interaction counting
+    if warmup > 0:
+        ctx.step(p, warmup)
+    t0 = time.perf_counter()  # nbg_step synchronises the device before it returns
+    st = ctx.step(p, steps)
+    t1 = time.perf_counter()
+    state = ctx.download()
+    st1 = ctx.accel(pc)  # interaction counts at the end state
+    if not np.all(np.isfinite(state["ax"])):
+        raise RuntimeError("non-finite accelerations")
+    return t1 - t0, st, st0, st1
+
+
+def force_error(ctx, pynbg, ic, base, targets, ref):
+    """Relative RMS force error of one evaluation at the initial state on the sampled targets."""
+    ctx.upload(ic["x"], ic["y"], ic["z"], ic["m"])
+    ctx.accel(pynbg.make_params("barnes_hut", **base))
+    a = ctx.accelerations()[targets].astype(np.float64)
+    return float(np.sqrt(np.sum((a - ref) ** 2) / np.sum(ref ** 2)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--variant", default="learned", choices=["learned", "plain"],
+                    help="learned: the learned theta=0.8 step timed next to plain theta=0.5 (the headline "
+                         "metric); plain: plain Barnes-Hut at --theta only")
+    ap.add_argument("--theta", type=float, default=0.5, help="plain Barnes-Hut opening angle")
+    ap.add_argument("--learned-theta", type=float, default=0.8)
+    ap.add_argument("--learn-band", type=float, default=0.6)
     ap.add_argument("--eps", type=float, default=0.005)
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--multipole", default="quadrupole", choices=["monopole", "quadrupole"])
     ap.add_argument("--no-kahan", action="store_true")
+    ap.add_argument("--error-sample", type=int, default=16384, help="targets of the direct-sum error check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args()
 
@@ -84,68 +137,116 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU decomposition is "
               f"design-only in this build", file=sys.stderr)
         sys.exit(2)
+    if args.variant == "learned" and args.multipole != "quadrupole":
+        print("the learned variant needs quadrupoles", file=sys.stderr)
+        sys.exit(2)
 
     import pynbg
+    from pynbg import learned
     n = args.n
     log(f"generating Plummer N={n} seed 0")
     ic = pynbg.to_f32(pynbg.ic_plummer(n, 0))
     kahan = not args.no_kahan
-    p = pynbg.make_params("barnes_hut", args.multipole, G=1.0, eps=args.eps, theta=args.theta, dt=args.dt,
-                          kahan=kahan, stage_times=True)
-    pc = pynbg.make_params("barnes_hut", args.multipole, G=1.0, eps=args.eps, theta=args.theta, dt=args.dt,
-                           kahan=kahan, count=True)
+    plain = dict(multipole=args.multipole, G=1.0, eps=args.eps, theta=args.theta, dt=args.dt, kahan=kahan)
+    lrn = dict(multipole="quadrupole", G=1.0, eps=args.eps, theta=args.learned_theta, dt=args.dt, kahan=kahan,
+               learned=True, learn_band=args.learn_band)
+    rng = np.random.default_rng(12345)
+    targets = np.sort(rng.choice(n, min(n, args.error_sample), replace=False))
     with pynbg.Context(n) as ctx:
-        ctx.upload(ic["x"], ic["y"], ic["z"], ic["m"], ic["vx"], ic["vy"], ic["vz"])
-        st0 = ctx.accel(pc)  # System::Prime, with interaction counting
-        log(f"prime: nodes={st0.n_nodes} leaves={st0.n_leaves} depth={st0.max_depth} "
-            f"p2p={st0.n_p2p} m2p={st0.n_m2p} walk={st0.ms_walk:.3f} ms")
-        if args.warmup > 0:
-            ctx.step(p, args.warmup)
-        # timed region: the step call synchronises the device before it returns
-        t0 = time.perf_counter()
-        st = ctx.step(p, args.steps)
-        t1 = time.perf_counter()
-        state = ctx.download()
-        st1 = ctx.accel(pc)  # interaction counts at the end state
-        wall = t1 - t0
-    ms_step = 1e3 * wall / args.steps
-    n_p2p = 0.5 * (st0.n_p2p + st1.n_p2p)
-    n_m2p = 0.5 * (st0.n_m2p + st1.n_m2p)
-    fm2p = FLOP_M2P_QUAD if args.multipole == "quadrupole" else FLOP_M2P_MONO
-    flops = FLOP_P2P * n_p2p + fm2p * n_m2p
-    walk_ms = st.ms_walk / args.steps
-    achieved = flops / (walk_ms * 1e-3) / 1e12
-    stage = {k: round(getattr(st, "ms_" + k) / args.steps, 4)
-             for k in ("keys", "sort", "build", "upsweep", "walk", "integrate")}
-    log(f"steps={args.steps} wall={wall:.4f} s ({ms_step:.3f} ms/step) stages(ms/step)={stage}")
+        ctx.upload(ic["x"], ic["y"], ic["z"], ic["m"])
+        ref = ctx.direct_targets(pynbg.make_params("direct", G=1.0, eps=args.eps), targets).astype(np.float64)
+        err_plain = force_error(ctx, pynbg, ic, plain, targets, ref)
+        wall_p, st_p, c0_p, c1_p = run_variant(ctx, pynbg, ic, plain, args.steps, args.warmup)
+        log(f"plain theta={args.theta}: {1e3 * wall_p / args.steps:.3f} ms/step, rel_rms {err_plain:.3e}")
+        if args.variant == "learned":
+            ctx.learn_set_weights(learned.load_weights())
+            err_l = force_error(ctx, pynbg, ic, lrn, targets, ref)
+            err_l_plain = force_error(ctx, pynbg, ic, dict(lrn, learned=False), targets, ref)
+            wall_l, st_l, c0_l, c1_l = run_variant(ctx, pynbg, ic, lrn, args.steps, args.warmup)
+            log(f"learned theta={args.learned_theta} band {args.learn_band}: {1e3 * wall_l / args.steps:.3f} "
+                f"ms/step, rel_rms {err_l:.3e} (plain at that theta {err_l_plain:.3e}); "
+                f"pairs {c0_l.n_learned}")
+
+    def walk_roofline(st, c0, c1, multipole, theta):
+        n_p2p = 0.5 * (c0.n_p2p + c1.n_p2p)
+        n_m2p = 0.5 * (c0.n_m2p + c1.n_m2p)
+        fm2p = FLOP_M2P_QUAD if multipole == "quadrupole" else FLOP_M2P_MONO
+        flops = FLOP_P2P * n_p2p + fm2p * n_m2p
+        walk_ms = st.ms_walk / args.steps
+        ach = flops / (walk_ms * 1e-3) / 1e12
+        return {"bound": "mfma", "pipe": "fp32-valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": walk_traffic(n, theta),
+                "kernel": "k_walk_bins", "kernel_ms": round(walk_ms, 4), "flop_per_launch": flops}, n_p2p, n_m2p
+
+    roof_p, p2p_p, m2p_p = walk_roofline(st_p, c0_p, c1_p, args.multipole, args.theta)
+    stage_p = {k: round(getattr(st_p, "ms_" + k) / args.steps, 4)
+               for k in ("keys", "sort", "build", "upsweep", "walk", "integrate")}
+    plain_block = {"theta": args.theta, "s_per_step": round(wall_p / args.steps, 6), "rel_rms_force": err_plain,
+                   "stage_ms_per_step": stage_p, "interactions_per_step": {"p2p": p2p_p, "m2p": m2p_p},
+                   "interactions_per_s": round((p2p_p + m2p_p) / (roof_p["kernel_ms"] * 1e-3), 1),
+                   "roofline": roof_p}
     out = {
         "metric": "s/step, 1M-body BH: plain θ=0.5 vs learned θ=0.8 at equal RMS force error",
-        "value": round(wall / args.steps, 6),
         "unit": "s/step",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (ic::Plummer<3> seed 0, generated on the host)",
-        "config": {"workload": "plummer_1m_barnes_hut_theta0.5" if args.theta == 0.5 else
-                   f"plummer_bh_theta{args.theta}", "n": n, "theta": args.theta, "eps": args.eps, "dt": args.dt,
-                   "multipole": args.multipole, "kahan": kahan, "variant": "plain"},
-        "roofline": {"bound": "mfma", "pipe": "fp32-valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": walk_traffic(n, args.theta),
-                     "kernel": "k_walk_bins", "kernel_ms": round(walk_ms, 4),
-                     "flop_per_launch": flops},
-        "interactions_per_s": round((n_p2p + n_m2p) / (walk_ms * 1e-3), 1),
-        "interactions_per_step": {"p2p": n_p2p, "m2p": n_m2p},
-        "stage_ms_per_step": stage,
-        "tree": {"nodes": st.n_nodes, "leaves": st.n_leaves, "max_depth": st.max_depth},
     }
-    if not np.all(np.isfinite(state["ax"])):
-        raise RuntimeError("non-finite accelerations")
+    if args.variant == "plain":
+        out.update({"value": plain_block["s_per_step"], "ms_per_step": round(1e3 * wall_p / args.steps, 4),
+                    "config": {"workload": f"plummer_{n}_barnes_hut_theta{args.theta}", "n": n,
+                               "theta": args.theta, "eps": args.eps, "dt": args.dt, "multipole": args.multipole,
+                               "kahan": kahan, "variant": "plain"},
+                    "roofline": roof_p, "rel_rms_force": err_plain,
+                    "interactions_per_s": plain_block["interactions_per_s"],
+                    "interactions_per_step": plain_block["interactions_per_step"],
+                    "stage_ms_per_step": stage_p,
+                    "tree": {"nodes": st_p.n_nodes, "leaves": st_p.n_leaves, "max_depth": st_p.max_depth}})
+    else:
+        roof_w, p2p_l, m2p_l = walk_roofline(st_l, c0_l, c1_l, "quadrupole", args.learned_theta)
+        pairs = 0.5 * (c0_l.n_learned + c1_l.n_learned)
+        mlp_ms = st_l.ms_learned / args.steps
+        mlp_ach = pairs * FLOP_MLP_PAIR / (mlp_ms * 1e-3) / 1e12
+        roof_m = {"bound": "mfma", "pipe": "bf16-mfma", "achieved": round(mlp_ach, 3), "peak": PEAK_BF16_TFLOPS,
+                  "unit": "TFLOP/s", "frac": round(mlp_ach / PEAK_BF16_TFLOPS, 5),
+                  "traffic": mlp_traffic(n, args.learned_theta, args.learn_band),
+                  "kernel": "k_learn_infer (+ k_learn_zero, k_learn_apply)", "kernel_ms": round(mlp_ms, 4),
+                  "flop_per_launch": pairs * FLOP_MLP_PAIR, "pairs_per_launch": pairs}
+        stage_l = {k: round(getattr(st_l, "ms_" + k) / args.steps, 4)
+                   for k in ("keys", "sort", "build", "upsweep", "walk", "learned", "integrate")}
+        dominant, secondary = (roof_m, roof_w) if mlp_ms >= roof_w["kernel_ms"] else (roof_w, roof_m)
+        s_l = wall_l / args.steps
+        out.update({
+            "value": round(s_l, 6),
+            "ms_per_step": round(1e3 * s_l, 4),
+            "config": {"workload": f"plummer_{n}_learned_theta{args.learned_theta}_vs_plain_theta{args.theta}",
+                       "n": n, "eps": args.eps, "dt": args.dt, "multipole": "quadrupole", "kahan": kahan,
+                       "variant": "learned", "theta": args.learned_theta, "learn_band": args.learn_band,
+                       "model": "13-64-64-64-3 SiLU, BF16 MFMA (opengl-physics_amd/learned/farfield_mlp.npy)"},
+            "rel_rms_force": err_l,
+            "comparison": {
+                "plain": plain_block,
+                "learned": {"theta": args.learned_theta, "learn_band": args.learn_band, "s_per_step": round(s_l, 6),
+                            "rel_rms_force": err_l, "rel_rms_force_without_model": err_l_plain,
+                            "stage_ms_per_step": stage_l, "learned_pairs_per_step": pairs,
+                            "interactions_per_step": {"p2p": p2p_l, "m2p": m2p_l}},
+                "speedup_vs_plain": round(wall_p / wall_l, 4),
+                "error_target": 1e-3,
+                "error_target_met": bool(err_l <= 1e-3),
+                "error_sample": f"{len(targets)} random targets (seed 12345) of the initial state, judged by the "
+                                f"FP32 compensated direct sum over all {n} sources",
+            },
+            "roofline": dominant,
+            "roofline_secondary": secondary,
+            "interactions_per_s": round((p2p_l + m2p_l) / (roof_w["kernel_ms"] * 1e-3), 1),
+            "stage_ms_per_step": stage_l,
+            "tree": {"nodes": st_l.n_nodes, "leaves": st_l.n_leaves, "max_depth": st_l.max_depth},
+        })
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ic, args.eps, args.theta, args.dt, args.multipole)
     else:

@@ -32,6 +32,9 @@ namespace nbg {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 hbf16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kF = NBG_LEARN_FEATURES;  // 13
 constexpr int kH = NBG_LEARN_HIDDEN;    // 64
@@ -54,11 +57,20 @@ __device__ __forceinline__ uint16_t bf16_rne(float f) {
     return static_cast<uint16_t>(b >> 16);
 }
 __device__ __forceinline__ float bf16_f32(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+// two floats -> two bf16 (round to nearest even) in one v_cvt_pk_bf16_f32; low half = a
+__device__ __forceinline__ uint32_t bf16x2_rne(float a, float b) {
+    const f32x2 v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, hbf16x2));
+}
 
 __device__ __forceinline__ float sigmoid_f(float z) {
     return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
 }
 __device__ __forceinline__ float silu_f(float z) { return z * sigmoid_f(z); }
+// The network's activation: SiLU of the BF16-rounded pre-activation, rounded to BF16
+// (inference reads it from a 64K-entry table indexed by those BF16 bits).
+__device__ __forceinline__ uint16_t act_bits(uint16_t zb) { return bf16_rne(silu_f(bf16_f32(zb))); }
+__device__ __forceinline__ uint16_t act_of(float z) { return act_bits(bf16_rne(z)); }
 __device__ __forceinline__ float dsilu_f(float z) {
     const float s = sigmoid_f(z);
     return s * (1.f + z * (1.f - s));
@@ -66,6 +78,17 @@ __device__ __forceinline__ float dsilu_f(float z) {
 
 __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+// K = 32: lane holds A[row c][k 8g .. 8g+7], B[k 8g .. 8g+7][col c]; D as for K = 16.
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ bf16x8 cat8(bf16x4 lo, bf16x4 hi) {
+    const uint64_t l = __builtin_bit_cast(uint64_t, lo), h = __builtin_bit_cast(uint64_t, hi);
+    struct P {
+        uint64_t a, b;
+    } v{l, h};
+    return __builtin_bit_cast(bf16x8, v);
 }
 
 // ---------------------------------------------------------------------------- features
@@ -253,19 +276,43 @@ __device__ __forceinline__ void load_frags(Frags& F, const uint16_t* __restrict_
     for (int j = 0; j < 4; ++j) F.b4[j] = (4 * g + j < 3) ? b32[kOffB4 + 4 * g + j] : 0.f;
 }
 
-__device__ __forceinline__ bf16x4 silu_pack(f32x4 z) {
-    bf16x4 h;
+// Activation of a D fragment, packed as the next layer's B fragment.
+struct ActDirect {  // computed (training)
+    __device__ __forceinline__ bf16x4 operator()(f32x4 z) const {
+        bf16x4 h;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) h[j] = static_cast<short>(bf16_rne(silu_f(z[j])));
-    return h;
-}
+        for (int j = 0; j < 4; ++j) h[j] = static_cast<short>(act_of(z[j]));
+        return h;
+    }
+};
+// Inference: the same function, half of the row blocks looked up in a 64K-entry LDS table
+// and half computed, so the LDS pipe (bank-conflicted lookups) and the transcendental VALU
+// work proceed side by side; both give identical bits.
+struct ActTable {
+    const uint16_t* tab;
+    __device__ __forceinline__ bf16x4 lookup(f32x4 z) const {
+        const uint32_t lo = bf16x2_rne(z[0], z[1]);
+        const uint32_t hi = bf16x2_rne(z[2], z[3]);
+        const uint32_t a = tab[lo & 0xffffu], b = tab[lo >> 16], c = tab[hi & 0xffffu], d = tab[hi >> 16];
+        return __builtin_bit_cast(bf16x4, (static_cast<uint64_t>(d << 16 | c) << 32) | (b << 16 | a));
+    }
+    __device__ __forceinline__ bf16x4 compute(f32x4 z) const {
+        const uint32_t lo = bf16x2_rne(z[0], z[1]);
+        const uint32_t hi = bf16x2_rne(z[2], z[3]);
+        const float s0 = silu_f(__uint_as_float(lo << 16)), s1 = silu_f(__uint_as_float(lo & 0xffff0000u));
+        const float s2 = silu_f(__uint_as_float(hi << 16)), s3 = silu_f(__uint_as_float(hi & 0xffff0000u));
+        const uint32_t r0 = bf16x2_rne(s0, s1), r1 = bf16x2_rne(s2, s3);
+        return __builtin_bit_cast(bf16x4, (static_cast<uint64_t>(r1) << 32) | r0);
+    }
+    __device__ __forceinline__ bf16x4 operator()(f32x4 z, int mb) const { return (mb & 1) ? compute(z) : lookup(z); }
+};
 
 // One 16-pair tile through the network.  x: this lane's B fragment of the standardised
 // features (rows 4g..4g+3 of column c).  Returns D of the output layer (lanes 0..15 hold
 // outputs 0..2 of pair c).  With Z != nullptr the pre-activations of the three hidden
 // layers are stored as Z[layer][pair][hidden] (pair = col0 + c).
-template <bool STORE>
-__device__ __forceinline__ f32x4 mlp_tile(const Frags& F, bf16x4 x, float* Z, int col0, int lane) {
+template <bool STORE, class Act>
+__device__ __forceinline__ f32x4 mlp_tile(const Frags& F, bf16x4 x, float* Z, int col0, int lane, const Act& act) {
     const int g = lane >> 4, c = lane & 15;
     bf16x4 h[4];
     f32x4 z[4];
@@ -274,7 +321,7 @@ __device__ __forceinline__ f32x4 mlp_tile(const Frags& F, bf16x4 x, float* Z, in
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
         if (STORE) *reinterpret_cast<f32x4*>(Z + (col0 + c) * kH + 16 * mb + 4 * g) = z[mb];
-        h[mb] = silu_pack(z[mb]);
+        h[mb] = act(z[mb]);
     }
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
@@ -286,7 +333,7 @@ __device__ __forceinline__ f32x4 mlp_tile(const Frags& F, bf16x4 x, float* Z, in
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
         if (STORE) *reinterpret_cast<f32x4*>(Z + 64 * kH + (col0 + c) * kH + 16 * mb + 4 * g) = z[mb];
-        h[mb] = silu_pack(z[mb]);
+        h[mb] = act(z[mb]);
     }
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
@@ -298,7 +345,7 @@ __device__ __forceinline__ f32x4 mlp_tile(const Frags& F, bf16x4 x, float* Z, in
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
         if (STORE) *reinterpret_cast<f32x4*>(Z + 2 * 64 * kH + (col0 + c) * kH + 16 * mb + 4 * g) = z[mb];
-        h[mb] = silu_pack(z[mb]);
+        h[mb] = act(z[mb]);
     }
     f32x4 y = F.b4;
 #pragma unroll
@@ -307,81 +354,311 @@ __device__ __forceinline__ f32x4 mlp_tile(const Frags& F, bf16x4 x, float* Z, in
 }
 
 // ---------------------------------------------------------------------------- inference
-struct InferArgs {
+// Inference fragments for the K = 32 instruction.  The hidden layers' D registers (rows
+// 4g..4g+3 of row blocks 2kb and 2kb+1) are concatenated into the B operand of K block
+// kb, so K position 8g + j of block kb is hidden unit 16 (2 kb + j / 4) + 4 g + j % 4;
+// the weight fragments below are gathered in that order.  The first layer (K = 13 -> 16)
+// keeps the K = 16 instruction.
+struct Frags32 {
+    bf16x4 a1[4];
+    bf16x8 a2[4][2], a3[4][2], a4[2];
+    f32x4 b1[4], b2[4], b3[4], b4;
+};
+
+__device__ __forceinline__ int hid32(int kb, int g, int j) { return 16 * (2 * kb + (j >> 2)) + 4 * g + (j & 3); }
+
+__device__ __forceinline__ void load_frags32(Frags32& F, const uint16_t* __restrict__ w16,
+                                             const float* __restrict__ b32, int lane) {
+    const int g = lane >> 4, c = lane & 15;
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        bf16x4 v;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int f = 4 * g + j;
+            v[j] = (f < kF) ? static_cast<short>(w16[kOffW1 + f * kH + 16 * mb + c]) : 0;
+        }
+        F.a1[mb] = v;
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) {
+            uint16_t u2[8], u3[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int i = hid32(kb, g, j);
+                u2[j] = w16[kOffW2 + i * kH + 16 * mb + c];
+                u3[j] = w16[kOffW3 + i * kH + 16 * mb + c];
+            }
+            F.a2[mb][kb] = __builtin_bit_cast(bf16x8, u2);
+            F.a3[mb][kb] = __builtin_bit_cast(bf16x8, u3);
+        }
+        f32x4 b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = b32[kOffB1 + 16 * mb + 4 * g + j];
+        F.b1[mb] = b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = b32[kOffB2 + 16 * mb + 4 * g + j];
+        F.b2[mb] = b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = b32[kOffB3 + 16 * mb + 4 * g + j];
+        F.b3[mb] = b;
+    }
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        uint16_t u4[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) u4[j] = (c < 3) ? w16[kOffW4 + hid32(kb, g, j) * 3 + c] : 0;
+        F.a4[kb] = __builtin_bit_cast(bf16x8, u4);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) F.b4[j] = (4 * g + j < 3) ? b32[kOffB4 + 4 * g + j] : 0.f;
+}
+
+template <class Act>
+__device__ __forceinline__ void mlp32_tile2(const Frags32& F, bf16x4 xa, bf16x4 xb, const Act& act, f32x4& ya,
+                                            f32x4& yb) {
+    bf16x4 ha[4], hb[4];
+    f32x4 za[4], zb[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        za[mb] = mfma16(F.a1[mb], xa, F.b1[mb]);
+        zb[mb] = mfma16(F.a1[mb], xb, F.b1[mb]);
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        ha[mb] = act(za[mb], mb);
+        hb[mb] = act(zb[mb], mb);
+    }
+#pragma unroll
+    for (int L = 0; L < 2; ++L) {
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            const f32x4 bias = L == 0 ? F.b2[mb] : F.b3[mb];
+            f32x4 a = bias, b = bias;
+#pragma unroll
+            for (int kb = 0; kb < 2; ++kb) {
+                const bf16x8 w = L == 0 ? F.a2[mb][kb] : F.a3[mb][kb];
+                a = mfma32(w, cat8(ha[2 * kb], ha[2 * kb + 1]), a);
+                b = mfma32(w, cat8(hb[2 * kb], hb[2 * kb + 1]), b);
+            }
+            za[mb] = a;
+            zb[mb] = b;
+        }
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            ha[mb] = act(za[mb], mb);
+            hb[mb] = act(zb[mb], mb);
+        }
+    }
+    f32x4 a = F.b4, b = F.b4;
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        a = mfma32(F.a4[kb], cat8(ha[2 * kb], ha[2 * kb + 1]), a);
+        b = mfma32(F.a4[kb], cat8(hb[2 * kb], hb[2 * kb + 1]), b);
+    }
+    ya = a;
+    yb = b;
+}
+
+// Two 16-pair tiles through the network in lockstep (independent MFMA chains and table
+// lookups interleave and hide each other's latency).
+template <class Act>
+__device__ __forceinline__ void mlp_tile2(const Frags& F, bf16x4 xa, bf16x4 xb, const Act& act, f32x4& ya,
+                                          f32x4& yb) {
+    bf16x4 ha[4], hb[4];
+    f32x4 za[4], zb[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        za[mb] = mfma16(F.a1[mb], xa, F.b1[mb]);
+        zb[mb] = mfma16(F.a1[mb], xb, F.b1[mb]);
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        ha[mb] = act(za[mb]);
+        hb[mb] = act(zb[mb]);
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        f32x4 a = F.b2[mb], b = F.b2[mb];
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            a = mfma16(F.a2[mb][kb], ha[kb], a);
+            b = mfma16(F.a2[mb][kb], hb[kb], b);
+        }
+        za[mb] = a;
+        zb[mb] = b;
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        ha[mb] = act(za[mb]);
+        hb[mb] = act(zb[mb]);
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        f32x4 a = F.b3[mb], b = F.b3[mb];
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            a = mfma16(F.a3[mb][kb], ha[kb], a);
+            b = mfma16(F.a3[mb][kb], hb[kb], b);
+        }
+        za[mb] = a;
+        zb[mb] = b;
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        ha[mb] = act(za[mb]);
+        hb[mb] = act(zb[mb]);
+    }
+    f32x4 a = F.b4, b = F.b4;
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+        a = mfma16(F.a4[kb], ha[kb], a);
+        b = mfma16(F.a4[kb], hb[kb], b);
+    }
+    ya = a;
+    yb = b;
+}
+
+// ---------------------------------------------------------------------------- inference
+// Gather: one thread per listed pair writes its standardised BF16 feature row (16 x 2 B;
+// entries 13..15 zero) and its output scale, so the matrix-core kernel streams contiguous
+// rows instead of chasing pair -> node loads.
+struct GatherArgs {
     const int2* pairs;
     const DevScalars* sc;
     uint32_t cap;
     const float4* nd_com;
     const float4* nd_aux;
     const float4* nd_mom;
-    const uint16_t* w16;
     const float* w32;  // full blob (standardisation first)
     float eps;
-    float4* corr;
+    uint4* feat;       // 2 per pair
+    float* scale;
     DevScalars* scw;
 };
 
-constexpr int kInferWaves = 4;
-
-__global__ __launch_bounds__(64 * kInferWaves) void k_learn_infer(InferArgs A) {
-    __shared__ __attribute__((aligned(16))) uint16_t s_x[kInferWaves][64][16];
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = lane_id();
-    const int g = lane >> 4, c = lane & 15;
+__global__ __launch_bounds__(256) void k_learn_gather(GatherArgs A) {
     const uint32_t listed = A.sc->learn_pairs;
     const uint32_t np = min(listed, A.cap);
     if (blockIdx.x == 0 && threadIdx.x == 0 && listed > A.cap) atomicOr(&A.scw->sticky_learn, 1);
+    const float* mu = A.w32;
+    const float* isd = A.w32 + 16;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < np; q += gridDim.x * blockDim.x) {
+        const int2 pr = A.pairs[q];
+        const float4 lc = A.nd_com[pr.x];
+        const float4 tgt = make_float4(lc.x, lc.y, lc.z, A.nd_aux[pr.x].z);
+        const float4 cm = A.nd_com[pr.y];
+        const float size2 = A.nd_aux[pr.y].y;
+        float f[kF], scale;
+        pair_features(tgt, cm, size2, A.nd_mom + 4 * pr.y, A.eps, f, scale);
+        float xv[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) xv[k] = (k < kF) ? (f[k] - mu[k]) * isd[k] : 0.f;
+        uint32_t w[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) w[k] = bf16x2_rne(xv[2 * k], xv[2 * k + 1]);
+        A.feat[2 * static_cast<size_t>(q)] = make_uint4(w[0], w[1], w[2], w[3]);
+        A.feat[2 * static_cast<size_t>(q) + 1] = make_uint4(w[4], w[5], w[6], w[7]);
+        A.scale[q] = scale;
+    }
+}
+
+struct InferArgs {
+    const int2* pairs;
+    const DevScalars* sc;
+    uint32_t cap;
+    const uint2* feat;   // 4 per pair (the gather's rows as 8-byte fragments)
+    const float* scale;
+    const uint16_t* w16;
+    const float* w32;    // full blob (standardisation first)
+    float4* corr;
+};
+
+constexpr int kInferWaves = 8;
+
+// Persistent: one 8-wave block per CU (the 128 KB activation table fills most of the
+// LDS).  A wave takes 64 pairs per step as four 16-pair tiles, two at a time, and loads
+// the next step's fragments, group ids and scales before computing the current one.
+__global__ __launch_bounds__(64 * kInferWaves) void k_learn_infer(InferArgs A) {
+    __shared__ __attribute__((aligned(16))) uint16_t s_act[65536];
+    for (int i = threadIdx.x; i < 65536; i += blockDim.x) s_act[i] = act_bits(static_cast<uint16_t>(i));
+    __syncthreads();
+    const ActTable act{s_act};
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = lane_id();
+    const int g = lane >> 4, c = lane & 15;
+    const uint32_t np = min(A.sc->learn_pairs, A.cap);
     const int64_t nchunks = (static_cast<int64_t>(np) + 63) / 64;
     const int64_t wid = static_cast<int64_t>(blockIdx.x) * kInferWaves + wave;
     const int64_t nw = static_cast<int64_t>(gridDim.x) * kInferWaves;
     if (wid >= nchunks) return;
-    Frags F;
-    load_frags(F, A.w16, A.w32 + kNorm, lane);
-    const float* mu = A.w32;
-    const float* isd = A.w32 + 16;
-    uint16_t(*xs)[16] = s_x[wave];
-    for (int64_t ch = wid; ch < nchunks; ch += nw) {
-        const int64_t q = ch * 64 + lane;
-        const bool valid = q < np;
-        int leaf = 0;
-        float scale = 0.f;
-        float f[kF];
-        if (valid) {
-            const int2 pr = A.pairs[q];
-            leaf = pr.x;
-            const float4 lc = A.nd_com[pr.x];
-            const float4 tgt = make_float4(lc.x, lc.y, lc.z, A.nd_aux[pr.x].z);
-            const float4 cm = A.nd_com[pr.y];
-            const float size2 = A.nd_aux[pr.y].y;
-            pair_features(tgt, cm, size2, A.nd_mom + 4 * pr.y, A.eps, f, scale);
-        } else {
-#pragma unroll
-            for (int k = 0; k < kF; ++k) f[k] = 0.f;
-        }
-#pragma unroll
-        for (int k = 0; k < 16; ++k) xs[lane][k] = (k < kF) ? bf16_rne((f[k] - mu[k]) * isd[k]) : 0;
-        wave_sync();
-        float o0 = 0.f, o1 = 0.f, o2 = 0.f;
+    Frags32 F;
+    load_frags32(F, A.w16, A.w32 + kNorm, lane);
+    // fragment of tile t: pair 64 ch + 16 t + c, features 4g..4g+3
+    auto load_chunk = [&](int64_t ch, uint2 (&x)[4], int& leaf, float& sc) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const bf16x4 x = *reinterpret_cast<const bf16x4*>(&xs[16 * t + c][4 * g]);
-            const f32x4 y = mlp_tile<false>(F, x, nullptr, 0, lane);
+            const int64_t q = ch * 64 + 16 * t + c;
+            x[t] = (q < np) ? A.feat[4 * q + g] : make_uint2(0u, 0u);
+        }
+        const int64_t q = ch * 64 + lane;
+        leaf = (q < np) ? A.pairs[q].x : -1;
+        sc = (q < np) ? A.scale[q] : 0.f;
+    };
+    uint2 xn[4];
+    int leaf_n;
+    float sc_n;
+    load_chunk(wid, xn, leaf_n, sc_n);
+    for (int64_t ch = wid; ch < nchunks; ch += nw) {
+        uint2 x[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) x[t] = xn[t];
+        const int leaf = leaf_n;
+        const float scale = sc_n;
+        if (ch + nw < nchunks) load_chunk(ch + nw, xn, leaf_n, sc_n);
+        float o0 = 0.f, o1 = 0.f, o2 = 0.f;
+#pragma unroll
+        for (int t = 0; t < 4; t += 2) {
+            f32x4 ya, yb;
+            mlp32_tile2(F, __builtin_bit_cast(bf16x4, x[t]), __builtin_bit_cast(bf16x4, x[t + 1]), act, ya, yb);
             // lanes 0..15 hold the outputs of pair 16 t + c; move them to lane 16 t + c
-            const float y0 = __shfl(y[0], lane & 15, 64), y1 = __shfl(y[1], lane & 15, 64),
-                        y2 = __shfl(y[2], lane & 15, 64);
+            const float a0 = __shfl(ya[0], c, 64), a1 = __shfl(ya[1], c, 64), a2 = __shfl(ya[2], c, 64);
+            const float b0 = __shfl(yb[0], c, 64), b1 = __shfl(yb[1], c, 64), b2 = __shfl(yb[2], c, 64);
             if (g == t) {
-                o0 = y0;
-                o1 = y1;
-                o2 = y2;
+                o0 = a0;
+                o1 = a1;
+                o2 = a2;
+            }
+            if (g == t + 1) {
+                o0 = b0;
+                o1 = b1;
+                o2 = b2;
             }
         }
-        if (valid) {
-            float* dst = reinterpret_cast<float*>(A.corr + leaf);
-            atomicAdd(dst + 0, scale * o0);
-            atomicAdd(dst + 1, scale * o1);
-            atomicAdd(dst + 2, scale * o2);
+        // segmented sum over runs of equal leaf ids (the walk lists a leaf's pairs
+        // together), then one atomic triple per run
+        float v0 = scale * o0, v1 = scale * o1, v2 = scale * o2;
+        const int prev = __shfl_up(leaf, 1, 64);
+        const bool head = lane == 0 || prev != leaf;
+        const uint64_t heads = ballot(head);
+        const int seg = __popcll(heads & ((2ull << lane) - 1ull));
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const float t0 = __shfl_up(v0, d, 64), t1 = __shfl_up(v1, d, 64), t2 = __shfl_up(v2, d, 64);
+            const int ts = __shfl_up(seg, d, 64);
+            if (lane >= d && ts == seg) {
+                v0 += t0;
+                v1 += t1;
+                v2 += t2;
+            }
         }
-        wave_sync();
+        const int next_seg = __shfl_down(seg, 1, 64);
+        const bool tail = lane == 63 || next_seg != seg;
+        if (tail && leaf >= 0) {
+            float* dst = reinterpret_cast<float*>(A.corr + leaf);
+            atomicAdd(dst + 0, v0);
+            atomicAdd(dst + 1, v1);
+            atomicAdd(dst + 2, v2);
+        }
     }
 }
 
@@ -621,7 +898,7 @@ __global__ __launch_bounds__(64) void k_learn_fwdbwd(TrainArgs A) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const bf16x4 x = *reinterpret_cast<const bf16x4*>(&s_x[16 * t + c][4 * g]);
-            const f32x4 y = mlp_tile<true>(F, x, &s_z[0][0][0], 16 * t, lane);
+            const f32x4 y = mlp_tile<true>(F, x, &s_z[0][0][0], 16 * t, lane, ActDirect{});
             if (g == 0) {
                 s_y[16 * t + c][0] = y[0];
                 s_y[16 * t + c][1] = y[1];
@@ -657,7 +934,7 @@ __global__ __launch_bounds__(64) void k_learn_fwdbwd(TrainArgs A) {
     {
         float a0 = 0.f, a1 = 0.f, a2 = 0.f;
         for (int n = 0; n < 64; ++n) {
-            const float h = bf16_f32(bf16_rne(silu_f(s_z[2][n][lane])));
+            const float h = bf16_f32(act_of(s_z[2][n][lane]));
             a0 += h * s_y[n][0];
             a1 += h * s_y[n][1];
             a2 += h * s_y[n][2];
@@ -699,7 +976,7 @@ __global__ __launch_bounds__(64) void k_learn_fwdbwd(TrainArgs A) {
                     bf16x4 v;
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        v[j] = static_cast<short>(bf16_rne(silu_f(zin[(16 * nb + 4 * g + j) * kH + 16 * ib + c])));
+                        v[j] = static_cast<short>(act_of(zin[(16 * nb + 4 * g + j) * kH + 16 * ib + c]));
                     Ain[ib][nb] = v;
                 }
 #pragma unroll
@@ -870,7 +1147,10 @@ int learn_alloc(nbg_ctx* c) {
 int learn_grow_pairs(nbg_ctx* c, int64_t need) {
     if (need <= c->lpairs_cap && c->lpairs) return NBG_OK;
     const int64_t cap = std::min<int64_t>(need + need / 4 + 1024, int64_t{0xffffffffll});
-    if (int r = lalloc(c, c->lpairs, static_cast<size_t>(cap))) {
+    int r = lalloc(c, c->lpairs, static_cast<size_t>(cap));
+    if (r == NBG_OK) r = lalloc(c, c->lfeat, 2 * static_cast<size_t>(cap));
+    if (r == NBG_OK) r = lalloc(c, c->lscale, static_cast<size_t>(cap));
+    if (r != NBG_OK) {
         c->lpairs_cap = 0;
         return r;
     }
@@ -893,19 +1173,29 @@ void launch_learn_zero(nbg_ctx* c) {
 }
 
 void launch_learn_correct(nbg_ctx* c, const nbg_params& p) {
+    GatherArgs G;
+    G.pairs = c->lpairs;
+    G.sc = c->dsc;
+    G.cap = static_cast<uint32_t>(c->lpairs_cap);
+    G.nd_com = c->nd_com;
+    G.nd_aux = c->nd_aux;
+    G.nd_mom = c->nd_mom;
+    G.w32 = c->lw32;
+    G.eps = static_cast<float>(std::sqrt(p.eps2));
+    G.feat = c->lfeat;
+    G.scale = c->lscale;
+    G.scw = c->dsc;
+    hipLaunchKernelGGL(k_learn_gather, dim3(256 * 16), dim3(256), 0, c->stream, G);
     InferArgs A;
     A.pairs = c->lpairs;
     A.sc = c->dsc;
     A.cap = static_cast<uint32_t>(c->lpairs_cap);
-    A.nd_com = c->nd_com;
-    A.nd_aux = c->nd_aux;
-    A.nd_mom = c->nd_mom;
+    A.feat = reinterpret_cast<const uint2*>(c->lfeat);
+    A.scale = c->lscale;
     A.w16 = c->lw16;
     A.w32 = c->lw32;
-    A.eps = static_cast<float>(std::sqrt(p.eps2));
     A.corr = c->lcorr;
-    A.scw = c->dsc;
-    hipLaunchKernelGGL(k_learn_infer, dim3(256 * 8), dim3(64 * kInferWaves), 0, c->stream, A);
+    hipLaunchKernelGGL(k_learn_infer, dim3(256), dim3(64 * kInferWaves), 0, c->stream, A);
     hipLaunchKernelGGL(k_learn_apply, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->lcorr, c->leaf_of,
                        c->perm, c->n, c->ax, c->ay, c->az);
 }

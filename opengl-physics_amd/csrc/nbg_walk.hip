@@ -704,22 +704,34 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 }
                 f_tail += __popcll(bm);
             }
-            // learned correction: list (leaf, cell) for the band pairs of leaves owned here
-            if (LEARN) {
+            // learned correction: list (leaf, cell) for the band pairs of leaves owned here,
+            // leaf-major within the step (one ballot per leaf), so that the consumer finds each
+            // leaf's pairs in runs and sums a run before one atomic update
+            if (LEARN && ballot((band_m & own_m) != 0)) {
                 const uint64_t em = band_m & own_m;
                 const int ne = __popcll(em);
                 const int incl = wave_incl_scan(ne);
                 const int tot = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
-                if (tot > 0) {
-                    uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(&A.sc->learn_pairs, static_cast<uint32_t>(tot));
-                    base = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(base)));
-                    uint32_t w = base + static_cast<uint32_t>(incl - ne);
-                    for (uint64_t t = em; t; t &= t - 1) {
-                        const int jj = __ffsll(static_cast<long long>(t)) - 1;
-                        if (w < A.lcap) A.lpairs[w] = make_int2(lid[jj], node);
-                        ++w;
+                uint32_t ulo = static_cast<uint32_t>(em), uhi = static_cast<uint32_t>(em >> 32);
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) {
+                    ulo |= static_cast<uint32_t>(__shfl_xor(static_cast<int>(ulo), o, 64));
+                    uhi |= static_cast<uint32_t>(__shfl_xor(static_cast<int>(uhi), o, 64));
+                }
+                const uint64_t used = (static_cast<uint64_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(uhi))) << 32) |
+                                      static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(ulo)));
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&A.sc->learn_pairs, static_cast<uint32_t>(tot));
+                uint32_t w = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(base)));
+                for (uint64_t t = used; t; t &= t - 1) {
+                    const int jj = __ffsll(static_cast<long long>(t)) - 1;
+                    const bool mine = (em >> jj) & 1ull;
+                    const uint64_t m = ballot(mine);
+                    if (mine) {
+                        const uint32_t idx = w + static_cast<uint32_t>(mbcnt(m));
+                        if (idx < A.lcap) A.lpairs[idx] = make_int2(lid[jj], node);
                     }
+                    w += static_cast<uint32_t>(__popcll(m));
                 }
             }
             // near leaves -> ring

@@ -79,18 +79,21 @@ def dsilu(z):
 
 def mlp_forward(blob, feats, emulate_bf16=False):
     """feats (n, >=13) raw features -> (y (n,3), cache).  With emulate_bf16 the inputs,
-    weights and hidden activations are rounded to bfloat16 as the matrix-core kernels do
-    (accumulation stays FP64 here, FP32 on the device)."""
+    weights and hidden activations are rounded to bfloat16 as the matrix-core kernels do,
+    and the activation takes the BF16-rounded pre-activation (accumulation stays FP64
+    here, FP32 on the device; the backward uses SiLU' of the unrounded pre-activation)."""
     P = unpack(blob)
     r = bf16 if emulate_bf16 else (lambda v: np.asarray(v, np.float64))
     x = r((np.asarray(feats, np.float64)[:, :NF] - P["mu"][:NF]) * P["isd"][:NF])
     W = {k: r(P[k]) for k in ("W1", "W2", "W3", "W4")}
+    # the device activation is SiLU of the BF16-rounded pre-activation, rounded to BF16
+    act = (lambda z: bf16(silu(bf16(z)))) if emulate_bf16 else silu
     z1 = x @ W["W1"] + P["b1"]
-    h1 = r(silu(z1))
+    h1 = act(z1)
     z2 = h1 @ W["W2"] + P["b2"]
-    h2 = r(silu(z2))
+    h2 = act(z2)
     z3 = h2 @ W["W3"] + P["b3"]
-    h3 = r(silu(z3))
+    h3 = act(z3)
     y = h3 @ W["W4"] + P["b4"]
     return y, dict(x=x, z1=z1, h1=h1, z2=z2, h2=h2, z3=z3, h3=h3, W=W)
 
