@@ -8,7 +8,7 @@ step: tree rebuilt from scratch (keys, sort, build, upsweep), group walk, two ki
 drift, replayed from a captured hipGraph.  Two variants are timed on the same state:
   plain    Barnes-Hut theta=0.5
   learned  Barnes-Hut theta=0.8 plus the learned far-field correction on the accepted
-           (leaf, cell) pairs with size/(distance - group radius) > 0.6
+           (leaf, cell) pairs with size/(distance - group radius) > 0.62
 Each variant's force error is its relative RMS against the direct sum on 16,384 random
 targets of the initial state.
 
@@ -124,7 +124,7 @@ def main():
                          "metric); plain: plain Barnes-Hut at --theta only")
     ap.add_argument("--theta", type=float, default=0.5, help="plain Barnes-Hut opening angle")
     ap.add_argument("--learned-theta", type=float, default=0.8)
-    ap.add_argument("--learn-band", type=float, default=0.6)
+    ap.add_argument("--learn-band", type=float, default=0.62)
     ap.add_argument("--eps", type=float, default=0.005)
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--multipole", default="quadrupole", choices=["monopole", "quadrupole"])

@@ -107,8 +107,7 @@ void enqueue_bh(nbg_ctx* c, const nbg_params& p, int levels, bool timed, bool li
     launch_build(c, levels);
     launch_gather(c, static_cast<float>(p.G));
     if (timed) rec(c, 3);
-    launch_upsweep(c, levels, p.multipole, p.G);
-    if (learn) launch_learn_moments(c, levels);
+    launch_upsweep(c, levels, p.multipole, p.G, learn);
     if (timed) rec(c, 4);
     if ((p.flags & NBG_FLAG_COUNT) && c->cnt_far) {
         launch_fill_i32(c, c->cnt_far, c->cap_nodes, 0);

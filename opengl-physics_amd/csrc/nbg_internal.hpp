@@ -202,7 +202,7 @@ namespace nbg {
 void launch_bbox_and_keys(nbg_ctx* c, int levels);
 void launch_sort(nbg_ctx* c, int levels);
 void launch_build(nbg_ctx* c, int levels);
-void launch_upsweep(nbg_ctx* c, int levels, int multipole, double G);
+void launch_upsweep(nbg_ctx* c, int levels, int multipole, double G, bool moments);
 void launch_fill_i32(nbg_ctx* c, int32_t* p, int64_t n, int32_t v);
 // walk.hip
 void launch_walk(nbg_ctx* c, const nbg_params& p);
@@ -217,7 +217,6 @@ int64_t diag_partials_needed(int64_t n);
 // learn.hip
 int learn_alloc(nbg_ctx* c);
 int learn_grow_pairs(nbg_ctx* c, int64_t need);
-void launch_learn_moments(nbg_ctx* c, int levels);
 void launch_learn_zero(nbg_ctx* c);
 void launch_learn_correct(nbg_ctx* c, const nbg_params& p);
 void launch_learn_samples(nbg_ctx* c, const nbg_params& p, int64_t npairs, int64_t nnew, int64_t dst);

@@ -134,95 +134,6 @@ __device__ __forceinline__ void pair_features(float4 tgt, float4 cm, float size2
     scale = M * (s * size2) * (ir2 * ir2 * ir);
 }
 
-// ---------------------------------------------------------------------------- moments
-// Second and third moments about the FP32 com with G*m weights, packed as
-//   [xx yy zz xy] [xz yz xxx yyy] [zzz xxy xxz xyy] [yyz xzz yzz xyz]
-__global__ __launch_bounds__(256) void k_mom_leaf(const int32_t* __restrict__ leaf_list,
-                                                  const int32_t* __restrict__ nd_start,
-                                                  const int32_t* __restrict__ nd_count,
-                                                  const float4* __restrict__ spos, const float4* __restrict__ nd_com,
-                                                  const DevScalars* __restrict__ sc, float4* __restrict__ nd_mom) {
-    const int nl = sc->n_leaves;
-    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nl; li += gridDim.x * blockDim.x) {
-        const int id = leaf_list[li];
-        const int st = nd_start[id], cnt = nd_count[id];
-        const float4 c = nd_com[id];
-        float a[16] = {};
-        if (c.w > 0.f) {
-            for (int k = st; k < st + cnt; ++k) {
-                const float4 p = spos[k];
-                const float dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z, w = p.w;
-                a[0] += w * dx * dx;
-                a[1] += w * dy * dy;
-                a[2] += w * dz * dz;
-                a[3] += w * dx * dy;
-                a[4] += w * dx * dz;
-                a[5] += w * dy * dz;
-                a[6] += w * dx * dx * dx;
-                a[7] += w * dy * dy * dy;
-                a[8] += w * dz * dz * dz;
-                a[9] += w * dx * dx * dy;
-                a[10] += w * dx * dx * dz;
-                a[11] += w * dx * dy * dy;
-                a[12] += w * dy * dy * dz;
-                a[13] += w * dx * dz * dz;
-                a[14] += w * dy * dz * dz;
-                a[15] += w * dx * dy * dz;
-            }
-        }
-        nd_mom[4 * id + 0] = make_float4(a[0], a[1], a[2], a[3]);
-        nd_mom[4 * id + 1] = make_float4(a[4], a[5], a[6], a[7]);
-        nd_mom[4 * id + 2] = make_float4(a[8], a[9], a[10], a[11]);
-        nd_mom[4 * id + 3] = make_float4(a[12], a[13], a[14], a[15]);
-    }
-}
-
-__global__ __launch_bounds__(256) void k_mom_internal(int depth, const int32_t* __restrict__ depth_nodes,
-                                                      const int32_t* __restrict__ nd_child,
-                                                      const float4* __restrict__ nd_com,
-                                                      const DevScalars* __restrict__ sc, float4* __restrict__ nd_mom) {
-    const int cnt = static_cast<int>(sc->depth_count[depth]);
-    const int off = static_cast<int>(sc->depth_offset[depth]);
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
-        const int id = depth_nodes[off + k];
-        const float4 p = nd_com[id];
-        float a[16] = {};
-        if (p.w > 0.f) {
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const int ch = nd_child[8 * id + c];
-                if (ch < 0) continue;
-                const float4 cc = nd_com[ch];
-                if (!(cc.w > 0.f)) continue;
-                const float4 q0 = nd_mom[4 * ch + 0], q1 = nd_mom[4 * ch + 1], q2 = nd_mom[4 * ch + 2],
-                             q3 = nd_mom[4 * ch + 3];
-                const float M = cc.w, dx = cc.x - p.x, dy = cc.y - p.y, dz = cc.z - p.z;
-                const float sxx = q0.x, syy = q0.y, szz = q0.z, sxy = q0.w, sxz = q1.x, syz = q1.y;
-                a[0] += sxx + M * dx * dx;
-                a[1] += syy + M * dy * dy;
-                a[2] += szz + M * dz * dz;
-                a[3] += sxy + M * dx * dy;
-                a[4] += sxz + M * dx * dz;
-                a[5] += syz + M * dy * dz;
-                a[6] += q1.z + 3.f * sxx * dx + M * dx * dx * dx;
-                a[7] += q1.w + 3.f * syy * dy + M * dy * dy * dy;
-                a[8] += q2.x + 3.f * szz * dz + M * dz * dz * dz;
-                a[9] += q2.y + sxx * dy + 2.f * sxy * dx + M * dx * dx * dy;
-                a[10] += q2.z + sxx * dz + 2.f * sxz * dx + M * dx * dx * dz;
-                a[11] += q2.w + syy * dx + 2.f * sxy * dy + M * dx * dy * dy;
-                a[12] += q3.x + syy * dz + 2.f * syz * dy + M * dy * dy * dz;
-                a[13] += q3.y + szz * dx + 2.f * sxz * dz + M * dx * dz * dz;
-                a[14] += q3.z + szz * dy + 2.f * syz * dz + M * dy * dz * dz;
-                a[15] += q3.w + sxy * dz + sxz * dy + syz * dx + M * dx * dy * dz;
-            }
-        }
-        nd_mom[4 * id + 0] = make_float4(a[0], a[1], a[2], a[3]);
-        nd_mom[4 * id + 1] = make_float4(a[4], a[5], a[6], a[7]);
-        nd_mom[4 * id + 2] = make_float4(a[8], a[9], a[10], a[11]);
-        nd_mom[4 * id + 3] = make_float4(a[12], a[13], a[14], a[15]);
-    }
-}
-
 // ---------------------------------------------------------------------------- weights
 // The per-lane weight fragments of the whole network (A operands) and its biases (as
 // C initial values), held in registers for the life of a wave.
@@ -285,9 +196,10 @@ struct ActDirect {  // computed (training)
         return h;
     }
 };
-// Inference: the same function, half of the row blocks looked up in a 64K-entry LDS table
-// and half computed, so the LDS pipe (bank-conflicted lookups) and the transcendental VALU
-// work proceed side by side; both give identical bits.
+// Inference: the same function, three of the four row blocks looked up in a 64K-entry LDS
+// table and one computed, so the LDS pipe (bank-conflicted lookups) and the transcendental
+// VALU work proceed side by side (measured best of 1/4, 1/2, 3/4, 4/4 looked up); both
+// give identical bits.
 struct ActTable {
     const uint16_t* tab;
     __device__ __forceinline__ bf16x4 lookup(f32x4 z) const {
@@ -304,7 +216,7 @@ struct ActTable {
         const uint32_t r0 = bf16x2_rne(s0, s1), r1 = bf16x2_rne(s2, s3);
         return __builtin_bit_cast(bf16x4, (static_cast<uint64_t>(r1) << 32) | r0);
     }
-    __device__ __forceinline__ bf16x4 operator()(f32x4 z, int mb) const { return (mb & 1) ? compute(z) : lookup(z); }
+    __device__ __forceinline__ bf16x4 operator()(f32x4 z, int mb) const { return (mb == 3) ? compute(z) : lookup(z); }
 };
 
 // One 16-pair tile through the network.  x: this lane's B fragment of the standardised
@@ -1156,16 +1068,6 @@ int learn_grow_pairs(nbg_ctx* c, int64_t need) {
     }
     c->lpairs_cap = cap;
     return NBG_OK;
-}
-
-void launch_learn_moments(nbg_ctx* c, int levels) {
-    hipStream_t st = c->stream;
-    hipLaunchKernelGGL(k_mom_leaf, dim3(2048), dim3(256), 0, st, c->leaf_list, c->nd_start, c->nd_count, c->spos,
-                       c->nd_com, c->dsc, c->nd_mom);
-    const int top = (levels == kLevelsShort) ? kLevelsShort - 1 : kMaxDepth - 1;
-    for (int d = top; d >= 0; --d)
-        hipLaunchKernelGGL(k_mom_internal, dim3(d < 6 ? 64 : 1024), dim3(256), 0, st, d, c->depth_nodes,
-                           c->nd_child, c->nd_com, c->dsc, c->nd_mom);
 }
 
 void launch_learn_zero(nbg_ctx* c) {

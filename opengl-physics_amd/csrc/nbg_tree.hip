@@ -758,7 +758,7 @@ void launch_build(nbg_ctx* c, int levels) {
 // never uses it as a source, and a massless group then opens every cell, as in the
 // reference.
 
-template <bool QUAD>
+template <bool QUAD, bool MOM>
 __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ leaf_list,
                                                  const int32_t* __restrict__ nd_start,
                                                  const int32_t* __restrict__ nd_count,
@@ -766,7 +766,8 @@ __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ lea
                                                  const uint32_t* __restrict__ perm, double G,
                                                  const DevScalars* __restrict__ sc, float4* __restrict__ nd_com,
                                                  float4* __restrict__ nd_quad, float4* __restrict__ nd_aux,
-                                                 double4* __restrict__ nd_com64, double* __restrict__ nd_rmax64) {
+                                                 double4* __restrict__ nd_com64, double* __restrict__ nd_rmax64,
+                                                 float4* __restrict__ nd_mom) {
     const int nl = sc->n_leaves;
     for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nl; li += gridDim.x * blockDim.x) {
         const int id = leaf_list[li];
@@ -805,6 +806,37 @@ __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ lea
                 }
             }
         }
+        if (MOM) {
+            // second and third moments about the FP32 com, G*m weights (learned correction):
+            // [xx yy zz xy] [xz yz xxx yyy] [zzz xxy xxz xyy] [yyz xzz yzz xyz]
+            float mo[16] = {};
+            if (M > 0.0) {
+                for (int k = st; k < st + cnt; ++k) {
+                    const float4 p = spos[k];
+                    const float dx = p.x - fx, dy = p.y - fy, dz = p.z - fz, w = p.w;
+                    mo[0] += w * dx * dx;
+                    mo[1] += w * dy * dy;
+                    mo[2] += w * dz * dz;
+                    mo[3] += w * dx * dy;
+                    mo[4] += w * dx * dz;
+                    mo[5] += w * dy * dz;
+                    mo[6] += w * dx * dx * dx;
+                    mo[7] += w * dy * dy * dy;
+                    mo[8] += w * dz * dz * dz;
+                    mo[9] += w * dx * dx * dy;
+                    mo[10] += w * dx * dx * dz;
+                    mo[11] += w * dx * dy * dy;
+                    mo[12] += w * dy * dy * dz;
+                    mo[13] += w * dx * dz * dz;
+                    mo[14] += w * dy * dz * dz;
+                    mo[15] += w * dx * dy * dz;
+                }
+            }
+            nd_mom[4 * id + 0] = make_float4(mo[0], mo[1], mo[2], mo[3]);
+            nd_mom[4 * id + 1] = make_float4(mo[4], mo[5], mo[6], mo[7]);
+            nd_mom[4 * id + 2] = make_float4(mo[8], mo[9], mo[10], mo[11]);
+            nd_mom[4 * id + 3] = make_float4(mo[12], mo[13], mo[14], mo[15]);
+        }
         nd_com64[id] = make_double4(gx, gy, gz, M);
         nd_rmax64[id] = rmax;
         nd_com[id] = make_float4(fx, fy, fz, static_cast<float>(G * M));
@@ -816,12 +848,13 @@ __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ lea
     }
 }
 
-template <bool QUAD>
+template <bool QUAD, bool MOM>
 __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* __restrict__ depth_nodes,
                                                      const int32_t* __restrict__ nd_child,
                                                      double G, const DevScalars* __restrict__ sc,
                                                      float4* __restrict__ nd_com, float4* __restrict__ nd_quad,
-                                                     float4* __restrict__ nd_aux, double4* __restrict__ nd_com64) {
+                                                     float4* __restrict__ nd_aux, double4* __restrict__ nd_com64,
+                                                     float4* __restrict__ nd_mom) {
     const int cnt = static_cast<int>(sc->depth_count[depth]);
     const int off = static_cast<int>(sc->depth_offset[depth]);
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
@@ -849,6 +882,7 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
         }
         const float fx = static_cast<float>(gx), fy = static_cast<float>(gy), fz = static_cast<float>(gz);
         float qxx = 0.f, qyy = 0.f, qxy = 0.f, qxz = 0.f, qyz = 0.f, rmax = 0.f;
+        float mo[16] = {};
         if (M > 0.0) {
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
@@ -858,6 +892,29 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
                 const float dx = cc.x - fx, dy = cc.y - fy, dz = cc.z - fz;
                 const float d2 = dx * dx + dy * dy + dz * dz;
                 rmax = fmaxf(rmax, ca.z + sqrtf(d2));
+                if (MOM && cc.w > 0.f) {
+                    // parallel-axis shift of the child's moments to this com
+                    const float4 q0 = nd_mom[4 * ch[c] + 0], q1 = nd_mom[4 * ch[c] + 1], q2 = nd_mom[4 * ch[c] + 2],
+                                 q3 = nd_mom[4 * ch[c] + 3];
+                    const float m = cc.w;
+                    const float sxx = q0.x, syy = q0.y, szz = q0.z, sxy = q0.w, sxz = q1.x, syz = q1.y;
+                    mo[0] += sxx + m * dx * dx;
+                    mo[1] += syy + m * dy * dy;
+                    mo[2] += szz + m * dz * dz;
+                    mo[3] += sxy + m * dx * dy;
+                    mo[4] += sxz + m * dx * dz;
+                    mo[5] += syz + m * dy * dz;
+                    mo[6] += q1.z + 3.f * sxx * dx + m * dx * dx * dx;
+                    mo[7] += q1.w + 3.f * syy * dy + m * dy * dy * dy;
+                    mo[8] += q2.x + 3.f * szz * dz + m * dz * dz * dz;
+                    mo[9] += q2.y + sxx * dy + 2.f * sxy * dx + m * dx * dx * dy;
+                    mo[10] += q2.z + sxx * dz + 2.f * sxz * dx + m * dx * dx * dz;
+                    mo[11] += q2.w + syy * dx + 2.f * sxy * dy + m * dx * dy * dy;
+                    mo[12] += q3.x + syy * dz + 2.f * syz * dy + m * dy * dy * dz;
+                    mo[13] += q3.y + szz * dx + 2.f * sxz * dz + m * dx * dz * dz;
+                    mo[14] += q3.z + szz * dy + 2.f * syz * dz + m * dy * dz * dz;
+                    mo[15] += q3.w + sxy * dz + sxz * dy + syz * dx + m * dx * dy * dz;
+                }
                 if (QUAD && cc.w > 0.f) {
                     const float4 cq = nd_quad[ch[c]];
                     qxx += cq.x + cc.w * (3.f * dx * dx - d2);
@@ -867,6 +924,12 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
                     qyz += ca.x + cc.w * 3.f * dy * dz;
                 }
             }
+        }
+        if (MOM) {
+            nd_mom[4 * id + 0] = make_float4(mo[0], mo[1], mo[2], mo[3]);
+            nd_mom[4 * id + 1] = make_float4(mo[4], mo[5], mo[6], mo[7]);
+            nd_mom[4 * id + 2] = make_float4(mo[8], mo[9], mo[10], mo[11]);
+            nd_mom[4 * id + 3] = make_float4(mo[12], mo[13], mo[14], mo[15]);
         }
         nd_com64[id] = make_double4(gx, gy, gz, M);
         nd_com[id] = make_float4(fx, fy, fz, static_cast<float>(G * M));
@@ -878,28 +941,36 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
     }
 }
 
-void launch_upsweep(nbg_ctx* c, int levels, int multipole, double G) {
+void launch_upsweep(nbg_ctx* c, int levels, int multipole, double G, bool moments) {
     hipStream_t st = c->stream;
     const int gnodes = 2048;
     const bool quad = multipole == NBG_QUADRUPOLE;
-    if (quad)
-        hipLaunchKernelGGL(k_up_leaf<true>, dim3(gnodes), dim3(256), 0, st, c->leaf_list, c->nd_start, c->nd_count,
-                           c->spos, c->m, c->perm, G, c->dsc, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64,
-                           c->nd_rmax64);
+    const bool mom = moments && c->nd_mom != nullptr;
+#define NBG_UPLEAF(Q, M)                                                                                             \
+    hipLaunchKernelGGL((k_up_leaf<Q, M>), dim3(gnodes), dim3(256), 0, st, c->leaf_list, c->nd_start, c->nd_count,     \
+                       c->spos, c->m, c->perm, G, c->dsc, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64,            \
+                       c->nd_rmax64, c->nd_mom)
+    if (mom)
+        NBG_UPLEAF(true, true);
+    else if (quad)
+        NBG_UPLEAF(true, false);
     else
-        hipLaunchKernelGGL(k_up_leaf<false>, dim3(gnodes), dim3(256), 0, st, c->leaf_list, c->nd_start, c->nd_count,
-                           c->spos, c->m, c->perm, G, c->dsc, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64,
-                           c->nd_rmax64);
+        NBG_UPLEAF(false, false);
+#undef NBG_UPLEAF
     // internal nodes sit at depths 0..levels-1 (depth-40 cells are always leaves)
     const int top = (levels == kLevelsShort) ? kLevelsShort - 1 : kMaxDepth - 1;
     for (int d = top; d >= 0; --d) {
         const int gd = d < 6 ? 64 : 1024;
-        if (quad)
-            hipLaunchKernelGGL(k_up_internal<true>, dim3(gd), dim3(256), 0, st, d, c->depth_nodes, c->nd_child, G,
-                               c->dsc, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64);
+#define NBG_UPINT(Q, M)                                                                                              \
+    hipLaunchKernelGGL((k_up_internal<Q, M>), dim3(gd), dim3(256), 0, st, d, c->depth_nodes, c->nd_child, G, c->dsc, \
+                       c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64, c->nd_mom)
+        if (mom)
+            NBG_UPINT(true, true);
+        else if (quad)
+            NBG_UPINT(true, false);
         else
-            hipLaunchKernelGGL(k_up_internal<false>, dim3(gd), dim3(256), 0, st, d, c->depth_nodes, c->nd_child, G,
-                               c->dsc, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64);
+            NBG_UPINT(false, false);
+#undef NBG_UPINT
     }
 }
 
