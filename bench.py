@@ -113,10 +113,100 @@ def force_error(ctx, pynbg, ic, base, targets, ref):
     return float(np.sqrt(np.sum((a - ref) ** 2) / np.sum(ref ** 2)))
 
 
+def ensemble_main(args):
+    """Batched ensemble: 512 Plummer/King(W0=6) clusters, N log-uniform in [1024, 65536]
+    (mt19937 seed 0), total mass 1 each, eps=0.01, theta=0.6, quadrupole, dt=1e-3, one
+    captured hipGraph per step over all clusters, tile sizes chosen by nbg_autotune."""
+    import pynbg
+    steps = args.steps if args.steps_given else 200
+    log("generating the 512-cluster ensemble")
+    off, st, kinds = pynbg.ensemble(512, 0, 1024, 65536, 6.0)
+    n = int(off[-1])
+    nsys = len(off) - 1
+    kahan = not args.no_kahan
+    base = dict(multipole="quadrupole", G=1.0, eps=0.01, theta=0.6, dt=1e-3, kahan=kahan)
+    p = pynbg.make_params("barnes_hut", stage_times=True, **base)
+    pc = pynbg.make_params("barnes_hut", count=True, **base)
+    with pynbg.Context(n) as ctx:
+        ctx.upload_batch(off, st["x"], st["y"], st["z"], st["m"], st["vx"], st["vy"], st["vz"])
+        best, cand_ms = ctx.autotune(pynbg.make_params("barnes_hut", **base), reps=3)
+        log(f"autotune: {best.as_dict()} candidates ms {np.round(cand_ms, 3).tolist()}")
+        e0 = np.array([ctx.diagnostics_system(p, k)["energy"] for k in range(nsys)])
+        c0 = ctx.accel(pc)  # System::Prime, with interaction counting
+        if args.warmup > 0:
+            ctx.step(p, args.warmup)
+        t0 = time.perf_counter()
+        stt = ctx.step(p, steps)
+        t1 = time.perf_counter()
+        e1 = np.array([ctx.diagnostics_system(p, k)["energy"] for k in range(nsys)])
+        c1 = ctx.accel(pc)
+    wall = t1 - t0
+    n_p2p = 0.5 * (c0.n_p2p + c1.n_p2p)
+    n_m2p = 0.5 * (c0.n_m2p + c1.n_m2p)
+    flops = FLOP_P2P * n_p2p + FLOP_M2P_QUAD * n_m2p
+    walk_ms = stt.ms_walk / steps
+    ach = flops / (walk_ms * 1e-3) / 1e12
+    drift = np.abs(e1 - e0) / np.abs(e0)
+    stage = {k: round(getattr(stt, "ms_" + k) / steps, 4)
+             for k in ("keys", "sort", "build", "upsweep", "walk", "integrate")}
+    log(f"{steps} steps: {1e3 * wall / steps:.3f} ms/step, stages {stage}, max |dE/E| {drift.max():.2e}")
+    out = {
+        "metric": "s/step, batched ensemble of 512 clusters (one captured hipGraph per step)",
+        "value": round(wall / steps, 6), "unit": "s/step", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / steps, 4), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (512 clusters: ic::Plummer<3> / ic::King<3>(W0=6) alternating, cluster k seed k)",
+        "config": {"workload": "ensemble_512_plummer_king", "clusters": nsys, "n_total": n,
+                   "n_min": int(np.diff(off).min()), "n_max": int(np.diff(off).max()), "theta": 0.6, "eps": 0.01,
+                   "dt": 1e-3, "multipole": "quadrupole", "kahan": kahan},
+        "particle_steps_per_s": round(n * steps / wall, 1),
+        "tuning": best.as_dict(),
+        "autotune_candidates_ms": {"bin_near4": cand_ms[0], "bin_near8": cand_ms[1], "bin_near16": cand_ms[2],
+                                   "leaf_groups": cand_ms[3]} if len(cand_ms) == 4 else cand_ms.tolist(),
+        "energy_drift": {"max": float(drift.max()), "median": float(np.median(drift)),
+                         "over_steps": steps + args.warmup},
+        "roofline": {"bound": "mfma", "pipe": "fp32-valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "kernel": "k_walk_bins", "kernel_ms": round(walk_ms, 4), "flop_per_launch": flops},
+        "interactions_per_step": {"p2p": n_p2p, "m2p": n_m2p},
+        "interactions_per_s": round((n_p2p + n_m2p) / (walk_ms * 1e-3), 1),
+        "stage_ms_per_step": stage,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = ensemble_cpu_baseline(off, st)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def ensemble_cpu_baseline(off, st, nclusters=64):
+    """One KDK step of the first `nclusters` clusters on the FP64 oracle."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    tot = 0.0
+    npart = 0
+    for k in range(nclusters):
+        sl = slice(off[k], off[k + 1])
+        s = {q: np.asarray(st[q][sl], np.float64) for q in st}
+        T = oracle.Tree(s["x"], s["y"], s["z"], s["m"], quad=True, fast=True)
+        a0 = T.accel_bh(1.0, 0.01 ** 2, 0.6)
+        del T
+        t0 = time.perf_counter()
+        oracle.leapfrog(s, a0, 1.0, 0.01 ** 2, 0.6, 1e-3, 1, quad=True, fast=True)
+        tot += time.perf_counter() - t0
+        npart += off[k + 1] - off[k]
+    return {"value": round(tot, 4), "unit": "s/step", "cores": threads, "kind": "port",
+            "sample": f"one KDK step of the first {nclusters} clusters ({npart} particles, "
+                      f"{npart / off[-1]:.1%} of the ensemble) on the FP64 oracle, OpenMP"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--workload", default="plummer_1m", choices=["plummer_1m", "ensemble"],
+                    help="plummer_1m: the headline learned/plain comparison; ensemble: 512 clusters")
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--variant", default="learned", choices=["learned", "plain"],
@@ -132,11 +222,17 @@ def main():
     ap.add_argument("--error-sample", type=int, default=16384, help="targets of the direct-sum error check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args()
+    args.steps_given = args.steps is not None
+    if args.steps is None:
+        args.steps = 10
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU decomposition is "
               f"design-only in this build", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble":
+        ensemble_main(args)
+        return
     if args.variant == "learned" and args.multipole != "quadrupole":
         print("the learned variant needs quadrupoles", file=sys.stderr)
         sys.exit(2)

@@ -162,6 +162,33 @@ int nbg_diagnostics(nbg_ctx* ctx, const nbg_params* p, int32_t with_potential, d
 int nbg_accel_direct_targets(nbg_ctx* ctx, const nbg_params* p, int64_t nt, const int64_t* targets, float* ax,
                              float* ay, float* az);
 
+/* Launch choices of the force kernels, set directly or by nbg_autotune.  No reference
+ * counterpart: the reference's GL kernels have fixed work-group sizes
+ * (projects/nbody_core/src/gpu/GpuBarnesHut.cpp, GpuDirect.cpp). */
+typedef struct nbg_tuning {
+    int32_t walk_kernel; /* 0: bin walk (one wavefront per 64 sorted targets), 1: one wavefront
+                            per leaf group (no learned-pair listing; learned runs use the bin walk) */
+    int32_t near_tile;   /* bin walk: near-field P2P sources loaded per batch: 4, 8 or 16 */
+    int32_t direct_tile; /* direct sum: sources loaded per batch: 2, 4 or 8 */
+    int32_t reserved;    /* 0 */
+} nbg_tuning;
+
+int nbg_tuning_get(nbg_ctx* ctx, nbg_tuning* t);
+/* Drops the captured step graph (it holds the previous kernels). */
+int nbg_tuning_set(nbg_ctx* ctx, const nbg_tuning* t);
+
+/* Time every candidate tuning of p->solver's force kernel on the uploaded state (one
+ * untimed and `reps` timed evaluations each, HIP-event kernel time), keep the fastest in
+ * the context and return it in *best.  ms (nullable, cap entries) receives the mean time
+ * of each candidate in candidate order, *n_out the number of candidates.  Candidates: bin
+ * walk with near_tile 4/8/16 and the leaf-group walk (Barnes-Hut), direct_tile 2/4/8
+ * (direct).  The accelerations are overwritten with those of the last evaluation. */
+int nbg_autotune(nbg_ctx* ctx, const nbg_params* p, int32_t reps, nbg_tuning* best, double* ms, int32_t cap,
+                 int32_t* n_out);
+
+/* nbg_diagnostics for system `sys` of a batch (nbg_upload_batch). */
+int nbg_diagnostics_system(nbg_ctx* ctx, const nbg_params* p, int32_t sys, int32_t with_potential, double* out);
+
 /* ---------------------------------------------------------------------------------
  * Learned far-field correction.  No reference counterpart: the reference has no learned
  * component.  A multilayer perceptron (13 inputs, three hidden layers of 64 with SiLU,

@@ -24,6 +24,13 @@ int ngh_ic_hernquist(int n, uint32_t seed, double total_mass, double scale_radiu
 int ngh_ic_king(int n, uint32_t seed, double total_mass, double core_radius, double w0, double G, double* x,
                 double* y, double* z, double* vx, double* vy, double* vz, double* m);
 
+/* Batched-ensemble layout (no reference counterpart: the reference runs one System).
+ * N_k = round(exp(U[ln nmin, ln nmax])) with U drawn from std::mt19937(seed) through
+ * std::uniform_real_distribution<double>, k = 0..nclusters-1; kinds[k] = k % 2 (0 Plummer,
+ * 1 King).  kinds may be NULL.  Returns the total particle count, or -1 on a bad argument. */
+int64_t ngh_ensemble_sizes(int32_t nclusters, uint32_t seed, int64_t nmin, int64_t nmax, int64_t* sizes,
+                           int32_t* kinds);
+
 /* Deck-driven batch run on the GPU backend: DeckSim::Configure + fw::RunHeadless +
  * fw::OutputWriter (projects/nbody_core/src/DeckSim.cpp:26-169,
  * framework/src/HeadlessRunner.cpp:11-41, framework/src/OutputWriter.cpp:38-169).

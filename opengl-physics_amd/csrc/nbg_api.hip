@@ -333,7 +333,7 @@ nbg_ctx* nbg_create(int device, int64_t max_n) {
         return nullptr;
     }
     c->cap_n = max_n;
-    if (const char* w = std::getenv("NBG_WALK_KERNEL")) c->walk_kernel = (std::string(w) == "leaf") ? 1 : 0;
+    if (const char* w = std::getenv("NBG_WALK_KERNEL")) c->tune.walk_kernel = (std::string(w) == "leaf") ? 1 : 0;
     c->cap_nodes = max_n + (40 * max_n + 8) / 9 + 64;
     const size_t N = static_cast<size_t>(max_n);
     const size_t C = static_cast<size_t>(c->cap_nodes);
@@ -752,7 +752,7 @@ int nbg_diagnostics(nbg_ctx* c, const nbg_params* p, int32_t with_potential, dou
     if (!c || !out) return NBG_E_INVALID;
     if (int r = validate(c, p)) return r;
     NBG_HIP_CHECK(c, hipSetDevice(c->device));
-    int r = diagnostics(c, *p, with_potential, out);
+    int r = diagnostics(c, *p, with_potential, out, 0);
     if (r == NBG_OK) r = check_launch(c);
     return r;
 }
@@ -1099,6 +1099,101 @@ int nbg_learn_grad(nbg_ctx* c, int64_t n, const float* feats, const float* targe
     }
     if (loss) *loss = lsum / static_cast<float>(n);
     return NBG_OK;
+}
+
+// ---------------------------------------------------------------------------- tuning
+
+namespace {
+int check_tuning(nbg_ctx* c, const nbg_tuning& t) {
+    const bool ok = (t.walk_kernel == 0 || t.walk_kernel == 1) &&
+                    (t.near_tile == 4 || t.near_tile == 8 || t.near_tile == 16) &&
+                    (t.direct_tile == 2 || t.direct_tile == 4 || t.direct_tile == 8);
+    if (!ok) {
+        c->set_error("tuning: walk_kernel 0|1, near_tile 4|8|16, direct_tile 2|4|8");
+        return NBG_E_INVALID;
+    }
+    return NBG_OK;
+}
+}  // namespace
+
+int nbg_tuning_get(nbg_ctx* c, nbg_tuning* t) {
+    if (!c || !t) return NBG_E_INVALID;
+    *t = c->tune;
+    return NBG_OK;
+}
+
+int nbg_tuning_set(nbg_ctx* c, const nbg_tuning* t) {
+    if (!c || !t) return NBG_E_INVALID;
+    if (int r = check_tuning(c, *t)) return r;
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    drop_graph(c);  // the captured step holds the old kernels
+    c->tune = *t;
+    c->tune.reserved = 0;
+    return NBG_OK;
+}
+
+int nbg_autotune(nbg_ctx* c, const nbg_params* p, int32_t reps, nbg_tuning* best, double* ms, int32_t cap,
+                 int32_t* n_out) {
+    if (!c) return NBG_E_INVALID;
+    if (int r = validate(c, p)) return r;
+    if (reps < 1 || reps > 1000) {
+        c->set_error("reps must be in 1..1000");
+        return NBG_E_INVALID;
+    }
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    std::vector<nbg_tuning> cand;
+    const nbg_tuning cur = c->tune;
+    if (p->solver == NBG_SOLVER_DIRECT) {
+        for (int u : {2, 4, 8}) cand.push_back(nbg_tuning{cur.walk_kernel, cur.near_tile, u, 0});
+    } else {
+        for (int t : {4, 8, 16}) cand.push_back(nbg_tuning{0, t, cur.direct_tile, 0});
+        if (!(p->flags & NBG_FLAG_LEARNED)) cand.push_back(nbg_tuning{1, 8, cur.direct_tile, 0});
+    }
+    nbg_params q = *p;
+    q.flags &= ~NBG_FLAG_COUNT;
+    drop_graph(c);
+    double best_ms = 1e300;
+    nbg_tuning pick = cur;
+    for (size_t k = 0; k < cand.size(); ++k) {
+        c->tune = cand[k];
+        if (c->n == 0) break;
+        if (int r = accel_eager(c, q, nullptr)) {
+            c->tune = cur;
+            return r;
+        }
+        double sum = 0.0;
+        for (int r = 0; r < reps; ++r) {
+            nbg_stats st;
+            if (int e = accel_eager(c, q, &st)) {
+                c->tune = cur;
+                return e;
+            }
+            sum += st.ms_walk;  // the kernel the tuning changes
+        }
+        const double mean = sum / reps;
+        if (ms && static_cast<int32_t>(k) < cap) ms[k] = mean;
+        if (mean < best_ms) {
+            best_ms = mean;
+            pick = cand[k];
+        }
+    }
+    c->tune = pick;
+    if (best) *best = pick;
+    if (n_out) *n_out = static_cast<int32_t>(cand.size());
+    return NBG_OK;
+}
+
+int nbg_diagnostics_system(nbg_ctx* c, const nbg_params* p, int32_t sys, int32_t with_potential, double* out) {
+    if (!c || !out) return NBG_E_INVALID;
+    if (int r = validate(c, p)) return r;
+    if (sys < 0 || sys >= c->nsys) {
+        c->set_error("system index out of range");
+        return NBG_E_INVALID;
+    }
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    int r = diagnostics(c, *p, with_potential, out, sys);
+    if (r == NBG_OK) r = check_launch(c);
+    return r;
 }
 
 }  // extern "C"

@@ -141,9 +141,9 @@ int64_t diag_partials_needed(int64_t n) {
     return std::max<int64_t>(1024 * kDiagVals, grid_for(n, kDiagThreads) + 16);
 }
 
-int diagnostics(nbg_ctx* c, const nbg_params& p, int with_potential, double* out) {
-    const int lo = static_cast<int>(c->h_sys_off[0]);
-    const int hi = static_cast<int>(c->h_sys_off[1]);
+int diagnostics(nbg_ctx* c, const nbg_params& p, int with_potential, double* out, int sys) {
+    const int lo = static_cast<int>(c->h_sys_off[static_cast<size_t>(sys)]);
+    const int hi = static_cast<int>(c->h_sys_off[static_cast<size_t>(sys) + 1]);
     const int nb = 1024;
     hipLaunchKernelGGL(k_diag_moments, dim3(nb), dim3(kDiagThreads), 0, c->stream, c->x, c->y, c->z, c->vx, c->vy,
                        c->vz, c->m, lo, hi, c->diag_part);

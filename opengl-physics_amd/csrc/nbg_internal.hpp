@@ -168,7 +168,7 @@ struct nbg_ctx {
     nbg_params graph_params{};
     bool graph_valid = false;
     bool capturing = false;  // inside hipStreamBeginCapture: events become graph nodes
-    int walk_kernel = 0;     // 0: bin walk (one wave per 64 targets), 1: one wave per leaf group
+    nbg_tuning tune{0, 8, 4, 0};  // walk kernel, near P2P tile, direct P2P tile (autotuner)
 
     // learned far-field correction (allocated on first use)
     float4* nd_mom = nullptr;       // 4 per node: second (6) and third (10) moments about the FP32 com, G*m weights
@@ -212,7 +212,7 @@ void launch_direct_targets(nbg_ctx* c, const nbg_params& p, int64_t nt, const in
 void launch_kick(nbg_ctx* c, float hdt);
 void launch_kick_drift(nbg_ctx* c, float hdt, float dt);
 void launch_copy_f32(nbg_ctx* c, float* dst, const float* src, int64_t n);
-int diagnostics(nbg_ctx* c, const nbg_params& p, int with_potential, double* out);
+int diagnostics(nbg_ctx* c, const nbg_params& p, int with_potential, double* out, int sys);
 int64_t diag_partials_needed(int64_t n);
 // learn.hip
 int learn_alloc(nbg_ctx* c);

@@ -412,7 +412,7 @@ __device__ __forceinline__ int wave_excl_small(int v, int* total) {
     return off;
 }
 
-template <bool QUAD, bool KAHAN, bool COUNT, bool LEARN>
+template <bool QUAD, bool KAHAN, bool COUNT, bool LEARN, int NT>
 __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     __shared__ int s_snode[kBinWaves][kBinStack];
     __shared__ uint64_t s_smask[kBinWaves][kBinStack];
@@ -562,9 +562,10 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             kadd<KAHAN>(az, cz, sz);
         };
         // evaluates the particles of `n` staged near leaves starting at ring position `h`,
-        // packed eight at a time across leaf boundaries (a scalar cursor walks the leaves;
-        // the eight scalar loads of a group are issued before any arithmetic).  Unused
-        // slots of the last group read the zero pad after the sorted particles.
+        // packed NT at a time across leaf boundaries (a scalar cursor walks the leaves; the
+        // NT scalar loads of a group are issued before any arithmetic -- NT is the P2P tile,
+        // chosen by the autotuner).  Unused slots of the last group read the zero pad after
+        // the sorted particles.
         auto eval_near = [&](int h, int n) {
             float sx = 0.f, sy = 0.f, sz = 0.f;
             int e = 0, k = 0, st = 0, c = 0;
@@ -578,10 +579,10 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             if (n > 0) load_entry(0);
             int groups = 0;
             while (e < n) {
-                int idx[8];
-                float us[8];
+                int idx[NT];
+                float us[NT];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < NT; ++u) {
                     if (e < n) {
                         idx[u] = st + k;
                         us[u] = (em & mybit) ? 1.f : 0.f;
@@ -595,12 +596,12 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                         us[u] = 0.f;
                     }
                 }
-                float4 sp8[8];
+                float4 spt[NT];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) sp8[u] = cld(A.spos, idx[u]);
+                for (int u = 0; u < NT; ++u) spt[u] = cld(A.spos, idx[u]);
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const float4 q = sp8[u];
+                for (int u = 0; u < NT; ++u) {
+                    const float4 q = spt[u];
                     const float dx = q.x - tp.x, dy = q.y - tp.y, dz = q.z - tp.z;
                     const float r2 = dx * dx + dy * dy + dz * dz + eps2;
                     const float inv = __builtin_amdgcn_rsqf(r2);
@@ -610,7 +611,9 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                     sy += f * dy;
                     sz += f * dz;
                 }
-                if ((++groups & 7) == 0) {
+                // compensated flush every 64 sources
+                if (++groups == 64 / NT) {
+                    groups = 0;
                     kadd<KAHAN>(ax, cx, sx);
                     kadd<KAHAN>(ay, cy, sy);
                     kadd<KAHAN>(az, cz, sz);
@@ -884,32 +887,41 @@ void launch_walk_bins(nbg_ctx* c, const nbg_params& p) {
     A.lcap = static_cast<uint32_t>(c->lpairs_cap);
     // 8 XCDs x 32 CUs x 3 resident blocks
     const dim3 grid(8 * 32 * 3), block(64 * kBinWaves);
-#define NBG_WALKB(Q, K, C, L) hipLaunchKernelGGL((k_walk_bins<Q, K, C, L>), grid, block, 0, c->stream, A)
+#define NBG_WALKB(Q, K, C, L, T) hipLaunchKernelGGL((k_walk_bins<Q, K, C, L, T>), grid, block, 0, c->stream, A)
+    const int nt = c->tune.near_tile;
     if (learn) {
         // the learned correction is defined against the quadrupole expansion
         if (kahan) {
-            if (count) NBG_WALKB(true, true, true, true); else NBG_WALKB(true, true, false, true);
+            if (count) NBG_WALKB(true, true, true, true, 8); else NBG_WALKB(true, true, false, true, 8);
         } else {
-            if (count) NBG_WALKB(true, false, true, true); else NBG_WALKB(true, false, false, true);
+            if (count) NBG_WALKB(true, false, true, true, 8); else NBG_WALKB(true, false, false, true, 8);
         }
-    } else if (quad) {
-        if (kahan) {
-            if (count) NBG_WALKB(true, true, true, false); else NBG_WALKB(true, true, false, false);
+    } else if (count) {
+        if (quad) {
+            if (kahan) NBG_WALKB(true, true, true, false, 8); else NBG_WALKB(true, false, true, false, 8);
         } else {
-            if (count) NBG_WALKB(true, false, true, false); else NBG_WALKB(true, false, false, false);
+            if (kahan) NBG_WALKB(false, true, true, false, 8); else NBG_WALKB(false, false, true, false, 8);
+        }
+    } else if (kahan) {
+        // the production variants take the tuned P2P tile
+        if (quad) {
+            if (nt == 4) NBG_WALKB(true, true, false, false, 4);
+            else if (nt == 16) NBG_WALKB(true, true, false, false, 16);
+            else NBG_WALKB(true, true, false, false, 8);
+        } else {
+            if (nt == 4) NBG_WALKB(false, true, false, false, 4);
+            else if (nt == 16) NBG_WALKB(false, true, false, false, 16);
+            else NBG_WALKB(false, true, false, false, 8);
         }
     } else {
-        if (kahan) {
-            if (count) NBG_WALKB(false, true, true, false); else NBG_WALKB(false, true, false, false);
-        } else {
-            if (count) NBG_WALKB(false, false, true, false); else NBG_WALKB(false, false, false, false);
-        }
+        if (quad) NBG_WALKB(true, false, false, false, 8); else NBG_WALKB(false, false, false, false, 8);
     }
 #undef NBG_WALKB
 }
 
 void launch_walk(nbg_ctx* c, const nbg_params& p) {
-    if (c->walk_kernel == 1)
+    // the leaf-group walk has no learned-pair listing
+    if (c->tune.walk_kernel == 1 && !(p.flags & NBG_FLAG_LEARNED))
         launch_walk_leaf(c, p);
     else
         launch_walk_bins(c, p);
@@ -921,7 +933,7 @@ void launch_walk(nbg_ctx* c, const nbg_params& p) {
 // Blocks never straddle systems (a sort block of 2048 keys = 8 direct blocks of 256).
 // ComputeAccelDirect (projects/nbody_core/src/Solvers.cpp:77-106): j == i skipped.
 
-template <bool KAHAN>
+template <bool KAHAN, int U>
 __global__ __launch_bounds__(256) void k_direct(const float4* __restrict__ src, const int32_t* __restrict__ blk_sys,
                                                 const int32_t* __restrict__ blk_start,
                                                 const int32_t* __restrict__ blk_count,
@@ -978,10 +990,12 @@ __global__ __launch_bounds__(256) void k_direct(const float4* __restrict__ src, 
     float sx = 0.f, sy = 0.f, sz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
     const int sbeg = __builtin_amdgcn_readfirstlane(s0);
     const int send = __builtin_amdgcn_readfirstlane(s1);
+    // U sources per step (the direct P2P tile, chosen by the autotuner): their scalar
+    // loads are issued together before the arithmetic
     int j = sbeg;
-    for (; j + 4 <= send; j += 4) {
+    for (; j + U <= send; j += U) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < U; ++u) {
             const float4 s = src[j + u];
             const float dx = s.x - t.x, dy = s.y - t.y, dz = s.z - t.z;
             const float r2 = dx * dx + dy * dy + dz * dz + eps2;
@@ -1027,18 +1041,26 @@ void launch_direct(nbg_ctx* c, const nbg_params& p) {
     hipLaunchKernelGGL(k_pack, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->x, c->y, c->z, c->m, c->n,
                        static_cast<float>(p.G), c->spos);
     const dim3 grid(c->n_blocks * 8), block(256);
-    if (p.flags & NBG_FLAG_KAHAN)
-        hipLaunchKernelGGL(k_direct<true>, grid, block, 0, c->stream, c->spos, c->blk_sys, c->blk_start, c->blk_count,
-                           c->sys_off, nullptr, 0, static_cast<float>(p.eps2), c->ax, c->ay, c->az, nullptr);
-    else
-        hipLaunchKernelGGL(k_direct<false>, grid, block, 0, c->stream, c->spos, c->blk_sys, c->blk_start, c->blk_count,
-                           c->sys_off, nullptr, 0, static_cast<float>(p.eps2), c->ax, c->ay, c->az, nullptr);
+    const int u = c->tune.direct_tile;
+#define NBG_DIRECT(K, U)                                                                                          \
+    hipLaunchKernelGGL((k_direct<K, U>), grid, block, 0, c->stream, c->spos, c->blk_sys, c->blk_start, c->blk_count, \
+                       c->sys_off, nullptr, 0, static_cast<float>(p.eps2), c->ax, c->ay, c->az, nullptr)
+    if (p.flags & NBG_FLAG_KAHAN) {
+        if (u == 2) NBG_DIRECT(true, 2);
+        else if (u == 8) NBG_DIRECT(true, 8);
+        else NBG_DIRECT(true, 4);
+    } else {
+        if (u == 2) NBG_DIRECT(false, 2);
+        else if (u == 8) NBG_DIRECT(false, 8);
+        else NBG_DIRECT(false, 4);
+    }
+#undef NBG_DIRECT
 }
 
 void launch_direct_targets(nbg_ctx* c, const nbg_params& p, int64_t nt, const int32_t* targets, float* out3) {
     hipLaunchKernelGGL(k_pack, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->x, c->y, c->z, c->m, c->n,
                        static_cast<float>(p.G), c->spos);
-    hipLaunchKernelGGL(k_direct<true>, dim3(grid_for(nt, 256)), dim3(256), 0, c->stream, c->spos, c->blk_sys,
+    hipLaunchKernelGGL((k_direct<true, 4>), dim3(grid_for(nt, 256)), dim3(256), 0, c->stream, c->spos, c->blk_sys,
                        c->blk_start, c->blk_count, c->sys_off, targets, static_cast<int>(nt),
                        static_cast<float>(p.eps2), c->ax, c->ay, c->az, out3);
 }

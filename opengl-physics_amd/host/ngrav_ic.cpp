@@ -255,4 +255,22 @@ int ngh_ic_king(int n, uint32_t seed, double totalMass, double coreRadius, doubl
     return 0;
 }
 
+// Cluster sizes and kinds of a batched ensemble: N_k = round(exp(U[ln nmin, ln nmax])),
+// U from std::uniform_real_distribution<double> over std::mt19937(seed), drawn k = 0, 1, ...;
+// kind_k = k % 2 (0 Plummer, 1 King).  Returns the total particle count, or -1.
+int64_t ngh_ensemble_sizes(int32_t nclusters, uint32_t seed, int64_t nmin, int64_t nmax, int64_t* sizes,
+                           int32_t* kinds) {
+    if (nclusters < 1 || nmin < 1 || nmax < nmin || !sizes) return -1;
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<double> u(std::log(static_cast<double>(nmin)), std::log(static_cast<double>(nmax)));
+    int64_t total = 0;
+    for (int32_t k = 0; k < nclusters; ++k) {
+        const int64_t nk = std::llround(std::exp(u(rng)));
+        sizes[k] = std::min(nmax, std::max(nmin, nk));
+        if (kinds) kinds[k] = k % 2;
+        total += sizes[k];
+    }
+    return total;
+}
+
 }  // extern "C"

@@ -29,7 +29,8 @@ ERRORS = {0: "NBG_OK", -1: "NBG_E_INVALID", -2: "NBG_E_OOM", -3: "NBG_E_CAPACITY
 EXPORTED = ["nbg_create", "nbg_destroy", "nbg_last_error", "nbg_upload", "nbg_upload_batch", "nbg_upload_accel",
             "nbg_accel", "nbg_step", "nbg_download", "nbg_tree_export", "nbg_interaction_counts", "nbg_diagnostics",
             "nbg_accel_direct_targets", "nbg_learn_set_weights", "nbg_learn_get_weights", "nbg_learn_collect",
-            "nbg_learn_clear", "nbg_learn_pool", "nbg_learn_standardize", "nbg_learn_train", "nbg_learn_grad"]
+            "nbg_learn_clear", "nbg_learn_pool", "nbg_learn_standardize", "nbg_learn_train", "nbg_learn_grad",
+            "nbg_tuning_get", "nbg_tuning_set", "nbg_autotune", "nbg_diagnostics_system"]
 
 LEARN_FEATURES = 13
 LEARN_HIDDEN = 64
@@ -41,6 +42,14 @@ class Params(ctypes.Structure):
     _fields_ = [("solver", ctypes.c_int), ("multipole", ctypes.c_int), ("G", ctypes.c_double),
                 ("eps2", ctypes.c_double), ("theta", ctypes.c_double), ("dt", ctypes.c_double),
                 ("flags", ctypes.c_uint32), ("learn_band", ctypes.c_double)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("walk_kernel", ctypes.c_int32), ("near_tile", ctypes.c_int32), ("direct_tile", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class Stats(ctypes.Structure):
@@ -93,6 +102,11 @@ def load():
     L.nbg_learn_standardize.argtypes = [ctypes.c_void_p, ctypes.c_int64]
     L.nbg_learn_train.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_uint64, _f]
     L.nbg_learn_grad.argtypes = [ctypes.c_void_p, ctypes.c_int64, _f, _f, _f, _f, _f]
+    L.nbg_tuning_get.argtypes = [ctypes.c_void_p, ctypes.POINTER(Tuning)]
+    L.nbg_tuning_set.argtypes = [ctypes.c_void_p, ctypes.POINTER(Tuning)]
+    L.nbg_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_int32, ctypes.POINTER(Tuning), _d,
+                               ctypes.c_int32, _i32]
+    L.nbg_diagnostics_system.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_int32, ctypes.c_int32, _d]
     L.nbg_destroy.restype = None
     for name in EXPORTED:
         if name not in ("nbg_create", "nbg_destroy", "nbg_last_error"):
@@ -234,6 +248,31 @@ class Context:
                                                      _p(ax, _f), _p(ay, _f), _p(az, _f)), "nbg_accel_direct_targets")
         return np.stack([ax, ay, az], axis=1)
 
+    # ------------------------------------------------------------------ tuning
+    def tuning(self):
+        t = Tuning()
+        self._check(self._L.nbg_tuning_get(self._h, ctypes.byref(t)), "nbg_tuning_get")
+        return t
+
+    def set_tuning(self, walk_kernel=0, near_tile=8, direct_tile=4):
+        t = Tuning(int(walk_kernel), int(near_tile), int(direct_tile), 0)
+        self._check(self._L.nbg_tuning_set(self._h, ctypes.byref(t)), "nbg_tuning_set")
+
+    def autotune(self, params, reps=3):
+        best = Tuning()
+        ms = np.zeros(16, np.float64)
+        n = ctypes.c_int32(0)
+        self._check(self._L.nbg_autotune(self._h, ctypes.byref(params), int(reps), ctypes.byref(best), _p(ms, _d), 16,
+                                         ctypes.byref(n)), "nbg_autotune")
+        return best, ms[:n.value].copy()
+
+    def diagnostics_system(self, params, sys, with_potential=True):
+        out = np.empty(11, np.float64)
+        self._check(self._L.nbg_diagnostics_system(self._h, ctypes.byref(params), int(sys), 1 if with_potential else 0,
+                                                   _p(out, _d)), "nbg_diagnostics_system")
+        return dict(kinetic=out[0], potential=out[1], energy=out[0] + out[1], com=out[2:5].copy(),
+                    L=out[5:8].copy(), P=out[8:11].copy())
+
     # ------------------------------------------------------------------ learned correction
     def learn_set_weights(self, blob):
         b = np.ascontiguousarray(blob, dtype=np.float32)
@@ -296,6 +335,8 @@ def load_host():
     H.ngh_ic_plummer.argtypes = [ctypes.c_int, ctypes.c_uint32] + [ctypes.c_double] * 3 + [_d] * 7
     H.ngh_ic_hernquist.argtypes = [ctypes.c_int, ctypes.c_uint32] + [ctypes.c_double] * 3 + [_d] * 7
     H.ngh_ic_king.argtypes = [ctypes.c_int, ctypes.c_uint32] + [ctypes.c_double] * 4 + [_d] * 7
+    H.ngh_ensemble_sizes.argtypes = [ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64, ctypes.c_int64, _i64, _i32]
+    H.ngh_ensemble_sizes.restype = ctypes.c_int64
     for f in (H.ngh_ic_plummer, H.ngh_ic_hernquist, H.ngh_ic_king):
         f.restype = ctypes.c_int
     _host = H
@@ -324,6 +365,26 @@ def ic_hernquist(n, seed, total_mass=1.0, scale_radius=1.0, G=1.0):
 def ic_king(n, seed, w0=6.0, total_mass=1.0, core_radius=1.0, G=1.0):
     """ic::King<3>(W0), FP64 arrays x,y,z,vx,vy,vz,m."""
     return _ic(load_host().ngh_ic_king, n, seed, total_mass, core_radius, w0, G)
+
+
+def ensemble(nclusters=512, seed=0, nmin=1024, nmax=65536, w0=6.0):
+    """Batched ensemble: cluster sizes N_k = round(exp(U[ln nmin, ln nmax])) (mt19937(seed)),
+    kinds alternating Plummer (even k) / King(w0) (odd k), cluster k drawn with seed k,
+    total mass 1 each.  Returns (offsets int64[n+1], FP32 state dict, kinds)."""
+    H = load_host()
+    sizes = np.empty(nclusters, np.int64)
+    kinds = np.empty(nclusters, np.int32)
+    total = H.ngh_ensemble_sizes(int(nclusters), int(seed), int(nmin), int(nmax), _p(sizes, _i64), _p(kinds, _i32))
+    if total < 0:
+        raise ValueError("bad ensemble arguments")
+    off = np.zeros(nclusters + 1, np.int64)
+    off[1:] = np.cumsum(sizes)
+    st = {k: np.empty(total, np.float32) for k in ("x", "y", "z", "vx", "vy", "vz", "m")}
+    for k in range(nclusters):
+        ic = ic_plummer(int(sizes[k]), k) if kinds[k] == 0 else ic_king(int(sizes[k]), k, w0=w0)
+        for key in st:
+            st[key][off[k]:off[k + 1]] = ic[key]
+    return off, st, kinds
 
 
 def to_f32(state):
