@@ -39,25 +39,20 @@ FLOP_M2P_MONO = 20.0
 FLOP_MLP_PAIR = 2.0 * (13 * 64 + 64 * 64 + 64 * 64 + 64 * 3)   # 18,432 multiply-add flop per pair
 
 
-def walk_traffic(n, theta):
-    """HBM bytes per walk launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes of
-    this workload (scaled by a copy kernel of known size), or None for other workloads."""
-    path = os.path.join(ROOT, "profiles", "walk_traffic_plummer_1m_theta0.5.json")
+def kernel_traffic(n, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    passes of the 1M Plummer workload (scaled by a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_plummer_1m.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
-    return d["bytes_per_launch"] if (d["n"] == n and d["theta"] == theta) else None
+    return d["bytes_per_launch"].get(kernel) if d["n"] == n else None
 
 
-def mlp_traffic(n, theta, band):
-    """HBM bytes per learned-correction launch from the committed rocprofv3 passes, or None."""
-    path = os.path.join(ROOT, "profiles", "learn_traffic_plummer_1m_theta0.8.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        d = json.load(f)
-    return d["bytes_per_launch"] if (d["n"] == n and d["theta"] == theta and d["band"] == band) else None
+def walk_kernel_name(quad, kahan, learn, near_tile):
+    b = lambda v: "true" if v else "false"
+    return f"void k_walk_bins<{b(quad)}, {b(kahan)}, false, {b(learn)}, {near_tile}>"
 
 
 def log(*a):
@@ -301,7 +296,7 @@ def main():
                 f"ms/step, rel_rms {err_l:.3e} (plain at that theta {err_l_plain:.3e}); "
                 f"pairs {c0_l.n_learned}")
 
-    def walk_roofline(st, c0, c1, multipole, theta):
+    def walk_roofline(st, c0, c1, multipole, theta, learn=False):
         n_p2p = 0.5 * (c0.n_p2p + c1.n_p2p)
         n_m2p = 0.5 * (c0.n_m2p + c1.n_m2p)
         fm2p = FLOP_M2P_QUAD if multipole == "quadrupole" else FLOP_M2P_MONO
@@ -309,8 +304,10 @@ def main():
         walk_ms = st.ms_walk / args.steps
         ach = flops / (walk_ms * 1e-3) / 1e12
         return {"bound": "mfma", "pipe": "fp32-valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": walk_traffic(n, theta),
-                "kernel": "k_walk_bins", "kernel_ms": round(walk_ms, 4), "flop_per_launch": flops}, n_p2p, n_m2p
+                "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4),
+                "traffic": kernel_traffic(n, walk_kernel_name(multipole == "quadrupole", kahan, learn, 8)),
+                "kernel": walk_kernel_name(multipole == "quadrupole", kahan, learn, 8), "kernel_ms": round(walk_ms, 4),
+                "flop_per_launch": flops}, n_p2p, n_m2p
 
     roof_p, p2p_p, m2p_p = walk_roofline(st_p, c0_p, c1_p, args.multipole, args.theta)
     stage_p = {k: round(getattr(st_p, "ms_" + k) / args.steps, 4)
@@ -342,13 +339,13 @@ def main():
                     "stage_ms_per_step": stage_p,
                     "tree": {"nodes": st_p.n_nodes, "leaves": st_p.n_leaves, "max_depth": st_p.max_depth}})
     else:
-        roof_w, p2p_l, m2p_l = walk_roofline(st_l, c0_l, c1_l, "quadrupole", args.learned_theta)
+        roof_w, p2p_l, m2p_l = walk_roofline(st_l, c0_l, c1_l, "quadrupole", args.learned_theta, learn=True)
         pairs = 0.5 * (c0_l.n_learned + c1_l.n_learned)
         mlp_ms = st_l.ms_learned / args.steps
         mlp_ach = pairs * FLOP_MLP_PAIR / (mlp_ms * 1e-3) / 1e12
         roof_m = {"bound": "mfma", "pipe": "bf16-mfma", "achieved": round(mlp_ach, 3), "peak": PEAK_BF16_TFLOPS,
                   "unit": "TFLOP/s", "frac": round(mlp_ach / PEAK_BF16_TFLOPS, 5),
-                  "traffic": mlp_traffic(n, args.learned_theta, args.learn_band),
+                  "traffic": kernel_traffic(n, "k_learn_infer"),
                   "kernel": "k_learn_infer (+ k_learn_zero, k_learn_apply)", "kernel_ms": round(mlp_ms, 4),
                   "flop_per_launch": pairs * FLOP_MLP_PAIR, "pairs_per_launch": pairs}
         stage_l = {k: round(getattr(st_l, "ms_" + k) / args.steps, 4)
