@@ -495,6 +495,25 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
         // bin gets its pairs listed there (its MAC decisions are the same in both bins)
         uint64_t own_m = all;
         if (LEARN && nl > 0 && A.nd_start[lid[0]] < start) own_m &= ~1ull;
+        // a sphere (cb, Rb) holding every group sphere (com_g, R_g) of the bin: a node whose
+        // distance to cb clears (fails) every group's MAC by more than the FP32 rounding
+        // margin is accepted (opened) for all groups at once; by the triangle inequality
+        // each group's own test would decide the same.  A NaN centre disables it.
+        float cbx, cby, cbz, Rb, cbscale;
+        {
+            const bool isfirst = tv && mylf != prevlf;
+            const float4 lc = isfirst ? lcom[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float inv_nl = 1.f / static_cast<float>(max(nl, 1));
+            cbx = wave_sum(lc.x) * inv_nl;
+            cby = wave_sum(lc.y) * inv_nl;
+            cbz = wave_sum(lc.z) * inv_nl;
+            const float ex = lc.x - cbx, ey = lc.y - cby, ez = lc.z - cbz;
+            float r = isfirst ? sqrtf(ex * ex + ey * ey + ez * ez) + lc.w : 0.f;
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) r = fmaxf(r, __shfl_xor(r, o, 64));
+            Rb = r * (1.f + 1e-5f);
+            cbscale = fabsf(cbx) + fabsf(cby) + fabsf(cbz) + Rb;
+        }
 
         float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
         int n_far = 0, n_near = 0;
@@ -657,7 +676,18 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                         const float sth = sqrtf(au.y) * inv_theta;  // size / theta
                         const float sbd = LEARN ? sqrtf(au.y) * A.inv_band : 0.f;
                         const float cscale = fabsf(cm.x) + fabsf(cm.y) + fabsf(cm.z);
-                        for (uint64_t todo = act; todo; todo &= todo - 1) {
+                        const float bx = cm.x - cbx, by = cm.y - cby, bz = cm.z - cbz;
+                        const float db = sqrtf(bx * bx + by * by + bz * bz);
+                        const float slack_b = 1e-4f * (db + Rb + sth) + 4e-6f * (cscale + cbscale);
+                        uint64_t todo_all = act;
+                        if (db - Rb - sth > slack_b && (!LEARN || db - Rb - sbd > slack_b)) {
+                            far_m = act;  // accepted by every group (and in no group's band)
+                            todo_all = 0;
+                        } else if (sth - db - Rb > slack_b) {
+                            open_m = act;  // opened by every group
+                            todo_all = 0;
+                        }
+                        for (uint64_t todo = todo_all; todo; todo &= todo - 1) {
                             const int jj = __ffsll(static_cast<long long>(todo)) - 1;
                             const uint64_t bit = 1ull << jj;
                             const float4 lc = lcom[jj];
