@@ -359,6 +359,7 @@ nbg_ctx* nbg_create(int device, int64_t max_n) {
     r |= alloc_into(c, c->leaf_depth, N);
     r |= alloc_into(c, c->cpl_prev, N);
     r |= alloc_into(c, c->node_scan, N + 1);
+    r |= alloc_into(c, c->blk_depth, N / 256 + 1);
     r |= alloc_into(c, c->leaf_list, N);
     r |= alloc_into(c, c->leaf_of, N);
     r |= alloc_into(c, c->spos, N + kSposPad);
@@ -409,7 +410,7 @@ void nbg_destroy(nbg_ctx* c) {
     free_dev(c->key_a); free_dev(c->key_b); free_dev(c->klo_a); free_dev(c->klo_b);
     free_dev(c->val_a); free_dev(c->val_b);
     free_dev(c->hist); free_dev(c->scan_part);
-    free_dev(c->leaf_depth); free_dev(c->cpl_prev); free_dev(c->node_scan); free_dev(c->leaf_list); free_dev(c->leaf_of);
+    free_dev(c->leaf_depth); free_dev(c->cpl_prev); free_dev(c->node_scan); free_dev(c->blk_depth); free_dev(c->leaf_list); free_dev(c->leaf_of);
     free_dev(c->spos);
     free_dev(c->nd_start); free_dev(c->nd_count); free_dev(c->nd_depth); free_dev(c->nd_parent);
     free_dev(c->nd_oct); free_dev(c->nd_child); free_dev(c->nd_geom);

@@ -128,6 +128,7 @@ struct nbg_ctx {
     int32_t* leaf_depth = nullptr;   // per sorted position
     int32_t* cpl_prev = nullptr;     // common-prefix length with the previous position
     uint64_t* node_scan = nullptr;   // (nodes started << 32 | leaf started), then its exclusive scan
+    int32_t* blk_depth = nullptr;    // deepest leaf per k_leaf_depth block
 
     // node arrays (DFS pre-order ids)
     int32_t* nd_start = nullptr;

@@ -423,51 +423,72 @@ __global__ __launch_bounds__(256) void k_leaf_depth(const uint64_t* __restrict__
                                                     const int32_t* __restrict__ pos_sys,
                                                     const int32_t* __restrict__ sys_off, int n,
                                                     int32_t* __restrict__ leaf_depth, int32_t* __restrict__ cpl_prev,
-                                                    uint64_t* __restrict__ node_scan, DevScalars* __restrict__ sc) {
+                                                    uint64_t* __restrict__ node_scan, DevScalars* __restrict__ sc,
+                                                    int32_t* __restrict__ blk_depth) {
+    __shared__ int s_dmax[4];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int s = pos_sys[i];
-    const int s0 = sys_off[s], s1 = sys_off[s + 1];
-    int L[kNcrit], R[kNcrit];
+    int D = 0;  // positions past n report depth 0 to the block maximum
+    if (i < n) {
+        const int s = pos_sys[i];
+        const int s0 = sys_off[s], s1 = sys_off[s + 1];
+        int L[kNcrit], R[kNcrit];
 #pragma unroll
-    for (int k = 1; k <= kNcrit; ++k) {
-        L[k - 1] = (i - k >= s0) ? cpl_at<LEVELS>(kh, kl, i - k, i) : -1;
-        R[k - 1] = (i + k < s1) ? cpl_at<LEVELS>(kh, kl, i, i + k) : -1;
-    }
-    // cell size at depth d: 1 + #{L >= d} + #{R >= d}; leaf depth = first d with size <= 8
-    int D = -1;
-    for (int d = 0; d <= LEVELS; ++d) {
-        int size = 1;
+        for (int k = 1; k <= kNcrit; ++k) {
+            L[k - 1] = (i - k >= s0) ? cpl_at<LEVELS>(kh, kl, i - k, i) : -1;
+            R[k - 1] = (i + k < s1) ? cpl_at<LEVELS>(kh, kl, i, i + k) : -1;
+        }
+        // cell size at depth d: 1 + #{L >= d} + #{R >= d}; leaf depth = first d with size <= 8
+        D = -1;
+        for (int d = 0; d <= LEVELS; ++d) {
+            int size = 1;
 #pragma unroll
-        for (int k = 0; k < kNcrit; ++k) size += (L[k] >= d) + (R[k] >= d);
-        if (size <= kNcrit) {
-            D = d;
-            break;
+            for (int k = 0; k < kNcrit; ++k) size += (L[k] >= d) + (R[k] >= d);
+            if (size <= kNcrit) {
+                D = d;
+                break;
+            }
         }
-    }
-    if (D < 0) {
-        if (LEVELS == kLevelsShort) {
-            atomicOr(&sc->overflow, 1);
-            atomicOr(&sc->sticky_overflow, 1);
-            D = LEVELS;
-        } else {
-            D = kMaxDepth;  // depth cap: the leaf keeps every particle (Tree.cpp:126)
+        if (D < 0) {
+            if (LEVELS == kLevelsShort) {
+                atomicOr(&sc->overflow, 1);
+                atomicOr(&sc->sticky_overflow, 1);
+                D = LEVELS;
+            } else {
+                D = kMaxDepth;  // depth cap: the leaf keeps every particle (Tree.cpp:126)
+            }
         }
+        const int cp = L[0];  // -1 at the start of a system
+        leaf_depth[i] = D;
+        cpl_prev[i] = cp;
+        const uint64_t ns = (D > cp) ? static_cast<uint64_t>(D - cp) : 0u;
+        const uint64_t ls = (D > cp) ? 1u : 0u;
+        node_scan[i] = (ns << 32) | ls;
     }
-    const int cp = L[0];  // -1 at the start of a system
-    leaf_depth[i] = D;
-    cpl_prev[i] = cp;
-    const uint64_t ns = (D > cp) ? static_cast<uint64_t>(D - cp) : 0u;
-    const uint64_t ls = (D > cp) ? 1u : 0u;
-    node_scan[i] = (ns << 32) | ls;
+    // deepest leaf: one value per block (the totals kernel takes the maximum), no
+    // same-address atomics; every thread reaches the one barrier
     int dmax = D;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) dmax = max(dmax, __shfl_xor(dmax, o, 64));
-    if (lane_id() == 0) atomicMax(&sc->max_depth, dmax);
+    if (lane_id() == 0) s_dmax[threadIdx.x >> 6] = dmax;
+    __syncthreads();
+    if (threadIdx.x == 0) blk_depth[blockIdx.x] = max(max(s_dmax[0], s_dmax[1]), max(s_dmax[2], s_dmax[3]));
 }
 
 // Grand totals of the node scan -> device scalars.
-__global__ void k_node_totals(const uint64_t* __restrict__ total, DevScalars* __restrict__ sc, int64_t cap_nodes) {
+__global__ __launch_bounds__(256) void k_node_totals(const uint64_t* __restrict__ total, DevScalars* __restrict__ sc,
+                                                     int64_t cap_nodes, const int32_t* __restrict__ blk_depth,
+                                                     int nblk) {
+    __shared__ int s_m[256];
+    int m = 0;
+    for (int b = threadIdx.x; b < nblk; b += 256) m = max(m, blk_depth[b]);
+    s_m[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o >= 1; o >>= 1) {
+        if (threadIdx.x < o) s_m[threadIdx.x] = max(s_m[threadIdx.x], s_m[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    sc->max_depth = s_m[0];
     const uint64_t t = *total;
     const int64_t nodes = static_cast<int64_t>(t >> 32);
     sc->n_leaves = static_cast<int32_t>(t & 0xffffffffu);
@@ -715,13 +736,14 @@ void launch_build(nbg_ctx* c, int levels) {
     const int gnodes = 2048;  // grid-stride kernels over the device-side node count
     if (levels == kLevelsShort)
         hipLaunchKernelGGL(k_leaf_depth<kLevelsShort>, dim3(g), dim3(256), 0, st, c->key_sorted, c->klo_sorted,
-                           c->pos_sys, c->sys_off, n, c->leaf_depth, c->cpl_prev, c->node_scan, c->dsc);
+                           c->pos_sys, c->sys_off, n, c->leaf_depth, c->cpl_prev, c->node_scan, c->dsc, c->blk_depth);
     else
         hipLaunchKernelGGL(k_leaf_depth<kLevelsDeep>, dim3(g), dim3(256), 0, st, c->key_sorted, c->klo_sorted,
-                           c->pos_sys, c->sys_off, n, c->leaf_depth, c->cpl_prev, c->node_scan, c->dsc);
+                           c->pos_sys, c->sys_off, n, c->leaf_depth, c->cpl_prev, c->node_scan, c->dsc, c->blk_depth);
     // node_scan[n] receives the grand total (the array holds n+1 entries)
     excl_scan<uint64_t>(st, c->node_scan, c->node_scan, n, c->scan_part, c->node_scan + n);
-    hipLaunchKernelGGL(k_node_totals, dim3(1), dim3(1), 0, st, c->node_scan + n, c->dsc, c->cap_nodes);
+    hipLaunchKernelGGL(k_node_totals, dim3(1), dim3(256), 0, st, c->node_scan + n, c->dsc, c->cap_nodes, c->blk_depth,
+                       g);
     if (levels == kLevelsShort)
         hipLaunchKernelGGL(k_nodes<kLevelsShort>, dim3(g), dim3(256), 0, st, c->key_sorted, c->klo_sorted, c->pos_sys,
                            c->sys_off, c->leaf_depth, c->cpl_prev, c->node_scan, n, c->cap_nodes, c->nd_start,

@@ -437,7 +437,6 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     float4* lcom = s_lcom[wave];
     int* lid = s_lid[wave];
     const float eps2 = A.eps2, inv_theta = A.inv_theta;
-
     // XCD-aware work distribution: workgroups are dispatched round-robin over the 8 XCDs,
     // so XCD x takes the contiguous bin range [x*per, (x+1)*per) and its waves pull bins
     // from it in order through one counter per XCD (neighbouring bins share tree nodes in
@@ -680,9 +679,25 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                         const float db = sqrtf(bx * bx + by * by + bz * bz);
                         const float slack_b = 1e-4f * (db + Rb + sth) + 4e-6f * (cscale + cbscale);
                         uint64_t todo_all = act;
-                        if (db - Rb - sth > slack_b && (!LEARN || db - Rb - sbd > slack_b)) {
-                            far_m = act;  // accepted by every group (and in no group's band)
+                        if (db - Rb - sth > slack_b) {
+                            far_m = act;  // accepted by every group
                             todo_all = 0;
+                            if (LEARN) {
+                                // band membership (a heuristic selection, decided the same way
+                                // when samples are collected): by the sphere when it settles
+                                // every group, else by each group's distance
+                                if (sbd - db - Rb > slack_b) {
+                                    band_m = act;
+                                } else if (!(db - Rb - sbd > slack_b)) {
+                                    for (uint64_t t = act; t; t &= t - 1) {
+                                        const int jj = __ffsll(static_cast<long long>(t)) - 1;
+                                        const float4 lc = lcom[jj];
+                                        const float dx = cm.x - lc.x, dy = cm.y - lc.y, dz = cm.z - lc.z;
+                                        const float qb = lc.w + sbd;
+                                        if (dx * dx + dy * dy + dz * dz <= qb * qb) band_m |= 1ull << jj;
+                                    }
+                                }
+                            }
                         } else if (sth - db - Rb > slack_b) {
                             open_m = act;  // opened by every group
                             todo_all = 0;
