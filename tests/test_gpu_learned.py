@@ -269,3 +269,99 @@ def test_learned_argument_errors(gpu_ctx_factory):
         ctx.accel(pynbg.make_params("barnes_hut", "monopole", theta=0.8, learned=True))
     with pytest.raises(pynbg.NbgError, match="multiple of 64"):
         ctx.learn_train(1, 100, 1e-3)
+
+
+def _run_learned(ctx, s, **kw):
+    import pynbg
+    ctx.upload(s["x"], s["y"], s["z"], s["m"], s["vx"], s["vy"], s["vz"])
+    st = ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=0.005, learned=True, **kw))
+    return ctx.accelerations().astype(np.float64), st
+
+
+def test_pair_list_regrowth_eager_and_graph(gpu_ctx_factory):
+    """More listed pairs than the initial list holds: the eager evaluation grows the list and
+    evaluates again; a graph batch that overflows restores its state and reruns eagerly."""
+    import pynbg
+    from pynbg import learned
+    s = plummer_f32(65536, 6)
+    blob = learned.init_blob(3)
+    # reference: a context whose list already holds every pair
+    ref_ctx = gpu_ctx_factory(65536)
+    ref_ctx.learn_set_weights(blob)
+    a_ref, st_ref = _run_learned(ref_ctx, s, theta=0.8, learn_band=0.0)
+    assert st_ref.n_learned > (1 << 20) * 2  # beyond the 2M initial capacity of this context size
+    ctx = gpu_ctx_factory(65536)
+    ctx.learn_set_weights(blob)
+    a, st = _run_learned(ctx, s, theta=0.8, learn_band=0.0)
+    assert st.n_learned == st_ref.n_learned
+    np.testing.assert_allclose(a, a_ref, rtol=0, atol=1e-6 * np.abs(a_ref).max())
+    # graph batch: captured while the list is small (band 0.9 lists few pairs), then run with
+    # band 0 so that the replay overflows
+    g_ctx = gpu_ctx_factory(65536)
+    g_ctx.learn_set_weights(blob)
+    g_ctx.upload(s["x"], s["y"], s["z"], s["m"], s["vx"], s["vy"], s["vz"])
+    p_small = pynbg.make_params("barnes_hut", G=1.0, eps=0.005, theta=0.8, dt=1e-4, learned=True, learn_band=0.9)
+    g_ctx.accel(p_small)
+    p_all = pynbg.make_params("barnes_hut", G=1.0, eps=0.005, theta=0.8, dt=1e-4, learned=True, learn_band=0.0)
+    e_ctx = gpu_ctx_factory(65536)
+    e_ctx.learn_set_weights(blob)
+    e_ctx.upload(s["x"], s["y"], s["z"], s["m"], s["vx"], s["vy"], s["vz"])
+    e_ctx.accel(p_small)
+    p_eager = pynbg.make_params("barnes_hut", G=1.0, eps=0.005, theta=0.8, dt=1e-4, learned=True, learn_band=0.0,
+                                no_graph=True)
+    g_ctx.step(p_all, 2)
+    e_ctx.step(p_eager, 2)
+    dg, de = g_ctx.download(), e_ctx.download()
+    for k in ("x", "vx", "ax"):
+        np.testing.assert_allclose(dg[k], de[k], rtol=0, atol=1e-5 * np.abs(de[k]).max())
+
+
+def test_learned_batch_keeps_systems_apart(gpu_ctx_factory):
+    """A batch of three systems: each gets the correction it gets alone."""
+    from pynbg import learned
+    blob = learned.load_weights(WEIGHTS) if os.path.exists(WEIGHTS) else learned.init_blob(4)
+    parts = [plummer_f32(n, 20 + k) for k, n in enumerate((3000, 7000, 5000))]
+    alone = []
+    for sp in parts:
+        ctx = gpu_ctx_factory(len(sp["x"]))
+        ctx.learn_set_weights(blob)
+        alone.append(_run_learned(ctx, sp, theta=0.8, learn_band=0.62)[0])
+    off = np.concatenate([[0], np.cumsum([len(sp["x"]) for sp in parts])]).astype(np.int64)
+    # systems of a batch are independent: they may overlap in space
+    cat = {k: np.concatenate([sp[k] for sp in parts]) for k in parts[0]}
+    ctx = gpu_ctx_factory(int(off[-1]))
+    ctx.learn_set_weights(blob)
+    import pynbg
+    ctx.upload_batch(off, cat["x"], cat["y"], cat["z"], cat["m"])
+    ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=0.005, theta=0.8, learned=True, learn_band=0.62))
+    a = ctx.accelerations().astype(np.float64)
+    for i in range(3):
+        sl = slice(off[i], off[i + 1])
+        # same trees and pairs; the per-group sums of the correction are float atomics, so
+        # only their order may differ
+        err = np.sqrt(np.sum((a[sl] - alone[i]) ** 2) / np.sum(alone[i] ** 2))
+        assert err < 1e-6, (i, err)
+
+
+def test_learned_without_pairs_equals_plain_and_deep_keys(gpu_ctx_factory):
+    import pynbg
+    from pynbg import learned
+    s = plummer_f32(8192, 9)
+    ctx = gpu_ctx_factory(8192)
+    ctx.learn_set_weights(learned.init_blob(2))
+    ctx.upload(s["x"], s["y"], s["z"], s["m"])
+    ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=0.005, theta=0.3))
+    plain = ctx.accelerations()
+    # theta = 0.3 with band 0.95: no accepted cell is in the band
+    st = ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=0.005, theta=0.3, learned=True, learn_band=0.95))
+    assert st.n_learned == 0
+    np.testing.assert_array_equal(ctx.accelerations(), plain)
+    # 40-level keys: 64 coincident particles force the deep path; the correction stays finite
+    x = np.concatenate([s["x"][:4000], np.full(64, 0.25, np.float32)])
+    y = np.concatenate([s["y"][:4000], np.full(64, -0.125, np.float32)])
+    z = np.concatenate([s["z"][:4000], np.full(64, 0.5, np.float32)])
+    m = np.full(len(x), 1.0 / len(x), np.float32)
+    ctx.upload(x, y, z, m)
+    st = ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=0.005, theta=0.8, learned=True, learn_band=0.6))
+    assert st.deep_keys == 1 and st.n_learned > 0
+    assert np.all(np.isfinite(ctx.accelerations()))
