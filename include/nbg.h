@@ -36,6 +36,15 @@ extern "C" {
 #define NBG_MONOPOLE 0
 #define NBG_QUADRUPOLE 1
 
+/* Softening kinds, ngrav::SofteningKind (projects/nbody_core/include/ngrav/Softening.hpp:30-33). */
+#define NBG_SOFT_PLUMMER 0
+#define NBG_SOFT_SPLINE 1   /* cubic-spline kernel of length h = sqrt(eps2) (Softening::Spline(h),
+                               Softening.hpp:132-138), exactly Newtonian beyond 2h */
+
+/* Opening criteria, ngrav::MacKind (projects/nbody_core/include/ngrav/Mac.hpp:13-16). */
+#define NBG_MAC_GEOMETRIC 0
+#define NBG_MAC_RELATIVE 1  /* per-particle acceleration criterion with the previous |a| */
+
 /* flags */
 #define NBG_FLAG_KAHAN 1u           /* compensated FP32 force sums */
 #define NBG_FLAG_COUNT 2u           /* record per-group interaction counts (nbg_interaction_counts) */
@@ -43,12 +52,16 @@ extern "C" {
 #define NBG_FLAG_NO_GRAPH 8u        /* nbg_step: launch eagerly instead of replaying a hipGraph */
 #define NBG_FLAG_STAGE_TIMES 16u    /* nbg_step: per-stage HIP-event times summed over all steps
                                        (one host synchronisation per step) */
+#define NBG_FLAG_PER_PARTICLE 32u   /* Barnes-Hut with one tree descent per particle
+                                       (ComputeAccelBarnesHutPerParticle, Solvers.cpp:359-382)
+                                       instead of the grouped walk; implied by NBG_MAC_RELATIVE */
 
 typedef struct nbg_ctx nbg_ctx;
 
-/* Everything one force evaluation needs; ngrav::StepParams
- * (projects/nbody_core/include/ngrav/Solvers.hpp:22-40) restricted to the Plummer,
- * geometric-MAC, fixed-dt path. */
+/* Everything one force evaluation and one step need; ngrav::StepParams
+ * (projects/nbody_core/include/ngrav/Solvers.hpp:22-40) without the FMM orders.
+ * Fields after learn_band were appended later; zero-initialised they select Plummer
+ * softening, the geometric MAC and fixed global steps. */
 typedef struct nbg_params {
     int solver;     /* NBG_SOLVER_* */
     int multipole;  /* NBG_MONOPOLE | NBG_QUADRUPOLE */
@@ -60,6 +73,16 @@ typedef struct nbg_params {
     double learn_band; /* NBG_FLAG_LEARNED: an accepted cell gets the learned correction when
                           size / (distance - group radius) exceeds learn_band (0 = every
                           accepted cell).  No reference counterpart. */
+    int32_t softening;  /* NBG_SOFT_* (StepParams::soft.kind) */
+    int32_t mac;        /* NBG_MAC_* (MacParams::kind) */
+    double alpha;       /* MacParams::alpha: relative-MAC force-error target */
+    int32_t adaptive;   /* StepParams::adaptive: global dt = eta min sqrt(h/|a|), clamped to
+                           [dt/4096, dt] (ChooseAdaptiveDt, Integrator.cpp:41-59) */
+    int32_t block;      /* StepParams::block: power-of-2 rung steps (System::BlockStep,
+                           System.hpp:239-294); takes precedence over adaptive */
+    double eta;         /* StepParams::eta */
+    int32_t block_max_rung; /* StepParams::blockMaxRung */
+    int32_t reserved;   /* 0 */
 } nbg_params;
 
 /* Opt-in stage timings and tree statistics, like GpuBarnesHutStats
@@ -81,6 +104,11 @@ typedef struct nbg_stats {
     double ms_total;
     int64_t n_learned;    /* (leaf, cell) pairs given the learned correction (last evaluation) */
     double ms_learned;    /* learned-correction kernels (inside ms_walk's evaluation) */
+    double t_advanced;    /* nbg_step: simulated time advanced (sum of the dt System::Step returns) */
+    double dt_last;       /* nbg_step: dt of the last step (the coarse dt for block steps) */
+    int64_t particle_evals; /* per-particle force evaluations (System::ParticleEvals) */
+    int32_t rung_max;     /* nbg_step with block: deepest rung of the last coarse step */
+    int32_t pad;
 } nbg_stats;
 
 /* Create a context on HIP device `device` able to hold up to max_n particles.
@@ -119,7 +147,14 @@ int nbg_accel(nbg_ctx* ctx, const nbg_params* p, nbg_stats* stats);
  * LeapfrogStep (projects/nbody_core/src/Integrator.cpp:9-39) as driven by System::Step
  * (projects/nbody_core/include/ngrav/System.hpp:89-108).  Precondition: the context holds
  * a(t) (call nbg_accel first, as System::Prime does).  Unless NBG_FLAG_NO_GRAPH is set,
- * one step is captured once into a hipGraph and replayed. */
+ * one step is captured once into a hipGraph and replayed.
+ *   p->adaptive: each step's dt is chosen on the device from a(t) (ChooseAdaptiveDt,
+ *     Integrator.cpp:41-59) and read by the kick/drift kernels; graph-captured too.
+ *   p->block: each call of System::BlockStep (System.hpp:239-294) is one coarse step of
+ *     p->dt split into power-of-2 rungs; every substep rebuilds the tree and evaluates
+ *     the closing set with the per-particle geometric walk (Barnes-Hut) or the direct
+ *     sum onto those targets.  Eager (the substep count depends on the rungs).
+ * stats->t_advanced / dt_last / particle_evals / rung_max report what the steps took. */
 int nbg_step(nbg_ctx* ctx, const nbg_params* p, int32_t nsteps, nbg_stats* stats);
 
 /* Copy state back to the host (original particle order).  Any pointer may be NULL.
@@ -162,6 +197,17 @@ int nbg_diagnostics(nbg_ctx* ctx, const nbg_params* p, int32_t with_potential, d
 int nbg_accel_direct_targets(nbg_ctx* ctx, const nbg_params* p, int64_t nt, const int64_t* targets, float* ax,
                              float* ay, float* az);
 
+/* Barnes-Hut accelerations onto a subset of targets by the per-particle geometric walk,
+ * all particles as sources, after a fresh tree build: ComputeAccelBarnesHutTargets
+ * (projects/nbody_core/src/Solvers.cpp:384-405), the force pass of block steps.
+ * Results go to host arrays of length nt; the context's accelerations are unchanged. */
+int nbg_accel_bh_targets(nbg_ctx* ctx, const nbg_params* p, int64_t nt, const int64_t* targets, float* ax,
+                         float* ay, float* az);
+
+/* Per-particle rungs of the last block step (System::Rungs, System.hpp:203-205), original
+ * order; zeros before any block step. */
+int nbg_rungs(nbg_ctx* ctx, int32_t* rungs);
+
 /* Launch choices of the force kernels, set directly or by nbg_autotune.  No reference
  * counterpart: the reference's GL kernels have fixed work-group sizes
  * (projects/nbody_core/src/gpu/GpuBarnesHut.cpp, GpuDirect.cpp). */
@@ -181,7 +227,8 @@ int nbg_tuning_set(nbg_ctx* ctx, const nbg_tuning* t);
  * untimed and `reps` timed evaluations each, HIP-event kernel time), keep the fastest in
  * the context and return it in *best.  ms (nullable, cap entries) receives the mean time
  * of each candidate in candidate order, *n_out the number of candidates.  Candidates: bin
- * walk with near_tile 4/8/16 and the leaf-group walk (Barnes-Hut), direct_tile 2/4/8
+ * walk with near_tile 4/8/16 and, for Plummer softening with the grouped geometric walk
+ * and no learned correction, the leaf-group walk (Barnes-Hut); direct_tile 2/4/8
  * (direct).  The accelerations are overwritten with those of the last evaluation. */
 int nbg_autotune(nbg_ctx* ctx, const nbg_params* p, int32_t reps, nbg_tuning* best, double* ms, int32_t cap,
                  int32_t* n_out);

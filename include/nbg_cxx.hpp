@@ -19,6 +19,8 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "nbg.h"
@@ -29,9 +31,16 @@ struct Params {
     int solver = NBG_SOLVER_BARNES_HUT;
     int multipole = NBG_QUADRUPOLE;  // System<3> always adds quadrupoles (System.hpp:59)
     double G = 1.0;
-    double eps = 0.05;               // Plummer softening length (Softening::Plummer)
+    double eps = 0.05;               // Softening::eps: Plummer length or spline h
+    int softening = NBG_SOFT_PLUMMER;
     double theta = 0.5;
+    int mac = NBG_MAC_GEOMETRIC;     // MacParams::kind
+    double alpha = 0.001;            // MacParams::alpha
     double dt = 1e-3;
+    bool adaptive = false;           // StepParams::adaptive / eta / block / blockMaxRung
+    double eta = 0.03;
+    bool block = false;
+    int block_max_rung = 8;
     bool kahan = true;
 
     nbg_params c() const {
@@ -43,6 +52,13 @@ struct Params {
         p.theta = theta;
         p.dt = dt;
         p.flags = kahan ? NBG_FLAG_KAHAN : 0u;
+        p.softening = softening;
+        p.mac = mac;
+        p.alpha = alpha;
+        p.adaptive = adaptive ? 1 : 0;
+        p.eta = eta;
+        p.block = block ? 1 : 0;
+        p.block_max_rung = block_max_rung;
         return p;
     }
 };
@@ -72,6 +88,11 @@ std::vector<float> to_f32(const V& v, size_t n) {
     for (size_t i = 0; i < n; ++i) f[i] = static_cast<float>(v[i]);
     return f;
 }
+// does T carry accelerations (SoA<3>: ax, ay, az) or only positions (PosMassView<3>)?
+template <class T, class = void>
+struct has_accel : std::false_type {};
+template <class T>
+struct has_accel<T, std::void_t<decltype(std::declval<const T&>().ax.size())>> : std::true_type {};
 }  // namespace detail
 
 // Upload the positions/masses of `in`, evaluate, and write FP64 accelerations into `out`
@@ -85,6 +106,14 @@ nbg_stats AccelInto(Context& ctx, const In& in, const Params& prm, Out& out) {
     ctx.check(nbg_upload(ctx.get(), static_cast<int64_t>(n), x.data(), y.data(), z.data(), m.data(), nullptr,
                          nullptr, nullptr),
               "nbg_upload");
+    // the relative MAC reads the previous |a|: the accelerations `in` carries (a(t) when
+    // called from LeapfrogStep, Integrator.cpp:28), zero when it has none
+    if constexpr (detail::has_accel<In>::value) {
+        if (prm.mac == NBG_MAC_RELATIVE && in.ax.size() == n) {
+            const auto ax = detail::to_f32(in.ax, n), ay = detail::to_f32(in.ay, n), az = detail::to_f32(in.az, n);
+            ctx.check(nbg_upload_accel(ctx.get(), ax.data(), ay.data(), az.data()), "nbg_upload_accel");
+        }
+    }
     const nbg_params p = prm.c();
     nbg_stats st{};
     ctx.check(nbg_accel(ctx.get(), &p, &st), "nbg_accel");
@@ -99,7 +128,9 @@ nbg_stats AccelInto(Context& ctx, const In& in, const Params& prm, Out& out) {
 }
 
 // A callable with the signature of ngrav::AccelFn<3>: void(const SoA&, const StepParams&, SoA&).
-// The StepParams fields read are G, soft.eps and mac.theta; the solver is fixed at creation.
+// The StepParams fields read are G, soft (kind, eps) and mac (kind, theta, alpha); the
+// solver is fixed at creation.  SofteningKind and MacKind number their values as
+// NBG_SOFT_* and NBG_MAC_* do (Softening.hpp:30-33, Mac.hpp:13-16).
 template <class SoA, class StepParams>
 std::function<void(const SoA&, const StepParams&, SoA&)> MakeAccelFn(std::shared_ptr<Context> ctx,
                                                                       int solver = NBG_SOLVER_BARNES_HUT,
@@ -110,7 +141,10 @@ std::function<void(const SoA&, const StepParams&, SoA&)> MakeAccelFn(std::shared
         p.multipole = multipole;
         p.G = sp.G;
         p.eps = sp.soft.eps;
+        p.softening = static_cast<int>(sp.soft.kind);
         p.theta = sp.mac.theta;
+        p.mac = static_cast<int>(sp.mac.kind);
+        p.alpha = sp.mac.alpha;
         AccelInto(*ctx, in, p, out);
     };
 }

@@ -24,6 +24,12 @@ int ngh_ic_hernquist(int n, uint32_t seed, double total_mass, double scale_radiu
 int ngh_ic_king(int n, uint32_t seed, double total_mass, double core_radius, double w0, double G, double* x,
                 double* y, double* z, double* vx, double* vy, double* vz, double* m);
 
+/* Uniform-sphere cluster (Cluster<3>, projects/nbody_core/src/Scenarios.cpp:104-142): R = 1,
+ * M = 1, Gaussian velocities of dispersion 0.3; cold != 0 zeroes the velocities after the
+ * same draws (ColdUniform, Scenarios.cpp:201-214, the cold_collapse scenario). */
+int ngh_ic_cluster(int n, uint32_t seed, int cold, double* x, double* y, double* z, double* vx, double* vy,
+                   double* vz, double* m);
+
 /* Batched-ensemble layout (no reference counterpart: the reference runs one System).
  * N_k = round(exp(U[ln nmin, ln nmax])) with U drawn from std::mt19937(seed) through
  * std::uniform_real_distribution<double>, k = 0..nclusters-1; kinds[k] = k % 2 (0 Plummer,

@@ -18,6 +18,7 @@ using namespace nbg;
 namespace {
 
 constexpr int kMaxSys = 1 << 20;
+constexpr int kMaxRung = 24;  // 2^24 substeps per coarse step at most
 
 template <typename T>
 int alloc_into(nbg_ctx* c, T*& p, size_t count) {
@@ -68,7 +69,36 @@ int validate(nbg_ctx* c, const nbg_params* p) {
         c->set_error("no particles uploaded");
         return NBG_E_STATE;
     }
+    if (p->softening != NBG_SOFT_PLUMMER && p->softening != NBG_SOFT_SPLINE) {
+        c->set_error("unknown softening kind");
+        return NBG_E_INVALID;
+    }
+    if (p->softening == NBG_SOFT_SPLINE && !(p->eps2 > 0.0)) {
+        c->set_error("spline softening needs a softening length h = sqrt(eps2) > 0");
+        return NBG_E_INVALID;
+    }
+    if (p->mac != NBG_MAC_GEOMETRIC && p->mac != NBG_MAC_RELATIVE) {
+        c->set_error("unknown opening criterion (mac)");
+        return NBG_E_INVALID;
+    }
+    if (p->mac == NBG_MAC_RELATIVE && !(p->alpha >= 0.0 && p->alpha < 1e300)) {
+        c->set_error("the relative MAC needs a finite alpha >= 0");
+        return NBG_E_INVALID;
+    }
+    if ((p->adaptive || p->block) && !(p->eta > 0.0 && p->dt > 0.0)) {
+        c->set_error("adaptive and block steps need eta > 0 and dt > 0");
+        return NBG_E_INVALID;
+    }
+    if (p->block && p->block_max_rung > kMaxRung) {
+        c->set_error("block_max_rung must be <= " + std::to_string(kMaxRung));
+        return NBG_E_INVALID;
+    }
     if (p->flags & NBG_FLAG_LEARNED) {
+        if (p->softening != NBG_SOFT_PLUMMER || p->mac != NBG_MAC_GEOMETRIC || (p->flags & NBG_FLAG_PER_PARTICLE) ||
+            p->block) {
+            c->set_error("the learned correction runs with Plummer softening, the grouped geometric walk and global steps");
+            return NBG_E_INVALID;
+        }
         if (p->solver != NBG_SOLVER_BARNES_HUT || p->multipole != NBG_QUADRUPOLE) {
             c->set_error("the learned correction needs the Barnes-Hut solver with quadrupoles");
             return NBG_E_INVALID;
@@ -162,6 +192,9 @@ int accel_eager(nbg_ctx* c, const nbg_params& p, nbg_stats* stats, bool list_onl
         }
         return NBG_OK;
     }
+    // the relative MAC reads |a| of the previous evaluation: capture it once, before the
+    // walk (a deep-key re-run below must not see this evaluation's accelerations)
+    if (p.mac == NBG_MAC_RELATIVE) launch_aold(c);
     int levels = kLevelsShort;
     for (;;) {
         enqueue_bh(c, p, levels, timed, list_only);
@@ -187,6 +220,7 @@ int accel_eager(nbg_ctx* c, const nbg_params& p, nbg_stats* stats, bool list_onl
     c->have_tree = true;
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
+        stats->particle_evals = c->n;
         fill_tree_stats(c, stats);
         stats->ms_keys = elapsed(c->ev[0], c->ev[1]);
         stats->ms_sort = elapsed(c->ev[1], c->ev[2]);
@@ -232,7 +266,12 @@ void enqueue_step(nbg_ctx* c, const nbg_params& p, int levels, bool timed) {
     const float dt = static_cast<float>(p.dt);
     const float hdt = static_cast<float>(0.5 * p.dt);
     if (timed) rec(c, 8);
-    launch_kick_drift(c, hdt, dt);
+    if (p.adaptive) {
+        launch_adaptive_dt(c, p);
+        launch_kick_drift_dev(c);
+    } else {
+        launch_kick_drift(c, hdt, dt);
+    }
     if (timed) rec(c, 9);
     if (p.solver == NBG_SOLVER_DIRECT) {
         if (timed) rec(c, 4);
@@ -241,10 +280,14 @@ void enqueue_step(nbg_ctx* c, const nbg_params& p, int levels, bool timed) {
     } else {
         nbg_params q = p;
         q.flags &= ~NBG_FLAG_COUNT;
+        if (p.mac == NBG_MAC_RELATIVE) launch_aold(c);
         enqueue_bh(c, q, levels, timed);
     }
     if (timed) rec(c, 10);
-    launch_kick(c, hdt);
+    if (p.adaptive)
+        launch_kick_dev(c);
+    else
+        launch_kick(c, hdt);
     if (timed) rec(c, 11);
 }
 
@@ -283,7 +326,8 @@ void drop_graph(nbg_ctx* c) {
 bool same_params(const nbg_params& a, const nbg_params& b) {
     return a.solver == b.solver && a.multipole == b.multipole && a.G == b.G && a.eps2 == b.eps2 &&
            a.theta == b.theta && a.dt == b.dt && (a.flags & ~NBG_FLAG_COUNT) == (b.flags & ~NBG_FLAG_COUNT) &&
-           a.learn_band == b.learn_band;
+           a.learn_band == b.learn_band && a.softening == b.softening && a.mac == b.mac && a.alpha == b.alpha &&
+           a.adaptive == b.adaptive && a.eta == b.eta;
 }
 
 // Eager steps with a scalar read-back per step to catch the deep-key case.
@@ -293,7 +337,12 @@ int steps_eager(nbg_ctx* c, const nbg_params& p, int nsteps, nbg_stats* stats) {
     const bool timed = (p.flags & NBG_FLAG_STAGE_TIMES) && stats;
     for (int s = 0; s < nsteps; ++s) {
         if (timed) rec(c, 8);
-        launch_kick_drift(c, hdt, dt);
+        if (p.adaptive) {
+            launch_adaptive_dt(c, p);
+            launch_kick_drift_dev(c);
+        } else {
+            launch_kick_drift(c, hdt, dt);
+        }
         if (timed) rec(c, 9);
         if (int r = check_launch(c)) return r;
         nbg_params q = p;
@@ -301,7 +350,10 @@ int steps_eager(nbg_ctx* c, const nbg_params& p, int nsteps, nbg_stats* stats) {
         nbg_stats one;
         if (int r = accel_eager(c, q, timed ? &one : nullptr)) return r;
         if (timed) rec(c, 10);
-        launch_kick(c, hdt);
+        if (p.adaptive)
+            launch_kick_dev(c);
+        else
+            launch_kick(c, hdt);
         if (timed) rec(c, 11);
         if (int r = check_launch(c)) return r;
         if (timed) {
@@ -314,6 +366,85 @@ int steps_eager(nbg_ctx* c, const nbg_params& p, int nsteps, nbg_stats* stats) {
             stats->ms_learned += one.ms_learned;
             stats->ms_integrate += elapsed(c->ev[8], c->ev[9]) + elapsed(c->ev[10], c->ev[11]);
         }
+    }
+    return NBG_OK;
+}
+
+// Tree of the current positions (keys, sort, build, gather, moments), with the two-word
+// key re-run when a depth-21 cell still holds more than ncrit particles.
+int build_tree_eager(nbg_ctx* c, const nbg_params& p) {
+    int levels = kLevelsShort;
+    for (;;) {
+        launch_bbox_and_keys(c, levels);
+        launch_sort(c, levels);
+        launch_build(c, levels);
+        launch_gather(c, static_cast<float>(p.G));
+        launch_upsweep(c, levels, p.multipole, p.G, false);
+        if (int r = check_launch(c)) return r;
+        if (int r = read_scalars(c)) return r;
+        if (c->hsc->cap_exceeded) {
+            c->set_error("node pool capacity exceeded");
+            return NBG_E_CAPACITY;
+        }
+        if (c->hsc->overflow && levels == kLevelsShort) {
+            levels = kLevelsDeep;
+            continue;
+        }
+        break;
+    }
+    c->last_levels = levels;
+    c->last_G = p.G;
+    c->have_tree = true;
+    return NBG_OK;
+}
+
+// nsteps calls of System::BlockStep (System.hpp:239-294): rungs from a(t), then 2^rmax
+// substeps of opening kicks, a global drift, a force pass over the closing set (fresh
+// tree + per-particle geometric walk, or the direct sum onto those targets) and closing
+// kicks.  The last substep's closing set is every particle, so a(t + dt) is complete.
+int block_steps(nbg_ctx* c, const nbg_params& p, int nsteps, nbg_stats* stats) {
+    const double base = p.dt;
+    const int max_rung = std::max(0, p.block_max_rung);
+    nbg_params q = p;
+    q.flags &= ~(NBG_FLAG_COUNT | NBG_FLAG_LEARNED);
+    q.mac = NBG_MAC_GEOMETRIC;  // ComputeAccelBarnesHutTargets always takes the geometric MAC
+    q.block_max_rung = max_rung;
+    NBG_HIP_CHECK(c, hipMemsetAsync(&c->dsc->particle_evals, 0, sizeof(uint64_t), c->stream));
+    int rmax = 0;
+    for (int s = 0; s < nsteps; ++s) {
+        launch_rungs(c, q);
+        if (int r = check_launch(c)) return r;
+        NBG_HIP_CHECK(c, hipMemcpyAsync(&c->hsc->rung_max, &c->dsc->rung_max, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                        c->stream));
+        NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        rmax = c->hsc->rung_max;
+        c->have_rungs = true;
+        const int nsub = 1 << rmax;
+        const float dt_min = static_cast<float>(base / nsub);
+        for (int sub = 0; sub < nsub; ++sub) {
+            launch_block_kick(c, q, sub, rmax, false);
+            launch_drift(c, dt_min);
+            if (p.solver == NBG_SOLVER_BARNES_HUT) {
+                if (int r = build_tree_eager(c, q)) return r;
+                launch_active_list(c, sub, rmax, true);
+                launch_walk_pp(c, q, c->tlist, &c->dsc->tcount, nullptr);
+            } else {
+                launch_active_list(c, sub, rmax, false);
+                launch_direct_targets(c, q, c->n, c->tlist, &c->dsc->tcount, nullptr);
+                c->have_tree = false;
+            }
+            launch_block_kick(c, q, sub, rmax, true);
+            if (int r = check_launch(c)) return r;
+        }
+    }
+    if (stats) {
+        NBG_HIP_CHECK(c, hipMemcpyAsync(&c->hsc->particle_evals, &c->dsc->particle_evals, sizeof(uint64_t),
+                                        hipMemcpyDeviceToHost, c->stream));
+        NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        stats->t_advanced = base * nsteps;
+        stats->dt_last = base;
+        stats->rung_max = rmax;
+        stats->particle_evals = static_cast<int64_t>(c->hsc->particle_evals);
     }
     return NBG_OK;
 }
@@ -378,6 +509,11 @@ nbg_ctx* nbg_create(int device, int64_t max_n) {
     r |= alloc_into(c, c->depth_nodes, C);
     r |= alloc_into(c, c->cnt_far, C);
     r |= alloc_into(c, c->cnt_near, C);
+    r |= alloc_into(c, c->aold, N);
+    r |= alloc_into(c, c->rung, N);
+    r |= alloc_into(c, c->tlist, N);
+    r |= alloc_into(c, c->list_off, N / 1024 + 2);
+    r |= alloc_into(c, c->red_part, kReduceBlocks);
     r |= alloc_into(c, c->dsc, 1);
     c->diag_part_cap = static_cast<int>(diag_partials_needed(max_n));
     r |= alloc_into(c, c->diag_part, static_cast<size_t>(c->diag_part_cap));
@@ -386,7 +522,9 @@ nbg_ctx* nbg_create(int device, int64_t max_n) {
         return nullptr;
     }
     // the walk reads up to 8 particles past a leaf start; the pad stays zero (zero mass)
-    if (hipMemset(c->spos, 0, (N + kSposPad) * sizeof(float4)) != hipSuccess) {
+    if (hipMemset(c->spos, 0, (N + kSposPad) * sizeof(float4)) != hipSuccess ||
+        hipMemset(c->rung, 0, N * sizeof(int32_t)) != hipSuccess ||
+        hipMemset(c->dsc, 0, sizeof(DevScalars)) != hipSuccess) {
         nbg_destroy(c);
         return nullptr;
     }
@@ -417,6 +555,7 @@ void nbg_destroy(nbg_ctx* c) {
     free_dev(c->nd_com); free_dev(c->nd_quad); free_dev(c->nd_aux); free_dev(c->nd_com64); free_dev(c->nd_rmax64);
     free_dev(c->depth_nodes); free_dev(c->cnt_far); free_dev(c->cnt_near);
     free_dev(c->dsc); free_dev(c->diag_part);
+    free_dev(c->aold); free_dev(c->rung); free_dev(c->tlist); free_dev(c->list_off); free_dev(c->red_part);
     free_dev(c->nd_mom); free_dev(c->lcorr); free_dev(c->lpairs); free_dev(c->lfeat); free_dev(c->lscale); free_dev(c->lw32); free_dev(c->lw16);
     free_dev(c->adam_m); free_dev(c->adam_v); free_dev(c->pool_feat); free_dev(c->pool_tgt); free_dev(c->amag2);
     free_dev(c->grad_part); free_dev(c->loss_part); free_dev(c->loss_dev);
@@ -524,9 +663,11 @@ int nbg_upload_batch(nbg_ctx* c, int32_t nsys, const int64_t* offsets, const flo
         NBG_HIP_CHECK(c, hipMemset(c->ay, 0, bytes));
         NBG_HIP_CHECK(c, hipMemset(c->az, 0, bytes));
     }
+    if (n > 0) NBG_HIP_CHECK(c, hipMemset(c->rung, 0, static_cast<size_t>(n) * sizeof(int32_t)));
     NBG_HIP_CHECK(c, hipDeviceSynchronize());
     c->have_state = true;
     c->have_tree = false;
+    c->have_rungs = false;
     return NBG_OK;
 }
 
@@ -582,6 +723,20 @@ int nbg_step(nbg_ctx* c, const nbg_params* p, int32_t nsteps, nbg_stats* stats) 
     if (stats) std::memset(stats, 0, sizeof(*stats));
     c->last_G = p->G;
     NBG_HIP_CHECK(c, hipEventRecord(c->ev[6], c->stream));
+    if (p->block) {
+        if (int r = block_steps(c, *p, nsteps, stats)) return r;
+        NBG_HIP_CHECK(c, hipEventRecord(c->ev[7], c->stream));
+        NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        if (stats) {
+            if (p->solver == NBG_SOLVER_BARNES_HUT) {
+                if (int r = read_scalars(c)) return r;
+                fill_tree_stats(c, stats);
+            }
+            stats->ms_total = elapsed(c->ev[6], c->ev[7]);
+        }
+        return NBG_OK;
+    }
+    if (p->adaptive) NBG_HIP_CHECK(c, hipMemsetAsync(&c->dsc->t_accum, 0, sizeof(double), c->stream));
     const bool use_graph = !(p->flags & NBG_FLAG_NO_GRAPH) && c->last_levels == kLevelsShort;
     if (!use_graph) {
         if (int r = steps_eager(c, *p, nsteps, stats)) return r;
@@ -631,6 +786,7 @@ int nbg_step(nbg_ctx* c, const nbg_params* p, int32_t nsteps, nbg_stats* stats) 
         if (overflow) {
             if (int r = restore_backup(c)) return r;
             if (stats) std::memset(stats, 0, sizeof(*stats));
+            if (p->adaptive) NBG_HIP_CHECK(c, hipMemsetAsync(&c->dsc->t_accum, 0, sizeof(double), c->stream));
             if (int r = steps_eager(c, *p, nsteps, stats)) return r;
         }
     }
@@ -641,6 +797,14 @@ int nbg_step(nbg_ctx* c, const nbg_params* p, int32_t nsteps, nbg_stats* stats) 
         if (int r = read_scalars(c)) return r;
         fill_tree_stats(c, stats);
         stats->ms_total = elapsed(c->ev[6], c->ev[7]);
+        stats->particle_evals = static_cast<int64_t>(nsteps) * c->n;
+        if (p->adaptive) {
+            stats->t_advanced = c->hsc->t_accum;
+            stats->dt_last = c->hsc->dt_step;
+        } else {
+            stats->t_advanced = p->dt * nsteps;
+            stats->dt_last = p->dt;
+        }
     }
     return NBG_OK;
 }
@@ -787,7 +951,7 @@ int nbg_accel_direct_targets(nbg_ctx* c, const nbg_params* p, int64_t nt, const 
     int r = NBG_OK;
     if (hipMemcpy(dt, t32.data(), t32.size() * 4, hipMemcpyHostToDevice) != hipSuccess) r = NBG_E_HIP;
     if (r == NBG_OK) {
-        launch_direct_targets(c, *p, nt, dt, dout);
+        launch_direct_targets(c, *p, nt, dt, nullptr, dout);
         r = check_launch(c);
     }
     std::vector<float> h(static_cast<size_t>(nt) * 3);
@@ -805,6 +969,68 @@ int nbg_accel_direct_targets(nbg_ctx* c, const nbg_params* p, int64_t nt, const 
         az[k] = h[3 * static_cast<size_t>(k) + 2];
     }
     c->have_tree = false;
+    return NBG_OK;
+}
+
+int nbg_accel_bh_targets(nbg_ctx* c, const nbg_params* p, int64_t nt, const int64_t* targets, float* ax, float* ay,
+                         float* az) {
+    if (!c) return NBG_E_INVALID;
+    if (int r = validate(c, p)) return r;
+    if (p->solver != NBG_SOLVER_BARNES_HUT || (p->flags & NBG_FLAG_LEARNED)) {
+        c->set_error("nbg_accel_bh_targets needs the Barnes-Hut solver without the learned correction");
+        return NBG_E_INVALID;
+    }
+    if (nt < 0 || nt > c->cap_n || (nt > 0 && (!targets || !ax || !ay || !az))) {
+        c->set_error("bad target list");
+        return NBG_E_INVALID;
+    }
+    for (int64_t k = 0; k < nt; ++k)
+        if (targets[k] < 0 || targets[k] >= c->n) {
+            c->set_error("target index out of range");
+            return NBG_E_INVALID;
+        }
+    if (nt == 0) return NBG_OK;
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    nbg_params q = *p;
+    q.mac = NBG_MAC_GEOMETRIC;
+    q.flags &= ~NBG_FLAG_COUNT;
+    if (int r = build_tree_eager(c, q)) return r;
+    // original index -> sorted position
+    const size_t n = static_cast<size_t>(c->n);
+    std::vector<uint32_t> perm(n);
+    std::vector<int32_t> inv(n), list(static_cast<size_t>(nt));
+    NBG_HIP_CHECK(c, hipMemcpy(perm.data(), c->perm, n * 4, hipMemcpyDeviceToHost));
+    for (size_t pos = 0; pos < n; ++pos) inv[perm[pos]] = static_cast<int32_t>(pos);
+    for (int64_t k = 0; k < nt; ++k) list[static_cast<size_t>(k)] = inv[static_cast<size_t>(targets[k])];
+    const uint32_t cnt = static_cast<uint32_t>(nt);
+    NBG_HIP_CHECK(c, hipMemcpy(c->tlist, list.data(), list.size() * 4, hipMemcpyHostToDevice));
+    NBG_HIP_CHECK(c, hipMemcpy(&c->dsc->tcount, &cnt, sizeof(cnt), hipMemcpyHostToDevice));
+    float* dout = nullptr;
+    NBG_HIP_CHECK(c, hipMalloc(&dout, list.size() * 12));
+    launch_walk_pp(c, q, c->tlist, &c->dsc->tcount, dout);
+    int r = check_launch(c);
+    std::vector<float> h(list.size() * 3);
+    if (r == NBG_OK && hipStreamSynchronize(c->stream) != hipSuccess) r = NBG_E_HIP;
+    if (r == NBG_OK && hipMemcpy(h.data(), dout, h.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) r = NBG_E_HIP;
+    (void)hipFree(dout);
+    if (r != NBG_OK) {
+        if (c->err.empty()) c->set_error("Barnes-Hut targets evaluation failed");
+        return r;
+    }
+    for (int64_t k = 0; k < nt; ++k) {
+        ax[k] = h[3 * static_cast<size_t>(k) + 0];
+        ay[k] = h[3 * static_cast<size_t>(k) + 1];
+        az[k] = h[3 * static_cast<size_t>(k) + 2];
+    }
+    return NBG_OK;
+}
+
+int nbg_rungs(nbg_ctx* c, int32_t* rungs) {
+    if (!c || !rungs) return NBG_E_INVALID;
+    NBG_HIP_CHECK(c, hipSetDevice(c->device));
+    NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if (c->n > 0)
+        NBG_HIP_CHECK(c, hipMemcpy(rungs, c->rung, static_cast<size_t>(c->n) * 4, hipMemcpyDeviceToHost));
     return NBG_OK;
 }
 
@@ -1148,7 +1374,10 @@ int nbg_autotune(nbg_ctx* c, const nbg_params* p, int32_t reps, nbg_tuning* best
         for (int u : {2, 4, 8}) cand.push_back(nbg_tuning{cur.walk_kernel, cur.near_tile, u, 0});
     } else {
         for (int t : {4, 8, 16}) cand.push_back(nbg_tuning{0, t, cur.direct_tile, 0});
-        if (!(p->flags & NBG_FLAG_LEARNED)) cand.push_back(nbg_tuning{1, 8, cur.direct_tile, 0});
+        // the leaf-group walk runs Plummer softening with the grouped geometric walk only
+        if (!(p->flags & (NBG_FLAG_LEARNED | NBG_FLAG_PER_PARTICLE)) && p->softening == NBG_SOFT_PLUMMER &&
+            p->mac == NBG_MAC_GEOMETRIC)
+            cand.push_back(nbg_tuning{1, 8, cur.direct_tile, 0});
     }
     nbg_params q = *p;
     q.flags &= ~NBG_FLAG_COUNT;

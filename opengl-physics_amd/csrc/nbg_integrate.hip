@@ -5,6 +5,7 @@
 // run as one HBM pass (bit-identical per particle to running them as two loops).
 // Diagnostics follow System::TotalEnergy / CenterOfMass / AngularMomentum
 // (projects/nbody_core/include/ngrav/System.hpp:119-193) in FP64.
+#include <cmath>
 #include <vector>
 
 #include "nbg_common.hpp"
@@ -107,13 +108,26 @@ __global__ __launch_bounds__(kDiagThreads) void k_diag_moments(const float* __re
     }
 }
 
+// phi(q) h / (G m) of the cubic-spline kernel, SplinePotential3D
+// (projects/nbody_core/include/ngrav/Softening.hpp:49-56).
+__device__ __forceinline__ double spline_phi(double q) {
+    if (q >= 2.0) return -1.0 / q;
+    if (q < 1.0) return -(1.4 - (2.0 / 3.0) * q * q + 0.3 * q * q * q * q - 0.1 * q * q * q * q * q);
+    const double t = 2.0 - q;
+    const double t2 = t * t, t4 = t2 * t2;
+    const double F = t4 / 4.0 - t4 * t / 5.0 + t4 * t2 / 24.0;
+    return -((1.0 - 4.0 * F) / q + t4 / 2.0 - t4 * t / 5.0);
+}
+
 // Pairwise potential, all ordered pairs i != j (each unordered pair twice, halved on the
-// host): -G m_i m_j / sqrt(r^2 + eps2), FP64.  Sources stream through scalar loads.
+// host), FP64: Plummer -G m_i m_j / sqrt(r^2 + eps2); spline G m_i m_j phi(r / h) / h
+// (System::TotalEnergy, System.hpp:141-162).  Sources stream through scalar loads.
+template <bool SPL>
 __global__ __launch_bounds__(kDiagThreads) void k_diag_potential(const float* __restrict__ x,
                                                                  const float* __restrict__ y,
                                                                  const float* __restrict__ z,
                                                                  const float* __restrict__ m, int lo, int hi,
-                                                                 double eps2, double* __restrict__ part) {
+                                                                 double eps2, double h, double* __restrict__ part) {
     __shared__ double s_red[kDiagThreads / 64];
     const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < hi;
@@ -124,15 +138,22 @@ __global__ __launch_bounds__(kDiagThreads) void k_diag_potential(const float* __
         const double dx = static_cast<double>(x[j]) - xi;
         const double dy = static_cast<double>(y[j]) - yi;
         const double dz = static_cast<double>(z[j]) - zi;
-        const double r2 = dx * dx + dy * dy + dz * dz + eps2;
-        // 1/sqrt(r2) to FP64 accuracy: hardware estimate + two Newton steps
-        double r = __builtin_amdgcn_rsq(r2);
-        r = r * (1.5 - 0.5 * r2 * r * r);
-        r = r * (1.5 - 0.5 * r2 * r * r);
-        const double term = static_cast<double>(m[j]) * r;
+        double term;
+        if (SPL) {
+            const double r = __dsqrt_rn(dx * dx + dy * dy + dz * dz);
+            const double q = (r <= 1e-300) ? 0.0 : r / h;
+            term = static_cast<double>(m[j]) * spline_phi(q);
+        } else {
+            const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+            // 1/sqrt(r2) to FP64 accuracy: hardware estimate + two Newton steps
+            double r = __builtin_amdgcn_rsq(r2);
+            r = r * (1.5 - 0.5 * r2 * r * r);
+            r = r * (1.5 - 0.5 * r2 * r * r);
+            term = -static_cast<double>(m[j]) * r;
+        }
         acc += (j == i) ? 0.0 : term;
     }
-    acc *= -mi;
+    acc *= SPL ? mi / h : mi;
     const double t = block_sum_d(acc, s_red);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
@@ -164,8 +185,12 @@ int diagnostics(nbg_ctx* c, const nbg_params& p, int with_potential, double* out
     double pot = 0.0;
     if (with_potential && hi > lo) {
         const int npb = grid_for(hi - lo, kDiagThreads);
-        hipLaunchKernelGGL(k_diag_potential, dim3(npb), dim3(kDiagThreads), 0, c->stream, c->x, c->y, c->z, c->m, lo,
-                           hi, p.eps2, c->diag_part);
+        if (p.softening == NBG_SOFT_SPLINE)
+            hipLaunchKernelGGL(k_diag_potential<true>, dim3(npb), dim3(kDiagThreads), 0, c->stream, c->x, c->y, c->z,
+                               c->m, lo, hi, p.eps2, std::sqrt(p.eps2), c->diag_part);
+        else
+            hipLaunchKernelGGL(k_diag_potential<false>, dim3(npb), dim3(kDiagThreads), 0, c->stream, c->x, c->y,
+                               c->z, c->m, lo, hi, p.eps2, 1.0, c->diag_part);
         std::vector<double> hp(static_cast<size_t>(npb));
         NBG_HIP_CHECK(c, hipMemcpyAsync(hp.data(), c->diag_part, hp.size() * sizeof(double), hipMemcpyDeviceToHost,
                                         c->stream));

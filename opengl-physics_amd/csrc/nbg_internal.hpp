@@ -48,6 +48,11 @@ struct DevScalars {
     uint32_t learn_pairs;       // (leaf, cell) pairs listed by the walk for the learned correction
     int32_t sticky_learn;       // the pair list overflowed during a graph batch (host clears)
     int32_t pad2[2];
+    double dt_step;             // adaptive steps: dt of the current step
+    double t_accum;             // adaptive steps: time advanced since the host cleared it
+    uint64_t particle_evals;    // block steps: targets evaluated since the host cleared it
+    uint32_t tcount;            // entries of the current target list
+    int32_t rung_max;           // block steps: deepest rung of the coarse step
 };
 
 // ---------------------------------------------------------------------------- helpers
@@ -160,6 +165,14 @@ struct nbg_ctx {
     double* diag_part = nullptr;
     int diag_part_cap = 0;
 
+    // per-particle walk, adaptive and block steps
+    double* aold = nullptr;          // previous |a| per particle (relative MAC)
+    int32_t* rung = nullptr;         // block-step rung per particle (original order)
+    int32_t* tlist = nullptr;        // target list (sorted positions or original indices)
+    uint32_t* list_off = nullptr;    // per-tile counts / offsets of the list compaction
+    double* red_part = nullptr;      // block partials of the dt / rung reductions
+    bool have_rungs = false;
+
     // timing
     hipEvent_t ev[14] = {};
 
@@ -208,7 +221,21 @@ void launch_fill_i32(nbg_ctx* c, int32_t* p, int64_t n, int32_t v);
 // walk.hip
 void launch_walk(nbg_ctx* c, const nbg_params& p);
 void launch_direct(nbg_ctx* c, const nbg_params& p);
-void launch_direct_targets(nbg_ctx* c, const nbg_params& p, int64_t nt, const int32_t* targets, float* out3);
+void launch_direct_targets(nbg_ctx* c, const nbg_params& p, int64_t nt, const int32_t* targets,
+                           const uint32_t* nt_dev, float* out3);
+void launch_pack(nbg_ctx* c, double G);
+// walk_pp.hip
+void launch_walk_pp(nbg_ctx* c, const nbg_params& p, const int32_t* tlist, const uint32_t* tcount, float* out3);
+void launch_aold(nbg_ctx* c);
+// steps.hip
+void launch_adaptive_dt(nbg_ctx* c, const nbg_params& p);
+void launch_kick_drift_dev(nbg_ctx* c);
+void launch_kick_dev(nbg_ctx* c);
+void launch_rungs(nbg_ctx* c, const nbg_params& p);
+void launch_block_kick(nbg_ctx* c, const nbg_params& p, int sub, int rmax, bool closing);
+void launch_drift(nbg_ctx* c, float dt);
+void launch_active_list(nbg_ctx* c, int sub, int rmax, bool sorted);
+constexpr int kReduceBlocks = 1024;  // partials of the steps' reductions
 // integrate.hip
 void launch_kick(nbg_ctx* c, float hdt);
 void launch_kick_drift(nbg_ctx* c, float hdt, float dt);

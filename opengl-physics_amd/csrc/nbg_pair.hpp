@@ -1,0 +1,115 @@
+// nbg_pair.hpp -- pair kernels and load helpers shared by the force kernels (gfx950).
+//
+// Pair kernels (FP32):
+//   Plummer  AccumPairPlummer   projects/nbody_core/src/Solvers.cpp:19-38
+//   spline   AccumPairSpline    projects/nbody_core/src/Solvers.cpp:43-63 with
+//            SplineChi3D        projects/nbody_core/include/ngrav/Softening.hpp:39-45
+//   monopole M2P = the same pair kernel with the cell's G*mass at its centre of mass
+//   quadrupole                  projects/nbody_core/src/Solvers.cpp:147-179 (unsoftened)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/nbg.h"
+
+namespace nbg {
+
+// Wave-uniform reads of data no kernel writes while a force kernel runs (tree and sorted
+// particles): going through the constant address space lets the compiler issue them as
+// scalar (SMEM) loads into SGPRs, one per wave instead of one per lane.
+typedef __attribute__((address_space(4))) const float cfloat;
+typedef __attribute__((address_space(4))) const int32_t cint32;
+typedef __attribute__((address_space(4))) const uint32_t cuint32;
+__device__ __forceinline__ float4 cld(const float4* p, int i) {
+    cfloat* q = (cfloat*)(p + i);
+    return make_float4(q[0], q[1], q[2], q[3]);
+}
+__device__ __forceinline__ int32_t cld(const int32_t* p, int i) { return ((cint32*)p)[i]; }
+__device__ __forceinline__ uint32_t cld_u32(const void* p, int i) { return ((cuint32*)p)[i]; }
+
+template <bool KAHAN>
+__device__ __forceinline__ void kadd(float& s, float& c, float v) {
+    if (KAHAN) {
+        const float y = v - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    } else {
+        s += v;
+    }
+}
+
+// Softening constants of one evaluation.  Spline: h = sqrt(eps2) (Softening::Spline(h)
+// stores eps = h, eps2 = h^2; sqrt of a correctly rounded square returns h exactly).
+struct Soft {
+    float eps2;    // Plummer eps^2
+    float inv_h;   // spline 1/h
+    float inv_h3;  // spline 1/h^3
+};
+
+inline Soft make_soft(const nbg_params& p) {
+    Soft s;
+    s.eps2 = static_cast<float>(p.eps2);
+    const double h = std::sqrt(p.eps2);
+    const double ih = (h > 0.0) ? 1.0 / h : 0.0;
+    s.inv_h = static_cast<float>(ih);
+    s.inv_h3 = static_cast<float>(ih * ih * ih);
+    return s;
+}
+
+// Coefficient f of a pair: the acceleration of a target from a source of G*m = gm at
+// displacement d = source - target (r2 = d.d, unsoftened) is f * d.
+//   Plummer: f = gm / (r^2 + eps^2)^(3/2).
+//   Spline:  f = gm chi(q) / r^3, q = r/h, chi the enclosed-mass fraction of the cubic
+//            spline kernel: chi = 1 for q >= 2; 1 - t^4 + 0.8 t^5 - t^6/6 (t = 2 - q) for
+//            1 <= q < 2; q^3 (4/3 - 1.2 q^2 + 0.5 q^3) for q < 1, where chi / r^3 =
+//            (4/3 - 1.2 q^2 + 0.5 q^3) / h^3 needs no division and stays finite at r = 0
+//            (d = 0 there, so the term vanishes as the reference's r <= 1e-300 branch makes it).
+template <bool SPLINE>
+__device__ __forceinline__ float pair_coef(float r2, float gm, const Soft& s) {
+    if (!SPLINE) {
+        const float inv = __builtin_amdgcn_rsqf(r2 + s.eps2);
+        return gm * (inv * inv * inv);
+    } else {
+        const float q = sqrtf(r2) * s.inv_h;
+        const float inv = __builtin_amdgcn_rsqf(r2);  // used only where q >= 1 (r2 > 0)
+        const float t = 2.f - q;
+        const float t2 = t * t;
+        const float chi_out = (q >= 2.f) ? 1.f : 1.f - (t2 * t2) * (1.f - t * (0.8f - t * (1.f / 6.f)));
+        const float f_out = gm * chi_out * (inv * inv * inv);
+        const float f_in = gm * s.inv_h3 * (4.f / 3.f + (q * q) * (0.5f * q - 1.2f));
+        return (q < 1.f) ? f_in : f_out;
+    }
+}
+
+// M2P of one cell (com + G*mass in cm, G*(Qxx,Qyy,Qxy,Qxz) in q, G*Qyz in qyz) onto a
+// target at (tx,ty,tz): softened monopole plus the unsoftened traceless quadrupole.
+template <bool QUAD, bool SPLINE>
+__device__ __forceinline__ void m2p_cell(float4 cm, float4 q, float qyz, float tx, float ty, float tz,
+                                         const Soft& soft, float& ex, float& ey, float& ez) {
+    const float dx = cm.x - tx, dy = cm.y - ty, dz = cm.z - tz;
+    const float dist2 = dx * dx + dy * dy + dz * dz;
+    const float f = pair_coef<SPLINE>(dist2, cm.w, soft);
+    ex = f * dx;
+    ey = f * dy;
+    ez = f * dz;
+    if (QUAD) {
+        const float qxx = q.x, qyy = q.y, qxy = q.z, qxz = q.w, qzz = -(qxx + qyy);
+        const float qdx = qxx * dx + qxy * dy + qxz * dz;
+        const float qdy = qxy * dx + qyy * dy + qyz * dz;
+        const float qdz = qxz * dx + qyz * dy + qzz * dz;
+        const float qd2 = dx * qdx + dy * qdy + dz * qdz;
+        const float ir = __builtin_amdgcn_rsqf(dist2);
+        const float ir2 = ir * ir;
+        const float ir5 = ir2 * ir2 * ir;
+        const float c2 = 2.5f * qd2 * (ir5 * ir2);
+        ex += c2 * dx - ir5 * qdx;
+        ey += c2 * dy - ir5 * qdy;
+        ez += c2 * dz - ir5 * qdz;
+    }
+}
+
+}  // namespace nbg

@@ -20,6 +20,7 @@
 // compensated (Kahan) accumulation.
 #include "nbg_common.hpp"
 #include "nbg_internal.hpp"
+#include "nbg_pair.hpp"
 
 namespace nbg {
 
@@ -36,18 +37,6 @@ struct Acc {
     float x[kMaxT], y[kMaxT], z[kMaxT];
     float cx[kMaxT], cy[kMaxT], cz[kMaxT];  // Kahan compensation
 };
-
-template <bool KAHAN>
-__device__ __forceinline__ void kadd(float& s, float& c, float v) {
-    if (KAHAN) {
-        const float y = v - c;
-        const float t = s + y;
-        c = (t - s) - y;
-        s = t;
-    } else {
-        s += v;
-    }
-}
 
 // P2P source (sx,sy,sz, G*m) onto all targets.  `self` = target slot to skip (-1 none).
 template <bool KAHAN>
@@ -345,18 +334,6 @@ __global__ __launch_bounds__(64 * kWalkWaves) void k_walk(WalkArgs A) {
 // looping over the bin's leaves (leaf data broadcast from LDS).
 constexpr int kBinWaves = 4;
 
-// Wave-uniform reads of data no kernel writes while the walk runs (tree and sorted
-// particles): going through the constant address space lets the compiler issue them as
-// scalar (SMEM) loads into SGPRs, one per wave instead of one per lane.
-typedef __attribute__((address_space(4))) const float cfloat;
-typedef __attribute__((address_space(4))) const int32_t cint32;
-typedef __attribute__((address_space(4))) const uint32_t cuint32;
-__device__ __forceinline__ float4 cld(const float4* p, int i) {
-    cfloat* q = (cfloat*)(p + i);
-    return make_float4(q[0], q[1], q[2], q[3]);
-}
-__device__ __forceinline__ int32_t cld(const int32_t* p, int i) { return ((cint32*)p)[i]; }
-__device__ __forceinline__ uint32_t cld_u32(const void* p, int i) { return ((cuint32*)p)[i]; }
 constexpr int kBinStack = 640;
 constexpr int kBinRing = 128;  // power of two; far and near rings
 
@@ -384,7 +361,7 @@ struct BinArgs {
     float* az;
     int32_t* cnt_far;
     int32_t* cnt_near;
-    float eps2;
+    Soft soft;
     float inv_theta;
     double theta2_64;
     // learned correction: accepted pairs with size / (dist - R) > band are listed
@@ -412,7 +389,7 @@ __device__ __forceinline__ int wave_excl_small(int v, int* total) {
     return off;
 }
 
-template <bool QUAD, bool KAHAN, bool COUNT, bool LEARN, int NT>
+template <bool QUAD, bool KAHAN, bool COUNT, bool LEARN, int NT, bool SPL>
 __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     __shared__ int s_snode[kBinWaves][kBinStack];
     __shared__ uint64_t s_smask[kBinWaves][kBinStack];
@@ -436,7 +413,8 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     uint64_t* qmask = s_qmask[wave];
     float4* lcom = s_lcom[wave];
     int* lid = s_lid[wave];
-    const float eps2 = A.eps2, inv_theta = A.inv_theta;
+    const Soft soft = A.soft;
+    const float inv_theta = A.inv_theta;
     // XCD-aware work distribution: workgroups are dispatched round-robin over the 8 XCDs,
     // so XCD x takes the contiguous bin range [x*per, (x+1)*per) and its waves pull bins
     // from it in order through one counter per XCD (neighbouring bins share tree nodes in
@@ -519,30 +497,6 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
         int sp = 1;
         int f_head = 0, f_tail = 0, q_head = 0, q_tail = nl;
 
-        // M2P of one far cell onto this lane's target
-        auto m2p_one = [&](float4 cm, float4 q, float qyz, float& ex, float& ey, float& ez) {
-            const float dx = cm.x - tp.x, dy = cm.y - tp.y, dz = cm.z - tp.z;
-            const float dist2 = dx * dx + dy * dy + dz * dz;
-            const float inv = __builtin_amdgcn_rsqf(dist2 + eps2);
-            const float f = cm.w * (inv * inv * inv);
-            ex = f * dx;
-            ey = f * dy;
-            ez = f * dz;
-            if (QUAD) {
-                const float qxx = q.x, qyy = q.y, qxy = q.z, qxz = q.w, qzz = -(qxx + qyy);
-                const float qdx = qxx * dx + qxy * dy + qxz * dz;
-                const float qdy = qxy * dx + qyy * dy + qyz * dz;
-                const float qdz = qxz * dx + qyz * dy + qzz * dz;
-                const float qd2 = dx * qdx + dy * qdy + dz * qdz;
-                const float ir = __builtin_amdgcn_rsqf(dist2);
-                const float ir2 = ir * ir;
-                const float ir5 = ir2 * ir2 * ir;
-                const float c2 = 2.5f * qd2 * (ir5 * ir2);
-                ex += c2 * dx - ir5 * qdx;
-                ey += c2 * dy - ir5 * qdy;
-                ez += c2 * dz - ir5 * qdz;
-            }
-        };
         // evaluates `n` staged far cells starting at ring position `h`, four at a time
         // (their scalar loads issued before any arithmetic); every lane computes every
         // cell and keeps it only if its leaf's bit is in the cell's mask
@@ -568,7 +522,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     float ex, ey, ez;
-                    m2p_one(cm[u], q[u], qyz[u], ex, ey, ez);
+                    m2p_cell<QUAD, SPL>(cm[u], q[u], qyz[u], tp.x, tp.y, tp.z, soft, ex, ey, ez);
                     sx += (um[u] > 0.f) ? ex : 0.f;
                     sy += (um[u] > 0.f) ? ey : 0.f;
                     sz += (um[u] > 0.f) ? ez : 0.f;
@@ -621,9 +575,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 for (int u = 0; u < NT; ++u) {
                     const float4 q = spt[u];
                     const float dx = q.x - tp.x, dy = q.y - tp.y, dz = q.z - tp.z;
-                    const float r2 = dx * dx + dy * dy + dz * dz + eps2;
-                    const float inv = __builtin_amdgcn_rsqf(r2);
-                    float f = q.w * (inv * inv * inv);
+                    float f = pair_coef<SPL>(dx * dx + dy * dy + dz * dz, q.w, soft);
                     f = (us[u] > 0.f && idx[u] != tpos) ? f : 0.f;
                     sx += f * dx;
                     sy += f * dy;
@@ -895,6 +847,46 @@ void launch_walk_leaf(nbg_ctx* c, const nbg_params& p) {
 #undef NBG_WALK
 }
 
+template <bool SPL>
+void dispatch_walk_bins(nbg_ctx* c, const BinArgs& A, bool quad, bool kahan, bool count, bool learn) {
+    // 8 XCDs x 32 CUs x 3 resident blocks
+    const dim3 grid(8 * 32 * 3), block(64 * kBinWaves);
+#define NBG_WALKB(Q, K, C, L, T) hipLaunchKernelGGL((k_walk_bins<Q, K, C, L, T, SPL>), grid, block, 0, c->stream, A)
+    const int nt = c->tune.near_tile;
+    if constexpr (!SPL) {
+        if (learn) {
+            // the learned correction is defined against the quadrupole expansion
+            if (kahan) {
+                if (count) NBG_WALKB(true, true, true, true, 8); else NBG_WALKB(true, true, false, true, 8);
+            } else {
+                if (count) NBG_WALKB(true, false, true, true, 8); else NBG_WALKB(true, false, false, true, 8);
+            }
+            return;
+        }
+    }
+    if (count) {
+        if (quad) {
+            if (kahan) NBG_WALKB(true, true, true, false, 8); else NBG_WALKB(true, false, true, false, 8);
+        } else {
+            if (kahan) NBG_WALKB(false, true, true, false, 8); else NBG_WALKB(false, false, true, false, 8);
+        }
+    } else if (kahan) {
+        // the production variants take the tuned P2P tile
+        if (quad) {
+            if (nt == 4) NBG_WALKB(true, true, false, false, 4);
+            else if (nt == 16) NBG_WALKB(true, true, false, false, 16);
+            else NBG_WALKB(true, true, false, false, 8);
+        } else {
+            if (nt == 4) NBG_WALKB(false, true, false, false, 4);
+            else if (nt == 16) NBG_WALKB(false, true, false, false, 16);
+            else NBG_WALKB(false, true, false, false, 8);
+        }
+    } else {
+        if (quad) NBG_WALKB(true, false, false, false, 8); else NBG_WALKB(false, false, false, false, 8);
+    }
+#undef NBG_WALKB
+}
+
 void launch_walk_bins(nbg_ctx* c, const nbg_params& p) {
     BinArgs A;
     A.n_pad = c->n;
@@ -920,7 +912,7 @@ void launch_walk_bins(nbg_ctx* c, const nbg_params& p) {
     A.az = c->az;
     A.cnt_far = c->cnt_far;
     A.cnt_near = c->cnt_near;
-    A.eps2 = static_cast<float>(p.eps2);
+    A.soft = make_soft(p);
     A.inv_theta = (p.theta > 0.0) ? static_cast<float>(1.0 / p.theta) : __builtin_inff();
     A.theta2_64 = p.theta * p.theta;
     const bool quad = p.multipole == NBG_QUADRUPOLE;
@@ -930,43 +922,19 @@ void launch_walk_bins(nbg_ctx* c, const nbg_params& p) {
     A.inv_band = (p.learn_band > 0.0) ? static_cast<float>(1.0 / p.learn_band) : 1e30f;
     A.lpairs = c->lpairs;
     A.lcap = static_cast<uint32_t>(c->lpairs_cap);
-    // 8 XCDs x 32 CUs x 3 resident blocks
-    const dim3 grid(8 * 32 * 3), block(64 * kBinWaves);
-#define NBG_WALKB(Q, K, C, L, T) hipLaunchKernelGGL((k_walk_bins<Q, K, C, L, T>), grid, block, 0, c->stream, A)
-    const int nt = c->tune.near_tile;
-    if (learn) {
-        // the learned correction is defined against the quadrupole expansion
-        if (kahan) {
-            if (count) NBG_WALKB(true, true, true, true, 8); else NBG_WALKB(true, true, false, true, 8);
-        } else {
-            if (count) NBG_WALKB(true, false, true, true, 8); else NBG_WALKB(true, false, false, true, 8);
-        }
-    } else if (count) {
-        if (quad) {
-            if (kahan) NBG_WALKB(true, true, true, false, 8); else NBG_WALKB(true, false, true, false, 8);
-        } else {
-            if (kahan) NBG_WALKB(false, true, true, false, 8); else NBG_WALKB(false, false, true, false, 8);
-        }
-    } else if (kahan) {
-        // the production variants take the tuned P2P tile
-        if (quad) {
-            if (nt == 4) NBG_WALKB(true, true, false, false, 4);
-            else if (nt == 16) NBG_WALKB(true, true, false, false, 16);
-            else NBG_WALKB(true, true, false, false, 8);
-        } else {
-            if (nt == 4) NBG_WALKB(false, true, false, false, 4);
-            else if (nt == 16) NBG_WALKB(false, true, false, false, 16);
-            else NBG_WALKB(false, true, false, false, 8);
-        }
-    } else {
-        if (quad) NBG_WALKB(true, false, false, false, 8); else NBG_WALKB(false, false, false, false, 8);
-    }
-#undef NBG_WALKB
+    if (p.softening == NBG_SOFT_SPLINE)
+        dispatch_walk_bins<true>(c, A, quad, kahan, count, false);
+    else
+        dispatch_walk_bins<false>(c, A, quad, kahan, count, learn);
 }
 
 void launch_walk(nbg_ctx* c, const nbg_params& p) {
-    // the leaf-group walk has no learned-pair listing
-    if (c->tune.walk_kernel == 1 && !(p.flags & NBG_FLAG_LEARNED))
+    if ((p.flags & NBG_FLAG_PER_PARTICLE) || p.mac == NBG_MAC_RELATIVE) {
+        launch_walk_pp(c, p, nullptr, nullptr, nullptr);
+        return;
+    }
+    // the leaf-group walk has no learned-pair listing and Plummer softening only
+    if (c->tune.walk_kernel == 1 && !(p.flags & NBG_FLAG_LEARNED) && p.softening == NBG_SOFT_PLUMMER)
         launch_walk_leaf(c, p);
     else
         launch_walk_bins(c, p);
@@ -978,59 +946,56 @@ void launch_walk(nbg_ctx* c, const nbg_params& p) {
 // Blocks never straddle systems (a sort block of 2048 keys = 8 direct blocks of 256).
 // ComputeAccelDirect (projects/nbody_core/src/Solvers.cpp:77-106): j == i skipped.
 
-template <bool KAHAN, int U>
+template <bool KAHAN, int U, bool SPL>
 __global__ __launch_bounds__(256) void k_direct(const float4* __restrict__ src, const int32_t* __restrict__ blk_sys,
                                                 const int32_t* __restrict__ blk_start,
                                                 const int32_t* __restrict__ blk_count,
-                                                const int32_t* __restrict__ sys_off, const int32_t* __restrict__ tlist,
-                                                int nt_list, float eps2, float* __restrict__ ax, float* __restrict__ ay,
+                                                const int32_t* __restrict__ sys_off,
+                                                const int32_t* __restrict__ pos_sys, const int32_t* __restrict__ tlist,
+                                                int nt_list, const uint32_t* __restrict__ nt_dev, Soft soft,
+                                                float* __restrict__ ax, float* __restrict__ ay,
                                                 float* __restrict__ az, float* __restrict__ out3) {
-    int i, s0, s1;
-    bool valid;
-    int slot = 0;
     if (tlist) {
-        // explicit target list: 256 targets per block, any systems (sources = target's system)
-        slot = blockIdx.x * 256 + threadIdx.x;
-        valid = slot < nt_list;
-        i = valid ? tlist[slot] : 0;
-        s0 = 0;
-        s1 = 0;
-    } else {
-        const int sb = blockIdx.x >> 3;
-        const int off = (blockIdx.x & 7) * 256 + threadIdx.x;
-        const int sys = blk_sys[sb];
-        s0 = sys_off[sys];
-        s1 = sys_off[sys + 1];
-        valid = off < blk_count[sb];
-        i = blk_start[sb] + off;
-    }
-    if (tlist) {
-        // systems may differ per lane: each lane loops its own range (tests/judge only)
-        if (!valid) return;
-        int lo = 0, hi = 0;
-        // find the system of i by scanning sys_off (few systems in this path)
-        int sys = 0;
-        while (sys_off[sys + 1] <= i) ++sys;
-        lo = sys_off[sys];
-        hi = sys_off[sys + 1];
+        // explicit target list (ComputeAccelDirectTargets, Solvers.cpp:108-134): one target
+        // per lane, any systems (sources = the target's own system); the count is nt_list,
+        // or *nt_dev when that is given.  Results go to out3[3 slot ..] or, without out3,
+        // to the context's accelerations of the target.
+        const int slot = blockIdx.x * 256 + threadIdx.x;
+        const int cnt = nt_dev ? static_cast<int>(*nt_dev) : nt_list;
+        if (slot >= cnt) return;
+        const int i = tlist[slot];
+        // a system's particles occupy the same index range in original and sorted order
+        const int tsys = pos_sys[i];
+        const int lo_s = sys_off[tsys], hi_s = sys_off[tsys + 1];
         const float4 t = src[i];
         float sx = 0.f, sy = 0.f, sz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
-        for (int j = lo; j < hi; ++j) {
+        for (int j = lo_s; j < hi_s; ++j) {
             if (j == i) continue;
-            const float4 s = src[j];
-            const float dx = s.x - t.x, dy = s.y - t.y, dz = s.z - t.z;
-            const float r2 = dx * dx + dy * dy + dz * dz + eps2;
-            const float inv = __builtin_amdgcn_rsqf(r2);
-            const float f = s.w * (inv * inv * inv);
+            const float4 q = src[j];
+            const float dx = q.x - t.x, dy = q.y - t.y, dz = q.z - t.z;
+            const float f = pair_coef<SPL>(dx * dx + dy * dy + dz * dz, q.w, soft);
             kadd<true>(sx, cx, f * dx);
             kadd<true>(sy, cy, f * dy);
             kadd<true>(sz, cz, f * dz);
         }
-        out3[3 * slot + 0] = sx - cx;
-        out3[3 * slot + 1] = sy - cy;
-        out3[3 * slot + 2] = sz - cz;
+        if (out3) {
+            out3[3 * slot + 0] = sx - cx;
+            out3[3 * slot + 1] = sy - cy;
+            out3[3 * slot + 2] = sz - cz;
+        } else {
+            ax[i] = sx - cx;
+            ay[i] = sy - cy;
+            az[i] = sz - cz;
+        }
         return;
     }
+    const int sb = blockIdx.x >> 3;
+    const int off = (blockIdx.x & 7) * 256 + threadIdx.x;
+    const int sys = blk_sys[sb];
+    const int s0 = sys_off[sys];
+    const int s1 = sys_off[sys + 1];
+    const bool valid = off < blk_count[sb];
+    const int i = blk_start[sb] + off;
     const float4 t = valid ? src[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     float sx = 0.f, sy = 0.f, sz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
     const int sbeg = __builtin_amdgcn_readfirstlane(s0);
@@ -1041,11 +1006,9 @@ __global__ __launch_bounds__(256) void k_direct(const float4* __restrict__ src, 
     for (; j + U <= send; j += U) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const float4 s = src[j + u];
-            const float dx = s.x - t.x, dy = s.y - t.y, dz = s.z - t.z;
-            const float r2 = dx * dx + dy * dy + dz * dz + eps2;
-            const float inv = __builtin_amdgcn_rsqf(r2);
-            float f = s.w * (inv * inv * inv);
+            const float4 q = src[j + u];
+            const float dx = q.x - t.x, dy = q.y - t.y, dz = q.z - t.z;
+            float f = pair_coef<SPL>(dx * dx + dy * dy + dz * dz, q.w, soft);
             f = (j + u == i) ? 0.f : f;
             kadd<KAHAN>(sx, cx, f * dx);
             kadd<KAHAN>(sy, cy, f * dy);
@@ -1053,11 +1016,9 @@ __global__ __launch_bounds__(256) void k_direct(const float4* __restrict__ src, 
         }
     }
     for (; j < send; ++j) {
-        const float4 s = src[j];
-        const float dx = s.x - t.x, dy = s.y - t.y, dz = s.z - t.z;
-        const float r2 = dx * dx + dy * dy + dz * dz + eps2;
-        const float inv = __builtin_amdgcn_rsqf(r2);
-        float f = s.w * (inv * inv * inv);
+        const float4 q = src[j];
+        const float dx = q.x - t.x, dy = q.y - t.y, dz = q.z - t.z;
+        float f = pair_coef<SPL>(dx * dx + dy * dy + dz * dz, q.w, soft);
         f = (j == i) ? 0.f : f;
         kadd<KAHAN>(sx, cx, f * dx);
         kadd<KAHAN>(sy, cy, f * dy);
@@ -1082,14 +1043,14 @@ __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ x, const
     if (i < n) out[i] = make_float4(x[i], y[i], z[i], G * m[i]);
 }
 
-void launch_direct(nbg_ctx* c, const nbg_params& p) {
-    hipLaunchKernelGGL(k_pack, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->x, c->y, c->z, c->m, c->n,
-                       static_cast<float>(p.G), c->spos);
+template <bool SPL>
+void dispatch_direct(nbg_ctx* c, const nbg_params& p) {
     const dim3 grid(c->n_blocks * 8), block(256);
     const int u = c->tune.direct_tile;
+    const Soft soft = make_soft(p);
 #define NBG_DIRECT(K, U)                                                                                          \
-    hipLaunchKernelGGL((k_direct<K, U>), grid, block, 0, c->stream, c->spos, c->blk_sys, c->blk_start, c->blk_count, \
-                       c->sys_off, nullptr, 0, static_cast<float>(p.eps2), c->ax, c->ay, c->az, nullptr)
+    hipLaunchKernelGGL((k_direct<K, U, SPL>), grid, block, 0, c->stream, c->spos, c->blk_sys, c->blk_start,        \
+                       c->blk_count, c->sys_off, nullptr, nullptr, 0, nullptr, soft, c->ax, c->ay, c->az, nullptr)
     if (p.flags & NBG_FLAG_KAHAN) {
         if (u == 2) NBG_DIRECT(true, 2);
         else if (u == 8) NBG_DIRECT(true, 8);
@@ -1102,12 +1063,33 @@ void launch_direct(nbg_ctx* c, const nbg_params& p) {
 #undef NBG_DIRECT
 }
 
-void launch_direct_targets(nbg_ctx* c, const nbg_params& p, int64_t nt, const int32_t* targets, float* out3) {
+void launch_pack(nbg_ctx* c, double G) {
     hipLaunchKernelGGL(k_pack, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->x, c->y, c->z, c->m, c->n,
-                       static_cast<float>(p.G), c->spos);
-    hipLaunchKernelGGL((k_direct<true, 4>), dim3(grid_for(nt, 256)), dim3(256), 0, c->stream, c->spos, c->blk_sys,
-                       c->blk_start, c->blk_count, c->sys_off, targets, static_cast<int>(nt),
-                       static_cast<float>(p.eps2), c->ax, c->ay, c->az, out3);
+                       static_cast<float>(G), c->spos);
+}
+
+void launch_direct(nbg_ctx* c, const nbg_params& p) {
+    launch_pack(c, p.G);
+    if (p.softening == NBG_SOFT_SPLINE)
+        dispatch_direct<true>(c, p);
+    else
+        dispatch_direct<false>(c, p);
+}
+
+void launch_direct_targets(nbg_ctx* c, const nbg_params& p, int64_t nt, const int32_t* targets,
+                           const uint32_t* nt_dev, float* out3) {
+    // sources in original order (the tree's sorted copy is not needed)
+    launch_pack(c, p.G);
+    const Soft soft = make_soft(p);
+    const dim3 grid(grid_for(nt, 256)), block(256);
+    if (p.softening == NBG_SOFT_SPLINE)
+        hipLaunchKernelGGL((k_direct<true, 4, true>), grid, block, 0, c->stream, c->spos, c->blk_sys, c->blk_start,
+                           c->blk_count, c->sys_off, c->pos_sys, targets, static_cast<int>(nt), nt_dev, soft, c->ax, c->ay, c->az,
+                           out3);
+    else
+        hipLaunchKernelGGL((k_direct<true, 4, false>), grid, block, 0, c->stream, c->spos, c->blk_sys, c->blk_start,
+                           c->blk_count, c->sys_off, c->pos_sys, targets, static_cast<int>(nt), nt_dev, soft, c->ax, c->ay, c->az,
+                           out3);
 }
 
 }  // namespace nbg

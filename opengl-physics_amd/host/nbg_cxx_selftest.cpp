@@ -18,11 +18,17 @@ struct SoA {
     std::vector<double> x, y, z, vx, vy, vz, m, ax, ay, az;
     size_t Count() const { return x.size(); }
 };
+enum class SofteningKind { Plummer, Spline };  // Softening.hpp:30-33
+enum class MacKind { Geometric, Relative };       // Mac.hpp:13-16
 struct Soft {
+    SofteningKind kind = SofteningKind::Plummer;
     double eps = 0.05;
+    double eps2 = 0.05 * 0.05;
 };
 struct Mac {
+    MacKind kind = MacKind::Geometric;
     double theta = 0.5;
+    double alpha = 0.001;
 };
 struct StepParams {
     double G = 1.0;

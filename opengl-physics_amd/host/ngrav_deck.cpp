@@ -6,7 +6,8 @@
 //     N_Body_Gravity_Plan.md:113) and `solver.multipole` ("quadrupole" default -- System<3>
 //     always adds quadrupoles, System.hpp:59 -- or "monopole");
 //   * scenario defaults of BuildScenario (projects/nbody_core/src/Scenarios.cpp:22-330) for the
-//     scenarios whose samplers exist here: plummer, hernquist, king, kepler, lagrange, ic_file;
+//     scenarios whose samplers exist here: plummer, hernquist, king, cluster, cold_collapse,
+//     kepler, lagrange, ic_file;
 //   * the frame loop of fw::RunHeadless (framework/src/HeadlessRunner.cpp:11-41): snapshot,
 //     then `substeps_per_frame` leapfrog steps, `frames` times;
 //   * the output layout of fw::OutputWriter (framework/src/OutputWriter.cpp:38-169): deck.toml
@@ -14,8 +15,8 @@
 //     frames/pos_NNNN.npy and vel_NNNN.npy (float32, N x 3), manifest.json.
 // Energy, angular momentum, momentum and centre of mass come from the FP64 GPU diagnostics
 // (System::TotalEnergy semantics, System.hpp:119-193) instead of an O(N^2) CPU loop.
-// Deck features the GPU backend does not implement (spline softening, relative MAC, adaptive
-// or block time steps, the FMM solvers, the other scenarios) raise std::runtime_error.
+// Softening kinds, the relative MAC, adaptive and block time steps run on the GPU backend;
+// the FMM solvers and the scenarios without a sampler here raise std::runtime_error.
 #include <algorithm>
 #include <array>
 #include <cctype>
@@ -227,6 +228,17 @@ Scenario BuildScenario(const std::string& type, int n, uint32_t seed, double w0,
             r.eps = std::max(0.01, 1.2 * spacing);
         }
         r.dt = twoPi / 800.0;
+    } else if (type == "cluster" || type == "cold_collapse") {  // Scenarios.cpp:104-142, 201-214
+        n = std::max(n, 1);
+        r.s.Resize(n);
+        auto& s = r.s;
+        const bool cold = type == "cold_collapse";
+        Check(ngh_ic_cluster(n, seed, cold ? 1 : 0, s.x.data(), s.y.data(), s.z.data(), s.vx.data(), s.vy.data(),
+                             s.vz.data(), s.m.data()),
+              "Cluster");
+        const double spacing = 1.0 / std::cbrt(static_cast<double>(n));
+        r.eps = cold ? std::max(0.01, 1.0 * spacing) : 2.0 * spacing;
+        r.dt = cold ? 1.0 / 2000.0 : 1.0 / 1000.0;
     } else if (type == "two_body_kepler" || type == "kepler") {  // Scenarios.cpp:31-76
         const double G = 1.0, m = 1.0, sep = 1.0;
         const double vCirc = std::sqrt(G * 2.0 * m / sep);
@@ -295,13 +307,6 @@ int ngh_run_deck(const char* deck_path, const char* out_dir, int frames, int qui
         const std::string title = deck.GetString("title", "N-Body Gravity");
         const std::string backend = deck.GetString("solver.backend", "gpu");
         if (backend != "gpu") throw std::runtime_error("solver.backend must be \"gpu\" for this runner");
-        if (deck.GetBool("time.adaptive", false)) throw std::runtime_error("time.adaptive is not supported by the GPU backend");
-        if (deck.GetString("time.scheme", "global") != "global")
-            throw std::runtime_error("time.scheme = \"block\" is not supported by the GPU backend");
-        if (deck.GetString("solver.mac", "geometric") != "geometric")
-            throw std::runtime_error("solver.mac = \"relative\" is not supported by the GPU backend");
-        if (deck.GetString("softening.kind", "plummer") != "plummer")
-            throw std::runtime_error("softening.kind = \"spline\" is not supported by the GPU backend");
         const std::string kind = deck.GetString("solver.kind", "barnes_hut");
         int solver;
         if (kind == "direct") solver = NBG_SOLVER_DIRECT;
@@ -328,6 +333,14 @@ int ngh_run_deck(const char* deck_path, const char* out_dir, int frames, int qui
         p.theta = deck.GetDouble("solver.theta", 0.5);
         p.dt = deck.Has("time.dt") ? deck.GetDouble("time.dt", 1e-3) : sc.dt;
         p.flags = NBG_FLAG_KAHAN;
+        // softening kind, opening criterion and time-step scheme (DeckSim.cpp:30-41, 61-62, 81)
+        p.softening = (deck.GetString("softening.kind", "plummer") == "spline") ? NBG_SOFT_SPLINE : NBG_SOFT_PLUMMER;
+        p.mac = (deck.GetString("solver.mac", "geometric") == "relative") ? NBG_MAC_RELATIVE : NBG_MAC_GEOMETRIC;
+        p.alpha = deck.GetDouble("solver.alpha", 0.001);
+        p.adaptive = deck.GetBool("time.adaptive", false) ? 1 : 0;
+        p.eta = deck.GetDouble("time.eta", 0.03);
+        p.block = (deck.GetString("time.scheme", "global") == "block") ? 1 : 0;
+        p.block_max_rung = deck.GetInt("time.block_max_rung", 8);
 
         const int n = static_cast<int>(sc.s.x.size());
         nbg_ctx* ctx = nbg_create(0, std::max(n, 1));
@@ -367,6 +380,8 @@ int ngh_run_deck(const char* deck_path, const char* out_dir, int frames, int qui
         std::vector<float> x(n), y(n), z(n), vx(n), vy(n), vz(n), pos(3 * static_cast<size_t>(n)),
             vel(3 * static_cast<size_t>(n));
         char tag[16];
+        double dt_last = p.dt;  // DeckSim::LoadScenario sets m_lastDt = dt (DeckSim.cpp:83)
+        int rung_max = 0;
         for (int fr = 0; fr < frames; ++fr) {
             const long step = static_cast<long>(fr) * substeps;
             const double t = static_cast<double>(step) * p.dt;
@@ -396,8 +411,8 @@ int ngh_run_deck(const char* deck_path, const char* out_dir, int frames, int qui
                                     diag[2],
                                     diag[3],
                                     diag[4],
-                                    p.dt,
-                                    0.0};
+                                    dt_last,
+                                    static_cast<double>(rung_max)};
             csv << std::setprecision(10) << t << ',' << step;
             for (double v : row) csv << ',' << v;
             csv << '\n';
@@ -406,7 +421,12 @@ int ngh_run_deck(const char* deck_path, const char* out_dir, int frames, int qui
                 std::printf("\rframe %d/%d  t=%.4f   ", fr + 1, frames, t);
                 std::fflush(stdout);
             }
-            if (fr + 1 < frames) chk(nbg_step(ctx, &p, substeps, nullptr), "nbg_step");
+            if (fr + 1 < frames) {
+                nbg_stats st{};
+                chk(nbg_step(ctx, &p, substeps, &st), "nbg_step");
+                dt_last = st.dt_last;
+                rung_max = st.rung_max;
+            }
         }
         csv.close();
         std::ofstream j(dir / "manifest.json", std::ios::binary);

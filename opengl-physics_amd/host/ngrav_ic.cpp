@@ -4,6 +4,7 @@
 //   Plummer    projects/nbody_core/include/ngrav/ic/Plummer.hpp:26-112
 //   Hernquist  projects/nbody_core/include/ngrav/ic/Hernquist.hpp:51-181
 //   King(W0)   projects/nbody_core/include/ngrav/ic/King.hpp:58-211
+//   Cluster / ColdUniform  projects/nbody_core/src/Scenarios.cpp:104-142, 201-214
 // Where the reference writes `Vec<3> v(gg(rng), gg(rng), gg(rng))` (unspecified argument
 // order) the GCC order -- right to left, z drawn first -- is used.  Built without FP
 // contraction so the FP64 values are the same on every host.
@@ -258,6 +259,34 @@ int ngh_ic_king(int n, uint32_t seed, double totalMass, double coreRadius, doubl
 // Cluster sizes and kinds of a batched ensemble: N_k = round(exp(U[ln nmin, ln nmax])),
 // U from std::uniform_real_distribution<double> over std::mt19937(seed), drawn k = 0, 1, ...;
 // kind_k = k % 2 (0 Plummer, 1 King).  Returns the total particle count, or -1.
+int ngh_ic_cluster(int n, uint32_t seed, int cold, double* x, double* y, double* z, double* vx, double* vy,
+                   double* vz, double* m) {
+    // Cluster<3> (Scenarios.cpp:104-142): uniform sphere R = 1, M = 1, G = 1, Gaussian
+    // velocities of dispersion 0.3 sqrt(G M / R) from one normal_distribution for the whole
+    // run; ColdUniform (Scenarios.cpp:201-214) draws the same and zeroes the velocities.
+    if (n < 1 || !x || !y || !z || !vx || !vy || !vz || !m) return -1;
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<double> u(0.0, 1.0);
+    std::normal_distribution<double> gauss(0.0, 1.0);
+    const double G = 1.0, R = 1.0, M = 1.0;
+    const double vScale = std::sqrt(G * M / R);
+    for (int i = 0; i < n; ++i) {
+        const double radius = R * std::cbrt(u(rng));
+        const Dir d = RandomDir(rng, u);
+        x[i] = d.x * radius;
+        y[i] = d.y * radius;
+        z[i] = d.z * radius;
+        const double gx = 0.3 * vScale * gauss(rng);
+        const double gy = 0.3 * vScale * gauss(rng);
+        const double gz = 0.3 * vScale * gauss(rng);
+        vx[i] = cold ? 0.0 : gx;
+        vy[i] = cold ? 0.0 : gy;
+        vz[i] = cold ? 0.0 : gz;
+        m[i] = M / n;
+    }
+    return 0;
+}
+
 int64_t ngh_ensemble_sizes(int32_t nclusters, uint32_t seed, int64_t nmin, int64_t nmax, int64_t* sizes,
                            int32_t* kinds) {
     if (nclusters < 1 || nmin < 1 || nmax < nmin || !sizes) return -1;

@@ -22,6 +22,11 @@ FLAG_COUNT = 2
 FLAG_LEARNED = 4
 FLAG_NO_GRAPH = 8
 FLAG_STAGE_TIMES = 16
+FLAG_PER_PARTICLE = 32
+SOFT_PLUMMER = 0
+SOFT_SPLINE = 1
+MAC_GEOMETRIC = 0
+MAC_RELATIVE = 1
 
 ERRORS = {0: "NBG_OK", -1: "NBG_E_INVALID", -2: "NBG_E_OOM", -3: "NBG_E_CAPACITY", -4: "NBG_E_HIP",
           -5: "NBG_E_STATE"}
@@ -30,7 +35,8 @@ EXPORTED = ["nbg_create", "nbg_destroy", "nbg_last_error", "nbg_upload", "nbg_up
             "nbg_accel", "nbg_step", "nbg_download", "nbg_tree_export", "nbg_interaction_counts", "nbg_diagnostics",
             "nbg_accel_direct_targets", "nbg_learn_set_weights", "nbg_learn_get_weights", "nbg_learn_collect",
             "nbg_learn_clear", "nbg_learn_pool", "nbg_learn_standardize", "nbg_learn_train", "nbg_learn_grad",
-            "nbg_tuning_get", "nbg_tuning_set", "nbg_autotune", "nbg_diagnostics_system"]
+            "nbg_tuning_get", "nbg_tuning_set", "nbg_autotune", "nbg_diagnostics_system", "nbg_accel_bh_targets",
+            "nbg_rungs"]
 
 LEARN_FEATURES = 13
 LEARN_HIDDEN = 64
@@ -41,7 +47,10 @@ LEARN_NPARAM = 32 + LEARN_NTRAIN
 class Params(ctypes.Structure):
     _fields_ = [("solver", ctypes.c_int), ("multipole", ctypes.c_int), ("G", ctypes.c_double),
                 ("eps2", ctypes.c_double), ("theta", ctypes.c_double), ("dt", ctypes.c_double),
-                ("flags", ctypes.c_uint32), ("learn_band", ctypes.c_double)]
+                ("flags", ctypes.c_uint32), ("learn_band", ctypes.c_double), ("softening", ctypes.c_int32),
+                ("mac", ctypes.c_int32), ("alpha", ctypes.c_double), ("adaptive", ctypes.c_int32),
+                ("block", ctypes.c_int32), ("eta", ctypes.c_double), ("block_max_rung", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class Tuning(ctypes.Structure):
@@ -57,7 +66,9 @@ class Stats(ctypes.Structure):
                 ("deep_keys", ctypes.c_int32), ("n_p2p", ctypes.c_int64), ("n_m2p", ctypes.c_int64),
                 ("ms_keys", ctypes.c_double), ("ms_sort", ctypes.c_double), ("ms_build", ctypes.c_double),
                 ("ms_upsweep", ctypes.c_double), ("ms_walk", ctypes.c_double), ("ms_integrate", ctypes.c_double),
-                ("ms_total", ctypes.c_double), ("n_learned", ctypes.c_int64), ("ms_learned", ctypes.c_double)]
+                ("ms_total", ctypes.c_double), ("n_learned", ctypes.c_int64), ("ms_learned", ctypes.c_double),
+                ("t_advanced", ctypes.c_double), ("dt_last", ctypes.c_double), ("particle_evals", ctypes.c_int64),
+                ("rung_max", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -107,6 +118,8 @@ def load():
     L.nbg_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_int32, ctypes.POINTER(Tuning), _d,
                                ctypes.c_int32, _i32]
     L.nbg_diagnostics_system.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_int32, ctypes.c_int32, _d]
+    L.nbg_accel_bh_targets.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_int64, _i64] + [_f] * 3
+    L.nbg_rungs.argtypes = [ctypes.c_void_p, _i32]
     L.nbg_destroy.restype = None
     for name in EXPORTED:
         if name not in ("nbg_create", "nbg_destroy", "nbg_last_error"):
@@ -128,14 +141,23 @@ class NbgError(RuntimeError):
 
 
 def make_params(solver="barnes_hut", multipole="quadrupole", G=1.0, eps=None, eps2=None, theta=0.5, dt=1e-3,
-                kahan=True, count=False, no_graph=False, learned=False, stage_times=False, learn_band=0.0):
+                kahan=True, count=False, no_graph=False, learned=False, stage_times=False, learn_band=0.0,
+                softening="plummer", mac="geometric", alpha=0.001, per_particle=False, adaptive=False,
+                block=False, eta=0.03, block_max_rung=8):
+    """nbg_params with the reference's StepParams defaults (Solvers.hpp:22-40): eps is the
+    Plummer softening length or the spline length h (eps2 = eps^2 either way)."""
     if eps2 is None:
         eps2 = (eps if eps is not None else 0.05) ** 2
+    if softening not in ("plummer", "spline") or mac not in ("geometric", "relative"):
+        raise ValueError("softening must be plummer|spline and mac geometric|relative")
     flags = (FLAG_KAHAN if kahan else 0) | (FLAG_COUNT if count else 0) | (FLAG_NO_GRAPH if no_graph else 0) | \
-        (FLAG_LEARNED if learned else 0) | (FLAG_STAGE_TIMES if stage_times else 0)
+        (FLAG_LEARNED if learned else 0) | (FLAG_STAGE_TIMES if stage_times else 0) | \
+        (FLAG_PER_PARTICLE if per_particle else 0)
     return Params(SOLVER_DIRECT if solver == "direct" else SOLVER_BARNES_HUT,
                   QUADRUPOLE if multipole == "quadrupole" else MONOPOLE, float(G), float(eps2), float(theta),
-                  float(dt), flags, float(learn_band))
+                  float(dt), flags, float(learn_band), SOFT_SPLINE if softening == "spline" else SOFT_PLUMMER,
+                  MAC_RELATIVE if mac == "relative" else MAC_GEOMETRIC, float(alpha), 1 if adaptive else 0,
+                  1 if block else 0, float(eta), int(block_max_rung), 0)
 
 
 class Context:
@@ -249,6 +271,19 @@ class Context:
         return np.stack([ax, ay, az], axis=1)
 
     # ------------------------------------------------------------------ tuning
+    def bh_targets(self, params, targets):
+        """Per-particle geometric Barnes-Hut walk onto targets (ComputeAccelBarnesHutTargets)."""
+        t = np.ascontiguousarray(targets, dtype=np.int64)
+        out = [np.empty(len(t), np.float32) for _ in range(3)]
+        self._check(self._L.nbg_accel_bh_targets(self._h, ctypes.byref(params), len(t), _p(t, _i64),
+                                                 *[_p(o, _f) for o in out]), "nbg_accel_bh_targets")
+        return np.stack(out, axis=1)
+
+    def rungs(self):
+        r = np.empty(self.n, np.int32)
+        self._check(self._L.nbg_rungs(self._h, _p(r, _i32)), "nbg_rungs")
+        return r
+
     def tuning(self):
         t = Tuning()
         self._check(self._L.nbg_tuning_get(self._h, ctypes.byref(t)), "nbg_tuning_get")

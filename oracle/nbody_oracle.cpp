@@ -19,7 +19,12 @@
 //                           pair kernel Solvers.cpp:19-38, quadrupole Solvers.cpp:147-179
 //   orc_accel_bh_pp         projects/nbody_core/src/Solvers.cpp:181-227, 359-382
 //   orc_accel_direct        projects/nbody_core/src/Solvers.cpp:77-106
-//   orc_leapfrog            projects/nbody_core/src/Integrator.cpp:9-39
+//   orc_accel_bh_targets    projects/nbody_core/src/Solvers.cpp:384-405
+//   softening kinds         Solvers.cpp:19-73, Softening.hpp:39-56, 120-139
+//   orc_steps / orc_leapfrog projects/nbody_core/src/Integrator.cpp:9-39 with System::Step
+//                           (System.hpp:89-108) and CaptureAOld (302-310)
+//   orc_adaptive_dt         projects/nbody_core/src/Integrator.cpp:41-59
+//   orc_rungs, orc_block_steps  System::BlockStep, System.hpp:239-301
 //   orc_total_energy        projects/nbody_core/include/ngrav/System.hpp:119-164
 //   orc_com / orc_angmom    projects/nbody_core/include/ngrav/System.hpp:166-193
 //   orc_morton_keys         derived contract (not a reference function): the octant
@@ -233,9 +238,36 @@ void Quadrupoles(Tree& t, const Pts& p) {
     }
 }
 
+// Softening::kind / eps / eps2 (Softening.hpp:120-139): spline stores eps = h, eps2 = h^2.
+struct Soft {
+    int spline;
+    double eps;
+    double eps2;
+};
+Soft MakeSoft(double eps2, int spline) { return Soft{spline, std::sqrt(eps2), eps2}; }
+
+// SplineChi3D (Softening.hpp:39-45).
+inline double SplineChi3D(double q) {
+    if (q >= 2.0) return 1.0;
+    if (q < 1.0) return q * q * q * (4.0 / 3.0 - 1.2 * q * q + 0.5 * q * q * q);
+    const double t = 2.0 - q;
+    const double t2 = t * t, t4 = t2 * t2;
+    return 1.0 - 4.0 * (t4 / 4.0 - t4 * t / 5.0 + t4 * t2 / 24.0);
+}
+
+// SplinePotential3D (Softening.hpp:49-56).
+inline double SplinePotential3D(double q) {
+    if (q >= 2.0) return -1.0 / q;
+    if (q < 1.0) return -(1.4 - (2.0 / 3.0) * q * q + 0.3 * q * q * q * q - 0.1 * q * q * q * q * q);
+    const double t = 2.0 - q;
+    const double t2 = t * t, t4 = t2 * t2;
+    const double F = t4 / 4.0 - t4 * t / 5.0 + t4 * t2 / 24.0;
+    return -((1.0 - 4.0 * F) / q + t4 / 2.0 - t4 * t / 5.0);
+}
+
 // Plummer pair kernel (Solvers.cpp:19-38); gmj = G*m_j.
-inline void AccumPair(double dx, double dy, double dz, double gmj, double eps2, double& ax, double& ay,
-                      double& az) {
+inline void AccumPairPlummer(double dx, double dy, double dz, double gmj, double eps2, double& ax, double& ay,
+                             double& az) {
     const double dist2 = dx * dx + dy * dy + dz * dz + eps2;
     const double inv = 1.0 / std::sqrt(dist2);
     const double inv3 = inv * inv * inv;
@@ -243,6 +275,29 @@ inline void AccumPair(double dx, double dy, double dz, double gmj, double eps2, 
     ax += s * dx;
     ay += s * dy;
     az += s * dz;
+}
+
+// Spline pair kernel (Solvers.cpp:43-63), h = Softening::eps.
+inline void AccumPairSpline(double dx, double dy, double dz, double gmj, double h, double& ax, double& ay,
+                            double& az) {
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double r = std::sqrt(r2);
+    if (r <= 1e-300) return;
+    const double q = r / h;
+    const double chi = SplineChi3D(q);
+    const double s = gmj * chi / (r2 * r);
+    ax += s * dx;
+    ay += s * dy;
+    az += s * dz;
+}
+
+// AccumPair (Solvers.cpp:65-73).
+inline void AccumPair(double dx, double dy, double dz, double gmj, const Soft& soft, double& ax, double& ay,
+                      double& az) {
+    if (!soft.spline)
+        AccumPairPlummer(dx, dy, dz, gmj, soft.eps2, ax, ay, az);
+    else
+        AccumPairSpline(dx, dy, dz, gmj, soft.eps, ax, ay, az);
 }
 
 // Traceless quadrupole term, unsoftened (Solvers.cpp:147-179).
@@ -289,7 +344,7 @@ void Collect(const Tree& t, int node, int group, double gx, double gy, double gz
 
 // Per-particle walk, geometric or relative MAC (Solvers.cpp:181-227).
 void WalkPP(const Tree& t, const Pts& p, int node, int i, double xi, double yi, double zi, double G, double theta2,
-            double eps2, int relative, double alpha, double aOld, double& ax, double& ay, double& az) {
+            const Soft& eps2, int relative, double alpha, double aOld, double& ax, double& ay, double& az) {
     if (t.mass[node] <= 0.0) return;
     if (t.childCount[node] == 0) {
         for (int k = t.first[node]; k < t.first[node] + t.count[node]; ++k) {
@@ -468,10 +523,11 @@ void orc_tree_export(const void* h, double* geom, double* mom, double* quad, int
 // far-node count and the near P2P source count (near leaves + own leaf, self excluded)
 // of that leaf's group; zero for internal nodes.
 void orc_accel_bh(const void* h, int n, const double* x, const double* y, const double* z, const double* m,
-                  double G, double eps2, double theta, double* ax, double* ay, double* az, int64_t* nFar,
-                  int64_t* nNear) {
+                  double G, double eps2_, double theta, int spline, double* ax, double* ay, double* az,
+                  int64_t* nFar, int64_t* nNear) {
     const Tree& t = *static_cast<const Tree*>(h);
     const Pts p{x, y, z, m, n};
+    const Soft eps2 = MakeSoft(eps2_, spline);
     const double theta2 = theta * theta;
     std::vector<int> leaves;
     for (int nd = 0; nd < static_cast<int>(t.mass.size()); ++nd)
@@ -558,10 +614,11 @@ void orc_group_lists(const void* h, double theta, int64_t* far_ptr, int32_t* far
 // ComputeAccelBarnesHutPerParticle (Solvers.cpp:359-382): relative MAC when
 // relative != 0 and aOld (nullable) holds each particle's previous |a|.
 void orc_accel_bh_pp(const void* h, int n, const double* x, const double* y, const double* z, const double* m,
-                     double G, double eps2, double theta, int relative, double alpha, const double* aOld, double* ax,
-                     double* ay, double* az) {
+                     double G, double eps2_, double theta, int spline, int relative, double alpha, const double* aOld,
+                     double* ax, double* ay, double* az) {
     const Tree& t = *static_cast<const Tree*>(h);
     const Pts p{x, y, z, m, n};
+    const Soft eps2 = MakeSoft(eps2_, spline);
     const double theta2 = theta * theta;
 #pragma omp parallel for schedule(dynamic, 64)
     for (int i = 0; i < n; ++i) {
@@ -573,9 +630,30 @@ void orc_accel_bh_pp(const void* h, int n, const double* x, const double* y, con
     }
 }
 
-// ComputeAccelDirect (Solvers.cpp:77-106), Plummer softening.
+// ComputeAccelBarnesHutTargets (Solvers.cpp:384-405): the per-particle geometric walk onto
+// targets[0..nt) (outputs by list slot).
+void orc_accel_bh_targets(const void* h, int n, const double* x, const double* y, const double* z, const double* m,
+                          double G, double eps2_, double theta, int spline, int nt, const int* targets, double* ax,
+                          double* ay, double* az) {
+    const Tree& t = *static_cast<const Tree*>(h);
+    const Pts p{x, y, z, m, n};
+    const Soft eps2 = MakeSoft(eps2_, spline);
+    const double theta2 = theta * theta;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int k = 0; k < nt; ++k) {
+        const int i = targets[k];
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        WalkPP(t, p, 0, i, x[i], y[i], z[i], G, theta2, eps2, 0, 0.0, 0.0, sx, sy, sz);
+        ax[k] = sx;
+        ay[k] = sy;
+        az[k] = sz;
+    }
+}
+
+// ComputeAccelDirect (Solvers.cpp:77-106).
 void orc_accel_direct(int n, const double* x, const double* y, const double* z, const double* m, double G,
-                      double eps2, double* ax, double* ay, double* az) {
+                      double eps2_, int spline, double* ax, double* ay, double* az) {
+    const Soft eps2 = MakeSoft(eps2_, spline);
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
         const double xi = x[i], yi = y[i], zi = z[i];
@@ -592,7 +670,9 @@ void orc_accel_direct(int n, const double* x, const double* y, const double* z, 
 
 // Direct sum onto a subset of targets (ComputeAccelDirectTargets, Solvers.cpp:108-134).
 void orc_accel_direct_targets(int n, const double* x, const double* y, const double* z, const double* m, double G,
-                              double eps2, int nt, const int* targets, double* ax, double* ay, double* az) {
+                              double eps2_, int spline, int nt, const int* targets, double* ax, double* ay,
+                              double* az) {
+    const Soft eps2 = MakeSoft(eps2_, spline);
 #pragma omp parallel for schedule(static)
     for (int t = 0; t < nt; ++t) {
         const int i = targets[t];
@@ -609,14 +689,61 @@ void orc_accel_direct_targets(int n, const double* x, const double* y, const dou
 }
 
 // ---------------------------------------------------------------- integrator
-// LeapfrogStep (Integrator.cpp:9-39) with the System's BarnesHut dispatch
-// (System.hpp:43-78: tree rebuilt per evaluation, quadrupoles when withQuad) or
-// Direct (solver 0).  a[] must hold a(t) on entry and holds a(t+dt) on exit.
-void orc_leapfrog(int n, double* x, double* y, double* z, double* vx, double* vy, double* vz, const double* m,
-                  double* ax, double* ay, double* az, double G, double eps2, double theta, int solver, int withQuad,
-                  double dt, int nsteps) {
-    const double hdt = 0.5 * dt;
+namespace {
+
+// System::ComputeAccel for the native solvers (System.hpp:43-78): Direct, or Barnes-Hut on
+// a tree rebuilt for this evaluation (quadrupoles when withQuad), grouped walk for the
+// geometric MAC, per-particle walk with aOld for the relative MAC (Solvers.cpp:407-414).
+void ComputeAccel(int n, const double* x, const double* y, const double* z, const double* m, double* ax,
+                  double* ay, double* az, double G, double eps2, int spline, double theta, int solver, int withQuad,
+                  int relative, double alpha, const double* aOld) {
+    if (solver == 0) {
+        orc_accel_direct(n, x, y, z, m, G, eps2, spline, ax, ay, az);
+        return;
+    }
+    void* t = orc_tree_build(n, x, y, z, m, 8, 40, withQuad);
+    if (relative)
+        orc_accel_bh_pp(t, n, x, y, z, m, G, eps2, theta, spline, 1, alpha, aOld, ax, ay, az);
+    else
+        orc_accel_bh(t, n, x, y, z, m, G, eps2, theta, spline, ax, ay, az, nullptr, nullptr);
+    orc_tree_free(t);
+}
+
+// System::CaptureAOld (System.hpp:302-310).
+void CaptureAOld(int n, const double* ax, const double* ay, const double* az, double* aOld) {
+    for (int i = 0; i < n; ++i) aOld[i] = std::sqrt(ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i]);
+}
+
+}  // namespace
+
+// ChooseAdaptiveDt (Integrator.cpp:41-59).
+double orc_adaptive_dt(int n, const double* ax, const double* ay, const double* az, double eta, double eps,
+                       double dtFallback) {
+    if (n == 0 || eps <= 0.0) return dtFallback;
+    double minTau = std::numeric_limits<double>::max();
+    for (int i = 0; i < n; ++i) {
+        const double a = std::sqrt(ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i]);
+        if (a > 0.0) minTau = std::min(minTau, std::sqrt(eps / a));
+    }
+    if (minTau == std::numeric_limits<double>::max()) return dtFallback;
+    const double dt = eta * minTau;
+    return std::clamp(dt, dtFallback / 4096.0, dtFallback);
+}
+
+// LeapfrogStep (Integrator.cpp:9-39) as System::Step drives it (System.hpp:89-108): global
+// steps, optional adaptive dt (from a(t)), aOld captured after every step for the relative
+// MAC.  a[] must hold a(t) on entry and aOld (nullable unless relative) the |a| captured
+// with it; both are updated.  Returns the time advanced.
+double orc_steps(int n, double* x, double* y, double* z, double* vx, double* vy, double* vz, const double* m,
+                 double* ax, double* ay, double* az, double* aOld, double G, double eps2, int spline, double theta,
+                 int solver, int withQuad, int relative, double alpha, int adaptive, double eta, double dt0,
+                 int nsteps) {
+    double t = 0.0;
+    std::vector<double> sx(n), sy(n), sz(n);
+    const double eps = std::sqrt(eps2);
     for (int s = 0; s < nsteps; ++s) {
+        const double dt = adaptive ? orc_adaptive_dt(n, ax, ay, az, eta, eps, dt0) : dt0;
+        const double hdt = 0.5 * dt;
         for (int i = 0; i < n; ++i) {
             vx[i] += hdt * ax[i];
             vy[i] += hdt * ay[i];
@@ -627,26 +754,117 @@ void orc_leapfrog(int n, double* x, double* y, double* z, double* vx, double* vy
             y[i] += dt * vy[i];
             z[i] += dt * vz[i];
         }
-        if (solver == 0) {
-            orc_accel_direct(n, x, y, z, m, G, eps2, ax, ay, az);
-        } else {
-            void* t = orc_tree_build(n, x, y, z, m, 8, 40, withQuad);
-            orc_accel_bh(t, n, x, y, z, m, G, eps2, theta, ax, ay, az, nullptr, nullptr);
-            orc_tree_free(t);
-        }
+        ComputeAccel(n, x, y, z, m, sx.data(), sy.data(), sz.data(), G, eps2, spline, theta, solver, withQuad,
+                     relative, alpha, aOld);
+        std::copy(sx.begin(), sx.end(), ax);
+        std::copy(sy.begin(), sy.end(), ay);
+        std::copy(sz.begin(), sz.end(), az);
         for (int i = 0; i < n; ++i) {
             vx[i] += hdt * ax[i];
             vy[i] += hdt * ay[i];
             vz[i] += hdt * az[i];
         }
+        if (aOld) CaptureAOld(n, ax, ay, az, aOld);
+        t += dt;
     }
+    return t;
+}
+
+// The Plummer, geometric, fixed-dt special case of orc_steps.
+void orc_leapfrog(int n, double* x, double* y, double* z, double* vx, double* vy, double* vz, const double* m,
+                  double* ax, double* ay, double* az, double G, double eps2, double theta, int solver, int withQuad,
+                  double dt, int nsteps) {
+    orc_steps(n, x, y, z, vx, vy, vz, m, ax, ay, az, nullptr, G, eps2, 0, theta, solver, withQuad, 0, 0.0, 0, 0.0, dt,
+              nsteps);
+}
+
+// Rungs of System::BlockStep (System.hpp:249-265) from a; returns the deepest rung.
+int orc_rungs(int n, const double* ax, const double* ay, const double* az, double eta, double eps, double base,
+              int maxRung, int* rung) {
+    maxRung = std::max(0, maxRung);
+    int rMax = 0;
+    for (int i = 0; i < n; ++i) {
+        const double amag = std::sqrt(ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i]);
+        int k = 0;
+        if (amag > 0.0 && eps > 0.0) {
+            const double tau = eta * std::sqrt(eps / amag);
+            if (tau < base) k = std::min(maxRung, static_cast<int>(std::ceil(std::log2(base / tau))));
+        }
+        rung[i] = k;
+        rMax = std::max(rMax, k);
+    }
+    return rMax;
+}
+
+// nsteps calls of System::BlockStep (System.hpp:239-294) with the Direct or Barnes-Hut
+// solver: the closing set of every substep is evaluated on a fresh tree by the per-particle
+// geometric walk (ComputeAccelTargets, System.hpp:220-230) or by the direct sum onto those
+// targets.  rung[] receives the last coarse step's rungs; *evals accumulates the
+// evaluated targets (System::ParticleEvals).  Returns the deepest rung of the last step.
+int orc_block_steps(int n, double* x, double* y, double* z, double* vx, double* vy, double* vz, const double* m,
+                    double* ax, double* ay, double* az, int* rung, double G, double eps2, int spline, double theta,
+                    int solver, int withQuad, double eta, double base, int maxRung, int nsteps, int64_t* evals) {
+    const double eps = std::sqrt(eps2);
+    int rMax = 0;
+    std::vector<int> active;
+    std::vector<double> tx, ty, tz;
+    auto half_kick = [&](int i, double h) {
+        vx[i] += h * ax[i];
+        vy[i] += h * ay[i];
+        vz[i] += h * az[i];
+    };
+    for (int s = 0; s < nsteps; ++s) {
+        rMax = orc_rungs(n, ax, ay, az, eta, eps, base, maxRung, rung);
+        const int nSub = 1 << rMax;
+        const double dtMin = base / nSub;
+        for (int sub = 0; sub < nSub; ++sub) {
+            for (int i = 0; i < n; ++i) {
+                const int stepUnits = 1 << (rMax - rung[i]);
+                if (sub % stepUnits == 0) half_kick(i, 0.5 * (base / (1 << rung[i])));
+            }
+            for (int i = 0; i < n; ++i) {
+                x[i] += dtMin * vx[i];
+                y[i] += dtMin * vy[i];
+                z[i] += dtMin * vz[i];
+            }
+            active.clear();
+            for (int i = 0; i < n; ++i) {
+                const int stepUnits = 1 << (rMax - rung[i]);
+                if ((sub + 1) % stepUnits == 0) active.push_back(i);
+            }
+            const int nt = static_cast<int>(active.size());
+            if (nt > 0) {
+                if (evals) *evals += nt;
+                tx.assign(nt, 0.0);
+                ty.assign(nt, 0.0);
+                tz.assign(nt, 0.0);
+                if (solver == 0) {
+                    orc_accel_direct_targets(n, x, y, z, m, G, eps2, spline, nt, active.data(), tx.data(), ty.data(),
+                                             tz.data());
+                } else {
+                    void* t = orc_tree_build(n, x, y, z, m, 8, 40, withQuad);
+                    orc_accel_bh_targets(t, n, x, y, z, m, G, eps2, theta, spline, nt, active.data(), tx.data(),
+                                         ty.data(), tz.data());
+                    orc_tree_free(t);
+                }
+                for (int k = 0; k < nt; ++k) {
+                    ax[active[k]] = tx[k];
+                    ay[active[k]] = ty[k];
+                    az[active[k]] = tz[k];
+                }
+            }
+            for (int i : active) half_kick(i, 0.5 * (base / (1 << rung[i])));
+        }
+    }
+    return rMax;
 }
 
 // ---------------------------------------------------------------- diagnostics
-// System::TotalEnergy (System.hpp:119-164), Plummer softening.  Per-i partial sums
-// are added in i order so the result does not depend on the thread count.
+// System::TotalEnergy (System.hpp:119-164).  Per-i partial sums are added in i order so
+// the result does not depend on the thread count.
 double orc_total_energy(int n, const double* x, const double* y, const double* z, const double* vx,
-                        const double* vy, const double* vz, const double* m, double G, double eps2) {
+                        const double* vy, const double* vz, const double* m, double G, double eps2, int spline) {
+    const Soft soft = MakeSoft(eps2, spline);
     double kinetic = 0.0;
     for (int i = 0; i < n; ++i) {
         const double v2 = vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i];
@@ -659,8 +877,14 @@ double orc_total_energy(int n, const double* x, const double* y, const double* z
         for (int j = i + 1; j < n; ++j) {
             const double dx = x[j] - x[i], dy = y[j] - y[i], dz = z[j] - z[i];
             const double mij = m[i] * m[j];
-            const double r2 = dx * dx + dy * dy + dz * dz + eps2;
-            acc -= G * mij / std::sqrt(r2);
+            if (soft.spline) {
+                const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+                const double q = (r <= 1e-300) ? 0.0 : r / soft.eps;
+                acc += G * mij * SplinePotential3D(q) / soft.eps;
+            } else {
+                const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+                acc -= G * mij / std::sqrt(r2);
+            }
         }
         local[i] = acc;
     }

@@ -39,18 +39,31 @@ def lib(fast=False):
     L.orc_tree_free.argtypes = [ctypes.c_void_p]
     L.orc_tree_num_nodes.argtypes = [ctypes.c_void_p]
     L.orc_tree_export.argtypes = [ctypes.c_void_p, _d, _d, _d, _i, _i, _i]
-    L.orc_accel_bh.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_d] * 4 + [ctypes.c_double] * 3 + [_d] * 3 + \
-        [_i64] * 2
+    L.orc_accel_bh.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_d] * 4 + [ctypes.c_double] * 3 + \
+        [ctypes.c_int] + [_d] * 3 + [_i64] * 2
     L.orc_accel_bh_pp.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_d] * 4 + [ctypes.c_double] * 3 + \
-        [ctypes.c_int, ctypes.c_double, _d] + [_d] * 3
+        [ctypes.c_int, ctypes.c_int, ctypes.c_double, _d] + [_d] * 3
+    L.orc_accel_bh_targets.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_d] * 4 + [ctypes.c_double] * 3 + \
+        [ctypes.c_int, ctypes.c_int, _i] + [_d] * 3
     L.orc_group_lists.argtypes = [ctypes.c_void_p, ctypes.c_double, _i64, _i, _i64, _i]
-    L.orc_accel_direct.argtypes = [ctypes.c_int] + [_d] * 4 + [ctypes.c_double] * 2 + [_d] * 3
+    L.orc_accel_direct.argtypes = [ctypes.c_int] + [_d] * 4 + [ctypes.c_double] * 2 + [ctypes.c_int] + [_d] * 3
     L.orc_accel_direct_targets.argtypes = [ctypes.c_int] + [_d] * 4 + [ctypes.c_double] * 2 + \
-        [ctypes.c_int, _i] + [_d] * 3
+        [ctypes.c_int, ctypes.c_int, _i] + [_d] * 3
+    L.orc_steps.restype = ctypes.c_double
+    L.orc_steps.argtypes = [ctypes.c_int] + [_d] * 11 + [ctypes.c_double] * 2 + [ctypes.c_int, ctypes.c_double] + \
+        [ctypes.c_int] * 3 + [ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    L.orc_adaptive_dt.restype = ctypes.c_double
+    L.orc_adaptive_dt.argtypes = [ctypes.c_int] + [_d] * 3 + [ctypes.c_double] * 3
+    L.orc_rungs.restype = ctypes.c_int
+    L.orc_rungs.argtypes = [ctypes.c_int] + [_d] * 3 + [ctypes.c_double] * 3 + [ctypes.c_int, _i]
+    L.orc_block_steps.restype = ctypes.c_int
+    L.orc_block_steps.argtypes = [ctypes.c_int] + [_d] * 10 + [_i] + [ctypes.c_double] * 2 + \
+        [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+         ctypes.c_int, ctypes.c_int, _i64]
     L.orc_leapfrog.argtypes = [ctypes.c_int] + [_d] * 10 + [ctypes.c_double] * 3 + [ctypes.c_int] * 2 + \
         [ctypes.c_double, ctypes.c_int]
     L.orc_total_energy.restype = ctypes.c_double
-    L.orc_total_energy.argtypes = [ctypes.c_int] + [_d] * 7 + [ctypes.c_double] * 2
+    L.orc_total_energy.argtypes = [ctypes.c_int] + [_d] * 7 + [ctypes.c_double] * 2 + [ctypes.c_int]
     L.orc_moments.argtypes = [ctypes.c_int] + [_d] * 7 + [_d] * 3
     L.orc_root_box.argtypes = [ctypes.c_int] + [_d] * 3 + [_d, _d]
     L.orc_morton_keys.argtypes = [ctypes.c_int] + [_d] * 3 + [_d, ctypes.c_double, ctypes.c_int, _u64, _u64]
@@ -115,13 +128,14 @@ class Tree:
                     quad=quad, parent=topo[:, 0], first=topo[:, 1], count=topo[:, 2], depth=topo[:, 3],
                     child_count=topo[:, 4], children=ch, order=order)
 
-    def accel_bh(self, G, eps2, theta, counts=False):
+    def accel_bh(self, G, eps2, theta, counts=False, spline=False):
         n = self.n
         ax, ay, az = (np.empty(n, np.float64) for _ in range(3))
         nfar = np.empty(self.num_nodes, np.int64) if counts else None
         nnear = np.empty(self.num_nodes, np.int64) if counts else None
         self._L.orc_accel_bh(self._h, n, _ptr(self.x, _d), _ptr(self.y, _d), _ptr(self.z, _d), _ptr(self.m, _d),
-                             G, eps2, theta, _ptr(ax, _d), _ptr(ay, _d), _ptr(az, _d), _ptr(nfar, _i64),
+                             G, eps2, theta, 1 if spline else 0, _ptr(ax, _d), _ptr(ay, _d), _ptr(az, _d),
+                             _ptr(nfar, _i64),
                              _ptr(nnear, _i64))
         a = np.stack([ax, ay, az], axis=1)
         return (a, nfar, nnear) if counts else a
@@ -139,31 +153,43 @@ class Tree:
         self._L.orc_group_lists(self._h, theta, _ptr(fp, _i64), _ptr(fi, _i), _ptr(qp, _i64), _ptr(qi, _i))
         return fp, fi, qp, qi
 
-    def accel_bh_per_particle(self, G, eps2, theta, relative=False, alpha=0.001, a_old=None):
+    def accel_bh_per_particle(self, G, eps2, theta, relative=False, alpha=0.001, a_old=None, spline=False):
+        """ComputeAccelBarnesHutPerParticle (Solvers.cpp:359-382)."""
         n = self.n
         ax, ay, az = (np.empty(n, np.float64) for _ in range(3))
         ao = _f64(a_old) if a_old is not None else None
         self._L.orc_accel_bh_pp(self._h, n, _ptr(self.x, _d), _ptr(self.y, _d), _ptr(self.z, _d),
-                                _ptr(self.m, _d), G, eps2, theta, 1 if relative else 0, alpha, _ptr(ao, _d),
-                                _ptr(ax, _d), _ptr(ay, _d), _ptr(az, _d))
+                                _ptr(self.m, _d), G, eps2, theta, 1 if spline else 0, 1 if relative else 0, alpha,
+                                _ptr(ao, _d), _ptr(ax, _d), _ptr(ay, _d), _ptr(az, _d))
+        return np.stack([ax, ay, az], axis=1)
+
+    def accel_bh_targets(self, G, eps2, theta, targets, spline=False):
+        """ComputeAccelBarnesHutTargets (Solvers.cpp:384-405), results by target slot."""
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        nt = len(t)
+        ax, ay, az = (np.empty(nt, np.float64) for _ in range(3))
+        self._L.orc_accel_bh_targets(self._h, self.n, _ptr(self.x, _d), _ptr(self.y, _d), _ptr(self.z, _d),
+                                     _ptr(self.m, _d), G, eps2, theta, 1 if spline else 0, nt, _ptr(t, _i),
+                                     _ptr(ax, _d), _ptr(ay, _d), _ptr(az, _d))
         return np.stack([ax, ay, az], axis=1)
 
 
-def accel_direct(x, y, z, m, G, eps2, fast=False):
+def accel_direct(x, y, z, m, G, eps2, fast=False, spline=False):
     x, y, z, m = (_f64(a) for a in (x, y, z, m))
     n = len(x)
     ax, ay, az = (np.empty(n, np.float64) for _ in range(3))
-    lib(fast).orc_accel_direct(n, _ptr(x, _d), _ptr(y, _d), _ptr(z, _d), _ptr(m, _d), G, eps2,
+    lib(fast).orc_accel_direct(n, _ptr(x, _d), _ptr(y, _d), _ptr(z, _d), _ptr(m, _d), G, eps2, 1 if spline else 0,
                                _ptr(ax, _d), _ptr(ay, _d), _ptr(az, _d))
     return np.stack([ax, ay, az], axis=1)
 
 
-def accel_direct_targets(x, y, z, m, G, eps2, targets, fast=False):
+def accel_direct_targets(x, y, z, m, G, eps2, targets, fast=False, spline=False):
     x, y, z, m = (_f64(a) for a in (x, y, z, m))
     t = np.ascontiguousarray(targets, dtype=np.int32)
     nt = len(t)
     ax, ay, az = (np.empty(nt, np.float64) for _ in range(3))
-    lib(fast).orc_accel_direct_targets(len(x), _ptr(x, _d), _ptr(y, _d), _ptr(z, _d), _ptr(m, _d), G, eps2, nt,
+    lib(fast).orc_accel_direct_targets(len(x), _ptr(x, _d), _ptr(y, _d), _ptr(z, _d), _ptr(m, _d), G, eps2,
+                                       1 if spline else 0, nt,
                                        _ptr(t, _i), _ptr(ax, _d), _ptr(ay, _d), _ptr(az, _d))
     return np.stack([ax, ay, az], axis=1)
 
@@ -178,9 +204,53 @@ def leapfrog(state, a, G, eps2, theta, dt, nsteps, solver="barnes_hut", quad=Tru
     return s, np.stack([ax, ay, az], axis=1)
 
 
-def total_energy(s, G, eps2, fast=False):
+def steps(state, a, G, eps2, theta, dt, nsteps, solver="barnes_hut", quad=True, spline=False, relative=False,
+          alpha=0.001, a_old=None, adaptive=False, eta=0.03, fast=False):
+    """System::Step x nsteps (global scheme) on copies: returns (state, a, a_old, time advanced).
+    a_old (previous |a|, zeros if None) is read by the relative MAC and captured after every step."""
+    s = {k: _f64(state[k]).copy() for k in ("x", "y", "z", "vx", "vy", "vz", "m")}
+    ax, ay, az = (np.ascontiguousarray(a[:, k], dtype=np.float64).copy() for k in range(3))
+    n = len(s["x"])
+    ao = _f64(a_old).copy() if a_old is not None else np.zeros(n, np.float64)
+    t = lib(fast).orc_steps(n, *[_ptr(s[k], _d) for k in ("x", "y", "z", "vx", "vy", "vz", "m")],
+                            _ptr(ax, _d), _ptr(ay, _d), _ptr(az, _d), _ptr(ao, _d), G, eps2, 1 if spline else 0,
+                            theta, 0 if solver == "direct" else 1, 1 if quad else 0, 1 if relative else 0, alpha,
+                            1 if adaptive else 0, eta, dt, nsteps)
+    return s, np.stack([ax, ay, az], axis=1), ao, t
+
+
+def adaptive_dt(a, eta, eps, dt):
+    """ChooseAdaptiveDt (Integrator.cpp:41-59) of accelerations a (n,3)."""
+    ax, ay, az = (np.ascontiguousarray(a[:, k], dtype=np.float64) for k in range(3))
+    return lib().orc_adaptive_dt(len(ax), _ptr(ax, _d), _ptr(ay, _d), _ptr(az, _d), eta, eps, dt)
+
+
+def rungs(a, eta, eps, base, max_rung=8):
+    """Block-step rungs (System.hpp:249-265) of accelerations a (n,3): (rungs, deepest)."""
+    ax, ay, az = (np.ascontiguousarray(a[:, k], dtype=np.float64) for k in range(3))
+    r = np.empty(len(ax), np.int32)
+    rmax = lib().orc_rungs(len(ax), _ptr(ax, _d), _ptr(ay, _d), _ptr(az, _d), eta, eps, base, max_rung, _ptr(r, _i))
+    return r, int(rmax)
+
+
+def block_steps(state, a, G, eps2, theta, base, nsteps, eta=0.03, max_rung=8, solver="barnes_hut", quad=True,
+                spline=False, fast=False):
+    """System::BlockStep x nsteps on copies: returns (state, a, rungs, deepest rung, particle evaluations)."""
+    s = {k: _f64(state[k]).copy() for k in ("x", "y", "z", "vx", "vy", "vz", "m")}
+    ax, ay, az = (np.ascontiguousarray(a[:, k], dtype=np.float64).copy() for k in range(3))
+    n = len(s["x"])
+    r = np.zeros(n, np.int32)
+    ev = np.zeros(1, np.int64)
+    rmax = lib(fast).orc_block_steps(n, *[_ptr(s[k], _d) for k in ("x", "y", "z", "vx", "vy", "vz", "m")],
+                                     _ptr(ax, _d), _ptr(ay, _d), _ptr(az, _d), _ptr(r, _i), G, eps2,
+                                     1 if spline else 0, theta, 0 if solver == "direct" else 1, 1 if quad else 0,
+                                     eta, base, max_rung, nsteps, _ptr(ev, _i64))
+    return s, np.stack([ax, ay, az], axis=1), r, int(rmax), int(ev[0])
+
+
+def total_energy(s, G, eps2, fast=False, spline=False):
     arrs = [_f64(s[k]) for k in ("x", "y", "z", "vx", "vy", "vz", "m")]
-    return lib(fast).orc_total_energy(len(arrs[0]), *[_ptr(a, _d) for a in arrs], G, eps2)
+    return lib(fast).orc_total_energy(len(arrs[0]), *[_ptr(a, _d) for a in arrs], G, eps2, 1 if spline else 0)
 
 
 def moments(s):

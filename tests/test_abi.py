@@ -60,6 +60,6 @@ def test_host_library_exports_its_header():
 def test_deck_runner_rejects_unsupported_features(tmp_path):
     """Error path on the CPU: the deck is validated before any device work."""
     exe = os.path.join(ROOT, "opengl-physics_amd", "nbg_deck")
-    r = subprocess.run([exe, "--deck", os.path.join(ROOT, "tests", "decks", "spline_unsupported.toml"), "--out",
+    r = subprocess.run([exe, "--deck", os.path.join(ROOT, "tests", "decks", "fmm_unsupported.toml"), "--out",
                         str(tmp_path), "--quiet"], capture_output=True, text=True, timeout=60)
-    assert r.returncode == 1 and "spline" in r.stderr
+    assert r.returncode == 1 and "fmm" in r.stderr
