@@ -208,6 +208,80 @@ def drift_main(args):
     print(json.dumps(out), flush=True)
 
 
+def modes_main(args):
+    """The other deck-reachable force and step modes on the 1M Plummer state (seed 0,
+    eps 0.005, quadrupoles): grouped walk with Plummer and spline softening, per-particle
+    walk with the geometric and the relative MAC, one adaptive step and one block (rung)
+    coarse step.  Force modes: mean HIP-event walk time of `steps` evaluations, interaction
+    counts and the FP32 rate they imply, relative RMS against the direct sum on 16,384
+    targets."""
+    import pynbg
+    n = args.n
+    reps = args.steps if args.steps_given else 5
+    eps, theta = args.eps, 0.5
+    ic = pynbg.to_f32(pynbg.ic_plummer(n, 0))
+    rng = np.random.default_rng(12345)
+    targets = np.sort(rng.choice(n, min(n, args.error_sample), replace=False))
+    base = dict(G=1.0, eps=eps, theta=theta, dt=args.dt)
+    modes = [("grouped_plummer", dict()), ("grouped_spline", dict(softening="spline")),
+             ("per_particle_geometric", dict(per_particle=True)),
+             ("relative_alpha0.001", dict(mac="relative", alpha=0.001)),
+             ("relative_alpha0.004", dict(mac="relative", alpha=0.004))]
+    out_modes = {}
+    with pynbg.Context(n) as ctx:
+        ctx.upload(ic["x"], ic["y"], ic["z"], ic["m"], ic["vx"], ic["vy"], ic["vz"])
+        ref_p = ctx.direct_targets(pynbg.make_params("direct", G=1.0, eps=eps), targets).astype(np.float64)
+        ref_s = ctx.direct_targets(pynbg.make_params("direct", G=1.0, eps=eps, softening="spline"),
+                                   targets).astype(np.float64)
+        # history for the relative MAC: |a| of a grouped theta = 0.5 evaluation (as after System::Prime)
+        ctx.accel(pynbg.make_params("barnes_hut", **base))
+        a_hist = ctx.accelerations()
+        for name, kw in modes:
+            pc = pynbg.make_params("barnes_hut", count=True, **base, **kw)
+            p = pynbg.make_params("barnes_hut", **base, **kw)
+            ctx.upload_accel(a_hist[:, 0], a_hist[:, 1], a_hist[:, 2])
+            stc = ctx.accel(pc)
+            a = ctx.accelerations()[targets].astype(np.float64)
+            ref = ref_s if kw.get("softening") == "spline" else ref_p
+            err = float(np.sqrt(np.sum((a - ref) ** 2) / np.sum(ref ** 2)))
+            ms = []
+            for _ in range(reps + 1):
+                ctx.upload_accel(a_hist[:, 0], a_hist[:, 1], a_hist[:, 2])
+                ms.append(ctx.accel(p).ms_walk)
+            walk_ms = float(np.mean(ms[1:]))
+            flop = FLOP_P2P * stc.n_p2p + FLOP_M2P_QUAD * stc.n_m2p
+            out_modes[name] = {"walk_ms": round(walk_ms, 4), "n_p2p": int(stc.n_p2p), "n_m2p": int(stc.n_m2p),
+                               "tflops": round(flop / (walk_ms * 1e-3) / 1e12, 2),
+                               "frac_fp32_peak": round(flop / (walk_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+                               "rel_rms_vs_direct": err}
+            log(f"{name}: walk {walk_ms:.3f} ms, m2p {stc.n_m2p:.3e} p2p {stc.n_p2p:.3e}, "
+                f"{out_modes[name]['tflops']} TFLOP/s, rel_rms {err:.3e}")
+        # one adaptive step (graph) and one block coarse step from the primed state
+        for name, kw in (("adaptive_step", dict(adaptive=True, eta=0.03)),
+                         ("block_coarse_step", dict(block=True, eta=0.03, block_max_rung=8))):
+            p = pynbg.make_params("barnes_hut", **base, **kw)
+            ctx.upload(ic["x"], ic["y"], ic["z"], ic["m"], ic["vx"], ic["vy"], ic["vz"])
+            ctx.accel(p)
+            ctx.step(p, 1)  # warm-up (graph capture)
+            t0 = time.perf_counter()
+            st = ctx.step(p, reps)
+            t1 = time.perf_counter()
+            out_modes[name] = {"ms_per_step": round(1e3 * (t1 - t0) / reps, 3), "dt_last": st.dt_last,
+                               "t_advanced": st.t_advanced, "rung_max": int(st.rung_max),
+                               "particle_evals_per_step": int(st.particle_evals) // reps}
+            log(f"{name}: {out_modes[name]}")
+    main_mode = out_modes["per_particle_geometric"]
+    out = {"metric": "force-evaluation walk time per mode, 1M-body Plummer", "value": main_mode["walk_ms"],
+           "unit": "ms/evaluation (per-particle geometric walk)", "n_gpus": 1, "steps": reps,
+           "higher_is_better": False, "dtype": "fp32", "data": "synthetic",
+           "config": {"workload": f"plummer_{n}_modes", "n": n, "eps": eps, "theta": theta, "multipole": "quadrupole",
+                      "dt": args.dt},
+           "modes": out_modes,
+           "roofline": {"bound": "fp32", "achieved": main_mode["tflops"], "peak": PEAK_FP32_TFLOPS,
+                        "unit": "TFLOP/s", "frac": main_mode["frac_fp32_peak"], "traffic": None}}
+    print(json.dumps(out), flush=True)
+
+
 def ensemble_cpu_baseline(off, st, nclusters=64):
     """One KDK step of the first `nclusters` clusters on the FP64 oracle."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -233,9 +307,10 @@ def ensemble_cpu_baseline(off, st, nclusters=64):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="plummer_1m", choices=["plummer_1m", "ensemble", "drift"],
+    ap.add_argument("--workload", default="plummer_1m", choices=["plummer_1m", "ensemble", "drift", "modes"],
                     help="plummer_1m: the headline learned/plain comparison; ensemble: 512 clusters; "
-                         "drift: 100-step energy/momentum drift of the 1M Plummer sphere")
+                         "drift: 100-step energy/momentum drift of the 1M Plummer sphere; modes: spline, "
+                         "per-particle/relative-MAC walks, adaptive and block steps on the 1M sphere")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1 << 20)
@@ -265,6 +340,9 @@ def main():
         return
     if args.workload == "drift":
         drift_main(args)
+        return
+    if args.workload == "modes":
+        modes_main(args)
         return
     if args.variant == "learned" and args.multipole != "quadrupole":
         print("the learned variant needs quadrupoles", file=sys.stderr)

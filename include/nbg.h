@@ -47,7 +47,8 @@ extern "C" {
 
 /* flags */
 #define NBG_FLAG_KAHAN 1u           /* compensated FP32 force sums */
-#define NBG_FLAG_COUNT 2u           /* record per-group interaction counts (nbg_interaction_counts) */
+#define NBG_FLAG_COUNT 2u           /* record interaction counts: per leaf group for the grouped walk
+                                       (nbg_interaction_counts), totals in nbg_stats for both walks */
 #define NBG_FLAG_LEARNED 4u         /* add the learned far-field correction (nbg_learn_set_weights) */
 #define NBG_FLAG_NO_GRAPH 8u        /* nbg_step: launch eagerly instead of replaying a hipGraph */
 #define NBG_FLAG_STAGE_TIMES 16u    /* nbg_step: per-stage HIP-event times summed over all steps

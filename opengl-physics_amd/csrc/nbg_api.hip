@@ -235,7 +235,11 @@ int accel_eager(nbg_ctx* c, const nbg_params& p, nbg_stats* stats, bool list_onl
                 stats->ms_total = elapsed(c->ev[0], c->ev[13]);
             }
         }
-        if (p.flags & NBG_FLAG_COUNT) {
+        if ((p.flags & NBG_FLAG_COUNT) && (p.mac == NBG_MAC_RELATIVE || (p.flags & NBG_FLAG_PER_PARTICLE))) {
+            // the per-particle walk counts its interactions directly
+            stats->n_m2p = static_cast<int64_t>(c->hsc->pp_m2p);
+            stats->n_p2p = static_cast<int64_t>(c->hsc->pp_p2p);
+        } else if (p.flags & NBG_FLAG_COUNT) {
             // interaction totals: per group counts times group sizes
             const int nn = c->hsc->n_nodes;
             std::vector<int32_t> f(nn), q(nn), cnt(nn);
@@ -560,7 +564,7 @@ void nbg_destroy(nbg_ctx* c) {
     free_dev(c->adam_m); free_dev(c->adam_v); free_dev(c->pool_feat); free_dev(c->pool_tgt); free_dev(c->amag2);
     free_dev(c->grad_part); free_dev(c->loss_part); free_dev(c->loss_dev);
     free_dev(c->sys_off); free_dev(c->sys_root); free_dev(c->sys_geom); free_dev(c->sys_bbox);
-    free_dev(c->blk_sys); free_dev(c->blk_start); free_dev(c->blk_count); free_dev(c->sys_blk0); free_dev(c->bin_start); free_dev(c->bin_count);
+    free_dev(c->blk_sys); free_dev(c->blk_start); free_dev(c->blk_count); free_dev(c->sys_blk0); free_dev(c->bin_start); free_dev(c->bin_count); free_dev(c->bin_cost);
     if (c->hsc) (void)hipHostFree(c->hsc);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -630,6 +634,7 @@ int nbg_upload_batch(nbg_ctx* c, int32_t nsys, const int64_t* offsets, const flo
     c->n_bins = static_cast<int>(wstart.size());
     r |= alloc_into(c, c->bin_start, std::max<size_t>(1, wstart.size()));
     r |= alloc_into(c, c->bin_count, std::max<size_t>(1, wcnt.size()));
+    r |= alloc_into(c, c->bin_cost, std::max<size_t>(1, wcnt.size()));
     const size_t parts = std::max<size_t>(grid_for(n + 1, 2048), grid_for(static_cast<int64_t>(kRadix) * nb, 2048)) + 8;
     r |= alloc_into(c, c->scan_part, parts);
     if (r != NBG_OK) return r;
@@ -644,6 +649,9 @@ int nbg_upload_batch(nbg_ctx* c, int32_t nsys, const int64_t* offsets, const flo
         NBG_HIP_CHECK(c, hipMemcpy(c->bin_count, wcnt.data(), wcnt.size() * 4, hipMemcpyHostToDevice));
     }
     NBG_HIP_CHECK(c, hipMemcpy(c->sys_off, off32.data(), off32.size() * 4, hipMemcpyHostToDevice));
+    // no cost history for the new layout
+    NBG_HIP_CHECK(c, hipMemset(c->bin_cost, 0, std::max<size_t>(1, wcnt.size()) * 4));
+    NBG_HIP_CHECK(c, hipMemset(&c->dsc->pp_cost_sum, 0, 2 * sizeof(uint64_t)));
     if (n > 0) {
         NBG_HIP_CHECK(c, hipMemcpy(c->pos_sys, psys.data(), psys.size() * 4, hipMemcpyHostToDevice));
         const size_t bytes = static_cast<size_t>(n) * sizeof(float);

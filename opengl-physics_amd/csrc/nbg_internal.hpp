@@ -53,6 +53,12 @@ struct DevScalars {
     uint64_t particle_evals;    // block steps: targets evaluated since the host cleared it
     uint32_t tcount;            // entries of the current target list
     int32_t rung_max;           // block steps: deepest rung of the coarse step
+    uint64_t pp_m2p;            // per-particle walk interactions (NBG_FLAG_COUNT), zeroed by the build
+    uint64_t pp_p2p;
+    uint32_t work2;             // work counter of the one-target-per-wave walk, zeroed by the build
+    uint32_t pad3;
+    uint64_t pp_cost_sum;       // relative-MAC walk: traversal steps over all bins (this walk)
+    uint64_t pp_cost_prev;      // the same of the previous walk (the build moves sum here)
 };
 
 // ---------------------------------------------------------------------------- helpers
@@ -111,6 +117,7 @@ struct nbg_ctx {
     int n_bins = 0;
     int32_t* bin_start = nullptr;
     int32_t* bin_count = nullptr;
+    uint32_t* bin_cost = nullptr;   // relative-MAC traversal steps of each bin, last walk
 
     // particle state, original order (FP32 SoA)
     float *x = nullptr, *y = nullptr, *z = nullptr, *m = nullptr;

@@ -48,6 +48,7 @@ struct Soft {
     float eps2;    // Plummer eps^2
     float inv_h;   // spline 1/h
     float inv_h3;  // spline 1/h^3
+    float four_h2; // spline (2h)^2: the kernel's support
 };
 
 inline Soft make_soft(const nbg_params& p) {
@@ -57,6 +58,7 @@ inline Soft make_soft(const nbg_params& p) {
     const double ih = (h > 0.0) ? 1.0 / h : 0.0;
     s.inv_h = static_cast<float>(ih);
     s.inv_h3 = static_cast<float>(ih * ih * ih);
+    s.four_h2 = static_cast<float>(4.0 * p.eps2);
     return s;
 }
 
@@ -68,20 +70,26 @@ inline Soft make_soft(const nbg_params& p) {
 //            1 <= q < 2; q^3 (4/3 - 1.2 q^2 + 0.5 q^3) for q < 1, where chi / r^3 =
 //            (4/3 - 1.2 q^2 + 0.5 q^3) / h^3 needs no division and stays finite at r = 0
 //            (d = 0 there, so the term vanishes as the reference's r <= 1e-300 branch makes it).
+//            Most pairs of a walk lie beyond 2h: the kernel branch runs only when a lane of
+//            the wave is inside the support (a wave-uniform branch on a ballot).
 template <bool SPLINE>
 __device__ __forceinline__ float pair_coef(float r2, float gm, const Soft& s) {
     if (!SPLINE) {
         const float inv = __builtin_amdgcn_rsqf(r2 + s.eps2);
         return gm * (inv * inv * inv);
     } else {
-        const float q = sqrtf(r2) * s.inv_h;
-        const float inv = __builtin_amdgcn_rsqf(r2);  // used only where q >= 1 (r2 > 0)
-        const float t = 2.f - q;
-        const float t2 = t * t;
-        const float chi_out = (q >= 2.f) ? 1.f : 1.f - (t2 * t2) * (1.f - t * (0.8f - t * (1.f / 6.f)));
-        const float f_out = gm * chi_out * (inv * inv * inv);
-        const float f_in = gm * s.inv_h3 * (4.f / 3.f + (q * q) * (0.5f * q - 1.2f));
-        return (q < 1.f) ? f_in : f_out;
+        const float inv = __builtin_amdgcn_rsqf(r2);  // inf at r2 = 0, replaced below
+        float f = gm * (inv * inv * inv);              // q >= 2: Newtonian
+        const bool inside = r2 < s.four_h2;
+        if (__ballot(inside)) {
+            const float q = sqrtf(r2) * s.inv_h;
+            const float t = 2.f - q;
+            const float t2 = t * t;
+            const float chi = 1.f - (t2 * t2) * (1.f - t * (0.8f - t * (1.f / 6.f)));
+            const float f_in = gm * s.inv_h3 * (4.f / 3.f + (q * q) * (0.5f * q - 1.2f));
+            f = inside ? ((q < 1.f) ? f_in : f * chi) : f;
+        }
+        return f;
     }
 }
 

@@ -150,7 +150,15 @@ __global__ void k_reset(DevScalars* __restrict__ sc, uint32_t* __restrict__ bbox
         sc->cap_exceeded = 0;
     }
     if (t < 8) sc->work[t] = 0;
-    if (t == 0) sc->learn_pairs = 0;
+    if (t == 0) {
+        sc->learn_pairs = 0;
+        sc->pp_m2p = 0;
+        sc->pp_p2p = 0;
+        sc->work2 = 0;
+        sc->tcount = 0;
+        sc->pp_cost_prev = sc->pp_cost_sum;
+        sc->pp_cost_sum = 0;
+    }
     for (int s = t; s < nsys; s += blockDim.x) {
         bbox[6 * s + 0] = 0xffffffffu;
         bbox[6 * s + 1] = 0xffffffffu;

@@ -370,6 +370,8 @@ def load_host():
     H.ngh_ic_plummer.argtypes = [ctypes.c_int, ctypes.c_uint32] + [ctypes.c_double] * 3 + [_d] * 7
     H.ngh_ic_hernquist.argtypes = [ctypes.c_int, ctypes.c_uint32] + [ctypes.c_double] * 3 + [_d] * 7
     H.ngh_ic_king.argtypes = [ctypes.c_int, ctypes.c_uint32] + [ctypes.c_double] * 4 + [_d] * 7
+    H.ngh_ic_cluster.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_int] + [_d] * 7
+    H.ngh_ic_cluster.restype = ctypes.c_int
     H.ngh_ensemble_sizes.argtypes = [ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64, ctypes.c_int64, _i64, _i32]
     H.ngh_ensemble_sizes.restype = ctypes.c_int64
     for f in (H.ngh_ic_plummer, H.ngh_ic_hernquist, H.ngh_ic_king):
@@ -400,6 +402,17 @@ def ic_hernquist(n, seed, total_mass=1.0, scale_radius=1.0, G=1.0):
 def ic_king(n, seed, w0=6.0, total_mass=1.0, core_radius=1.0, G=1.0):
     """ic::King<3>(W0), FP64 arrays x,y,z,vx,vy,vz,m."""
     return _ic(load_host().ngh_ic_king, n, seed, total_mass, core_radius, w0, G)
+
+
+def ic_cluster(n, seed, cold=False):
+    """Uniform-sphere cluster (Cluster<3>; cold=True: ColdUniform, the cold_collapse
+    scenario), FP64 arrays x,y,z,vx,vy,vz,m."""
+    out = {k: np.empty(n, np.float64) for k in ("x", "y", "z", "vx", "vy", "vz", "m")}
+    rc = load_host().ngh_ic_cluster(int(n), int(seed), 1 if cold else 0,
+                                    *[_p(out[k], _d) for k in ("x", "y", "z", "vx", "vy", "vz", "m")])
+    if rc != 0:
+        raise NbgError(f"initial-condition sampler failed (rc={rc})")
+    return out
 
 
 def ensemble(nclusters=512, seed=0, nmin=1024, nmax=65536, w0=6.0):

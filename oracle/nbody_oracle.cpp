@@ -344,13 +344,15 @@ void Collect(const Tree& t, int node, int group, double gx, double gy, double gz
 
 // Per-particle walk, geometric or relative MAC (Solvers.cpp:181-227).
 void WalkPP(const Tree& t, const Pts& p, int node, int i, double xi, double yi, double zi, double G, double theta2,
-            const Soft& eps2, int relative, double alpha, double aOld, double& ax, double& ay, double& az) {
+            const Soft& eps2, int relative, double alpha, double aOld, double& ax, double& ay, double& az,
+            int64_t* cnt = nullptr) {
     if (t.mass[node] <= 0.0) return;
     if (t.childCount[node] == 0) {
         for (int k = t.first[node]; k < t.first[node] + t.count[node]; ++k) {
             const int q = t.order[k];
             if (q == i) continue;
             AccumPair(p.x[q] - xi, p.y[q] - yi, p.z[q] - zi, G * p.m[q], eps2, ax, ay, az);
+            if (cnt) ++cnt[1];
         }
         return;
     }
@@ -365,11 +367,12 @@ void WalkPP(const Tree& t, const Pts& p, int node, int i, double xi, double yi, 
     if (accept) {
         AccumPair(dx, dy, dz, G * t.mass[node], eps2, ax, ay, az);
         AddQuad(t, node, dx, dy, dz, dist2, G, ax, ay, az);
+        if (cnt) ++cnt[0];
         return;
     }
     for (int c = 0; c < 8; ++c)
         if (t.child[node][c] != -1)
-            WalkPP(t, p, t.child[node][c], i, xi, yi, zi, G, theta2, eps2, relative, alpha, aOld, ax, ay, az);
+            WalkPP(t, p, t.child[node][c], i, xi, yi, zi, G, theta2, eps2, relative, alpha, aOld, ax, ay, az, cnt);
 }
 
 } // namespace
@@ -613,20 +616,30 @@ void orc_group_lists(const void* h, double theta, int64_t* far_ptr, int32_t* far
 
 // ComputeAccelBarnesHutPerParticle (Solvers.cpp:359-382): relative MAC when
 // relative != 0 and aOld (nullable) holds each particle's previous |a|.
+// counts (nullable, 2 entries) receives the totals of M2P and P2P interactions.
 void orc_accel_bh_pp(const void* h, int n, const double* x, const double* y, const double* z, const double* m,
                      double G, double eps2_, double theta, int spline, int relative, double alpha, const double* aOld,
-                     double* ax, double* ay, double* az) {
+                     double* ax, double* ay, double* az, int64_t* counts) {
     const Tree& t = *static_cast<const Tree*>(h);
     const Pts p{x, y, z, m, n};
     const Soft eps2 = MakeSoft(eps2_, spline);
     const double theta2 = theta * theta;
-#pragma omp parallel for schedule(dynamic, 64)
+    int64_t nm2p = 0, np2p = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : nm2p, np2p)
     for (int i = 0; i < n; ++i) {
         double sx = 0.0, sy = 0.0, sz = 0.0;
-        WalkPP(t, p, 0, i, x[i], y[i], z[i], G, theta2, eps2, relative, alpha, aOld ? aOld[i] : 0.0, sx, sy, sz);
+        int64_t cnt[2] = {0, 0};
+        WalkPP(t, p, 0, i, x[i], y[i], z[i], G, theta2, eps2, relative, alpha, aOld ? aOld[i] : 0.0, sx, sy, sz,
+               cnt);
         ax[i] = sx;
         ay[i] = sy;
         az[i] = sz;
+        nm2p += cnt[0];
+        np2p += cnt[1];
+    }
+    if (counts) {
+        counts[0] = nm2p;
+        counts[1] = np2p;
     }
 }
 
@@ -703,7 +716,7 @@ void ComputeAccel(int n, const double* x, const double* y, const double* z, cons
     }
     void* t = orc_tree_build(n, x, y, z, m, 8, 40, withQuad);
     if (relative)
-        orc_accel_bh_pp(t, n, x, y, z, m, G, eps2, theta, spline, 1, alpha, aOld, ax, ay, az);
+        orc_accel_bh_pp(t, n, x, y, z, m, G, eps2, theta, spline, 1, alpha, aOld, ax, ay, az, nullptr);
     else
         orc_accel_bh(t, n, x, y, z, m, G, eps2, theta, spline, ax, ay, az, nullptr, nullptr);
     orc_tree_free(t);

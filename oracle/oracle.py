@@ -42,7 +42,7 @@ def lib(fast=False):
     L.orc_accel_bh.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_d] * 4 + [ctypes.c_double] * 3 + \
         [ctypes.c_int] + [_d] * 3 + [_i64] * 2
     L.orc_accel_bh_pp.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_d] * 4 + [ctypes.c_double] * 3 + \
-        [ctypes.c_int, ctypes.c_int, ctypes.c_double, _d] + [_d] * 3
+        [ctypes.c_int, ctypes.c_int, ctypes.c_double, _d] + [_d] * 3 + [_i64]
     L.orc_accel_bh_targets.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_d] * 4 + [ctypes.c_double] * 3 + \
         [ctypes.c_int, ctypes.c_int, _i] + [_d] * 3
     L.orc_group_lists.argtypes = [ctypes.c_void_p, ctypes.c_double, _i64, _i, _i64, _i]
@@ -153,15 +153,19 @@ class Tree:
         self._L.orc_group_lists(self._h, theta, _ptr(fp, _i64), _ptr(fi, _i), _ptr(qp, _i64), _ptr(qi, _i))
         return fp, fi, qp, qi
 
-    def accel_bh_per_particle(self, G, eps2, theta, relative=False, alpha=0.001, a_old=None, spline=False):
-        """ComputeAccelBarnesHutPerParticle (Solvers.cpp:359-382)."""
+    def accel_bh_per_particle(self, G, eps2, theta, relative=False, alpha=0.001, a_old=None, spline=False,
+                              counts=False):
+        """ComputeAccelBarnesHutPerParticle (Solvers.cpp:359-382); with counts, also the
+        totals (M2P, P2P) of the interactions."""
         n = self.n
         ax, ay, az = (np.empty(n, np.float64) for _ in range(3))
         ao = _f64(a_old) if a_old is not None else None
+        cnt = np.zeros(2, np.int64)
         self._L.orc_accel_bh_pp(self._h, n, _ptr(self.x, _d), _ptr(self.y, _d), _ptr(self.z, _d),
                                 _ptr(self.m, _d), G, eps2, theta, 1 if spline else 0, 1 if relative else 0, alpha,
-                                _ptr(ao, _d), _ptr(ax, _d), _ptr(ay, _d), _ptr(az, _d))
-        return np.stack([ax, ay, az], axis=1)
+                                _ptr(ao, _d), _ptr(ax, _d), _ptr(ay, _d), _ptr(az, _d), _ptr(cnt, _i64))
+        a = np.stack([ax, ay, az], axis=1)
+        return (a, int(cnt[0]), int(cnt[1])) if counts else a
 
     def accel_bh_targets(self, G, eps2, theta, targets, spline=False):
         """ComputeAccelBarnesHutTargets (Solvers.cpp:384-405), results by target slot."""

@@ -57,3 +57,51 @@ def test_kepler_deck_conserves_energy(tmp_path):
     rows = list(csv.DictReader(open(tmp_path / "diagnostics.csv")))
     drift = max(abs(float(r_["energy_drift_pct"])) for r_ in rows)
     assert drift < 0.05, drift
+
+
+def _rows(path):
+    return list(csv.DictReader(open(path / "diagnostics.csv")))
+
+
+def test_spline_deck_energy_matches_oracle(tmp_path):
+    import oracle
+    import pynbg
+    r = _run_deck("plummer_spline_gpu.toml", tmp_path)
+    assert r.returncode == 0, r.stderr
+    rows = _rows(tmp_path)
+    ic = {k: v.astype(np.float32).astype(np.float64) for k, v in pynbg.ic_plummer(2000, 1).items()}
+    e_ref = oracle.total_energy(ic, 1.0, 0.05 ** 2, spline=True)
+    assert abs(float(rows[0]["energy"]) - e_ref) <= 1e-9 * abs(e_ref)
+    assert max(abs(float(r_["energy_drift_pct"])) for r_ in rows) < 0.5
+
+
+def test_relative_mac_deck(tmp_path):
+    r = _run_deck("plummer_relative_mac_gpu.toml", tmp_path)
+    assert r.returncode == 0, r.stderr
+    rows = _rows(tmp_path)
+    assert len(rows) == 8
+    assert max(abs(float(r_["energy_drift_pct"])) for r_ in rows) < 0.5
+
+
+def test_adaptive_deck_dt_shrinks_at_periapsis(tmp_path):
+    r = _run_deck("eccentric_adaptive_gpu.toml", tmp_path)
+    assert r.returncode == 0, r.stderr
+    rows = _rows(tmp_path)
+    dts = [float(r_["dt_last"]) for r_ in rows]
+    nominal = dts[0]  # frame 0 reports the scenario's dt (DeckSim.cpp:83)
+    assert min(dts[1:]) < 0.2 * nominal and all(d <= nominal for d in dts)
+    assert max(abs(float(r_["energy_drift_pct"])) for r_ in rows) < 0.1
+
+
+def test_block_deck_cold_collapse(tmp_path):
+    r = _run_deck("cold_collapse_block_gpu.toml", tmp_path)
+    assert r.returncode == 0, r.stderr
+    rows = _rows(tmp_path)
+    assert len(rows) == 6 and all(float(r_["dt_last"]) == 3e-3 for r_ in rows)
+    assert all(int(float(r_["rung_max"])) >= 0 for r_ in rows)
+    pos = np.load(tmp_path / "frames" / "pos_0000.npy")
+    # the cold_collapse scenario: a uniform unit sphere
+    assert pos.shape == (4000, 3) and np.max(np.linalg.norm(pos, axis=1)) <= 1.0 + 1e-6
+    vel = np.load(tmp_path / "frames" / "vel_0000.npy")
+    assert not np.any(vel)
+    assert max(abs(float(r_["energy_drift_pct"])) for r_ in rows) < 0.5

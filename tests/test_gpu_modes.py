@@ -122,15 +122,21 @@ def test_per_particle_geometric_walk(gpu_ctx_factory, spline):
     soft = "spline" if spline else "plummer"
     ctx = gpu_ctx_factory(n)
     ctx.upload(s["x"], s["y"], s["z"], s["m"])
-    st = ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=eps, theta=0.5, per_particle=True, softening=soft))
+    st = ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=eps, theta=0.5, per_particle=True, softening=soft,
+                                     count=True))
     a = ctx.accelerations()
     T = oracle.Tree(x, y, z, m, quad=True)
-    ref = T.accel_bh_per_particle(1.0, eps * eps, 0.5, spline=spline)
+    ref, n_m2p, n_p2p = T.accel_bh_per_particle(1.0, eps * eps, 0.5, spline=spline, counts=True)
     err, worst = oracle.rel_rms(a, ref), _per_particle_max(a, ref)
-    print(f"per-particle walk ({soft}) rel_rms={err:.3e} max={worst:.3e} walk_ms={st.ms_walk:.3f}")
+    print(f"per-particle walk ({soft}) rel_rms={err:.3e} max={worst:.3e} walk_ms={st.ms_walk:.3f} "
+          f"m2p {st.n_m2p}/{n_m2p} p2p {st.n_p2p}/{n_p2p}")
     assert err < REL_RMS_TOL and worst < PER_PARTICLE_TOL
-    # theta = 0: every leaf exact, the direct sum
-    ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=eps, theta=0.0, per_particle=True, softening=soft))
+    # the interaction totals of every target's list are the reference's
+    assert st.n_m2p == n_m2p and st.n_p2p == n_p2p
+    # theta = 0: every leaf exact, the direct sum: n - 1 P2P sources per target, no M2P
+    st0 = ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=eps, theta=0.0, per_particle=True, softening=soft,
+                                      count=True))
+    assert st0.n_p2p == n * (n - 1) and st0.n_m2p == 0
     a0 = ctx.accelerations()
     ctx.accel(pynbg.make_params("direct", G=1.0, eps=eps, softening=soft))
     assert oracle.rel_rms(a0, ctx.accelerations()) < 2e-6
@@ -155,10 +161,17 @@ def test_relative_mac_walk(gpu_ctx_factory, alpha):
     # with history: aOld = |a| of the accelerations the context holds (a FP32 direct sum here)
     ctx.accel(pynbg.make_params("direct", G=1.0, eps=eps))
     a_prev = ctx.accelerations()
-    st = ctx.accel(prm)
+    st = ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=eps, theta=0.5, mac="relative", alpha=alpha,
+                                     count=True))
     a = ctx.accelerations()
     T = oracle.Tree(x, y, z, m, quad=True)
-    ref = T.accel_bh_per_particle(1.0, eps * eps, 0.5, relative=True, alpha=alpha, a_old=_amag64(a_prev))
+    ref, n_m2p, n_p2p = T.accel_bh_per_particle(1.0, eps * eps, 0.5, relative=True, alpha=alpha,
+                                                a_old=_amag64(a_prev), counts=True)
+    assert st.n_m2p == n_m2p and st.n_p2p == n_p2p
+    # the production (non-counting) kernel on the same history
+    ctx.upload_accel(a_prev[:, 0], a_prev[:, 1], a_prev[:, 2])
+    st = ctx.accel(prm)
+    a = ctx.accelerations()
     err, worst = oracle.rel_rms(a, ref), _per_particle_max(a, ref)
     exact = oracle.accel_direct(x, y, z, m, 1.0, eps * eps)
     print(f"relative MAC alpha={alpha} rel_rms vs oracle {err:.3e} max {worst:.3e}; "
