@@ -43,6 +43,21 @@ int64_t ngh_ensemble_sizes(int32_t nclusters, uint32_t seed, int64_t nmin, int64
  * frames <= 0 uses the deck's time.frames.  Returns 0, or -1 with a message in err. */
 int ngh_run_deck(const char* deck_path, const char* out_dir, int frames, int quiet, char* err, int err_len);
 
+/* --dump-ic of the reference command line (projects/02_nbody_gravity/src/main.cpp:249-280):
+ * the seed-1 Plummer sphere of N bodies (G = M = a = 1) as raw float32 records
+ * x,y,z,m,vx,vy,vz -- the ic_file format.  Returns 0, or -1 on a bad argument or I/O error. */
+int ngh_dump_ic(const char* path, int n);
+
+/* --bench of the reference command line (main.cpp:282-393) on the GPU backend: mean wall
+ * time of `reps` force evaluations (after one warm-up, upload and readback included) of
+ * `solver` (direct | gpu_direct | barnes_hut | bh | gpu_bh) on the seed-1 Plummer sphere with
+ * softening eps and opening angle theta, and the mean per-particle relative error against
+ * the device's compensated direct sum (-1 above N = 60,000).  Writes the CSV line
+ * "solver,n,theta,order,ms_per_eval,mean_rel_err" into csv.  Returns 0, or -1 with a
+ * message in err. */
+int ngh_bench(const char* solver, int n, double theta, double eps, int reps, int order, char* csv, int csv_len,
+              char* err, int err_len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,3 +63,28 @@ def test_deck_runner_rejects_unsupported_features(tmp_path):
     r = subprocess.run([exe, "--deck", os.path.join(ROOT, "tests", "decks", "fmm_unsupported.toml"), "--out",
                         str(tmp_path), "--quiet"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "fmm" in r.stderr
+
+
+def test_dump_ic_writes_the_bench_sphere(tmp_path):
+    """--dump-ic (main.cpp:249-280): the seed-1 Plummer sphere as raw float32 records
+    x,y,z,m,vx,vy,vz -- no GPU involved."""
+    import sys
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "opengl-physics_amd"))
+    import pynbg
+    exe = os.path.join(ROOT, "opengl-physics_amd", "nbg_deck")
+    out = tmp_path / "ic.bin"
+    r = subprocess.run([exe, "--dump-ic", str(out), "--bench-n", "1500"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert "N=1500" in r.stderr and "seed=1" in r.stderr
+    rec = np.fromfile(out, dtype="<f4").reshape(-1, 7)
+    ic = pynbg.ic_plummer(1500, 1)
+    want = np.stack([ic[k] for k in ("x", "y", "z", "m", "vx", "vy", "vz")], axis=1).astype(np.float32)
+    np.testing.assert_array_equal(rec, want)
+
+
+def test_cli_rejects_unknown_bench_solver():
+    exe = os.path.join(ROOT, "opengl-physics_amd", "nbg_deck")
+    r = subprocess.run([exe, "--bench", "spherical_fmm", "--bench-n", "100"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 2 and "unknown solver" in r.stderr

@@ -105,3 +105,38 @@ def test_block_deck_cold_collapse(tmp_path):
     vel = np.load(tmp_path / "frames" / "vel_0000.npy")
     assert not np.any(vel)
     assert max(abs(float(r_["energy_drift_pct"])) for r_ in rows) < 0.5
+
+
+def test_cli_bench_csv(tmp_path):
+    """--bench (main.cpp:282-393): one CSV line solver,n,theta,order,ms_per_eval,mean_rel_err."""
+    exe = os.path.join(PKG, "nbg_deck")
+    rows = {}
+    for solver in ("gpu_bh", "gpu_direct"):
+        r = subprocess.run([exe, "--bench", solver, "--bench-n", "20000", "--bench-reps", "3"], capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert "solver,n,theta,order,ms_per_eval,mean_rel_err" in r.stderr
+        f = r.stdout.strip().split(",")
+        rows[solver] = f
+        assert f[0] == solver and f[1] == "20000" and f[2] == "0.500" and f[3] == "5"
+        assert float(f[4]) > 0
+    assert 0 < float(rows["gpu_bh"][5]) < 5e-3
+    assert float(rows["gpu_direct"][5]) < 1e-6
+
+
+def test_ic_file_deck_reads_the_dump(tmp_path):
+    """--dump-ic output run through a deck with scenario.type = ic_file (Scenarios.cpp:292-329)."""
+    exe = os.path.join(PKG, "nbg_deck")
+    ic = tmp_path / "ic.bin"
+    r = subprocess.run([exe, "--dump-ic", str(ic), "--bench-n", "3000"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    deck = tmp_path / "ic_file.toml"
+    deck.write_text(f'title = "ic_file"\n[time]\nframes = 3\nsubsteps_per_frame = 2\n[solver]\nkind = "barnes_hut"\n'
+                    f'backend = "gpu"\n[scenario]\ntype = "ic_file"\nic_file = "{ic}"\n')
+    out = tmp_path / "out"
+    r = subprocess.run([os.path.join(PKG, "nbg_deck"), "--deck", str(deck), "--out", str(out), "--quiet"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rec = np.fromfile(ic, dtype="<f4").reshape(-1, 7)
+    pos0 = np.load(out / "frames" / "pos_0000.npy")
+    np.testing.assert_array_equal(pos0, rec[:, :3])

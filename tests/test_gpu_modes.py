@@ -3,7 +3,7 @@ FP64 oracle on the same FP32-rounded inputs:
   * spline softening (AccumPairSpline, projects/nbody_core/src/Solvers.cpp:43-63) in the
     direct sum, the grouped walk and the FP64 energy (SplinePotential3D, Softening.hpp:49-56);
   * the per-particle walk (WalkBH, Solvers.cpp:181-227) with the geometric and the relative
-    MAC (Mac.hpp:34-39), and onto target subsets (ComputeAccelBarnesHutTargets,
+    MAC (Mac.hpp:37-42), and onto target subsets (ComputeAccelBarnesHutTargets,
     Solvers.cpp:384-405);
   * adaptive global dt (ChooseAdaptiveDt, Integrator.cpp:41-59);
   * block / rung steps (System::BlockStep, System.hpp:239-294).

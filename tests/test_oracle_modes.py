@@ -11,7 +11,7 @@ the oracle's own formula:
     value phi(0) = -1.4 G m / h;
   * theta = 0 walks (grouped, per-particle, targets) against the direct sum (SelfTest.cpp:
     43-74 pins the same for Plummer);
-  * the relative MAC (Mac.hpp:34-39) with no history equals the geometric per-particle walk,
+  * the relative MAC (Mac.hpp:37-42) with no history equals the geometric per-particle walk,
     with alpha = 0 equals the direct sum, and its error shrinks with alpha;
   * adaptive dt (Integrator.cpp:41-59) and rungs (System.hpp:249-265) against hand values;
   * block steps (System.hpp:239-294): one rung reproduces the global KDK step bit for bit,
