@@ -371,3 +371,126 @@ def test_block_bh_cold_collapse_energy(gpu_ctx_factory):
     drift = abs((ctx.diagnostics(p)["energy"] - e0) / e0)
     print(f"BH block cold collapse: drift {drift:.2e}, deepest rung {deepest}")
     assert drift < 0.05
+
+
+# ------------------------------------------------------------------------- edge cases
+def test_relative_mac_steps_graph_equals_eager_and_follow_the_oracle(gpu_ctx_factory):
+    """System::Step with the relative MAC: aOld is |a| captured after every step
+    (System.hpp:302-310); the graph-replayed steps equal eager ones bit for bit and follow
+    the oracle's FP64 steps."""
+    import oracle
+    import pynbg
+    n, eps, dt = 4096, 0.02, 2e-3
+    s = plummer_f32(n, 8)
+    st64 = {k: s[k].astype(np.float64) for k in s}
+    runs = []
+    for no_graph in (True, False):
+        ctx = gpu_ctx_factory(n)
+        ctx.upload(s["x"], s["y"], s["z"], s["m"], s["vx"], s["vy"], s["vz"])
+        p = pynbg.make_params("barnes_hut", G=1.0, eps=eps, theta=0.5, dt=dt, mac="relative", alpha=0.002,
+                              no_graph=no_graph)
+        ctx.accel(p)   # Prime: no history yet -> the geometric per-particle walk
+        a0 = ctx.accelerations()
+        ctx.step(p, 6)
+        runs.append((ctx.download(), a0))
+    for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az"):
+        np.testing.assert_array_equal(runs[0][0][k], runs[1][0][k])
+    g, a0 = runs[1]
+    T = oracle.Tree(st64["x"], st64["y"], st64["z"], st64["m"], quad=True)
+    a0_ref = T.accel_bh_per_particle(1.0, eps * eps, 0.5)
+    assert oracle.rel_rms(a0, a0_ref) < REL_RMS_TOL
+    aold0 = np.sqrt((a0_ref[:, 0] ** 2 + a0_ref[:, 1] ** 2) + a0_ref[:, 2] ** 2)
+    o, a_o, _, _ = oracle.steps(st64, a0_ref, 1.0, eps * eps, 0.5, dt, 6, relative=True, alpha=0.002, a_old=aold0)
+    pos_g = np.stack([g["x"], g["y"], g["z"]], 1).astype(np.float64)
+    pos_o = np.stack([o["x"], o["y"], o["z"]], 1)
+    a_g = np.stack([g["ax"], g["ay"], g["az"]], 1).astype(np.float64)
+    print(f"relative steps: pos rel {oracle.rel_rms(pos_g, pos_o):.2e}, a rel {oracle.rel_rms(a_g, a_o):.2e}")
+    assert oracle.rel_rms(pos_g, pos_o) < 1e-6
+    assert oracle.rel_rms(a_g, a_o) < 1e-4
+
+
+@pytest.mark.parametrize("mode", ["per_particle", "relative"])
+def test_per_particle_walks_on_a_batch(gpu_ctx_factory, mode):
+    """Independent systems in one upload: every system's per-particle walk sees only its
+    own tree (several systems can share a wave of targets)."""
+    import oracle
+    import pynbg
+    off, st, _ = pynbg.ensemble(10, 2, 100, 3000)
+    n = int(off[-1])
+    eps = 0.01
+    ctx = gpu_ctx_factory(n)
+    ctx.upload_batch(off, st["x"], st["y"], st["z"], st["m"])
+    kw = dict(per_particle=True) if mode == "per_particle" else dict(mac="relative", alpha=0.003)
+    if mode == "relative":
+        ctx.accel(pynbg.make_params("direct", G=1.0, eps=eps))
+        a_prev = ctx.accelerations()
+    ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=eps, theta=0.6, **kw))
+    a = ctx.accelerations().astype(np.float64)
+    worst = 0.0
+    for k in range(len(off) - 1):
+        sl = slice(off[k], off[k + 1])
+        x, y, z, m = (st[q][sl].astype(np.float64) for q in ("x", "y", "z", "m"))
+        T = oracle.Tree(x, y, z, m, quad=True)
+        if mode == "relative":
+            ref = T.accel_bh_per_particle(1.0, eps * eps, 0.6, relative=True, alpha=0.003,
+                                          a_old=_amag64(a_prev[sl]))
+        else:
+            ref = T.accel_bh_per_particle(1.0, eps * eps, 0.6)
+        worst = max(worst, _per_particle_max(a[sl], ref))
+    print(f"batch of {len(off) - 1} systems ({mode}): worst per-particle error {worst:.2e}")
+    assert worst < PER_PARTICLE_TOL
+
+
+def test_per_particle_walk_with_deep_keys(gpu_ctx_factory):
+    """A cluster needing the 40-level key path (coincident points, a tight clump) under the
+    per-particle walk and block steps."""
+    import oracle
+    import pynbg
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1, 1, 3000)
+    y = rng.uniform(-1, 1, 3000)
+    z = rng.uniform(-1, 1, 3000)
+    x[:40] = 0.25 + 1e-9 * rng.standard_normal(40)
+    y[:40] = -0.5
+    z[:40] = 0.125
+    m = np.full(3000, 1.0 / 3000)
+    f = {k: v.astype(np.float32) for k, v in dict(x=x, y=y, z=z, m=m).items()}
+    ctx = gpu_ctx_factory(3000)
+    ctx.upload(f["x"], f["y"], f["z"], f["m"])
+    st = ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=0.02, theta=0.5, per_particle=True))
+    assert st.deep_keys == 1
+    a = ctx.accelerations()
+    T = oracle.Tree(*(f[k].astype(np.float64) for k in ("x", "y", "z", "m")), quad=True)
+    ref = T.accel_bh_per_particle(1.0, 0.02 ** 2, 0.5)
+    assert _per_particle_max(a, ref) < PER_PARTICLE_TOL
+    tg = np.arange(0, 3000, 7)
+    at = ctx.bh_targets(pynbg.make_params("barnes_hut", G=1.0, eps=0.02, theta=0.5), tg)
+    assert _per_particle_max(at, ref[tg]) < PER_PARTICLE_TOL
+
+
+def test_mode_argument_errors(gpu_ctx_factory):
+    import pynbg
+    s = plummer_f32(256, 0)
+    ctx = gpu_ctx_factory(256)
+    ctx.upload(s["x"], s["y"], s["z"], s["m"])
+    ctx.learn_set_weights(np.zeros(pynbg.LEARN_NPARAM, np.float32))
+    bad = [
+        (dict(softening="spline", eps=0.0), "spline"),
+        (dict(mac="relative", alpha=-1.0), "alpha"),
+        (dict(adaptive=True, eta=0.0), "eta"),
+        (dict(block=True, block_max_rung=30), "block_max_rung"),
+        (dict(learned=True, softening="spline"), "learned"),
+        (dict(learned=True, mac="relative"), "learned"),
+        (dict(learned=True, block=True), "learned"),
+    ]
+    for kw, word in bad:
+        base = dict(G=1.0, eps=0.01, theta=0.5)
+        base.update(kw)
+        with pytest.raises(pynbg.NbgError, match=word):
+            ctx.accel(pynbg.make_params("barnes_hut", **base))
+    # a target index out of range is refused before any device work
+    with pytest.raises(pynbg.NbgError, match="target"):
+        ctx.bh_targets(pynbg.make_params("barnes_hut", G=1.0, eps=0.01), [0, 256])
+    # the context still works
+    ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=0.01, theta=0.5, softening="spline"))
+    assert np.all(np.isfinite(ctx.accelerations()))
