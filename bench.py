@@ -8,7 +8,8 @@ step: tree rebuilt from scratch (keys, sort, build, upsweep), group walk, two ki
 drift, replayed from a captured hipGraph.  Two variants are timed on the same state:
   plain    Barnes-Hut theta=0.5
   learned  Barnes-Hut theta=0.8 plus the learned far-field correction on the accepted
-           (leaf, cell) pairs with size/(distance - group radius) > 0.62
+           (leaf, cell) pairs whose correction scale G M s^3 / r^5 is at least 1e-3
+           (about 0.2% of a typical |a| of this sphere)
 Each variant's force error is its relative RMS against the direct sum on 16,384 random
 targets of the initial state.
 
@@ -50,9 +51,9 @@ def kernel_traffic(n, kernel):
     return d["bytes_per_launch"].get(kernel) if d["n"] == n else None
 
 
-def walk_kernel_name(quad, kahan, learn, near_tile):
+def walk_kernel_name(quad, kahan, learn, near_tile, spline=False):
     b = lambda v: "true" if v else "false"
-    return f"void k_walk_bins<{b(quad)}, {b(kahan)}, false, {b(learn)}, {near_tile}>"
+    return f"void k_walk_bins<{b(quad)}, {b(kahan)}, false, {b(learn)}, {near_tile}, {b(spline)}>"
 
 
 def log(*a):
@@ -319,7 +320,9 @@ def main():
                          "metric); plain: plain Barnes-Hut at --theta only")
     ap.add_argument("--theta", type=float, default=0.5, help="plain Barnes-Hut opening angle")
     ap.add_argument("--learned-theta", type=float, default=0.8)
-    ap.add_argument("--learn-band", type=float, default=0.62)
+    ap.add_argument("--learn-band", type=float, default=0.0)
+    ap.add_argument("--learn-amin", type=float, default=1e-3,
+                    help="learned pairs also need a correction scale G M s^3 / r^5 >= this")
     ap.add_argument("--eps", type=float, default=0.005)
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--multipole", default="quadrupole", choices=["monopole", "quadrupole"])
@@ -356,7 +359,7 @@ def main():
     kahan = not args.no_kahan
     plain = dict(multipole=args.multipole, G=1.0, eps=args.eps, theta=args.theta, dt=args.dt, kahan=kahan)
     lrn = dict(multipole="quadrupole", G=1.0, eps=args.eps, theta=args.learned_theta, dt=args.dt, kahan=kahan,
-               learned=True, learn_band=args.learn_band)
+               learned=True, learn_band=args.learn_band, learn_amin=args.learn_amin)
     rng = np.random.default_rng(12345)
     targets = np.sort(rng.choice(n, min(n, args.error_sample), replace=False))
     with pynbg.Context(n) as ctx:
@@ -436,11 +439,13 @@ def main():
             "config": {"workload": f"plummer_{n}_learned_theta{args.learned_theta}_vs_plain_theta{args.theta}",
                        "n": n, "eps": args.eps, "dt": args.dt, "multipole": "quadrupole", "kahan": kahan,
                        "variant": "learned", "theta": args.learned_theta, "learn_band": args.learn_band,
+                       "learn_amin": args.learn_amin,
                        "model": "13-64-64-64-3 SiLU, BF16 MFMA (opengl-physics_amd/learned/farfield_mlp.npy)"},
             "rel_rms_force": err_l,
             "comparison": {
                 "plain": plain_block,
-                "learned": {"theta": args.learned_theta, "learn_band": args.learn_band, "s_per_step": round(s_l, 6),
+                "learned": {"theta": args.learned_theta, "learn_band": args.learn_band, "learn_amin": args.learn_amin,
+                            "s_per_step": round(s_l, 6),
                             "rel_rms_force": err_l, "rel_rms_force_without_model": err_l_plain,
                             "stage_ms_per_step": stage_l, "learned_pairs_per_step": pairs,
                             "interactions_per_step": {"p2p": p2p_l, "m2p": m2p_l}},

@@ -84,6 +84,9 @@ typedef struct nbg_params {
     double eta;         /* StepParams::eta */
     int32_t block_max_rung; /* StepParams::blockMaxRung */
     int32_t reserved;   /* 0 */
+    double learn_amin;  /* NBG_FLAG_LEARNED: an accepted cell in the band also needs a correction
+                           scale G M s^3 / r^5 >= learn_amin (r: cell to leaf-group centre of
+                           mass) to be corrected (0 = no such limit).  No reference counterpart. */
 } nbg_params;
 
 /* Opt-in stage timings and tree statistics, like GpuBarnesHutStats

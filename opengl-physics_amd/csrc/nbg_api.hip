@@ -107,8 +107,8 @@ int validate(nbg_ctx* c, const nbg_params* p) {
             c->set_error("no learned model installed (nbg_learn_set_weights)");
             return NBG_E_STATE;
         }
-        if (!(p->learn_band >= 0.0)) {
-            c->set_error("learn_band must be >= 0");
+        if (!(p->learn_band >= 0.0) || !(p->learn_amin >= 0.0)) {
+            c->set_error("learn_band and learn_amin must be >= 0");
             return NBG_E_INVALID;
         }
         if (int r = learn_alloc(c)) return r;
@@ -331,7 +331,7 @@ bool same_params(const nbg_params& a, const nbg_params& b) {
     return a.solver == b.solver && a.multipole == b.multipole && a.G == b.G && a.eps2 == b.eps2 &&
            a.theta == b.theta && a.dt == b.dt && (a.flags & ~NBG_FLAG_COUNT) == (b.flags & ~NBG_FLAG_COUNT) &&
            a.learn_band == b.learn_band && a.softening == b.softening && a.mac == b.mac && a.alpha == b.alpha &&
-           a.adaptive == b.adaptive && a.eta == b.eta;
+           a.adaptive == b.adaptive && a.eta == b.eta && a.learn_amin == b.learn_amin;
 }
 
 // Eager steps with a scalar read-back per step to catch the deep-key case.

@@ -364,8 +364,10 @@ struct BinArgs {
     Soft soft;
     float inv_theta;
     double theta2_64;
-    // learned correction: accepted pairs with size / (dist - R) > band are listed
+    // learned correction: accepted pairs with size / (dist - R) > band and a correction
+    // scale G M s^3 / r^5 >= amin are listed
     float inv_band;
+    float amin;
     int2* lpairs;
     uint32_t lcap;
 };
@@ -690,6 +692,25 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                                 open_m |= bit;
                             }
                         }
+                        if (LEARN && band_m && A.amin > 0.f) {
+                            // only pairs whose correction scale G M s^3 / r^5 (r = |com_cell -
+                            // com_leaf|) reaches amin: by the sphere's bounds on r when they
+                            // settle it, else leaf by leaf
+                            const float gms3 = cm.w * au.y * sqrtf(au.y);
+                            const float rlo = fmaxf(db - Rb, 0.f), rhi = db + Rb;
+                            const float rlo2 = rlo * rlo, rhi2 = rhi * rhi;
+                            if (gms3 < A.amin * (rlo2 * rlo2 * rlo)) {
+                                band_m = 0;
+                            } else if (!(gms3 >= A.amin * (rhi2 * rhi2 * rhi))) {
+                                for (uint64_t t = band_m; t; t &= t - 1) {
+                                    const int jj = __ffsll(static_cast<long long>(t)) - 1;
+                                    const float4 lc = lcom[jj];
+                                    const float dx = cm.x - lc.x, dy = cm.y - lc.y, dz = cm.z - lc.z;
+                                    const float r2 = dx * dx + dy * dy + dz * dz;
+                                    if (gms3 < A.amin * (r2 * r2 * sqrtf(r2))) band_m &= ~(1ull << jj);
+                                }
+                            }
+                        }
                     }
                 }
             }
@@ -920,6 +941,7 @@ void launch_walk_bins(nbg_ctx* c, const nbg_params& p) {
     const bool count = (p.flags & NBG_FLAG_COUNT) != 0 && c->cnt_far != nullptr;
     const bool learn = quad && (p.flags & NBG_FLAG_LEARNED) != 0 && c->lpairs != nullptr;
     A.inv_band = (p.learn_band > 0.0) ? static_cast<float>(1.0 / p.learn_band) : 1e30f;
+    A.amin = static_cast<float>(p.learn_amin);
     A.lpairs = c->lpairs;
     A.lcap = static_cast<uint32_t>(c->lpairs_cap);
     if (p.softening == NBG_SOFT_SPLINE)

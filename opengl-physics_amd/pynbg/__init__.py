@@ -50,7 +50,7 @@ class Params(ctypes.Structure):
                 ("flags", ctypes.c_uint32), ("learn_band", ctypes.c_double), ("softening", ctypes.c_int32),
                 ("mac", ctypes.c_int32), ("alpha", ctypes.c_double), ("adaptive", ctypes.c_int32),
                 ("block", ctypes.c_int32), ("eta", ctypes.c_double), ("block_max_rung", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("reserved", ctypes.c_int32), ("learn_amin", ctypes.c_double)]
 
 
 class Tuning(ctypes.Structure):
@@ -143,7 +143,7 @@ class NbgError(RuntimeError):
 def make_params(solver="barnes_hut", multipole="quadrupole", G=1.0, eps=None, eps2=None, theta=0.5, dt=1e-3,
                 kahan=True, count=False, no_graph=False, learned=False, stage_times=False, learn_band=0.0,
                 softening="plummer", mac="geometric", alpha=0.001, per_particle=False, adaptive=False,
-                block=False, eta=0.03, block_max_rung=8):
+                block=False, eta=0.03, block_max_rung=8, learn_amin=0.0):
     """nbg_params with the reference's StepParams defaults (Solvers.hpp:22-40): eps is the
     Plummer softening length or the spline length h (eps2 = eps^2 either way)."""
     if eps2 is None:
@@ -157,7 +157,7 @@ def make_params(solver="barnes_hut", multipole="quadrupole", G=1.0, eps=None, ep
                   QUADRUPOLE if multipole == "quadrupole" else MONOPOLE, float(G), float(eps2), float(theta),
                   float(dt), flags, float(learn_band), SOFT_SPLINE if softening == "spline" else SOFT_PLUMMER,
                   MAC_RELATIVE if mac == "relative" else MAC_GEOMETRIC, float(alpha), 1 if adaptive else 0,
-                  1 if block else 0, float(eta), int(block_max_rung), 0)
+                  1 if block else 0, float(eta), int(block_max_rung), 0, float(learn_amin))
 
 
 class Context:

@@ -98,6 +98,37 @@ def test_band_selects_the_marginal_pairs(gpu_ctx_factory):
     assert len(got) == n and 0.1 < n / len(fi) < 0.5
 
 
+def test_amin_selects_by_correction_scale(gpu_ctx_factory):
+    """learn_amin: a listed pair also needs G M s^3 / r^5 >= learn_amin, r the distance of
+    the cell's and the leaf group's centres of mass (the output scale of the model)."""
+    import oracle
+    import pynbg
+    s = plummer_f32(4096, 2)
+    x, y, z, m = f64(s)
+    ctx = gpu_ctx_factory(4096)
+    ctx.upload(s["x"], s["y"], s["z"], s["m"])
+    T = oracle.Tree(x, y, z, m, quad=True)
+    e = T.export()
+    fp, fi, _, _ = T.group_lists(0.8)
+    leaves = np.nonzero((e["child_count"] == 0) & (e["count"] > 0))[0]
+    ol = np.repeat(leaves, np.diff(fp))
+    r = np.linalg.norm(e["com"][fi] - e["com"][ol], axis=1)
+    scale = e["mass"][fi] * (2.0 * e["half"][fi]) ** 3 / r ** 5
+    for amin in (1e-4, 1e-3):
+        ctx.learn_clear()
+        n = ctx.learn_collect(pynbg.make_params("barnes_hut", G=1.0, eps=0.005, theta=0.8, learn_band=0.0,
+                                                learn_amin=amin))
+        feats, _ = ctx.learn_pool(0, n)
+        gl, gc = _ids(feats)
+        sel = scale >= amin
+        want = set(zip(ol[sel].tolist(), fi[sel].tolist()))
+        got = set(zip(gl.tolist(), gc.tolist()))
+        edge = np.abs(scale / amin - 1.0) < 1e-5
+        print(f"amin {amin}: listed {n} of {len(fi)}, expected {int(sel.sum())}, edge {int(edge.sum())}")
+        assert len(want ^ got) <= int(edge.sum())
+        assert 0 < n < len(fi)
+
+
 def test_samples_match_oracle(gpu_ctx_factory):
     import learned as LO
     import oracle
