@@ -23,6 +23,8 @@
 // target per lane).  Decisions within FP32 rounding of the boundary are re-made in FP64
 // exactly as the reference writes them, so every target's interaction list is the
 // reference's.
+#include <type_traits>
+
 #include "nbg_common.hpp"
 #include "nbg_internal.hpp"
 #include "nbg_pair.hpp"
@@ -194,7 +196,6 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
             tpos = cld(A.bin_start, b) + ((lane < cnt) ? lane : 0);
         }
         const bool tv = lane < cnt;
-        const uint64_t tmask = ballot(tv);
         const float4 tp = A.spos[tpos];
         const uint32_t orig = A.perm[tpos];
         const float tscale = fabsf(tp.x) + fabsf(tp.y) + fabsf(tp.z);
@@ -251,14 +252,14 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
             float sx = 0.f, sy = 0.f, sz = 0.f;
             for (int e0 = 0; e0 < n; e0 += 4) {
                 int nd[4];
-                float um[4];
+                bool um[4];
                 float4 cm[4], q[4];
                 float qyz[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int slot = (h + min(e0 + u, n - 1)) & (kPPRing - 1);
                     nd[u] = __builtin_amdgcn_readfirstlane(fnode[slot]);
-                    um[u] = (e0 + u < n && (fmask[slot] & mybit)) ? 1.f : 0.f;
+                    um[u] = e0 + u < n && (fmask[slot] & mybit);
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -268,12 +269,8 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    float ex, ey, ez;
-                    m2p_cell<QUAD, SPL>(cm[u], q[u], qyz[u], tp.x, tp.y, tp.z, soft, ex, ey, ez);
-                    sx += (um[u] > 0.f) ? ex : 0.f;
-                    sy += (um[u] > 0.f) ? ey : 0.f;
-                    sz += (um[u] > 0.f) ? ez : 0.f;
-                    if (COUNT) n_m2p += um[u] > 0.f;
+                    m2p_add<QUAD, SPL>(cm[u], q[u], qyz[u], tp.x, tp.y, tp.z, soft, um[u], sx, sy, sz);
+                    if (COUNT) n_m2p += um[u];
                 }
             }
             kadd<KAHAN>(ax, cx, sx);
@@ -282,15 +279,22 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
         };
         // evaluates the particles of `n` staged leaves from ring position `h`, eight per
         // batch across leaf boundaries; the last batch's unused slots read the zero pad
-        auto eval_near = [&](int h, int n) {
+        // FAST: every pair term is finite (Plummer with eps > 0, or the spline), so a lane
+        // weighs a source by its use (1 or 0) instead of selecting, and its own particle
+        // adds an exact zero (d = 0); otherwise unused sources and the target itself are
+        // dropped by a select.
+        auto eval_near_t = [&](int h, int n, auto fast) {
+            constexpr bool FAST = decltype(fast)::value;
             float sx = 0.f, sy = 0.f, sz = 0.f;
             int e = 0, k = 0, st = 0, c = 0;
             uint64_t em = 0;
+            float wuse = 0.f;
             auto load_entry = [&](int ee) {
                 const int slot = (h + ee) & (kPPRing - 1);
                 st = __builtin_amdgcn_readfirstlane(qstart[slot]);
                 c = __builtin_amdgcn_readfirstlane(qcnt[slot]);
                 em = qmask[slot];
+                wuse = (em & mybit) ? 1.f : 0.f;
             };
             if (n > 0) load_entry(0);
             int groups = 0;
@@ -301,7 +305,7 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
                 for (int u = 0; u < 8; ++u) {
                     if (e < n) {
                         idx[u] = st + k;
-                        us[u] = (em & mybit) ? 1.f : 0.f;
+                        us[u] = wuse;
                         if (COUNT) n_p2p += (em & mybit) && (st + k != tpos);
                         if (++k == c) {
                             k = 0;
@@ -319,8 +323,13 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
                 for (int u = 0; u < 8; ++u) {
                     const float4 q = spt[u];
                     const float dx = q.x - tp.x, dy = q.y - tp.y, dz = q.z - tp.z;
-                    float f = pair_coef<SPL>(dx * dx + dy * dy + dz * dz, q.w, soft);
-                    f = (us[u] > 0.f && idx[u] != tpos) ? f : 0.f;
+                    float f;
+                    if (FAST) {
+                        f = pair_coef<SPL>(dx * dx + dy * dy + dz * dz, q.w * us[u], soft);
+                    } else {
+                        f = pair_coef<SPL>(dx * dx + dy * dy + dz * dz, q.w, soft);
+                        f = (us[u] > 0.f && idx[u] != tpos) ? f : 0.f;
+                    }
                     sx += f * dx;
                     sy += f * dy;
                     sz += f * dz;
@@ -337,6 +346,13 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
             kadd<KAHAN>(ax, cx, sx);
             kadd<KAHAN>(ay, cy, sy);
             kadd<KAHAN>(az, cz, sz);
+        };
+        const bool near_fast = SPL || soft.eps2 > 0.f;
+        auto eval_near = [&](int h, int n) {
+            if (near_fast)
+                eval_near_t(h, n, std::true_type{});
+            else
+                eval_near_t(h, n, std::false_type{});
         };
 
         while (sp > 0) {
