@@ -627,18 +627,27 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             wave_sync();
             uint64_t far_m = 0, near_m = 0, open_m = 0, band_m = 0;
             int lstart = 0, lcount = 0;
+            float4 cm = make_float4(0.f, 0.f, 0.f, 0.f), au = cm;
+            uint32_t info = 0;
             if (node >= 0) {
-                const float4 cm = A.nd_com[node];
-                const float4 au = A.nd_aux[node];
-                const uint32_t info = __float_as_uint(au.w);
+                cm = A.nd_com[node];
+                au = A.nd_aux[node];
+                info = __float_as_uint(au.w);
                 if (info & kLeafBit) {
-                    // a bin leaf never takes itself as a near leaf
-                    uint64_t excl = 0;
-                    for (int jj = 0; jj < nl; ++jj) excl |= (lid[jj] == node) ? (1ull << jj) : 0ull;
-                    act &= ~excl;
                     lstart = A.nd_start[node];
                     lcount = static_cast<int>(info & ~kLeafBit);
                 }
+            }
+            {
+                // a bin leaf never takes itself as a near leaf: a leaf is one of the bin's
+                // iff its particles meet the bin's, and its bit is that of the target lane
+                // holding its first particle inside the bin
+                const bool own = (info & kLeafBit) && lstart < start + cnt && lstart + lcount > start;
+                const int src = own ? max(lstart, start) - start : 0;
+                const int jsrc = __shfl(j, src, 64);
+                if (own) act &= ~(1ull << jsrc);
+            }
+            if (node >= 0) {
                 if (cm.w > 0.f && act) {
                     if (info & kLeafBit) {
                         near_m = act;
