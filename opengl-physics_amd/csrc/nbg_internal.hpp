@@ -163,6 +163,7 @@ struct nbg_ctx {
 
     // sorted particles: x,y,z, G*m
     float4* spos = nullptr;
+    float4* pos4 = nullptr;          // x,y,z, G*m in input order (read whole by the sorted gather)
 
     // scalars
     nbg::DevScalars* dsc = nullptr;

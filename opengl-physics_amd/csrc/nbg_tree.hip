@@ -727,14 +727,20 @@ __global__ __launch_bounds__(256) void k_leaf_order(const int32_t* __restrict__ 
     }
 }
 
-__global__ __launch_bounds__(256) void k_gather(const float* __restrict__ x, const float* __restrict__ y,
-                                                const float* __restrict__ z, const float* __restrict__ m,
-                                                const uint32_t* __restrict__ perm, int n, float G,
-                                                float4* __restrict__ spos) {
+// Sorted particles: the input-order records packed first (x,y,z, G*m; coalesced), so the
+// permuted read touches one 16-byte record per particle instead of four scattered words.
+__global__ __launch_bounds__(256) void k_pack4(const float* __restrict__ x, const float* __restrict__ y,
+                                               const float* __restrict__ z, const float* __restrict__ m, int n,
+                                               float G, float4* __restrict__ pos4) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) pos4[i] = make_float4(x[i], y[i], z[i], G * m[i]);
+}
+
+__global__ __launch_bounds__(256) void k_gather(const float4* __restrict__ pos4, const uint32_t* __restrict__ perm,
+                                                int n, float4* __restrict__ spos) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const uint32_t j = perm[k];
-    spos[k] = make_float4(x[j], y[j], z[j], G * m[j]);
+    spos[k] = pos4[perm[k]];
 }
 
 void launch_build(nbg_ctx* c, int levels) {
@@ -1006,8 +1012,10 @@ void launch_upsweep(nbg_ctx* c, int levels, int multipole, double G, bool moment
 
 // gather is part of the build step in the driver (needs G)
 void launch_gather(nbg_ctx* c, float G) {
-    hipLaunchKernelGGL(k_gather, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->x, c->y, c->z, c->m, c->perm,
-                       c->n, G, c->spos);
+    hipLaunchKernelGGL(k_pack4, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->x, c->y, c->z, c->m, c->n, G,
+                       c->pos4);
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->pos4, c->perm, c->n,
+                       c->spos);
 }
 
 }  // namespace nbg

@@ -652,11 +652,16 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                     if (info & kLeafBit) {
                         near_m = act;
                     } else {
-                        const float sth = sqrtf(au.y) * inv_theta;  // size / theta
+                        // the sphere shortcuts and the FP32 leaf test below have a slack of
+                        // 1e-4 relative, which covers the hardware square root (1 ulp)
+                        const float size = __builtin_amdgcn_sqrtf(au.y);
+                        const float sth = size * inv_theta;  // size / theta
                         const float sbd = LEARN ? sqrtf(au.y) * A.inv_band : 0.f;
                         const float cscale = fabsf(cm.x) + fabsf(cm.y) + fabsf(cm.z);
                         const float bx = cm.x - cbx, by = cm.y - cby, bz = cm.z - cbz;
-                        const float db = sqrtf(bx * bx + by * by + bz * bz);
+                        const float db2 = bx * bx + by * by + bz * bz;
+                        // (the learned band's sphere tests keep the correctly rounded root)
+                        const float db = LEARN ? sqrtf(db2) : __builtin_amdgcn_sqrtf(db2);
                         const float slack_b = 1e-4f * (db + Rb + sth) + 4e-6f * (cscale + cbscale);
                         uint64_t todo_all = act;
                         if (db - Rb - sth > slack_b) {
@@ -682,10 +687,17 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                             open_m = act;  // opened by every group
                             todo_all = 0;
                         }
-                        for (uint64_t todo = todo_all; todo; todo &= todo - 1) {
-                            const int jj = __ffsll(static_cast<long long>(todo)) - 1;
+                        // leaf by leaf, the next leaf's sphere read ahead of the current test
+                        uint64_t todo = todo_all;
+                        int jn = todo ? __ffsll(static_cast<long long>(todo)) - 1 : 0;
+                        float4 lcn = lcom[jn];
+                        while (todo) {
+                            const int jj = jn;
+                            const float4 lc = lcn;
+                            todo &= todo - 1;
+                            jn = todo ? __ffsll(static_cast<long long>(todo)) - 1 : jj;
+                            lcn = lcom[jn];
                             const uint64_t bit = 1ull << jj;
-                            const float4 lc = lcom[jj];
                             const float dx = cm.x - lc.x, dy = cm.y - lc.y, dz = cm.z - lc.z;
                             const float dist2 = dx * dx + dy * dy + dz * dz;
                             // worst > 0 && size^2 < theta^2 worst^2  <=>  dist > R + size/theta
@@ -722,7 +734,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                             // only pairs whose correction scale G M s^3 / r^5 (r = |com_cell -
                             // com_leaf|) reaches amin: by the sphere's bounds on r when they
                             // settle it, else leaf by leaf
-                            const float gms3 = cm.w * au.y * sqrtf(au.y);
+                            const float gms3 = cm.w * au.y * size;
                             const float rlo = fmaxf(db - Rb, 0.f), rhi = db + Rb;
                             const float rlo2 = rlo * rlo, rhi2 = rhi * rhi;
                             if (gms3 < A.amin * (rlo2 * rlo2 * rlo)) {
@@ -733,7 +745,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                                     const float4 lc = lcom[jj];
                                     const float dx = cm.x - lc.x, dy = cm.y - lc.y, dz = cm.z - lc.z;
                                     const float r2 = dx * dx + dy * dy + dz * dz;
-                                    if (gms3 < A.amin * (r2 * r2 * sqrtf(r2))) band_m &= ~(1ull << jj);
+                                    if (gms3 < A.amin * (r2 * r2 * __builtin_amdgcn_sqrtf(r2))) band_m &= ~(1ull << jj);
                                 }
                             }
                         }
