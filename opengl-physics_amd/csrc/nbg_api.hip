@@ -150,6 +150,10 @@ void enqueue_bh(nbg_ctx* c, const nbg_params& p, int levels, bool timed, bool li
         launch_learn_zero(c);
         launch_learn_correct(c, p);
         if (timed) rec(c, 13);
+    } else if (learn) {
+        // the listing walk's own accelerations, uncorrected, to input order
+        launch_learn_zero(c);
+        launch_learn_apply(c);
     }
 }
 
@@ -498,7 +502,7 @@ nbg_ctx* nbg_create(int device, int64_t max_n) {
     r |= alloc_into(c, c->leaf_list, N);
     r |= alloc_into(c, c->leaf_of, N);
     r |= alloc_into(c, c->spos, N + kSposPad);
-    r |= alloc_into(c, c->pos4, N);
+    r |= alloc_into(c, c->stage4, N);
     r |= alloc_into(c, c->nd_start, C);
     r |= alloc_into(c, c->nd_count, C);
     r |= alloc_into(c, c->nd_depth, C);
@@ -555,7 +559,7 @@ void nbg_destroy(nbg_ctx* c) {
     free_dev(c->hist); free_dev(c->scan_part);
     free_dev(c->leaf_depth); free_dev(c->cpl_prev); free_dev(c->node_scan); free_dev(c->blk_depth); free_dev(c->leaf_list); free_dev(c->leaf_of);
     free_dev(c->spos);
-    free_dev(c->pos4);
+    free_dev(c->stage4);
     free_dev(c->nd_start); free_dev(c->nd_count); free_dev(c->nd_depth); free_dev(c->nd_parent);
     free_dev(c->nd_oct); free_dev(c->nd_child); free_dev(c->nd_geom);
     free_dev(c->nd_com); free_dev(c->nd_quad); free_dev(c->nd_aux); free_dev(c->nd_com64); free_dev(c->nd_rmax64);

@@ -163,7 +163,8 @@ struct nbg_ctx {
 
     // sorted particles: x,y,z, G*m
     float4* spos = nullptr;
-    float4* pos4 = nullptr;          // x,y,z, G*m in input order (read whole by the sorted gather)
+    float4* stage4 = nullptr;        // N records: x,y,z, G*m in input order for the sorted gather,
+                                     // then the learned walk's accelerations in sorted order
 
     // scalars
     nbg::DevScalars* dsc = nullptr;
@@ -255,6 +256,7 @@ int learn_alloc(nbg_ctx* c);
 int learn_grow_pairs(nbg_ctx* c, int64_t need);
 void launch_learn_zero(nbg_ctx* c);
 void launch_learn_correct(nbg_ctx* c, const nbg_params& p);
+void launch_learn_apply(nbg_ctx* c);  // sorted walk accelerations + leaf correction -> input order
 void launch_learn_samples(nbg_ctx* c, const nbg_params& p, int64_t npairs, int64_t nnew, int64_t dst);
 void launch_learn_amag2(nbg_ctx* c);
 void launch_learn_stats(nbg_ctx* c, int64_t count, double* out /* device, 16+16+1 */);

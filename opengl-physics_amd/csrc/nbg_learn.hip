@@ -580,15 +580,18 @@ __global__ void k_learn_zero(float4* __restrict__ corr, const DevScalars* __rest
         corr[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// The walk's accelerations (by sorted position) plus the correction of the particle's
+// leaf, written once to input order.
 __global__ void k_learn_apply(const float4* __restrict__ corr, const int32_t* __restrict__ leaf_of,
-                              const uint32_t* __restrict__ perm, int n, float* __restrict__ ax,
-                              float* __restrict__ ay, float* __restrict__ az) {
+                              const uint32_t* __restrict__ perm, const float4* __restrict__ acc_sorted, int n,
+                              float* __restrict__ ax, float* __restrict__ ay, float* __restrict__ az) {
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const float4 a = acc_sorted[p];
         const float4 cr = corr[leaf_of[p]];
         const uint32_t o = perm[p];
-        ax[o] += cr.x;
-        ay[o] += cr.y;
-        az[o] += cr.z;
+        ax[o] = a.x + cr.x;
+        ay[o] = a.y + cr.y;
+        az[o] = a.z + cr.z;
     }
 }
 
@@ -1098,8 +1101,12 @@ void launch_learn_correct(nbg_ctx* c, const nbg_params& p) {
     A.w32 = c->lw32;
     A.corr = c->lcorr;
     hipLaunchKernelGGL(k_learn_infer, dim3(256), dim3(64 * kInferWaves), 0, c->stream, A);
+    launch_learn_apply(c);
+}
+
+void launch_learn_apply(nbg_ctx* c) {
     hipLaunchKernelGGL(k_learn_apply, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->lcorr, c->leaf_of,
-                       c->perm, c->n, c->ax, c->ay, c->az);
+                       c->perm, c->stage4, c->n, c->ax, c->ay, c->az);
 }
 
 void launch_learn_amag2(nbg_ctx* c) {

@@ -1013,8 +1013,8 @@ void launch_upsweep(nbg_ctx* c, int levels, int multipole, double G, bool moment
 // gather is part of the build step in the driver (needs G)
 void launch_gather(nbg_ctx* c, float G) {
     hipLaunchKernelGGL(k_pack4, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->x, c->y, c->z, c->m, c->n, G,
-                       c->pos4);
-    hipLaunchKernelGGL(k_gather, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->pos4, c->perm, c->n,
+                       c->stage4);
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->stage4, c->perm, c->n,
                        c->spos);
 }
 

@@ -361,6 +361,7 @@ struct BinArgs {
     float* ax;
     float* ay;
     float* az;
+    float4* acc_sorted;  // learned walk: accelerations by sorted position (the correction adds in)
     int32_t* cnt_far;
     int32_t* cnt_near;
     Soft soft;
@@ -845,10 +846,16 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
         eval_far(f_head, f_tail - f_head);
         eval_near(q_head, q_tail - q_head);
         if (tv) {
-            const uint32_t o = A.perm[tpos];
-            A.ax[o] = KAHAN ? ax - cx : ax;
-            A.ay[o] = KAHAN ? ay - cy : ay;
-            A.az[o] = KAHAN ? az - cz : az;
+            const float rx = KAHAN ? ax - cx : ax, ry = KAHAN ? ay - cy : ay, rz = KAHAN ? az - cz : az;
+            if (LEARN) {
+                // coalesced; the correction's apply pass scatters to input order once
+                A.acc_sorted[tpos] = make_float4(rx, ry, rz, 0.f);
+            } else {
+                const uint32_t o = A.perm[tpos];
+                A.ax[o] = rx;
+                A.ay[o] = ry;
+                A.az[o] = rz;
+            }
             if (COUNT) {
                 A.cnt_far[mylf] = n_far;
                 A.cnt_near[mylf] = n_near;
@@ -969,6 +976,7 @@ void launch_walk_bins(nbg_ctx* c, const nbg_params& p) {
     A.ax = c->ax;
     A.ay = c->ay;
     A.az = c->az;
+    A.acc_sorted = c->stage4;
     A.cnt_far = c->cnt_far;
     A.cnt_near = c->cnt_near;
     A.soft = make_soft(p);
