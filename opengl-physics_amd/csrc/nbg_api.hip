@@ -566,7 +566,7 @@ void nbg_destroy(nbg_ctx* c) {
     free_dev(c->depth_nodes); free_dev(c->cnt_far); free_dev(c->cnt_near);
     free_dev(c->dsc); free_dev(c->diag_part);
     free_dev(c->aold); free_dev(c->rung); free_dev(c->tlist); free_dev(c->list_off); free_dev(c->red_part);
-    free_dev(c->nd_mom); free_dev(c->lcorr); free_dev(c->lpairs); free_dev(c->lfeat); free_dev(c->lscale); free_dev(c->lw32); free_dev(c->lw16);
+    free_dev(c->nd_mom); free_dev(c->lcorr); free_dev(c->lpairs); free_dev(c->lw32); free_dev(c->lw16);
     free_dev(c->adam_m); free_dev(c->adam_v); free_dev(c->pool_feat); free_dev(c->pool_tgt); free_dev(c->amag2);
     free_dev(c->grad_part); free_dev(c->loss_part); free_dev(c->loss_dev);
     free_dev(c->sys_off); free_dev(c->sys_root); free_dev(c->sys_geom); free_dev(c->sys_bbox);

@@ -197,8 +197,6 @@ struct nbg_ctx {
     float4* nd_mom = nullptr;       // 4 per node: second (6) and third (10) moments about the FP32 com, G*m weights
     float4* lcorr = nullptr;        // per node: summed correction of a leaf group
     int2* lpairs = nullptr;         // (leaf, cell) pairs listed by the walk
-    uint4* lfeat = nullptr;         // 2 per pair: standardised BF16 features of the pair
-    float* lscale = nullptr;        // per pair: output scale G M s^3 / r^5
     int64_t lpairs_cap = 0;
     float* lw32 = nullptr;          // FP32 model blob (NBG_LEARN_NPARAM)
     uint16_t* lw16 = nullptr;       // BF16 copy of the trainable part (blob order)

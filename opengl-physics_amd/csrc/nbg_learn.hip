@@ -431,55 +431,18 @@ __device__ __forceinline__ void mlp_tile2(const Frags& F, bf16x4 xa, bf16x4 xb, 
 }
 
 // ---------------------------------------------------------------------------- inference
-// Gather: one thread per listed pair writes its standardised BF16 feature row (16 x 2 B;
-// entries 13..15 zero) and its output scale, so the matrix-core kernel streams contiguous
-// rows instead of chasing pair -> node loads.
-struct GatherArgs {
+// One persistent kernel per evaluation: a lane computes the standardised BF16 feature row
+// of one listed pair (16 x 2 B, entries 13..15 zero) and its output scale from the node
+// data, the wave stages its 64 rows in LDS and reads them back as matrix-core fragments.
+struct InferArgs {
     const int2* pairs;
     const DevScalars* sc;
+    DevScalars* scw;
     uint32_t cap;
     const float4* nd_com;
     const float4* nd_aux;
     const float4* nd_mom;
-    const float* w32;  // full blob (standardisation first)
     float eps;
-    uint4* feat;       // 2 per pair
-    float* scale;
-    DevScalars* scw;
-};
-
-__global__ __launch_bounds__(256) void k_learn_gather(GatherArgs A) {
-    const uint32_t listed = A.sc->learn_pairs;
-    const uint32_t np = min(listed, A.cap);
-    if (blockIdx.x == 0 && threadIdx.x == 0 && listed > A.cap) atomicOr(&A.scw->sticky_learn, 1);
-    const float* mu = A.w32;
-    const float* isd = A.w32 + 16;
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < np; q += gridDim.x * blockDim.x) {
-        const int2 pr = A.pairs[q];
-        const float4 lc = A.nd_com[pr.x];
-        const float4 tgt = make_float4(lc.x, lc.y, lc.z, A.nd_aux[pr.x].z);
-        const float4 cm = A.nd_com[pr.y];
-        const float size2 = A.nd_aux[pr.y].y;
-        float f[kF], scale;
-        pair_features(tgt, cm, size2, A.nd_mom + 4 * pr.y, A.eps, f, scale);
-        float xv[16];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) xv[k] = (k < kF) ? (f[k] - mu[k]) * isd[k] : 0.f;
-        uint32_t w[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) w[k] = bf16x2_rne(xv[2 * k], xv[2 * k + 1]);
-        A.feat[2 * static_cast<size_t>(q)] = make_uint4(w[0], w[1], w[2], w[3]);
-        A.feat[2 * static_cast<size_t>(q) + 1] = make_uint4(w[4], w[5], w[6], w[7]);
-        A.scale[q] = scale;
-    }
-}
-
-struct InferArgs {
-    const int2* pairs;
-    const DevScalars* sc;
-    uint32_t cap;
-    const uint2* feat;   // 4 per pair (the gather's rows as 8-byte fragments)
-    const float* scale;
     const uint16_t* w16;
     const float* w32;    // full blob (standardisation first)
     float4* corr;
@@ -487,46 +450,85 @@ struct InferArgs {
 
 constexpr int kInferWaves = 8;
 
+// The node words one pair's features are computed from.
+struct PairRaw {
+    float4 lc, cm, m0, m1, m2, m3;
+    float lr, size2;
+};
+
 // Persistent: one 8-wave block per CU (the 128 KB activation table fills most of the
-// LDS).  A wave takes 64 pairs per step as four 16-pair tiles, two at a time, and loads
-// the next step's fragments, group ids and scales before computing the current one.
+// LDS).  A wave takes 64 pairs per step as four 16-pair tiles, two at a time.  The pair
+// ids run two steps ahead and the node words one step ahead of the step computed.
 __global__ __launch_bounds__(64 * kInferWaves) void k_learn_infer(InferArgs A) {
     __shared__ __attribute__((aligned(16))) uint16_t s_act[65536];
+    __shared__ uint2 s_row[kInferWaves][64][4];
     for (int i = threadIdx.x; i < 65536; i += blockDim.x) s_act[i] = act_bits(static_cast<uint16_t>(i));
     __syncthreads();
     const ActTable act{s_act};
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = lane_id();
     const int g = lane >> 4, c = lane & 15;
-    const uint32_t np = min(A.sc->learn_pairs, A.cap);
+    const uint32_t listed = A.sc->learn_pairs;
+    const uint32_t np = min(listed, A.cap);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && listed > A.cap) atomicOr(&A.scw->sticky_learn, 1);
     const int64_t nchunks = (static_cast<int64_t>(np) + 63) / 64;
     const int64_t wid = static_cast<int64_t>(blockIdx.x) * kInferWaves + wave;
     const int64_t nw = static_cast<int64_t>(gridDim.x) * kInferWaves;
     if (wid >= nchunks) return;
     Frags32 F;
     load_frags32(F, A.w16, A.w32 + kNorm, lane);
-    // fragment of tile t: pair 64 ch + 16 t + c, features 4g..4g+3
-    auto load_chunk = [&](int64_t ch, uint2 (&x)[4], int& leaf, float& sc) {
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int64_t q = ch * 64 + 16 * t + c;
-            x[t] = (q < np) ? A.feat[4 * q + g] : make_uint2(0u, 0u);
-        }
+    const float* mu = A.w32;
+    const float* isd = A.w32 + 16;
+    uint2(*row)[4] = s_row[wave];
+    auto load_pair = [&](int64_t ch) {
         const int64_t q = ch * 64 + lane;
-        leaf = (q < np) ? A.pairs[q].x : -1;
-        sc = (q < np) ? A.scale[q] : 0.f;
+        return (ch < nchunks && q < np) ? A.pairs[q] : make_int2(-1, -1);
     };
-    uint2 xn[4];
-    int leaf_n;
-    float sc_n;
-    load_chunk(wid, xn, leaf_n, sc_n);
+    auto load_raw = [&](int2 pr, PairRaw& r) {
+        if (pr.x >= 0) {
+            r.lc = A.nd_com[pr.x];
+            r.lr = A.nd_aux[pr.x].z;
+            r.cm = A.nd_com[pr.y];
+            r.size2 = A.nd_aux[pr.y].y;
+            r.m0 = A.nd_mom[4 * pr.y + 0];
+            r.m1 = A.nd_mom[4 * pr.y + 1];
+            r.m2 = A.nd_mom[4 * pr.y + 2];
+            r.m3 = A.nd_mom[4 * pr.y + 3];
+        }
+    };
+    int2 pr_n2 = load_pair(wid + nw);
+    int2 pr_n = load_pair(wid);
+    PairRaw raw_n{};
+    load_raw(pr_n, raw_n);
     for (int64_t ch = wid; ch < nchunks; ch += nw) {
+        const int leaf = pr_n.x;
+        const PairRaw raw = raw_n;
+        pr_n = pr_n2;
+        pr_n2 = load_pair(ch + 2 * nw);
+        load_raw(pr_n, raw_n);
+        // this lane's pair: features, standardised, as a BF16 row in LDS
+        float scale = 0.f;
+        {
+            const float4 mom[4] = {raw.m0, raw.m1, raw.m2, raw.m3};
+            float f[kF];
+            pair_features(make_float4(raw.lc.x, raw.lc.y, raw.lc.z, raw.lr), raw.cm, raw.size2, mom, A.eps, f,
+                          scale);
+            float xv[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) xv[k] = (k < kF) ? (f[k] - mu[k]) * isd[k] : 0.f;
+            uint32_t w[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) w[k] = leaf >= 0 ? bf16x2_rne(xv[2 * k], xv[2 * k + 1]) : 0u;
+            scale = leaf >= 0 ? scale : 0.f;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) row[lane][k] = make_uint2(w[2 * k], w[2 * k + 1]);
+        }
+        wave_sync();
+        // fragment of tile t: pair 16 t + c, features 4g..4g+3
         uint2 x[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) x[t] = xn[t];
-        const int leaf = leaf_n;
-        const float scale = sc_n;
-        if (ch + nw < nchunks) load_chunk(ch + nw, xn, leaf_n, sc_n);
+        for (int t = 0; t < 4; ++t) x[t] = row[16 * t + c][g];
+        wave_sync();
         float o0 = 0.f, o1 = 0.f, o2 = 0.f;
 #pragma unroll
         for (int t = 0; t < 4; t += 2) {
@@ -1063,8 +1065,6 @@ int learn_grow_pairs(nbg_ctx* c, int64_t need) {
     if (need <= c->lpairs_cap && c->lpairs) return NBG_OK;
     const int64_t cap = std::min<int64_t>(need + need / 4 + 1024, int64_t{0xffffffffll});
     int r = lalloc(c, c->lpairs, static_cast<size_t>(cap));
-    if (r == NBG_OK) r = lalloc(c, c->lfeat, 2 * static_cast<size_t>(cap));
-    if (r == NBG_OK) r = lalloc(c, c->lscale, static_cast<size_t>(cap));
     if (r != NBG_OK) {
         c->lpairs_cap = 0;
         return r;
@@ -1078,25 +1078,15 @@ void launch_learn_zero(nbg_ctx* c) {
 }
 
 void launch_learn_correct(nbg_ctx* c, const nbg_params& p) {
-    GatherArgs G;
-    G.pairs = c->lpairs;
-    G.sc = c->dsc;
-    G.cap = static_cast<uint32_t>(c->lpairs_cap);
-    G.nd_com = c->nd_com;
-    G.nd_aux = c->nd_aux;
-    G.nd_mom = c->nd_mom;
-    G.w32 = c->lw32;
-    G.eps = static_cast<float>(std::sqrt(p.eps2));
-    G.feat = c->lfeat;
-    G.scale = c->lscale;
-    G.scw = c->dsc;
-    hipLaunchKernelGGL(k_learn_gather, dim3(256 * 16), dim3(256), 0, c->stream, G);
     InferArgs A;
     A.pairs = c->lpairs;
     A.sc = c->dsc;
+    A.scw = c->dsc;
     A.cap = static_cast<uint32_t>(c->lpairs_cap);
-    A.feat = reinterpret_cast<const uint2*>(c->lfeat);
-    A.scale = c->lscale;
+    A.nd_com = c->nd_com;
+    A.nd_aux = c->nd_aux;
+    A.nd_mom = c->nd_mom;
+    A.eps = static_cast<float>(std::sqrt(p.eps2));
     A.w16 = c->lw16;
     A.w32 = c->lw32;
     A.corr = c->lcorr;
