@@ -505,7 +505,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
         // evaluates `n` staged far cells starting at ring position `h`, four at a time
         // (their scalar loads issued before any arithmetic); every lane computes every
         // cell and keeps it only if its leaf's bit is in the cell's mask
-        auto eval_far = [&](int h, int n) {
+        auto eval_far = [&](int h, int n) __attribute__((always_inline)) {
             float sx = 0.f, sy = 0.f, sz = 0.f;
             for (int e0 = 0; e0 < n; e0 += 4) {
                 int nd[4];
@@ -543,13 +543,13 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
         // weighs a source by its use (1 or 0) instead of selecting, and its own particle
         // adds an exact zero (d = 0); otherwise unused sources and the target itself are
         // dropped by a select.
-        auto eval_near_t = [&](int h, int n, auto fast) {
+        auto eval_near_t = [&](int h, int n, auto fast) __attribute__((always_inline)) {
             constexpr bool FAST = decltype(fast)::value;
             float sx = 0.f, sy = 0.f, sz = 0.f;
             int e = 0, k = 0, st = 0, c = 0;
             uint64_t em = 0;
             float wuse = 0.f;
-            auto load_entry = [&](int ee) {
+            auto load_entry = [&](int ee) __attribute__((always_inline)) {
                 const int slot = (h + ee) & (kBinRing - 1);
                 st = __builtin_amdgcn_readfirstlane(qstart[slot]);
                 c = __builtin_amdgcn_readfirstlane(qcnt[slot]);
@@ -608,7 +608,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             kadd<KAHAN>(az, cz, sz);
         };
         const bool near_fast = SPL || soft.eps2 > 0.f;
-        auto eval_near = [&](int h, int n) {
+        auto eval_near = [&](int h, int n) __attribute__((always_inline)) {
             if (near_fast)
                 eval_near_t(h, n, std::true_type{});
             else

@@ -248,7 +248,7 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
         uint32_t steps = 0;             // traversal steps of this bin (relative MAC cost record)
 
         // evaluates `n` staged far cells from ring position `h`, four scalar loads ahead
-        auto eval_far = [&](int h, int n) {
+        auto eval_far = [&](int h, int n) __attribute__((always_inline)) {
             float sx = 0.f, sy = 0.f, sz = 0.f;
             for (int e0 = 0; e0 < n; e0 += 4) {
                 int nd[4];
@@ -283,13 +283,13 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
         // weighs a source by its use (1 or 0) instead of selecting, and its own particle
         // adds an exact zero (d = 0); otherwise unused sources and the target itself are
         // dropped by a select.
-        auto eval_near_t = [&](int h, int n, auto fast) {
+        auto eval_near_t = [&](int h, int n, auto fast) __attribute__((always_inline)) {
             constexpr bool FAST = decltype(fast)::value;
             float sx = 0.f, sy = 0.f, sz = 0.f;
             int e = 0, k = 0, st = 0, c = 0;
             uint64_t em = 0;
             float wuse = 0.f;
-            auto load_entry = [&](int ee) {
+            auto load_entry = [&](int ee) __attribute__((always_inline)) {
                 const int slot = (h + ee) & (kPPRing - 1);
                 st = __builtin_amdgcn_readfirstlane(qstart[slot]);
                 c = __builtin_amdgcn_readfirstlane(qcnt[slot]);
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
             kadd<KAHAN>(az, cz, sz);
         };
         const bool near_fast = SPL || soft.eps2 > 0.f;
-        auto eval_near = [&](int h, int n) {
+        auto eval_near = [&](int h, int n) __attribute__((always_inline)) {
             if (near_fast)
                 eval_near_t(h, n, std::true_type{});
             else
