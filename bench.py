@@ -61,24 +61,36 @@ def log(*a):
 
 
 def cpu_baseline(state32, eps, theta, dt, multipole):
-    """One KDK step of the same workload on the FP64 oracle (reference flags build)."""
+    """One KDK step of the same workload on the FP64 oracle (reference flags build), with the
+    stages of its force evaluation timed apart on the priming evaluation: the serial
+    insertion tree, the quadrupoles (a monopole-only build subtracted) and the group walk."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     st = {k: np.asarray(v, np.float64) for k, v in state32.items()}
     quad = multipole == "quadrupole"
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     t0 = time.perf_counter()
-    T = oracle.Tree(st["x"], st["y"], st["z"], st["m"], quad=quad, fast=True)
-    a0 = T.accel_bh(1.0, eps * eps, theta)
+    T = oracle.Tree(st["x"], st["y"], st["z"], st["m"], quad=False, fast=True)
     del T
     t1 = time.perf_counter()
-    oracle.leapfrog(st, a0, 1.0, eps * eps, theta, dt, 1, quad=quad, fast=True)
+    T = oracle.Tree(st["x"], st["y"], st["z"], st["m"], quad=quad, fast=True)
     t2 = time.perf_counter()
-    log(f"cpu baseline: prime {t1 - t0:.2f} s, step {t2 - t1:.2f} s on {threads} threads")
-    return {"value": round(t2 - t1, 4), "unit": "s/step", "cores": threads, "kind": "port",
+    a0 = T.accel_bh(1.0, eps * eps, theta)
+    t3 = time.perf_counter()
+    del T
+    oracle.leapfrog(st, a0, 1.0, eps * eps, theta, dt, 1, quad=quad, fast=True)
+    t4 = time.perf_counter()
+    stages = {"tree_build_s": round(t1 - t0, 4), "quadrupoles_s": round(max(t2 - t1 - (t1 - t0), 0.0), 4),
+              "walk_s": round(t3 - t2, 4)}
+    stages["leapfrog_kicks_drift_s"] = round(max(t4 - t3 - (t2 - t1) - (t3 - t2), 0.0), 4)
+    log(f"cpu baseline: tree {t1 - t0:.2f} s, quadrupoles {stages['quadrupoles_s']:.2f} s, walk {t3 - t2:.2f} s; "
+        f"step {t4 - t3:.2f} s on {threads} threads")
+    return {"value": round(t4 - t3, 4), "unit": "s/step", "cores": threads, "kind": "port",
+            "stages_of_one_evaluation": stages,
             "sample": f"one KDK leapfrog step of the full workload (N={len(st['x'])}), FP64 oracle "
-                      f"(serial insertion tree + quadrupoles + OpenMP group walk), after one untimed force "
-                      f"evaluation; built -O3 -march=x86-64-v3 -funroll-loops -fopenmp"}
+                      f"(serial insertion tree + quadrupoles + OpenMP group walk), after one force "
+                      f"evaluation timed by stage (kicks and drift = the step minus that evaluation's "
+                      f"stages); built -O3 -march=x86-64-v3 -funroll-loops -fopenmp"}
 
 
 def run_variant(ctx, pynbg, ic, base, steps, warmup):
