@@ -1049,14 +1049,16 @@ int lalloc(nbg_ctx* c, T*& p, size_t count) {
 }
 }  // namespace
 
+// Allocates whatever of the learned buffers is missing (a failed earlier attempt leaves
+// some of them unset), so that a learned evaluation never runs without all of them.
 int learn_alloc(nbg_ctx* c) {
-    if (c->nd_mom) return NBG_OK;
     const size_t C = static_cast<size_t>(c->cap_nodes);
-    int r = NBG_OK;
-    r |= lalloc(c, c->nd_mom, 4 * C);
-    r |= lalloc(c, c->lcorr, C);
-    r |= lalloc(c, c->amag2, static_cast<size_t>(c->cap_n));
-    if (r != NBG_OK) return r;
+    if (!c->nd_mom)
+        if (int r = lalloc(c, c->nd_mom, 4 * C)) return r;
+    if (!c->lcorr)
+        if (int r = lalloc(c, c->lcorr, C)) return r;
+    if (!c->amag2)
+        if (int r = lalloc(c, c->amag2, static_cast<size_t>(c->cap_n))) return r;
     if (!c->lpairs) return learn_grow_pairs(c, std::max<int64_t>(int64_t{1} << 20, 32 * c->cap_n));
     return NBG_OK;
 }

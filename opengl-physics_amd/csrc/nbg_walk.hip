@@ -985,7 +985,9 @@ void launch_walk_bins(nbg_ctx* c, const nbg_params& p) {
     const bool quad = p.multipole == NBG_QUADRUPOLE;
     const bool kahan = (p.flags & NBG_FLAG_KAHAN) != 0;
     const bool count = (p.flags & NBG_FLAG_COUNT) != 0 && c->cnt_far != nullptr;
-    const bool learn = quad && (p.flags & NBG_FLAG_LEARNED) != 0 && c->lpairs != nullptr;
+    // (validation guarantees quadrupoles and the learned buffers; the evaluation's apply
+    // pass relies on this walk writing the sorted accelerations)
+    const bool learn = (p.flags & NBG_FLAG_LEARNED) != 0;
     A.inv_band = (p.learn_band > 0.0) ? static_cast<float>(1.0 / p.learn_band) : 1e30f;
     A.amin = static_cast<float>(p.learn_amin);
     A.lpairs = c->lpairs;
