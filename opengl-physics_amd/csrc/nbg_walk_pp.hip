@@ -111,8 +111,9 @@ __device__ __forceinline__ bool pp_accept(const PPArgs& A, int nd, float4 cm, fl
         const float window = 1e-4f * (lhs + rhs) + rhs * rel;
         amb = !(fabsf(lhs - rhs) > window) || !A.rel_fast;
     } else {
-        // s^2 < theta^2 d^2  <=>  d > s / theta
-        const float sth = sqrtf(au.y) * A.inv_theta;
+        // s^2 < theta^2 d^2  <=>  d > s / theta (the hardware square root is within 1 ulp,
+        // far inside the slack below, past which the FP64 test decides)
+        const float sth = __builtin_amdgcn_sqrtf(au.y) * A.inv_theta;
         const float q2 = sth * sth;
         accept = d2 > q2;
         const float slack = 2.f * sth * (1e-4f * sth + 2e-6f * (cscale + tscale));
