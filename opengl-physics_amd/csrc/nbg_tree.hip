@@ -321,31 +321,49 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t* __r
     hist[hist_index(b, threadIdx.x, blk_sys, sys_blk0)] = s_cnt[threadIdx.x];
 }
 
+// The block's keys are ranked stably by digit (wave ballots per round of 256 keys, per-wave
+// LDS digit counts, a running count per digit across rounds), placed in LDS in digit
+// order, then written out by consecutive threads to consecutive positions of each digit's
+// output run, so the global writes come in runs instead of one scattered word per key.
 template <bool EXTRA>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, const uint64_t* __restrict__ xin,
     uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t* __restrict__ xout, int shift,
     const int32_t* __restrict__ blk_sys, const int32_t* __restrict__ blk_start, const int32_t* __restrict__ blk_count,
     const int32_t* __restrict__ sys_blk0, const uint32_t* __restrict__ offs) {
-    __shared__ uint32_t s_base[kRadix];
+    __shared__ uint32_t s_run[kRadix];   // keys of each digit ranked so far, then the block-local digit offsets
+    __shared__ uint32_t s_gbase[kRadix]; // global output position of the digit's first key of this block
     __shared__ uint32_t s_wcnt[kSortThreads / 64][kRadix];
+    __shared__ uint64_t s_key[kSortTile];
+    __shared__ uint32_t s_val[kSortTile];
+    __shared__ uint64_t s_xtra[EXTRA ? kSortTile : 1];
     const int b = blockIdx.x;
     const int wave = threadIdx.x >> 6;
     const int lane = lane_id();
     const uint64_t lt = (1ull << lane) - 1ull;
-    s_base[threadIdx.x] = offs[hist_index(b, threadIdx.x, blk_sys, sys_blk0)];
+    s_gbase[threadIdx.x] = offs[hist_index(b, threadIdx.x, blk_sys, sys_blk0)];
+    s_run[threadIdx.x] = 0;
     const int start = blk_start[b], cnt = blk_count[b];
+    uint64_t key[kSortItems], xtra[kSortItems];
+    uint32_t val[kSortItems], loc[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int k = r * kSortThreads + threadIdx.x;
+        key[r] = 0;
+        val[r] = 0;
+        xtra[r] = 0;
+        if (k < cnt) {
+            key[r] = kin[start + k];
+            val[r] = vin[start + k];
+            if (EXTRA) xtra[r] = xin[start + k];
+        }
+    }
+    __syncthreads();
+#pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
         const int k = r * kSortThreads + threadIdx.x;
         const bool valid = k < cnt;
-        uint64_t key = 0, xtra = 0;
-        uint32_t v = 0;
-        if (valid) {
-            key = kin[start + k];
-            v = vin[start + k];
-            if (EXTRA) xtra = xin[start + k];
-        }
-        const int d = static_cast<int>((key >> shift) & (kRadix - 1));
+        const int d = static_cast<int>((key[r] >> shift) & (kRadix - 1));
         uint64_t peers = ballot(valid);
 #pragma unroll
         for (int bit = 0; bit < kRadixBits; ++bit) {
@@ -360,18 +378,57 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
         if (valid && rank == 0) s_wcnt[wave][d] = static_cast<uint32_t>(__popcll(peers));
         __syncthreads();
         if (valid) {
-            uint32_t pos = s_base[d] + static_cast<uint32_t>(rank);
+            uint32_t pos = s_run[d] + static_cast<uint32_t>(rank);
             for (int w = 0; w < wave; ++w) pos += s_wcnt[w][d];
-            kout[pos] = key;
-            vout[pos] = v;
-            if (EXTRA) xout[pos] = xtra;
+            loc[r] = pos;  // rank among the block's keys of digit d
         }
         __syncthreads();
         uint32_t add = 0;
 #pragma unroll
         for (int w = 0; w < kSortThreads / 64; ++w) add += s_wcnt[w][threadIdx.x];
-        s_base[threadIdx.x] += add;
+        s_run[threadIdx.x] += add;
         __syncthreads();
+    }
+    // block-local digit offsets: exclusive scan of the digit counts (one digit per thread)
+    {
+        const uint32_t c = s_run[threadIdx.x];
+        uint32_t x = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(x), o, 64));
+            if (lane >= o) x += y;
+        }
+        __shared__ uint32_t s_wsum[kSortThreads / 64];
+        if (lane == 63) s_wsum[wave] = x;
+        __syncthreads();
+        uint32_t before = 0;
+        for (int w = 0; w < wave; ++w) before += s_wsum[w];
+        s_run[threadIdx.x] = before + x - c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int k = r * kSortThreads + threadIdx.x;
+        if (k < cnt) {
+            const int d = static_cast<int>((key[r] >> shift) & (kRadix - 1));
+            const uint32_t p = s_run[d] + loc[r];
+            s_key[p] = key[r];
+            s_val[p] = val[r];
+            if (EXTRA) s_xtra[p] = xtra[r];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int p = r * kSortThreads + threadIdx.x;
+        if (p < cnt) {
+            const uint64_t kk = s_key[p];
+            const int d = static_cast<int>((kk >> shift) & (kRadix - 1));
+            const uint32_t pos = s_gbase[d] + static_cast<uint32_t>(p) - s_run[d];
+            kout[pos] = kk;
+            vout[pos] = s_val[p];
+            if (EXTRA) xout[pos] = s_xtra[p];
+        }
     }
 }
 
