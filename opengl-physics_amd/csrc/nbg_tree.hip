@@ -123,11 +123,47 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const T* __restrict_
     }
 }
 
-// part must hold grid_for(n, kScanTile) elements.
+// The down pass for a modest number of tiles: each block sums the tile totals before it
+// itself (at most kScanInline of them), so the separate partials pass is not needed; the
+// last block stores the grand total.
+constexpr int kScanInline = 4096;
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_down_inline(const T* __restrict__ in, T* __restrict__ out,
+                                                                   int64_t n, const T* __restrict__ part,
+                                                                   T* __restrict__ total_out) {
+    __shared__ T s_wave[kScanThreads / 64];
+    T pre = 0;
+    for (int i = threadIdx.x; i < static_cast<int>(blockIdx.x); i += kScanThreads) pre += part[i];
+    T before;
+    (void)block_excl_scan(pre, s_wave, &before);
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+    T v[kScanItems];
+    T sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : T(0);
+        sum += v[k];
+    }
+    T tot;
+    T off = block_excl_scan(sum, s_wave, &tot) + before;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = off;
+        off += v[k];
+    }
+    if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total_out = before + tot;
+}
+
+// part must hold grid_for(n, kScanTile) elements.  In place (in == out) is allowed.
 template <typename T>
 void excl_scan(hipStream_t st, const T* in, T* out, int64_t n, T* part, T* total_out) {
     const int nb = grid_for(n, kScanTile);
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(kScanThreads), 0, st, in, n, part);
+    if (nb <= kScanInline) {
+        hipLaunchKernelGGL(k_scan_down_inline<T>, dim3(nb), dim3(kScanThreads), 0, st, in, out, n, part, total_out);
+        return;
+    }
     hipLaunchKernelGGL(k_scan_partials<T>, dim3(1), dim3(kScanThreads), 0, st, part, static_cast<int64_t>(nb), total_out);
     hipLaunchKernelGGL(k_scan_down<T>, dim3(nb), dim3(kScanThreads), 0, st, in, out, n, part);
 }
