@@ -285,6 +285,31 @@ def test_learned_graph_matches_eager(gpu_ctx_factory):
     assert err < 1e-5
 
 
+def test_listing_walk_forces_equal_plain(gpu_ctx_factory):
+    """The learned walk writes its accelerations by sorted position and the apply pass
+    scatters them: with a model whose output is zero everywhere the learned evaluation,
+    and the listing walk of sample collection (no correction), reproduce the plain walk
+    at the same theta bit for bit."""
+    import pynbg
+    from pynbg import learned
+    s = plummer_f32(8192, 6)
+    ctx = gpu_ctx_factory(8192)
+    ctx.upload(s["x"], s["y"], s["z"], s["m"])
+    ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=0.01, theta=0.8))
+    plain = ctx.accelerations().copy()
+    # W4 and b4 (the last 64*3 + 3 words of the blob) zero: the network outputs zero
+    zero = learned.init_blob(3)
+    zero[-(64 * 3 + 3):] = 0.0
+    ctx.learn_set_weights(zero)
+    st = ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=0.01, theta=0.8, learned=True, learn_band=0.0))
+    assert st.n_learned > 0
+    np.testing.assert_array_equal(ctx.accelerations(), plain)
+    ctx.learn_clear()
+    n = ctx.learn_collect(pynbg.make_params("barnes_hut", G=1.0, eps=0.01, theta=0.8, learn_band=0.0))
+    assert n > 0
+    np.testing.assert_array_equal(ctx.accelerations(), plain)
+
+
 def test_learned_argument_errors(gpu_ctx_factory):
     import pynbg
     s = plummer_f32(512, 1)
