@@ -93,6 +93,18 @@ __device__ __forceinline__ float pair_coef(float r2, float gm, const Soft& s) {
     }
 }
 
+// pair_coef of the displacement (dx, dy, dz); for Plummer the softening enters the
+// fused multiply-add chain of r^2 + eps^2 (one instruction less than adding it after).
+template <bool SPLINE>
+__device__ __forceinline__ float pair_coef_d(float dx, float dy, float dz, float gm, const Soft& s) {
+    if (!SPLINE) {
+        const float inv = __builtin_amdgcn_rsqf(fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, s.eps2))));
+        return gm * (inv * inv * inv);
+    } else {
+        return pair_coef<true>(dx * dx + dy * dy + dz * dz, gm, s);
+    }
+}
+
 // M2P of one cell (com + G*mass in cm, G*(Qxx,Qyy,Qxy,Qxz) in q, G*Qyz in qyz) onto a
 // target at (tx,ty,tz): softened monopole plus the unsoftened traceless quadrupole.
 template <bool QUAD, bool SPLINE>

@@ -585,7 +585,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                     const float dx = q.x - tp.x, dy = q.y - tp.y, dz = q.z - tp.z;
                     float f;
                     if (FAST) {
-                        f = pair_coef<SPL>(dx * dx + dy * dy + dz * dz, q.w * us[u], soft);
+                        f = pair_coef_d<SPL>(dx, dy, dz, q.w * us[u], soft);
                     } else {
                         f = pair_coef<SPL>(dx * dx + dy * dy + dz * dz, q.w, soft);
                         f = (us[u] > 0.f && idx[u] != tpos) ? f : 0.f;
