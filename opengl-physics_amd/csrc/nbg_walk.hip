@@ -398,8 +398,8 @@ template <bool QUAD, bool KAHAN, bool COUNT, bool LEARN, int NT, bool SPL>
 __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     __shared__ int s_snode[kBinWaves][kBinStack];
     __shared__ uint64_t s_smask[kBinWaves][kBinStack];
-    __shared__ int s_fnode[kBinWaves][kBinRing];
-    __shared__ uint64_t s_fmask[kBinWaves][kBinRing];
+    __shared__ __attribute__((aligned(16))) int s_fnode[kBinWaves][kBinRing];
+    __shared__ __attribute__((aligned(16))) uint64_t s_fmask[kBinWaves][kBinRing];
     __shared__ int s_qstart[kBinWaves][kBinRing];
     __shared__ int s_qcnt[kBinWaves][kBinRing];
     __shared__ uint64_t s_qmask[kBinWaves][kBinRing];
@@ -505,18 +505,26 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
         // evaluates `n` staged far cells starting at ring position `h`, four at a time
         // (their scalar loads issued before any arithmetic); every lane computes every
         // cell and keeps it only if its leaf's bit is in the cell's mask
+        // (the ring is consumed from multiples of 64, so a group of four slots is one aligned
+        // 16-byte node read and two 16-byte mask reads; slots past n repeat the first node)
         auto eval_far = [&](int h, int n) __attribute__((always_inline)) {
             float sx = 0.f, sy = 0.f, sz = 0.f;
+            const int hb = h & (kBinRing - 1);
             for (int e0 = 0; e0 < n; e0 += 4) {
                 int nd[4];
                 bool um[4];
                 float4 cm[4], q[4];
                 float qyz[4];
+                const int4 n4 = *reinterpret_cast<const int4*>(fnode + hb + e0);
+                const ulonglong2 m01 = *reinterpret_cast<const ulonglong2*>(fmask + hb + e0);
+                const ulonglong2 m23 = *reinterpret_cast<const ulonglong2*>(fmask + hb + e0 + 2);
+                const int nv[4] = {n4.x, n4.y, n4.z, n4.w};
+                const uint64_t mv[4] = {m01.x, m01.y, m23.x, m23.y};
+                const int nd0 = __builtin_amdgcn_readfirstlane(nv[0]);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int slot = (h + min(e0 + u, n - 1)) & (kBinRing - 1);
-                    nd[u] = __builtin_amdgcn_readfirstlane(fnode[slot]);
-                    um[u] = e0 + u < n && (fmask[slot] & mybit);
+                    nd[u] = (u == 0 || e0 + u < n) ? __builtin_amdgcn_readfirstlane(nv[u]) : nd0;
+                    um[u] = e0 + u < n && (mv[u] & mybit);
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
