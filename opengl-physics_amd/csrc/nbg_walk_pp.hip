@@ -141,8 +141,8 @@ template <bool QUAD, bool KAHAN, bool SPL, bool REL, bool COUNT>
 __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
     __shared__ int s_snode[kPPWaves][kPPStack];
     __shared__ uint64_t s_smask[kPPWaves][kPPStack];
-    __shared__ int s_fnode[kPPWaves][kPPRing];
-    __shared__ uint64_t s_fmask[kPPWaves][kPPRing];
+    __shared__ __attribute__((aligned(16))) int s_fnode[kPPWaves][kPPRing];
+    __shared__ __attribute__((aligned(16))) uint64_t s_fmask[kPPWaves][kPPRing];
     __shared__ int s_qstart[kPPWaves][kPPRing];
     __shared__ int s_qcnt[kPPWaves][kPPRing];
     __shared__ uint64_t s_qmask[kPPWaves][kPPRing];
@@ -249,18 +249,26 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
         uint32_t steps = 0;             // traversal steps of this bin (relative MAC cost record)
 
         // evaluates `n` staged far cells from ring position `h`, four scalar loads ahead
+        // (the ring is consumed from multiples of 64, so a group of four slots is one aligned
+        // 16-byte node read and two 16-byte mask reads; slots past n repeat the first node)
         auto eval_far = [&](int h, int n) __attribute__((always_inline)) {
             float sx = 0.f, sy = 0.f, sz = 0.f;
+            const int hb = h & (kPPRing - 1);
             for (int e0 = 0; e0 < n; e0 += 4) {
                 int nd[4];
                 bool um[4];
                 float4 cm[4], q[4];
                 float qyz[4];
+                const int4 n4 = *reinterpret_cast<const int4*>(fnode + hb + e0);
+                const ulonglong2 m01 = *reinterpret_cast<const ulonglong2*>(fmask + hb + e0);
+                const ulonglong2 m23 = *reinterpret_cast<const ulonglong2*>(fmask + hb + e0 + 2);
+                const int nv[4] = {n4.x, n4.y, n4.z, n4.w};
+                const uint64_t mv[4] = {m01.x, m01.y, m23.x, m23.y};
+                const int nd0 = __builtin_amdgcn_readfirstlane(nv[0]);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int slot = (h + min(e0 + u, n - 1)) & (kPPRing - 1);
-                    nd[u] = __builtin_amdgcn_readfirstlane(fnode[slot]);
-                    um[u] = e0 + u < n && (fmask[slot] & mybit);
+                    nd[u] = (u == 0 || e0 + u < n) ? __builtin_amdgcn_readfirstlane(nv[u]) : nd0;
+                    um[u] = e0 + u < n && (mv[u] & mybit);
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
