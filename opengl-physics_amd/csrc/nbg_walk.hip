@@ -400,8 +400,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     __shared__ uint64_t s_smask[kBinWaves][kBinStack];
     __shared__ __attribute__((aligned(16))) int s_fnode[kBinWaves][kBinRing];
     __shared__ __attribute__((aligned(16))) uint64_t s_fmask[kBinWaves][kBinRing];
-    __shared__ int s_qstart[kBinWaves][kBinRing];
-    __shared__ int s_qcnt[kBinWaves][kBinRing];
+    __shared__ int2 s_qsc[kBinWaves][kBinRing];  // near leaf: first sorted particle, count
     __shared__ uint64_t s_qmask[kBinWaves][kBinRing];
     __shared__ float4 s_lcom[kBinWaves][64];
     __shared__ int s_lid[kBinWaves][64];
@@ -413,8 +412,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     uint64_t* smask = s_smask[wave];
     int* fnode = s_fnode[wave];
     uint64_t* fmask = s_fmask[wave];
-    int* qstart = s_qstart[wave];
-    int* qcnt = s_qcnt[wave];
+    int2* qsc = s_qsc[wave];
     uint64_t* qmask = s_qmask[wave];
     float4* lcom = s_lcom[wave];
     int* lid = s_lid[wave];
@@ -463,8 +461,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             lcom[j] = make_float4(c.x, c.y, c.z, A.nd_aux[mylf].z);
             lid[j] = mylf;
             // the leaf's own members (self excluded when evaluated)
-            qstart[j] = A.nd_start[mylf];
-            qcnt[j] = static_cast<int>(__float_as_uint(A.nd_aux[mylf].w) & ~kLeafBit);
+            qsc[j] = make_int2(A.nd_start[mylf], static_cast<int>(__float_as_uint(A.nd_aux[mylf].w) & ~kLeafBit));
             qmask[j] = 1ull << j;
         }
         const int root = cld(A.sys_root, cld(A.pos_sys, start));
@@ -559,8 +556,9 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             float wuse = 0.f;
             auto load_entry = [&](int ee) __attribute__((always_inline)) {
                 const int slot = (h + ee) & (kBinRing - 1);
-                st = __builtin_amdgcn_readfirstlane(qstart[slot]);
-                c = __builtin_amdgcn_readfirstlane(qcnt[slot]);
+                const int2 e2 = qsc[slot];
+                st = __builtin_amdgcn_readfirstlane(e2.x);
+                c = __builtin_amdgcn_readfirstlane(e2.y);
                 em = qmask[slot];
                 wuse = (em & mybit) ? 1.f : 0.f;
             };
@@ -808,8 +806,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 const uint64_t bm = ballot(has);
                 if (has) {
                     const int slot = (q_tail + mbcnt(bm)) & (kBinRing - 1);
-                    qstart[slot] = lstart;
-                    qcnt[slot] = lcount;
+                    qsc[slot] = make_int2(lstart, lcount);
                     qmask[slot] = near_m;
                 }
                 q_tail += __popcll(bm);

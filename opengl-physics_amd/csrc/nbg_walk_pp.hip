@@ -143,8 +143,7 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
     __shared__ uint64_t s_smask[kPPWaves][kPPStack];
     __shared__ __attribute__((aligned(16))) int s_fnode[kPPWaves][kPPRing];
     __shared__ __attribute__((aligned(16))) uint64_t s_fmask[kPPWaves][kPPRing];
-    __shared__ int s_qstart[kPPWaves][kPPRing];
-    __shared__ int s_qcnt[kPPWaves][kPPRing];
+    __shared__ int2 s_qsc[kPPWaves][kPPRing];  // near leaf: first sorted particle, count
     __shared__ uint64_t s_qmask[kPPWaves][kPPRing];
     __shared__ int s_next[kPPWaves];
 
@@ -154,8 +153,7 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
     uint64_t* smask = s_smask[wave];
     int* fnode = s_fnode[wave];
     uint64_t* fmask = s_fmask[wave];
-    int* qstart = s_qstart[wave];
-    int* qcnt = s_qcnt[wave];
+    int2* qsc = s_qsc[wave];
     uint64_t* qmask = s_qmask[wave];
     const Soft soft = A.soft;
     const int n_items = A.tlist ? static_cast<int>(__builtin_amdgcn_readfirstlane(static_cast<int>(*A.tcount))) : 0;
@@ -300,8 +298,9 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
             float wuse = 0.f;
             auto load_entry = [&](int ee) __attribute__((always_inline)) {
                 const int slot = (h + ee) & (kPPRing - 1);
-                st = __builtin_amdgcn_readfirstlane(qstart[slot]);
-                c = __builtin_amdgcn_readfirstlane(qcnt[slot]);
+                const int2 e2 = qsc[slot];
+                st = __builtin_amdgcn_readfirstlane(e2.x);
+                c = __builtin_amdgcn_readfirstlane(e2.y);
                 em = qmask[slot];
                 wuse = (em & mybit) ? 1.f : 0.f;
             };
@@ -398,8 +397,7 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
                     // a leaf: P2P for every active target (self excluded when evaluated)
                     if (lane == 0) {
                         const int slot = q_tail & (kPPRing - 1);
-                        qstart[slot] = cld(A.nd_start, nd[u]);
-                        qcnt[slot] = static_cast<int>(info & ~kLeafBit);
+                        qsc[slot] = make_int2(cld(A.nd_start, nd[u]), static_cast<int>(info & ~kLeafBit));
                         qmask[slot] = am_in;
                     }
                     ++q_tail;
