@@ -117,7 +117,8 @@ typedef struct nbg_stats {
 
 /* Create a context on HIP device `device` able to hold up to max_n particles.
  * Replaces the process-global lazily created GL state of
- * projects/nbody_core/src/gpu/GpuBarnesHut.cpp:452-511.  Returns NULL on failure. */
+ * projects/nbody_core/src/gpu/GpuBarnesHut.cpp:452-511.  1 <= max_n <= 2^28 - 1024
+ * (268,434,432).  Returns NULL on failure. */
 nbg_ctx* nbg_create(int device, int64_t max_n);
 void nbg_destroy(nbg_ctx* ctx);
 const char* nbg_last_error(const nbg_ctx* ctx);

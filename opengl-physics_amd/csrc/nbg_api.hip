@@ -462,7 +462,8 @@ int block_steps(nbg_ctx* c, const nbg_params& p, int nsteps, nbg_stats* stats) {
 extern "C" {
 
 nbg_ctx* nbg_create(int device, int64_t max_n) {
-    if (max_n < 1 || max_n > (int64_t{1} << 30)) return nullptr;
+    // The force kernels address sorted particles through 32-bit byte offsets (index << 4).
+    if (max_n < 1 || max_n > kMaxParticles) return nullptr;
     if (hipSetDevice(device) != hipSuccess) return nullptr;
     nbg_ctx* c = new (std::nothrow) nbg_ctx();
     if (!c) return nullptr;

@@ -27,6 +27,14 @@ __device__ __forceinline__ float4 cld(const float4* p, int i) {
     cfloat* q = (cfloat*)(p + i);
     return make_float4(q[0], q[1], q[2], q[3]);
 }
+// The same read through a 32-bit byte offset: the SMEM load then takes the offset in an
+// SGPR instead of a 64-bit address computed per load.  Valid for i < 2^28 (nbg_create caps
+// the particle count so every sorted-particle index qualifies).
+__device__ __forceinline__ float4 cld_off(const float4* p, int i) {
+    typedef __attribute__((address_space(4))) const char cchar;
+    cfloat* q = (cfloat*)((cchar*)p + (static_cast<uint32_t>(i) << 4));
+    return make_float4(q[0], q[1], q[2], q[3]);
+}
 __device__ __forceinline__ int32_t cld(const int32_t* p, int i) { return ((cint32*)p)[i]; }
 __device__ __forceinline__ uint32_t cld_u32(const void* p, int i) { return ((cuint32*)p)[i]; }
 

@@ -584,7 +584,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 }
                 float4 spt[NT];
 #pragma unroll
-                for (int u = 0; u < NT; ++u) spt[u] = cld(A.spos, idx[u]);
+                for (int u = 0; u < NT; ++u) spt[u] = cld_off(A.spos, idx[u]);
 #pragma unroll
                 for (int u = 0; u < NT; ++u) {
                     const float4 q = spt[u];

@@ -326,7 +326,7 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
                 }
                 float4 spt[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) spt[u] = cld(A.spos, idx[u]);
+                for (int u = 0; u < 8; ++u) spt[u] = cld_off(A.spos, idx[u]);
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     const float4 q = spt[u];

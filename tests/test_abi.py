@@ -39,6 +39,16 @@ def test_exported_symbols_are_c_linkage():
     assert set(declared_functions(HEADER)) <= exported
 
 
+def test_create_rejects_sizes_outside_its_range():
+    # The range check comes before any HIP call, so it runs without a GPU.  The force
+    # kernels address sorted particles with 32-bit byte offsets, hence the 2^28 - 1024 cap.
+    lib = ctypes.CDLL(LIB)
+    lib.nbg_create.restype = ctypes.c_void_p
+    lib.nbg_create.argtypes = [ctypes.c_int, ctypes.c_int64]
+    for bad in (0, -5, (1 << 28) - 1023, 1 << 30):
+        assert lib.nbg_create(0, bad) is None, bad
+
+
 def test_binding_covers_the_header():
     import sys
     sys.path.insert(0, os.path.join(ROOT, "opengl-physics_amd"))
