@@ -977,6 +977,17 @@ __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ lea
     }
 }
 
+// An empty asm that takes v as input and output: the compiler must have the loaded value
+// in registers here and cannot move its load into a later branch.
+__device__ __forceinline__ void pin(float4& v) {
+    asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
+}
+
+// Branch-free over the children: an absent child (index < 0) reads this node's own slot
+// and every accumulation it would make is discarded by a select, so all of a node's child
+// loads are in flight together (behind per-child branches they were 40-odd dependent
+// memory round trips per node, 7 us for the root alone).  The selects keep the arithmetic
+// of the skip-by-branch form term for term.
 template <bool QUAD, bool MOM>
 __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* __restrict__ depth_nodes,
                                                      const int32_t* __restrict__ nd_child,
@@ -988,20 +999,46 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
     const int off = static_cast<int>(sc->depth_offset[depth]);
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
         const int id = depth_nodes[off + k];
-        int ch[8];
+        const int4 c0 = reinterpret_cast<const int4*>(nd_child)[2 * id];
+        const int4 c1 = reinterpret_cast<const int4*>(nd_child)[2 * id + 1];
+        const int ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+        int src[8];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) ch[c] = nd_child[8 * id + c];
+        for (int c = 0; c < 8; ++c) src[c] = ch[c] >= 0 ? ch[c] : id;
+        double4 cd[8];
+        float4 cc[8], ca[8], cq[8], cm[MOM ? 32 : 1];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            cd[c] = nd_com64[src[c]];
+            cc[c] = nd_com[src[c]];
+            ca[c] = nd_aux[src[c]];
+            if (QUAD) cq[c] = nd_quad[src[c]];
+            if (MOM) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) cm[4 * c + j] = nd_mom[4 * src[c] + j];
+            }
+        }
+        // Values used only on one side of a select would otherwise be loaded inside a
+        // branch (one memory round trip per child); pinning them here keeps one wait.
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            pin(cc[c]);
+            pin(ca[c]);
+            if (QUAD) pin(cq[c]);
+            if (MOM) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) pin(cm[4 * c + j]);
+            }
+        }
         double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            if (ch[c] < 0) continue;
-            const double4 cc = nd_com64[ch[c]];
-            const double w = cc.w;
-            if (w == 0.0) continue;
-            M += w;
-            sx += w * cc.x;
-            sy += w * cc.y;
-            sz += w * cc.z;
+            const double w = cd[c].w;
+            const bool use = ch[c] >= 0 && w != 0.0;
+            M = use ? M + w : M;
+            sx = use ? sx + w * cd[c].x : sx;
+            sy = use ? sy + w * cd[c].y : sy;
+            sz = use ? sz + w * cd[c].z : sz;
         }
         double gx = __longlong_as_double(0x7ff8000000000000ll), gy = gx, gz = gx;
         if (M != 0.0) {
@@ -1012,46 +1049,46 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
         const float fx = static_cast<float>(gx), fy = static_cast<float>(gy), fz = static_cast<float>(gz);
         float qxx = 0.f, qyy = 0.f, qxy = 0.f, qxz = 0.f, qyz = 0.f, rmax = 0.f;
         float mo[16] = {};
-        if (M > 0.0) {
+        const bool massive = M > 0.0;
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                if (ch[c] < 0) continue;
-                const float4 cc = nd_com[ch[c]];
-                const float4 ca = nd_aux[ch[c]];
-                const float dx = cc.x - fx, dy = cc.y - fy, dz = cc.z - fz;
-                const float d2 = dx * dx + dy * dy + dz * dz;
-                rmax = fmaxf(rmax, ca.z + sqrtf(d2));
-                if (MOM && cc.w > 0.f) {
-                    // parallel-axis shift of the child's moments to this com
-                    const float4 q0 = nd_mom[4 * ch[c] + 0], q1 = nd_mom[4 * ch[c] + 1], q2 = nd_mom[4 * ch[c] + 2],
-                                 q3 = nd_mom[4 * ch[c] + 3];
-                    const float m = cc.w;
-                    const float sxx = q0.x, syy = q0.y, szz = q0.z, sxy = q0.w, sxz = q1.x, syz = q1.y;
-                    mo[0] += sxx + m * dx * dx;
-                    mo[1] += syy + m * dy * dy;
-                    mo[2] += szz + m * dz * dz;
-                    mo[3] += sxy + m * dx * dy;
-                    mo[4] += sxz + m * dx * dz;
-                    mo[5] += syz + m * dy * dz;
-                    mo[6] += q1.z + 3.f * sxx * dx + m * dx * dx * dx;
-                    mo[7] += q1.w + 3.f * syy * dy + m * dy * dy * dy;
-                    mo[8] += q2.x + 3.f * szz * dz + m * dz * dz * dz;
-                    mo[9] += q2.y + sxx * dy + 2.f * sxy * dx + m * dx * dx * dy;
-                    mo[10] += q2.z + sxx * dz + 2.f * sxz * dx + m * dx * dx * dz;
-                    mo[11] += q2.w + syy * dx + 2.f * sxy * dy + m * dx * dy * dy;
-                    mo[12] += q3.x + syy * dz + 2.f * syz * dy + m * dy * dy * dz;
-                    mo[13] += q3.y + szz * dx + 2.f * sxz * dz + m * dx * dz * dz;
-                    mo[14] += q3.z + szz * dy + 2.f * syz * dz + m * dy * dz * dz;
-                    mo[15] += q3.w + sxy * dz + sxz * dy + syz * dx + m * dx * dy * dz;
-                }
-                if (QUAD && cc.w > 0.f) {
-                    const float4 cq = nd_quad[ch[c]];
-                    qxx += cq.x + cc.w * (3.f * dx * dx - d2);
-                    qyy += cq.y + cc.w * (3.f * dy * dy - d2);
-                    qxy += cq.z + cc.w * 3.f * dx * dy;
-                    qxz += cq.w + cc.w * 3.f * dx * dz;
-                    qyz += ca.x + cc.w * 3.f * dy * dz;
-                }
+        for (int c = 0; c < 8; ++c) {
+            const bool live = massive && ch[c] >= 0;
+            const float dx = cc[c].x - fx, dy = cc[c].y - fy, dz = cc[c].z - fz;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            rmax = live ? fmaxf(rmax, ca[c].z + sqrtf(d2)) : rmax;
+            const bool heavy = live && cc[c].w > 0.f;
+            if (MOM) {
+                // parallel-axis shift of the child's moments to this com
+                const float4 q0 = cm[4 * c + 0], q1 = cm[4 * c + 1], q2 = cm[4 * c + 2], q3 = cm[4 * c + 3];
+                const float m = cc[c].w;
+                const float sxx = q0.x, syy = q0.y, szz = q0.z, sxy = q0.w, sxz = q1.x, syz = q1.y;
+                float t[16];
+                t[0] = sxx + m * dx * dx;
+                t[1] = syy + m * dy * dy;
+                t[2] = szz + m * dz * dz;
+                t[3] = sxy + m * dx * dy;
+                t[4] = sxz + m * dx * dz;
+                t[5] = syz + m * dy * dz;
+                t[6] = q1.z + 3.f * sxx * dx + m * dx * dx * dx;
+                t[7] = q1.w + 3.f * syy * dy + m * dy * dy * dy;
+                t[8] = q2.x + 3.f * szz * dz + m * dz * dz * dz;
+                t[9] = q2.y + sxx * dy + 2.f * sxy * dx + m * dx * dx * dy;
+                t[10] = q2.z + sxx * dz + 2.f * sxz * dx + m * dx * dx * dz;
+                t[11] = q2.w + syy * dx + 2.f * sxy * dy + m * dx * dy * dy;
+                t[12] = q3.x + syy * dz + 2.f * syz * dy + m * dy * dy * dz;
+                t[13] = q3.y + szz * dx + 2.f * sxz * dz + m * dx * dz * dz;
+                t[14] = q3.z + szz * dy + 2.f * syz * dz + m * dy * dz * dz;
+                t[15] = q3.w + sxy * dz + sxz * dy + syz * dx + m * dx * dy * dz;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) mo[i] = heavy ? mo[i] + t[i] : mo[i];
+            }
+            if (QUAD) {
+                const float w = cc[c].w;
+                qxx = heavy ? qxx + (cq[c].x + w * (3.f * dx * dx - d2)) : qxx;
+                qyy = heavy ? qyy + (cq[c].y + w * (3.f * dy * dy - d2)) : qyy;
+                qxy = heavy ? qxy + (cq[c].z + w * 3.f * dx * dy) : qxy;
+                qxz = heavy ? qxz + (cq[c].w + w * 3.f * dx * dz) : qxz;
+                qyz = heavy ? qyz + (ca[c].x + w * 3.f * dy * dz) : qyz;
             }
         }
         if (MOM) {
