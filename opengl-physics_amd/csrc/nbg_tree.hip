@@ -517,6 +517,30 @@ __device__ __forceinline__ int cpl_at(const uint64_t* __restrict__ kh, const uin
     return cpl_deep(kh[i], kl[i], kh[j], kl[j]);
 }
 
+// Number of particles in the depth-d cell that starts at sorted position i (its system
+// ends at s1): the first j > i outside the cell, by bisection on the common prefix.
+template <int LEVELS>
+__device__ __forceinline__ int cell_count(const uint64_t* __restrict__ kh, const uint64_t* __restrict__ kl, int i,
+                                          int s1, int d) {
+    int lo = i + 1, hi = s1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cpl_at<LEVELS>(kh, kl, i, mid) >= d)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo - i;
+}
+// The same for a leaf above the depth cap, which holds at most kNcrit particles.
+template <int LEVELS>
+__device__ __forceinline__ int leaf_count(const uint64_t* __restrict__ kh, const uint64_t* __restrict__ kl, int i,
+                                          int s1, int d) {
+    int count = 1;
+    for (int k = 1; k <= kNcrit && i + k < s1; ++k) count += (cpl_at<LEVELS>(kh, kl, i, i + k) >= d);
+    return count;
+}
+
 // Leaf depth and common prefix with the predecessor, per sorted position; the number of
 // nodes and leaves each position starts, packed for one 64-bit scan.
 template <int LEVELS>
@@ -634,22 +658,11 @@ __global__ __launch_bounds__(256) void k_nodes(const uint64_t* __restrict__ kh, 
             return;
         }
         const int id = static_cast<int>(id64);
-        // particle range end: first j > i outside the depth-d cell
-        int count;
-        if (d == D && D < LEVELS) {
-            count = 1;
-            for (int k = 1; k <= kNcrit && i + k < s1; ++k) count += (cpl_at<LEVELS>(kh, kl, i, i + k) >= d);
-        } else {
-            int lo = i + 1, hi = s1;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (cpl_at<LEVELS>(kh, kl, i, mid) >= d)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            count = lo - i;
-        }
+        // particle range of the leaf: first j > i outside the depth-D cell.  Internal
+        // nodes (d < D) get theirs in k_node_finish, one search per node thread: searched
+        // here, a position that starts a long chain of cells ran one search per level
+        // serially and set the kernel's time.
+        if (d == D) nd_count[id] = (D < LEVELS) ? leaf_count<LEVELS>(kh, kl, i, s1, d) : cell_count<LEVELS>(kh, kl, i, s1, d);
         int parent;
         if (d == 0) {
             parent = -1;
@@ -669,7 +682,6 @@ __global__ __launch_bounds__(256) void k_nodes(const uint64_t* __restrict__ kh, 
             parent = static_cast<int>(node_scan[lo] >> 32) + (d - 1) - (cpl_prev[lo] + 1);
         }
         nd_start[id] = i;
-        nd_count[id] = count;
         nd_depth[id] = d;
         nd_parent[id] = parent;
         nd_oct[id] = (d > 0) ? key_digit(kh_i, kl_i, d) : 0;
@@ -688,15 +700,16 @@ __global__ __launch_bounds__(256) void k_link(const int32_t* __restrict__ nd_par
     }
 }
 
-// Child counts, FP64 geometry (replaying Tree.cpp:96-99 along the key), leaf info word,
-// the per-depth histogram of internal nodes (privatised in LDS, one global atomic per
-// block and depth) and the sorted-position -> leaf map.
+// Child counts, internal nodes' particle counts, FP64 geometry (replaying Tree.cpp:96-99
+// along the key), leaf info word, the per-depth histogram of internal nodes (privatised in
+// LDS, one global atomic per block and depth) and the sorted-position -> leaf map.
 template <int LEVELS>
 __global__ __launch_bounds__(256) void k_node_finish(const uint64_t* __restrict__ kh, const uint64_t* __restrict__ kl,
                                                      const int32_t* __restrict__ pos_sys,
+                                                     const int32_t* __restrict__ sys_off,
                                                      const double* __restrict__ sys_geom,
                                                      const int32_t* __restrict__ nd_start,
-                                                     const int32_t* __restrict__ nd_count,
+                                                     int32_t* __restrict__ nd_count,
                                                      const int32_t* __restrict__ nd_depth,
                                                      const int32_t* __restrict__ nd_child, double* __restrict__ nd_geom,
                                                      float4* __restrict__ nd_aux, int32_t* __restrict__ leaf_of,
@@ -729,7 +742,13 @@ __global__ __launch_bounds__(256) void k_node_finish(const uint64_t* __restrict_
         nd_geom[4 * id + 2] = cz;
         nd_geom[4 * id + 3] = h;
         const double size = 2.0 * h;
-        const int cnt = nd_count[id];
+        int cnt;
+        if (nk > 0) {
+            cnt = cell_count<LEVELS>(kh, kl, i, sys_off[s + 1], d);
+            nd_count[id] = cnt;
+        } else {
+            cnt = nd_count[id];
+        }
         const uint32_t info = (nk == 0) ? (kLeafBit | static_cast<uint32_t>(cnt)) : static_cast<uint32_t>(nk);
         nd_aux[id] = make_float4(0.0f, static_cast<float>(size * size), 0.0f, __uint_as_float(info));
         if (nk > 0) {
@@ -864,11 +883,13 @@ void launch_build(nbg_ctx* c, int levels) {
     hipLaunchKernelGGL(k_link, dim3(gnodes), dim3(256), 0, st, c->nd_parent, c->nd_oct, c->dsc, c->nd_child);
     if (levels == kLevelsShort)
         hipLaunchKernelGGL(k_node_finish<kLevelsShort>, dim3(gnodes), dim3(256), 0, st, c->key_sorted, c->klo_sorted,
-                           c->pos_sys, c->sys_geom, c->nd_start, c->nd_count, c->nd_depth, c->nd_child, c->nd_geom,
+                           c->pos_sys, c->sys_off, c->sys_geom, c->nd_start, c->nd_count, c->nd_depth, c->nd_child,
+                           c->nd_geom,
                            c->nd_aux, c->leaf_of, c->dsc);
     else
         hipLaunchKernelGGL(k_node_finish<kLevelsDeep>, dim3(gnodes), dim3(256), 0, st, c->key_sorted, c->klo_sorted,
-                           c->pos_sys, c->sys_geom, c->nd_start, c->nd_count, c->nd_depth, c->nd_child, c->nd_geom,
+                           c->pos_sys, c->sys_off, c->sys_geom, c->nd_start, c->nd_count, c->nd_depth, c->nd_child,
+                           c->nd_geom,
                            c->nd_aux, c->leaf_of, c->dsc);
     hipLaunchKernelGGL(k_depth_scan, dim3(1), dim3(64), 0, st, c->dsc);
     hipLaunchKernelGGL(k_depth_scatter, dim3(gnodes), dim3(256), 0, st, c->nd_depth, c->nd_aux, c->dsc,
