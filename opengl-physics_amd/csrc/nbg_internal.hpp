@@ -30,7 +30,8 @@ constexpr uint32_t kLeafBit = 0x80000000u;
 // Zero float4 entries after the sorted particles: the walk loads 8 particles at a time
 // from a leaf start without a bound check.
 constexpr int kSposPad = 8;
-// Largest context: sorted-particle indices (padding included) stay below 2^28.
+// Largest context: sorted-particle indices (padding included) stay below 2^28, so the
+// force kernels' 32-bit byte offsets (index << 4) into the sorted particles cannot wrap.
 constexpr int64_t kMaxParticles = (int64_t{1} << 28) - 1024;
 
 // Device-side scalars written by the build, read back only for stats.

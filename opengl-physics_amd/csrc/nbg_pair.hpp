@@ -27,15 +27,23 @@ __device__ __forceinline__ float4 cld(const float4* p, int i) {
     cfloat* q = (cfloat*)(p + i);
     return make_float4(q[0], q[1], q[2], q[3]);
 }
-// The same read through a 32-bit byte offset: the SMEM load then takes the offset in an
-// SGPR instead of a 64-bit address computed per load.  Valid for i < 2^28 (nbg_create caps
-// the particle count so every sorted-particle index qualifies).
-__device__ __forceinline__ float4 cld_off(const float4* p, int i) {
-    typedef __attribute__((address_space(4))) const char cchar;
-    cfloat* q = (cfloat*)((cchar*)p + (static_cast<uint32_t>(i) << 4));
-    return make_float4(q[0], q[1], q[2], q[3]);
-}
 __device__ __forceinline__ int32_t cld(const int32_t* p, int i) { return ((cint32*)p)[i]; }
+
+// Records of a float4 array through a raw buffer resource: a wave-uniform index goes into
+// the load's SGPR offset (index << 4 in 32 bits: valid for index < 2^28, which nbg_create's
+// particle cap guarantees for sorted particles) and the record lands in VGPRs, so a group
+// of such loads is in flight together without holding four SGPRs per record (scalar loads
+// of a P2P group were split into three waits by SGPR pressure).  The third descriptor word
+// is the gfx9-family default (32-bit data format).
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rec_rsrc(const float4* p, int n_records) {
+    const uint32_t bytes = static_cast<uint32_t>(n_records) * 16u;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(p), 0, static_cast<int>(bytes), 0x00020000);
+}
+__device__ __forceinline__ float4 bld(__amdgpu_buffer_rsrc_t r, int i) {
+    const f32x4v t = __builtin_amdgcn_raw_buffer_load_b128(r, 0, static_cast<int>(static_cast<uint32_t>(i) << 4), 0);
+    return make_float4(t[0], t[1], t[2], t[3]);
+}
 __device__ __forceinline__ uint32_t cld_u32(const void* p, int i) { return ((cuint32*)p)[i]; }
 
 template <bool KAHAN>

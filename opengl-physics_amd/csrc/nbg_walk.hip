@@ -554,6 +554,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             int e = 0, k = 0, st = 0, c = 0;
             uint64_t em = 0;
             float wuse = 0.f;
+            const __amdgpu_buffer_rsrc_t srec = rec_rsrc(A.spos, A.n_pad + 1);
             auto load_entry = [&](int ee) __attribute__((always_inline)) {
                 const int slot = (h + ee) & (kBinRing - 1);
                 const int2 e2 = qsc[slot];
@@ -584,7 +585,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 }
                 float4 spt[NT];
 #pragma unroll
-                for (int u = 0; u < NT; ++u) spt[u] = cld_off(A.spos, idx[u]);
+                for (int u = 0; u < NT; ++u) spt[u] = bld(srec, idx[u]);
 #pragma unroll
                 for (int u = 0; u < NT; ++u) {
                     const float4 q = spt[u];
