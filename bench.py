@@ -376,6 +376,7 @@ def main():
     targets = np.sort(rng.choice(n, min(n, args.error_sample), replace=False))
     with pynbg.Context(n) as ctx:
         ctx.upload(ic["x"], ic["y"], ic["z"], ic["m"])
+        tile = ctx.tuning().near_tile if kahan else 8
         ref = ctx.direct_targets(pynbg.make_params("direct", G=1.0, eps=args.eps), targets).astype(np.float64)
         err_plain = force_error(ctx, pynbg, ic, plain, targets, ref)
         wall_p, st_p, c0_p, c1_p = run_variant(ctx, pynbg, ic, plain, args.steps, args.warmup)
@@ -395,11 +396,12 @@ def main():
         fm2p = FLOP_M2P_QUAD if multipole == "quadrupole" else FLOP_M2P_MONO
         flops = FLOP_P2P * n_p2p + fm2p * n_m2p
         walk_ms = st.ms_walk / args.steps
+        wtile = tile if (not learn or tile == 16) else 8
         ach = flops / (walk_ms * 1e-3) / 1e12
         return {"bound": "mfma", "pipe": "fp32-valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4),
-                "traffic": kernel_traffic(n, walk_kernel_name(multipole == "quadrupole", kahan, learn, 8)),
-                "kernel": walk_kernel_name(multipole == "quadrupole", kahan, learn, 8), "kernel_ms": round(walk_ms, 4),
+                "traffic": kernel_traffic(n, walk_kernel_name(multipole == "quadrupole", kahan, learn, wtile)),
+                "kernel": walk_kernel_name(multipole == "quadrupole", kahan, learn, wtile), "kernel_ms": round(walk_ms, 4),
                 "flop_per_launch": flops}, n_p2p, n_m2p
 
     roof_p, p2p_p, m2p_p = walk_roofline(st_p, c0_p, c1_p, args.multipole, args.theta)

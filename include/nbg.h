@@ -219,7 +219,8 @@ int nbg_rungs(nbg_ctx* ctx, int32_t* rungs);
 typedef struct nbg_tuning {
     int32_t walk_kernel; /* 0: bin walk (one wavefront per 64 sorted targets), 1: one wavefront
                             per leaf group (no learned-pair listing; learned runs use the bin walk) */
-    int32_t near_tile;   /* bin walk: near-field P2P sources loaded per batch: 4, 8 or 16 */
+    int32_t near_tile;   /* bin walk: near-field P2P sources loaded per batch: 4, 8 or 16 (default
+                            16; the learned walk takes 16 or, for 4 and 8, 8) */
     int32_t direct_tile; /* direct sum: sources loaded per batch: 2, 4 or 8 */
     int32_t reserved;    /* 0 */
 } nbg_tuning;

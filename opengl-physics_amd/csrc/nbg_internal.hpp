@@ -194,7 +194,7 @@ struct nbg_ctx {
     nbg_params graph_params{};
     bool graph_valid = false;
     bool capturing = false;  // inside hipStreamBeginCapture: events become graph nodes
-    nbg_tuning tune{0, 8, 4, 0};  // walk kernel, near P2P tile, direct P2P tile (autotuner)
+    nbg_tuning tune{0, 16, 4, 0};  // walk kernel, near P2P tile, direct P2P tile (autotuner)
 
     // learned far-field correction (allocated on first use)
     float4* nd_mom = nullptr;       // 4 per node: second (6) and third (10) moments about the FP32 com, G*m weights

@@ -929,7 +929,12 @@ void dispatch_walk_bins(nbg_ctx* c, const BinArgs& A, bool quad, bool kahan, boo
         if (learn) {
             // the learned correction is defined against the quadrupole expansion
             if (kahan) {
-                if (count) NBG_WALKB(true, true, true, true, 8); else NBG_WALKB(true, true, false, true, 8);
+                if (count)
+                    NBG_WALKB(true, true, true, true, 8);
+                else if (nt == 16)
+                    NBG_WALKB(true, true, false, true, 16);
+                else
+                    NBG_WALKB(true, true, false, true, 8);
             } else {
                 if (count) NBG_WALKB(true, false, true, true, 8); else NBG_WALKB(true, false, false, true, 8);
             }

@@ -289,7 +289,7 @@ class Context:
         self._check(self._L.nbg_tuning_get(self._h, ctypes.byref(t)), "nbg_tuning_get")
         return t
 
-    def set_tuning(self, walk_kernel=0, near_tile=8, direct_tile=4):
+    def set_tuning(self, walk_kernel=0, near_tile=16, direct_tile=4):
         t = Tuning(int(walk_kernel), int(near_tile), int(direct_tile), 0)
         self._check(self._L.nbg_tuning_set(self._h, ctypes.byref(t)), "nbg_tuning_set")
 
