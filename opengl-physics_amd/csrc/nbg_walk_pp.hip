@@ -35,6 +35,7 @@ constexpr int kPPWaves = 4;
 constexpr int kPPStack = 768;
 constexpr int kPPRing = 128;  // power of two, >= 64 + kPPBatch
 constexpr int kPPBatch = 8;   // nodes classified per traversal step
+constexpr int kPPTile = 16;   // near-field sources loaded per batch (VGPR-resident)
 
 struct PPArgs {
     const int32_t* tlist;   // sorted positions of the targets (null: every particle)
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
             kadd<KAHAN>(ay, cy, sy);
             kadd<KAHAN>(az, cz, sz);
         };
-        // evaluates the particles of `n` staged leaves from ring position `h`, eight per
+        // evaluates the particles of `n` staged leaves from ring position `h`, kPPTile per
         // batch across leaf boundaries; the last batch's unused slots read the zero pad
         // FAST: every pair term is finite (Plummer with eps > 0, or the spline), so a lane
         // weighs a source by its use (1 or 0) instead of selecting, and its own particle
@@ -308,10 +309,10 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
             if (n > 0) load_entry(0);
             int groups = 0;
             while (e < n) {
-                int idx[8];
-                float us[8];
+                int idx[kPPTile];
+                float us[kPPTile];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < kPPTile; ++u) {
                     if (e < n) {
                         idx[u] = st + k;
                         us[u] = wuse;
@@ -325,11 +326,11 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
                         us[u] = 0.f;
                     }
                 }
-                float4 spt[8];
+                float4 spt[kPPTile];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) spt[u] = bld(srec, idx[u]);
+                for (int u = 0; u < kPPTile; ++u) spt[u] = bld(srec, idx[u]);
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < kPPTile; ++u) {
                     const float4 q = spt[u];
                     const float dx = q.x - tp.x, dy = q.y - tp.y, dz = q.z - tp.z;
                     float f;
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
                     sz += f * dz;
                 }
                 // compensated flush every 64 sources
-                if (++groups == 8) {
+                if (++groups == 64 / kPPTile) {
                     groups = 0;
                     kadd<KAHAN>(ax, cx, sx);
                     kadd<KAHAN>(ay, cy, sy);

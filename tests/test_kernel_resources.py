@@ -37,7 +37,7 @@ def kernel_metadata(src, tmp_path):
 @pytest.mark.skipif(not shutil.which(HIPCC) and not os.path.exists(HIPCC), reason="hipcc not available")
 @pytest.mark.parametrize("src,prefix,max_vgpr", [
     ("nbg_walk.hip", "_ZN3nbg11k_walk_bins", 168),
-    ("nbg_walk_pp.hip", "_ZN3nbg9k_walk_pp", 128),
+    ("nbg_walk_pp.hip", "_ZN3nbg9k_walk_pp", 168),
     ("nbg_learn.hip", "_ZN3nbg13k_learn_infer", 256),
 ])
 def test_walk_kernels_use_no_scratch(src, prefix, max_vgpr, tmp_path):
