@@ -1031,7 +1031,6 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             cd[c] = nd_com64[src[c]];
-            cc[c] = nd_com[src[c]];
             ca[c] = nd_aux[src[c]];
             if (QUAD) cq[c] = nd_quad[src[c]];
             if (MOM) {
@@ -1043,7 +1042,10 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
         // branch (one memory round trip per child); pinning them here keeps one wait.
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            pin(cc[c]);
+            // a child's FP32 record is the rounding of its FP64 one (both written below and
+            // by k_up_leaf from the same values), so it is rebuilt instead of gathered
+            cc[c] = make_float4(static_cast<float>(cd[c].x), static_cast<float>(cd[c].y), static_cast<float>(cd[c].z),
+                                static_cast<float>(G * cd[c].w));
             pin(ca[c]);
             if (QUAD) pin(cq[c]);
             if (MOM) {
