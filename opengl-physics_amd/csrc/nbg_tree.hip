@@ -922,15 +922,41 @@ __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ lea
     for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nl; li += gridDim.x * blockDim.x) {
         const int id = leaf_list[li];
         const int st = nd_start[id], cnt = nd_count[id];
+        // A leaf below the depth cap holds at most kNcrit particles: they are loaded once into
+        // registers (all loads in flight together) and the three passes below run on them;
+        // a capped leaf of any size streams its particles from memory in each pass.  Both
+        // forms accumulate in ascending position order, so they give the same bits.
+        const bool small = cnt <= kNcrit;
+        float4 pr[kNcrit];
+        double wr[kNcrit];
+        if (small) {
+            uint32_t pi[kNcrit];
+#pragma unroll
+            for (int u = 0; u < kNcrit; ++u) {
+                const int k = st + min(u, cnt - 1);
+                pr[u] = spos[k];
+                pi[u] = perm[k];
+            }
+#pragma unroll
+            for (int u = 0; u < kNcrit; ++u) wr[u] = mass[pi[u]];
+        }
+        // calls f(p, w) for every member in ascending order
+        auto each = [&](auto f) __attribute__((always_inline)) {
+            if (small) {
+#pragma unroll
+                for (int u = 0; u < kNcrit; ++u)
+                    if (u < cnt) f(pr[u], wr[u]);
+            } else {
+                for (int k = st; k < st + cnt; ++k) f(spos[k], static_cast<double>(mass[perm[k]]));
+            }
+        };
         double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-        for (int k = st; k < st + cnt; ++k) {
-            const float4 p = spos[k];
-            const double w = mass[perm[k]];
+        each([&](const float4& p, double w) __attribute__((always_inline)) {
             M += w;
             sx += w * p.x;
             sy += w * p.y;
             sz += w * p.z;
-        }
+        });
         double gx = __longlong_as_double(0x7ff8000000000000ll), gy = gx, gz = gx;
         if (M != 0.0) {
             gx = sx / M;
@@ -941,8 +967,7 @@ __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ lea
         float qxx = 0.f, qyy = 0.f, qxy = 0.f, qxz = 0.f, qyz = 0.f;
         double rmax = 0.0;
         if (M > 0.0) {
-            for (int k = st; k < st + cnt; ++k) {
-                const float4 p = spos[k];
+            each([&](const float4& p, double) __attribute__((always_inline)) {
                 const double ex = p.x - gx, ey = p.y - gy, ez = p.z - gz;
                 rmax = fmax(rmax, sqrt(ex * ex + ey * ey + ez * ez));
                 if (QUAD && cnt > 1) {
@@ -954,15 +979,14 @@ __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ lea
                     qxz += p.w * 3.f * dx * dz;
                     qyz += p.w * 3.f * dy * dz;
                 }
-            }
+            });
         }
         if (MOM) {
             // second and third moments about the FP32 com, G*m weights (learned correction):
             // [xx yy zz xy] [xz yz xxx yyy] [zzz xxy xxz xyy] [yyz xzz yzz xyz]
             float mo[16] = {};
             if (M > 0.0) {
-                for (int k = st; k < st + cnt; ++k) {
-                    const float4 p = spos[k];
+                each([&](const float4& p, double) __attribute__((always_inline)) {
                     const float dx = p.x - fx, dy = p.y - fy, dz = p.z - fz, w = p.w;
                     mo[0] += w * dx * dx;
                     mo[1] += w * dy * dy;
@@ -980,7 +1004,7 @@ __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ lea
                     mo[13] += w * dx * dz * dz;
                     mo[14] += w * dy * dz * dz;
                     mo[15] += w * dx * dy * dz;
-                }
+                });
             }
             nd_mom[4 * id + 0] = make_float4(mo[0], mo[1], mo[2], mo[3]);
             nd_mom[4 * id + 1] = make_float4(mo[4], mo[5], mo[6], mo[7]);
