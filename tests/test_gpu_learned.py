@@ -310,6 +310,37 @@ def test_listing_walk_forces_equal_plain(gpu_ctx_factory):
     np.testing.assert_array_equal(ctx.accelerations(), plain)
 
 
+def test_learned_walk_tiles_agree(gpu_ctx_factory):
+    """The learned walk comes in two near-field tiles (16, the default, and 8 for tunings
+    of 4 or 8): they list the same pairs and differ only in how the P2P sums are grouped,
+    so with the trained model their forces agree to FP32 summation noise.  The tile-8
+    plain walk and the tile-8 learned walk with a zero model are equal bit for bit."""
+    import pynbg
+    from pynbg import learned
+    s = plummer_f32(16384, 8)
+    ctx = gpu_ctx_factory(16384)
+    ctx.upload(s["x"], s["y"], s["z"], s["m"])
+    ctx.learn_set_weights(learned.load_weights())
+    p = pynbg.make_params("barnes_hut", G=1.0, eps=0.01, theta=0.8, learned=True, learn_band=0.0, learn_amin=1e-3)
+    out = {}
+    for nt in (16, 8):
+        ctx.set_tuning(0, nt, 4)
+        st = ctx.accel(p)
+        out[nt] = (ctx.accelerations().astype(np.float64), st.n_learned)
+    assert out[16][1] == out[8][1] > 0
+    a16, a8 = out[16][0], out[8][0]
+    rel = np.sqrt(np.sum((a16 - a8) ** 2, axis=1)) / np.sqrt(np.sum(a16 ** 2, axis=1))
+    assert rel.max() < 1e-5, rel.max()
+    ctx.set_tuning(0, 8, 4)
+    ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=0.01, theta=0.8))
+    plain8 = ctx.accelerations().copy()
+    zero = learned.init_blob(3)
+    zero[-(64 * 3 + 3):] = 0.0
+    ctx.learn_set_weights(zero)
+    ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=0.01, theta=0.8, learned=True, learn_band=0.0))
+    np.testing.assert_array_equal(ctx.accelerations(), plain8)
+
+
 def test_learned_argument_errors(gpu_ctx_factory):
     import pynbg
     s = plummer_f32(512, 1)
