@@ -13,6 +13,7 @@
 //   orc_ic_plummer          projects/nbody_core/include/ngrav/ic/Plummer.hpp:26-112
 //   orc_ic_hernquist        projects/nbody_core/include/ngrav/ic/Hernquist.hpp:51-181
 //   orc_ic_king             projects/nbody_core/include/ngrav/ic/King.hpp:58-211
+//   orc_king_tidal_radius   projects/nbody_core/include/ngrav/ic/King.hpp:78-106
 //   orc_tree_build          projects/nbody_core/src/Tree.cpp:33-203 (serial insertion)
 //   quadrupoles             projects/nbody_core/src/Tree.cpp:205-254
 //   orc_accel_bh            projects/nbody_core/src/Solvers.cpp:248-355 (grouped walk),
@@ -416,6 +417,227 @@ void orc_ic_plummer(int n, uint32_t seed, double totalMass, double scaleRadius, 
         m[i] = mi;
     }
     double cp[3] = {0, 0, 0}, cv[3] = {0, 0, 0}, tm = 0.0;
+    for (int i = 0; i < n; ++i) {
+        cp[0] += m[i] * x[i];
+        cp[1] += m[i] * y[i];
+        cp[2] += m[i] * z[i];
+        cv[0] += m[i] * vx[i];
+        cv[1] += m[i] * vy[i];
+        cv[2] += m[i] * vz[i];
+        tm += m[i];
+    }
+    for (int k = 0; k < 3; ++k) {
+        cp[k] /= tm;
+        cv[k] /= tm;
+    }
+    for (int i = 0; i < n; ++i) {
+        x[i] -= cp[0];
+        y[i] -= cp[1];
+        z[i] -= cp[2];
+        vx[i] -= cv[0];
+        vy[i] -= cv[1];
+        vz[i] -= cv[2];
+    }
+}
+
+// ic::Hernquist<3> (Hernquist.hpp:131-181) with its Jeans table (Hernquist.hpp:51-127).
+// The table is rebuilt per call (the reference keeps one static instance; the values are
+// the same).  Velocity components: the reference writes Vec<3> v(gg(rng), gg(rng), gg(rng))
+// (Hernquist.hpp:160), whose argument order C++ leaves unspecified; GCC on x86-64 evaluates
+// it right to left, so the first normal draw is v.z, then v.y, then v.x.
+void orc_ic_hernquist(int n, uint32_t seed, double totalMass, double scaleRadius, double G, double* x, double* y,
+                      double* z, double* vx, double* vy, double* vz, double* m) {
+    // HernquistSigmaTable (Hernquist.hpp:69-122): kN = 2048 log-spaced radii on [1e-4, 200]
+    const int kN = 2048;
+    const double kRMin = 1e-4, kRMax = 200.0;
+    const double logMin = std::log(kRMin), logMax = std::log(kRMax);
+    auto rhoUnit = [](double r) {  // Hernquist.hpp:54-57
+        const double rp1 = r + 1.0;
+        return 1.0 / (2.0 * M_PI * r * rp1 * rp1 * rp1);
+    };
+    auto massUnit = [](double r) {  // Hernquist.hpp:58-61
+        const double rp1 = r + 1.0;
+        return r * r / (rp1 * rp1);
+    };
+    std::vector<double> grid(kN + 1), integrand(kN + 1), tailInt(kN + 1, 0.0), sigma2(kN + 1);
+    for (int i = 0; i <= kN; ++i) grid[i] = std::exp(logMin + (logMax - logMin) * i / kN);      // :80-83
+    for (int i = 0; i <= kN; ++i)
+        integrand[i] = rhoUnit(grid[i]) * massUnit(grid[i]) / (grid[i] * grid[i]);           // :84-91
+    for (int i = kN - 1; i >= 0; --i)                                                           // :96-102
+        tailInt[i] = tailInt[i + 1] + 0.5 * (integrand[i] + integrand[i + 1]) * (grid[i + 1] - grid[i]);
+    for (int i = 0; i <= kN; ++i) {                                                             // :103-108
+        const double rho = rhoUnit(grid[i]);
+        sigma2[i] = (rho > 1e-300) ? (tailInt[i] / rho) : 0.0;
+    }
+    auto sigma2At = [&](double r) {  // Sigma2 (Hernquist.hpp:111-117)
+        r = std::max(kRMin, std::min(kRMax, r));
+        const double t = (std::log(r) - logMin) / (logMax - logMin) * kN;
+        const int i0 = std::min(kN - 1, static_cast<int>(t));
+        const double frac = t - i0;
+        return sigma2[i0] * (1.0 - frac) + sigma2[i0 + 1] * frac;
+    };
+
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<double> u(0.0, 1.0);
+    const double mi = totalMass / n;
+    const double vUnit = std::sqrt(G * totalMass / scaleRadius);
+    for (int i = 0; i < n; ++i) {
+        const double x1 = u(rng);                                       // :148-150
+        const double sq = std::sqrt(x1);
+        const double r = std::min(sq / (1.0 - sq), 40.0);
+        const double cosT = 2.0 * u(rng) - 1.0;                         // :152-156
+        const double sinT = std::sqrt(std::max(0.0, 1.0 - cosT * cosT));
+        const double phi = 2.0 * M_PI * u(rng);
+        const double rs = r * scaleRadius;
+        x[i] = sinT * std::cos(phi) * rs;
+        y[i] = sinT * std::sin(phi) * rs;
+        z[i] = cosT * rs;
+        const double sigma = std::sqrt(std::max(0.0, sigma2At(r)));     // :158-161
+        std::normal_distribution<double> gg(0.0, sigma);
+        const double drawZ = gg(rng);
+        const double drawY = gg(rng);
+        const double drawX = gg(rng);
+        vx[i] = drawX * vUnit;
+        vy[i] = drawY * vUnit;
+        vz[i] = drawZ * vUnit;
+        m[i] = mi;
+    }
+    // recentre (Hernquist.hpp:165-179)
+    double cp[3] = {0, 0, 0}, cv[3] = {0, 0, 0}, tm = 0.0;
+    for (int i = 0; i < n; ++i) {
+        cp[0] += m[i] * x[i];
+        cp[1] += m[i] * y[i];
+        cp[2] += m[i] * z[i];
+        cv[0] += m[i] * vx[i];
+        cv[1] += m[i] * vy[i];
+        cv[2] += m[i] * vz[i];
+        tm += m[i];
+    }
+    for (int k = 0; k < 3; ++k) {
+        cp[k] /= tm;
+        cv[k] /= tm;
+    }
+    for (int i = 0; i < n; ++i) {
+        x[i] -= cp[0];
+        y[i] -= cp[1];
+        z[i] -= cp[2];
+        vx[i] -= cv[0];
+        vy[i] -= cv[1];
+        vz[i] -= cv[2];
+    }
+}
+
+namespace {
+// KingG (King.hpp:60-64): g(W) = e^W erf(sqrt W) - sqrt(4W/pi) (1 + 2W/3).
+double OrcKingG(double W) {
+    if (W <= 0.0) return 0.0;
+    const double sq = std::sqrt(W);
+    return std::exp(W) * std::erf(sq) - std::sqrt(4.0 * W / M_PI) * (1.0 + 2.0 * W / 3.0);
+}
+
+// KingProfile (King.hpp:71-164): the RK4 solution of W'' + (2/x) W' = -9 g(W)/g(W0) and the
+// tables derived from it.
+struct OrcKingProfile {
+    double xt = 0.0;
+    std::vector<double> r, cumMassShape, rhoShape, sigma2;
+    OrcKingProfile(double w0, double r0, double G, double totalMass) {
+        const double g0 = OrcKingG(w0);
+        const double h = 2e-3, xEps = 1e-4;                                       // :80-81
+        std::vector<double> xs{0.0, xEps};                                        // :83-85
+        std::vector<double> Ws{w0, w0 - 1.5 * xEps * xEps};
+        double xc = xEps, Wc = Ws.back(), Wpc = -3.0 * xEps;
+        auto dWp = [&](double xx, double WW, double WWp) {                       // :87-90
+            return -9.0 * OrcKingG(WW) / g0 - ((xx > 1e-12) ? (2.0 / xx) * WWp : 0.0);
+        };
+        while (Wc > 0.0 && xs.size() < 200000) {                                  // :93-105
+            const double k1W = Wpc, k1P = dWp(xc, Wc, Wpc);
+            const double k2W = Wpc + h / 2 * k1P, k2P = dWp(xc + h / 2, Wc + h / 2 * k1W, Wpc + h / 2 * k1P);
+            const double k3W = Wpc + h / 2 * k2P, k3P = dWp(xc + h / 2, Wc + h / 2 * k2W, Wpc + h / 2 * k2P);
+            const double k4W = Wpc + h * k3P, k4P = dWp(xc + h, Wc + h * k3W, Wpc + h * k3P);
+            Wc = Wc + (h / 6.0) * (k1W + 2 * k2W + 2 * k3W + k4W);
+            Wpc = Wpc + (h / 6.0) * (k1P + 2 * k2P + 2 * k3P + k4P);
+            xc += h;
+            xs.push_back(xc);
+            Ws.push_back(std::max(Wc, 0.0));
+        }
+        xt = xc;
+        const size_t np = xs.size();                                              // :108-114
+        r.resize(np);
+        rhoShape.resize(np);
+        for (size_t i = 0; i < np; ++i) {
+            r[i] = xs[i] * r0;
+            rhoShape[i] = OrcKingG(Ws[i]) / g0;
+        }
+        cumMassShape.assign(np, 0.0);                                             // :116-121
+        for (size_t i = 1; i < np; ++i) {
+            const double f0 = rhoShape[i - 1] * r[i - 1] * r[i - 1];
+            const double f1 = rhoShape[i] * r[i] * r[i];
+            cumMassShape[i] = cumMassShape[i - 1] + 0.5 * (f0 + f1) * (r[i] - r[i - 1]);
+        }
+        const double totalShape = cumMassShape.back();
+        std::vector<double> massPhys(np), tail(np, 0.0);                          // :126-135
+        for (size_t i = 0; i < np; ++i) massPhys[i] = totalMass * cumMassShape[i] / totalShape;
+        for (size_t i = np - 1; i-- > 0;) {
+            const double ra = r[i], rb = r[i + 1];
+            const double f0 = (ra > 1e-300) ? rhoShape[i] * G * massPhys[i] / (ra * ra) : 0.0;
+            const double f1 = rhoShape[i + 1] * G * massPhys[i + 1] / (rb * rb);
+            tail[i] = tail[i + 1] + 0.5 * (f0 + f1) * (rb - ra);
+        }
+        sigma2.resize(np);                                                        // :136-139
+        for (size_t i = 0; i < np; ++i) sigma2[i] = (rhoShape[i] > 1e-12) ? (tail[i] / rhoShape[i]) : 0.0;
+    }
+    // SampleRadius (King.hpp:144-153)
+    double SampleRadius(double x1) const {
+        const double target = x1 * cumMassShape.back();
+        size_t i1 = static_cast<size_t>(std::lower_bound(cumMassShape.begin(), cumMassShape.end(), target) -
+                                        cumMassShape.begin());
+        i1 = std::min(std::max(i1, size_t{1}), cumMassShape.size() - 1);
+        const size_t i0 = i1 - 1;
+        const double c0 = cumMassShape[i0], c1 = cumMassShape[i1];
+        const double frac = (c1 > c0) ? (target - c0) / (c1 - c0) : 0.0;
+        return r[i0] * (1.0 - frac) + r[i1] * frac;
+    }
+    // Sigma2At (King.hpp:155-163)
+    double Sigma2At(double rr) const {
+        size_t i1 = static_cast<size_t>(std::lower_bound(r.begin(), r.end(), rr) - r.begin());
+        i1 = std::min(std::max(i1, size_t{1}), r.size() - 1);
+        const size_t i0 = i1 - 1;
+        const double frac = (r[i1] > r[i0]) ? (rr - r[i0]) / (r[i1] - r[i0]) : 0.0;
+        return sigma2[i0] * (1.0 - frac) + sigma2[i1] * frac;
+    }
+};
+}  // namespace
+
+// Dimensionless tidal radius x_t of the King(W0) profile (KingProfile::xt, King.hpp:106).
+double orc_king_tidal_radius(double w0) { return OrcKingProfile(w0, 1.0, 1.0, 1.0).xt; }
+
+// ic::King<3> (King.hpp:168-211).  Velocity components in the GCC (right-to-left) order of
+// Vec<3> v(gg(rng), gg(rng), gg(rng)) (King.hpp:190), as in orc_ic_hernquist.
+void orc_ic_king(int n, uint32_t seed, double totalMass, double coreRadius, double w0, double G, double* x,
+                 double* y, double* z, double* vx, double* vy, double* vz, double* m) {
+    const OrcKingProfile profile(w0, coreRadius, G, totalMass);
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<double> u(0.0, 1.0);
+    const double mi = totalMass / n;
+    for (int i = 0; i < n; ++i) {
+        const double r = profile.SampleRadius(u(rng));                  // :180
+        const double cosT = 2.0 * u(rng) - 1.0;                         // :182-186
+        const double sinT = std::sqrt(std::max(0.0, 1.0 - cosT * cosT));
+        const double phi = 2.0 * M_PI * u(rng);
+        x[i] = sinT * std::cos(phi) * r;
+        y[i] = sinT * std::sin(phi) * r;
+        z[i] = cosT * r;
+        const double sigma = std::sqrt(std::max(0.0, profile.Sigma2At(r)));  // :188-191
+        std::normal_distribution<double> gg(0.0, sigma);
+        const double drawZ = gg(rng);
+        const double drawY = gg(rng);
+        const double drawX = gg(rng);
+        vx[i] = drawX;
+        vy[i] = drawY;
+        vz[i] = drawZ;
+        m[i] = mi;
+    }
+    double cp[3] = {0, 0, 0}, cv[3] = {0, 0, 0}, tm = 0.0;              // :195-209
     for (int i = 0; i < n; ++i) {
         cp[0] += m[i] * x[i];
         cp[1] += m[i] * y[i];

@@ -67,9 +67,11 @@ def lib(fast=False):
     L.orc_moments.argtypes = [ctypes.c_int] + [_d] * 7 + [_d] * 3
     L.orc_root_box.argtypes = [ctypes.c_int] + [_d] * 3 + [_d, _d]
     L.orc_morton_keys.argtypes = [ctypes.c_int] + [_d] * 3 + [_d, ctypes.c_double, ctypes.c_int, _u64, _u64]
-    for opt in ("orc_ic_hernquist", "orc_ic_king"):
-        if hasattr(L, opt):
-            pass
+    L.orc_ic_hernquist.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double] + [_d] * 7
+    L.orc_ic_king.argtypes = [ctypes.c_int, ctypes.c_uint32] + [ctypes.c_double] * 4 + [_d] * 7
+    L.orc_king_tidal_radius.restype = ctypes.c_double
+    L.orc_king_tidal_radius.argtypes = [ctypes.c_double]
     _LIBS[name] = L
     return L
 
@@ -84,6 +86,27 @@ def plummer(n, seed, total_mass=1.0, scale_radius=1.0, G=1.0, fast=False):
     lib(fast).orc_ic_plummer(n, seed, total_mass, scale_radius, G,
                              *[_ptr(out[k], _d) for k in ("x", "y", "z", "vx", "vy", "vz", "m")])
     return out
+
+
+def hernquist(n, seed, total_mass=1.0, scale_radius=1.0, G=1.0):
+    """ic::Hernquist<3> in FP64 -> dict of x,y,z,vx,vy,vz,m."""
+    out = {k: np.empty(n, np.float64) for k in ("x", "y", "z", "vx", "vy", "vz", "m")}
+    lib().orc_ic_hernquist(n, seed, total_mass, scale_radius, G,
+                           *[_ptr(out[k], _d) for k in ("x", "y", "z", "vx", "vy", "vz", "m")])
+    return out
+
+
+def king(n, seed, w0=6.0, total_mass=1.0, core_radius=1.0, G=1.0):
+    """ic::King<3> in FP64 -> dict of x,y,z,vx,vy,vz,m."""
+    out = {k: np.empty(n, np.float64) for k in ("x", "y", "z", "vx", "vy", "vz", "m")}
+    lib().orc_ic_king(n, seed, total_mass, core_radius, w0, G,
+                      *[_ptr(out[k], _d) for k in ("x", "y", "z", "vx", "vy", "vz", "m")])
+    return out
+
+
+def king_tidal_radius(w0):
+    """Dimensionless tidal radius x_t where the King(W0) potential W reaches 0."""
+    return float(lib().orc_king_tidal_radius(w0))
 
 
 def uniform_sequence(seed, lo, hi, n):
