@@ -269,7 +269,9 @@ int nbg_learn_get_weights(nbg_ctx* ctx, float* blob, int64_t n);
  * lists its accepted (leaf, cell) pairs, then for each pair (up to max_new) its 13
  * features, the exact residual (FP64 direct sum over the cell minus the walk's multipole
  * force, averaged over the leaf's members, divided by the pair's scale) and a loss weight.
- * *n_pool receives the pool size. */
+ * *n_pool receives the pool size.  The context's positions, velocities and accelerations
+ * are left as they were (the evaluations it makes are scratch), so a primed nbg_step may
+ * follow directly. */
 int nbg_learn_collect(nbg_ctx* ctx, const nbg_params* p, int64_t max_new, int64_t* n_pool);
 int nbg_learn_clear(nbg_ctx* ctx);
 /* Download pool samples [offset, offset+count): feats 16 floats per sample (0..12 the

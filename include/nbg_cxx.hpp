@@ -10,7 +10,7 @@
 //   * nbg::BarnesHut / nbg::Direct
 //                         the argument lists of ngrav::gpu::ComputeAccelGpuBarnesHut /
 //                         ComputeAccelGpuDirect (projects/nbody_core/include/ngrav/gpu/GpuBarnesHut.hpp:40-42,
-//                         GpuDirect.hpp:76-77), writing FP64 accelerations into a SoA-like `out`.
+//                         GpuDirect.hpp:32-33), writing FP64 accelerations into a SoA-like `out`.
 // Errors are reported as std::runtime_error, the reference's style (Scenarios.cpp:296).
 #pragma once
 

@@ -795,6 +795,9 @@ int nbg_step(nbg_ctx* c, const nbg_params* p, int32_t nsteps, nbg_stats* stats) 
         const bool overflow = c->hsc->sticky_overflow != 0 || c->hsc->sticky_learn != 0;
         const bool cap = c->hsc->sticky_cap != 0;
         if (cap) {
+            // steps past the truncated tree are discarded: the caller gets the state it passed in
+            if (int r = restore_backup(c)) return r;
+            c->have_tree = false;
             c->set_error("node pool capacity exceeded");
             return NBG_E_CAPACITY;
         }
@@ -1152,6 +1155,9 @@ int nbg_learn_collect(nbg_ctx* c, const nbg_params* p, int64_t max_new, int64_t*
         if (n_pool) *n_pool = c->pool_n;
         return NBG_OK;
     }
+    // the direct sum and the listing walk below overwrite the context's accelerations:
+    // they are restored at the end, so a(t) of the caller's last evaluation survives
+    if (int r = save_backup(c)) return r;
     // direct-sum accelerations of the snapshot: the loss weights use |a|
     nbg_params d = q;
     d.solver = NBG_SOLVER_DIRECT;
@@ -1172,11 +1178,9 @@ int nbg_learn_collect(nbg_ctx* c, const nbg_params* p, int64_t max_new, int64_t*
             return r;
         c->pool_cap = cap;
     }
-    if (nnew > 0) {
-        launch_learn_samples(c, q, npairs, nnew, c->pool_n);
-        if (int r = check_launch(c)) return r;
-        NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    }
+    if (nnew > 0) launch_learn_samples(c, q, npairs, nnew, c->pool_n);
+    if (int r = restore_backup(c)) return r;
+    NBG_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     c->pool_n += nnew;
     c->have_tree = true;
     if (n_pool) *n_pool = c->pool_n;

@@ -717,7 +717,9 @@ __global__ __launch_bounds__(256) void k_node_finish(const uint64_t* __restrict_
     __shared__ uint32_t s_hist[64];
     if (threadIdx.x < 64) s_hist[threadIdx.x] = 0;
     __syncthreads();
-    const int nn = sc->n_nodes;
+    // A truncated node pool (the host reports NBG_E_CAPACITY) can leave an internal node
+    // without its children and a leaf without its count: write nothing from it.
+    const int nn = sc->cap_exceeded ? 0 : sc->n_nodes;
     for (int base = blockIdx.x * blockDim.x; base < nn; base += gridDim.x * blockDim.x) {
         const int id = base + threadIdx.x;
         if (id >= nn) break;

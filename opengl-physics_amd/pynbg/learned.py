@@ -59,8 +59,8 @@ def snapshot(kind, n, seed):
     return to_f32(ic)
 
 
-def collect(ctx, snapshots, n, eps, theta, band, max_per_snapshot=0, log=print):
-    p = make_params("barnes_hut", "quadrupole", G=1.0, eps=eps, theta=theta, learn_band=band)
+def collect(ctx, snapshots, n, eps, theta, band, max_per_snapshot=0, log=print, amin=0.0):
+    p = make_params("barnes_hut", "quadrupole", G=1.0, eps=eps, theta=theta, learn_band=band, learn_amin=amin)
     total = 0
     for kind, seed in snapshots:
         s = snapshot(kind, n, seed)
@@ -94,6 +94,8 @@ def main(argv=None):
     ap.add_argument("--eps", type=float, default=0.005)
     ap.add_argument("--theta", type=float, default=0.8)
     ap.add_argument("--band", type=float, default=0.6)
+    ap.add_argument("--amin", type=float, default=0.0,
+                    help="pairs also need a correction scale G M s^3 / r^5 >= this (the evaluation's learn_amin)")
     ap.add_argument("--max-per-snapshot", type=int, default=0)
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--batch", type=int, default=8192)
@@ -105,7 +107,7 @@ def main(argv=None):
     t0 = time.time()
     with Context(a.n) as ctx:
         ctx.learn_set_weights(init_blob(a.seed))
-        pool = collect(ctx, snaps, a.n, a.eps, a.theta, a.band, a.max_per_snapshot, log)
+        pool = collect(ctx, snaps, a.n, a.eps, a.theta, a.band, a.max_per_snapshot, log, amin=a.amin)
         ctx.learn_standardize()
         t1 = time.time()
         loss = train(ctx, a.steps, a.batch, a.lr, a.seed, log=log)
@@ -114,7 +116,7 @@ def main(argv=None):
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     np.save(a.out, blob.astype(np.float32), allow_pickle=False)
     meta = dict(samples=int(pool), snapshots=[f"{k}:{s}" for k, s in snaps], n=a.n, eps=a.eps, theta=a.theta,
-                band=a.band, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed,
+                band=a.band, amin=a.amin, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed,
                 loss_first=float(np.mean(loss[:50])), loss_last=float(np.mean(loss[-200:])),
                 collect_s=round(t1 - t0, 2), train_s=round(t2 - t1, 2))
     with open(os.path.splitext(a.out)[0] + ".json", "w") as f:

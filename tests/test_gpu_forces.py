@@ -75,23 +75,6 @@ def test_bh_c1_plummer_65536(gpu_ctx_factory, multipole):
     assert st.n_p2p == int(np.sum(nnear[leaves] * counts[leaves]))
 
 
-def test_bh_theta0_equals_direct(gpu_ctx_factory):
-    """theta=0 opens everything: BH == direct (SelfTest.cpp:43-74 set-up: N=1000 uniform
-    cube, masses 0.5..1.5, eps=0.02)."""
-    import pynbg
-    rng = np.random.default_rng(12345)
-    p = rng.uniform(-1, 1, (1000, 3)).astype(np.float32)
-    m = rng.uniform(0.5, 1.5, 1000).astype(np.float32)
-    ctx = gpu_ctx_factory(1000)
-    ctx.upload(p[:, 0], p[:, 1], p[:, 2], m)
-    ctx.accel(pynbg.make_params("direct", G=1.0, eps=0.02))
-    ad = ctx.accelerations().astype(np.float64)
-    ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=0.02, theta=0.0))
-    ab = ctx.accelerations().astype(np.float64)
-    rel = np.linalg.norm(ab - ad, axis=1) / np.linalg.norm(ad, axis=1)
-    assert rel.max() < 1e-5, rel.max()
-
-
 def test_leapfrog_one_step_c1(gpu_ctx_factory):
     """C1 step: Prime, then one KDK step of dt=1e-3 (Integrator.cpp:9-39) vs the oracle."""
     import oracle

@@ -240,9 +240,16 @@ def _rel_rms(a, ref):
     return float(np.sqrt(np.sum((a - ref) ** 2) / np.sum(ref ** 2)))
 
 
-def test_trained_model_gain_theta08(gpu_ctx_factory):
+@pytest.mark.parametrize("selection", ["band0.6", "shipped"])
+def test_trained_model_gain_theta08(gpu_ctx_factory, selection):
     """The committed model on an unseen Plummer draw (N=65,536, seed 11, eps=0.005):
-    learned theta=0.8 against plain theta=0.8, both judged by the FP32 direct sum."""
+    learned theta=0.8 against plain theta=0.8, both judged by the FP32 direct sum.
+    Selections: the band-0.6 pairs the model was trained on, and the shipped one (every
+    accepted pair whose correction scale G M s^3 / r^5 is at least 1e-3).  The shipped
+    selection halves the plain theta=0.8 error here (measured 1.07e-3 against 2.18e-3);
+    the 1e-3 target itself is asserted at the 1M-body size it is stated for
+    (test_gpu_fullsize.py), since at N=65,536 the per-leaf correction's floor (leaves of
+    2.5x the 1M leaves' size) keeps the learned step just above it."""
     import pynbg
     from pynbg import learned
     if not os.path.exists(WEIGHTS):
@@ -255,13 +262,14 @@ def test_trained_model_gain_theta08(gpu_ctx_factory):
     ctx.learn_set_weights(learned.load_weights(WEIGHTS))
     res = {}
     for name, kw in (("plain 0.8", dict(theta=0.8)), ("plain 0.5", dict(theta=0.5)),
-                     ("learned 0.8", dict(theta=0.8, learned=True, learn_band=0.6))):
+                     ("learned 0.8", dict(theta=0.8, learned=True, learn_band=0.6) if selection == "band0.6" else
+                      dict(theta=0.8, learned=True, learn_band=0.0, learn_amin=1e-3))):
         st = ctx.accel(pynbg.make_params("barnes_hut", G=1.0, eps=0.005, **kw))
         res[name] = _rel_rms(ctx.accelerations().astype(np.float64), ref)
         if "learned" in name:
             assert st.n_learned > 0
-    print({k: f"{v:.3e}" for k, v in res.items()})
-    assert res["learned 0.8"] < 0.8 * res["plain 0.8"]
+    print(selection, {k: f"{v:.3e}" for k, v in res.items()})
+    assert res["learned 0.8"] < (0.8 if selection == "band0.6" else 0.6) * res["plain 0.8"]
 
 
 def test_learned_graph_matches_eager(gpu_ctx_factory):
