@@ -11,15 +11,22 @@ drift, replayed from a captured hipGraph.  Two variants are timed on the same st
            (leaf, cell) pairs whose correction scale G M s^3 / r^5 is at least 1e-3
            (about 0.2% of a typical |a| of this sphere)
 Each variant's force error is its relative RMS against the direct sum on 16,384 random
-targets of the initial state.
+targets of the initial state.  Two more plain steps are timed for the comparison at equal
+error: plain Barnes-Hut at the opening angle whose error matches the learned step's
+(bisected), and plain Barnes-Hut at the learned step's own theta=0.8 (what the network
+costs and what it buys).
 
 Prints one JSON line.  `value` is seconds per step of the learned variant (lower is
-better); `comparison` holds both variants' s/step and errors and the speedup.  `roofline`
+better); `comparison` holds the variants' s/step and errors and the speedups, against
+plain theta=0.5 and against plain Barnes-Hut at equal error.  `stage_hbm` gives the HBM
+rate of the tree and integration stages against their algorithmic bytes.  `roofline`
 is the dominant kernel's algorithmic rate over its HIP-event time inside the timed steps:
 the walk against the FP32 peak (20 flop per P2P, 65 per quadrupole M2P, counted from the
 actual interaction lists) or the MLP against the dense BF16 MFMA peak (18,432 flop per
 pair).  `cpu_baseline` times the FP64 oracle (the reference CPU path restated, OpenMP,
-plain theta=0.5) on a bounded sample of the same workload on this host's cores.
+plain theta=0.5), built with the reference's flags on this host, on a bounded sample of
+the same workload on this host's physical cores (model, sockets, cores and threads
+recorded).
 """
 import argparse
 import json
@@ -51,6 +58,31 @@ def kernel_traffic(n, kernel):
     return d["bytes_per_launch"].get(kernel) if d["n"] == n else None
 
 
+PEAK_HBM_GBS = 8000.0
+SORT_PASSES = 8  # 8-bit digits over the 63-bit key
+
+
+def stage_hbm(st, steps, n):
+    """Per-stage HBM rate of the tree and integration stages from their HIP-event times
+    inside the timed steps, against the algorithmic bytes of each (per particle unless
+    noted; C = nodes): keys 28 (read x, y, z, m; write the 8-byte key and the 4-byte
+    index), radix sort 24 per pass (a 64-bit key and a 32-bit value read and written) x 8
+    passes, build 8 N + 40 C (keys in, node records out) + 32 N (the particle records
+    gathered into sorted order), upsweep 16 N + 56 C (particles in; centre of mass, mass,
+    quadrupole, geometry and radius out), kick-drift-kick 108."""
+    C = float(st.n_nodes)
+    algo = {"keys": 28.0 * n, "sort": 24.0 * SORT_PASSES * n, "build": 8.0 * n + 40.0 * C + 32.0 * n,
+            "upsweep": 16.0 * n + 56.0 * C, "integrate": 108.0 * n}
+    out = {}
+    for k, b in algo.items():
+        ms = getattr(st, "ms_" + k) / steps
+        gbs = b / (ms * 1e-3) / 1e9 if ms > 0 else None
+        out[k] = {"ms": round(ms, 4), "algorithmic_bytes": int(b),
+                  "GB_s": round(gbs, 1) if gbs else None,
+                  "frac_hbm": round(gbs / PEAK_HBM_GBS, 4) if gbs else None}
+    return out
+
+
 def walk_kernel_name(quad, kahan, learn, near_tile, spline=False):
     b = lambda v: "true" if v else "false"
     return f"void k_walk_bins<{b(quad)}, {b(kahan)}, false, {b(learn)}, {near_tile}, {b(spline)}>"
@@ -60,37 +92,103 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(state32, eps, theta, dt, multipole):
-    """One KDK step of the same workload on the FP64 oracle (reference flags build), with the
-    stages of its force evaluation timed apart on the priming evaluation: the serial
-    insertion tree, the quadrupoles (a monopole-only build subtracted) and the group walk."""
+CPU_SHARE = 16  # host threads one GPU's job may use on the benchmark host
+
+
+def host_cpu():
+    """The host's CPU as lscpu reports it, restricted to the CPUs this process may run on:
+    model, sockets, logical CPUs, physical cores, and the OpenMP threads the CPU baseline
+    uses (one per physical core, at most the per-GPU share)."""
+    import subprocess
+    allowed = sorted(os.sched_getaffinity(0))
+    info = {"model": None, "sockets": None, "logical_cpus": len(allowed), "physical_cores": None}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, check=True).stdout
+        for line in txt.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+        par = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+        cores, sockets = set(), set()
+        for line in par.splitlines():
+            if line.startswith("#"):
+                continue
+            cpu, core, sock = (int(v) for v in line.split(",")[:3])
+            if cpu in allowed:
+                cores.add((sock, core))
+                sockets.add(sock)
+        info["physical_cores"] = len(cores)
+        info["sockets"] = len(sockets)
+    except (OSError, ValueError, subprocess.CalledProcessError):
+        pass
+    phys = info["physical_cores"] or len(allowed)
+    info["threads_used"] = min(phys, CPU_SHARE)
+    return info
+
+
+def oracle_for_baseline():
+    """The FP64 oracle built with the reference's flags (-O3 -march=native -funroll-loops
+    -fopenmp, projects/nbody_core/CMakeLists.txt:36-38), compiled on this host so that
+    -march=native means this CPU; the prebuilt x86-64-v3 build if no compiler is here.
+    OpenMP placement is fixed before the library (and its OpenMP runtime) loads:
+    OMP_PROC_BIND=spread, OMP_PLACES=cores, one thread per physical core
+    (scripts/perlmutter_nbody_bench_cpu.sbatch:28-31)."""
+    import subprocess
+    cpu = host_cpu()
+    os.environ["OMP_NUM_THREADS"] = str(cpu["threads_used"])
+    os.environ["OMP_PROC_BIND"] = "spread"
+    os.environ["OMP_PLACES"] = "cores"
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
+    flavour, flags = True, "-O3 -march=x86-64-v3 -funroll-loops -fopenmp (prebuilt)"
+    try:
+        subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle"), "native"], check=True,
+                       capture_output=True, timeout=300)
+        flavour, flags = "native", "-O3 -march=native -funroll-loops -fopenmp (built on this host)"
+    except (OSError, subprocess.SubprocessError) as e:
+        log(f"cpu baseline: no native build ({e}); using the x86-64-v3 build")
+    oracle.lib(flavour)
+    return oracle, flavour, flags, cpu
+
+
+def cpu_baseline(state32, eps, theta, dt, multipole, reps=3):
+    """KDK steps of the same workload on the FP64 oracle, the reference CPU path restated:
+    one warm-up step, then the mean of `reps` timed steps (steady clock), as the reference's
+    --bench harness times (projects/02_nbody_gravity/src/main.cpp:316-336).  The stages of
+    one force evaluation are timed apart: the serial insertion tree, the quadrupoles (a
+    monopole-only build subtracted) and the OpenMP group walk; the kicks and the drift are
+    a step minus its evaluation."""
+    oracle, flavour, flags, cpu = oracle_for_baseline()
     st = {k: np.asarray(v, np.float64) for k, v in state32.items()}
     quad = multipole == "quadrupole"
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     t0 = time.perf_counter()
-    T = oracle.Tree(st["x"], st["y"], st["z"], st["m"], quad=False, fast=True)
+    T = oracle.Tree(st["x"], st["y"], st["z"], st["m"], quad=False, fast=flavour)
     del T
     t1 = time.perf_counter()
-    T = oracle.Tree(st["x"], st["y"], st["z"], st["m"], quad=quad, fast=True)
+    T = oracle.Tree(st["x"], st["y"], st["z"], st["m"], quad=quad, fast=flavour)
     t2 = time.perf_counter()
     a0 = T.accel_bh(1.0, eps * eps, theta)
     t3 = time.perf_counter()
     del T
-    oracle.leapfrog(st, a0, 1.0, eps * eps, theta, dt, 1, quad=quad, fast=True)
-    t4 = time.perf_counter()
+    s1, a1 = oracle.leapfrog(st, a0, 1.0, eps * eps, theta, dt, 1, quad=quad, fast=flavour)  # warm-up step
+    times = []
+    for _ in range(reps):
+        ts = time.perf_counter()
+        s1, a1 = oracle.leapfrog(s1, a1, 1.0, eps * eps, theta, dt, 1, quad=quad, fast=flavour)
+        times.append(time.perf_counter() - ts)
+    step = float(np.mean(times))
     stages = {"tree_build_s": round(t1 - t0, 4), "quadrupoles_s": round(max(t2 - t1 - (t1 - t0), 0.0), 4),
               "walk_s": round(t3 - t2, 4)}
-    stages["leapfrog_kicks_drift_s"] = round(max(t4 - t3 - (t2 - t1) - (t3 - t2), 0.0), 4)
+    stages["leapfrog_kicks_drift_s"] = round(max(step - (t2 - t1) - (t3 - t2), 0.0), 4)
     log(f"cpu baseline: tree {t1 - t0:.2f} s, quadrupoles {stages['quadrupoles_s']:.2f} s, walk {t3 - t2:.2f} s; "
-        f"step {t4 - t3:.2f} s on {threads} threads")
-    return {"value": round(t4 - t3, 4), "unit": "s/step", "cores": threads, "kind": "port",
+        f"step {step:.2f} s (mean of {reps}) on {cpu['threads_used']} threads")
+    return {"value": round(step, 4), "unit": "s/step", "cores": cpu["threads_used"], "kind": "port",
+            "host": cpu, "omp": {"OMP_NUM_THREADS": os.environ["OMP_NUM_THREADS"], "OMP_PROC_BIND": "spread",
+                                 "OMP_PLACES": "cores"},
+            "build": flags, "reps": reps, "warmup": 1, "step_times_s": [round(t, 4) for t in times],
             "stages_of_one_evaluation": stages,
-            "sample": f"one KDK leapfrog step of the full workload (N={len(st['x'])}), FP64 oracle "
-                      f"(serial insertion tree + quadrupoles + OpenMP group walk), after one force "
-                      f"evaluation timed by stage (kicks and drift = the step minus that evaluation's "
-                      f"stages); built -O3 -march=x86-64-v3 -funroll-loops -fopenmp"}
+            "sample": f"KDK leapfrog steps of the full workload (N={len(st['x'])}, theta={theta}), FP64 oracle "
+                      f"(serial insertion tree + quadrupoles + OpenMP group walk, dynamic,32): 1 warm-up step, "
+                      f"then the mean of {reps} timed steps; one force evaluation timed by stage"}
 
 
 def run_variant(ctx, pynbg, ic, base, steps, warmup):
@@ -173,7 +271,7 @@ def ensemble_main(args):
                                    "leaf_groups": cand_ms[3]} if len(cand_ms) == 4 else cand_ms.tolist(),
         "energy_drift": {"max": float(drift.max()), "median": float(np.median(drift)),
                          "over_steps": steps + args.warmup},
-        "roofline": {"bound": "mfma", "pipe": "fp32-valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
+        "roofline": {"bound": "valu", "pipe": "fp32-valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
                      "kernel": "k_walk_bins", "kernel_ms": round(walk_ms, 4), "flop_per_launch": flops},
         "interactions_per_step": {"p2p": n_p2p, "m2p": n_m2p},
@@ -290,29 +388,28 @@ def modes_main(args):
            "config": {"workload": f"plummer_{n}_modes", "n": n, "eps": eps, "theta": theta, "multipole": "quadrupole",
                       "dt": args.dt},
            "modes": out_modes,
-           "roofline": {"bound": "fp32", "achieved": main_mode["tflops"], "peak": PEAK_FP32_TFLOPS,
+           "roofline": {"bound": "valu", "pipe": "fp32-valu", "achieved": main_mode["tflops"], "peak": PEAK_FP32_TFLOPS,
                         "unit": "TFLOP/s", "frac": main_mode["frac_fp32_peak"], "traffic": None}}
     print(json.dumps(out), flush=True)
 
 
 def ensemble_cpu_baseline(off, st, nclusters=64):
     """One KDK step of the first `nclusters` clusters on the FP64 oracle."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    oracle, flavour, flags, cpu = oracle_for_baseline()
     tot = 0.0
     npart = 0
     for k in range(nclusters):
         sl = slice(off[k], off[k + 1])
         s = {q: np.asarray(st[q][sl], np.float64) for q in st}
-        T = oracle.Tree(s["x"], s["y"], s["z"], s["m"], quad=True, fast=True)
+        T = oracle.Tree(s["x"], s["y"], s["z"], s["m"], quad=True, fast=flavour)
         a0 = T.accel_bh(1.0, 0.01 ** 2, 0.6)
         del T
         t0 = time.perf_counter()
-        oracle.leapfrog(s, a0, 1.0, 0.01 ** 2, 0.6, 1e-3, 1, quad=True, fast=True)
+        oracle.leapfrog(s, a0, 1.0, 0.01 ** 2, 0.6, 1e-3, 1, quad=True, fast=flavour)
         tot += time.perf_counter() - t0
         npart += off[k + 1] - off[k]
-    return {"value": round(tot, 4), "unit": "s/step", "cores": threads, "kind": "port",
+    return {"value": round(tot, 4), "unit": "s/step", "cores": cpu["threads_used"], "kind": "port", "host": cpu,
+            "build": flags,
             "sample": f"one KDK step of the first {nclusters} clusters ({npart} particles, "
                       f"{npart / off[-1]:.1%} of the ensemble) on the FP64 oracle, OpenMP"}
 
@@ -389,6 +486,30 @@ def main():
             log(f"learned theta={args.learned_theta} band {args.learn_band}: {1e3 * wall_l / args.steps:.3f} "
                 f"ms/step, rel_rms {err_l:.3e} (plain at that theta {err_l_plain:.3e}); "
                 f"pairs {c0_l.n_learned}")
+            # plain Barnes-Hut at the learned step's error: bisect theta between the plain
+            # and the learned opening angles until the errors agree within 2%
+            plain_at = dict(plain, multipole="quadrupole")
+            lo, hi, e_lo, e_hi = args.theta, args.learned_theta, err_plain, err_l_plain
+            probes = []
+            theta_iso, err_iso = None, None
+            if e_lo <= err_l <= e_hi:
+                for _ in range(14):
+                    mid = 0.5 * (lo + hi)
+                    e = force_error(ctx, pynbg, ic, dict(plain_at, theta=mid), targets, ref)
+                    probes.append([round(mid, 6), e])
+                    theta_iso, err_iso = mid, e
+                    if abs(e - err_l) <= 0.02 * err_l:
+                        break
+                    if e < err_l:
+                        lo = mid
+                    else:
+                        hi = mid
+                wall_iso, st_iso, _, _ = run_variant(ctx, pynbg, ic, dict(plain_at, theta=theta_iso), args.steps,
+                                                     args.warmup)
+                log(f"plain theta={theta_iso:.4f} (equal error {err_iso:.3e}): {1e3 * wall_iso / args.steps:.3f} ms/step")
+            wall_08, st_08, _, _ = run_variant(ctx, pynbg, ic, dict(plain_at, theta=args.learned_theta), args.steps,
+                                               args.warmup)
+            log(f"plain theta={args.learned_theta}: {1e3 * wall_08 / args.steps:.3f} ms/step")
 
     def walk_roofline(st, c0, c1, multipole, theta, learn=False):
         n_p2p = 0.5 * (c0.n_p2p + c1.n_p2p)
@@ -398,7 +519,7 @@ def main():
         walk_ms = st.ms_walk / args.steps
         wtile = tile if (not learn or tile == 16) else 8
         ach = flops / (walk_ms * 1e-3) / 1e12
-        return {"bound": "mfma", "pipe": "fp32-valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
+        return {"bound": "valu", "pipe": "fp32-valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4),
                 "traffic": kernel_traffic(n, walk_kernel_name(multipole == "quadrupole", kahan, learn, wtile)),
                 "kernel": walk_kernel_name(multipole == "quadrupole", kahan, learn, wtile), "kernel_ms": round(walk_ms, 4),
@@ -408,7 +529,8 @@ def main():
     stage_p = {k: round(getattr(st_p, "ms_" + k) / args.steps, 4)
                for k in ("keys", "sort", "build", "upsweep", "walk", "integrate")}
     plain_block = {"theta": args.theta, "s_per_step": round(wall_p / args.steps, 6), "rel_rms_force": err_plain,
-                   "stage_ms_per_step": stage_p, "interactions_per_step": {"p2p": p2p_p, "m2p": m2p_p},
+                   "stage_ms_per_step": stage_p, "stage_hbm": stage_hbm(st_p, args.steps, n),
+                   "interactions_per_step": {"p2p": p2p_p, "m2p": m2p_p},
                    "interactions_per_s": round((p2p_p + m2p_p) / (roof_p["kernel_ms"] * 1e-3), 1),
                    "roofline": roof_p}
     out = {
@@ -431,7 +553,7 @@ def main():
                     "roofline": roof_p, "rel_rms_force": err_plain,
                     "interactions_per_s": plain_block["interactions_per_s"],
                     "interactions_per_step": plain_block["interactions_per_step"],
-                    "stage_ms_per_step": stage_p,
+                    "stage_ms_per_step": stage_p, "stage_hbm": plain_block["stage_hbm"],
                     "tree": {"nodes": st_p.n_nodes, "leaves": st_p.n_leaves, "max_depth": st_p.max_depth}})
     else:
         roof_w, p2p_l, m2p_l = walk_roofline(st_l, c0_l, c1_l, "quadrupole", args.learned_theta, learn=True)
@@ -464,6 +586,17 @@ def main():
                             "stage_ms_per_step": stage_l, "learned_pairs_per_step": pairs,
                             "interactions_per_step": {"p2p": p2p_l, "m2p": m2p_l}},
                 "speedup_vs_plain": round(wall_p / wall_l, 4),
+                "plain_iso_error": None if theta_iso is None else {
+                    "theta": round(theta_iso, 6), "rel_rms_force": err_iso, "s_per_step": round(wall_iso / args.steps, 6),
+                    "speedup_of_learned": round(wall_iso / wall_l, 4),
+                    "stage_ms_per_step": {k: round(getattr(st_iso, "ms_" + k) / args.steps, 4)
+                                          for k in ("keys", "sort", "build", "upsweep", "walk", "integrate")},
+                    "bisection": probes,
+                    "how": f"theta bisected in [{args.theta}, {args.learned_theta}] until the plain relative RMS "
+                           f"error is within 2% of the learned step's, on the same targets"},
+                "plain_at_learned_theta": {"theta": args.learned_theta, "rel_rms_force": err_l_plain,
+                                           "s_per_step": round(wall_08 / args.steps, 6),
+                                           "cost_of_the_network_s_per_step": round((wall_l - wall_08) / args.steps, 6)},
                 "error_target": 1e-3,
                 "error_target_met": bool(err_l <= 1e-3),
                 "error_sample": f"{len(targets)} random targets (seed 12345) of the initial state, judged by the "
@@ -473,6 +606,7 @@ def main():
             "roofline_secondary": secondary,
             "interactions_per_s": round((p2p_l + m2p_l) / (roof_w["kernel_ms"] * 1e-3), 1),
             "stage_ms_per_step": stage_l,
+            "stage_hbm": stage_hbm(st_l, args.steps, n),
             "tree": {"nodes": st_l.n_nodes, "leaves": st_l.n_leaves, "max_depth": st_l.max_depth},
         })
     if not args.no_cpu:

@@ -218,6 +218,13 @@ int accel_eager(nbg_ctx* c, const nbg_params& p, nbg_stats* stats, bool list_onl
             if (int r = learn_grow_pairs(c, c->hsc->learn_pairs)) return r;
             continue;
         }
+        if (learn && !list_only && c->hsc->learn_fix_over && !c->learn_two_pass) {
+            // a correction outside the one-pass fixed-point range: this context sums them
+            // with per-leaf units from now on (graphs are captured again)
+            drop_graph(c);
+            c->learn_two_pass = true;
+            continue;
+        }
         break;
     }
     c->last_levels = levels;
@@ -567,7 +574,7 @@ void nbg_destroy(nbg_ctx* c) {
     free_dev(c->depth_nodes); free_dev(c->cnt_far); free_dev(c->cnt_near);
     free_dev(c->dsc); free_dev(c->diag_part);
     free_dev(c->aold); free_dev(c->rung); free_dev(c->tlist); free_dev(c->list_off); free_dev(c->red_part);
-    free_dev(c->nd_mom); free_dev(c->lcorr); free_dev(c->lpairs); free_dev(c->lw32); free_dev(c->lw16);
+    free_dev(c->nd_mom); free_dev(c->lacc); free_dev(c->lunit); free_dev(c->lpairs); free_dev(c->lpv); free_dev(c->lw32); free_dev(c->lw16);
     free_dev(c->adam_m); free_dev(c->adam_v); free_dev(c->pool_feat); free_dev(c->pool_tgt); free_dev(c->amag2);
     free_dev(c->grad_part); free_dev(c->loss_part); free_dev(c->loss_dev);
     free_dev(c->sys_off); free_dev(c->sys_root); free_dev(c->sys_geom); free_dev(c->sys_bbox);

@@ -50,7 +50,8 @@ struct DevScalars {
     uint32_t work[8];           // per-XCD work counters of the walk
     uint32_t learn_pairs;       // (leaf, cell) pairs listed by the walk for the learned correction
     int32_t sticky_learn;       // the pair list overflowed during a graph batch (host clears)
-    int32_t pad2[2];
+    int32_t learn_fix_e;        // learned correction: fixed-point sums hold v * 2^learn_fix_e
+    int32_t learn_fix_over;     // a correction term exceeded the one-pass fixed-point range
     double dt_step;             // adaptive steps: dt of the current step
     double t_accum;             // adaptive steps: time advanced since the host cleared it
     uint64_t particle_evals;    // block steps: targets evaluated since the host cleared it
@@ -198,8 +199,11 @@ struct nbg_ctx {
 
     // learned far-field correction (allocated on first use)
     float4* nd_mom = nullptr;       // 4 per node: second (6) and third (10) moments about the FP32 com, G*m weights
-    float4* lcorr = nullptr;        // per node: summed correction of a leaf group
+    bool learn_two_pass = false;    // corrections summed with per-leaf fixed-point units (after a range overflow)
+    long long* lacc = nullptr;      // per node, 4 words: a leaf group's correction as exact fixed-point sums
+    uint32_t* lunit = nullptr;      // per node: bits of the largest |component| of its pairs' corrections
     int2* lpairs = nullptr;         // (leaf, cell) pairs listed by the walk
+    float4* lpv = nullptr;          // per listed pair: its correction (x, y, z)
     int64_t lpairs_cap = 0;
     float* lw32 = nullptr;          // FP32 model blob (NBG_LEARN_NPARAM)
     uint16_t* lw16 = nullptr;       // BF16 copy of the trainable part (blob order)

@@ -445,7 +445,9 @@ struct InferArgs {
     float eps;
     const uint16_t* w16;
     const float* w32;    // full blob (standardisation first)
-    float4* corr;
+    float4* pv;          // two-pass sums: per pair, the correction
+    uint32_t* unit;      // two-pass sums: per leaf, largest |component| of its corrections (float bits)
+    long long* acc;      // one-pass sums: per leaf, 4 words of fixed point at the evaluation's unit
 };
 
 constexpr int kInferWaves = 8;
@@ -459,6 +461,7 @@ struct PairRaw {
 // Persistent: one 8-wave block per CU (the 128 KB activation table fills most of the
 // LDS).  A wave takes 64 pairs per step as four 16-pair tiles, two at a time.  The pair
 // ids run two steps ahead and the node words one step ahead of the step computed.
+template <bool TWO_PASS>
 __global__ __launch_bounds__(64 * kInferWaves) void k_learn_infer(InferArgs A) {
     __shared__ __attribute__((aligned(16))) uint16_t s_act[65536];
     __shared__ uint2 s_row[kInferWaves][64][4];
@@ -548,52 +551,192 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_learn_infer(InferArgs A) {
                 o2 = b2;
             }
         }
-        // segmented sum over runs of equal leaf ids (the walk lists a leaf's pairs
-        // together), then one atomic triple per run
-        float v0 = scale * o0, v1 = scale * o1, v2 = scale * o2;
+        const float v0 = scale * o0, v1 = scale * o1, v2 = scale * o2;
         const int prev = __shfl_up(leaf, 1, 64);
         const bool head = lane == 0 || prev != leaf;
         const uint64_t heads = ballot(head);
         const int seg = __popcll(heads & ((2ull << lane) - 1ull));
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const float t0 = __shfl_up(v0, d, 64), t1 = __shfl_up(v1, d, 64), t2 = __shfl_up(v2, d, 64);
-            const int ts = __shfl_up(seg, d, 64);
-            if (lane >= d && ts == seg) {
-                v0 += t0;
-                v1 += t1;
-                v2 += t2;
-            }
-        }
         const int next_seg = __shfl_down(seg, 1, 64);
         const bool tail = lane == 63 || next_seg != seg;
-        if (tail && leaf >= 0) {
-            float* dst = reinterpret_cast<float*>(A.corr + leaf);
-            atomicAdd(dst + 0, v0);
-            atomicAdd(dst + 1, v1);
-            atomicAdd(dst + 2, v2);
+        if (TWO_PASS) {
+            // each pair's correction to its slot; the largest |component| over each run of
+            // equal leaf ids (the walk lists a leaf's pairs together) to the leaf, by one
+            // atomic max per run: the maximum does not depend on how the runs are cut, so
+            // the leaf's fixed-point unit is the same on every run
+            const int64_t q = ch * 64 + lane;
+            if (leaf >= 0) A.pv[q] = make_float4(v0, v1, v2, 0.f);
+            float mx = fmaxf(fabsf(v0), fmaxf(fabsf(v1), fabsf(v2)));
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const float t = __shfl_up(mx, d, 64);
+                const int ts = __shfl_up(seg, d, 64);
+                if (lane >= d && ts == seg) mx = fmaxf(mx, t);
+            }
+            if (tail && leaf >= 0 && mx > 0.f) atomicMax(A.unit + leaf, __float_as_uint(mx));
+        } else {
+            // exact sums in 64-bit fixed point at the evaluation's unit (integer addition is
+            // associative: the sums do not depend on the order in which the walk listed the
+            // pairs), a segmented sum over each run of equal leaf ids, one atomic triple per
+            // run; a term beyond the range flags the evaluation for the two-pass sums
+            const double u = ldexp(1.0, A.sc->learn_fix_e);
+            double t0 = static_cast<double>(v0) * u, t1 = static_cast<double>(v1) * u, t2 = static_cast<double>(v2) * u;
+            const double lim = 0x1p49;
+            if (!(fabs(t0) < lim && fabs(t1) < lim && fabs(t2) < lim)) {
+                atomicOr(&A.scw->learn_fix_over, 1);
+                atomicOr(&A.scw->sticky_learn, 1);
+                t0 = t1 = t2 = 0.0;
+            }
+            long long i0 = __double2ll_rn(t0), i1 = __double2ll_rn(t1), i2 = __double2ll_rn(t2);
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const long long a0 = __shfl_up(i0, d, 64), a1 = __shfl_up(i1, d, 64), a2 = __shfl_up(i2, d, 64);
+                const int ts = __shfl_up(seg, d, 64);
+                if (lane >= d && ts == seg) {
+                    i0 += a0;
+                    i1 += a1;
+                    i2 += a2;
+                }
+            }
+            if (tail && leaf >= 0) {
+                unsigned long long* dst = reinterpret_cast<unsigned long long*>(A.acc + 4 * static_cast<int64_t>(leaf));
+                atomicAdd(dst + 0, static_cast<unsigned long long>(i0));
+                atomicAdd(dst + 1, static_cast<unsigned long long>(i1));
+                atomicAdd(dst + 2, static_cast<unsigned long long>(i2));
+            }
         }
     }
 }
 
-__global__ void k_learn_zero(float4* __restrict__ corr, const DevScalars* __restrict__ sc) {
+// One-pass fixed point: terms are v * 2^e with e = kFixOneBits - (binary exponent of the
+// largest characteristic acceleration G M / h^2 of the batch's systems, h the root cell's
+// half size), so the resolution is 2^-kFixOneBits of that acceleration and a term stays
+// below 2^49 (2^13 terms per leaf fit in 63 bits) for corrections up to 2^18 times it.
+constexpr int kFixOneBits = 30;
+
+// Two-pass fixed point, the fallback when a term leaves the one-pass range: the scale of
+// a leaf whose largest |component| has the float bits `ub` multiplies its values by
+// 2^(kFixBits - e), e the binary exponent of that maximum, so each |term| < 2^(kFixBits + 1)
+// and fewer than 2^21 terms per leaf fit in 63 bits; the resolution is 2^-kFixBits of the
+// largest term (5e-13 relative), far below an FP32 ulp of the sum.  Returns 0 for a leaf
+// without (nonzero) corrections.
+constexpr int kFixBits = 41;
+__device__ __forceinline__ int fix_exp(uint32_t ub) {
+    return (ub == 0u) ? 0 : kFixBits - (ilogb(static_cast<double>(__uint_as_float(ub))));
+}
+
+// Two-pass sums: per-leaf sums of the per-pair corrections, exact in 64-bit fixed point
+// at each leaf's own unit (integer addition is associative, so the sums do not depend on
+// the order in which the persistent walk listed the pairs).
+// A wave takes kSumChunks chunks of 64 pairs per round, all of their loads in flight
+// before the first segmented sum (one chunk at a time waited on two dependent loads).
+constexpr int kSumChunks = 4;
+__global__ __launch_bounds__(256) void k_learn_sum(const int2* __restrict__ pairs, const float4* __restrict__ pv,
+                                                   const uint32_t* __restrict__ unit, const DevScalars* __restrict__ sc,
+                                                   uint32_t cap, long long* __restrict__ acc) {
+    const int lane = lane_id();
+    const uint32_t np = min(sc->learn_pairs, cap);
+    const int64_t nchunks = (static_cast<int64_t>(np) + 63) / 64;
+    const int64_t nw = static_cast<int64_t>(gridDim.x) * (blockDim.x >> 6);
+    for (int64_t ch0 = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); ch0 < nchunks;
+         ch0 += kSumChunks * nw) {
+        int leaf[kSumChunks];
+        float4 v[kSumChunks];
+        uint32_t ub[kSumChunks];
+#pragma unroll
+        for (int k = 0; k < kSumChunks; ++k) {
+            const int64_t q = (ch0 + k * nw) * 64 + lane;
+            leaf[k] = (q < np) ? pairs[q].x : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < kSumChunks; ++k) {
+            const int64_t q = (ch0 + k * nw) * 64 + lane;
+            v[k] = (leaf[k] >= 0) ? pv[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+            ub[k] = (leaf[k] >= 0) ? unit[leaf[k]] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < kSumChunks; ++k) {
+            const int e = fix_exp(ub[k]);
+            long long i0 = __double2ll_rn(ldexp(static_cast<double>(v[k].x), e));
+            long long i1 = __double2ll_rn(ldexp(static_cast<double>(v[k].y), e));
+            long long i2 = __double2ll_rn(ldexp(static_cast<double>(v[k].z), e));
+            const int lf = leaf[k];
+            const int prev = __shfl_up(lf, 1, 64);
+            const bool head = lane == 0 || prev != lf;
+            const uint64_t heads = ballot(head);
+            const int seg = __popcll(heads & ((2ull << lane) - 1ull));
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const long long t0 = __shfl_up(i0, d, 64), t1 = __shfl_up(i1, d, 64), t2 = __shfl_up(i2, d, 64);
+                const int ts = __shfl_up(seg, d, 64);
+                if (lane >= d && ts == seg) {
+                    i0 += t0;
+                    i1 += t1;
+                    i2 += t2;
+                }
+            }
+            const int next_seg = __shfl_down(seg, 1, 64);
+            const bool tail = lane == 63 || next_seg != seg;
+            if (tail && lf >= 0) {
+                unsigned long long* dst = reinterpret_cast<unsigned long long*>(acc + 4 * static_cast<int64_t>(lf));
+                atomicAdd(dst + 0, static_cast<unsigned long long>(i0));
+                atomicAdd(dst + 1, static_cast<unsigned long long>(i1));
+                atomicAdd(dst + 2, static_cast<unsigned long long>(i2));
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_learn_zero(long long* __restrict__ acc, uint32_t* __restrict__ unit,
+                                                    DevScalars* __restrict__ sc, const int32_t* __restrict__ sys_root,
+                                                    const double* __restrict__ sys_geom,
+                                                    const float4* __restrict__ nd_com, int nsys) {
+    if (blockIdx.x == 0) {
+        // the one-pass unit: the largest G M / h^2 over the systems
+        __shared__ float s_a[256];
+        float a = 0.f;
+        for (int k = threadIdx.x; k < nsys; k += blockDim.x) {
+            const double h = sys_geom[4 * k + 3];
+            const float gm = nd_com[sys_root[k]].w;
+            const float ak = static_cast<float>(static_cast<double>(gm) / (h * h));
+            if (ak > a) a = ak;  // a NaN (no mass, no extent) is skipped
+        }
+        s_a[threadIdx.x] = a;
+        __syncthreads();
+        for (int o = 128; o >= 1; o >>= 1) {
+            if (threadIdx.x < o) s_a[threadIdx.x] = fmaxf(s_a[threadIdx.x], s_a[threadIdx.x + o]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            const float am = s_a[0];
+            sc->learn_fix_e = (am > 0.f && am < 3.0e38f) ? kFixOneBits - ilogbf(am) : 0;
+        }
+    }
     const int nn = sc->n_nodes;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x)
-        corr[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
+        reinterpret_cast<longlong2*>(acc)[2 * i] = make_longlong2(0, 0);
+        reinterpret_cast<longlong2*>(acc)[2 * i + 1] = make_longlong2(0, 0);
+        unit[i] = 0u;
+    }
 }
 
 // The walk's accelerations (by sorted position) plus the correction of the particle's
 // leaf, written once to input order.
-__global__ void k_learn_apply(const float4* __restrict__ corr, const int32_t* __restrict__ leaf_of,
+template <bool TWO_PASS>
+__global__ void k_learn_apply(const long long* __restrict__ acc, const uint32_t* __restrict__ unit,
+                              const DevScalars* __restrict__ sc, const int32_t* __restrict__ leaf_of,
                               const uint32_t* __restrict__ perm, const float4* __restrict__ acc_sorted, int n,
                               float* __restrict__ ax, float* __restrict__ ay, float* __restrict__ az) {
+    const int e1 = sc->learn_fix_e;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
         const float4 a = acc_sorted[p];
-        const float4 cr = corr[leaf_of[p]];
+        const int lf = leaf_of[p];
+        const int e = TWO_PASS ? fix_exp(unit[lf]) : e1;
+        const longlong2 c01 = reinterpret_cast<const longlong2*>(acc)[2 * lf];
+        const long long c2 = acc[4 * static_cast<int64_t>(lf) + 2];
         const uint32_t o = perm[p];
-        ax[o] = a.x + cr.x;
-        ay[o] = a.y + cr.y;
-        az[o] = a.z + cr.z;
+        ax[o] = a.x + static_cast<float>(ldexp(static_cast<double>(c01.x), -e));
+        ay[o] = a.y + static_cast<float>(ldexp(static_cast<double>(c01.y), -e));
+        az[o] = a.z + static_cast<float>(ldexp(static_cast<double>(c2), -e));
     }
 }
 
@@ -1055,18 +1198,21 @@ int learn_alloc(nbg_ctx* c) {
     const size_t C = static_cast<size_t>(c->cap_nodes);
     if (!c->nd_mom)
         if (int r = lalloc(c, c->nd_mom, 4 * C)) return r;
-    if (!c->lcorr)
-        if (int r = lalloc(c, c->lcorr, C)) return r;
+    if (!c->lacc)
+        if (int r = lalloc(c, c->lacc, 4 * C)) return r;
+    if (!c->lunit)
+        if (int r = lalloc(c, c->lunit, C)) return r;
     if (!c->amag2)
         if (int r = lalloc(c, c->amag2, static_cast<size_t>(c->cap_n))) return r;
-    if (!c->lpairs) return learn_grow_pairs(c, std::max<int64_t>(int64_t{1} << 20, 32 * c->cap_n));
+    if (!c->lpairs || !c->lpv) return learn_grow_pairs(c, std::max<int64_t>(int64_t{1} << 20, 32 * c->cap_n));
     return NBG_OK;
 }
 
 int learn_grow_pairs(nbg_ctx* c, int64_t need) {
-    if (need <= c->lpairs_cap && c->lpairs) return NBG_OK;
+    if (need <= c->lpairs_cap && c->lpairs && c->lpv) return NBG_OK;
     const int64_t cap = std::min<int64_t>(need + need / 4 + 1024, int64_t{0xffffffffll});
     int r = lalloc(c, c->lpairs, static_cast<size_t>(cap));
+    if (r == NBG_OK) r = lalloc(c, c->lpv, static_cast<size_t>(cap));
     if (r != NBG_OK) {
         c->lpairs_cap = 0;
         return r;
@@ -1076,7 +1222,8 @@ int learn_grow_pairs(nbg_ctx* c, int64_t need) {
 }
 
 void launch_learn_zero(nbg_ctx* c) {
-    hipLaunchKernelGGL(k_learn_zero, dim3(1024), dim3(256), 0, c->stream, c->lcorr, c->dsc);
+    hipLaunchKernelGGL(k_learn_zero, dim3(1024), dim3(256), 0, c->stream, c->lacc, c->lunit, c->dsc, c->sys_root,
+                       c->sys_geom, c->nd_com, c->nsys);
 }
 
 void launch_learn_correct(nbg_ctx* c, const nbg_params& p) {
@@ -1091,14 +1238,26 @@ void launch_learn_correct(nbg_ctx* c, const nbg_params& p) {
     A.eps = static_cast<float>(std::sqrt(p.eps2));
     A.w16 = c->lw16;
     A.w32 = c->lw32;
-    A.corr = c->lcorr;
-    hipLaunchKernelGGL(k_learn_infer, dim3(256), dim3(64 * kInferWaves), 0, c->stream, A);
+    A.pv = c->lpv;
+    A.unit = c->lunit;
+    A.acc = c->lacc;
+    if (c->learn_two_pass) {
+        hipLaunchKernelGGL(k_learn_infer<true>, dim3(256), dim3(64 * kInferWaves), 0, c->stream, A);
+        hipLaunchKernelGGL(k_learn_sum, dim3(2048), dim3(256), 0, c->stream, c->lpairs, c->lpv, c->lunit, c->dsc,
+                           static_cast<uint32_t>(c->lpairs_cap), c->lacc);
+    } else {
+        hipLaunchKernelGGL(k_learn_infer<false>, dim3(256), dim3(64 * kInferWaves), 0, c->stream, A);
+    }
     launch_learn_apply(c);
 }
 
 void launch_learn_apply(nbg_ctx* c) {
-    hipLaunchKernelGGL(k_learn_apply, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->lcorr, c->leaf_of,
-                       c->perm, c->stage4, c->n, c->ax, c->ay, c->az);
+    if (c->learn_two_pass)
+        hipLaunchKernelGGL(k_learn_apply<true>, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->lacc,
+                           c->lunit, c->dsc, c->leaf_of, c->perm, c->stage4, c->n, c->ax, c->ay, c->az);
+    else
+        hipLaunchKernelGGL(k_learn_apply<false>, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->lacc,
+                           c->lunit, c->dsc, c->leaf_of, c->perm, c->stage4, c->n, c->ax, c->ay, c->az);
 }
 
 void launch_learn_amag2(nbg_ctx* c) {

@@ -188,6 +188,7 @@ __global__ void k_reset(DevScalars* __restrict__ sc, uint32_t* __restrict__ bbox
     if (t < 8) sc->work[t] = 0;
     if (t == 0) {
         sc->learn_pairs = 0;
+        sc->learn_fix_over = 0;
         sc->pp_m2p = 0;
         sc->pp_p2p = 0;
         sc->work2 = 0;

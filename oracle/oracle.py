@@ -22,8 +22,10 @@ def _ptr(a, t):
 
 
 def lib(fast=False):
-    """Load liboracle.so (checker build) or liboracle_fast.so (reference flags)."""
-    name = "liboracle_fast.so" if fast else "liboracle.so"
+    """Load liboracle.so (checker build) or, for fast=True, liboracle_fast.so (reference
+    flags with -march=x86-64-v3); fast="native" loads liboracle_native.so (reference flags
+    with -march=native, built on the host that runs it by `make -C oracle native`)."""
+    name = {False: "liboracle.so", True: "liboracle_fast.so", "native": "liboracle_native.so"}[fast]
     if name in _LIBS:
         return _LIBS[name]
     path = os.path.join(_HERE, name)
