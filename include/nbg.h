@@ -292,6 +292,24 @@ int nbg_learn_train(nbg_ctx* ctx, int32_t steps, int32_t batch, float lr, uint64
 int nbg_learn_grad(nbg_ctx* ctx, int64_t n, const float* feats, const float* targets, float* out, float* loss,
                    float* grad);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory:
+ * code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * The direct sum of nbg_accel with NBG_SOLVER_DIRECT (same kernels, same results) on
+ * caller-owned DEVICE arrays of one system: x, y, z, m in, ax, ay, az out (G included),
+ * enqueued on `stream` (a hipStream_t; NULL = the default stream) of HIP device `device`,
+ * which the call makes current; it returns once the work is enqueued.  Uses p->G,
+ * p->eps2, p->softening and NBG_FLAG_KAHAN; p->solver must be NBG_SOLVER_DIRECT.
+ * No context; calls no hipMalloc, hipFree or allocating routine: the packed source
+ * records are staged in a library-wide array of NBG_NOALLOC_MAX_N records (16 MiB) that
+ * each device holds from its first call until the process ends, so
+ * 1 <= n <= NBG_NOALLOC_MAX_N (NBG_E_INVALID otherwise).  Calls whose work overlaps on
+ * one device (two host threads, or two streams) race on that array.
+ * Returns NBG_OK, NBG_E_INVALID (null pointer, n out of range, bad params or device) or
+ * NBG_E_HIP (the symbol lookup or the launch failed). */
+#define NBG_NOALLOC_MAX_N (1 << 20)
+int nbg_accel_direct_noalloc(int device, void* stream, const nbg_params* p, int64_t n, const float* x,
+                             const float* y, const float* z, const float* m, float* ax, float* ay, float* az);
+
 #ifdef __cplusplus
 }
 #endif

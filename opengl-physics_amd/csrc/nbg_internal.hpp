@@ -238,6 +238,8 @@ void launch_direct(nbg_ctx* c, const nbg_params& p);
 void launch_direct_targets(nbg_ctx* c, const nbg_params& p, int64_t nt, const int32_t* targets,
                            const uint32_t* nt_dev, float* out3);
 void launch_pack(nbg_ctx* c, double G);
+void launch_direct_raw(hipStream_t stream, const nbg_params& p, int n, const float* x, const float* y,
+                       const float* z, const float* m, float4* stage, float* ax, float* ay, float* az);
 // walk_pp.hip
 void launch_walk_pp(nbg_ctx* c, const nbg_params& p, const int32_t* tlist, const uint32_t* tcount, float* out3);
 void launch_aold(nbg_ctx* c);

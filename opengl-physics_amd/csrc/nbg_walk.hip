@@ -1070,13 +1070,17 @@ __global__ __launch_bounds__(256) void k_direct(const float4* __restrict__ src, 
         }
         return;
     }
+    // blocks of 2048 targets (the sort tiles of the context's layout); without that layout
+    // (blk_sys NULL) one system of nt_list particles cut into consecutive tiles
     const int sb = blockIdx.x >> 3;
     const int off = (blockIdx.x & 7) * 256 + threadIdx.x;
-    const int sys = blk_sys[sb];
-    const int s0 = sys_off[sys];
-    const int s1 = sys_off[sys + 1];
-    const bool valid = off < blk_count[sb];
-    const int i = blk_start[sb] + off;
+    const bool layout = blk_sys != nullptr;
+    const int sys = layout ? blk_sys[sb] : 0;
+    const int s0 = layout ? sys_off[sys] : 0;
+    const int s1 = layout ? sys_off[sys + 1] : nt_list;
+    const int bstart = layout ? blk_start[sb] : sb * kSortTile;
+    const bool valid = off < (layout ? blk_count[sb] : min(kSortTile, nt_list - bstart));
+    const int i = bstart + off;
     const float4 t = valid ? src[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     float sx = 0.f, sy = 0.f, sz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
     const int sbeg = __builtin_amdgcn_readfirstlane(s0);
@@ -1155,6 +1159,33 @@ void launch_direct(nbg_ctx* c, const nbg_params& p) {
         dispatch_direct<true>(c, p);
     else
         dispatch_direct<false>(c, p);
+}
+
+// One system of n particles given as device arrays: packed into `stage` (n float4
+// records), then the direct sum with the context path's default tile (4 sources per step)
+// into ax, ay, az on `stream`.
+void launch_direct_raw(hipStream_t stream, const nbg_params& p, int n, const float* x, const float* y,
+                       const float* z, const float* m, float4* stage, float* ax, float* ay, float* az) {
+    hipLaunchKernelGGL(k_pack, dim3(grid_for(n, 256)), dim3(256), 0, stream, x, y, z, m, n, static_cast<float>(p.G),
+                       stage);
+    const dim3 grid(((n + kSortTile - 1) / kSortTile) * 8), block(256);
+    const Soft soft = make_soft(p);
+    const bool kahan = (p.flags & NBG_FLAG_KAHAN) != 0;
+#define NBG_DIRECT_RAW(K, S)                                                                                       \
+    hipLaunchKernelGGL((k_direct<K, 4, S>), grid, block, 0, stream, stage, nullptr, nullptr, nullptr, nullptr,     \
+                       nullptr, nullptr, n, nullptr, soft, ax, ay, az, nullptr)
+    if (p.softening == NBG_SOFT_SPLINE) {
+        if (kahan)
+            NBG_DIRECT_RAW(true, true);
+        else
+            NBG_DIRECT_RAW(false, true);
+    } else {
+        if (kahan)
+            NBG_DIRECT_RAW(true, false);
+        else
+            NBG_DIRECT_RAW(false, false);
+    }
+#undef NBG_DIRECT_RAW
 }
 
 void launch_direct_targets(nbg_ctx* c, const nbg_params& p, int64_t nt, const int32_t* targets,

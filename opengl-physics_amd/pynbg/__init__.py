@@ -36,7 +36,7 @@ EXPORTED = ["nbg_create", "nbg_destroy", "nbg_last_error", "nbg_upload", "nbg_up
             "nbg_accel_direct_targets", "nbg_learn_set_weights", "nbg_learn_get_weights", "nbg_learn_collect",
             "nbg_learn_clear", "nbg_learn_pool", "nbg_learn_standardize", "nbg_learn_train", "nbg_learn_grad",
             "nbg_tuning_get", "nbg_tuning_set", "nbg_autotune", "nbg_diagnostics_system", "nbg_accel_bh_targets",
-            "nbg_rungs"]
+            "nbg_rungs", "nbg_accel_direct_noalloc"]
 
 LEARN_FEATURES = 13
 LEARN_HIDDEN = 64
