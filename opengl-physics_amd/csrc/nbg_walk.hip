@@ -470,6 +470,10 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             smask[0] = all;
         }
         wave_sync();
+        // lane j holds leaf group j's sphere (centre of mass, radius) and node id for the
+        // leaf-major group tests (lanes past the bin's leaves hold leaf 0 and are never asked)
+        const float4 my_lc = lcom[lane < nl ? lane : 0];
+        const int my_lid = lid[lane < nl ? lane : 0];
         // leaves whose first particle lies in this bin: a leaf that starts in the previous
         // bin gets its pairs listed there (its MAC decisions are the same in both bins)
         uint64_t own_m = all;
@@ -634,6 +638,8 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             sp -= k;
             wave_sync();
             uint64_t far_m = 0, near_m = 0, open_m = 0, band_m = 0;
+            uint64_t todo = 0;  // groups the sphere shortcuts left undecided for this lane's node
+            float n_sth = 0.f, n_cscale = 0.f, n_sbd = 0.f, n_db2 = 0.f;
             int lstart = 0, lcount = 0;
             float4 cm = make_float4(0.f, 0.f, 0.f, 0.f), au = cm;
             uint32_t info = 0;
@@ -668,6 +674,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                         const float cscale = fabsf(cm.x) + fabsf(cm.y) + fabsf(cm.z);
                         const float bx = cm.x - cbx, by = cm.y - cby, bz = cm.z - cbz;
                         const float db2 = bx * bx + by * by + bz * bz;
+                        n_db2 = db2;
                         // (the learned band's sphere tests keep the correctly rounded root)
                         const float db = LEARN ? sqrtf(db2) : __builtin_amdgcn_sqrtf(db2);
                         const float slack_b = 1e-4f * (db + Rb + sth) + 4e-6f * (cscale + cbscale);
@@ -695,49 +702,69 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                             open_m = act;  // opened by every group
                             todo_all = 0;
                         }
-                        // leaf by leaf, the next leaf's sphere read ahead of the current test
-                        uint64_t todo = todo_all;
-                        int jn = todo ? __ffsll(static_cast<long long>(todo)) - 1 : 0;
-                        float4 lcn = lcom[jn];
-                        while (todo) {
-                            const int jj = jn;
-                            const float4 lc = lcn;
-                            todo &= todo - 1;
-                            jn = todo ? __ffsll(static_cast<long long>(todo)) - 1 : jj;
-                            lcn = lcom[jn];
-                            const uint64_t bit = 1ull << jj;
-                            const float dx = cm.x - lc.x, dy = cm.y - lc.y, dz = cm.z - lc.z;
-                            const float dist2 = dx * dx + dy * dy + dz * dz;
-                            // worst > 0 && size^2 < theta^2 worst^2  <=>  dist > R + size/theta
-                            const float q = lc.w + sth;
-                            bool accept = dist2 > q * q;
-                            const float slack =
-                                2.f * q * (1e-4f * q + 2e-6f * (cscale + fabsf(lc.x) + fabsf(lc.y) + fabsf(lc.z) + lc.w));
-                            if (fabsf(dist2 - q * q) <= slack) {
-                                // FP64 decision exactly as CollectGroupInteractions writes it
-                                // (Solvers.cpp:261-269)
-                                const double4 c64 = A.nd_com64[node];
-                                const double4 g64 = A.nd_com64[lid[jj]];
-                                const double gR64 = A.nd_rmax64[lid[jj]];
-                                const double ex = __dsub_rn(c64.x, g64.x), ey = __dsub_rn(c64.y, g64.y),
-                                             ez = __dsub_rn(c64.z, g64.z);
-                                const double d2 = __dadd_rn(__dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey)),
-                                                            __dmul_rn(ez, ez));
-                                const double size = 2.0 * A.nd_geom[4 * node + 3];
-                                const double w64 = __dsub_rn(__dsqrt_rn(d2), gR64);
-                                accept = (w64 > 0.0) &&
-                                         (__dmul_rn(size, size) < __dmul_rn(__dmul_rn(A.theta2_64, w64), w64));
-                            }
-                            if (accept) {
-                                far_m |= bit;
-                                if (LEARN) {
-                                    const float qb = lc.w + sbd;
-                                    if (dist2 <= qb * qb) band_m |= bit;
-                                }
-                            } else {
-                                open_m |= bit;
-                            }
-                        }
+                        // the groups left undecided are tested below, leaf-major
+                        todo = todo_all;
+                        n_sth = sth;
+                        n_cscale = cscale;
+                        if (LEARN) n_sbd = sbd;
+                    }
+                }
+            }
+            // Leaf-major group tests: one iteration per node left undecided by the sphere
+            // shortcuts, each lane testing its own leaf group against that node, so that an
+            // iteration serves every undecided group of the node (node-major, the wave looped
+            // over the most groups any one node had left, with few lanes busy per trip).
+            for (uint64_t und = ballot(todo != 0); und; und &= und - 1) {
+                const int L = __ffsll(static_cast<long long>(und)) - 1;
+                const float ncx = __shfl(cm.x, L, 64), ncy = __shfl(cm.y, L, 64), ncz = __shfl(cm.z, L, 64);
+                const float nsth = __shfl(n_sth, L, 64), ncsc = __shfl(n_cscale, L, 64);
+                const uint32_t tlo = static_cast<uint32_t>(__shfl(static_cast<int>(todo), L, 64));
+                const uint32_t thi = static_cast<uint32_t>(__shfl(static_cast<int>(todo >> 32), L, 64));
+                const uint64_t tL = (static_cast<uint64_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(thi))) << 32) |
+                                    static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(tlo)));
+                const bool mine = (tL >> lane) & 1ull;
+                const float dx = ncx - my_lc.x, dy = ncy - my_lc.y, dz = ncz - my_lc.z;
+                const float dist2 = dx * dx + dy * dy + dz * dz;
+                // worst > 0 && size^2 < theta^2 worst^2  <=>  dist > R + size/theta
+                const float q = my_lc.w + nsth;
+                bool accept = dist2 > q * q;
+                const float slack = 2.f * q *
+                                    (1e-4f * q + 2e-6f * (ncsc + fabsf(my_lc.x) + fabsf(my_lc.y) + fabsf(my_lc.z) + my_lc.w));
+                const bool close = mine && fabsf(dist2 - q * q) <= slack;
+                if (ballot(close)) {
+                    const int nodeL = __builtin_amdgcn_readfirstlane(__shfl(node, L, 64));
+                    if (close) {
+                        // FP64 decision exactly as CollectGroupInteractions writes it
+                        // (Solvers.cpp:261-269)
+                        const double4 c64 = A.nd_com64[nodeL];
+                        const double4 g64 = A.nd_com64[my_lid];
+                        const double gR64 = A.nd_rmax64[my_lid];
+                        const double ex = __dsub_rn(c64.x, g64.x), ey = __dsub_rn(c64.y, g64.y),
+                                     ez = __dsub_rn(c64.z, g64.z);
+                        const double d2 =
+                            __dadd_rn(__dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey)), __dmul_rn(ez, ez));
+                        const double size = 2.0 * A.nd_geom[4 * nodeL + 3];
+                        const double w64 = __dsub_rn(__dsqrt_rn(d2), gR64);
+                        accept = (w64 > 0.0) && (__dmul_rn(size, size) < __dmul_rn(__dmul_rn(A.theta2_64, w64), w64));
+                    }
+                }
+                const uint64_t acc_b = ballot(mine && accept);
+                uint64_t band_b = 0;
+                if (LEARN) {
+                    const float qb = my_lc.w + __shfl(n_sbd, L, 64);
+                    band_b = ballot(mine && accept && dist2 <= qb * qb);
+                }
+                if (lane == L) {
+                    far_m |= acc_b;
+                    open_m |= tL & ~acc_b;
+                    band_m |= band_b;
+                }
+            }
+            if (node >= 0 && !(info & kLeafBit)) {
+                {
+                    {
+                        const float size = __builtin_amdgcn_sqrtf(au.y);
+                        const float db = LEARN ? sqrtf(n_db2) : 0.f;
                         if (LEARN && band_m && A.amin > 0.f) {
                             // only pairs whose correction scale G M s^3 / r^5 (r = |com_cell -
                             // com_leaf|) reaches amin: by the sphere's bounds on r when they
