@@ -159,10 +159,12 @@ __device__ __forceinline__ void m2p_add(float4 cm, float4 q, float qyz, float tx
     const float dist2 = dx * dx + dy * dy + dz * dz;
     float g = pair_coef<SPLINE>(dist2, cm.w, soft);
     if (QUAD) {
-        const float qxx = q.x, qyy = q.y, qxy = q.z, qxz = q.w, qzz = -(qxx + qyy);
+        const float qxx = q.x, qyy = q.y, qxy = q.z, qxz = q.w;
         const float qdx = qxx * dx + qxy * dy + qxz * dz;
         const float qdy = qxy * dx + qyy * dy + qyz * dz;
-        const float qdz = qxz * dx + qyz * dy + qzz * dz;
+        // Qzz = -(Qxx + Qyy) (traceless), applied term by term: both Q words are
+        // wave-uniform, and their sum would first need one of them moved to a vector register
+        const float qdz = qxz * dx + qyz * dy - qxx * dz - qyy * dz;
         const float qd2 = dx * qdx + dy * qdy + dz * qdz;
         const float ir = __builtin_amdgcn_rsqf(dist2);
         const float ir2 = ir * ir;
