@@ -292,6 +292,20 @@ int nbg_learn_train(nbg_ctx* ctx, int32_t steps, int32_t batch, float lr, uint64
 int nbg_learn_grad(nbg_ctx* ctx, int64_t n, const float* feats, const float* targets, float* out, float* loss,
                    float* grad);
 
+/* Upload with FP64 positions; masses and velocities FP32.  No counterpart in the
+ * reference's GPU path; the reference's own state is FP64 (State.hpp:16-109) and this is
+ * the way to give the device that state without rounding positions to FP32.  The tree is
+ * built from the FP64 positions (root box, keys and centres of mass in FP64, as
+ * Tree.cpp:33-254 does), and the grouped walk forms every pair difference in FP32 from
+ * positions relative to FP64 points near the pair (sources relative to their leaf's
+ * centre, targets and cells' centres of mass relative to the centre of the walk bin's
+ * first leaf, plus the rounded offsets between those points), so no absolute position is
+ * rounded to FP32 on the way to a force.  The direct sum, the per-particle walks and the
+ * learned correction take the FP32 roundings of the positions.  Force evaluations only:
+ * nbg_step returns NBG_E_STATE.  nbg_upload / nbg_upload_batch return to an FP32 state. */
+int nbg_upload_f64(nbg_ctx* ctx, int64_t n, const double* x, const double* y, const double* z, const float* m,
+                   const float* vx, const float* vy, const float* vz);
+
 /* No counterpart in the reference.  For callers that cannot allocate device memory:
  * code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
  * The direct sum of nbg_accel with NBG_SOLVER_DIRECT (same kernels, same results) on

@@ -138,6 +138,7 @@ void enqueue_bh(nbg_ctx* c, const nbg_params& p, int levels, bool timed, bool li
     launch_gather(c, static_cast<float>(p.G));
     if (timed) rec(c, 3);
     launch_upsweep(c, levels, p.multipole, p.G, learn);
+    if (c->f64_state) launch_rel(c);
     if (timed) rec(c, 4);
     if ((p.flags & NBG_FLAG_COUNT) && c->cnt_far) {
         launch_fill_i32(c, c->cnt_far, c->cap_nodes, 0);
@@ -574,6 +575,7 @@ void nbg_destroy(nbg_ctx* c) {
     free_dev(c->depth_nodes); free_dev(c->cnt_far); free_dev(c->cnt_near);
     free_dev(c->dsc); free_dev(c->diag_part);
     free_dev(c->aold); free_dev(c->rung); free_dev(c->tlist); free_dev(c->list_off); free_dev(c->red_part);
+    free_dev(c->x64); free_dev(c->y64); free_dev(c->z64); free_dev(c->sys_bbox64); free_dev(c->srel);
     free_dev(c->nd_mom); free_dev(c->lacc); free_dev(c->lunit); free_dev(c->lpairs); free_dev(c->lpv); free_dev(c->lw32); free_dev(c->lw16);
     free_dev(c->adam_m); free_dev(c->adam_v); free_dev(c->pool_feat); free_dev(c->pool_tgt); free_dev(c->amag2);
     free_dev(c->grad_part); free_dev(c->loss_part); free_dev(c->loss_dev);
@@ -611,6 +613,7 @@ int nbg_upload_batch(nbg_ctx* c, int32_t nsys, const int64_t* offsets, const flo
     }
     NBG_HIP_CHECK(c, hipSetDevice(c->device));
     drop_graph(c);
+    c->f64_state = false;
     c->n = static_cast<int>(n);
     c->nsys = nsys;
     c->h_sys_off.assign(offsets, offsets + nsys + 1);
@@ -699,6 +702,48 @@ int nbg_upload(nbg_ctx* c, int64_t n, const float* x, const float* y, const floa
     return nbg_upload_batch(c, 1, off, x, y, z, m, vx, vy, vz);
 }
 
+int nbg_upload_f64(nbg_ctx* c, int64_t n, const double* x, const double* y, const double* z, const float* m,
+                   const float* vx, const float* vy, const float* vz) {
+    if (!c) return NBG_E_INVALID;
+    if (n < 0 || n > c->cap_n) {
+        c->set_error("particle count exceeds the context capacity");
+        return NBG_E_INVALID;
+    }
+    if (n > 0 && (!x || !y || !z || !m)) {
+        c->set_error("positions and masses are required");
+        return NBG_E_INVALID;
+    }
+    // the FP32 roundings serve every kernel that does not take the FP64 positions
+    std::vector<float> fx(static_cast<size_t>(n)), fy(static_cast<size_t>(n)), fz(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) {
+        fx[static_cast<size_t>(i)] = static_cast<float>(x[i]);
+        fy[static_cast<size_t>(i)] = static_cast<float>(y[i]);
+        fz[static_cast<size_t>(i)] = static_cast<float>(z[i]);
+    }
+    if (int r = nbg_upload(c, n, fx.data(), fy.data(), fz.data(), m, vx, vy, vz)) return r;
+    const size_t cap = static_cast<size_t>(c->cap_n);
+    int r = NBG_OK;
+    if (!c->x64) r |= alloc_into(c, c->x64, cap);
+    if (!c->y64) r |= alloc_into(c, c->y64, cap);
+    if (!c->z64) r |= alloc_into(c, c->z64, cap);
+    r |= alloc_into(c, c->sys_bbox64, 6 * static_cast<size_t>(c->nsys));
+    if (!c->srel) {
+        r |= alloc_into(c, c->srel, cap + kSposPad);
+        // the zero pad after the records (the near field loads whole groups); entries past
+        // n stay finite as n changes
+        if (c->srel) NBG_HIP_CHECK(c, hipMemset(c->srel, 0, (cap + kSposPad) * sizeof(float4)));
+    }
+    if (r != NBG_OK) return r;
+    if (n > 0) {
+        const size_t bytes = static_cast<size_t>(n) * sizeof(double);
+        NBG_HIP_CHECK(c, hipMemcpy(c->x64, x, bytes, hipMemcpyHostToDevice));
+        NBG_HIP_CHECK(c, hipMemcpy(c->y64, y, bytes, hipMemcpyHostToDevice));
+        NBG_HIP_CHECK(c, hipMemcpy(c->z64, z, bytes, hipMemcpyHostToDevice));
+    }
+    c->f64_state = true;
+    return NBG_OK;
+}
+
 int nbg_upload_accel(nbg_ctx* c, const float* ax, const float* ay, const float* az) {
     if (!c) return NBG_E_INVALID;
     if (!c->have_state) {
@@ -736,6 +781,10 @@ int nbg_step(nbg_ctx* c, const nbg_params* p, int32_t nsteps, nbg_stats* stats) 
     if (nsteps < 0) {
         c->set_error("nsteps must be >= 0");
         return NBG_E_INVALID;
+    }
+    if (c->f64_state) {
+        c->set_error("an FP64 state (nbg_upload_f64) takes force evaluations only");
+        return NBG_E_STATE;
     }
     NBG_HIP_CHECK(c, hipSetDevice(c->device));
     if (c->n == 0 || nsteps == 0) {

@@ -129,6 +129,12 @@ struct nbg_ctx {
     float *ax = nullptr, *ay = nullptr, *az = nullptr;
     // backup of the state for the deep-key re-run of a graph batch
     float* backup = nullptr;
+    // FP64 positions (nbg_upload_f64, allocated on first use): the tree is built from them
+    // and the grouped walk differences positions relative to leaf centres (srel)
+    bool f64_state = false;
+    double *x64 = nullptr, *y64 = nullptr, *z64 = nullptr;
+    uint64_t* sys_bbox64 = nullptr;  // 6 per system, order-preserving FP64 bounds
+    float4* srel = nullptr;          // per sorted position: offset from its leaf's centre, G*m (+ zero pad)
 
     // keys and sort
     uint64_t *key_a = nullptr, *key_b = nullptr;   // levels 1..21 (ping-pong)
@@ -238,6 +244,7 @@ void launch_direct(nbg_ctx* c, const nbg_params& p);
 void launch_direct_targets(nbg_ctx* c, const nbg_params& p, int64_t nt, const int32_t* targets,
                            const uint32_t* nt_dev, float* out3);
 void launch_pack(nbg_ctx* c, double G);
+void launch_rel(nbg_ctx* c);  // srel from the FP64 positions and the built tree
 void launch_direct_raw(hipStream_t stream, const nbg_params& p, int n, const float* x, const float* y,
                        const float* z, const float* m, float4* stage, float* ax, float* ay, float* az);
 // walk_pp.hip

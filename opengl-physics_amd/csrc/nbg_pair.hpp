@@ -28,6 +28,11 @@ __device__ __forceinline__ float4 cld(const float4* p, int i) {
     return make_float4(q[0], q[1], q[2], q[3]);
 }
 __device__ __forceinline__ int32_t cld(const int32_t* p, int i) { return ((cint32*)p)[i]; }
+typedef __attribute__((address_space(4))) const double cdouble;
+__device__ __forceinline__ double4 cld(const double4* p, int i) {
+    cdouble* q = (cdouble*)(p + i);
+    return make_double4(q[0], q[1], q[2], q[3]);
+}
 
 // Records of a float4 array through a raw buffer resource: a wave-uniform index goes into
 // the load's SGPR offset (index << 4 in 32 bits: valid for index < 2^28, which nbg_create's

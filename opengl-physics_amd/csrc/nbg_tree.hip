@@ -171,7 +171,8 @@ void excl_scan(hipStream_t st, const T* in, T* out, int64_t n, T* part, T* total
 // ============================================================================ bbox + keys
 
 // Resets the per-call device scalars and each system's bbox accumulators.
-__global__ void k_reset(DevScalars* __restrict__ sc, uint32_t* __restrict__ bbox, int nsys) {
+__global__ void k_reset(DevScalars* __restrict__ sc, uint32_t* __restrict__ bbox, uint64_t* __restrict__ bbox64,
+                        int nsys) {
     const int t = threadIdx.x;
     if (t < 64) {
         sc->depth_count[t] = 0;
@@ -203,22 +204,59 @@ __global__ void k_reset(DevScalars* __restrict__ sc, uint32_t* __restrict__ bbox
         bbox[6 * s + 3] = 0u;
         bbox[6 * s + 4] = 0u;
         bbox[6 * s + 5] = 0u;
+        if (bbox64) {
+            for (int a = 0; a < 3; ++a) {
+                bbox64[6 * s + a] = ~0ull;
+                bbox64[6 * s + 3 + a] = 0ull;
+            }
+        }
     }
 }
 
-// Per sort block (a slice of one system): exact FP32 min/max, one atomic per block.
-__global__ __launch_bounds__(256) void k_bbox(const float* __restrict__ x, const float* __restrict__ y,
-                                              const float* __restrict__ z, const int32_t* __restrict__ blk_sys,
+// Order-preserving map of FP64 values to unsigned 64-bit integers (atomic min/max).
+__device__ __forceinline__ uint64_t d2ord(double d) {
+    const uint64_t b = static_cast<uint64_t>(__double_as_longlong(d));
+    return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double ord2d(uint64_t u) {
+    const uint64_t b = (u & 0x8000000000000000ull) ? (u & 0x7fffffffffffffffull) : ~u;
+    return __longlong_as_double(static_cast<long long>(b));
+}
+// the order-preserving map of a position type and its inverse, widened to FP64
+template <typename T>
+struct PosOrd;
+template <>
+struct PosOrd<float> {
+    typedef uint32_t U;
+    static __device__ __forceinline__ U enc(float v) { return f2ord(v); }
+    static __device__ __forceinline__ double dec(U u) { return static_cast<double>(ord2f(u)); }
+    static constexpr U lo0 = 0xffffffffu, hi0 = 0u;
+};
+template <>
+struct PosOrd<double> {
+    typedef unsigned long long U;
+    static __device__ __forceinline__ U enc(double v) { return d2ord(v); }
+    static __device__ __forceinline__ double dec(U u) { return ord2d(u); }
+    static constexpr U lo0 = ~0ull, hi0 = 0ull;
+};
+
+// Per sort block (a slice of one system): exact min/max of the positions (FP32, or FP64
+// for an FP64 state), one atomic per block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bbox(const T* __restrict__ x, const T* __restrict__ y,
+                                              const T* __restrict__ z, const int32_t* __restrict__ blk_sys,
                                               const int32_t* __restrict__ blk_start,
-                                              const int32_t* __restrict__ blk_count, uint32_t* __restrict__ bbox) {
-    __shared__ uint32_t s_red[6][4];
+                                              const int32_t* __restrict__ blk_count,
+                                              typename PosOrd<T>::U* __restrict__ bbox) {
+    typedef typename PosOrd<T>::U U;
+    __shared__ U s_red[6][4];
     const int b = blockIdx.x;
     const int s = blk_sys[b];
     const int start = blk_start[b], cnt = blk_count[b];
-    uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
+    U lo[3] = {PosOrd<T>::lo0, PosOrd<T>::lo0, PosOrd<T>::lo0}, hi[3] = {PosOrd<T>::hi0, PosOrd<T>::hi0, PosOrd<T>::hi0};
     for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
         const int i = start + k;
-        const uint32_t v[3] = {f2ord(x[i]), f2ord(y[i]), f2ord(z[i])};
+        const U v[3] = {PosOrd<T>::enc(x[i]), PosOrd<T>::enc(y[i]), PosOrd<T>::enc(z[i])};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             lo[a] = min(lo[a], v[a]);
@@ -229,8 +267,8 @@ __global__ __launch_bounds__(256) void k_bbox(const float* __restrict__ x, const
     for (int o = 32; o >= 1; o >>= 1) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            lo[a] = min(lo[a], static_cast<uint32_t>(__shfl_xor(static_cast<int>(lo[a]), o, 64)));
-            hi[a] = max(hi[a], static_cast<uint32_t>(__shfl_xor(static_cast<int>(hi[a]), o, 64)));
+            lo[a] = min(lo[a], static_cast<U>(__shfl_xor(lo[a], o, 64)));
+            hi[a] = max(hi[a], static_cast<U>(__shfl_xor(hi[a], o, 64)));
         }
     }
     const int wave = threadIdx.x >> 6;
@@ -244,7 +282,7 @@ __global__ __launch_bounds__(256) void k_bbox(const float* __restrict__ x, const
     __syncthreads();
     if (threadIdx.x < 3) {
         const int a = threadIdx.x;
-        uint32_t l = s_red[a][0], h = s_red[3 + a][0];
+        U l = s_red[a][0], h = s_red[3 + a][0];
         for (int w = 1; w < 4; ++w) {
             l = min(l, s_red[a][w]);
             h = max(h, s_red[3 + a][w]);
@@ -257,15 +295,16 @@ __global__ __launch_bounds__(256) void k_bbox(const float* __restrict__ x, const
 }
 
 // Root geometry per system, FP64, as Tree.cpp:54-88.
-__global__ void k_root(const uint32_t* __restrict__ bbox, const int32_t* __restrict__ sys_off, int nsys,
+template <typename T>
+__global__ void k_root(const typename PosOrd<T>::U* __restrict__ bbox, const int32_t* __restrict__ sys_off, int nsys,
                        double* __restrict__ geom) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsys) return;
     double lo[3], hi[3];
     const bool empty = sys_off[s + 1] == sys_off[s];
     for (int a = 0; a < 3; ++a) {
-        lo[a] = empty ? 0.0 : static_cast<double>(ord2f(bbox[6 * s + a]));
-        hi[a] = empty ? 0.0 : static_cast<double>(ord2f(bbox[6 * s + 3 + a]));
+        lo[a] = empty ? 0.0 : PosOrd<T>::dec(bbox[6 * s + a]);
+        hi[a] = empty ? 0.0 : PosOrd<T>::dec(bbox[6 * s + 3 + a]);
     }
     double half = 0.5 * (hi[0] - lo[0]);
     half = fmax(half, 0.5 * (hi[1] - lo[1]));
@@ -277,9 +316,9 @@ __global__ void k_root(const uint32_t* __restrict__ bbox, const int32_t* __restr
 
 // Morton key of every particle from the FP64 centre recurrence (see file header).
 // LEVELS = 21 fills the short key; LEVELS = 40 also fills the low word.
-template <int LEVELS>
-__global__ __launch_bounds__(256) void k_keys(const float* __restrict__ x, const float* __restrict__ y,
-                                              const float* __restrict__ z, const int32_t* __restrict__ pos_sys,
+template <int LEVELS, typename T>
+__global__ __launch_bounds__(256) void k_keys(const T* __restrict__ x, const T* __restrict__ y,
+                                              const T* __restrict__ z, const int32_t* __restrict__ pos_sys,
                                               const double* __restrict__ geom, int n, uint64_t* __restrict__ key,
                                               uint64_t* __restrict__ klo, uint32_t* __restrict__ val) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -308,17 +347,35 @@ __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ x, const
 
 void launch_bbox_and_keys(nbg_ctx* c, int levels) {
     hipStream_t st = c->stream;
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, st, c->dsc, c->sys_bbox, c->nsys);
-    hipLaunchKernelGGL(k_bbox, dim3(c->n_blocks), dim3(256), 0, st, c->x, c->y, c->z, c->blk_sys, c->blk_start,
-                       c->blk_count, c->sys_bbox);
-    hipLaunchKernelGGL(k_root, dim3(grid_for(c->nsys, 64)), dim3(64), 0, st, c->sys_bbox, c->sys_off, c->nsys,
+    const bool f64 = c->f64_state;
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, st, c->dsc, c->sys_bbox, f64 ? c->sys_bbox64 : nullptr,
+                       c->nsys);
+    // the root box and the keys from the positions of the state: FP32, or FP64 (Tree.cpp's
+    // own precision) for an FP64 state
+    if (f64) {
+        unsigned long long* bb = reinterpret_cast<unsigned long long*>(c->sys_bbox64);
+        hipLaunchKernelGGL(k_bbox<double>, dim3(c->n_blocks), dim3(256), 0, st, c->x64, c->y64, c->z64, c->blk_sys,
+                           c->blk_start, c->blk_count, bb);
+        hipLaunchKernelGGL(k_root<double>, dim3(grid_for(c->nsys, 64)), dim3(64), 0, st, bb, c->sys_off, c->nsys,
+                           c->sys_geom);
+        if (levels == kLevelsShort)
+            hipLaunchKernelGGL((k_keys<kLevelsShort, double>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x64,
+                               c->y64, c->z64, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
+        else
+            hipLaunchKernelGGL((k_keys<kLevelsDeep, double>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x64,
+                               c->y64, c->z64, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
+        return;
+    }
+    hipLaunchKernelGGL(k_bbox<float>, dim3(c->n_blocks), dim3(256), 0, st, c->x, c->y, c->z, c->blk_sys,
+                       c->blk_start, c->blk_count, c->sys_bbox);
+    hipLaunchKernelGGL(k_root<float>, dim3(grid_for(c->nsys, 64)), dim3(64), 0, st, c->sys_bbox, c->sys_off, c->nsys,
                        c->sys_geom);
     if (levels == kLevelsShort)
-        hipLaunchKernelGGL(k_keys<kLevelsShort>, dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x, c->y, c->z,
-                           c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
+        hipLaunchKernelGGL((k_keys<kLevelsShort, float>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x, c->y,
+                           c->z, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
     else
-        hipLaunchKernelGGL(k_keys<kLevelsDeep>, dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x, c->y, c->z,
-                           c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
+        hipLaunchKernelGGL((k_keys<kLevelsDeep, float>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x, c->y,
+                           c->z, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
 }
 
 // ============================================================================ radix sort
@@ -911,7 +968,10 @@ void launch_build(nbg_ctx* c, int levels) {
 // never uses it as a source, and a massless group then opens every cell, as in the
 // reference.
 
-template <bool QUAD, bool MOM>
+// F64: the positions of an FP64 state (x64, y64, z64, input order) give the FP64 centre
+// of mass and radius, and the quadrupole's offsets from that centre (rounded to FP32);
+// otherwise the FP32 sorted records do.
+template <bool QUAD, bool MOM, bool F64>
 __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ leaf_list,
                                                  const int32_t* __restrict__ nd_start,
                                                  const int32_t* __restrict__ nd_count,
@@ -920,7 +980,8 @@ __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ lea
                                                  const DevScalars* __restrict__ sc, float4* __restrict__ nd_com,
                                                  float4* __restrict__ nd_quad, float4* __restrict__ nd_aux,
                                                  double4* __restrict__ nd_com64, double* __restrict__ nd_rmax64,
-                                                 float4* __restrict__ nd_mom) {
+                                                 float4* __restrict__ nd_mom, const double* __restrict__ x64,
+                                                 const double* __restrict__ y64, const double* __restrict__ z64) {
     const int nl = sc->n_leaves;
     for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nl; li += gridDim.x * blockDim.x) {
         const int id = leaf_list[li];
@@ -932,6 +993,7 @@ __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ lea
         const bool small = cnt <= kNcrit;
         float4 pr[kNcrit];
         double wr[kNcrit];
+        double3 pd[F64 ? kNcrit : 1];
         if (small) {
             uint32_t pi[kNcrit];
 #pragma unroll
@@ -941,24 +1003,32 @@ __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ lea
                 pi[u] = perm[k];
             }
 #pragma unroll
-            for (int u = 0; u < kNcrit; ++u) wr[u] = mass[pi[u]];
+            for (int u = 0; u < kNcrit; ++u) {
+                wr[u] = mass[pi[u]];
+                if (F64) pd[u] = make_double3(x64[pi[u]], y64[pi[u]], z64[pi[u]]);
+            }
         }
-        // calls f(p, w) for every member in ascending order
+        // calls f(p, w, position in FP64) for every member in ascending order
         auto each = [&](auto f) __attribute__((always_inline)) {
             if (small) {
 #pragma unroll
                 for (int u = 0; u < kNcrit; ++u)
-                    if (u < cnt) f(pr[u], wr[u]);
+                    if (u < cnt) f(pr[u], wr[u], F64 ? pd[u] : make_double3(pr[u].x, pr[u].y, pr[u].z));
             } else {
-                for (int k = st; k < st + cnt; ++k) f(spos[k], static_cast<double>(mass[perm[k]]));
+                for (int k = st; k < st + cnt; ++k) {
+                    const float4 p = spos[k];
+                    const uint32_t i = perm[k];
+                    f(p, static_cast<double>(mass[i]), F64 ? make_double3(x64[i], y64[i], z64[i])
+                                                           : make_double3(p.x, p.y, p.z));
+                }
             }
         };
         double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-        each([&](const float4& p, double w) __attribute__((always_inline)) {
+        each([&](const float4&, double w, const double3& q) __attribute__((always_inline)) {
             M += w;
-            sx += w * p.x;
-            sy += w * p.y;
-            sz += w * p.z;
+            sx += w * q.x;
+            sy += w * q.y;
+            sz += w * q.z;
         });
         double gx = __longlong_as_double(0x7ff8000000000000ll), gy = gx, gz = gx;
         if (M != 0.0) {
@@ -970,11 +1040,13 @@ __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ lea
         float qxx = 0.f, qyy = 0.f, qxy = 0.f, qxz = 0.f, qyz = 0.f;
         double rmax = 0.0;
         if (M > 0.0) {
-            each([&](const float4& p, double) __attribute__((always_inline)) {
-                const double ex = p.x - gx, ey = p.y - gy, ez = p.z - gz;
+            each([&](const float4& p, double, const double3& q) __attribute__((always_inline)) {
+                const double ex = q.x - gx, ey = q.y - gy, ez = q.z - gz;
                 rmax = fmax(rmax, sqrt(ex * ex + ey * ey + ez * ez));
                 if (QUAD && cnt > 1) {
-                    const float dx = p.x - fx, dy = p.y - fy, dz = p.z - fz;
+                    const float dx = F64 ? static_cast<float>(ex) : p.x - fx;
+                    const float dy = F64 ? static_cast<float>(ey) : p.y - fy;
+                    const float dz = F64 ? static_cast<float>(ez) : p.z - fz;
                     const float d2 = dx * dx + dy * dy + dz * dz;
                     qxx += p.w * (3.f * dx * dx - d2);
                     qyy += p.w * (3.f * dy * dy - d2);
@@ -989,7 +1061,7 @@ __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ lea
             // [xx yy zz xy] [xz yz xxx yyy] [zzz xxy xxz xyy] [yyz xzz yzz xyz]
             float mo[16] = {};
             if (M > 0.0) {
-                each([&](const float4& p, double) __attribute__((always_inline)) {
+                each([&](const float4& p, double, const double3&) __attribute__((always_inline)) {
                     const float dx = p.x - fx, dy = p.y - fy, dz = p.z - fz, w = p.w;
                     mo[0] += w * dx * dx;
                     mo[1] += w * dy * dy;
@@ -1036,7 +1108,9 @@ __device__ __forceinline__ void pin(float4& v) {
 // loads are in flight together (behind per-child branches they were 40-odd dependent
 // memory round trips per node, 7 us for the root alone).  The selects keep the arithmetic
 // of the skip-by-branch form term for term.
-template <bool QUAD, bool MOM>
+// F64: the offsets of the children's centres of mass (quadrupole shifts and radius) are
+// taken in FP64 before rounding (an FP64 state); otherwise from their FP32 roundings.
+template <bool QUAD, bool MOM, bool F64>
 __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* __restrict__ depth_nodes,
                                                      const int32_t* __restrict__ nd_child,
                                                      double G, const DevScalars* __restrict__ sc,
@@ -1103,7 +1177,9 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             const bool live = massive && ch[c] >= 0;
-            const float dx = cc[c].x - fx, dy = cc[c].y - fy, dz = cc[c].z - fz;
+            const float dx = F64 ? static_cast<float>(cd[c].x - gx) : cc[c].x - fx;
+            const float dy = F64 ? static_cast<float>(cd[c].y - gy) : cc[c].y - fy;
+            const float dz = F64 ? static_cast<float>(cd[c].z - gz) : cc[c].z - fz;
             const float d2 = dx * dx + dy * dy + dz * dz;
             rmax = live ? fmaxf(rmax, ca[c].z + sqrtf(d2)) : rmax;
             const bool heavy = live && cc[c].w > 0.f;
@@ -1162,35 +1238,63 @@ void launch_upsweep(nbg_ctx* c, int levels, int multipole, double G, bool moment
     const int gnodes = 2048;
     const bool quad = multipole == NBG_QUADRUPOLE;
     const bool mom = moments && c->nd_mom != nullptr;
-#define NBG_UPLEAF(Q, M)                                                                                             \
-    hipLaunchKernelGGL((k_up_leaf<Q, M>), dim3(gnodes), dim3(256), 0, st, c->leaf_list, c->nd_start, c->nd_count,     \
+#define NBG_UPLEAF(Q, M, F)                                                                                          \
+    hipLaunchKernelGGL((k_up_leaf<Q, M, F>), dim3(gnodes), dim3(256), 0, st, c->leaf_list, c->nd_start, c->nd_count,  \
                        c->spos, c->m, c->perm, G, c->dsc, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64,            \
-                       c->nd_rmax64, c->nd_mom)
+                       c->nd_rmax64, c->nd_mom, c->x64, c->y64, c->z64)
     if (mom)
-        NBG_UPLEAF(true, true);
+        NBG_UPLEAF(true, true, false);
+    else if (quad && c->f64_state)
+        NBG_UPLEAF(true, false, true);
     else if (quad)
-        NBG_UPLEAF(true, false);
+        NBG_UPLEAF(true, false, false);
+    else if (c->f64_state)
+        NBG_UPLEAF(false, false, true);
     else
-        NBG_UPLEAF(false, false);
+        NBG_UPLEAF(false, false, false);
 #undef NBG_UPLEAF
     // internal nodes sit at depths 0..levels-1 (depth-40 cells are always leaves)
     const int top = (levels == kLevelsShort) ? kLevelsShort - 1 : kMaxDepth - 1;
     for (int d = top; d >= 0; --d) {
         const int gd = d < 6 ? 64 : 1024;
-#define NBG_UPINT(Q, M)                                                                                              \
-    hipLaunchKernelGGL((k_up_internal<Q, M>), dim3(gd), dim3(256), 0, st, d, c->depth_nodes, c->nd_child, G, c->dsc, \
-                       c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64, c->nd_mom)
+#define NBG_UPINT(Q, M, F)                                                                                           \
+    hipLaunchKernelGGL((k_up_internal<Q, M, F>), dim3(gd), dim3(256), 0, st, d, c->depth_nodes, c->nd_child, G,       \
+                       c->dsc, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64, c->nd_mom)
         if (mom)
-            NBG_UPINT(true, true);
+            NBG_UPINT(true, true, false);
+        else if (quad && c->f64_state)
+            NBG_UPINT(true, false, true);
         else if (quad)
-            NBG_UPINT(true, false);
+            NBG_UPINT(true, false, false);
+        else if (c->f64_state)
+            NBG_UPINT(false, false, true);
         else
-            NBG_UPINT(false, false);
+            NBG_UPINT(false, false, false);
 #undef NBG_UPINT
     }
 }
 
 // gather is part of the build step in the driver (needs G)
+// Offsets of the sorted particles from their leaf's FP64 centre, rounded to FP32, with G*m:
+// the grouped walk of an FP64 state differences these (plus FP32 offsets between leaf
+// centres), so a pair difference never rounds an absolute position.
+__global__ __launch_bounds__(256) void k_rel(const int32_t* __restrict__ leaf_of, const uint32_t* __restrict__ perm,
+                                             const double* __restrict__ nd_geom, const float4* __restrict__ spos,
+                                             const double* __restrict__ x64, const double* __restrict__ y64,
+                                             const double* __restrict__ z64, int n, float4* __restrict__ srel) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int lf = leaf_of[p];
+    const uint32_t i = perm[p];
+    srel[p] = make_float4(static_cast<float>(x64[i] - nd_geom[4 * lf + 0]), static_cast<float>(y64[i] - nd_geom[4 * lf + 1]),
+                          static_cast<float>(z64[i] - nd_geom[4 * lf + 2]), spos[p].w);
+}
+
+void launch_rel(nbg_ctx* c) {
+    hipLaunchKernelGGL(k_rel, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->leaf_of, c->perm, c->nd_geom,
+                       c->spos, c->x64, c->y64, c->z64, c->n, c->srel);
+}
+
 void launch_gather(nbg_ctx* c, float G) {
     hipLaunchKernelGGL(k_pack4, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->x, c->y, c->z, c->m, c->n, G,
                        c->stage4);

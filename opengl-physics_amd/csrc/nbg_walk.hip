@@ -356,6 +356,7 @@ struct BinArgs {
     const int32_t* sys_root;
     const int32_t* pos_sys;
     const float4* spos;
+    const float4* srel;  // FP64 state: sorted positions relative to their leaf's centre
     const uint32_t* perm;
     DevScalars* sc;
     float* ax;
@@ -394,7 +395,12 @@ __device__ __forceinline__ int wave_excl_small(int v, int* total) {
     return off;
 }
 
-template <bool QUAD, bool KAHAN, bool COUNT, bool LEARN, int NT, bool SPL>
+// REL (an FP64 state): every difference a force takes is formed in FP32 from positions
+// relative to FP64 points near the pair -- targets relative to the centre of their bin's
+// first leaf (cb), sources relative to their own leaf's centre, cells' centres of mass
+// relative to cb -- plus the FP32 rounding of the offset between those FP64 points, so no
+// absolute position is rounded to FP32.  Near leaves are then evaluated leaf by leaf.
+template <bool QUAD, bool KAHAN, bool COUNT, bool LEARN, int NT, bool SPL, bool REL>
 __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     __shared__ int s_snode[kBinWaves][kBinStack];
     __shared__ uint64_t s_smask[kBinWaves][kBinStack];
@@ -402,6 +408,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     __shared__ __attribute__((aligned(16))) uint64_t s_fmask[kBinWaves][kBinRing];
     __shared__ int2 s_qsc[kBinWaves][kBinRing];  // near leaf: first sorted particle, count
     __shared__ uint64_t s_qmask[kBinWaves][kBinRing];
+    __shared__ int s_qnd[kBinWaves][REL ? kBinRing : 1];  // REL: node of each staged near leaf
     __shared__ float4 s_lcom[kBinWaves][64];
     __shared__ int s_lid[kBinWaves][64];
     __shared__ int s_next[kBinWaves];
@@ -414,6 +421,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     uint64_t* fmask = s_fmask[wave];
     int2* qsc = s_qsc[wave];
     uint64_t* qmask = s_qmask[wave];
+    int* qnd = s_qnd[wave];
     float4* lcom = s_lcom[wave];
     int* lid = s_lid[wave];
     const Soft soft = A.soft;
@@ -448,7 +456,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
         const int cnt = cld(A.bin_count, b);
         const bool tv = lane < cnt;
         const int tpos = start + (tv ? lane : 0);
-        const float4 tp = A.spos[tpos];
+        float4 tp = A.spos[tpos];
         const int mylf = tv ? A.leaf_of[tpos] : -1;
         const int prevlf = (tv && lane > 0) ? A.leaf_of[tpos - 1] : -2;
         const uint64_t first = ballot(tv && mylf != prevlf);
@@ -463,6 +471,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             // the leaf's own members (self excluded when evaluated)
             qsc[j] = make_int2(A.nd_start[mylf], static_cast<int>(__float_as_uint(A.nd_aux[mylf].w) & ~kLeafBit));
             qmask[j] = 1ull << j;
+            if (REL) qnd[j] = mylf;
         }
         const int root = cld(A.sys_root, cld(A.pos_sys, start));
         if (lane == 0) {
@@ -474,6 +483,22 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
         // leaf-major group tests (lanes past the bin's leaves hold leaf 0 and are never asked)
         const float4 my_lc = lcom[lane < nl ? lane : 0];
         const int my_lid = lid[lane < nl ? lane : 0];
+        // REL: the bin's reference point cb (its first leaf's FP64 centre) and each target
+        // relative to it, from its offset in its own leaf plus the FP64 offset of the leaves
+        double rbx = 0.0, rby = 0.0, rbz = 0.0;
+        if (REL) {
+            const double4 g0 = cld(reinterpret_cast<const double4*>(A.nd_geom), __builtin_amdgcn_readfirstlane(lid[0]));
+            rbx = g0.x;
+            rby = g0.y;
+            rbz = g0.z;
+            if (tv) {
+                const float4 r = A.srel[tpos];
+                const double4 gl = reinterpret_cast<const double4*>(A.nd_geom)[mylf];
+                tp.x = static_cast<float>((gl.x - rbx) + static_cast<double>(r.x));
+                tp.y = static_cast<float>((gl.y - rby) + static_cast<double>(r.y));
+                tp.z = static_cast<float>((gl.z - rbz) + static_cast<double>(r.z));
+            }
+        }
         // leaves whose first particle lies in this bin: a leaf that starts in the previous
         // bin gets its pairs listed there (its MAC decisions are the same in both bins)
         uint64_t own_m = all;
@@ -530,6 +555,13 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     cm[u] = cld(A.nd_com, nd[u]);
+                    if (REL) {
+                        // the centre of mass relative to cb: an FP64 difference, rounded once
+                        const double4 c64 = cld(A.nd_com64, nd[u]);
+                        cm[u].x = static_cast<float>(c64.x - rbx);
+                        cm[u].y = static_cast<float>(c64.y - rby);
+                        cm[u].z = static_cast<float>(c64.z - rbz);
+                    }
                     q[u] = cld(A.nd_quad, nd[u]);
                     qyz[u] = __uint_as_float(cld_u32(A.nd_aux, 4 * nd[u]));
                 }
@@ -618,9 +650,59 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             kadd<KAHAN>(ay, cy, sy);
             kadd<KAHAN>(az, cz, sz);
         };
+        // REL: the near leaves one at a time -- the target's offset from the leaf's centre
+        // (tp minus the FP32 offset of that centre from cb) against the sources' offsets
+        // from the same centre; the target itself and the pad slots are dropped by select
+        auto eval_near_rel = [&](int h, int n) __attribute__((always_inline)) {
+            float sx = 0.f, sy = 0.f, sz = 0.f;
+            const __amdgpu_buffer_rsrc_t srec = rec_rsrc(A.srel, A.n_pad + 1);
+            for (int e = 0; e < n; ++e) {
+                const int slot = (h + e) & (kBinRing - 1);
+                const int2 e2 = qsc[slot];
+                const int st = __builtin_amdgcn_readfirstlane(e2.x);
+                const int c = __builtin_amdgcn_readfirstlane(e2.y);
+                const int nd = __builtin_amdgcn_readfirstlane(qnd[slot]);
+                const bool use = (qmask[slot] & mybit) != 0;
+                const double4 g = cld(reinterpret_cast<const double4*>(A.nd_geom), nd);
+                const float txr = tp.x - static_cast<float>(g.x - rbx);
+                const float tyr = tp.y - static_cast<float>(g.y - rby);
+                const float tzr = tp.z - static_cast<float>(g.z - rbz);
+                for (int k0 = 0; k0 < c; k0 += 8) {
+                    int idx[8];
+                    float4 spt[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        idx[u] = (k0 + u < c) ? st + k0 + u : A.n_pad;
+                        spt[u] = bld(srec, idx[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const float dx = spt[u].x - txr, dy = spt[u].y - tyr, dz = spt[u].z - tzr;
+                        const bool take = use && k0 + u < c && idx[u] != tpos;
+                        float f = pair_coef<SPL>(dx * dx + dy * dy + dz * dz, spt[u].w, soft);
+                        f = take ? f : 0.f;
+                        if (COUNT) n_near += take;
+                        sx += f * dx;
+                        sy += f * dy;
+                        sz += f * dz;
+                    }
+                }
+                if ((e & 7) == 7) {
+                    kadd<KAHAN>(ax, cx, sx);
+                    kadd<KAHAN>(ay, cy, sy);
+                    kadd<KAHAN>(az, cz, sz);
+                    sx = sy = sz = 0.f;
+                }
+            }
+            kadd<KAHAN>(ax, cx, sx);
+            kadd<KAHAN>(ay, cy, sy);
+            kadd<KAHAN>(az, cz, sz);
+        };
         const bool near_fast = SPL || soft.eps2 > 0.f;
         auto eval_near = [&](int h, int n) __attribute__((always_inline)) {
-            if (near_fast)
+            if (REL)
+                eval_near_rel(h, n);
+            else if (near_fast)
                 eval_near_t(h, n, std::true_type{});
             else
                 eval_near_t(h, n, std::false_type{});
@@ -836,6 +918,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                     const int slot = (q_tail + mbcnt(bm)) & (kBinRing - 1);
                     qsc[slot] = make_int2(lstart, lcount);
                     qmask[slot] = near_m;
+                    if (REL) qnd[slot] = node;
                 }
                 q_tail += __popcll(bm);
             }
@@ -910,7 +993,7 @@ void launch_walk_leaf(nbg_ctx* c, const nbg_params& p) {
     A.sys_root = c->sys_root;
     A.pos_sys = c->pos_sys;
     A.spos = c->spos;
-    A.perm = c->perm;
+        A.perm = c->perm;
     A.sc = c->dsc;
     A.ax = c->ax;
     A.ay = c->ay;
@@ -950,8 +1033,20 @@ template <bool SPL>
 void dispatch_walk_bins(nbg_ctx* c, const BinArgs& A, bool quad, bool kahan, bool count, bool learn) {
     // 8 XCDs x 32 CUs x 3 resident blocks
     const dim3 grid(8 * 32 * 3), block(64 * kBinWaves);
-#define NBG_WALKB(Q, K, C, L, T) hipLaunchKernelGGL((k_walk_bins<Q, K, C, L, T, SPL>), grid, block, 0, c->stream, A)
+#define NBG_WALKB(Q, K, C, L, T) hipLaunchKernelGGL((k_walk_bins<Q, K, C, L, T, SPL, false>), grid, block, 0, c->stream, A)
+#define NBG_WALKR(Q, K, C) hipLaunchKernelGGL((k_walk_bins<Q, K, C, false, 8, SPL, true>), grid, block, 0, c->stream, A)
     const int nt = c->tune.near_tile;
+    if (c->f64_state && !learn) {
+        // an FP64 state: the relative-difference walk (its near field does not take the tile)
+        if (quad) {
+            if (count) { if (kahan) NBG_WALKR(true, true, true); else NBG_WALKR(true, false, true); }
+            else { if (kahan) NBG_WALKR(true, true, false); else NBG_WALKR(true, false, false); }
+        } else {
+            if (count) { if (kahan) NBG_WALKR(false, true, true); else NBG_WALKR(false, false, true); }
+            else { if (kahan) NBG_WALKR(false, true, false); else NBG_WALKR(false, false, false); }
+        }
+        return;
+    }
     if constexpr (!SPL) {
         if (learn) {
             // the learned correction is defined against the quadrupole expansion
@@ -989,6 +1084,7 @@ void dispatch_walk_bins(nbg_ctx* c, const BinArgs& A, bool quad, bool kahan, boo
         if (quad) NBG_WALKB(true, false, false, false, 8); else NBG_WALKB(false, false, false, false, 8);
     }
 #undef NBG_WALKB
+#undef NBG_WALKR
 }
 
 void launch_walk_bins(nbg_ctx* c, const nbg_params& p) {
@@ -1009,6 +1105,7 @@ void launch_walk_bins(nbg_ctx* c, const nbg_params& p) {
     A.sys_root = c->sys_root;
     A.pos_sys = c->pos_sys;
     A.spos = c->spos;
+    A.srel = c->srel;
     A.perm = c->perm;
     A.sc = c->dsc;
     A.ax = c->ax;

@@ -36,7 +36,7 @@ EXPORTED = ["nbg_create", "nbg_destroy", "nbg_last_error", "nbg_upload", "nbg_up
             "nbg_accel_direct_targets", "nbg_learn_set_weights", "nbg_learn_get_weights", "nbg_learn_collect",
             "nbg_learn_clear", "nbg_learn_pool", "nbg_learn_standardize", "nbg_learn_train", "nbg_learn_grad",
             "nbg_tuning_get", "nbg_tuning_set", "nbg_autotune", "nbg_diagnostics_system", "nbg_accel_bh_targets",
-            "nbg_rungs", "nbg_accel_direct_noalloc"]
+            "nbg_rungs", "nbg_accel_direct_noalloc", "nbg_upload_f64"]
 
 LEARN_FEATURES = 13
 LEARN_HIDDEN = 64
@@ -98,6 +98,7 @@ def load():
     L.nbg_upload.argtypes = [ctypes.c_void_p, ctypes.c_int64] + [_f] * 7
     L.nbg_upload_batch.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i64] + [_f] * 7
     L.nbg_upload_accel.argtypes = [ctypes.c_void_p] + [_f] * 3
+    L.nbg_upload_f64.argtypes = [ctypes.c_void_p, ctypes.c_int64] + [_d] * 3 + [_f] * 4
     L.nbg_accel.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.POINTER(Stats)]
     L.nbg_step.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_int32, ctypes.POINTER(Stats)]
     L.nbg_download.argtypes = [ctypes.c_void_p] + [_f] * 9
@@ -194,6 +195,16 @@ class Context:
         self.n = len(arrs[0])
         self._keep = arrs
         self._check(self._L.nbg_upload(self._h, self.n, *[_p(a, _f) for a in arrs]), "nbg_upload")
+
+    def upload_f64(self, x, y, z, m, vx=None, vy=None, vz=None):
+        """FP64 positions (nbg_upload_f64): the tree from them, the grouped walk on
+        leaf-relative differences; masses and velocities are rounded to FP32."""
+        pos = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z)]
+        rest = [_f32(a) for a in (m, vx, vy, vz)]
+        self.n = len(pos[0])
+        self._keep = pos + rest
+        self._check(self._L.nbg_upload_f64(self._h, self.n, *[_p(a, _d) for a in pos], *[_p(a, _f) for a in rest]),
+                    "nbg_upload_f64")
 
     def upload_batch(self, offsets, x, y, z, m, vx=None, vy=None, vz=None):
         off = np.ascontiguousarray(offsets, dtype=np.int64)
