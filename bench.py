@@ -62,30 +62,126 @@ PEAK_HBM_GBS = 8000.0
 SORT_PASSES = 8  # 8-bit digits over the 63-bit key
 
 
-def stage_hbm(st, steps, n):
+def stage_hbm(st, steps, n, moments=False):
     """Per-stage HBM rate of the tree and integration stages from their HIP-event times
     inside the timed steps, against the algorithmic bytes of each (per particle unless
     noted; C = nodes): keys 28 (read x, y, z, m; write the 8-byte key and the 4-byte
     index), radix sort 24 per pass (a 64-bit key and a 32-bit value read and written) x 8
     passes, build 8 N + 40 C (keys in, node records out) + 32 N (the particle records
     gathered into sorted order), upsweep 16 N + 56 C (particles in; centre of mass, mass,
-    quadrupole, geometry and radius out), kick-drift-kick 108."""
+    quadrupole, geometry and radius out), kick-drift-kick 108.  counter_bytes: the same
+    stage's HBM traffic per step from the committed FETCH_SIZE / WRITE_SIZE passes."""
     C = float(st.n_nodes)
     algo = {"keys": 28.0 * n, "sort": 24.0 * SORT_PASSES * n, "build": 8.0 * n + 40.0 * C + 32.0 * n,
             "upsweep": 16.0 * n + 56.0 * C, "integrate": 108.0 * n}
+    counted = stage_counter_bytes(n, moments)
     out = {}
     for k, b in algo.items():
         ms = getattr(st, "ms_" + k) / steps
         gbs = b / (ms * 1e-3) / 1e9 if ms > 0 else None
         out[k] = {"ms": round(ms, 4), "algorithmic_bytes": int(b),
                   "GB_s": round(gbs, 1) if gbs else None,
-                  "frac_hbm": round(gbs / PEAK_HBM_GBS, 4) if gbs else None}
+                  "frac_hbm": round(gbs / PEAK_HBM_GBS, 4) if gbs else None,
+                  "counter_bytes": counted.get(k)}
+    return out
+
+
+# the kernels of each tree / integration stage and their launches per step (short names
+# as profiles/traffic_plummer_1m.json keys them); the upsweep's internal pass runs once per
+# depth 20..0
+STAGE_KERNELS = {
+    "keys": [("k_reset", 1), ("void k_bbox<float>", 1), ("void k_root<float>", 1), ("void k_keys<21, float>", 1)],
+    "sort": [("k_radix_hist", SORT_PASSES), ("void k_scan_reduce<unsigned int>", SORT_PASSES),
+             ("void k_scan_down_inline<unsigned int>", SORT_PASSES), ("void k_radix_scatter<false>", SORT_PASSES)],
+    "build": [("void k_leaf_depth<21>", 1), ("void k_scan_reduce<unsigned long>", 1),
+              ("void k_scan_down_inline<unsigned long>", 1), ("k_node_totals", 1), ("void k_nodes<21>", 1),
+              ("k_link", 1), ("void k_node_finish<21>", 1), ("k_depth_scan", 1), ("k_depth_scatter", 1),
+              ("k_leaf_order", 1), ("k_pack4", 1), ("k_gather", 1)],
+    "integrate": [("k_kick_drift", 1), ("k_kick", 1)],
+}
+
+
+def stage_counter_bytes(n, moments):
+    """HBM bytes per step of each stage from the committed counter passes, or {}."""
+    path = os.path.join(ROOT, "profiles", "traffic_plummer_1m.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        d = json.load(f)
+    if d.get("n") != n:
+        return {}
+    t = d["bytes_per_launch"]
+    m = "true" if moments else "false"
+    kern = dict(STAGE_KERNELS)
+    kern["upsweep"] = [(f"void k_up_leaf<true, {m}, false>", 1), (f"void k_up_internal<true, {m}, false>", 21)]
+    out = {}
+    for stage, ks in kern.items():
+        if all(k in t for k, _ in ks):
+            out[stage] = int(sum(t[k] * c for k, c in ks))
+    return out
+
+
+def _short_kernel_name(name):
+    """'void nbg::k_walk_bins<true, ...>(nbg::BinArgs)' -> 'void k_walk_bins<true, ...>' (the
+    template arguments kept, namespace and parameters dropped)."""
+    name = name.replace("nbg::", "")
+    depth, cut = 0, len(name)
+    for i, ch in enumerate(name):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            cut = i
+            break
+    return name[:cut].strip()
+
+
+def traffic_from_counters(fetch_csv, write_csv, out_path, n, workload):
+    """HBM bytes per launch of every kernel from two rocprofv3 --pmc passes over the same
+    command (FETCH_SIZE, then WRITE_SIZE; counter_collection.csv of --output-format csv),
+    each counter in KiB per dispatch.  The counters are calibrated on k_copy_f32, a plain
+    copy of n floats (4 n bytes read, 4 n written): scale = algorithmic bytes / counter
+    bytes of that copy, applied to every kernel."""
+    import csv
+    import collections
+
+    def per_kernel(path, counter):
+        per_disp = collections.defaultdict(float)
+        names = {}
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row["Counter_Name"] != counter:
+                    continue
+                d = row["Dispatch_Id"]
+                per_disp[d] += float(row["Counter_Value"])
+                names[d] = _short_kernel_name(row["Kernel_Name"])
+        acc = collections.defaultdict(list)
+        for d, v in per_disp.items():
+            acc[names[d]].append(v)
+        return {k: sum(v) / len(v) for k, v in acc.items()}
+
+    fetch = per_kernel(fetch_csv, "FETCH_SIZE")
+    write = per_kernel(write_csv, "WRITE_SIZE")
+    copy = "k_copy_f32"
+    fs = 4.0 * n / (fetch[copy] * 1024.0)
+    ws = 4.0 * n / (write[copy] * 1024.0)
+    names = sorted(set(fetch) | set(write), key=lambda k: -(fetch.get(k, 0) * fs + write.get(k, 0) * ws))
+    table = {k: int(1024.0 * (fetch.get(k, 0.0) * fs + write.get(k, 0.0) * ws)) for k in names}
+    out = {"workload": workload, "n": n, "fetch_scale": round(fs, 4), "write_scale": round(ws, 4),
+           "scale_meaning": "bytes = counter KiB * 1024 * scale; scale = algorithmic bytes of the copy / its counter bytes",
+           "calibration": f"k_copy_f32 of {n:,} floats",
+           "fetch_bytes_per_launch": {k: int(1024.0 * fetch.get(k, 0.0) * fs) for k in names},
+           "write_bytes_per_launch": {k: int(1024.0 * write.get(k, 0.0) * ws) for k in names},
+           "bytes_per_launch": table}
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
     return out
 
 
 def walk_kernel_name(quad, kahan, learn, near_tile, spline=False):
     b = lambda v: "true" if v else "false"
-    return f"void k_walk_bins<{b(quad)}, {b(kahan)}, false, {b(learn)}, {near_tile}, {b(spline)}>"
+    return f"void k_walk_bins<{b(quad)}, {b(kahan)}, false, {b(learn)}, {near_tile}, {b(spline)}, false>"
 
 
 def log(*a):
@@ -438,7 +534,17 @@ def main():
     ap.add_argument("--no-kahan", action="store_true")
     ap.add_argument("--error-sample", type=int, default=16384, help="targets of the direct-sum error check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="instead of running: write profiles/traffic_plummer_1m.json from the counter_collection.csv "
+                         "files of two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) over `bench.py --no-cpu`")
     args = ap.parse_args()
+    if args.traffic_from:
+        out = traffic_from_counters(args.traffic_from[0], args.traffic_from[1],
+                                    os.path.join(ROOT, "profiles", "traffic_plummer_1m.json"), args.n,
+                                    "plummer_1m: bench.py --no-cpu (learned theta=0.8 and plain theta=0.5 steps, "
+                                    "their error checks and the equal-error plain steps)")
+        log(json.dumps({k: v for k, v in out.items() if not k.endswith("per_launch")}))
+        return
     args.steps_given = args.steps is not None
     if args.steps is None:
         args.steps = 10
@@ -606,7 +712,7 @@ def main():
             "roofline_secondary": secondary,
             "interactions_per_s": round((p2p_l + m2p_l) / (roof_w["kernel_ms"] * 1e-3), 1),
             "stage_ms_per_step": stage_l,
-            "stage_hbm": stage_hbm(st_l, args.steps, n),
+            "stage_hbm": stage_hbm(st_l, args.steps, n, moments=True),
             "tree": {"nodes": st_l.n_nodes, "leaves": st_l.n_leaves, "max_depth": st_l.max_depth},
         })
     if not args.no_cpu:
