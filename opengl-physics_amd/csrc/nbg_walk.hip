@@ -721,7 +721,11 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             wave_sync();
             uint64_t far_m = 0, near_m = 0, open_m = 0, band_m = 0;
             uint64_t todo = 0;  // groups the sphere shortcuts left undecided for this lane's node
-            float n_sth = 0.f, n_cscale = 0.f, n_sbd = 0.f, n_db2 = 0.f;
+            float n_sth = 0.f, n_cscale = 0.f, n_sbd = 0.f, n_gms3 = 0.f;
+            // learned selection of a sphere-accepted node left to the groups: band membership
+            // (bneed, the groups to test) and the correction-scale test (aneed)
+            uint64_t bneed = 0;
+            bool aneed = false;
             int lstart = 0, lcount = 0;
             float4 cm = make_float4(0.f, 0.f, 0.f, 0.f), au = cm;
             uint32_t info = 0;
@@ -756,7 +760,6 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                         const float cscale = fabsf(cm.x) + fabsf(cm.y) + fabsf(cm.z);
                         const float bx = cm.x - cbx, by = cm.y - cby, bz = cm.z - cbz;
                         const float db2 = bx * bx + by * by + bz * bz;
-                        n_db2 = db2;
                         // (the learned band's sphere tests keep the correctly rounded root)
                         const float db = LEARN ? sqrtf(db2) : __builtin_amdgcn_sqrtf(db2);
                         const float slack_b = 1e-4f * (db + Rb + sth) + 4e-6f * (cscale + cbscale);
@@ -771,13 +774,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                                 if (sbd - db - Rb > slack_b) {
                                     band_m = act;
                                 } else if (!(db - Rb - sbd > slack_b)) {
-                                    for (uint64_t t = act; t; t &= t - 1) {
-                                        const int jj = __ffsll(static_cast<long long>(t)) - 1;
-                                        const float4 lc = lcom[jj];
-                                        const float dx = cm.x - lc.x, dy = cm.y - lc.y, dz = cm.z - lc.z;
-                                        const float qb = lc.w + sbd;
-                                        if (dx * dx + dy * dy + dz * dz <= qb * qb) band_m |= 1ull << jj;
-                                    }
+                                    bneed = act;  // tested group by group below, leaf-major
                                 }
                             }
                         } else if (sth - db - Rb > slack_b) {
@@ -788,7 +785,23 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                         todo = todo_all;
                         n_sth = sth;
                         n_cscale = cscale;
-                        if (LEARN) n_sbd = sbd;
+                        if (LEARN) {
+                            n_sbd = sbd;
+                            n_gms3 = cm.w * au.y * size;  // G M s^3: the numerator of the correction scale
+                            if (A.amin > 0.f && (band_m | bneed)) {
+                                // the scale test G M s^3 / r^5 >= amin of a sphere-accepted node: by
+                                // the bin sphere's bounds on r when they settle it, else group by
+                                // group below (the undecided nodes take it inside their tests)
+                                const float rlo = fmaxf(db - Rb, 0.f), rhi = db + Rb;
+                                const float rlo2 = rlo * rlo, rhi2 = rhi * rhi;
+                                if (n_gms3 < A.amin * (rlo2 * rlo2 * rlo)) {
+                                    band_m = 0;
+                                    bneed = 0;
+                                } else if (!(n_gms3 >= A.amin * (rhi2 * rhi2 * rhi))) {
+                                    aneed = true;
+                                }
+                            }
+                        }
                     }
                 }
             }
@@ -834,7 +847,12 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 uint64_t band_b = 0;
                 if (LEARN) {
                     const float qb = my_lc.w + __shfl(n_sbd, L, 64);
-                    band_b = ballot(mine && accept && dist2 <= qb * qb);
+                    bool inb = mine && accept && dist2 <= qb * qb;
+                    if (A.amin > 0.f) {
+                        const float g3 = __shfl(n_gms3, L, 64);
+                        inb = inb && !(g3 < A.amin * (dist2 * dist2 * __builtin_amdgcn_sqrtf(dist2)));
+                    }
+                    band_b = ballot(inb);
                 }
                 if (lane == L) {
                     far_m |= acc_b;
@@ -842,31 +860,24 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                     band_m |= band_b;
                 }
             }
-            if (node >= 0 && !(info & kLeafBit)) {
-                {
-                    {
-                        const float size = __builtin_amdgcn_sqrtf(au.y);
-                        const float db = LEARN ? sqrtf(n_db2) : 0.f;
-                        if (LEARN && band_m && A.amin > 0.f) {
-                            // only pairs whose correction scale G M s^3 / r^5 (r = |com_cell -
-                            // com_leaf|) reaches amin: by the sphere's bounds on r when they
-                            // settle it, else leaf by leaf
-                            const float gms3 = cm.w * au.y * size;
-                            const float rlo = fmaxf(db - Rb, 0.f), rhi = db + Rb;
-                            const float rlo2 = rlo * rlo, rhi2 = rhi * rhi;
-                            if (gms3 < A.amin * (rlo2 * rlo2 * rlo)) {
-                                band_m = 0;
-                            } else if (!(gms3 >= A.amin * (rhi2 * rhi2 * rhi))) {
-                                for (uint64_t t = band_m; t; t &= t - 1) {
-                                    const int jj = __ffsll(static_cast<long long>(t)) - 1;
-                                    const float4 lc = lcom[jj];
-                                    const float dx = cm.x - lc.x, dy = cm.y - lc.y, dz = cm.z - lc.z;
-                                    const float r2 = dx * dx + dy * dy + dz * dz;
-                                    if (gms3 < A.amin * (r2 * r2 * __builtin_amdgcn_sqrtf(r2))) band_m &= ~(1ull << jj);
-                                }
-                            }
-                        }
-                    }
+            if (LEARN) {
+                // the learned selection of the sphere-accepted nodes left to the groups,
+                // leaf-major: one iteration per such node, each lane testing its own group
+                for (uint64_t und = ballot(bneed != 0 || aneed); und; und &= und - 1) {
+                    const int L = __ffsll(static_cast<long long>(und)) - 1;
+                    const float ncx = __shfl(cm.x, L, 64), ncy = __shfl(cm.y, L, 64), ncz = __shfl(cm.z, L, 64);
+                    const uint64_t bnL = shfl64(bneed, L), bmL = shfl64(band_m, L);
+                    const bool anL = __shfl(static_cast<int>(aneed), L, 64) != 0;
+                    // (cross-lane reads stay outside lane-dependent branches: a read from a lane
+                    // that is switched off returns nothing)
+                    const float sbdL = __shfl(n_sbd, L, 64), g3L = __shfl(n_gms3, L, 64);
+                    const float dx = ncx - my_lc.x, dy = ncy - my_lc.y, dz = ncz - my_lc.z;
+                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    const float qb = my_lc.w + sbdL;
+                    bool inb = ((bnL >> lane) & 1ull) ? (r2 <= qb * qb) : (((bmL >> lane) & 1ull) != 0);
+                    if (anL) inb = inb && !(g3L < A.amin * (r2 * r2 * __builtin_amdgcn_sqrtf(r2)));
+                    const uint64_t res = ballot(inb);
+                    if (lane == L) band_m = res;
                 }
             }
             // far cells -> ring
