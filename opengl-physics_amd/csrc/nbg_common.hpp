@@ -19,6 +19,22 @@ __device__ __forceinline__ int mbcnt(uint64_t m) {
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
+// Lane `l`'s value of v, for a wave-uniform l: one v_readlane into a scalar register (a
+// lane shuffle goes through the LDS crossbar and waits on its latency).  Reads the lane's
+// register whatever the execution mask.
+__device__ __forceinline__ int rdlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ uint32_t rdlane(uint32_t v, int l) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), l));
+}
+__device__ __forceinline__ float rdlane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ uint64_t rdlane(uint64_t v, int l) {
+    return (static_cast<uint64_t>(rdlane(static_cast<uint32_t>(v >> 32), l)) << 32) | rdlane(static_cast<uint32_t>(v), l);
+}
+// v with lane l's value replaced by the wave-uniform w, for a wave-uniform l.
+__device__ __forceinline__ uint64_t wrlane(uint64_t v, uint64_t w, int l) { return lane_id() == l ? w : v; }
+
 // Orders LDS traffic of one wavefront: every LDS write issued before is visible to
 // LDS reads issued after, for all lanes of the wave.  Waves of a workgroup never
 // share LDS regions in the kernels that use this.

@@ -225,7 +225,7 @@ __global__ __launch_bounds__(64 * kWalkWaves) void k_walk(WalkArgs A) {
                     for (int c = 0; c < 8; ++c) nk += kids[c] >= 0;
                 }
                 const int kincl = wave_incl_scan(nk);
-                const int ktot = __shfl(kincl, 63, 64);
+                const int ktot = rdlane(kincl, 63);
                 if (is_open) {
                     int w = sp + kincl - nk;
 #pragma unroll
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(64 * kWalkWaves) void k_walk(WalkArgs A) {
                 const bool small = is_near && lcount <= kNcrit;
                 const int ncnt = small ? lcount : 0;
                 const int nincl = wave_incl_scan(ncnt);
-                const int ntot = __shfl(nincl, 63, 64);
+                const int ntot = rdlane(nincl, 63);
                 if (small) {
                     const int w = near_tail + nincl - ncnt;
                     for (int q = 0; q < lcount; ++q) near[(w + q) & (kNearRing - 1)] = lstart + q;
@@ -248,8 +248,8 @@ __global__ __launch_bounds__(64 * kWalkWaves) void k_walk(WalkArgs A) {
                 while (bm) {
                     const int src = __ffsll(static_cast<long long>(bm)) - 1;
                     bm &= bm - 1;
-                    const int bs = __shfl(lstart, src, 64);
-                    const int bc = __shfl(lcount, src, 64);
+                    const int bs = rdlane(lstart, src);
+                    const int bc = rdlane(lcount, src);
                     for (int q0 = 0; q0 < bc; q0 += 64) {
                         const int q = q0 + lane;
                         if (q < bc) p2p<KAHAN>(acc, tx, ty, tz, nt, A.spos[bs + q], eps2, -1);
@@ -372,15 +372,10 @@ struct BinArgs {
     // scale G M s^3 / r^5 >= amin are listed
     float inv_band;
     float amin;
+    float inv_amin;  // 1 / amin (0 without amin)
     int2* lpairs;
     uint32_t lcap;
 };
-
-__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
-    const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(v)), src, 64));
-    const uint32_t hi = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(v >> 32)), src, 64));
-    return (static_cast<uint64_t>(hi) << 32) | lo;
-}
 
 // Exclusive prefix over the wave of small non-negative counts (< 16) via bit-sliced ballots.
 __device__ __forceinline__ int wave_excl_small(int v, int* total) {
@@ -409,8 +404,11 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     __shared__ int2 s_qsc[kBinWaves][kBinRing];  // near leaf: first sorted particle, count
     __shared__ uint64_t s_qmask[kBinWaves][kBinRing];
     __shared__ int s_qnd[kBinWaves][REL ? kBinRing : 1];  // REL: node of each staged near leaf
+    // the bin's leaf spheres at set-up; during the traversal, the records of the nodes whose
+    // groups are tested one node at a time (centre of mass, size / theta)
     __shared__ float4 s_lcom[kBinWaves][64];
     __shared__ int s_lid[kBinWaves][64];
+    __shared__ float2 s_lrec[kBinWaves][LEARN ? 64 : 1];  // LEARN: (band distance, r2max) of those nodes
     __shared__ int s_next[kBinWaves];
 
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -424,6 +422,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     int* qnd = s_qnd[wave];
     float4* lcom = s_lcom[wave];
     int* lid = s_lid[wave];
+    float2* lrec = s_lrec[wave];
     const Soft soft = A.soft;
     const float inv_theta = A.inv_theta;
     // XCD-aware work distribution: workgroups are dispatched round-robin over the 8 XCDs,
@@ -483,6 +482,8 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
         // leaf-major group tests (lanes past the bin's leaves hold leaf 0 and are never asked)
         const float4 my_lc = lcom[lane < nl ? lane : 0];
         const int my_lid = lid[lane < nl ? lane : 0];
+        // the group's share of the FP32 test's rounding bound (position magnitudes + radius)
+        const float my_abs = fabsf(my_lc.x) + fabsf(my_lc.y) + fabsf(my_lc.z) + my_lc.w;
         // REL: the bin's reference point cb (its first leaf's FP64 centre) and each target
         // relative to it, from its offset in its own leaf plus the FP64 offset of the leaves
         double rbx = 0.0, rby = 0.0, rbz = 0.0;
@@ -721,7 +722,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             wave_sync();
             uint64_t far_m = 0, near_m = 0, open_m = 0, band_m = 0;
             uint64_t todo = 0;  // groups the sphere shortcuts left undecided for this lane's node
-            float n_sth = 0.f, n_cscale = 0.f, n_sbd = 0.f, n_gms3 = 0.f;
+            float n_sth = 0.f, n_sbd = 0.f, n_r2max = 0.f;
             // learned selection of a sphere-accepted node left to the groups: band membership
             // (bneed, the groups to test) and the correction-scale test (aneed)
             uint64_t bneed = 0;
@@ -784,20 +785,22 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                         // the groups left undecided are tested below, leaf-major
                         todo = todo_all;
                         n_sth = sth;
-                        n_cscale = cscale;
                         if (LEARN) {
                             n_sbd = sbd;
-                            n_gms3 = cm.w * au.y * size;  // G M s^3: the numerator of the correction scale
+                            // the correction scale G M s^3 / r^5 reaches amin iff r^2 <= r2max =
+                            // (G M s^3 / amin)^(2/5) (no bound without amin)
+                            const float gms3 = cm.w * au.y * size;
+                            n_r2max = A.amin > 0.f ? __builtin_amdgcn_exp2f(0.4f * __builtin_amdgcn_logf(gms3 * A.inv_amin))
+                                                   : __builtin_inff();
                             if (A.amin > 0.f && (band_m | bneed)) {
-                                // the scale test G M s^3 / r^5 >= amin of a sphere-accepted node: by
-                                // the bin sphere's bounds on r when they settle it, else group by
-                                // group below (the undecided nodes take it inside their tests)
+                                // the scale test of a sphere-accepted node: by the bin sphere's
+                                // bounds on r when they settle it, else group by group below (the
+                                // undecided nodes take it inside their tests)
                                 const float rlo = fmaxf(db - Rb, 0.f), rhi = db + Rb;
-                                const float rlo2 = rlo * rlo, rhi2 = rhi * rhi;
-                                if (n_gms3 < A.amin * (rlo2 * rlo2 * rlo)) {
+                                if (rlo * rlo > n_r2max) {
                                     band_m = 0;
                                     bneed = 0;
-                                } else if (!(n_gms3 >= A.amin * (rhi2 * rhi2 * rhi))) {
+                                } else if (rhi * rhi > n_r2max) {
                                     aneed = true;
                                 }
                             }
@@ -808,29 +811,32 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             // Leaf-major group tests: one iteration per node left undecided by the sphere
             // shortcuts, each lane testing its own leaf group against that node, so that an
             // iteration serves every undecided group of the node (node-major, the wave looped
-            // over the most groups any one node had left, with few lanes busy per trip).
+            // over the most groups any one node had left, with few lanes busy per trip).  The
+            // node's record is one broadcast LDS read; the group masks are combined in scalar
+            // registers, and node L's results are written into lane L with v_writelane.
+            if (ballot(todo != 0 || (LEARN && (bneed != 0 || aneed)))) {
+                lcom[lane] = make_float4(cm.x, cm.y, cm.z, n_sth);
+                if (LEARN) lrec[lane] = make_float2(n_sbd, n_r2max);
+                wave_sync();
+            }
             for (uint64_t und = ballot(todo != 0); und; und &= und - 1) {
                 const int L = __ffsll(static_cast<long long>(und)) - 1;
-                const float ncx = __shfl(cm.x, L, 64), ncy = __shfl(cm.y, L, 64), ncz = __shfl(cm.z, L, 64);
-                const float nsth = __shfl(n_sth, L, 64), ncsc = __shfl(n_cscale, L, 64);
-                const uint32_t tlo = static_cast<uint32_t>(__shfl(static_cast<int>(todo), L, 64));
-                const uint32_t thi = static_cast<uint32_t>(__shfl(static_cast<int>(todo >> 32), L, 64));
-                const uint64_t tL = (static_cast<uint64_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(thi))) << 32) |
-                                    static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(tlo)));
-                const bool mine = (tL >> lane) & 1ull;
-                const float dx = ncx - my_lc.x, dy = ncy - my_lc.y, dz = ncz - my_lc.z;
+                const uint64_t tL = rdlane(todo, L);  // the groups to test (lanes = groups)
+                const float4 nr = lcom[L];
+                const float dx = nr.x - my_lc.x, dy = nr.y - my_lc.y, dz = nr.z - my_lc.z;
                 const float dist2 = dx * dx + dy * dy + dz * dz;
                 // worst > 0 && size^2 < theta^2 worst^2  <=>  dist > R + size/theta
-                const float q = my_lc.w + nsth;
-                bool accept = dist2 > q * q;
-                const float slack = 2.f * q *
-                                    (1e-4f * q + 2e-6f * (ncsc + fabsf(my_lc.x) + fabsf(my_lc.y) + fabsf(my_lc.z) + my_lc.w));
-                const bool close = mine && fabsf(dist2 - q * q) <= slack;
-                if (ballot(close)) {
-                    const int nodeL = __builtin_amdgcn_readfirstlane(__shfl(node, L, 64));
-                    if (close) {
-                        // FP64 decision exactly as CollectGroupInteractions writes it
-                        // (Solvers.cpp:261-269)
+                const float q = my_lc.w + nr.w;
+                const float q2 = q * q;
+                uint64_t acc = ballot(dist2 > q2) & tL;
+                // within the FP32 test's rounding bound, 2q (1e-4 q + 2e-6 (|c_node| + |c_group| + R)),
+                // the FP64 decision exactly as CollectGroupInteractions writes it (Solvers.cpp:261-269)
+                const float u = fmaf(q, 2e-4f, 4e-6f * ((fabsf(nr.x) + fabsf(nr.y)) + (fabsf(nr.z) + my_abs)));
+                const uint64_t closem = ballot(fabsf(dist2 - q2) <= q * u) & tL;
+                if (closem) {
+                    const int nodeL = rdlane(node, L);
+                    bool a64 = false;
+                    if ((closem >> lane) & 1ull) {
                         const double4 c64 = A.nd_com64[nodeL];
                         const double4 g64 = A.nd_com64[my_lid];
                         const double gR64 = A.nd_rmax64[my_lid];
@@ -840,44 +846,35 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                             __dadd_rn(__dadd_rn(__dmul_rn(ex, ex), __dmul_rn(ey, ey)), __dmul_rn(ez, ez));
                         const double size = 2.0 * A.nd_geom[4 * nodeL + 3];
                         const double w64 = __dsub_rn(__dsqrt_rn(d2), gR64);
-                        accept = (w64 > 0.0) && (__dmul_rn(size, size) < __dmul_rn(__dmul_rn(A.theta2_64, w64), w64));
+                        a64 = (w64 > 0.0) && (__dmul_rn(size, size) < __dmul_rn(__dmul_rn(A.theta2_64, w64), w64));
                     }
+                    acc = (acc & ~closem) | (ballot(a64) & closem);
                 }
-                const uint64_t acc_b = ballot(mine && accept);
-                uint64_t band_b = 0;
+                // (an undecided node has no far or band groups before its tests)
+                far_m = wrlane(far_m, acc, L);
                 if (LEARN) {
-                    const float qb = my_lc.w + __shfl(n_sbd, L, 64);
-                    bool inb = mine && accept && dist2 <= qb * qb;
-                    if (A.amin > 0.f) {
-                        const float g3 = __shfl(n_gms3, L, 64);
-                        inb = inb && !(g3 < A.amin * (dist2 * dist2 * __builtin_amdgcn_sqrtf(dist2)));
-                    }
-                    band_b = ballot(inb);
-                }
-                if (lane == L) {
-                    far_m |= acc_b;
-                    open_m |= tL & ~acc_b;
-                    band_m |= band_b;
+                    // band and correction scale, both by distance
+                    const float2 lr = lrec[L];
+                    const float qb = my_lc.w + lr.x;
+                    band_m = wrlane(band_m, acc & ballot(dist2 <= fminf(qb * qb, lr.y)), L);
                 }
             }
+            open_m |= todo & ~far_m;
             if (LEARN) {
                 // the learned selection of the sphere-accepted nodes left to the groups,
                 // leaf-major: one iteration per such node, each lane testing its own group
                 for (uint64_t und = ballot(bneed != 0 || aneed); und; und &= und - 1) {
                     const int L = __ffsll(static_cast<long long>(und)) - 1;
-                    const float ncx = __shfl(cm.x, L, 64), ncy = __shfl(cm.y, L, 64), ncz = __shfl(cm.z, L, 64);
-                    const uint64_t bnL = shfl64(bneed, L), bmL = shfl64(band_m, L);
-                    const bool anL = __shfl(static_cast<int>(aneed), L, 64) != 0;
-                    // (cross-lane reads stay outside lane-dependent branches: a read from a lane
-                    // that is switched off returns nothing)
-                    const float sbdL = __shfl(n_sbd, L, 64), g3L = __shfl(n_gms3, L, 64);
-                    const float dx = ncx - my_lc.x, dy = ncy - my_lc.y, dz = ncz - my_lc.z;
+                    const uint64_t bnL = rdlane(bneed, L), bmL = rdlane(band_m, L);
+                    const bool anL = rdlane(static_cast<int>(aneed), L) != 0;
+                    const float4 nr = lcom[L];
+                    const float2 lr = lrec[L];
+                    const float dx = nr.x - my_lc.x, dy = nr.y - my_lc.y, dz = nr.z - my_lc.z;
                     const float r2 = dx * dx + dy * dy + dz * dz;
-                    const float qb = my_lc.w + sbdL;
-                    bool inb = ((bnL >> lane) & 1ull) ? (r2 <= qb * qb) : (((bmL >> lane) & 1ull) != 0);
-                    if (anL) inb = inb && !(g3L < A.amin * (r2 * r2 * __builtin_amdgcn_sqrtf(r2)));
-                    const uint64_t res = ballot(inb);
-                    if (lane == L) band_m = res;
+                    const float qb = my_lc.w + lr.x;
+                    uint64_t res = (bnL & ballot(r2 <= qb * qb)) | (bmL & ~bnL);
+                    if (anL) res &= ballot(r2 <= lr.y);
+                    band_m = wrlane(band_m, res, L);
                 }
             }
             // far cells -> ring
@@ -898,7 +895,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 const uint64_t em = band_m & own_m;
                 const int ne = __popcll(em);
                 const int incl = wave_incl_scan(ne);
-                const int tot = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+                const int tot = rdlane(incl, 63);
                 uint32_t ulo = static_cast<uint32_t>(em), uhi = static_cast<uint32_t>(em >> 32);
 #pragma unroll
                 for (int o = 32; o >= 1; o >>= 1) {
@@ -916,7 +913,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                     const uint64_t m = ballot(mine);
                     if (mine) {
                         const uint32_t idx = w + static_cast<uint32_t>(mbcnt(m));
-                        if (idx < A.lcap) A.lpairs[idx] = make_int2(lid[jj], node);
+                        if (idx < A.lcap) A.lpairs[idx] = make_int2(rdlane(my_lid, jj), node);
                     }
                     w += static_cast<uint32_t>(__popcll(m));
                 }
@@ -1136,6 +1133,7 @@ void launch_walk_bins(nbg_ctx* c, const nbg_params& p) {
     const bool learn = (p.flags & NBG_FLAG_LEARNED) != 0;
     A.inv_band = (p.learn_band > 0.0) ? static_cast<float>(1.0 / p.learn_band) : 1e30f;
     A.amin = static_cast<float>(p.learn_amin);
+    A.inv_amin = p.learn_amin > 0.0 ? static_cast<float>(1.0 / p.learn_amin) : 0.f;
     A.lpairs = c->lpairs;
     A.lcap = static_cast<uint32_t>(c->lpairs_cap);
     if (p.softening == NBG_SOFT_SPLINE)

@@ -230,7 +230,7 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
             uint64_t rem = ballot(tv && !defer);
             while (rem) {
                 const int l = __ffsll(static_cast<long long>(rem)) - 1;
-                const int r = __builtin_amdgcn_readfirstlane(__shfl(myroot, l, 64));
+                const int r = rdlane(myroot, l);
                 const uint64_t m = ballot(tv && !defer && myroot == r);
                 if (lane == 0) {
                     snode[sp] = r;

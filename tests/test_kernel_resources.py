@@ -1,6 +1,7 @@
 """CPU check of the compiled walk kernels' resources: every variant of the grouped and
 per-particle walks (and the learned inference) must run without a private segment
-(scratch) and within the register budget that keeps three 4-wave blocks per CU.  A
+(scratch) and within the register (and, for the grouped walk, LDS) budget that keeps three
+4-wave blocks per CU.  A
 non-inlined helper or a spilled array shows up here as scratch bytes; the COUNT variants
 once ran three times slower that way.  The device code is compiled to gfx950 assembly
 (no GPU needed) and the kernel descriptors are read from its metadata."""
@@ -47,3 +48,6 @@ def test_walk_kernels_use_no_scratch(src, prefix, max_vgpr, tmp_path):
     for name, md in picked.items():
         assert md.get("private_segment_fixed_size", 0) == 0, (name, md)
         assert md["vgpr_count"] <= max_vgpr, (name, md)
+        # 160 KB of LDS per CU holds three blocks of the grouped walk
+        if prefix.endswith("k_walk_bins"):
+            assert md["group_segment_fixed_size"] <= 160 * 1024 // 3, (name, md)
