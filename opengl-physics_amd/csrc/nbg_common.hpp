@@ -32,8 +32,16 @@ __device__ __forceinline__ float rdlane(float v, int l) {
 __device__ __forceinline__ uint64_t rdlane(uint64_t v, int l) {
     return (static_cast<uint64_t>(rdlane(static_cast<uint32_t>(v >> 32), l)) << 32) | rdlane(static_cast<uint32_t>(v), l);
 }
-// v with lane l's value replaced by the wave-uniform w, for a wave-uniform l.
-__device__ __forceinline__ uint64_t wrlane(uint64_t v, uint64_t w, int l) { return lane_id() == l ? w : v; }
+// v_writelane: v with lane l's value replaced by the wave-uniform w (l wave-uniform).  The
+// LLVM intrinsic has no clang builtin here; it is reached through its symbol name.
+extern "C" __device__ int nbg_llvm_writelane(int w, int l, int v) __asm("llvm.amdgcn.writelane.i32");
+__device__ __forceinline__ int wrlane(int v, int w, int l) { return nbg_llvm_writelane(w, l, v); }
+__device__ __forceinline__ uint64_t wrlane(uint64_t v, uint64_t w, int l) {
+    const uint32_t lo = static_cast<uint32_t>(wrlane(static_cast<int>(static_cast<uint32_t>(v)), static_cast<int>(static_cast<uint32_t>(w)), l));
+    const uint32_t hi = static_cast<uint32_t>(
+        wrlane(static_cast<int>(static_cast<uint32_t>(v >> 32)), static_cast<int>(static_cast<uint32_t>(w >> 32)), l));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
 
 // Orders LDS traffic of one wavefront: every LDS write issued before is visible to
 // LDS reads issued after, for all lanes of the wave.  Waves of a workgroup never

@@ -49,6 +49,11 @@ __device__ __forceinline__ float4 bld(__amdgpu_buffer_rsrc_t r, int i) {
     const f32x4v t = __builtin_amdgcn_raw_buffer_load_b128(r, 0, static_cast<int>(static_cast<uint32_t>(i) << 4), 0);
     return make_float4(t[0], t[1], t[2], t[3]);
 }
+// The same for a per-lane index (a VGPR offset): each lane loads its own record.
+__device__ __forceinline__ float4 bld_lane(__amdgpu_buffer_rsrc_t r, int i) {
+    const f32x4v t = __builtin_amdgcn_raw_buffer_load_b128(r, static_cast<int>(static_cast<uint32_t>(i) << 4), 0, 0);
+    return make_float4(t[0], t[1], t[2], t[3]);
+}
 __device__ __forceinline__ uint32_t cld_u32(const void* p, int i) { return ((cuint32*)p)[i]; }
 
 template <bool KAHAN>

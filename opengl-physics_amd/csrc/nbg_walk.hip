@@ -409,6 +409,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     __shared__ float4 s_lcom[kBinWaves][64];
     __shared__ int s_lid[kBinWaves][64];
     __shared__ float2 s_lrec[kBinWaves][LEARN ? 64 : 1];  // LEARN: (band distance, r2max) of those nodes
+    __shared__ float4 s_nstage[kBinWaves][NT];             // the near field's current group of sources
     __shared__ int s_next[kBinWaves];
 
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -423,6 +424,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     float4* lcom = s_lcom[wave];
     int* lid = s_lid[wave];
     float2* lrec = s_lrec[wave];
+    float4* nstage = s_nstage[wave];
     const Soft soft = A.soft;
     const float inv_theta = A.inv_theta;
     // XCD-aware work distribution: workgroups are dispatched round-robin over the 8 XCDs,
@@ -577,10 +579,12 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             kadd<KAHAN>(az, cz, sz);
         };
         // evaluates the particles of `n` staged near leaves starting at ring position `h`,
-        // packed NT at a time across leaf boundaries (a scalar cursor walks the leaves; the
-        // NT scalar loads of a group are issued before any arithmetic -- NT is the P2P tile,
-        // chosen by the autotuner).  Unused slots of the last group read the zero pad after
-        // the sorted particles.
+        // packed NT at a time across leaf boundaries (a scalar cursor walks the leaves -- NT is
+        // the P2P tile, chosen by the autotuner).  Lanes 0..NT-1 load the group's NT records
+        // with one per-lane load and stage them in LDS, from where every lane reads each
+        // record (a wave-uniform record loaded by all 64 lanes costs the texture path 64
+        // deliveries).  Unused slots of the last group read the zero pad after the sorted
+        // particles.
         // FAST: every pair term is finite (Plummer with eps > 0, or the spline), so a lane
         // weighs a source by its use (1 or 0) instead of selecting, and its own particle
         // adds an exact zero (d = 0); otherwise unused sources and the target itself are
@@ -592,40 +596,45 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             uint64_t em = 0;
             float wuse = 0.f;
             const __amdgpu_buffer_rsrc_t srec = rec_rsrc(A.spos, A.n_pad + 1);
+            // the n <= 64 leaves, one per lane (two LDS reads for all of them): the cursor takes
+            // each leaf's words with v_readlane instead of waiting on an LDS read per leaf.  Past
+            // the last leaf the cursor reads a sentinel (no use, count 0: never exhausted), so
+            // a slot costs one test of the current leaf's end.
+            const int lslot = (h + lane) & (kBinRing - 1);
+            const int2 my_q = lane < n ? qsc[lslot] : make_int2(0, 0);
+            const uint64_t my_qm = lane < n ? qmask[lslot] : 0ull;
             auto load_entry = [&](int ee) __attribute__((always_inline)) {
-                const int slot = (h + ee) & (kBinRing - 1);
-                const int2 e2 = qsc[slot];
-                st = __builtin_amdgcn_readfirstlane(e2.x);
-                c = __builtin_amdgcn_readfirstlane(e2.y);
-                em = qmask[slot];
+                // (lane ee & 63 is read unconditionally and dropped past the last leaf)
+                const int st_r = rdlane(my_q.x, ee & 63), c_r = rdlane(my_q.y, ee & 63);
+                const uint64_t em_r = rdlane(my_qm, ee & 63);
+                const bool live = ee < n;
+                st = live ? st_r : A.n_pad;
+                c = live ? c_r : 0;
+                em = live ? em_r : 0ull;
                 wuse = (em & mybit) ? 1.f : 0.f;
             };
-            if (n > 0) load_entry(0);
+            load_entry(0);
             int groups = 0;
             while (e < n) {
                 int idx[NT];
                 float us[NT];
+                int vidx = A.n_pad;  // lane u < NT: the group's u-th source
 #pragma unroll
                 for (int u = 0; u < NT; ++u) {
-                    if (e < n) {
-                        idx[u] = st + k;
-                        us[u] = wuse;
-                        if (COUNT) n_near += (em & mybit) && (st + k != tpos);
-                        if (++k == c) {
-                            k = 0;
-                            if (++e < n) load_entry(e);
-                        }
-                    } else {
-                        idx[u] = A.n_pad;
-                        us[u] = 0.f;
+                    idx[u] = min(st + k, A.n_pad);
+                    us[u] = wuse;
+                    vidx = wrlane(vidx, idx[u], u);
+                    if (COUNT) n_near += (em & mybit) && (st + k != tpos);
+                    if (++k == c) {
+                        k = 0;
+                        load_entry(++e);
                     }
                 }
-                float4 spt[NT];
-#pragma unroll
-                for (int u = 0; u < NT; ++u) spt[u] = bld(srec, idx[u]);
+                if (lane < NT) nstage[lane] = bld_lane(srec, vidx);
+                wave_sync();
 #pragma unroll
                 for (int u = 0; u < NT; ++u) {
-                    const float4 q = spt[u];
+                    const float4 q = nstage[u];
                     const float dx = q.x - tp.x, dy = q.y - tp.y, dz = q.z - tp.z;
                     float f;
                     if (FAST) {
