@@ -48,6 +48,37 @@ __device__ __forceinline__ uint64_t wrlane(uint64_t v, uint64_t w, int l) {
 // share LDS regions in the kernels that use this.
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); __builtin_amdgcn_wave_barrier(); }
 
+// Row (16-lane) reductions through DPP: each step is one VALU op with a lane-permuted
+// operand (quad xor 1, quad xor 2, half-row mirror, row mirror), no LDS round trip; every lane
+// of a row ends with the row's result, and the wave's is the combination of lanes 15, 31,
+// 47 and 63 read as scalars.
+__device__ __forceinline__ uint32_t dpp_or_row(uint32_t v) {
+    v |= static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0xB1, 0xF, 0xF, false));
+    v |= static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x4E, 0xF, 0xF, false));
+    v |= static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x141, 0xF, 0xF, false));
+    v |= static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x140, 0xF, 0xF, false));
+    return v;
+}
+__device__ __forceinline__ int dpp_add_row(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);
+    return v;
+}
+// OR of v over the wave (wave-uniform result)
+__device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
+    const uint32_t lo = dpp_or_row(static_cast<uint32_t>(v)), hi = dpp_or_row(static_cast<uint32_t>(v >> 32));
+    const uint32_t rlo = rdlane(lo, 15) | rdlane(lo, 31) | rdlane(lo, 47) | rdlane(lo, 63);
+    const uint32_t rhi = rdlane(hi, 15) | rdlane(hi, 31) | rdlane(hi, 47) | rdlane(hi, 63);
+    return (static_cast<uint64_t>(rhi) << 32) | rlo;
+}
+// sum of v over the wave (wave-uniform result)
+__device__ __forceinline__ int wave_add(int v) {
+    v = dpp_add_row(v);
+    return rdlane(v, 15) + rdlane(v, 31) + rdlane(v, 47) + rdlane(v, 63);
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T v) {
     const int lane = lane_id();

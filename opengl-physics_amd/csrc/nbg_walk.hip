@@ -592,14 +592,14 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
         auto eval_near_t = [&](int h, int n, auto fast) __attribute__((always_inline)) {
             constexpr bool FAST = decltype(fast)::value;
             float sx = 0.f, sy = 0.f, sz = 0.f;
-            int e = 0, k = 0, st = 0, c = 0;
+            int e = 0, cur = 0, end = 0;  // the cursor: leaf e, its next particle cur, its end
             uint64_t em = 0;
             float wuse = 0.f;
             const __amdgpu_buffer_rsrc_t srec = rec_rsrc(A.spos, A.n_pad + 1);
             // the n <= 64 leaves, one per lane (two LDS reads for all of them): the cursor takes
             // each leaf's words with v_readlane instead of waiting on an LDS read per leaf.  Past
-            // the last leaf the cursor reads a sentinel (no use, count 0: never exhausted), so
-            // a slot costs one test of the current leaf's end.
+            // the last leaf the cursor reads a sentinel (no use, an end never reached), so a
+            // slot costs one test of the current leaf's end.
             const int lslot = (h + lane) & (kBinRing - 1);
             const int2 my_q = lane < n ? qsc[lslot] : make_int2(0, 0);
             const uint64_t my_qm = lane < n ? qmask[lslot] : 0ull;
@@ -608,8 +608,8 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 const int st_r = rdlane(my_q.x, ee & 63), c_r = rdlane(my_q.y, ee & 63);
                 const uint64_t em_r = rdlane(my_qm, ee & 63);
                 const bool live = ee < n;
-                st = live ? st_r : A.n_pad;
-                c = live ? c_r : 0;
+                cur = live ? st_r : A.n_pad;
+                end = live ? st_r + c_r : -1;
                 em = live ? em_r : 0ull;
                 wuse = (em & mybit) ? 1.f : 0.f;
             };
@@ -621,14 +621,11 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 int vidx = A.n_pad;  // lane u < NT: the group's u-th source
 #pragma unroll
                 for (int u = 0; u < NT; ++u) {
-                    idx[u] = min(st + k, A.n_pad);
+                    idx[u] = min(cur, A.n_pad);
                     us[u] = wuse;
                     vidx = wrlane(vidx, idx[u], u);
-                    if (COUNT) n_near += (em & mybit) && (st + k != tpos);
-                    if (++k == c) {
-                        k = 0;
-                        load_entry(++e);
-                    }
+                    if (COUNT) n_near += (em & mybit) && (cur != tpos);
+                    if (__builtin_expect(++cur == end, 0)) load_entry(++e);
                 }
                 if (lane < NT) nstage[lane] = bld_lane(srec, vidx);
                 wave_sync();
@@ -902,17 +899,8 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             // leaf's pairs in runs and sums a run before one atomic update
             if (LEARN && ballot((band_m & own_m) != 0)) {
                 const uint64_t em = band_m & own_m;
-                const int ne = __popcll(em);
-                const int incl = wave_incl_scan(ne);
-                const int tot = rdlane(incl, 63);
-                uint32_t ulo = static_cast<uint32_t>(em), uhi = static_cast<uint32_t>(em >> 32);
-#pragma unroll
-                for (int o = 32; o >= 1; o >>= 1) {
-                    ulo |= static_cast<uint32_t>(__shfl_xor(static_cast<int>(ulo), o, 64));
-                    uhi |= static_cast<uint32_t>(__shfl_xor(static_cast<int>(uhi), o, 64));
-                }
-                const uint64_t used = (static_cast<uint64_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(uhi))) << 32) |
-                                      static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(ulo)));
+                const int tot = wave_add(__popcll(em));
+                const uint64_t used = wave_or64(em);  // the leaves with pairs in this step
                 uint32_t base = 0;
                 if (lane == 0) base = atomicAdd(&A.sc->learn_pairs, static_cast<uint32_t>(tot));
                 uint32_t w = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(base)));
