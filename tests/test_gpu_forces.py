@@ -1,7 +1,7 @@
 """GPU force parity against the FP64 oracle.
 
-Bar: accelerations within a relative RMS of 1e-5 of the
-reference's FP64 CPU path on identical FP32-rounded inputs; Barnes-Hut interaction lists
+Bar: accelerations within a relative RMS of 1e-5 of the reference's FP64 CPU path on
+identical FP32-rounded inputs; Barnes-Hut interaction lists
 identical per leaf group (CollectGroupInteractions, projects/nbody_core/src/Solvers.cpp:248-277).
 rel_rms = sqrt(sum |a - a_ref|^2 / sum |a_ref|^2).
 """
