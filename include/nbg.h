@@ -112,7 +112,7 @@ typedef struct nbg_stats {
     double dt_last;       /* nbg_step: dt of the last step (the coarse dt for block steps) */
     int64_t particle_evals; /* per-particle force evaluations (System::ParticleEvals) */
     int32_t rung_max;     /* nbg_step with block: deepest rung of the last coarse step */
-    int32_t pad;
+    int32_t key_levels;   /* tree levels the last tree's keys held: 16, 21 or 40 (nbg_tree_export's keys) */
 } nbg_stats;
 
 /* Create a context on HIP device `device` able to hold up to max_n particles.

@@ -127,6 +127,17 @@ inline void rec(nbg_ctx* c, int i) {
 void drop_graph(nbg_ctx* c);
 
 // Enqueue one Barnes-Hut force evaluation (no host synchronisation).
+// The key levels to retry with after a tree at `levels` overflowed (a cell at the deepest
+// level held more than ncrit particles).  A depth-16 overflow raises the context's first
+// try to 21 levels until the next upload; the two-word keys are taken per evaluation.
+int next_levels(nbg_ctx* c, int levels) {
+    if (levels == kLevelsFast) {
+        c->key_levels = kLevelsShort;
+        return kLevelsShort;
+    }
+    return kLevelsDeep;
+}
+
 void enqueue_bh(nbg_ctx* c, const nbg_params& p, int levels, bool timed, bool list_only = false) {
     const bool learn = (p.flags & NBG_FLAG_LEARNED) != 0;
     if (timed) rec(c, 0);
@@ -175,6 +186,7 @@ void fill_tree_stats(nbg_ctx* c, nbg_stats* s) {
     s->n_leaves = c->hsc->n_leaves;
     s->max_depth = c->hsc->max_depth;
     s->deep_keys = (c->last_levels == kLevelsDeep) ? 1 : 0;
+    s->key_levels = c->last_levels;
 }
 
 // One full force evaluation with the short-key path, falling back to two-word keys
@@ -200,7 +212,7 @@ int accel_eager(nbg_ctx* c, const nbg_params& p, nbg_stats* stats, bool list_onl
     // the relative MAC reads |a| of the previous evaluation: capture it once, before the
     // walk (a deep-key re-run below must not see this evaluation's accelerations)
     if (p.mac == NBG_MAC_RELATIVE) launch_aold(c);
-    int levels = kLevelsShort;
+    int levels = c->key_levels;
     for (;;) {
         enqueue_bh(c, p, levels, timed, list_only);
         if (int r = check_launch(c)) return r;
@@ -209,8 +221,10 @@ int accel_eager(nbg_ctx* c, const nbg_params& p, nbg_stats* stats, bool list_onl
             c->set_error("node pool capacity exceeded");
             return NBG_E_CAPACITY;
         }
-        if (c->hsc->overflow && levels == kLevelsShort) {
-            levels = kLevelsDeep;
+        if (c->hsc->overflow && levels < kLevelsDeep) {
+            // a cell at the keys' deepest level holds more than ncrit particles: more levels
+            // (16 -> 21, kept for this state; 21 -> the two-word keys, for this evaluation)
+            levels = next_levels(c, levels);
             continue;
         }
         if (learn && static_cast<int64_t>(c->hsc->learn_pairs) > c->lpairs_cap) {
@@ -389,7 +403,7 @@ int steps_eager(nbg_ctx* c, const nbg_params& p, int nsteps, nbg_stats* stats) {
 // Tree of the current positions (keys, sort, build, gather, moments), with the two-word
 // key re-run when a depth-21 cell still holds more than ncrit particles.
 int build_tree_eager(nbg_ctx* c, const nbg_params& p) {
-    int levels = kLevelsShort;
+    int levels = c->key_levels;
     for (;;) {
         launch_bbox_and_keys(c, levels);
         launch_sort(c, levels);
@@ -402,8 +416,8 @@ int build_tree_eager(nbg_ctx* c, const nbg_params& p) {
             c->set_error("node pool capacity exceeded");
             return NBG_E_CAPACITY;
         }
-        if (c->hsc->overflow && levels == kLevelsShort) {
-            levels = kLevelsDeep;
+        if (c->hsc->overflow && levels < kLevelsDeep) {
+            levels = next_levels(c, levels);
             continue;
         }
         break;
@@ -614,6 +628,7 @@ int nbg_upload_batch(nbg_ctx* c, int32_t nsys, const int64_t* offsets, const flo
     NBG_HIP_CHECK(c, hipSetDevice(c->device));
     drop_graph(c);
     c->f64_state = false;
+    c->key_levels = kLevelsFast;  // a new state tries the 16-level keys again
     c->n = static_cast<int>(n);
     c->nsys = nsys;
     c->h_sys_off.assign(offsets, offsets + nsys + 1);
@@ -808,16 +823,16 @@ int nbg_step(nbg_ctx* c, const nbg_params* p, int32_t nsteps, nbg_stats* stats) 
         return NBG_OK;
     }
     if (p->adaptive) NBG_HIP_CHECK(c, hipMemsetAsync(&c->dsc->t_accum, 0, sizeof(double), c->stream));
-    const bool use_graph = !(p->flags & NBG_FLAG_NO_GRAPH) && c->last_levels == kLevelsShort;
+    const bool use_graph = !(p->flags & NBG_FLAG_NO_GRAPH) && c->last_levels <= kLevelsShort;
     if (!use_graph) {
         if (int r = steps_eager(c, *p, nsteps, stats)) return r;
     } else {
-        if (!c->graph_valid || !same_params(c->graph_params, *p)) {
+        if (!c->graph_valid || !same_params(c->graph_params, *p) || c->graph_levels != c->last_levels) {
             drop_graph(c);
             (void)hipGetLastError();
             NBG_HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             c->capturing = true;
-            enqueue_step(c, *p, kLevelsShort, (p->flags & NBG_FLAG_STAGE_TIMES) != 0);
+            enqueue_step(c, *p, c->last_levels, (p->flags & NBG_FLAG_STAGE_TIMES) != 0);
             c->capturing = false;
             const hipError_t le = hipGetLastError();
             hipGraph_t g = nullptr;
@@ -830,6 +845,7 @@ int nbg_step(nbg_ctx* c, const nbg_params* p, int32_t nsteps, nbg_stats* stats) 
             c->graph = g;
             NBG_HIP_CHECK(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
             c->graph_params = *p;
+            c->graph_levels = c->last_levels;
             c->graph_valid = true;
         }
         if (int r = save_backup(c)) return r;

@@ -315,7 +315,8 @@ __global__ void k_root(const typename PosOrd<T>::U* __restrict__ bbox, const int
 }
 
 // Morton key of every particle from the FP64 centre recurrence (see file header).
-// LEVELS = 21 fills the short key; LEVELS = 40 also fills the low word.
+// LEVELS = 21 fills the short key; LEVELS = 40 also fills the low word; LEVELS = 16 fills
+// levels 1..16 of the short key (at their places in it, levels 17..21 zero).
 template <int LEVELS, typename T>
 __global__ __launch_bounds__(256) void k_keys(const T* __restrict__ x, const T* __restrict__ y,
                                               const T* __restrict__ z, const int32_t* __restrict__ pos_sys,
@@ -340,6 +341,7 @@ __global__ __launch_bounds__(256) void k_keys(const T* __restrict__ x, const T* 
         else
             lo = (lo << 3) | oct;
     }
+    if (LEVELS < 21) hi <<= 3 * (21 - LEVELS);
     key[i] = hi;
     if (LEVELS > 21) klo[i] = lo;
     val[i] = static_cast<uint32_t>(i);
@@ -358,7 +360,10 @@ void launch_bbox_and_keys(nbg_ctx* c, int levels) {
                            c->blk_start, c->blk_count, bb);
         hipLaunchKernelGGL(k_root<double>, dim3(grid_for(c->nsys, 64)), dim3(64), 0, st, bb, c->sys_off, c->nsys,
                            c->sys_geom);
-        if (levels == kLevelsShort)
+        if (levels == kLevelsFast)
+            hipLaunchKernelGGL((k_keys<kLevelsFast, double>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x64,
+                               c->y64, c->z64, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
+        else if (levels == kLevelsShort)
             hipLaunchKernelGGL((k_keys<kLevelsShort, double>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x64,
                                c->y64, c->z64, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
         else
@@ -370,7 +375,10 @@ void launch_bbox_and_keys(nbg_ctx* c, int levels) {
                        c->blk_start, c->blk_count, c->sys_bbox);
     hipLaunchKernelGGL(k_root<float>, dim3(grid_for(c->nsys, 64)), dim3(64), 0, st, c->sys_bbox, c->sys_off, c->nsys,
                        c->sys_geom);
-    if (levels == kLevelsShort)
+    if (levels == kLevelsFast)
+        hipLaunchKernelGGL((k_keys<kLevelsFast, float>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x, c->y,
+                           c->z, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
+    else if (levels == kLevelsShort)
         hipLaunchKernelGGL((k_keys<kLevelsShort, float>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x, c->y,
                            c->z, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
     else
@@ -556,10 +564,13 @@ void launch_sort(nbg_ctx* c, int levels) {
                  even ? c->val_b : c->val_a, even ? c->klo_a : c->klo_b, even ? c->klo_b : c->klo_a, 8 * p);
         }
     } else {
-        for (int p = 0; p < 8; ++p) {
+        // the key's bits 3 (21 - levels) .. 62: 8 passes for 21 levels, 6 for 16 (bits 15..62)
+        const int low = 3 * (kLevelsShort - levels);
+        const int passes = (63 - low + 7) / 8;
+        for (int p = 0; p < passes; ++p) {
             const bool even = (p & 1) == 0;
             pass(even ? c->key_a : c->key_b, even ? c->key_b : c->key_a, even ? c->val_a : c->val_b,
-                 even ? c->val_b : c->val_a, nullptr, nullptr, 8 * p);
+                 even ? c->val_b : c->val_a, nullptr, nullptr, low + 8 * p);
         }
     }
     c->key_sorted = c->key_a;
@@ -571,7 +582,8 @@ void launch_sort(nbg_ctx* c, int levels) {
 
 template <int LEVELS>
 __device__ __forceinline__ int cpl_at(const uint64_t* __restrict__ kh, const uint64_t* __restrict__ kl, int i, int j) {
-    if (LEVELS == kLevelsShort) return cpl_short(kh[i], kh[j]);
+    // (a 16-level key holds no deeper levels: equal there is as far as it can tell)
+    if (LEVELS <= kLevelsShort) return min(cpl_short(kh[i], kh[j]), LEVELS);
     return cpl_deep(kh[i], kl[i], kh[j], kl[j]);
 }
 
@@ -632,7 +644,7 @@ __global__ __launch_bounds__(256) void k_leaf_depth(const uint64_t* __restrict__
             }
         }
         if (D < 0) {
-            if (LEVELS == kLevelsShort) {
+            if (LEVELS <= kLevelsShort) {
                 atomicOr(&sc->overflow, 1);
                 atomicOr(&sc->sticky_overflow, 1);
                 D = LEVELS;
@@ -707,7 +719,7 @@ __global__ __launch_bounds__(256) void k_nodes(const uint64_t* __restrict__ kh, 
     const int64_t base = static_cast<int64_t>(sc_i >> 32);
     const int lbase = static_cast<int>(sc_i & 0xffffffffu);
     const uint64_t kh_i = kh[i];
-    const uint64_t kl_i = (LEVELS == kLevelsShort) ? 0u : kl[i];
+    const uint64_t kl_i = (LEVELS <= kLevelsShort) ? 0u : kl[i];
     for (int d = cp + 1; d <= D; ++d) {
         const int64_t id64 = base + (d - (cp + 1));
         if (id64 >= cap_nodes) {
@@ -788,7 +800,7 @@ __global__ __launch_bounds__(256) void k_node_finish(const uint64_t* __restrict_
         const int d = nd_depth[id];
         const int s = pos_sys[i];
         const uint64_t h_ = kh[i];
-        const uint64_t l_ = (LEVELS == kLevelsShort) ? 0u : kl[i];
+        const uint64_t l_ = (LEVELS <= kLevelsShort) ? 0u : kl[i];
         double cx = sys_geom[4 * s + 0], cy = sys_geom[4 * s + 1], cz = sys_geom[4 * s + 2], h = sys_geom[4 * s + 3];
         for (int l = 1; l <= d; ++l) {
             const int oct = key_digit(h_, l_, l);
@@ -915,47 +927,39 @@ __global__ __launch_bounds__(256) void k_gather(const float4* __restrict__ pos4,
     spos[k] = pos4[perm[k]];
 }
 
-void launch_build(nbg_ctx* c, int levels) {
+template <int LEVELS>
+void launch_build_t(nbg_ctx* c) {
     hipStream_t st = c->stream;
     const int n = c->n;
     const int g = grid_for(n, 256);
     const int gnodes = 2048;  // grid-stride kernels over the device-side node count
-    if (levels == kLevelsShort)
-        hipLaunchKernelGGL(k_leaf_depth<kLevelsShort>, dim3(g), dim3(256), 0, st, c->key_sorted, c->klo_sorted,
-                           c->pos_sys, c->sys_off, n, c->leaf_depth, c->cpl_prev, c->node_scan, c->dsc, c->blk_depth);
-    else
-        hipLaunchKernelGGL(k_leaf_depth<kLevelsDeep>, dim3(g), dim3(256), 0, st, c->key_sorted, c->klo_sorted,
-                           c->pos_sys, c->sys_off, n, c->leaf_depth, c->cpl_prev, c->node_scan, c->dsc, c->blk_depth);
+    hipLaunchKernelGGL(k_leaf_depth<LEVELS>, dim3(g), dim3(256), 0, st, c->key_sorted, c->klo_sorted, c->pos_sys,
+                       c->sys_off, n, c->leaf_depth, c->cpl_prev, c->node_scan, c->dsc, c->blk_depth);
     // node_scan[n] receives the grand total (the array holds n+1 entries)
     excl_scan<uint64_t>(st, c->node_scan, c->node_scan, n, c->scan_part, c->node_scan + n);
     hipLaunchKernelGGL(k_node_totals, dim3(1), dim3(256), 0, st, c->node_scan + n, c->dsc, c->cap_nodes, c->blk_depth,
                        g);
-    if (levels == kLevelsShort)
-        hipLaunchKernelGGL(k_nodes<kLevelsShort>, dim3(g), dim3(256), 0, st, c->key_sorted, c->klo_sorted, c->pos_sys,
-                           c->sys_off, c->leaf_depth, c->cpl_prev, c->node_scan, n, c->cap_nodes, c->nd_start,
-                           c->nd_count, c->nd_depth, c->nd_parent, c->nd_oct, c->nd_child, c->leaf_list, c->sys_root,
-                           c->dsc);
-    else
-        hipLaunchKernelGGL(k_nodes<kLevelsDeep>, dim3(g), dim3(256), 0, st, c->key_sorted, c->klo_sorted, c->pos_sys,
-                           c->sys_off, c->leaf_depth, c->cpl_prev, c->node_scan, n, c->cap_nodes, c->nd_start,
-                           c->nd_count, c->nd_depth, c->nd_parent, c->nd_oct, c->nd_child, c->leaf_list, c->sys_root,
-                           c->dsc);
+    hipLaunchKernelGGL(k_nodes<LEVELS>, dim3(g), dim3(256), 0, st, c->key_sorted, c->klo_sorted, c->pos_sys,
+                       c->sys_off, c->leaf_depth, c->cpl_prev, c->node_scan, n, c->cap_nodes, c->nd_start, c->nd_count,
+                       c->nd_depth, c->nd_parent, c->nd_oct, c->nd_child, c->leaf_list, c->sys_root, c->dsc);
     hipLaunchKernelGGL(k_link, dim3(gnodes), dim3(256), 0, st, c->nd_parent, c->nd_oct, c->dsc, c->nd_child);
-    if (levels == kLevelsShort)
-        hipLaunchKernelGGL(k_node_finish<kLevelsShort>, dim3(gnodes), dim3(256), 0, st, c->key_sorted, c->klo_sorted,
-                           c->pos_sys, c->sys_off, c->sys_geom, c->nd_start, c->nd_count, c->nd_depth, c->nd_child,
-                           c->nd_geom,
-                           c->nd_aux, c->leaf_of, c->dsc);
-    else
-        hipLaunchKernelGGL(k_node_finish<kLevelsDeep>, dim3(gnodes), dim3(256), 0, st, c->key_sorted, c->klo_sorted,
-                           c->pos_sys, c->sys_off, c->sys_geom, c->nd_start, c->nd_count, c->nd_depth, c->nd_child,
-                           c->nd_geom,
-                           c->nd_aux, c->leaf_of, c->dsc);
+    hipLaunchKernelGGL(k_node_finish<LEVELS>, dim3(gnodes), dim3(256), 0, st, c->key_sorted, c->klo_sorted,
+                       c->pos_sys, c->sys_off, c->sys_geom, c->nd_start, c->nd_count, c->nd_depth, c->nd_child,
+                       c->nd_geom, c->nd_aux, c->leaf_of, c->dsc);
     hipLaunchKernelGGL(k_depth_scan, dim3(1), dim3(64), 0, st, c->dsc);
     hipLaunchKernelGGL(k_depth_scatter, dim3(gnodes), dim3(256), 0, st, c->nd_depth, c->nd_aux, c->dsc,
                        c->depth_nodes);
     hipLaunchKernelGGL(k_leaf_order, dim3(gnodes), dim3(256), 0, st, c->leaf_list, c->nd_start, c->nd_count, c->dsc,
                        c->perm);
+}
+
+void launch_build(nbg_ctx* c, int levels) {
+    if (levels == kLevelsFast)
+        launch_build_t<kLevelsFast>(c);
+    else if (levels == kLevelsShort)
+        launch_build_t<kLevelsShort>(c);
+    else
+        launch_build_t<kLevelsDeep>(c);
 }
 
 // ============================================================================ upsweep
@@ -1254,7 +1258,7 @@ void launch_upsweep(nbg_ctx* c, int levels, int multipole, double G, bool moment
         NBG_UPLEAF(false, false, false);
 #undef NBG_UPLEAF
     // internal nodes sit at depths 0..levels-1 (depth-40 cells are always leaves)
-    const int top = (levels == kLevelsShort) ? kLevelsShort - 1 : kMaxDepth - 1;
+    const int top = (levels <= kLevelsShort) ? levels - 1 : kMaxDepth - 1;
     for (int d = top; d >= 0; --d) {
         const int gd = d < 6 ? 64 : 1024;
 #define NBG_UPINT(Q, M, F)                                                                                           \

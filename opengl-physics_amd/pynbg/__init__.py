@@ -68,7 +68,7 @@ class Stats(ctypes.Structure):
                 ("ms_upsweep", ctypes.c_double), ("ms_walk", ctypes.c_double), ("ms_integrate", ctypes.c_double),
                 ("ms_total", ctypes.c_double), ("n_learned", ctypes.c_int64), ("ms_learned", ctypes.c_double),
                 ("t_advanced", ctypes.c_double), ("dt_last", ctypes.c_double), ("particle_evals", ctypes.c_int64),
-                ("rung_max", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("rung_max", ctypes.c_int32), ("key_levels", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

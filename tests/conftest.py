@@ -22,6 +22,14 @@ def plummer_f32(n, seed, total_mass=1.0):
     return out
 
 
+def keys_at(keys21, levels):
+    """The oracle's 21-level keys as a tree build with `levels`-level keys holds them
+    (levels 1..levels at their places, the deeper levels zero; 40 keeps the short word)."""
+    if levels >= 21:
+        return keys21
+    return keys21 & ~np.uint64((1 << (3 * (21 - levels))) - 1)
+
+
 def f64(d, keys=("x", "y", "z", "m")):
     return [np.asarray(d[k], np.float64) for k in keys]
 

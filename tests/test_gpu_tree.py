@@ -3,7 +3,7 @@ the oracle's serial-insertion AdaptiveTree<3> (projects/nbody_core/src/Tree.cpp:
 import numpy as np
 import pytest
 
-from conftest import f64, plummer_f32
+from conftest import f64, keys_at, plummer_f32
 
 pytestmark = pytest.mark.gpu
 
@@ -49,10 +49,49 @@ def test_plummer_tree_bit_exact(gpu_ctx_factory, n, seed):
     T = oracle.Tree(x, y, z, m, quad=True)
     o = T.export()
     keys_hi, _ = oracle.morton_keys(x, y, z, 21)
-    np.testing.assert_array_equal(g["keys"], np.sort(keys_hi))
+    assert st.key_levels == 16  # no cell at depth 16 holds more than 8 particles here
+    np.testing.assert_array_equal(g["keys"], np.sort(keys_at(keys_hi, st.key_levels)))
     _compare_trees(g, o, n)
     assert st.deep_keys == 0
     assert st.max_depth == o["depth"].max()
+
+
+def test_depth16_overflow_takes_21_level_keys(gpu_ctx_factory):
+    """A dense clump far below a wide root box (Plummer x 2e-5 plus one particle at 1): its
+    tree is deeper than 16 levels, so the 16-level keys overflow and the build is redone with
+    21-level keys -- topology bit-exact against the oracle -- and the context keeps 21 levels
+    (graph-replayed steps then equal eager ones bitwise) until a new upload, which tries 16
+    again."""
+    import oracle
+    import pynbg
+    s = plummer_f32(4096, 11)
+    for k in ("x", "y", "z"):
+        s[k] = (s[k] * np.float32(2e-5)).astype(np.float32)
+    s["x"][0], s["y"][0], s["z"][0] = np.float32(1.0), np.float32(0.0), np.float32(0.0)
+    x, y, z, m = f64(s)
+    ctx, g, st = _run(gpu_ctx_factory, pynbg, s["x"], s["y"], s["z"], s["m"])
+    o = oracle.Tree(x, y, z, m, quad=True).export()
+    print(f"depth {st.max_depth} key levels {st.key_levels}", flush=True)
+    assert 16 < o["depth"].max() <= 21
+    assert st.key_levels == 21 and st.deep_keys == 0
+    keys_hi, _ = oracle.morton_keys(x, y, z, 21)
+    np.testing.assert_array_equal(g["keys"], np.sort(keys_hi))
+    _compare_trees(g, o, len(x))
+    # steps: the captured graph uses the 21-level keys too
+    p = pynbg.make_params("barnes_hut", G=1.0, eps=1e-5, theta=0.5, dt=1e-6)
+    out = []
+    for flags in ({}, {"no_graph": True}):
+        ctx.upload(s["x"], s["y"], s["z"], s["m"])
+        ctx.accel(p)
+        q = pynbg.make_params("barnes_hut", G=1.0, eps=1e-5, theta=0.5, dt=1e-6, **flags)
+        ctx.step(q, 3)
+        out.append(ctx.download())
+    for k in out[0]:
+        np.testing.assert_array_equal(out[0][k], out[1][k])
+    # a shallow state in the same context starts from 16 levels again
+    t = plummer_f32(4096, 12)
+    ctx.upload(t["x"], t["y"], t["z"], t["m"])
+    assert ctx.accel(_params(pynbg)).key_levels == 16
 
 
 def test_tiny_and_degenerate_inputs(gpu_ctx_factory):
