@@ -632,18 +632,15 @@ __global__ __launch_bounds__(256) void k_leaf_depth(const uint64_t* __restrict__
             L[k - 1] = (i - k >= s0) ? cpl_at<LEVELS>(kh, kl, i - k, i) : -1;
             R[k - 1] = (i + k < s1) ? cpl_at<LEVELS>(kh, kl, i, i + k) : -1;
         }
-        // cell size at depth d: 1 + #{L >= d} + #{R >= d}; leaf depth = first d with size <= 8
-        D = -1;
-        for (int d = 0; d <= LEVELS; ++d) {
-            int size = 1;
+        // cell size at depth d: 1 + #{L >= d} + #{R >= d}; the leaf depth is the first d with
+        // size <= 8, i.e. one past v8, the 8th largest of the 16 prefix lengths.  L and R are
+        // non-increasing (sorted keys), so v8 = max over a of min(L[a-1], R[7-a]) (taking a
+        // from L and 8-a from R; an empty side counts as +infinity).
+        int v8 = max(R[kNcrit - 1], L[kNcrit - 1]);
 #pragma unroll
-            for (int k = 0; k < kNcrit; ++k) size += (L[k] >= d) + (R[k] >= d);
-            if (size <= kNcrit) {
-                D = d;
-                break;
-            }
-        }
-        if (D < 0) {
+        for (int a = 1; a < kNcrit; ++a) v8 = max(v8, min(L[a - 1], R[kNcrit - 1 - a]));
+        D = max(v8 + 1, 0);
+        if (D > LEVELS) {
             if (LEVELS <= kLevelsShort) {
                 atomicOr(&sc->overflow, 1);
                 atomicOr(&sc->sticky_overflow, 1);
