@@ -59,22 +59,25 @@ def kernel_traffic(n, kernel):
 
 
 PEAK_HBM_GBS = 8000.0
-SORT_PASSES = 8  # 8-bit digits over the 63-bit key
+def sort_passes(levels):
+    """LSD radix passes of 8-bit digits over a key of 3 bits per tree level (16 levels: 6)."""
+    return (3 * levels + 7) // 8
 
 
 def stage_hbm(st, steps, n, moments=False):
     """Per-stage HBM rate of the tree and integration stages from their HIP-event times
     inside the timed steps, against the algorithmic bytes of each (per particle unless
     noted; C = nodes): keys 28 (read x, y, z, m; write the 8-byte key and the 4-byte
-    index), radix sort 24 per pass (a 64-bit key and a 32-bit value read and written) x 8
-    passes, build 8 N + 40 C (keys in, node records out) + 32 N (the particle records
+    index), radix sort 24 per pass (a 64-bit key and a 32-bit value read and written) x the
+    passes of the tree's key levels (6 for 16 levels, 8 for 21), build 8 N + 40 C (keys in, node records out) + 32 N (the particle records
     gathered into sorted order), upsweep 16 N + 56 C (particles in; centre of mass, mass,
     quadrupole, geometry and radius out), kick-drift-kick 108.  counter_bytes: the same
     stage's HBM traffic per step from the committed FETCH_SIZE / WRITE_SIZE passes."""
     C = float(st.n_nodes)
-    algo = {"keys": 28.0 * n, "sort": 24.0 * SORT_PASSES * n, "build": 8.0 * n + 40.0 * C + 32.0 * n,
+    levels = st.key_levels
+    algo = {"keys": 28.0 * n, "sort": 24.0 * sort_passes(levels) * n, "build": 8.0 * n + 40.0 * C + 32.0 * n,
             "upsweep": 16.0 * n + 56.0 * C, "integrate": 108.0 * n}
-    counted = stage_counter_bytes(n, moments)
+    counted = stage_counter_bytes(n, moments, levels)
     out = {}
     for k, b in algo.items():
         ms = getattr(st, "ms_" + k) / steps
@@ -87,21 +90,26 @@ def stage_hbm(st, steps, n, moments=False):
 
 
 # the kernels of each tree / integration stage and their launches per step (short names
-# as profiles/traffic_plummer_1m.json keys them); the upsweep's internal pass runs once per
-# depth 20..0
-STAGE_KERNELS = {
-    "keys": [("k_reset", 1), ("void k_bbox<float>", 1), ("void k_root<float>", 1), ("void k_keys<21, float>", 1)],
-    "sort": [("k_radix_hist", SORT_PASSES), ("void k_scan_reduce<unsigned int>", SORT_PASSES),
-             ("void k_scan_down_inline<unsigned int>", SORT_PASSES), ("void k_radix_scatter<false>", SORT_PASSES)],
-    "build": [("void k_leaf_depth<21>", 1), ("void k_scan_reduce<unsigned long>", 1),
-              ("void k_scan_down_inline<unsigned long>", 1), ("k_node_totals", 1), ("void k_nodes<21>", 1),
-              ("k_link", 1), ("void k_node_finish<21>", 1), ("k_depth_scan", 1), ("k_depth_scatter", 1),
-              ("k_leaf_order", 1), ("k_pack4", 1), ("k_gather", 1)],
-    "integrate": [("k_kick_drift", 1), ("k_kick", 1)],
-}
+# as profiles/traffic_plummer_1m.json keys them) for keys of L levels; the upsweep's
+# internal pass runs once per level
+def stage_kernels(levels, moments):
+    P = sort_passes(levels)
+    m = "true" if moments else "false"
+    return {
+        "keys": [("k_reset", 1), ("void k_bbox<float>", 1), ("void k_root<float>", 1),
+                 (f"void k_keys<{levels}, float>", 1)],
+        "sort": [("k_radix_hist", P), ("void k_scan_reduce<unsigned int>", P),
+                 ("void k_scan_down_inline<unsigned int>", P), ("void k_radix_scatter<false>", P)],
+        "build": [(f"void k_leaf_depth<{levels}>", 1), ("void k_scan_reduce<unsigned long>", 1),
+                  ("void k_scan_down_inline<unsigned long>", 1), ("k_node_totals", 1), (f"void k_nodes<{levels}>", 1),
+                  ("k_link", 1), (f"void k_node_finish<{levels}>", 1), ("k_depth_scan", 1), ("k_depth_scatter", 1),
+                  ("k_leaf_order", 1), ("k_pack4", 1), ("k_gather", 1)],
+        "upsweep": [(f"void k_up_leaf<true, {m}, false>", 1), (f"void k_up_internal<true, {m}, false>", levels)],
+        "integrate": [("k_kick_drift", 1), ("k_kick", 1)],
+    }
 
 
-def stage_counter_bytes(n, moments):
+def stage_counter_bytes(n, moments, levels):
     """HBM bytes per step of each stage from the committed counter passes, or {}."""
     path = os.path.join(ROOT, "profiles", "traffic_plummer_1m.json")
     if not os.path.exists(path):
@@ -111,11 +119,8 @@ def stage_counter_bytes(n, moments):
     if d.get("n") != n:
         return {}
     t = d["bytes_per_launch"]
-    m = "true" if moments else "false"
-    kern = dict(STAGE_KERNELS)
-    kern["upsweep"] = [(f"void k_up_leaf<true, {m}, false>", 1), (f"void k_up_internal<true, {m}, false>", 21)]
     out = {}
-    for stage, ks in kern.items():
+    for stage, ks in stage_kernels(levels, moments).items():
         if all(k in t for k, _ in ks):
             out[stage] = int(sum(t[k] * c for k, c in ks))
     return out
@@ -531,6 +536,8 @@ def main():
     ap.add_argument("--learn-band", type=float, default=0.0)
     ap.add_argument("--learn-amin", type=float, default=1e-3,
                     help="learned pairs also need a correction scale G M s^3 / r^5 >= this")
+    ap.add_argument("--weights", default=None,
+                    help="learned model blob (.npy); default: the committed opengl-physics_amd/learned/farfield_mlp.npy")
     ap.add_argument("--eps", type=float, default=0.005)
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--multipole", default="quadrupole", choices=["monopole", "quadrupole"])
@@ -588,7 +595,9 @@ def main():
         wall_p, st_p, c0_p, c1_p = run_variant(ctx, pynbg, ic, plain, args.steps, args.warmup)
         log(f"plain theta={args.theta}: {1e3 * wall_p / args.steps:.3f} ms/step, rel_rms {err_plain:.3e}")
         if args.variant == "learned":
-            ctx.learn_set_weights(learned.load_weights())
+            blob = learned.load_weights(args.weights) if args.weights else learned.load_weights()
+            act_name = {v: k for k, v in pynbg.LEARN_ACT.items()}[int(blob[pynbg.LEARN_ACT_SLOT])]
+            ctx.learn_set_weights(blob)
             err_l = force_error(ctx, pynbg, ic, lrn, targets, ref)
             err_l_plain = force_error(ctx, pynbg, ic, dict(lrn, learned=False), targets, ref)
             wall_l, st_l, c0_l, c1_l = run_variant(ctx, pynbg, ic, lrn, args.steps, args.warmup)
@@ -685,7 +694,8 @@ def main():
                        "n": n, "eps": args.eps, "dt": args.dt, "multipole": "quadrupole", "kahan": kahan,
                        "variant": "learned", "theta": args.learned_theta, "learn_band": args.learn_band,
                        "learn_amin": args.learn_amin,
-                       "model": "13-64-64-64-3 SiLU, BF16 MFMA (opengl-physics_amd/learned/farfield_mlp.npy)"},
+                       "model": f"13-64-64-64-3 {'ReLU' if act_name == 'relu' else 'SiLU'}, BF16 MFMA "
+                                f"({os.path.relpath(args.weights, ROOT) if args.weights else 'opengl-physics_amd/learned/farfield_mlp.npy'})"},
             "rel_rms_force": err_l,
             "comparison": {
                 "plain": plain_block,

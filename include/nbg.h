@@ -244,22 +244,29 @@ int nbg_diagnostics_system(nbg_ctx* ctx, const nbg_params* p, int32_t sys, int32
 
 /* ---------------------------------------------------------------------------------
  * Learned far-field correction.  No reference counterpart: the reference has no learned
- * component.  A multilayer perceptron (13 inputs, three hidden layers of 64 with SiLU,
- * 3 outputs) maps the geometry and moments of one accepted (leaf group, cell) pair to the
- * residual of that cell's multipole force against direct summation, averaged over the
- * group's members; the walk adds the prediction to every member of the group.  Inference
- * and training run in BF16 on the matrix cores with FP32 accumulation.
+ * component.  A multilayer perceptron (13 inputs, three hidden layers of 64 with SiLU or
+ * ReLU, 3 outputs) maps the geometry and moments of one accepted (leaf group, cell) pair
+ * to the residual of that cell's multipole force against direct summation, averaged over
+ * the group's members; the walk adds the prediction to every member of the group.
+ * Inference and training run in BF16 on the matrix cores with FP32 accumulation.
  *
  * Model blob (NBG_LEARN_NPARAM floats): feature mean[16], feature 1/std[16], then
  * W1[13][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64][3] b4[3] (y = x W + b).
+ * Features 13..15 do not exist; the unused mean[NBG_LEARN_ACT_SLOT] holds the model's
+ * activation (NBG_LEARN_ACT_SILU = 0, NBG_LEARN_ACT_RELU = 1), which nbg_learn_standardize
+ * keeps.
  * ------------------------------------------------------------------------------- */
 #define NBG_LEARN_FEATURES 13
 #define NBG_LEARN_HIDDEN 64
 #define NBG_LEARN_NTRAIN 9411 /* trainable parameters */
 #define NBG_LEARN_NPARAM (32 + NBG_LEARN_NTRAIN)
+#define NBG_LEARN_ACT_SLOT 15
+#define NBG_LEARN_ACT_SILU 0
+#define NBG_LEARN_ACT_RELU 1
 
 /* Install a model blob (FP32 master copy; BF16 copy derived).  Required before
- * NBG_FLAG_LEARNED evaluations and before nbg_learn_train. */
+ * NBG_FLAG_LEARNED evaluations and before nbg_learn_train.  NBG_E_INVALID unless
+ * blob[NBG_LEARN_ACT_SLOT] is one of the NBG_LEARN_ACT_* codes. */
 int nbg_learn_set_weights(nbg_ctx* ctx, const float* blob, int64_t n);
 /* Copy the current model blob (after training) to the host. */
 int nbg_learn_get_weights(nbg_ctx* ctx, float* blob, int64_t n);

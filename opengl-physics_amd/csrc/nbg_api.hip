@@ -1163,6 +1163,11 @@ int nbg_learn_set_weights(nbg_ctx* c, const float* blob, int64_t n) {
             c->set_error("model blob holds a non-finite value");
             return NBG_E_INVALID;
         }
+    const float act = blob[NBG_LEARN_ACT_SLOT];
+    if (act != static_cast<float>(NBG_LEARN_ACT_SILU) && act != static_cast<float>(NBG_LEARN_ACT_RELU)) {
+        c->set_error("model blob: unknown activation code in blob[NBG_LEARN_ACT_SLOT]");
+        return NBG_E_INVALID;
+    }
     NBG_HIP_CHECK(c, hipSetDevice(c->device));
     int r = NBG_OK;
     if (!c->lw32) {
@@ -1180,6 +1185,7 @@ int nbg_learn_set_weights(nbg_ctx* c, const float* blob, int64_t n) {
     NBG_HIP_CHECK(c, hipMemset(c->adam_m, 0, NBG_LEARN_NTRAIN * sizeof(float)));
     NBG_HIP_CHECK(c, hipMemset(c->adam_v, 0, NBG_LEARN_NTRAIN * sizeof(float)));
     c->adam_t = 0;
+    c->learn_act = static_cast<int>(act);
     c->have_weights = true;
     return NBG_OK;
 }
@@ -1313,6 +1319,7 @@ int nbg_learn_standardize(nbg_ctx* c, int64_t count) {
         norm[k] = used ? static_cast<float>(mu) : 0.f;
         norm[16 + k] = used ? static_cast<float>(1.0 / std::sqrt(var)) : 1.f;
     }
+    norm[NBG_LEARN_ACT_SLOT] = static_cast<float>(c->learn_act);  // the model's activation code stays
     const double wmean = h[32] / static_cast<double>(n);
     c->pool_inv_wmean = (wmean > 0.0) ? static_cast<float>(1.0 / wmean) : 1.f;
     NBG_HIP_CHECK(c, hipMemcpy(c->lw32, norm, sizeof(norm), hipMemcpyHostToDevice));

@@ -208,6 +208,7 @@ struct nbg_ctx {
 
     // learned far-field correction (allocated on first use)
     float4* nd_mom = nullptr;       // 4 per node: second (6) and third (10) moments about the FP32 com, G*m weights
+    int learn_act = NBG_LEARN_ACT_SILU;  // activation of the installed model (blob[NBG_LEARN_ACT_SLOT])
     bool learn_two_pass = false;    // corrections summed with per-leaf fixed-point units (after a range overflow)
     long long* lacc = nullptr;      // per node, 4 words: a leaf group's correction as exact fixed-point sums
     uint32_t* lunit = nullptr;      // per node: bits of the largest |component| of its pairs' corrections

@@ -23,6 +23,7 @@
 // the B operand of the next with no data movement: D rows 4g+j of row block mb are the
 // B rows 4g+j of K block mb.
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "nbg_common.hpp"
@@ -49,7 +50,10 @@ constexpr int kOffW4 = kOffB3 + kH;        // 9216
 constexpr int kOffB4 = kOffW4 + kH * 3;    // 9408
 constexpr int kNTrain = kOffB4 + 3;        // 9411
 static_assert(kNTrain == NBG_LEARN_NTRAIN, "blob layout");
+static_assert(NBG_LEARN_ACT_SLOT >= NBG_LEARN_FEATURES && NBG_LEARN_ACT_SLOT < 16, "activation slot is an unused mean");
 constexpr int kNorm = 32;                  // mean[16], 1/std[16]
+constexpr int kActSilu = NBG_LEARN_ACT_SILU;
+constexpr int kActRelu = NBG_LEARN_ACT_RELU;
 
 __device__ __forceinline__ uint16_t bf16_rne(float f) {
     uint32_t b = __float_as_uint(f);
@@ -67,13 +71,26 @@ __device__ __forceinline__ float sigmoid_f(float z) {
     return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
 }
 __device__ __forceinline__ float silu_f(float z) { return z * sigmoid_f(z); }
-// The network's activation: SiLU of the BF16-rounded pre-activation, rounded to BF16
-// (inference reads it from a 64K-entry table indexed by those BF16 bits).
+// The network's activation, chosen by the model (kActSilu, kActRelu), of the BF16-rounded
+// pre-activation, rounded to BF16.  SiLU inference reads it from a 64K-entry table indexed
+// by those BF16 bits; ReLU is exact on the BF16 bits (a negative value, -0 included, has
+// its sign bit set and becomes +0).
 __device__ __forceinline__ uint16_t act_bits(uint16_t zb) { return bf16_rne(silu_f(bf16_f32(zb))); }
-__device__ __forceinline__ uint16_t act_of(float z) { return act_bits(bf16_rne(z)); }
+template <int ACT>
+__device__ __forceinline__ uint16_t act_of(float z) {
+    const uint16_t b = bf16_rne(z);
+    if (ACT == kActRelu) return (b & 0x8000u) ? uint16_t{0} : b;
+    return act_bits(b);
+}
 __device__ __forceinline__ float dsilu_f(float z) {
     const float s = sigmoid_f(z);
     return s * (1.f + z * (1.f - s));
+}
+// The activation's derivative at the unrounded pre-activation (training).
+template <int ACT>
+__device__ __forceinline__ float dact(float z) {
+    if (ACT == kActRelu) return z > 0.f ? 1.f : 0.f;
+    return dsilu_f(z);
 }
 
 __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
@@ -188,12 +205,25 @@ __device__ __forceinline__ void load_frags(Frags& F, const uint16_t* __restrict_
 }
 
 // Activation of a D fragment, packed as the next layer's B fragment.
+template <int ACT>
 struct ActDirect {  // computed (training)
     __device__ __forceinline__ bf16x4 operator()(f32x4 z) const {
         bf16x4 h;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) h[j] = static_cast<short>(act_of(z[j]));
+        for (int j = 0; j < 4; ++j) h[j] = static_cast<short>(act_of<ACT>(z[j]));
         return h;
+    }
+};
+// Inference with ReLU: one v_cvt_pk_bf16_f32 and one v_pk_max_i16 per two values (the
+// BF16 bits of a negative value, -0 included, are a negative 16-bit integer).
+struct ActRelu {
+    __device__ __forceinline__ bf16x4 operator()(f32x4 z, int) const {
+        typedef short s16x2 __attribute__((ext_vector_type(2)));
+        const s16x2 zero = {0, 0};
+        const s16x2 lo = __builtin_elementwise_max(__builtin_bit_cast(s16x2, bf16x2_rne(z[0], z[1])), zero);
+        const s16x2 hi = __builtin_elementwise_max(__builtin_bit_cast(s16x2, bf16x2_rne(z[2], z[3])), zero);
+        return __builtin_bit_cast(bf16x4, (static_cast<uint64_t>(__builtin_bit_cast(uint32_t, hi)) << 32) |
+                                              __builtin_bit_cast(uint32_t, lo));
     }
 };
 // Inference: the same function, three of the four row blocks looked up in a 64K-entry LDS
@@ -371,65 +401,6 @@ __device__ __forceinline__ void mlp32_tile2(const Frags32& F, bf16x4 xa, bf16x4 
     yb = b;
 }
 
-// Two 16-pair tiles through the network in lockstep (independent MFMA chains and table
-// lookups interleave and hide each other's latency).
-template <class Act>
-__device__ __forceinline__ void mlp_tile2(const Frags& F, bf16x4 xa, bf16x4 xb, const Act& act, f32x4& ya,
-                                          f32x4& yb) {
-    bf16x4 ha[4], hb[4];
-    f32x4 za[4], zb[4];
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb) {
-        za[mb] = mfma16(F.a1[mb], xa, F.b1[mb]);
-        zb[mb] = mfma16(F.a1[mb], xb, F.b1[mb]);
-    }
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb) {
-        ha[mb] = act(za[mb]);
-        hb[mb] = act(zb[mb]);
-    }
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb) {
-        f32x4 a = F.b2[mb], b = F.b2[mb];
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) {
-            a = mfma16(F.a2[mb][kb], ha[kb], a);
-            b = mfma16(F.a2[mb][kb], hb[kb], b);
-        }
-        za[mb] = a;
-        zb[mb] = b;
-    }
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb) {
-        ha[mb] = act(za[mb]);
-        hb[mb] = act(zb[mb]);
-    }
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb) {
-        f32x4 a = F.b3[mb], b = F.b3[mb];
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) {
-            a = mfma16(F.a3[mb][kb], ha[kb], a);
-            b = mfma16(F.a3[mb][kb], hb[kb], b);
-        }
-        za[mb] = a;
-        zb[mb] = b;
-    }
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb) {
-        ha[mb] = act(za[mb]);
-        hb[mb] = act(zb[mb]);
-    }
-    f32x4 a = F.b4, b = F.b4;
-#pragma unroll
-    for (int kb = 0; kb < 4; ++kb) {
-        a = mfma16(F.a4[kb], ha[kb], a);
-        b = mfma16(F.a4[kb], hb[kb], b);
-    }
-    ya = a;
-    yb = b;
-}
-
 // ---------------------------------------------------------------------------- inference
 // One persistent kernel per evaluation: a lane computes the standardised BF16 feature row
 // of one listed pair (16 x 2 B, entries 13..15 zero) and its output scale from the node
@@ -451,6 +422,7 @@ struct InferArgs {
 };
 
 constexpr int kInferWaves = 8;
+constexpr int kInferCUs = 256;        // MI355X: 8 XCDs x 32 CUs
 
 // The node words one pair's features are computed from.
 struct PairRaw {
@@ -458,16 +430,21 @@ struct PairRaw {
     float lr, size2;
 };
 
-// Persistent: one 8-wave block per CU (the 128 KB activation table fills most of the
-// LDS).  A wave takes 64 pairs per step as four 16-pair tiles, two at a time.  The pair
-// ids run two steps ahead and the node words one step ahead of the step computed.
-template <bool TWO_PASS>
+// Persistent: one 8-wave block per CU (the weight fragments held in registers take ~250
+// VGPRs, so two waves per SIMD; with SiLU the 128 KB activation table fills most of the
+// LDS as well).  A wave takes 64 pairs per step as four 16-pair tiles, two at a time.  The pair ids run two steps ahead and the node words one
+// step ahead of the step computed.
+template <bool TWO_PASS, int ACT>
 __global__ __launch_bounds__(64 * kInferWaves) void k_learn_infer(InferArgs A) {
-    __shared__ __attribute__((aligned(16))) uint16_t s_act[65536];
+    __shared__ __attribute__((aligned(16))) uint16_t s_act[ACT == kActSilu ? 65536 : 2];
     __shared__ uint2 s_row[kInferWaves][64][4];
-    for (int i = threadIdx.x; i < 65536; i += blockDim.x) s_act[i] = act_bits(static_cast<uint16_t>(i));
-    __syncthreads();
-    const ActTable act{s_act};
+    if (ACT == kActSilu) {
+        for (int i = threadIdx.x; i < 65536; i += blockDim.x) s_act[i] = act_bits(static_cast<uint16_t>(i));
+        __syncthreads();
+    }
+    typedef typename std::conditional<ACT == kActSilu, ActTable, ActRelu>::type Act;
+    Act act;
+    if constexpr (ACT == kActSilu) act.tab = s_act;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = lane_id();
     const int g = lane >> 4, c = lane & 15;
@@ -914,6 +891,7 @@ struct TrainArgs {
 
 // One wave per block, 64 samples: forward with the pre-activations kept in LDS, then the
 // backward pass; weight gradients are matrix-core products over the block's samples.
+template <int ACT>
 __global__ __launch_bounds__(64) void k_learn_fwdbwd(TrainArgs A) {
     __shared__ __attribute__((aligned(16))) uint16_t s_x[64][16];
     __shared__ __attribute__((aligned(16))) float s_z[3][64][kH];
@@ -958,7 +936,7 @@ __global__ __launch_bounds__(64) void k_learn_fwdbwd(TrainArgs A) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const bf16x4 x = *reinterpret_cast<const bf16x4*>(&s_x[16 * t + c][4 * g]);
-            const f32x4 y = mlp_tile<true>(F, x, &s_z[0][0][0], 16 * t, lane, ActDirect{});
+            const f32x4 y = mlp_tile<true>(F, x, &s_z[0][0][0], 16 * t, lane, ActDirect<ACT>{});
             if (g == 0) {
                 s_y[16 * t + c][0] = y[0];
                 s_y[16 * t + c][1] = y[1];
@@ -994,7 +972,7 @@ __global__ __launch_bounds__(64) void k_learn_fwdbwd(TrainArgs A) {
     {
         float a0 = 0.f, a1 = 0.f, a2 = 0.f;
         for (int n = 0; n < 64; ++n) {
-            const float h = bf16_f32(act_of(s_z[2][n][lane]));
+            const float h = bf16_f32(act_of<ACT>(s_z[2][n][lane]));
             a0 += h * s_y[n][0];
             a1 += h * s_y[n][1];
             a2 += h * s_y[n][2];
@@ -1010,7 +988,7 @@ __global__ __launch_bounds__(64) void k_learn_fwdbwd(TrainArgs A) {
         const uint16_t* w4 = A.w16 + kOffW4;  // the BF16 W4[i][o] the forward used
         for (int i = 0; i < kH; ++i) {
             const float dh = dy0 * bf16_f32(w4[3 * i]) + dy1 * bf16_f32(w4[3 * i + 1]) + dy2 * bf16_f32(w4[3 * i + 2]);
-            s_d[lane][i] = bf16_rne(dh * dsilu_f(s_z[2][lane][i]));
+            s_d[lane][i] = bf16_rne(dh * dact<ACT>(s_z[2][lane][i]));
         }
     }
     __syncthreads();
@@ -1036,7 +1014,7 @@ __global__ __launch_bounds__(64) void k_learn_fwdbwd(TrainArgs A) {
                     bf16x4 v;
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        v[j] = static_cast<short>(act_of(zin[(16 * nb + 4 * g + j) * kH + 16 * ib + c]));
+                        v[j] = static_cast<short>(act_of<ACT>(zin[(16 * nb + 4 * g + j) * kH + 16 * ib + c]));
                     Ain[ib][nb] = v;
                 }
 #pragma unroll
@@ -1065,7 +1043,7 @@ __global__ __launch_bounds__(64) void k_learn_fwdbwd(TrainArgs A) {
             for (int n = 0; n < 64; ++n) s += bf16_f32(dcur[n * kH + lane]);
             gp[offB + lane] = s;
         }
-        // dH_in^T[i][n] = sum_h W[i][h] dZ^T[h][n]; dZ_in = dH_in * silu'(z_in)
+        // dH_in^T[i][n] = sum_h W[i][h] dZ^T[h][n]; dZ_in = dH_in * act'(z_in)
         {
             f32x4 acc[4][4];  // [ib][nb]
 #pragma unroll
@@ -1096,7 +1074,7 @@ __global__ __launch_bounds__(64) void k_learn_fwdbwd(TrainArgs A) {
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int i = 16 * ib + 4 * g + j;
-                        v[j] = static_cast<short>(bf16_rne(acc[ib][nb][j] * dsilu_f(zin[n * kH + i])));
+                        v[j] = static_cast<short>(bf16_rne(acc[ib][nb][j] * dact<ACT>(zin[n * kH + i])));
                     }
                     *reinterpret_cast<bf16x4*>(dnext + n * kH + 16 * ib + 4 * g) = v;
                 }
@@ -1241,12 +1219,20 @@ void launch_learn_correct(nbg_ctx* c, const nbg_params& p) {
     A.pv = c->lpv;
     A.unit = c->lunit;
     A.acc = c->lacc;
+    const dim3 blk(64 * kInferWaves), grid(kInferCUs);
+    const bool relu = c->learn_act == kActRelu;
     if (c->learn_two_pass) {
-        hipLaunchKernelGGL(k_learn_infer<true>, dim3(256), dim3(64 * kInferWaves), 0, c->stream, A);
+        if (relu)
+            hipLaunchKernelGGL((k_learn_infer<true, kActRelu>), grid, blk, 0, c->stream, A);
+        else
+            hipLaunchKernelGGL((k_learn_infer<true, kActSilu>), grid, blk, 0, c->stream, A);
         hipLaunchKernelGGL(k_learn_sum, dim3(2048), dim3(256), 0, c->stream, c->lpairs, c->lpv, c->lunit, c->dsc,
                            static_cast<uint32_t>(c->lpairs_cap), c->lacc);
     } else {
-        hipLaunchKernelGGL(k_learn_infer<false>, dim3(256), dim3(64 * kInferWaves), 0, c->stream, A);
+        if (relu)
+            hipLaunchKernelGGL((k_learn_infer<false, kActRelu>), grid, blk, 0, c->stream, A);
+        else
+            hipLaunchKernelGGL((k_learn_infer<false, kActSilu>), grid, blk, 0, c->stream, A);
     }
     launch_learn_apply(c);
 }
@@ -1305,7 +1291,10 @@ void launch_learn_fwdbwd(nbg_ctx* c, const float* feat, const float* tgt, int64_
     A.grad_part = c->grad_part;
     A.loss_part = c->loss_part;
     A.out = out;
-    hipLaunchKernelGGL(k_learn_fwdbwd, dim3(nblocks), dim3(64), 0, c->stream, A);
+    if (c->learn_act == kActRelu)
+        hipLaunchKernelGGL(k_learn_fwdbwd<kActRelu>, dim3(nblocks), dim3(64), 0, c->stream, A);
+    else
+        hipLaunchKernelGGL(k_learn_fwdbwd<kActSilu>, dim3(nblocks), dim3(64), 0, c->stream, A);
 }
 
 void launch_learn_adam(nbg_ctx* c, int nblocks, float lr, int64_t t, float* loss_slot) {

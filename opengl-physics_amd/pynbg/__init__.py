@@ -42,6 +42,8 @@ LEARN_FEATURES = 13
 LEARN_HIDDEN = 64
 LEARN_NTRAIN = 9411
 LEARN_NPARAM = 32 + LEARN_NTRAIN
+LEARN_ACT_SLOT = 15          # blob[15] (an unused feature mean): the model's activation
+LEARN_ACT = {"silu": 0, "relu": 1}
 
 
 class Params(ctypes.Structure):

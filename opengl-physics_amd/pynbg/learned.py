@@ -20,19 +20,21 @@ import time
 
 import numpy as np
 
-from . import (LEARN_FEATURES, LEARN_HIDDEN, LEARN_NPARAM, LEARN_NTRAIN, Context, ic_hernquist, ic_plummer,
-               make_params, to_f32)
+from . import (LEARN_ACT, LEARN_ACT_SLOT, LEARN_FEATURES, LEARN_HIDDEN, LEARN_NPARAM, LEARN_NTRAIN, Context,
+               ic_hernquist, ic_plummer, make_params, to_f32)
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DEFAULT_WEIGHTS = os.path.join(_PKG, "learned", "farfield_mlp.npy")
 
 
-def init_blob(seed=0):
+def init_blob(seed=0, act=LEARN_ACT["silu"]):
     """He-style initialisation of the 13-64-64-64-3 network (output layer scaled by 0.1);
-    standardisation entries start as mean 0, 1/std 1."""
+    standardisation entries start as mean 0, 1/std 1; blob[LEARN_ACT_SLOT] = the
+    activation code (0 SiLU, 1 ReLU)."""
     rng = np.random.default_rng(seed)
     blob = np.zeros(LEARN_NPARAM, np.float32)
     blob[16:32] = 1.0
+    blob[LEARN_ACT_SLOT] = act
     sizes = [("W1", LEARN_FEATURES * LEARN_HIDDEN, LEARN_FEATURES), ("b1", LEARN_HIDDEN, 0),
              ("W2", LEARN_HIDDEN * LEARN_HIDDEN, LEARN_HIDDEN), ("b2", LEARN_HIDDEN, 0),
              ("W3", LEARN_HIDDEN * LEARN_HIDDEN, LEARN_HIDDEN), ("b3", LEARN_HIDDEN, 0),
@@ -101,12 +103,14 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--act", choices=sorted(LEARN_ACT), default="relu",
+                    help="hidden-layer activation (ReLU: one packed instruction per two values in inference)")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
     snaps = [("plummer", s) for s in a.plummer_seeds] + [("hernquist", s) for s in a.hernquist_seeds]
     t0 = time.time()
     with Context(a.n) as ctx:
-        ctx.learn_set_weights(init_blob(a.seed))
+        ctx.learn_set_weights(init_blob(a.seed, LEARN_ACT[a.act]))
         pool = collect(ctx, snaps, a.n, a.eps, a.theta, a.band, a.max_per_snapshot, log, amin=a.amin)
         ctx.learn_standardize()
         t1 = time.time()
@@ -116,7 +120,7 @@ def main(argv=None):
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     np.save(a.out, blob.astype(np.float32), allow_pickle=False)
     meta = dict(samples=int(pool), snapshots=[f"{k}:{s}" for k, s in snaps], n=a.n, eps=a.eps, theta=a.theta,
-                band=a.band, amin=a.amin, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed,
+                band=a.band, amin=a.amin, activation=a.act, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed,
                 loss_first=float(np.mean(loss[:50])), loss_last=float(np.mean(loss[-200:])),
                 collect_s=round(t1 - t0, 2), train_s=round(t2 - t1, 2))
     with open(os.path.splitext(a.out)[0] + ".json", "w") as f:

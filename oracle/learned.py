@@ -32,6 +32,8 @@ for _name, _size in (("W1", NF * NH), ("b1", NH), ("W2", NH * NH), ("b2", NH), (
     _o += _size
 NTRAIN = _o            # 9411
 NPARAM = 32 + NTRAIN   # standardisation (mean[16], 1/std[16]) first
+ACT_SLOT = 15          # the unused mean[15] holds the activation code
+ACT_SILU, ACT_RELU = 0, 1
 
 
 # ---------------------------------------------------------------------------- model blob
@@ -47,11 +49,12 @@ def unpack(blob):
     return d
 
 
-def init_blob(seed=0):
+def init_blob(seed=0, act=ACT_SILU):
     """Seeded He-style initialisation (the same recipe the training driver uses)."""
     rng = np.random.default_rng(seed)
     blob = np.zeros(NPARAM, np.float32)
     blob[16:32] = 1.0
+    blob[ACT_SLOT] = act
     t = blob[32:]
     for k, fan_in in (("W1", NF), ("W2", NH), ("W3", NH), ("W4", NH)):
         o, s = OFF[k]
@@ -77,17 +80,37 @@ def dsilu(z):
     return s * (1.0 + z * (1.0 - s))
 
 
+def relu(z):
+    return np.maximum(z, 0.0)
+
+
+def drelu(z):
+    return (np.asarray(z) > 0.0).astype(np.float64)
+
+
+def activation(blob):
+    """(act, act') of the model: blob[ACT_SLOT] = 0 SiLU, 1 ReLU."""
+    code = int(np.asarray(blob)[ACT_SLOT])
+    if code == ACT_SILU:
+        return silu, dsilu
+    if code == ACT_RELU:
+        return relu, drelu
+    raise ValueError(f"unknown activation code {code}")
+
+
 def mlp_forward(blob, feats, emulate_bf16=False):
     """feats (n, >=13) raw features -> (y (n,3), cache).  With emulate_bf16 the inputs,
     weights and hidden activations are rounded to bfloat16 as the matrix-core kernels do,
     and the activation takes the BF16-rounded pre-activation (accumulation stays FP64
-    here, FP32 on the device; the backward uses SiLU' of the unrounded pre-activation)."""
+    here, FP32 on the device; the backward uses the activation's derivative at the
+    unrounded pre-activation).  The activation is the model's (blob[ACT_SLOT])."""
     P = unpack(blob)
     r = bf16 if emulate_bf16 else (lambda v: np.asarray(v, np.float64))
     x = r((np.asarray(feats, np.float64)[:, :NF] - P["mu"][:NF]) * P["isd"][:NF])
     W = {k: r(P[k]) for k in ("W1", "W2", "W3", "W4")}
-    # the device activation is SiLU of the BF16-rounded pre-activation, rounded to BF16
-    act = (lambda z: bf16(silu(bf16(z)))) if emulate_bf16 else silu
+    # the device activation is that of the BF16-rounded pre-activation, rounded to BF16
+    f, _ = activation(blob)
+    act = (lambda z: bf16(f(bf16(z)))) if emulate_bf16 else f
     z1 = x @ W["W1"] + P["b1"]
     h1 = act(z1)
     z2 = h1 @ W["W2"] + P["b2"]
@@ -109,6 +132,7 @@ def mlp_backward(blob, feats, targets, emulate_bf16=False):
     """Gradient of mlp_loss with respect to the trainable parameters, in blob order
     (NTRAIN entries), and the predictions."""
     y, c = mlp_forward(blob, feats, emulate_bf16)
+    _, dact = activation(blob)
     t = np.asarray(targets, np.float64)
     n = len(t)
     dy = 2.0 * t[:, 3:4] * (y - t[:, :3]) / n
@@ -116,13 +140,13 @@ def mlp_backward(blob, feats, targets, emulate_bf16=False):
     g = {}
     g["W4"] = c["h3"].T @ dy
     g["b4"] = dy.sum(0)
-    dz3 = (dy @ W["W4"].T) * dsilu(c["z3"])
+    dz3 = (dy @ W["W4"].T) * dact(c["z3"])
     g["W3"] = c["h2"].T @ dz3
     g["b3"] = dz3.sum(0)
-    dz2 = (dz3 @ W["W3"].T) * dsilu(c["z2"])
+    dz2 = (dz3 @ W["W3"].T) * dact(c["z2"])
     g["W2"] = c["h1"].T @ dz2
     g["b2"] = dz2.sum(0)
-    dz1 = (dz2 @ W["W2"].T) * dsilu(c["z1"])
+    dz1 = (dz2 @ W["W2"].T) * dact(c["z1"])
     g["W1"] = c["x"].T @ dz1
     g["b1"] = dz1.sum(0)
     out = np.zeros(NTRAIN)

@@ -184,11 +184,12 @@ def test_samples_match_oracle(gpu_ctx_factory):
     np.testing.assert_allclose(Wg, W_or, rtol=2e-3)
 
 
-def test_forward_backward_match_oracle(gpu_ctx_factory):
+@pytest.mark.parametrize("act", ["silu", "relu"])
+def test_forward_backward_match_oracle(gpu_ctx_factory, act):
     import learned as LO
     import pynbg
     rng = np.random.default_rng(11)
-    blob = LO.init_blob(11)
+    blob = LO.init_blob(11, pynbg.LEARN_ACT[act])
     blob[:13] = rng.normal(size=13) * 0.2
     blob[16:29] = 0.5 + rng.random(13)
     t = blob[32:]
@@ -207,7 +208,7 @@ def test_forward_backward_match_oracle(gpu_ctx_factory):
     ey = np.abs(y - y_or).max() / np.abs(y_or).max()
     eg = np.linalg.norm(grad - g_or) / np.linalg.norm(g_or)
     cos = float(grad @ g_or / (np.linalg.norm(grad) * np.linalg.norm(g_or)))
-    print(f"forward max rel {ey:.2e}  loss {loss:.6f} vs {loss_or:.6f}  grad rel L2 {eg:.2e} cos {cos:.6f}")
+    print(f"{act}: forward max rel {ey:.2e}  loss {loss:.6f} vs {loss_or:.6f}  grad rel L2 {eg:.2e} cos {cos:.6f}")
     # BF16 operands: inputs, weights and activations round identically on both sides;
     # the device also rounds the backward deltas to BF16 (2^-8 relative each)
     assert ey < 5e-3
@@ -363,6 +364,10 @@ def test_learned_argument_errors(gpu_ctx_factory):
     with pytest.raises(pynbg.NbgError, match="NBG_LEARN_NPARAM"):
         ctx.learn_set_weights(np.zeros(10, np.float32))
     from pynbg import learned
+    bad = learned.init_blob(0)
+    bad[pynbg.LEARN_ACT_SLOT] = 2
+    with pytest.raises(pynbg.NbgError, match="activation"):
+        ctx.learn_set_weights(bad)
     ctx.learn_set_weights(learned.init_blob(0))
     with pytest.raises(pynbg.NbgError, match="quadrupoles"):
         ctx.accel(pynbg.make_params("barnes_hut", "monopole", theta=0.8, learned=True))
@@ -482,14 +487,16 @@ def _two_scale_state(n=4096):
     return {k: v.astype(np.float32) for k, v in out.items()}
 
 
+@pytest.mark.parametrize("act", ["silu", "relu"])
 @pytest.mark.parametrize("case", ["plummer", "two_scale"])
-def test_learned_correction_equals_oracle_mlp_sums(gpu_ctx_factory, case):
+def test_learned_correction_equals_oracle_mlp_sums(gpu_ctx_factory, case, act):
     """The learned evaluation minus the plain walk at the same theta, per particle, equals
     the sum over its leaf's listed pairs of scale * MLP(features) computed by the oracle
     (FP64 MLP with the kernels' BF16 roundings, scale = G M s^3 / r^5 from the exported
     tree).  The two-scale state drives the sums onto the per-leaf-unit fallback (its terms
     are checked to exceed the one-pass range), the Plummer state keeps the one-pass sums;
-    both are also bitwise reproducible."""
+    both are also bitwise reproducible.  The model is the committed one when its activation
+    is `act`, else a seeded initialisation with that activation."""
     import learned as learned_oracle
     import pynbg
     from pynbg import learned
@@ -498,7 +505,10 @@ def test_learned_correction_equals_oracle_mlp_sums(gpu_ctx_factory, case):
     else:
         s, eps = _two_scale_state(), 5e-8
     n = len(s["x"])
-    blob = learned.load_weights(WEIGHTS) if os.path.exists(WEIGHTS) else learned.init_blob(4)
+    code = pynbg.LEARN_ACT[act]
+    blob = learned.load_weights(WEIGHTS) if os.path.exists(WEIGHTS) else None
+    if blob is None or int(blob[pynbg.LEARN_ACT_SLOT]) != code:
+        blob = learned.init_blob(4, code)
     kw = dict(G=1.0, eps=eps, theta=0.8, learn_band=0.0)
     ctx = gpu_ctx_factory(n)
     ctx.learn_set_weights(blob)
@@ -534,7 +544,7 @@ def test_learned_correction_equals_oracle_mlp_sums(gpu_ctx_factory, case):
     root = int(np.argmin(e["depth"]))
     a_ref = float(e["mass"][root]) / float(e["half"][root]) ** 2
     over = np.abs(terms).max() * 2.0 ** (30 - np.floor(np.log2(a_ref))) >= 2.0 ** 49
-    print(f"{case}: pairs {npairs}, correction rel err vs oracle {err:.2e}, largest term / one-pass limit "
+    print(f"{case} {act}: pairs {npairs}, correction rel err vs oracle {err:.2e}, largest term / one-pass limit "
           f"{np.abs(terms).max() * 2.0 ** (30 - np.floor(np.log2(a_ref))) / 2.0 ** 49:.2e}")
     assert over == (case == "two_scale")
     assert err < 1e-2

@@ -15,9 +15,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 WEIGHTS = os.path.join(ROOT, "opengl-physics_amd", "learned", "farfield_mlp.npy")
 
 
-def _random_case(seed, n=9):
+def _random_case(seed, n=9, act=L.ACT_SILU):
     rng = np.random.default_rng(seed)
-    blob = L.init_blob(seed).astype(np.float64)
+    blob = L.init_blob(seed, act).astype(np.float64)
     blob[:13] = rng.normal(size=13) * 0.1
     blob[16:29] = 0.5 + rng.random(13)
     t = blob[32:]
@@ -43,6 +43,38 @@ def test_mlp_backward_matches_finite_differences():
         b2[32 + i] -= h
         fd = (L.mlp_loss(b1, X, T) - L.mlp_loss(b2, X, T)) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])) + 1e-9, (i, fd, g[i])
+
+
+def test_relu_mlp_backward_matches_finite_differences():
+    """The ReLU model (blob[ACT_SLOT] = 1): the same finite-difference check on a case
+    whose pre-activations all stay at least 1e-3 away from the kink, so a +-1e-6 step
+    never crosses it (seeds are tried in order until one qualifies)."""
+    for seed in range(3, 40):
+        blob, X, T = _random_case(seed, act=L.ACT_RELU)
+        _, c = L.mlp_forward(blob, X)
+        if min(np.min(np.abs(c[k])) for k in ("z1", "z2", "z3")) > 1e-3:
+            break
+    else:
+        pytest.fail("no case with pre-activations away from zero")
+    g, _ = L.mlp_backward(blob, X, T)
+    assert np.all(c["h1"] >= 0.0) and np.any(c["h1"] == 0.0)      # the kink is exercised
+    rng = np.random.default_rng(1)
+    idx = np.concatenate([rng.choice(L.NTRAIN, 60, replace=False), [0, 12 * 64 + 5, L.OFF["b4"][0] + 2]])
+    h = 1e-6
+    for i in idx:
+        b1 = blob.copy()
+        b1[32 + i] += h
+        b2 = blob.copy()
+        b2[32 + i] -= h
+        fd = (L.mlp_loss(b1, X, T) - L.mlp_loss(b2, X, T)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])) + 1e-9, (i, fd, g[i])
+
+
+def test_activation_code_is_checked():
+    blob, X, _ = _random_case(5)
+    blob[L.ACT_SLOT] = 2
+    with pytest.raises(ValueError):
+        L.mlp_forward(blob, X)
 
 
 def test_mlp_backward_planted_mistake_is_caught():
