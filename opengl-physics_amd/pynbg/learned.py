@@ -99,7 +99,7 @@ def main(argv=None):
     ap.add_argument("--amin", type=float, default=0.0,
                     help="pairs also need a correction scale G M s^3 / r^5 >= this (the evaluation's learn_amin)")
     ap.add_argument("--max-per-snapshot", type=int, default=0)
-    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=100000)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=0)
