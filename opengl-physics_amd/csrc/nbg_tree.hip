@@ -1118,6 +1118,7 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
                                                      float4* __restrict__ nd_com, float4* __restrict__ nd_quad,
                                                      float4* __restrict__ nd_aux, double4* __restrict__ nd_com64,
                                                      float4* __restrict__ nd_mom) {
+    static_assert(!(MOM && F64), "the learned moments are built on FP32 states only");
     const int cnt = static_cast<int>(sc->depth_count[depth]);
     const int off = static_cast<int>(sc->depth_offset[depth]);
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
@@ -1129,16 +1130,12 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
 #pragma unroll
         for (int c = 0; c < 8; ++c) src[c] = ch[c] >= 0 ? ch[c] : id;
         double4 cd[8];
-        float4 cc[8], ca[8], cq[8], cm[MOM ? 32 : 1];
+        float4 cc[8], ca[8], cq[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             cd[c] = nd_com64[src[c]];
             ca[c] = nd_aux[src[c]];
             if (QUAD) cq[c] = nd_quad[src[c]];
-            if (MOM) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) cm[4 * c + j] = nd_mom[4 * src[c] + j];
-            }
         }
         // Values used only on one side of a select would otherwise be loaded inside a
         // branch (one memory round trip per child); pinning them here keeps one wait.
@@ -1150,10 +1147,6 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
                                 static_cast<float>(G * cd[c].w));
             pin(ca[c]);
             if (QUAD) pin(cq[c]);
-            if (MOM) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) pin(cm[4 * c + j]);
-            }
         }
         double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
 #pragma unroll
@@ -1173,7 +1166,6 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
         }
         const float fx = static_cast<float>(gx), fy = static_cast<float>(gy), fz = static_cast<float>(gz);
         float qxx = 0.f, qyy = 0.f, qxy = 0.f, qxz = 0.f, qyz = 0.f, rmax = 0.f;
-        float mo[16] = {};
         const bool massive = M > 0.0;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
@@ -1184,9 +1176,35 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
             const float d2 = dx * dx + dy * dy + dz * dz;
             rmax = live ? fmaxf(rmax, ca[c].z + sqrtf(d2)) : rmax;
             const bool heavy = live && cc[c].w > 0.f;
-            if (MOM) {
+            if (QUAD) {
+                const float w = cc[c].w;
+                qxx = heavy ? qxx + (cq[c].x + w * (3.f * dx * dx - d2)) : qxx;
+                qyy = heavy ? qyy + (cq[c].y + w * (3.f * dy * dy - d2)) : qyy;
+                qxy = heavy ? qxy + (cq[c].z + w * 3.f * dx * dy) : qxy;
+                qxz = heavy ? qxz + (cq[c].w + w * 3.f * dx * dz) : qxz;
+                qyz = heavy ? qyz + (ca[c].x + w * 3.f * dy * dz) : qyz;
+            }
+        }
+        // the learned moments: the children's 16 words are gathered after the centre of
+        // mass, four children at a time (all eight at once held ~300 registers, one wave
+        // per SIMD); each moment still sums the children in order
+        float mo[16] = {};
+#pragma unroll
+        for (int c0 = 0; c0 < (MOM ? 8 : 0); c0 += 4) {
+            float4 cm[16];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) cm[4 * c + j] = nd_mom[4 * src[c0 + c] + j];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) pin(cm[4 * c]), pin(cm[4 * c + 1]), pin(cm[4 * c + 2]), pin(cm[4 * c + 3]);
+#pragma unroll
+            for (int cc4 = 0; cc4 < 4; ++cc4) {
+                const int c = c0 + cc4;
+                const bool heavy = massive && ch[c] >= 0 && cc[c].w > 0.f;
+                const float dx = cc[c].x - fx, dy = cc[c].y - fy, dz = cc[c].z - fz;
                 // parallel-axis shift of the child's moments to this com
-                const float4 q0 = cm[4 * c + 0], q1 = cm[4 * c + 1], q2 = cm[4 * c + 2], q3 = cm[4 * c + 3];
+                const float4 q0 = cm[4 * cc4 + 0], q1 = cm[4 * cc4 + 1], q2 = cm[4 * cc4 + 2], q3 = cm[4 * cc4 + 3];
                 const float m = cc[c].w;
                 const float sxx = q0.x, syy = q0.y, szz = q0.z, sxy = q0.w, sxz = q1.x, syz = q1.y;
                 float t[16];
@@ -1208,14 +1226,6 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
                 t[15] = q3.w + sxy * dz + sxz * dy + syz * dx + m * dx * dy * dz;
 #pragma unroll
                 for (int i = 0; i < 16; ++i) mo[i] = heavy ? mo[i] + t[i] : mo[i];
-            }
-            if (QUAD) {
-                const float w = cc[c].w;
-                qxx = heavy ? qxx + (cq[c].x + w * (3.f * dx * dx - d2)) : qxx;
-                qyy = heavy ? qyy + (cq[c].y + w * (3.f * dy * dy - d2)) : qyy;
-                qxy = heavy ? qxy + (cq[c].z + w * 3.f * dx * dy) : qxy;
-                qxz = heavy ? qxz + (cq[c].w + w * 3.f * dx * dz) : qxz;
-                qyz = heavy ? qyz + (ca[c].x + w * 3.f * dy * dz) : qyz;
             }
         }
         if (MOM) {
