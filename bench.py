@@ -59,6 +59,8 @@ def kernel_traffic(n, kernel):
 
 
 PEAK_HBM_GBS = 8000.0
+
+
 def sort_passes(levels):
     """LSD radix passes of 8-bit digits over a key of 3 bits per tree level (16 levels: 6)."""
     return (3 * levels + 7) // 8
@@ -142,12 +144,14 @@ def _short_kernel_name(name):
     return name[:cut].strip()
 
 
-def traffic_from_counters(fetch_csv, write_csv, out_path, n, workload):
+def traffic_from_counters(fetch_csv, write_csv, out_path, n, workload, plain_csvs=None):
     """HBM bytes per launch of every kernel from two rocprofv3 --pmc passes over the same
     command (FETCH_SIZE, then WRITE_SIZE; counter_collection.csv of --output-format csv),
     each counter in KiB per dispatch.  The counters are calibrated on k_copy_f32, a plain
     copy of n floats (4 n bytes read, 4 n written): scale = algorithmic bytes / counter
-    bytes of that copy, applied to every kernel."""
+    bytes of that copy, applied to every kernel.  plain_csvs: the same two passes over the
+    plain theta=0.5 command, whose plain walk kernels replace those of the first pair (there
+    the same kernel also runs the bisection probes and theta=0.8)."""
     import csv
     import collections
 
@@ -171,6 +175,16 @@ def traffic_from_counters(fetch_csv, write_csv, out_path, n, workload):
     copy = "k_copy_f32"
     fs = 4.0 * n / (fetch[copy] * 1024.0)
     ws = 4.0 * n / (write[copy] * 1024.0)
+    if plain_csvs:
+        pf = per_kernel(plain_csvs[0], "FETCH_SIZE")
+        pw = per_kernel(plain_csvs[1], "WRITE_SIZE")
+        pfs = 4.0 * n / (pf[copy] * 1024.0)
+        pws = 4.0 * n / (pw[copy] * 1024.0)
+        for k in pf:
+            if k.startswith("void k_walk_bins<") and ", false, 16, false, false>" in k:
+                # rescaled to the first pair's calibration
+                fetch[k] = pf[k] * pfs / fs
+                write[k] = pw.get(k, 0.0) * pws / ws
     names = sorted(set(fetch) | set(write), key=lambda k: -(fetch.get(k, 0) * fs + write.get(k, 0) * ws))
     table = {k: int(1024.0 * (fetch.get(k, 0.0) * fs + write.get(k, 0.0) * ws)) for k in names}
     out = {"workload": workload, "n": n, "fetch_scale": round(fs, 4), "write_scale": round(ws, 4),
@@ -544,15 +558,24 @@ def main():
     ap.add_argument("--no-kahan", action="store_true")
     ap.add_argument("--error-sample", type=int, default=16384, help="targets of the direct-sum error check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+    ap.add_argument("--traffic-from", nargs="+", metavar="CSV",
                     help="instead of running: write profiles/traffic_plummer_1m.json from the counter_collection.csv "
-                         "files of two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) over `bench.py --no-cpu`")
+                         "files of two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) over `bench.py --no-cpu`, "
+                         "optionally followed by the same two passes over `bench.py --no-cpu --variant plain` "
+                         "(the plain theta=0.5 walk's own bytes)")
     args = ap.parse_args()
     if args.traffic_from:
+        if len(args.traffic_from) not in (2, 4):
+            print("--traffic-from takes FETCH WRITE [FETCH_PLAIN WRITE_PLAIN]", file=sys.stderr)
+            sys.exit(2)
         out = traffic_from_counters(args.traffic_from[0], args.traffic_from[1],
                                     os.path.join(ROOT, "profiles", "traffic_plummer_1m.json"), args.n,
                                     "plummer_1m: bench.py --no-cpu (learned theta=0.8 and plain theta=0.5 steps, "
-                                    "their error checks and the equal-error plain steps)")
+                                    "their error checks and the equal-error plain steps); the plain theta=0.5 walk "
+                                    "from bench.py --no-cpu --variant plain" if len(args.traffic_from) == 4 else
+                                    "plummer_1m: bench.py --no-cpu (learned theta=0.8 and plain theta=0.5 steps, "
+                                    "their error checks and the equal-error plain steps)",
+                                    plain_csvs=args.traffic_from[2:] or None)
         log(json.dumps({k: v for k, v in out.items() if not k.endswith("per_launch")}))
         return
     args.steps_given = args.steps is not None
@@ -680,7 +703,7 @@ def main():
         mlp_ach = pairs * FLOP_MLP_PAIR / (mlp_ms * 1e-3) / 1e12
         roof_m = {"bound": "mfma", "pipe": "bf16-mfma", "achieved": round(mlp_ach, 3), "peak": PEAK_BF16_TFLOPS,
                   "unit": "TFLOP/s", "frac": round(mlp_ach / PEAK_BF16_TFLOPS, 5),
-                  "traffic": kernel_traffic(n, "k_learn_infer"),
+                  "traffic": kernel_traffic(n, f"void k_learn_infer<false, {pynbg.LEARN_ACT[act_name]}>"),
                   "kernel": "k_learn_infer (+ k_learn_zero, k_learn_apply)", "kernel_ms": round(mlp_ms, 4),
                   "flop_per_launch": pairs * FLOP_MLP_PAIR, "pairs_per_launch": pairs}
         stage_l = {k: round(getattr(st_l, "ms_" + k) / args.steps, 4)
