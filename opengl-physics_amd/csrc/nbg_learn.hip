@@ -401,6 +401,55 @@ __device__ __forceinline__ void mlp32_tile2(const Frags32& F, bf16x4 xa, bf16x4 
     yb = b;
 }
 
+// ---------------------------------------------------------------------------- fixed point
+// v * 2^e rounded to the nearest integer (ties to even) as a two's-complement 64-bit word:
+// the product is exact in FP32 (|v 2^e| < 2^49 is checked; an underflow rounds to 0 as the
+// exact value would), and r - 2^32 floor(r / 2^32) keeps at most the 24 significant bits of
+// r below bit 32, so it is exact too.  Equal to rounding the FP64 product.
+__device__ __forceinline__ uint64_t fix_term(float v, int e, bool& over) {
+    const float t = ldexpf(v, e);
+    over |= !(fabsf(t) < 0x1p49f);
+    const float r = __builtin_rintf(t);
+    const float hi = floorf(r * 0x1p-32f);
+    const float lo = fmaf(hi, -0x1p32f, r);
+    return (static_cast<uint64_t>(static_cast<uint32_t>(static_cast<int32_t>(hi))) << 32) |
+           static_cast<uint64_t>(static_cast<uint32_t>(lo));
+}
+
+// DPP move with control CTRL over the rows in ROWS: a lane whose source lies outside the
+// wave (or whose row is not in ROWS) keeps `fill`.  Row shifts right (0x111..0x11f) move
+// within rows of 16 lanes; row_bcast15 (0x142, rows 1 and 3) and row_bcast31 (0x143, rows
+// 2 and 3) bring the last lane of the row(s) before.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_mov(uint32_t v, uint32_t fill) {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(static_cast<int>(fill), static_cast<int>(v), CTRL, ROWS, 0xf, false));
+}
+// One step of an inclusive segmented sum over the wave (Hillis-Steele on DPP moves): a lane
+// adds the partial sums of its DPP source lane when both lie in the same run.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void seg_sum_step(int seg, uint64_t& a, uint64_t& b, uint64_t& c) {
+    const bool same = static_cast<int>(dpp_mov<CTRL, ROWS>(static_cast<uint32_t>(seg), 0xffffffffu)) == seg;
+    auto mv = [](uint64_t v) {
+        return (static_cast<uint64_t>(dpp_mov<CTRL, ROWS>(static_cast<uint32_t>(v >> 32), 0u)) << 32) |
+               dpp_mov<CTRL, ROWS>(static_cast<uint32_t>(v), 0u);
+    };
+    const uint64_t sa = mv(a), sb = mv(b), sc = mv(c);
+    a += same ? sa : 0ull;
+    b += same ? sb : 0ull;
+    c += same ? sc : 0ull;
+}
+// Inclusive sums of (a, b, c) over each run of equal `seg` (runs are contiguous lanes):
+// shifts by 1, 2, 4, 8 within rows, then the row broadcasts carry runs across rows.
+__device__ __forceinline__ void wave_seg_sum(int seg, uint64_t& a, uint64_t& b, uint64_t& c) {
+    seg_sum_step<0x111, 0xf>(seg, a, b, c);
+    seg_sum_step<0x112, 0xf>(seg, a, b, c);
+    seg_sum_step<0x114, 0xf>(seg, a, b, c);
+    seg_sum_step<0x118, 0xf>(seg, a, b, c);
+    seg_sum_step<0x142, 0xa>(seg, a, b, c);
+    seg_sum_step<0x143, 0xc>(seg, a, b, c);
+}
+
 // ---------------------------------------------------------------------------- inference
 // One persistent kernel per evaluation: a lane computes the standardised BF16 feature row
 // of one listed pair (16 x 2 B, entries 13..15 zero) and its output scale from the node
@@ -529,13 +578,13 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_learn_infer(InferArgs A) {
             }
         }
         const float v0 = scale * o0, v1 = scale * o1, v2 = scale * o2;
-        const int prev = __shfl_up(leaf, 1, 64);
-        const bool head = lane == 0 || prev != leaf;
-        const uint64_t heads = ballot(head);
-        const int seg = __popcll(heads & ((2ull << lane) - 1ull));
-        const int next_seg = __shfl_down(seg, 1, 64);
-        const bool tail = lane == 63 || next_seg != seg;
         if (TWO_PASS) {
+            const int prev = __shfl_up(leaf, 1, 64);
+            const bool head = lane == 0 || prev != leaf;
+            const uint64_t heads = ballot(head);
+            const int seg = __popcll(heads & ((2ull << lane) - 1ull));
+            const int next_seg = __shfl_down(seg, 1, 64);
+            const bool tail = lane == 63 || next_seg != seg;
             // each pair's correction to its slot; the largest |component| over each run of
             // equal leaf ids (the walk lists a leaf's pairs together) to the leaf, by one
             // atomic max per run: the maximum does not depend on how the runs are cut, so
@@ -553,27 +602,23 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_learn_infer(InferArgs A) {
         } else {
             // exact sums in 64-bit fixed point at the evaluation's unit (integer addition is
             // associative: the sums do not depend on the order in which the walk listed the
-            // pairs), a segmented sum over each run of equal leaf ids, one atomic triple per
-            // run; a term beyond the range flags the evaluation for the two-pass sums
-            const double u = ldexp(1.0, A.sc->learn_fix_e);
-            double t0 = static_cast<double>(v0) * u, t1 = static_cast<double>(v1) * u, t2 = static_cast<double>(v2) * u;
-            const double lim = 0x1p49;
-            if (!(fabs(t0) < lim && fabs(t1) < lim && fabs(t2) < lim)) {
+            // pairs), a segmented sum over each run of equal leaf ids (DPP moves: no LDS round
+            // trips), one atomic triple per run; a term beyond the range flags the evaluation
+            // for the two-pass sums
+            const int e = A.sc->learn_fix_e;
+            bool over = false;
+            uint64_t i0 = fix_term(v0, e, over), i1 = fix_term(v1, e, over), i2 = fix_term(v2, e, over);
+            if (over) {
                 atomicOr(&A.scw->learn_fix_over, 1);
                 atomicOr(&A.scw->sticky_learn, 1);
-                t0 = t1 = t2 = 0.0;
+                i0 = i1 = i2 = 0;
             }
-            long long i0 = __double2ll_rn(t0), i1 = __double2ll_rn(t1), i2 = __double2ll_rn(t2);
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const long long a0 = __shfl_up(i0, d, 64), a1 = __shfl_up(i1, d, 64), a2 = __shfl_up(i2, d, 64);
-                const int ts = __shfl_up(seg, d, 64);
-                if (lane >= d && ts == seg) {
-                    i0 += a0;
-                    i1 += a1;
-                    i2 += a2;
-                }
-            }
+            // runs of equal leaf ids (lane 0 compares with the fill, which no leaf id equals)
+            const int prev = __shfl_up(leaf, 1, 64);
+            const uint64_t heads = ballot(lane == 0 || prev != leaf);
+            const int seg = __popcll(heads & ((2ull << lane) - 1ull));
+            const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+            wave_seg_sum(seg, i0, i1, i2);
             if (tail && leaf >= 0) {
                 unsigned long long* dst = reinterpret_cast<unsigned long long*>(A.acc + 4 * static_cast<int64_t>(leaf));
                 atomicAdd(dst + 0, static_cast<unsigned long long>(i0));
