@@ -811,13 +811,8 @@ __global__ __launch_bounds__(256) void k_node_finish(const uint64_t* __restrict_
         nd_geom[4 * id + 2] = cz;
         nd_geom[4 * id + 3] = h;
         const double size = 2.0 * h;
-        int cnt;
-        if (nk > 0) {
-            cnt = cell_count<LEVELS>(kh, kl, i, sys_off[s + 1], d);
-            nd_count[id] = cnt;
-        } else {
-            cnt = nd_count[id];
-        }
+        // an internal node's particle count is its children's sum, added by the upsweep
+        const int cnt = (nk == 0) ? nd_count[id] : 0;
         const uint32_t info = (nk == 0) ? (kLeafBit | static_cast<uint32_t>(cnt)) : static_cast<uint32_t>(nk);
         nd_aux[id] = make_float4(0.0f, static_cast<float>(size * size), 0.0f, __uint_as_float(info));
         if (nk > 0) {
@@ -1114,6 +1109,7 @@ __device__ __forceinline__ void pin(float4& v) {
 template <bool QUAD, bool MOM, bool F64>
 __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* __restrict__ depth_nodes,
                                                      const int32_t* __restrict__ nd_child,
+                                                     int32_t* __restrict__ nd_count,
                                                      double G, const DevScalars* __restrict__ sc,
                                                      float4* __restrict__ nd_com, float4* __restrict__ nd_quad,
                                                      float4* __restrict__ nd_aux, double4* __restrict__ nd_com64,
@@ -1131,11 +1127,13 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
         for (int c = 0; c < 8; ++c) src[c] = ch[c] >= 0 ? ch[c] : id;
         double4 cd[8];
         float4 cc[8], ca[8], cq[8];
+        int cn[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             cd[c] = nd_com64[src[c]];
             ca[c] = nd_aux[src[c]];
             if (QUAD) cq[c] = nd_quad[src[c]];
+            cn[c] = nd_count[src[c]];
         }
         // Values used only on one side of a select would otherwise be loaded inside a
         // branch (one memory round trip per child); pinning them here keeps one wait.
@@ -1234,6 +1232,10 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
             nd_mom[4 * id + 2] = make_float4(mo[8], mo[9], mo[10], mo[11]);
             nd_mom[4 * id + 3] = make_float4(mo[12], mo[13], mo[14], mo[15]);
         }
+        int count = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) count += ch[c] >= 0 ? cn[c] : 0;
+        nd_count[id] = count;
         nd_com64[id] = make_double4(gx, gy, gz, M);
         nd_com[id] = make_float4(fx, fy, fz, static_cast<float>(G * M));
         nd_quad[id] = make_float4(qxx, qyy, qxy, qxz);
@@ -1269,7 +1271,8 @@ void launch_upsweep(nbg_ctx* c, int levels, int multipole, double G, bool moment
     for (int d = top; d >= 0; --d) {
         const int gd = d < 6 ? 64 : 1024;
 #define NBG_UPINT(Q, M, F)                                                                                           \
-    hipLaunchKernelGGL((k_up_internal<Q, M, F>), dim3(gd), dim3(256), 0, st, d, c->depth_nodes, c->nd_child, G,       \
+    hipLaunchKernelGGL((k_up_internal<Q, M, F>), dim3(gd), dim3(256), 0, st, d, c->depth_nodes, c->nd_child,         \
+                       c->nd_count, G,                                                                                \
                        c->dsc, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64, c->nd_mom)
         if (mom)
             NBG_UPINT(true, true, false);
