@@ -128,9 +128,14 @@ void drop_graph(nbg_ctx* c);
 
 // Enqueue one Barnes-Hut force evaluation (no host synchronisation).
 // The key levels to retry with after a tree at `levels` overflowed (a cell at the deepest
-// level held more than ncrit particles).  A depth-16 overflow raises the context's first
-// try to 21 levels until the next upload; the two-word keys are taken per evaluation.
+// level held more than ncrit particles).  An overflow at 13 or 16 levels raises the
+// context's first try to the next tier (16, 21) until the next upload; the two-word keys
+// are taken per evaluation.
 int next_levels(nbg_ctx* c, int levels) {
+    if (levels == kLevelsTiny) {
+        c->key_levels = kLevelsFast;
+        return kLevelsFast;
+    }
     if (levels == kLevelsFast) {
         c->key_levels = kLevelsShort;
         return kLevelsShort;
@@ -223,7 +228,7 @@ int accel_eager(nbg_ctx* c, const nbg_params& p, nbg_stats* stats, bool list_onl
         }
         if (c->hsc->overflow && levels < kLevelsDeep) {
             // a cell at the keys' deepest level holds more than ncrit particles: more levels
-            // (16 -> 21, kept for this state; 21 -> the two-word keys, for this evaluation)
+            // (13 -> 16 -> 21, kept for this state; 21 -> the two-word keys, for this evaluation)
             levels = next_levels(c, levels);
             continue;
         }
@@ -628,7 +633,7 @@ int nbg_upload_batch(nbg_ctx* c, int32_t nsys, const int64_t* offsets, const flo
     NBG_HIP_CHECK(c, hipSetDevice(c->device));
     drop_graph(c);
     c->f64_state = false;
-    c->key_levels = kLevelsFast;  // a new state tries the 16-level keys again
+    c->key_levels = kLevelsTiny;  // a new state tries the 13-level keys again
     c->n = static_cast<int>(n);
     c->nsys = nsys;
     c->h_sys_off.assign(offsets, offsets + nsys + 1);

@@ -15,7 +15,8 @@ namespace nbg {
 // ---------------------------------------------------------------------------- constants
 constexpr int kNcrit = 8;           // AdaptiveTree ncrit (projects/nbody_core/include/ngrav/Tree.hpp:38)
 constexpr int kMaxDepth = 40;       // AdaptiveTree maxDepth (same line)
-constexpr int kLevelsFast = 16;     // levels of the first key tried: 48 bits, 6 sort passes
+constexpr int kLevelsTiny = 13;     // levels of the first key tried: 39 bits, 5 sort passes
+constexpr int kLevelsFast = 16;     // the next tier: 48 bits, 6 sort passes
 constexpr int kLevelsShort = 21;    // levels held by one 63-bit key
 constexpr int kLevelsDeep = 40;     // levels held by the two-word key
 constexpr int kSortThreads = 256;   // radix-sort workgroup
@@ -102,8 +103,8 @@ struct nbg_ctx {
     int nsys = 0;         // systems in the batch
     bool have_state = false;
     bool have_tree = false;
-    int last_levels = nbg::kLevelsFast;  // key levels of the last tree
-    int key_levels = nbg::kLevelsFast;   // key levels a tree build tries first (raised after a depth-16 overflow)
+    int last_levels = nbg::kLevelsTiny;  // key levels of the last tree
+    int key_levels = nbg::kLevelsTiny;   // key levels a tree build tries first (raised after an overflow)
     int graph_levels = 0;           // key levels the captured step graph was built with
     double last_G = 1.0;  // G folded into the device masses of the last evaluation
 

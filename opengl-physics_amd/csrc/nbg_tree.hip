@@ -360,7 +360,10 @@ void launch_bbox_and_keys(nbg_ctx* c, int levels) {
                            c->blk_start, c->blk_count, bb);
         hipLaunchKernelGGL(k_root<double>, dim3(grid_for(c->nsys, 64)), dim3(64), 0, st, bb, c->sys_off, c->nsys,
                            c->sys_geom);
-        if (levels == kLevelsFast)
+        if (levels == kLevelsTiny)
+            hipLaunchKernelGGL((k_keys<kLevelsTiny, double>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x64,
+                               c->y64, c->z64, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
+        else if (levels == kLevelsFast)
             hipLaunchKernelGGL((k_keys<kLevelsFast, double>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x64,
                                c->y64, c->z64, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
         else if (levels == kLevelsShort)
@@ -375,7 +378,10 @@ void launch_bbox_and_keys(nbg_ctx* c, int levels) {
                        c->blk_start, c->blk_count, c->sys_bbox);
     hipLaunchKernelGGL(k_root<float>, dim3(grid_for(c->nsys, 64)), dim3(64), 0, st, c->sys_bbox, c->sys_off, c->nsys,
                        c->sys_geom);
-    if (levels == kLevelsFast)
+    if (levels == kLevelsTiny)
+        hipLaunchKernelGGL((k_keys<kLevelsTiny, float>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x, c->y,
+                           c->z, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
+    else if (levels == kLevelsFast)
         hipLaunchKernelGGL((k_keys<kLevelsFast, float>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x, c->y,
                            c->z, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
     else if (levels == kLevelsShort)
@@ -534,7 +540,8 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     }
 }
 
-// After an even number of passes the sorted data is back in key_a/val_a(/klo_a).
+// After an even number of passes the sorted data is back in key_a/val_a(/klo_a), after an
+// odd number in key_b/val_b.
 void launch_sort(nbg_ctx* c, int levels) {
     hipStream_t st = c->stream;
     const int nb = c->n_blocks;
@@ -564,13 +571,20 @@ void launch_sort(nbg_ctx* c, int levels) {
                  even ? c->val_b : c->val_a, even ? c->klo_a : c->klo_b, even ? c->klo_b : c->klo_a, 8 * p);
         }
     } else {
-        // the key's bits 3 (21 - levels) .. 62: 8 passes for 21 levels, 6 for 16 (bits 15..62)
+        // the key's bits 3 (21 - levels) .. 62: 8 passes for 21 levels, 6 for 16 (bits 15..62),
+        // 5 for 13 (bits 24..62; the sorted data then ends in the b buffers)
         const int low = 3 * (kLevelsShort - levels);
         const int passes = (63 - low + 7) / 8;
         for (int p = 0; p < passes; ++p) {
             const bool even = (p & 1) == 0;
             pass(even ? c->key_a : c->key_b, even ? c->key_b : c->key_a, even ? c->val_a : c->val_b,
                  even ? c->val_b : c->val_a, nullptr, nullptr, low + 8 * p);
+        }
+        if (passes & 1) {
+            c->key_sorted = c->key_b;
+            c->klo_sorted = c->klo_b;
+            c->perm = c->val_b;
+            return;
         }
     }
     c->key_sorted = c->key_a;
@@ -946,7 +960,9 @@ void launch_build_t(nbg_ctx* c) {
 }
 
 void launch_build(nbg_ctx* c, int levels) {
-    if (levels == kLevelsFast)
+    if (levels == kLevelsTiny)
+        launch_build_t<kLevelsTiny>(c);
+    else if (levels == kLevelsFast)
         launch_build_t<kLevelsFast>(c);
     else if (levels == kLevelsShort)
         launch_build_t<kLevelsShort>(c);

@@ -49,19 +49,44 @@ def test_plummer_tree_bit_exact(gpu_ctx_factory, n, seed):
     T = oracle.Tree(x, y, z, m, quad=True)
     o = T.export()
     keys_hi, _ = oracle.morton_keys(x, y, z, 21)
-    assert st.key_levels == 16  # no cell at depth 16 holds more than 8 particles here
+    assert st.key_levels == 13  # no cell at depth 13 holds more than 8 particles here
     np.testing.assert_array_equal(g["keys"], np.sort(keys_at(keys_hi, st.key_levels)))
     _compare_trees(g, o, n)
     assert st.deep_keys == 0
     assert st.max_depth == o["depth"].max()
 
 
+def test_depth13_overflow_takes_16_level_keys(gpu_ctx_factory):
+    """A clump of depth 15 (Plummer x 3e-4 plus one particle at 1): the 13-level keys
+    overflow and the build is redone with 16-level keys (6 sort passes instead of 5, the
+    sorted data back in the other ping-pong buffers) -- keys and topology bit-exact against
+    the oracle -- and the context keeps 16 levels for its next evaluation."""
+    import oracle
+    import pynbg
+    s = plummer_f32(4096, 11)
+    for k in ("x", "y", "z"):
+        s[k] = (s[k] * np.float32(3e-4)).astype(np.float32)
+    s["x"][0], s["y"][0], s["z"][0] = np.float32(1.0), np.float32(0.0), np.float32(0.0)
+    x, y, z, m = f64(s)
+    ctx, g, st = _run(gpu_ctx_factory, pynbg, s["x"], s["y"], s["z"], s["m"])
+    o = oracle.Tree(x, y, z, m, quad=True).export()
+    print(f"depth {st.max_depth} key levels {st.key_levels}", flush=True)
+    assert 13 < o["depth"].max() <= 16
+    assert st.key_levels == 16 and st.deep_keys == 0
+    keys_hi, _ = oracle.morton_keys(x, y, z, 21)
+    np.testing.assert_array_equal(g["keys"], np.sort(keys_at(keys_hi, 16)))
+    _compare_trees(g, o, len(x))
+    st2 = ctx.accel(_params(pynbg))
+    assert st2.key_levels == 16
+    np.testing.assert_array_equal(ctx.tree_export()["order"], g["order"])
+
+
 def test_depth16_overflow_takes_21_level_keys(gpu_ctx_factory):
     """A dense clump far below a wide root box (Plummer x 2e-5 plus one particle at 1): its
-    tree is deeper than 16 levels, so the 16-level keys overflow and the build is redone with
-    21-level keys -- topology bit-exact against the oracle -- and the context keeps 21 levels
-    (graph-replayed steps then equal eager ones bitwise) until a new upload, which tries 16
-    again."""
+    tree is deeper than 16 levels, so the 13- and then the 16-level keys overflow and the
+    build is redone with 21-level keys -- topology bit-exact against the oracle -- and the
+    context keeps 21 levels (graph-replayed steps then equal eager ones bitwise) until a new
+    upload, which tries 13 again."""
     import oracle
     import pynbg
     s = plummer_f32(4096, 11)
@@ -88,10 +113,10 @@ def test_depth16_overflow_takes_21_level_keys(gpu_ctx_factory):
         out.append(ctx.download())
     for k in out[0]:
         np.testing.assert_array_equal(out[0][k], out[1][k])
-    # a shallow state in the same context starts from 16 levels again
+    # a shallow state in the same context starts from 13 levels again
     t = plummer_f32(4096, 12)
     ctx.upload(t["x"], t["y"], t["z"], t["m"])
-    assert ctx.accel(_params(pynbg)).key_levels == 16
+    assert ctx.accel(_params(pynbg)).key_levels == 13
 
 
 def test_tiny_and_degenerate_inputs(gpu_ctx_factory):
