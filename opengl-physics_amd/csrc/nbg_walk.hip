@@ -705,6 +705,38 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             kadd<KAHAN>(ay, cy, sy);
             kadd<KAHAN>(az, cz, sz);
         };
+        // learned pair listing: up to 128 pending (cell, leaf mask) entries, two per lane, are
+        // written leaf-major (one ballot per leaf and row), so that the consumer finds each
+        // leaf's pairs of up to 128 entries in one run and sums a run before one atomic update
+        int pnode0 = 0, pnode1 = 0;
+        uint64_t pm0 = 0, pm1 = 0;
+        int npend = 0;
+        auto flush_pairs = [&]() __attribute__((always_inline)) {
+            const int tot = wave_add(__popcll(pm0) + __popcll(pm1));
+            const uint64_t used = wave_or64(pm0 | pm1);
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&A.sc->learn_pairs, static_cast<uint32_t>(tot));
+            uint32_t w = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(base)));
+            for (uint64_t t = used; t; t &= t - 1) {
+                const int jj = __ffsll(static_cast<long long>(t)) - 1;
+                const int lf = rdlane(my_lid, jj);
+                const bool a0 = (pm0 >> jj) & 1ull, a1 = (pm1 >> jj) & 1ull;
+                const uint64_t b0 = ballot(a0), b1 = ballot(a1);
+                if (a0) {
+                    const uint32_t idx = w + static_cast<uint32_t>(mbcnt(b0));
+                    if (idx < A.lcap) A.lpairs[idx] = make_int2(lf, pnode0);
+                }
+                w += static_cast<uint32_t>(__popcll(b0));
+                if (a1) {
+                    const uint32_t idx = w + static_cast<uint32_t>(mbcnt(b1));
+                    if (idx < A.lcap) A.lpairs[idx] = make_int2(lf, pnode1);
+                }
+                w += static_cast<uint32_t>(__popcll(b1));
+            }
+            pm0 = 0;
+            pm1 = 0;
+            npend = 0;
+        };
         const bool near_fast = SPL || soft.eps2 > 0.f;
         auto eval_near = [&](int h, int n) __attribute__((always_inline)) {
             if (REL)
@@ -894,25 +926,36 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 }
                 f_tail += __popcll(bm);
             }
-            // learned correction: list (leaf, cell) for the band pairs of leaves owned here,
-            // leaf-major within the step (one ballot per leaf), so that the consumer finds each
-            // leaf's pairs in runs and sums a run before one atomic update
-            if (LEARN && ballot((band_m & own_m) != 0)) {
+            // learned correction: the (cell, mask of owned leaves with a band pair) entries of
+            // the step join the wave's pending entries (compacted through LDS: lcom is free
+            // between the group tests and the next step), which are listed leaf-major when 128
+            // are pending and at the end of the bin
+            if (LEARN) {
                 const uint64_t em = band_m & own_m;
-                const int tot = wave_add(__popcll(em));
-                const uint64_t used = wave_or64(em);  // the leaves with pairs in this step
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(&A.sc->learn_pairs, static_cast<uint32_t>(tot));
-                uint32_t w = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(base)));
-                for (uint64_t t = used; t; t &= t - 1) {
-                    const int jj = __ffsll(static_cast<long long>(t)) - 1;
-                    const bool mine = (em >> jj) & 1ull;
-                    const uint64_t m = ballot(mine);
-                    if (mine) {
-                        const uint32_t idx = w + static_cast<uint32_t>(mbcnt(m));
-                        if (idx < A.lcap) A.lpairs[idx] = make_int2(rdlane(my_lid, jj), node);
+                const bool has = em != 0;
+                const uint64_t bm = ballot(has);
+                const int k = __popcll(bm);
+                if (k) {
+                    if (npend + k > 2 * 64) flush_pairs();
+                    int4* scr = reinterpret_cast<int4*>(lcom);
+                    if (has)
+                        scr[mbcnt(bm)] = make_int4(node, static_cast<int>(static_cast<uint32_t>(em)),
+                                                   static_cast<int>(static_cast<uint32_t>(em >> 32)), 0);
+                    wave_sync();
+                    // entry e of the pending list sits in lane e % 64 of row e / 64
+                    const int e0 = lane - npend, e1 = lane + 64 - npend;
+                    if (e0 >= 0 && e0 < k) {
+                        const int4 v = scr[e0];
+                        pnode0 = v.x;
+                        pm0 = (static_cast<uint64_t>(static_cast<uint32_t>(v.z)) << 32) | static_cast<uint32_t>(v.y);
                     }
-                    w += static_cast<uint32_t>(__popcll(m));
+                    if (e1 >= 0 && e1 < k) {
+                        const int4 v = scr[e1];
+                        pnode1 = v.x;
+                        pm1 = (static_cast<uint64_t>(static_cast<uint32_t>(v.z)) << 32) | static_cast<uint32_t>(v.y);
+                    }
+                    wave_sync();
+                    npend += k;
                 }
             }
             // near leaves -> ring
@@ -966,6 +1009,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
         }
         eval_far(f_head, f_tail - f_head);
         eval_near(q_head, q_tail - q_head);
+        if (LEARN && npend > 0) flush_pairs();
         if (tv) {
             const float rx = KAHAN ? ax - cx : ax, ry = KAHAN ? ay - cy : ay, rz = KAHAN ? az - cz : az;
             if (LEARN) {
