@@ -437,6 +437,9 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     int victim = 0;
     // stack headroom: a depth-first descent pushes at most 7 siblings per level
     const int reserve = 8 * (A.sc->max_depth + 1);
+    // learned selection with a band rule (learn_band > 0; without one every accepted pair is
+    // in the band and only the correction scale selects)
+    const bool band_on = LEARN && A.inv_band < 1e29f;
 
     for (;;) {
         int b = -1;
@@ -795,12 +798,13 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                         // 1e-4 relative, which covers the hardware square root (1 ulp)
                         const float size = __builtin_amdgcn_sqrtf(au.y);
                         const float sth = size * inv_theta;  // size / theta
-                        const float sbd = LEARN ? sqrtf(au.y) * A.inv_band : 0.f;
+                        // (the learned band's distance and sphere tests keep the correctly rounded
+                        // root; without a band the hardware root serves, within the slacks)
+                        const float sbd = (LEARN && band_on) ? sqrtf(au.y) * A.inv_band : __builtin_inff();
                         const float cscale = fabsf(cm.x) + fabsf(cm.y) + fabsf(cm.z);
                         const float bx = cm.x - cbx, by = cm.y - cby, bz = cm.z - cbz;
                         const float db2 = bx * bx + by * by + bz * bz;
-                        // (the learned band's sphere tests keep the correctly rounded root)
-                        const float db = LEARN ? sqrtf(db2) : __builtin_amdgcn_sqrtf(db2);
+                        const float db = (LEARN && band_on) ? sqrtf(db2) : __builtin_amdgcn_sqrtf(db2);
                         const float slack_b = 1e-4f * (db + Rb + sth) + 4e-6f * (cscale + cbscale);
                         uint64_t todo_all = act;
                         if (db - Rb - sth > slack_b) {
@@ -810,7 +814,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                                 // band membership (a heuristic selection, decided the same way
                                 // when samples are collected): by the sphere when it settles
                                 // every group, else by each group's distance
-                                if (sbd - db - Rb > slack_b) {
+                                if (!band_on || sbd - db - Rb > slack_b) {
                                     band_m = act;
                                 } else if (!(db - Rb - sbd > slack_b)) {
                                     bneed = act;  // tested group by group below, leaf-major
@@ -834,7 +838,9 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                                 // the scale test of a sphere-accepted node: by the bin sphere's
                                 // bounds on r when they settle it, else group by group below (the
                                 // undecided nodes take it inside their tests)
-                                const float rlo = fmaxf(db - Rb, 0.f), rhi = db + Rb;
+                                // (1e-6 relative covers the root's rounding: the bounds only
+                                // widen, the groups' own tests decide what they leave)
+                                const float rlo = fmaxf(db * (1.f - 1e-6f) - Rb, 0.f), rhi = db * (1.f + 1e-6f) + Rb;
                                 if (rlo * rlo > n_r2max) {
                                     band_m = 0;
                                     bneed = 0;
@@ -894,7 +900,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                     // band and correction scale, both by distance
                     const float2 lr = lrec[L];
                     const float qb = my_lc.w + lr.x;
-                    band_m = wrlane(band_m, acc & ballot(dist2 <= fminf(qb * qb, lr.y)), L);
+                    band_m = wrlane(band_m, acc & ballot(dist2 <= (band_on ? fminf(qb * qb, lr.y) : lr.y)), L);
                 }
             }
             open_m |= todo & ~far_m;
@@ -910,7 +916,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                     const float dx = nr.x - my_lc.x, dy = nr.y - my_lc.y, dz = nr.z - my_lc.z;
                     const float r2 = dx * dx + dy * dy + dz * dz;
                     const float qb = my_lc.w + lr.x;
-                    uint64_t res = (bnL & ballot(r2 <= qb * qb)) | (bmL & ~bnL);
+                    uint64_t res = band_on ? (bnL & ballot(r2 <= qb * qb)) | (bmL & ~bnL) : bmL;
                     if (anL) res &= ballot(r2 <= lr.y);
                     band_m = wrlane(band_m, res, L);
                 }
