@@ -71,14 +71,15 @@ def stage_hbm(st, steps, n, moments=False):
     inside the timed steps, against the algorithmic bytes of each (per particle unless
     noted; C = nodes): keys 28 (read x, y, z, m; write the 8-byte key and the 4-byte
     index), radix sort 24 per pass (a 64-bit key and a 32-bit value read and written) x the
-    passes of the tree's key levels (6 for 16 levels, 8 for 21), build 8 N + 40 C (keys in, node records out) + 32 N (the particle records
-    gathered into sorted order), upsweep 16 N + 56 C (particles in; centre of mass, mass,
-    quadrupole, geometry and radius out), kick-drift-kick 108.  counter_bytes: the same
+    passes of the tree's key levels (5 for 13 levels, 6 for 16, 8 for 21), build 8 N + 40 C
+    (keys in, node records out), upsweep 32 N + 56 C (particle records in and gathered into
+    sorted order; centre of mass, mass, quadrupole, geometry and radius out), kick-drift-kick
+    108.  counter_bytes: the same
     stage's HBM traffic per step from the committed FETCH_SIZE / WRITE_SIZE passes."""
     C = float(st.n_nodes)
     levels = st.key_levels
-    algo = {"keys": 28.0 * n, "sort": 24.0 * sort_passes(levels) * n, "build": 8.0 * n + 40.0 * C + 32.0 * n,
-            "upsweep": 16.0 * n + 56.0 * C, "integrate": 108.0 * n}
+    algo = {"keys": 28.0 * n, "sort": 24.0 * sort_passes(levels) * n, "build": 8.0 * n + 40.0 * C,
+            "upsweep": 32.0 * n + 56.0 * C, "integrate": 108.0 * n}
     counted = stage_counter_bytes(n, moments, levels)
     out = {}
     for k, b in algo.items():
@@ -93,7 +94,7 @@ def stage_hbm(st, steps, n, moments=False):
 
 # the kernels of each tree / integration stage and their launches per step (short names
 # as profiles/traffic_plummer_1m.json keys them) for keys of L levels; the upsweep's
-# internal pass runs once per level
+# internal nodes sit at depths 0..L-1, two depths per launch and depth 0 alone when L is odd
 def stage_kernels(levels, moments):
     P = sort_passes(levels)
     m = "true" if moments else "false"
@@ -104,9 +105,10 @@ def stage_kernels(levels, moments):
                  ("void k_scan_down_inline<unsigned int>", P), ("void k_radix_scatter<false>", P)],
         "build": [(f"void k_leaf_depth<{levels}>", 1), ("void k_scan_reduce<unsigned long>", 1),
                   ("void k_scan_down_inline<unsigned long>", 1), ("k_node_totals", 1), (f"void k_nodes<{levels}>", 1),
-                  ("k_link", 1), (f"void k_node_finish<{levels}>", 1), ("k_depth_scan", 1), ("k_depth_scatter", 1),
-                  ("k_leaf_order", 1), ("k_pack4", 1), ("k_gather", 1)],
-        "upsweep": [(f"void k_up_leaf<true, {m}, false>", 1), (f"void k_up_internal<true, {m}, false>", levels)],
+                  ("k_link", 1), (f"void k_node_finish<{levels}>", 1), ("void k_scan_reduce<unsigned int>", 1),
+                  ("void k_scan_down_inline<unsigned int>", 1), ("k_depth_scatter", 1), ("k_pack4", 1)],
+        "upsweep": [(f"void k_leaves<true, {m}, false>", 1), (f"void k_up_pair<true, {m}, false>", levels // 2)]
+                   + ([(f"void k_up_internal<true, {m}, false>", 1)] if levels % 2 else []),
         "integrate": [("k_kick_drift", 1), ("k_kick", 1)],
     }
 

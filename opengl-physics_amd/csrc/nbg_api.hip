@@ -10,7 +10,7 @@
 #include "nbg_internal.hpp"
 
 namespace nbg {
-void launch_gather(nbg_ctx* c, float G);
+void launch_pack4(nbg_ctx* c);  // input-order records for the per-leaf gather
 }
 
 using namespace nbg;
@@ -151,7 +151,7 @@ void enqueue_bh(nbg_ctx* c, const nbg_params& p, int levels, bool timed, bool li
     launch_sort(c, levels);
     if (timed) rec(c, 2);
     launch_build(c, levels);
-    launch_gather(c, static_cast<float>(p.G));
+    launch_pack4(c);
     if (timed) rec(c, 3);
     launch_upsweep(c, levels, p.multipole, p.G, learn);
     if (c->f64_state) launch_rel(c);
@@ -413,7 +413,7 @@ int build_tree_eager(nbg_ctx* c, const nbg_params& p) {
         launch_bbox_and_keys(c, levels);
         launch_sort(c, levels);
         launch_build(c, levels);
-        launch_gather(c, static_cast<float>(p.G));
+        launch_pack4(c);
         launch_upsweep(c, levels, p.multipole, p.G, false);
         if (int r = check_launch(c)) return r;
         if (int r = read_scalars(c)) return r;
@@ -544,6 +544,9 @@ nbg_ctx* nbg_create(int device, int64_t max_n) {
     r |= alloc_into(c, c->nd_com64, C);
     r |= alloc_into(c, c->nd_rmax64, C);
     r |= alloc_into(c, c->depth_nodes, C);
+    r |= alloc_into(c, c->dep_hist, static_cast<size_t>(kMaxDepth) * kNodeBlocks);
+    r |= alloc_into(c, c->dep_off, static_cast<size_t>(kMaxDepth) * kNodeBlocks);
+    r |= alloc_into(c, c->dep_part, static_cast<size_t>(kMaxDepth) + 8);
     r |= alloc_into(c, c->cnt_far, C);
     r |= alloc_into(c, c->cnt_near, C);
     r |= alloc_into(c, c->aold, N);
@@ -591,14 +594,14 @@ void nbg_destroy(nbg_ctx* c) {
     free_dev(c->nd_start); free_dev(c->nd_count); free_dev(c->nd_depth); free_dev(c->nd_parent);
     free_dev(c->nd_oct); free_dev(c->nd_child); free_dev(c->nd_geom);
     free_dev(c->nd_com); free_dev(c->nd_quad); free_dev(c->nd_aux); free_dev(c->nd_com64); free_dev(c->nd_rmax64);
-    free_dev(c->depth_nodes); free_dev(c->cnt_far); free_dev(c->cnt_near);
+    free_dev(c->depth_nodes); free_dev(c->dep_hist); free_dev(c->dep_off); free_dev(c->dep_part); free_dev(c->cnt_far); free_dev(c->cnt_near);
     free_dev(c->dsc); free_dev(c->diag_part);
     free_dev(c->aold); free_dev(c->rung); free_dev(c->tlist); free_dev(c->list_off); free_dev(c->red_part);
-    free_dev(c->x64); free_dev(c->y64); free_dev(c->z64); free_dev(c->sys_bbox64); free_dev(c->srel);
+    free_dev(c->x64); free_dev(c->y64); free_dev(c->z64); free_dev(c->blk_bbox64); free_dev(c->srel);
     free_dev(c->nd_mom); free_dev(c->lacc); free_dev(c->lunit); free_dev(c->lpairs); free_dev(c->lpv); free_dev(c->lw32); free_dev(c->lw16);
     free_dev(c->adam_m); free_dev(c->adam_v); free_dev(c->pool_feat); free_dev(c->pool_tgt); free_dev(c->amag2);
     free_dev(c->grad_part); free_dev(c->loss_part); free_dev(c->loss_dev);
-    free_dev(c->sys_off); free_dev(c->sys_root); free_dev(c->sys_geom); free_dev(c->sys_bbox);
+    free_dev(c->sys_off); free_dev(c->sys_root); free_dev(c->sys_geom); free_dev(c->blk_bbox);
     free_dev(c->blk_sys); free_dev(c->blk_start); free_dev(c->blk_count); free_dev(c->sys_blk0); free_dev(c->bin_start); free_dev(c->bin_count); free_dev(c->bin_cost);
     if (c->hsc) (void)hipHostFree(c->hsc);
     for (auto& e : c->ev)
@@ -666,7 +669,7 @@ int nbg_upload_batch(nbg_ctx* c, int32_t nsys, const int64_t* offsets, const flo
     r |= alloc_into(c, c->sys_off, nsys + 1);
     r |= alloc_into(c, c->sys_root, nsys);
     r |= alloc_into(c, c->sys_geom, 4 * static_cast<size_t>(nsys));
-    r |= alloc_into(c, c->sys_bbox, 6 * static_cast<size_t>(nsys));
+    r |= alloc_into(c, c->blk_bbox, 6 * nb);
     r |= alloc_into(c, c->hist, static_cast<size_t>(kRadix) * nb);
     c->n_bins = static_cast<int>(wstart.size());
     r |= alloc_into(c, c->bin_start, std::max<size_t>(1, wstart.size()));
@@ -746,7 +749,7 @@ int nbg_upload_f64(nbg_ctx* c, int64_t n, const double* x, const double* y, cons
     if (!c->x64) r |= alloc_into(c, c->x64, cap);
     if (!c->y64) r |= alloc_into(c, c->y64, cap);
     if (!c->z64) r |= alloc_into(c, c->z64, cap);
-    r |= alloc_into(c, c->sys_bbox64, 6 * static_cast<size_t>(c->nsys));
+    r |= alloc_into(c, c->blk_bbox64, 6 * static_cast<size_t>(std::max(1, c->n_blocks)));
     if (!c->srel) {
         r |= alloc_into(c, c->srel, cap + kSposPad);
         // the zero pad after the records (the near field loads whole groups); entries past

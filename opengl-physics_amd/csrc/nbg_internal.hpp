@@ -19,6 +19,10 @@ constexpr int kLevelsTiny = 13;     // levels of the first key tried: 39 bits, 5
 constexpr int kLevelsFast = 16;     // the next tier: 48 bits, 6 sort passes
 constexpr int kLevelsShort = 21;    // levels held by one 63-bit key
 constexpr int kLevelsDeep = 40;     // levels held by the two-word key
+// Depths that hold internal nodes for keys of L levels: 0..L-1 (a depth-L cell of a short
+// key is a leaf, and so is every depth-40 cell of the two-word key).
+template <int L>
+constexpr int kInternalDepths = (L <= kLevelsShort) ? L : kMaxDepth;
 constexpr int kSortThreads = 256;   // radix-sort workgroup
 constexpr int kSortItems = 8;       // keys per thread
 constexpr int kSortTile = kSortThreads * kSortItems;  // keys per sort block (2048)
@@ -28,6 +32,9 @@ constexpr int kRadix = 1 << kRadixBits;
 // Leaf info word stored (as float bits) in node_aux.w: bit 31 set = leaf, low bits =
 // particle count (leaf) or child count (internal).
 constexpr uint32_t kLeafBit = 0x80000000u;
+// Blocks of the grid-stride node kernels (k_node_finish, k_depth_scatter): one scan tile of
+// per-block depth counts per depth.
+constexpr int kNodeBlocks = 2048;
 
 // Zero float4 entries after the sorted particles: the walk loads 8 particles at a time
 // from a leaf start without a bound check.
@@ -114,7 +121,7 @@ struct nbg_ctx {
     int32_t* pos_sys = nullptr;     // system of each particle / sorted position
     int32_t* sys_root = nullptr;    // root node of each system
     double* sys_geom = nullptr;     // 4 per system: root centre x,y,z, halfSize
-    uint32_t* sys_bbox = nullptr;   // 6 per system: order-preserving min x,y,z, max x,y,z
+    uint32_t* blk_bbox = nullptr;   // 6 per sort block: order-preserving min x,y,z, max x,y,z
     // sort blocks: each covers at most kSortTile keys of one system
     int n_blocks = 0;
     int32_t* blk_sys = nullptr;
@@ -137,7 +144,7 @@ struct nbg_ctx {
     // and the grouped walk differences positions relative to leaf centres (srel)
     bool f64_state = false;
     double *x64 = nullptr, *y64 = nullptr, *z64 = nullptr;
-    uint64_t* sys_bbox64 = nullptr;  // 6 per system, order-preserving FP64 bounds
+    uint64_t* blk_bbox64 = nullptr;  // 6 per sort block, order-preserving FP64 bounds
     float4* srel = nullptr;          // per sorted position: offset from its leaf's centre, G*m (+ zero pad)
 
     // keys and sort
@@ -172,6 +179,9 @@ struct nbg_ctx {
     int32_t* leaf_list = nullptr;    // leaf node ids, in pre-order
     int32_t* leaf_of = nullptr;      // sorted position -> its leaf node id
     int32_t* depth_nodes = nullptr;  // internal node ids grouped by depth
+    uint32_t* dep_hist = nullptr;    // internal nodes per (depth, node-kernel block), depth-major
+    uint32_t* dep_off = nullptr;     // its exclusive scan: first depth_nodes slot per (depth, block)
+    uint32_t* dep_part = nullptr;    // the scan's tile sums: internal nodes per depth
     int32_t* cnt_far = nullptr;      // per node (leaf groups), NBG_FLAG_COUNT
     int32_t* cnt_near = nullptr;
 

@@ -170,9 +170,8 @@ void excl_scan(hipStream_t st, const T* in, T* out, int64_t n, T* part, T* total
 
 // ============================================================================ bbox + keys
 
-// Resets the per-call device scalars and each system's bbox accumulators.
-__global__ void k_reset(DevScalars* __restrict__ sc, uint32_t* __restrict__ bbox, uint64_t* __restrict__ bbox64,
-                        int nsys) {
+// Resets the per-call device scalars.
+__global__ void k_reset(DevScalars* __restrict__ sc) {
     const int t = threadIdx.x;
     if (t < 64) {
         sc->depth_count[t] = 0;
@@ -196,20 +195,6 @@ __global__ void k_reset(DevScalars* __restrict__ sc, uint32_t* __restrict__ bbox
         sc->tcount = 0;
         sc->pp_cost_prev = sc->pp_cost_sum;
         sc->pp_cost_sum = 0;
-    }
-    for (int s = t; s < nsys; s += blockDim.x) {
-        bbox[6 * s + 0] = 0xffffffffu;
-        bbox[6 * s + 1] = 0xffffffffu;
-        bbox[6 * s + 2] = 0xffffffffu;
-        bbox[6 * s + 3] = 0u;
-        bbox[6 * s + 4] = 0u;
-        bbox[6 * s + 5] = 0u;
-        if (bbox64) {
-            for (int a = 0; a < 3; ++a) {
-                bbox64[6 * s + a] = ~0ull;
-                bbox64[6 * s + 3 + a] = 0ull;
-            }
-        }
     }
 }
 
@@ -241,26 +226,38 @@ struct PosOrd<double> {
 };
 
 // Per sort block (a slice of one system): exact min/max of the positions (FP32, or FP64
-// for an FP64 state), one atomic per block.
+// for an FP64 state), written as the block's partial (k_root reduces a system's blocks:
+// hundreds of same-address atomics from every block of a launch cost more than the loads).
 template <typename T>
-__global__ __launch_bounds__(256) void k_bbox(const T* __restrict__ x, const T* __restrict__ y,
-                                              const T* __restrict__ z, const int32_t* __restrict__ blk_sys,
-                                              const int32_t* __restrict__ blk_start,
+__global__ __launch_bounds__(kSortThreads) void k_bbox(const T* __restrict__ x, const T* __restrict__ y,
+                                              const T* __restrict__ z, const int32_t* __restrict__ blk_start,
                                               const int32_t* __restrict__ blk_count,
-                                              typename PosOrd<T>::U* __restrict__ bbox) {
+                                              typename PosOrd<T>::U* __restrict__ blk_bbox) {
     typedef typename PosOrd<T>::U U;
     __shared__ U s_red[6][4];
     const int b = blockIdx.x;
-    const int s = blk_sys[b];
     const int start = blk_start[b], cnt = blk_count[b];
     U lo[3] = {PosOrd<T>::lo0, PosOrd<T>::lo0, PosOrd<T>::lo0}, hi[3] = {PosOrd<T>::hi0, PosOrd<T>::hi0, PosOrd<T>::hi0};
-    for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
-        const int i = start + k;
-        const U v[3] = {PosOrd<T>::enc(x[i]), PosOrd<T>::enc(y[i]), PosOrd<T>::enc(z[i])};
+    if (cnt > 0) {
+        // a block holds at most kSortItems positions per thread: all loads are issued
+        // before the first is used (one memory round trip), those past the block's end
+        // repeat its last position, which leaves min and max unchanged
+        T px[kSortItems], py[kSortItems], pz[kSortItems];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = min(lo[a], v[a]);
-            hi[a] = max(hi[a], v[a]);
+        for (int u = 0; u < kSortItems; ++u) {
+            const int i = start + min(static_cast<int>(threadIdx.x) + u * kSortThreads, cnt - 1);
+            px[u] = x[i];
+            py[u] = y[i];
+            pz[u] = z[i];
+        }
+#pragma unroll
+        for (int u = 0; u < kSortItems; ++u) {
+            const U v[3] = {PosOrd<T>::enc(px[u]), PosOrd<T>::enc(py[u]), PosOrd<T>::enc(pz[u])};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = min(lo[a], v[a]);
+                hi[a] = max(hi[a], v[a]);
+            }
         }
     }
 #pragma unroll
@@ -287,30 +284,63 @@ __global__ __launch_bounds__(256) void k_bbox(const T* __restrict__ x, const T* 
             l = min(l, s_red[a][w]);
             h = max(h, s_red[3 + a][w]);
         }
-        if (cnt > 0) {
-            atomicMin(&bbox[6 * s + a], l);
-            atomicMax(&bbox[6 * s + 3 + a], h);
-        }
+        blk_bbox[6 * b + a] = l;  // an empty block leaves the neutral bounds
+        blk_bbox[6 * b + 3 + a] = h;
     }
 }
 
-// Root geometry per system, FP64, as Tree.cpp:54-88.
+// Root geometry per system, FP64, as Tree.cpp:54-88: one block per system reduces the
+// bounds of its sort blocks.
 template <typename T>
-__global__ void k_root(const typename PosOrd<T>::U* __restrict__ bbox, const int32_t* __restrict__ sys_off, int nsys,
-                       double* __restrict__ geom) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nsys) return;
-    double lo[3], hi[3];
+__global__ __launch_bounds__(256) void k_root(const typename PosOrd<T>::U* __restrict__ blk_bbox,
+                                              const int32_t* __restrict__ sys_blk0,
+                                              const int32_t* __restrict__ sys_off, double* __restrict__ geom) {
+    typedef typename PosOrd<T>::U U;
+    __shared__ U s_red[6][4];
+    const int s = blockIdx.x;
+    const int b0 = sys_blk0[s], b1 = sys_blk0[s + 1];
+    U lo[3] = {PosOrd<T>::lo0, PosOrd<T>::lo0, PosOrd<T>::lo0}, hi[3] = {PosOrd<T>::hi0, PosOrd<T>::hi0, PosOrd<T>::hi0};
+    for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = min(lo[a], blk_bbox[6 * b + a]);
+            hi[a] = max(hi[a], blk_bbox[6 * b + 3 + a]);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = min(lo[a], static_cast<U>(__shfl_xor(lo[a], o, 64)));
+            hi[a] = max(hi[a], static_cast<U>(__shfl_xor(hi[a], o, 64)));
+        }
+    }
+    const int wave = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            s_red[a][wave] = lo[a];
+            s_red[3 + a][wave] = hi[a];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double dlo[3], dhi[3];
     const bool empty = sys_off[s + 1] == sys_off[s];
     for (int a = 0; a < 3; ++a) {
-        lo[a] = empty ? 0.0 : PosOrd<T>::dec(bbox[6 * s + a]);
-        hi[a] = empty ? 0.0 : PosOrd<T>::dec(bbox[6 * s + 3 + a]);
+        U l = s_red[a][0], h = s_red[3 + a][0];
+        for (int w = 1; w < 4; ++w) {
+            l = min(l, s_red[a][w]);
+            h = max(h, s_red[3 + a][w]);
+        }
+        dlo[a] = empty ? 0.0 : PosOrd<T>::dec(l);
+        dhi[a] = empty ? 0.0 : PosOrd<T>::dec(h);
     }
-    double half = 0.5 * (hi[0] - lo[0]);
-    half = fmax(half, 0.5 * (hi[1] - lo[1]));
-    half = fmax(half, 0.5 * (hi[2] - lo[2]));
+    double half = 0.5 * (dhi[0] - dlo[0]);
+    half = fmax(half, 0.5 * (dhi[1] - dlo[1]));
+    half = fmax(half, 0.5 * (dhi[2] - dlo[2]));
     if (!(half > 1e-12)) half = 1.0;
-    for (int a = 0; a < 3; ++a) geom[4 * s + a] = 0.5 * (lo[a] + hi[a]);
+    for (int a = 0; a < 3; ++a) geom[4 * s + a] = 0.5 * (dlo[a] + dhi[a]);
     geom[4 * s + 3] = half * 1.001;
 }
 
@@ -350,15 +380,14 @@ __global__ __launch_bounds__(256) void k_keys(const T* __restrict__ x, const T* 
 void launch_bbox_and_keys(nbg_ctx* c, int levels) {
     hipStream_t st = c->stream;
     const bool f64 = c->f64_state;
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, st, c->dsc, c->sys_bbox, f64 ? c->sys_bbox64 : nullptr,
-                       c->nsys);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, st, c->dsc);
     // the root box and the keys from the positions of the state: FP32, or FP64 (Tree.cpp's
     // own precision) for an FP64 state
     if (f64) {
-        unsigned long long* bb = reinterpret_cast<unsigned long long*>(c->sys_bbox64);
-        hipLaunchKernelGGL(k_bbox<double>, dim3(c->n_blocks), dim3(256), 0, st, c->x64, c->y64, c->z64, c->blk_sys,
+        unsigned long long* bb = reinterpret_cast<unsigned long long*>(c->blk_bbox64);
+        hipLaunchKernelGGL(k_bbox<double>, dim3(c->n_blocks), dim3(kSortThreads), 0, st, c->x64, c->y64, c->z64,
                            c->blk_start, c->blk_count, bb);
-        hipLaunchKernelGGL(k_root<double>, dim3(grid_for(c->nsys, 64)), dim3(64), 0, st, bb, c->sys_off, c->nsys,
+        hipLaunchKernelGGL(k_root<double>, dim3(c->nsys), dim3(256), 0, st, bb, c->sys_blk0, c->sys_off,
                            c->sys_geom);
         if (levels == kLevelsTiny)
             hipLaunchKernelGGL((k_keys<kLevelsTiny, double>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x64,
@@ -374,9 +403,9 @@ void launch_bbox_and_keys(nbg_ctx* c, int levels) {
                                c->y64, c->z64, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
         return;
     }
-    hipLaunchKernelGGL(k_bbox<float>, dim3(c->n_blocks), dim3(256), 0, st, c->x, c->y, c->z, c->blk_sys,
-                       c->blk_start, c->blk_count, c->sys_bbox);
-    hipLaunchKernelGGL(k_root<float>, dim3(grid_for(c->nsys, 64)), dim3(64), 0, st, c->sys_bbox, c->sys_off, c->nsys,
+    hipLaunchKernelGGL(k_bbox<float>, dim3(c->n_blocks), dim3(kSortThreads), 0, st, c->x, c->y, c->z,
+                       c->blk_start, c->blk_count, c->blk_bbox);
+    hipLaunchKernelGGL(k_root<float>, dim3(c->nsys), dim3(256), 0, st, c->blk_bbox, c->sys_blk0, c->sys_off,
                        c->sys_geom);
     if (levels == kLevelsTiny)
         hipLaunchKernelGGL((k_keys<kLevelsTiny, float>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x, c->y,
@@ -781,9 +810,9 @@ __global__ __launch_bounds__(256) void k_link(const int32_t* __restrict__ nd_par
     }
 }
 
-// Child counts, internal nodes' particle counts, FP64 geometry (replaying Tree.cpp:96-99
-// along the key), leaf info word, the per-depth histogram of internal nodes (privatised in
-// LDS, one global atomic per block and depth) and the sorted-position -> leaf map.
+// Child counts, FP64 geometry (replaying Tree.cpp:96-99 along the key), leaf info word and
+// each block's count of internal nodes per depth (counted in LDS, written as one row entry
+// per depth: global same-address atomics from every block cost more than the rest).
 template <int LEVELS>
 __global__ __launch_bounds__(256) void k_node_finish(const uint64_t* __restrict__ kh, const uint64_t* __restrict__ kl,
                                                      const int32_t* __restrict__ pos_sys,
@@ -793,8 +822,8 @@ __global__ __launch_bounds__(256) void k_node_finish(const uint64_t* __restrict_
                                                      int32_t* __restrict__ nd_count,
                                                      const int32_t* __restrict__ nd_depth,
                                                      const int32_t* __restrict__ nd_child, double* __restrict__ nd_geom,
-                                                     float4* __restrict__ nd_aux, int32_t* __restrict__ leaf_of,
-                                                     DevScalars* __restrict__ sc) {
+                                                     float4* __restrict__ nd_aux, const DevScalars* __restrict__ sc,
+                                                     uint32_t* __restrict__ dep_hist) {
     __shared__ uint32_t s_hist[64];
     if (threadIdx.x < 64) s_hist[threadIdx.x] = 0;
     __syncthreads();
@@ -829,35 +858,26 @@ __global__ __launch_bounds__(256) void k_node_finish(const uint64_t* __restrict_
         const int cnt = (nk == 0) ? nd_count[id] : 0;
         const uint32_t info = (nk == 0) ? (kLeafBit | static_cast<uint32_t>(cnt)) : static_cast<uint32_t>(nk);
         nd_aux[id] = make_float4(0.0f, static_cast<float>(size * size), 0.0f, __uint_as_float(info));
-        if (nk > 0) {
-            atomicAdd(&s_hist[d], 1u);
-        } else {
-            for (int k = i; k < i + cnt; ++k) leaf_of[k] = id;
-        }
+        if (nk > 0) atomicAdd(&s_hist[d], 1u);
     }
     __syncthreads();
-    if (threadIdx.x < 64 && s_hist[threadIdx.x]) atomicAdd(&sc->depth_count[threadIdx.x], s_hist[threadIdx.x]);
+    if (threadIdx.x < kInternalDepths<LEVELS>) dep_hist[threadIdx.x * kNodeBlocks + blockIdx.x] = s_hist[threadIdx.x];
 }
 
-__global__ void k_depth_scan(DevScalars* __restrict__ sc) {
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (int d = 0; d < 64; ++d) {
-            sc->depth_offset[d] = acc;
-            sc->depth_fill[d] = acc;
-            acc += sc->depth_count[d];
-        }
-    }
-}
-
-// Internal nodes grouped by depth: each block ranks its nodes per depth in LDS, then
-// reserves one range per depth with a single global atomic.
+// Internal nodes grouped by depth.  The per-block counts, laid out depth-major, are
+// scanned exclusively (depth_ranges): entry (d, b) is then the first slot of block b's nodes
+// of depth d, and row d's tile sum the number of nodes at depth d.  Each block ranks its
+// nodes per depth in LDS and places them from those slots; it covers the same node ids as
+// in k_node_finish (same grid, same stride).
 __global__ __launch_bounds__(256) void k_depth_scatter(const int32_t* __restrict__ nd_depth,
-                                                       const float4* __restrict__ nd_aux, DevScalars* __restrict__ sc,
+                                                       const float4* __restrict__ nd_aux,
+                                                       const DevScalars* __restrict__ sc,
+                                                       const uint32_t* __restrict__ dep_off, int ndepth,
                                                        int32_t* __restrict__ depth_nodes) {
     __shared__ uint32_t s_cnt[64];
     __shared__ uint32_t s_base[64];
-    const int nn = sc->n_nodes;
+    const int nn = sc->cap_exceeded ? 0 : sc->n_nodes;
+    if (threadIdx.x < ndepth) s_base[threadIdx.x] = dep_off[threadIdx.x * kNodeBlocks + blockIdx.x];
     for (int base = blockIdx.x * blockDim.x; base < nn; base += gridDim.x * blockDim.x) {
         if (threadIdx.x < 64) s_cnt[threadIdx.x] = 0;
         __syncthreads();
@@ -869,68 +889,19 @@ __global__ __launch_bounds__(256) void k_depth_scatter(const int32_t* __restrict
             rank = atomicAdd(&s_cnt[d], 1u);
         }
         __syncthreads();
-        if (threadIdx.x < 64 && s_cnt[threadIdx.x]) s_base[threadIdx.x] = atomicAdd(&sc->depth_fill[threadIdx.x], s_cnt[threadIdx.x]);
-        __syncthreads();
         if (d >= 0) depth_nodes[s_base[d] + rank] = id;
         __syncthreads();
+        if (threadIdx.x < ndepth) s_base[threadIdx.x] += s_cnt[threadIdx.x];
     }
 }
 
-// Leaf members in ascending original index (the reference bucket order).
-__global__ __launch_bounds__(256) void k_leaf_order(const int32_t* __restrict__ leaf_list,
-                                                    const int32_t* __restrict__ nd_start,
-                                                    const int32_t* __restrict__ nd_count,
-                                                    const DevScalars* __restrict__ sc, uint32_t* __restrict__ perm) {
-    const int nl = sc->n_leaves;
-    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nl; li += gridDim.x * blockDim.x) {
-        const int id = leaf_list[li];
-        const int st = nd_start[id], cnt = nd_count[id];
-        if (cnt <= kNcrit) {
-            uint32_t v[kNcrit];
-#pragma unroll
-            for (int k = 0; k < kNcrit; ++k) v[k] = (k < cnt) ? perm[st + k] : 0xffffffffu;
-            // insertion sort of <= 8 values
-#pragma unroll
-            for (int a = 1; a < kNcrit; ++a) {
-#pragma unroll
-                for (int b = a; b > 0; --b) {
-                    const uint32_t lo = min(v[b - 1], v[b]), hi = max(v[b - 1], v[b]);
-                    v[b - 1] = lo;
-                    v[b] = hi;
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < kNcrit; ++k)
-                if (k < cnt) perm[st + k] = v[k];
-        } else {
-            // depth-capped leaf: plain insertion sort in place
-            for (int a = st + 1; a < st + cnt; ++a) {
-                const uint32_t key = perm[a];
-                int b = a - 1;
-                while (b >= st && perm[b] > key) {
-                    perm[b + 1] = perm[b];
-                    --b;
-                }
-                perm[b + 1] = key;
-            }
-        }
-    }
-}
-
-// Sorted particles: the input-order records packed first (x,y,z, G*m; coalesced), so the
-// permuted read touches one 16-byte record per particle instead of four scattered words.
+// The input-order records packed (x, y, z, m; coalesced), so the per-leaf gather of the
+// sorted particles reads one 16-byte record per particle instead of four scattered words.
 __global__ __launch_bounds__(256) void k_pack4(const float* __restrict__ x, const float* __restrict__ y,
                                                const float* __restrict__ z, const float* __restrict__ m, int n,
-                                               float G, float4* __restrict__ pos4) {
+                                               float4* __restrict__ pos4) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) pos4[i] = make_float4(x[i], y[i], z[i], G * m[i]);
-}
-
-__global__ __launch_bounds__(256) void k_gather(const float4* __restrict__ pos4, const uint32_t* __restrict__ perm,
-                                                int n, float4* __restrict__ spos) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    spos[k] = pos4[perm[k]];
+    if (i < n) pos4[i] = make_float4(x[i], y[i], z[i], m[i]);
 }
 
 template <int LEVELS>
@@ -938,7 +909,7 @@ void launch_build_t(nbg_ctx* c) {
     hipStream_t st = c->stream;
     const int n = c->n;
     const int g = grid_for(n, 256);
-    const int gnodes = 2048;  // grid-stride kernels over the device-side node count
+    const int gnodes = kNodeBlocks;  // grid-stride kernels over the device-side node count
     hipLaunchKernelGGL(k_leaf_depth<LEVELS>, dim3(g), dim3(256), 0, st, c->key_sorted, c->klo_sorted, c->pos_sys,
                        c->sys_off, n, c->leaf_depth, c->cpl_prev, c->node_scan, c->dsc, c->blk_depth);
     // node_scan[n] receives the grand total (the array holds n+1 entries)
@@ -951,12 +922,11 @@ void launch_build_t(nbg_ctx* c) {
     hipLaunchKernelGGL(k_link, dim3(gnodes), dim3(256), 0, st, c->nd_parent, c->nd_oct, c->dsc, c->nd_child);
     hipLaunchKernelGGL(k_node_finish<LEVELS>, dim3(gnodes), dim3(256), 0, st, c->key_sorted, c->klo_sorted,
                        c->pos_sys, c->sys_off, c->sys_geom, c->nd_start, c->nd_count, c->nd_depth, c->nd_child,
-                       c->nd_geom, c->nd_aux, c->leaf_of, c->dsc);
-    hipLaunchKernelGGL(k_depth_scan, dim3(1), dim3(64), 0, st, c->dsc);
-    hipLaunchKernelGGL(k_depth_scatter, dim3(gnodes), dim3(256), 0, st, c->nd_depth, c->nd_aux, c->dsc,
-                       c->depth_nodes);
-    hipLaunchKernelGGL(k_leaf_order, dim3(gnodes), dim3(256), 0, st, c->leaf_list, c->nd_start, c->nd_count, c->dsc,
-                       c->perm);
+                       c->nd_geom, c->nd_aux, c->dsc, c->dep_hist);
+    constexpr int nd = kInternalDepths<LEVELS>;
+    excl_scan<uint32_t>(st, c->dep_hist, c->dep_off, static_cast<int64_t>(nd) * kNodeBlocks, c->dep_part, nullptr);
+    hipLaunchKernelGGL(k_depth_scatter, dim3(gnodes), dim3(256), 0, st, c->nd_depth, c->nd_aux, c->dsc, c->dep_off,
+                       nd, c->depth_nodes);
 }
 
 void launch_build(nbg_ctx* c, int levels) {
@@ -972,52 +942,88 @@ void launch_build(nbg_ctx* c, int levels) {
 
 // ============================================================================ upsweep
 // Moments of every node (G-scaled masses): leaves from their particles, internal nodes
-// from their children one depth at a time, deepest first (Tree.cpp:179-202, 205-254).
-// Centres of mass are reduced in FP64 (exact MAC decisions need them to FP64 accuracy;
-// the walk's M2P uses their FP32 rounding); quadrupoles are accumulated in FP32 from
-// offsets relative to the node's centre of mass.  A massless cell gets an undefined
-// (NaN) centre of mass, as the incremental update of Tree.cpp:120-122 produces: the walk
-// never uses it as a source, and a massless group then opens every cell, as in the
-// reference.
+// from their children, deepest first (Tree.cpp:179-202, 205-254).  Centres of mass are
+// reduced in FP64 (exact MAC decisions need them to FP64 accuracy; the walk's M2P uses
+// their FP32 rounding); quadrupoles are accumulated in FP32 from offsets relative to the
+// node's centre of mass.  A massless cell gets an undefined (NaN) centre of mass, as the
+// incremental update of Tree.cpp:120-122 produces: the walk never uses it as a source, and
+// a massless group then opens every cell, as in the reference.
 
+// Per leaf, one lane each: its members put in ascending original index (the reference
+// bucket order, so `order` lists them as Finalize does), their packed records gathered into
+// sorted order as (x, y, z, G*m), the sorted-position -> leaf map, and the leaf's moments.
 // F64: the positions of an FP64 state (x64, y64, z64, input order) give the FP64 centre
 // of mass and radius, and the quadrupole's offsets from that centre (rounded to FP32);
-// otherwise the FP32 sorted records do.
+// otherwise the FP32 records do.
 template <bool QUAD, bool MOM, bool F64>
-__global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ leaf_list,
-                                                 const int32_t* __restrict__ nd_start,
-                                                 const int32_t* __restrict__ nd_count,
-                                                 const float4* __restrict__ spos, const float* __restrict__ mass,
-                                                 const uint32_t* __restrict__ perm, double G,
-                                                 const DevScalars* __restrict__ sc, float4* __restrict__ nd_com,
-                                                 float4* __restrict__ nd_quad, float4* __restrict__ nd_aux,
-                                                 double4* __restrict__ nd_com64, double* __restrict__ nd_rmax64,
-                                                 float4* __restrict__ nd_mom, const double* __restrict__ x64,
-                                                 const double* __restrict__ y64, const double* __restrict__ z64) {
-    const int nl = sc->n_leaves;
+__global__ __launch_bounds__(256) void k_leaves(const int32_t* __restrict__ leaf_list,
+                                                const int32_t* __restrict__ nd_start,
+                                                const int32_t* __restrict__ nd_count,
+                                                const float4* __restrict__ pos4, uint32_t* __restrict__ perm, float Gf,
+                                                double G, const DevScalars* __restrict__ sc,
+                                                float4* __restrict__ spos, int32_t* __restrict__ leaf_of,
+                                                float4* __restrict__ nd_com, float4* __restrict__ nd_quad,
+                                                float4* __restrict__ nd_aux, double4* __restrict__ nd_com64,
+                                                double* __restrict__ nd_rmax64, float4* __restrict__ nd_mom,
+                                                const double* __restrict__ x64, const double* __restrict__ y64,
+                                                const double* __restrict__ z64) {
+    const int nl = sc->cap_exceeded ? 0 : sc->n_leaves;  // a truncated pool: nothing to do
     for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nl; li += gridDim.x * blockDim.x) {
         const int id = leaf_list[li];
         const int st = nd_start[id], cnt = nd_count[id];
-        // A leaf below the depth cap holds at most kNcrit particles: they are loaded once into
-        // registers (all loads in flight together) and the three passes below run on them;
-        // a capped leaf of any size streams its particles from memory in each pass.  Both
-        // forms accumulate in ascending position order, so they give the same bits.
+        // A leaf below the depth cap holds at most kNcrit particles: they are sorted and
+        // gathered in registers (all loads in flight together) and the passes below run on
+        // them; a capped leaf of any size is sorted in place and streamed from memory in
+        // each pass.  Both forms accumulate in ascending position order.
         const bool small = cnt <= kNcrit;
         float4 pr[kNcrit];
         double wr[kNcrit];
         double3 pd[F64 ? kNcrit : 1];
         if (small) {
-            uint32_t pi[kNcrit];
+            uint32_t v[kNcrit];
 #pragma unroll
-            for (int u = 0; u < kNcrit; ++u) {
-                const int k = st + min(u, cnt - 1);
-                pr[u] = spos[k];
-                pi[u] = perm[k];
+            for (int k = 0; k < kNcrit; ++k) v[k] = (k < cnt) ? perm[st + k] : 0xffffffffu;
+            // insertion network over <= 8 values (absent ones sort last)
+#pragma unroll
+            for (int a = 1; a < kNcrit; ++a) {
+#pragma unroll
+                for (int b = a; b > 0; --b) {
+                    const uint32_t lo = min(v[b - 1], v[b]), hi = max(v[b - 1], v[b]);
+                    v[b - 1] = lo;
+                    v[b] = hi;
+                }
             }
 #pragma unroll
             for (int u = 0; u < kNcrit; ++u) {
-                wr[u] = mass[pi[u]];
-                if (F64) pd[u] = make_double3(x64[pi[u]], y64[pi[u]], z64[pi[u]]);
+                const uint32_t q = (u < cnt) ? v[u] : v[0];
+                const float4 r = pos4[q];
+                pr[u] = make_float4(r.x, r.y, r.z, Gf * r.w);
+                wr[u] = static_cast<double>(r.w);
+                if (F64) pd[u] = make_double3(x64[q], y64[q], z64[q]);
+            }
+#pragma unroll
+            for (int u = 0; u < kNcrit; ++u) {
+                if (u < cnt) {
+                    perm[st + u] = v[u];
+                    spos[st + u] = pr[u];
+                    leaf_of[st + u] = id;
+                }
+            }
+        } else {
+            // depth-capped leaf: plain insertion sort in place, then the gather
+            for (int a = st + 1; a < st + cnt; ++a) {
+                const uint32_t key = perm[a];
+                int b = a - 1;
+                while (b >= st && perm[b] > key) {
+                    perm[b + 1] = perm[b];
+                    --b;
+                }
+                perm[b + 1] = key;
+            }
+            for (int k = st; k < st + cnt; ++k) {
+                const float4 r = pos4[perm[k]];
+                spos[k] = make_float4(r.x, r.y, r.z, Gf * r.w);
+                leaf_of[k] = id;
             }
         }
         // calls f(p, w, position in FP64) for every member in ascending order
@@ -1028,10 +1034,11 @@ __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ lea
                     if (u < cnt) f(pr[u], wr[u], F64 ? pd[u] : make_double3(pr[u].x, pr[u].y, pr[u].z));
             } else {
                 for (int k = st; k < st + cnt; ++k) {
-                    const float4 p = spos[k];
                     const uint32_t i = perm[k];
-                    f(p, static_cast<double>(mass[i]), F64 ? make_double3(x64[i], y64[i], z64[i])
-                                                           : make_double3(p.x, p.y, p.z));
+                    const float4 r = pos4[i];
+                    const float4 p = make_float4(r.x, r.y, r.z, Gf * r.w);
+                    f(p, static_cast<double>(r.w), F64 ? make_double3(x64[i], y64[i], z64[i])
+                                                       : make_double3(p.x, p.y, p.z));
                 }
             }
         };
@@ -1102,10 +1109,10 @@ __global__ __launch_bounds__(256) void k_up_leaf(const int32_t* __restrict__ lea
         nd_rmax64[id] = rmax;
         nd_com[id] = make_float4(fx, fy, fz, static_cast<float>(G * M));
         nd_quad[id] = make_float4(qxx, qyy, qxy, qxz);
-        float4 a = nd_aux[id];
-        a.x = qyz;
-        a.z = static_cast<float>(rmax);
-        nd_aux[id] = a;
+        // the aux word's x (qyz) and z (radius); y (size^2) and w (info) stay as built
+        float* aux = reinterpret_cast<float*>(nd_aux + id);
+        aux[0] = qyz;
+        aux[2] = static_cast<float>(rmax);
     }
 }
 
@@ -1115,196 +1122,335 @@ __device__ __forceinline__ void pin(float4& v) {
     asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
 }
 
-// Branch-free over the children: an absent child (index < 0) reads this node's own slot
-// and every accumulation it would make is discarded by a select, so all of a node's child
-// loads are in flight together (behind per-child branches they were 40-odd dependent
-// memory round trips per node, 7 us for the root alone).  The selects keep the arithmetic
-// of the skip-by-branch form term for term.
+// What a node's parent combines from it: FP64 centre of mass and mass, the aux word (x:
+// qyz, z: radius), the quadrupole's other four terms and the particle count.
+struct UpRec {
+    double4 cd;
+    float4 ca;
+    float4 cq;
+    int cn;
+};
+
+// One internal node from its eight child slots (ch[c] < 0: no child; its slot may hold
+// anything).  rec(c) gives child c's record, mom(c, j) word j of its learned moments.
+// Branch-free over the children: every child's record is fetched before the first is used
+// (behind per-child branches they were 40-odd dependent memory round trips per node, 7 us
+// for the root alone), and every accumulation an absent child would make is discarded by a
+// select that keeps the arithmetic of the skip-by-branch form term for term.
 // F64: the offsets of the children's centres of mass (quadrupole shifts and radius) are
 // taken in FP64 before rounding (an FP64 state); otherwise from their FP32 roundings.
+template <bool QUAD, bool MOM, bool F64, class Rec, class Mom>
+__device__ __forceinline__ void up_combine(const int (&ch)[8], double G, Rec rec, Mom mom, UpRec& out,
+                                           float (&mo)[16]) {
+    static_assert(!(MOM && F64), "the learned moments are built on FP32 states only");
+    double4 cd[8];
+    float4 cc[8], ca[8], cq[8];
+    int cn[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const UpRec r = rec(c);
+        cd[c] = r.cd;
+        ca[c] = r.ca;
+        cq[c] = r.cq;
+        cn[c] = r.cn;
+    }
+    // Values used only on one side of a select would otherwise be loaded inside a branch
+    // (one memory round trip per child); pinning them here keeps one wait.
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        // a child's FP32 record is the rounding of its FP64 one (both written from the same
+        // values), so it is rebuilt instead of gathered
+        cc[c] = make_float4(static_cast<float>(cd[c].x), static_cast<float>(cd[c].y), static_cast<float>(cd[c].z),
+                            static_cast<float>(G * cd[c].w));
+        pin(ca[c]);
+        if (QUAD) pin(cq[c]);
+    }
+    double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const double w = cd[c].w;
+        const bool use = ch[c] >= 0 && w != 0.0;
+        M = use ? M + w : M;
+        sx = use ? sx + w * cd[c].x : sx;
+        sy = use ? sy + w * cd[c].y : sy;
+        sz = use ? sz + w * cd[c].z : sz;
+    }
+    double gx = __longlong_as_double(0x7ff8000000000000ll), gy = gx, gz = gx;
+    if (M != 0.0) {
+        gx = sx / M;
+        gy = sy / M;
+        gz = sz / M;
+    }
+    const float fx = static_cast<float>(gx), fy = static_cast<float>(gy), fz = static_cast<float>(gz);
+    float qxx = 0.f, qyy = 0.f, qxy = 0.f, qxz = 0.f, qyz = 0.f, rmax = 0.f;
+    const bool massive = M > 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const bool live = massive && ch[c] >= 0;
+        const float dx = F64 ? static_cast<float>(cd[c].x - gx) : cc[c].x - fx;
+        const float dy = F64 ? static_cast<float>(cd[c].y - gy) : cc[c].y - fy;
+        const float dz = F64 ? static_cast<float>(cd[c].z - gz) : cc[c].z - fz;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        rmax = live ? fmaxf(rmax, ca[c].z + sqrtf(d2)) : rmax;
+        const bool heavy = live && cc[c].w > 0.f;
+        if (QUAD) {
+            const float w = cc[c].w;
+            qxx = heavy ? qxx + (cq[c].x + w * (3.f * dx * dx - d2)) : qxx;
+            qyy = heavy ? qyy + (cq[c].y + w * (3.f * dy * dy - d2)) : qyy;
+            qxy = heavy ? qxy + (cq[c].z + w * 3.f * dx * dy) : qxy;
+            qxz = heavy ? qxz + (cq[c].w + w * 3.f * dx * dz) : qxz;
+            qyz = heavy ? qyz + (ca[c].x + w * 3.f * dy * dz) : qyz;
+        }
+    }
+    // the learned moments: the children's 16 words are fetched after the centre of mass,
+    // four children at a time (all eight at once held ~300 registers, one wave per SIMD);
+    // each moment still sums the children in order
+#pragma unroll
+    for (int i = 0; i < 16; ++i) mo[i] = 0.f;
+#pragma unroll
+    for (int c0 = 0; c0 < (MOM ? 8 : 0); c0 += 4) {
+        float4 cm[16];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) cm[4 * c + j] = mom(c0 + c, j);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) pin(cm[4 * c]), pin(cm[4 * c + 1]), pin(cm[4 * c + 2]), pin(cm[4 * c + 3]);
+#pragma unroll
+        for (int cc4 = 0; cc4 < 4; ++cc4) {
+            const int c = c0 + cc4;
+            const bool heavy = massive && ch[c] >= 0 && cc[c].w > 0.f;
+            const float dx = cc[c].x - fx, dy = cc[c].y - fy, dz = cc[c].z - fz;
+            // parallel-axis shift of the child's moments to this com
+            const float4 q0 = cm[4 * cc4 + 0], q1 = cm[4 * cc4 + 1], q2 = cm[4 * cc4 + 2], q3 = cm[4 * cc4 + 3];
+            const float m = cc[c].w;
+            const float sxx = q0.x, syy = q0.y, szz = q0.z, sxy = q0.w, sxz = q1.x, syz = q1.y;
+            float t[16];
+            t[0] = sxx + m * dx * dx;
+            t[1] = syy + m * dy * dy;
+            t[2] = szz + m * dz * dz;
+            t[3] = sxy + m * dx * dy;
+            t[4] = sxz + m * dx * dz;
+            t[5] = syz + m * dy * dz;
+            t[6] = q1.z + 3.f * sxx * dx + m * dx * dx * dx;
+            t[7] = q1.w + 3.f * syy * dy + m * dy * dy * dy;
+            t[8] = q2.x + 3.f * szz * dz + m * dz * dz * dz;
+            t[9] = q2.y + sxx * dy + 2.f * sxy * dx + m * dx * dx * dy;
+            t[10] = q2.z + sxx * dz + 2.f * sxz * dx + m * dx * dx * dz;
+            t[11] = q2.w + syy * dx + 2.f * sxy * dy + m * dx * dy * dy;
+            t[12] = q3.x + syy * dz + 2.f * syz * dy + m * dy * dy * dz;
+            t[13] = q3.y + szz * dx + 2.f * sxz * dz + m * dx * dz * dz;
+            t[14] = q3.z + szz * dy + 2.f * syz * dz + m * dy * dz * dz;
+            t[15] = q3.w + sxy * dz + sxz * dy + syz * dx + m * dx * dy * dz;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) mo[i] = heavy ? mo[i] + t[i] : mo[i];
+        }
+    }
+    int count = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) count += ch[c] >= 0 ? cn[c] : 0;
+    out.cd = make_double4(gx, gy, gz, M);
+    out.ca = make_float4(qyz, 0.f, rmax, 0.f);
+    out.cq = make_float4(qxx, qyy, qxy, qxz);
+    out.cn = count;
+}
+
+// An internal node's combined record to its arrays (the aux word's y and w stay as built).
+template <bool MOM>
+__device__ __forceinline__ void up_store(int id, const UpRec& r, const float (&mo)[16], double G,
+                                         int32_t* __restrict__ nd_count, float4* __restrict__ nd_com,
+                                         float4* __restrict__ nd_quad, float4* __restrict__ nd_aux,
+                                         double4* __restrict__ nd_com64, float4* __restrict__ nd_mom) {
+    if (MOM) {
+        nd_mom[4 * id + 0] = make_float4(mo[0], mo[1], mo[2], mo[3]);
+        nd_mom[4 * id + 1] = make_float4(mo[4], mo[5], mo[6], mo[7]);
+        nd_mom[4 * id + 2] = make_float4(mo[8], mo[9], mo[10], mo[11]);
+        nd_mom[4 * id + 3] = make_float4(mo[12], mo[13], mo[14], mo[15]);
+    }
+    nd_count[id] = r.cn;
+    nd_com64[id] = r.cd;
+    nd_com[id] = make_float4(static_cast<float>(r.cd.x), static_cast<float>(r.cd.y), static_cast<float>(r.cd.z),
+                             static_cast<float>(G * r.cd.w));
+    nd_quad[id] = r.cq;
+    float* aux = reinterpret_cast<float*>(nd_aux + id);
+    aux[0] = r.ca.x;
+    aux[2] = r.ca.z;
+}
+
+// A node's record as its arrays hold it (a leaf's, or an internal node combined earlier).
+template <bool QUAD>
+__device__ __forceinline__ UpRec up_load(int i, const int32_t* __restrict__ nd_count,
+                                         const float4* __restrict__ nd_quad, const float4* __restrict__ nd_aux,
+                                         const double4* __restrict__ nd_com64) {
+    UpRec r;
+    r.cd = nd_com64[i];
+    r.ca = nd_aux[i];
+    if (QUAD) r.cq = nd_quad[i];
+    r.cn = nd_count[i];
+    return r;
+}
+
+// Combines internal node `id` from its children's records in memory.
+template <bool QUAD, bool MOM, bool F64>
+__device__ __forceinline__ void up_node_mem(int id, const int (&ch)[8], double G, int32_t* __restrict__ nd_count,
+                                            float4* __restrict__ nd_com, float4* __restrict__ nd_quad,
+                                            float4* __restrict__ nd_aux, double4* __restrict__ nd_com64,
+                                            float4* __restrict__ nd_mom, UpRec& out, float (&mo)[16]) {
+    int src[8];  // an absent child reads the node's own slot (discarded)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) src[c] = ch[c] >= 0 ? ch[c] : id;
+    up_combine<QUAD, MOM, F64>(
+        ch, G,
+        [&](int c) __attribute__((always_inline)) { return up_load<QUAD>(src[c], nd_count, nd_quad, nd_aux, nd_com64); },
+        [&](int c, int j) __attribute__((always_inline)) { return nd_mom[4 * src[c] + j]; }, out, mo);
+    up_store<MOM>(id, out, mo, G, nd_count, nd_com, nd_quad, nd_aux, nd_com64, nd_mom);
+}
+
+// One depth: a lane per internal node of that depth.
 template <bool QUAD, bool MOM, bool F64>
 __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* __restrict__ depth_nodes,
                                                      const int32_t* __restrict__ nd_child,
-                                                     int32_t* __restrict__ nd_count,
-                                                     double G, const DevScalars* __restrict__ sc,
+                                                     int32_t* __restrict__ nd_count, double G,
+                                                     const uint32_t* __restrict__ dep_part,
+                                                     const uint32_t* __restrict__ dep_off,
                                                      float4* __restrict__ nd_com, float4* __restrict__ nd_quad,
                                                      float4* __restrict__ nd_aux, double4* __restrict__ nd_com64,
                                                      float4* __restrict__ nd_mom) {
-    static_assert(!(MOM && F64), "the learned moments are built on FP32 states only");
-    const int cnt = static_cast<int>(sc->depth_count[depth]);
-    const int off = static_cast<int>(sc->depth_offset[depth]);
+    const int cnt = static_cast<int>(dep_part[depth]);
+    const int off = static_cast<int>(dep_off[depth * kNodeBlocks]);
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
         const int id = depth_nodes[off + k];
         const int4 c0 = reinterpret_cast<const int4*>(nd_child)[2 * id];
         const int4 c1 = reinterpret_cast<const int4*>(nd_child)[2 * id + 1];
         const int ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-        int src[8];
+        UpRec r;
+        float mo[16];
+        up_node_mem<QUAD, MOM, F64>(id, ch, G, nd_count, nd_com, nd_quad, nd_aux, nd_com64, nd_mom, r, mo);
+    }
+}
+
+// Two depths per launch, which halves the launches and the memory round trips per depth
+// (index list, child lists) that bound this small, latency-bound work: 8 lanes per internal
+// node of `depth`, lane a on its child a (at depth + 1).  An internal child is combined from
+// its own children (done by the previous launch) and written; a leaf child's record is read.
+// The 8 records go through LDS to the group's first lane, which combines the node.  Every
+// node is combined by up_combine, as in the one-depth kernel, so the results are the same
+// bits.
+constexpr int kUpPairNodes = 32;  // nodes per 256-thread block
+template <bool QUAD, bool MOM, bool F64>
+__global__ __launch_bounds__(256) void k_up_pair(int depth, const int32_t* __restrict__ depth_nodes,
+                                                 const int32_t* __restrict__ nd_child,
+                                                 int32_t* __restrict__ nd_count, double G,
+                                                 const uint32_t* __restrict__ dep_part,
+                                                 const uint32_t* __restrict__ dep_off, float4* __restrict__ nd_com,
+                                                 float4* __restrict__ nd_quad, float4* __restrict__ nd_aux,
+                                                 double4* __restrict__ nd_com64, float4* __restrict__ nd_mom) {
+    __shared__ UpRec s_rec[256];
+    __shared__ float4 s_mom[MOM ? 4 * 256 : 1];
+    __shared__ int s_ch[256];
+    const int cnt = static_cast<int>(dep_part[depth]);
+    const int off = static_cast<int>(dep_off[depth * kNodeBlocks]);
+    const int t = threadIdx.x;
+    const int a = t & 7, g0 = t & ~7;
+    // block-uniform loop: every lane of a wave reaches each wave_sync
+    for (int base = blockIdx.x * kUpPairNodes; base < cnt; base += gridDim.x * kUpPairNodes) {
+        const int k = base + (t >> 3);
+        const int id = (k < cnt) ? depth_nodes[off + k] : -1;
+        const int c = (id >= 0) ? nd_child[8 * id + a] : -1;
+        s_ch[t] = c;
+        if (c >= 0) {
+            const int4 c0 = reinterpret_cast<const int4*>(nd_child)[2 * c];
+            const int4 c1 = reinterpret_cast<const int4*>(nd_child)[2 * c + 1];
+            const int gch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+            const bool internal = max(max(max(c0.x, c0.y), max(c0.z, c0.w)), max(max(c1.x, c1.y), max(c1.z, c1.w))) >= 0;
+            UpRec r;
+            float mo[16];
+            if (internal) {
+                up_node_mem<QUAD, MOM, F64>(c, gch, G, nd_count, nd_com, nd_quad, nd_aux, nd_com64, nd_mom, r, mo);
+            } else {
+                r = up_load<QUAD>(c, nd_count, nd_quad, nd_aux, nd_com64);
+                if (MOM) {
 #pragma unroll
-        for (int c = 0; c < 8; ++c) src[c] = ch[c] >= 0 ? ch[c] : id;
-        double4 cd[8];
-        float4 cc[8], ca[8], cq[8];
-        int cn[8];
+                    for (int j = 0; j < 4; ++j) {
+                        const float4 v = nd_mom[4 * c + j];
+                        mo[4 * j + 0] = v.x;
+                        mo[4 * j + 1] = v.y;
+                        mo[4 * j + 2] = v.z;
+                        mo[4 * j + 3] = v.w;
+                    }
+                }
+            }
+            s_rec[t] = r;
+            if (MOM) {
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            cd[c] = nd_com64[src[c]];
-            ca[c] = nd_aux[src[c]];
-            if (QUAD) cq[c] = nd_quad[src[c]];
-            cn[c] = nd_count[src[c]];
-        }
-        // Values used only on one side of a select would otherwise be loaded inside a
-        // branch (one memory round trip per child); pinning them here keeps one wait.
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            // a child's FP32 record is the rounding of its FP64 one (both written below and
-            // by k_up_leaf from the same values), so it is rebuilt instead of gathered
-            cc[c] = make_float4(static_cast<float>(cd[c].x), static_cast<float>(cd[c].y), static_cast<float>(cd[c].z),
-                                static_cast<float>(G * cd[c].w));
-            pin(ca[c]);
-            if (QUAD) pin(cq[c]);
-        }
-        double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const double w = cd[c].w;
-            const bool use = ch[c] >= 0 && w != 0.0;
-            M = use ? M + w : M;
-            sx = use ? sx + w * cd[c].x : sx;
-            sy = use ? sy + w * cd[c].y : sy;
-            sz = use ? sz + w * cd[c].z : sz;
-        }
-        double gx = __longlong_as_double(0x7ff8000000000000ll), gy = gx, gz = gx;
-        if (M != 0.0) {
-            gx = sx / M;
-            gy = sy / M;
-            gz = sz / M;
-        }
-        const float fx = static_cast<float>(gx), fy = static_cast<float>(gy), fz = static_cast<float>(gz);
-        float qxx = 0.f, qyy = 0.f, qxy = 0.f, qxz = 0.f, qyz = 0.f, rmax = 0.f;
-        const bool massive = M > 0.0;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const bool live = massive && ch[c] >= 0;
-            const float dx = F64 ? static_cast<float>(cd[c].x - gx) : cc[c].x - fx;
-            const float dy = F64 ? static_cast<float>(cd[c].y - gy) : cc[c].y - fy;
-            const float dz = F64 ? static_cast<float>(cd[c].z - gz) : cc[c].z - fz;
-            const float d2 = dx * dx + dy * dy + dz * dz;
-            rmax = live ? fmaxf(rmax, ca[c].z + sqrtf(d2)) : rmax;
-            const bool heavy = live && cc[c].w > 0.f;
-            if (QUAD) {
-                const float w = cc[c].w;
-                qxx = heavy ? qxx + (cq[c].x + w * (3.f * dx * dx - d2)) : qxx;
-                qyy = heavy ? qyy + (cq[c].y + w * (3.f * dy * dy - d2)) : qyy;
-                qxy = heavy ? qxy + (cq[c].z + w * 3.f * dx * dy) : qxy;
-                qxz = heavy ? qxz + (cq[c].w + w * 3.f * dx * dz) : qxz;
-                qyz = heavy ? qyz + (ca[c].x + w * 3.f * dy * dz) : qyz;
+                for (int j = 0; j < 4; ++j) s_mom[4 * t + j] = make_float4(mo[4 * j], mo[4 * j + 1], mo[4 * j + 2], mo[4 * j + 3]);
             }
         }
-        // the learned moments: the children's 16 words are gathered after the centre of
-        // mass, four children at a time (all eight at once held ~300 registers, one wave
-        // per SIMD); each moment still sums the children in order
-        float mo[16] = {};
+        wave_sync();
+        if (a == 0 && id >= 0) {
+            int ch[8];
 #pragma unroll
-        for (int c0 = 0; c0 < (MOM ? 8 : 0); c0 += 4) {
-            float4 cm[16];
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) cm[4 * c + j] = nd_mom[4 * src[c0 + c] + j];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) pin(cm[4 * c]), pin(cm[4 * c + 1]), pin(cm[4 * c + 2]), pin(cm[4 * c + 3]);
-#pragma unroll
-            for (int cc4 = 0; cc4 < 4; ++cc4) {
-                const int c = c0 + cc4;
-                const bool heavy = massive && ch[c] >= 0 && cc[c].w > 0.f;
-                const float dx = cc[c].x - fx, dy = cc[c].y - fy, dz = cc[c].z - fz;
-                // parallel-axis shift of the child's moments to this com
-                const float4 q0 = cm[4 * cc4 + 0], q1 = cm[4 * cc4 + 1], q2 = cm[4 * cc4 + 2], q3 = cm[4 * cc4 + 3];
-                const float m = cc[c].w;
-                const float sxx = q0.x, syy = q0.y, szz = q0.z, sxy = q0.w, sxz = q1.x, syz = q1.y;
-                float t[16];
-                t[0] = sxx + m * dx * dx;
-                t[1] = syy + m * dy * dy;
-                t[2] = szz + m * dz * dz;
-                t[3] = sxy + m * dx * dy;
-                t[4] = sxz + m * dx * dz;
-                t[5] = syz + m * dy * dz;
-                t[6] = q1.z + 3.f * sxx * dx + m * dx * dx * dx;
-                t[7] = q1.w + 3.f * syy * dy + m * dy * dy * dy;
-                t[8] = q2.x + 3.f * szz * dz + m * dz * dz * dz;
-                t[9] = q2.y + sxx * dy + 2.f * sxy * dx + m * dx * dx * dy;
-                t[10] = q2.z + sxx * dz + 2.f * sxz * dx + m * dx * dx * dz;
-                t[11] = q2.w + syy * dx + 2.f * sxy * dy + m * dx * dy * dy;
-                t[12] = q3.x + syy * dz + 2.f * syz * dy + m * dy * dy * dz;
-                t[13] = q3.y + szz * dx + 2.f * sxz * dz + m * dx * dz * dz;
-                t[14] = q3.z + szz * dy + 2.f * syz * dz + m * dy * dz * dz;
-                t[15] = q3.w + sxy * dz + sxz * dy + syz * dx + m * dx * dy * dz;
-#pragma unroll
-                for (int i = 0; i < 16; ++i) mo[i] = heavy ? mo[i] + t[i] : mo[i];
-            }
+            for (int j = 0; j < 8; ++j) ch[j] = s_ch[g0 + j];
+            UpRec r;
+            float mo[16];
+            up_combine<QUAD, MOM, F64>(
+                ch, G, [&](int j) __attribute__((always_inline)) { return s_rec[g0 + j]; },
+                [&](int j, int w) __attribute__((always_inline)) { return s_mom[4 * (g0 + j) + w]; }, r, mo);
+            up_store<MOM>(id, r, mo, G, nd_count, nd_com, nd_quad, nd_aux, nd_com64, nd_mom);
         }
-        if (MOM) {
-            nd_mom[4 * id + 0] = make_float4(mo[0], mo[1], mo[2], mo[3]);
-            nd_mom[4 * id + 1] = make_float4(mo[4], mo[5], mo[6], mo[7]);
-            nd_mom[4 * id + 2] = make_float4(mo[8], mo[9], mo[10], mo[11]);
-            nd_mom[4 * id + 3] = make_float4(mo[12], mo[13], mo[14], mo[15]);
-        }
-        int count = 0;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) count += ch[c] >= 0 ? cn[c] : 0;
-        nd_count[id] = count;
-        nd_com64[id] = make_double4(gx, gy, gz, M);
-        nd_com[id] = make_float4(fx, fy, fz, static_cast<float>(G * M));
-        nd_quad[id] = make_float4(qxx, qyy, qxy, qxz);
-        float4 a = nd_aux[id];
-        a.x = qyz;
-        a.z = rmax;
-        nd_aux[id] = a;
+        wave_sync();
     }
 }
 
 void launch_upsweep(nbg_ctx* c, int levels, int multipole, double G, bool moments) {
     hipStream_t st = c->stream;
-    const int gnodes = 2048;
+    const int gnodes = kNodeBlocks;
     const bool quad = multipole == NBG_QUADRUPOLE;
     const bool mom = moments && c->nd_mom != nullptr;
-#define NBG_UPLEAF(Q, M, F)                                                                                          \
-    hipLaunchKernelGGL((k_up_leaf<Q, M, F>), dim3(gnodes), dim3(256), 0, st, c->leaf_list, c->nd_start, c->nd_count,  \
-                       c->spos, c->m, c->perm, G, c->dsc, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64,            \
-                       c->nd_rmax64, c->nd_mom, c->x64, c->y64, c->z64)
+    const float Gf = static_cast<float>(G);
+#define NBG_LEAVES(Q, M, F)                                                                                          \
+    hipLaunchKernelGGL((k_leaves<Q, M, F>), dim3(gnodes), dim3(256), 0, st, c->leaf_list, c->nd_start, c->nd_count,   \
+                       c->stage4, c->perm, Gf, G, c->dsc, c->spos, c->leaf_of, c->nd_com, c->nd_quad, c->nd_aux,    \
+                       c->nd_com64, c->nd_rmax64, c->nd_mom, c->x64, c->y64, c->z64)
     if (mom)
-        NBG_UPLEAF(true, true, false);
+        NBG_LEAVES(true, true, false);
     else if (quad && c->f64_state)
-        NBG_UPLEAF(true, false, true);
+        NBG_LEAVES(true, false, true);
     else if (quad)
-        NBG_UPLEAF(true, false, false);
+        NBG_LEAVES(true, false, false);
     else if (c->f64_state)
-        NBG_UPLEAF(false, false, true);
+        NBG_LEAVES(false, false, true);
     else
-        NBG_UPLEAF(false, false, false);
-#undef NBG_UPLEAF
-    // internal nodes sit at depths 0..levels-1 (depth-40 cells are always leaves)
+        NBG_LEAVES(false, false, false);
+#undef NBG_LEAVES
+    // internal nodes sit at depths 0..top (depth-40 cells are always leaves): two depths
+    // per launch from the deepest, (top-1, top), (top-3, top-2), ..., then depth 0 alone
+    // when it is left over
+#define NBG_UP(K, D, GD)                                                                                             \
+    do {                                                                                                             \
+        if (mom)                                                                                                     \
+            hipLaunchKernelGGL((K<true, true, false>), dim3(GD), dim3(256), 0, st, D, c->depth_nodes, c->nd_child,    \
+                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64, c->nd_mom);   \
+        else if (quad && c->f64_state)                                                                               \
+            hipLaunchKernelGGL((K<true, false, true>), dim3(GD), dim3(256), 0, st, D, c->depth_nodes, c->nd_child,    \
+                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64, c->nd_mom);   \
+        else if (quad)                                                                                               \
+            hipLaunchKernelGGL((K<true, false, false>), dim3(GD), dim3(256), 0, st, D, c->depth_nodes, c->nd_child,   \
+                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64, c->nd_mom);   \
+        else if (c->f64_state)                                                                                       \
+            hipLaunchKernelGGL((K<false, false, true>), dim3(GD), dim3(256), 0, st, D, c->depth_nodes, c->nd_child,   \
+                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64, c->nd_mom);   \
+        else                                                                                                         \
+            hipLaunchKernelGGL((K<false, false, false>), dim3(GD), dim3(256), 0, st, D, c->depth_nodes, c->nd_child,  \
+                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64, c->nd_mom);   \
+    } while (0)
     const int top = (levels <= kLevelsShort) ? levels - 1 : kMaxDepth - 1;
-    for (int d = top; d >= 0; --d) {
-        const int gd = d < 6 ? 64 : 1024;
-#define NBG_UPINT(Q, M, F)                                                                                           \
-    hipLaunchKernelGGL((k_up_internal<Q, M, F>), dim3(gd), dim3(256), 0, st, d, c->depth_nodes, c->nd_child,         \
-                       c->nd_count, G,                                                                                \
-                       c->dsc, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64, c->nd_mom)
-        if (mom)
-            NBG_UPINT(true, true, false);
-        else if (quad && c->f64_state)
-            NBG_UPINT(true, false, true);
-        else if (quad)
-            NBG_UPINT(true, false, false);
-        else if (c->f64_state)
-            NBG_UPINT(false, false, true);
-        else
-            NBG_UPINT(false, false, false);
-#undef NBG_UPINT
-    }
+    int d = top;
+    for (; d >= 1; d -= 2) NBG_UP(k_up_pair, d - 1, d - 1 < 5 ? 64 : 1024);
+    if (d == 0) NBG_UP(k_up_internal, 0, 64);
+#undef NBG_UP
 }
 
-// gather is part of the build step in the driver (needs G)
 // Offsets of the sorted particles from their leaf's FP64 centre, rounded to FP32, with G*m:
 // the grouped walk of an FP64 state differences these (plus FP32 offsets between leaf
 // centres), so a pair difference never rounds an absolute position.
@@ -1325,11 +1471,9 @@ void launch_rel(nbg_ctx* c) {
                        c->spos, c->x64, c->y64, c->z64, c->n, c->srel);
 }
 
-void launch_gather(nbg_ctx* c, float G) {
-    hipLaunchKernelGGL(k_pack4, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->x, c->y, c->z, c->m, c->n, G,
+void launch_pack4(nbg_ctx* c) {
+    hipLaunchKernelGGL(k_pack4, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->x, c->y, c->z, c->m, c->n,
                        c->stage4);
-    hipLaunchKernelGGL(k_gather, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->stage4, c->perm, c->n,
-                       c->spos);
 }
 
 }  // namespace nbg

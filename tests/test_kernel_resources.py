@@ -1,5 +1,5 @@
 """CPU check of the compiled walk kernels' resources: every variant of the grouped and
-per-particle walks (and the learned inference) must run without a private segment
+per-particle walks (and the learned inference and the upsweep) must run without a private segment
 (scratch) and within the register (and, for the grouped walk, LDS) budget that keeps three
 4-wave blocks per CU.  A
 non-inlined helper or a spilled array shows up here as scratch bytes; the COUNT variants
@@ -40,6 +40,8 @@ def kernel_metadata(src, tmp_path):
     ("nbg_walk.hip", "_ZN3nbg11k_walk_bins", 168),
     ("nbg_walk_pp.hip", "_ZN3nbg9k_walk_pp", 168),
     ("nbg_learn.hip", "_ZN3nbg13k_learn_infer", 256),
+    ("nbg_tree.hip", "_ZN3nbg9k_up_pair", 256),
+    ("nbg_tree.hip", "_ZN3nbg8k_leaves", 256),
 ])
 def test_walk_kernels_use_no_scratch(src, prefix, max_vgpr, tmp_path):
     kernels = kernel_metadata(os.path.join(CSRC, src), tmp_path)
