@@ -458,19 +458,26 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t* __r
     hist[hist_index(b, threadIdx.x, blk_sys, sys_blk0)] = s_cnt[threadIdx.x];
 }
 
-// The block's keys are ranked stably by digit (wave ballots per round of 256 keys, per-wave
-// LDS digit counts, a running count per digit across rounds), placed in LDS in digit
-// order, then written out by consecutive threads to consecutive positions of each digit's
-// output run, so the global writes come in runs instead of one scattered word per key.
+// Each wave owns a contiguous slice of the block's keys (kSortItems rounds of 64), so the
+// block's input order is (wave, round, lane).  A wave ranks its keys stably by digit with
+// ballots and a running count per digit in its own LDS row, with no block barrier; the
+// per-wave counts then give each digit's offset per wave and the block-local digit offsets.
+// The keys are placed in LDS in digit order and written out by consecutive threads to
+// consecutive positions of each digit's output run, so the global writes come in runs
+// instead of one scattered word per key.
 template <bool EXTRA>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, const uint64_t* __restrict__ xin,
     uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t* __restrict__ xout, int shift,
     const int32_t* __restrict__ blk_sys, const int32_t* __restrict__ blk_start, const int32_t* __restrict__ blk_count,
     const int32_t* __restrict__ sys_blk0, const uint32_t* __restrict__ offs) {
-    __shared__ uint32_t s_run[kRadix];   // keys of each digit ranked so far, then the block-local digit offsets
-    __shared__ uint32_t s_gbase[kRadix]; // global output position of the digit's first key of this block
-    __shared__ uint32_t s_wcnt[kSortThreads / 64][kRadix];
+    constexpr int kWaves = kSortThreads / 64;
+    constexpr int kSlice = 64 * kSortItems;  // keys per wave
+    static_assert(kRadix == kSortThreads, "one digit per thread");
+    __shared__ uint32_t s_wrun[kWaves][kRadix];  // a wave's keys per digit so far, then its offset in the digit
+    __shared__ uint32_t s_run[kRadix];           // block-local digit offsets
+    __shared__ uint32_t s_gbase[kRadix];         // global output position of the digit's first key of this block
+    __shared__ uint32_t s_wsum[kWaves];
     __shared__ uint64_t s_key[kSortTile];
     __shared__ uint32_t s_val[kSortTile];
     __shared__ uint64_t s_xtra[EXTRA ? kSortTile : 1];
@@ -479,13 +486,12 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const int lane = lane_id();
     const uint64_t lt = (1ull << lane) - 1ull;
     s_gbase[threadIdx.x] = offs[hist_index(b, threadIdx.x, blk_sys, sys_blk0)];
-    s_run[threadIdx.x] = 0;
     const int start = blk_start[b], cnt = blk_count[b];
     uint64_t key[kSortItems], xtra[kSortItems];
     uint32_t val[kSortItems], loc[kSortItems];
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
-        const int k = r * kSortThreads + threadIdx.x;
+        const int k = wave * kSlice + r * 64 + lane;
         key[r] = 0;
         val[r] = 0;
         xtra[r] = 0;
@@ -495,10 +501,13 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
             if (EXTRA) xtra[r] = xin[start + k];
         }
     }
-    __syncthreads();
+    uint32_t* wrun = s_wrun[wave];
+#pragma unroll
+    for (int j = 0; j < kRadix / 64; ++j) wrun[j * 64 + lane] = 0;
+    wave_sync();
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
-        const int k = r * kSortThreads + threadIdx.x;
+        const int k = wave * kSlice + r * 64 + lane;
         const bool valid = k < cnt;
         const int d = static_cast<int>((key[r] >> shift) & (kRadix - 1));
         uint64_t peers = ballot(valid);
@@ -509,46 +518,42 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
             peers &= set ? m : ~m;
         }
         const int rank = __popcll(peers & lt);
-#pragma unroll
-        for (int w = 0; w < kSortThreads / 64; ++w) s_wcnt[w][threadIdx.x] = 0;
-        __syncthreads();
-        if (valid && rank == 0) s_wcnt[wave][d] = static_cast<uint32_t>(__popcll(peers));
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = s_run[d] + static_cast<uint32_t>(rank);
-            for (int w = 0; w < wave; ++w) pos += s_wcnt[w][d];
-            loc[r] = pos;  // rank among the block's keys of digit d
-        }
-        __syncthreads();
-        uint32_t add = 0;
-#pragma unroll
-        for (int w = 0; w < kSortThreads / 64; ++w) add += s_wcnt[w][threadIdx.x];
-        s_run[threadIdx.x] += add;
-        __syncthreads();
+        loc[r] = valid ? wrun[d] + static_cast<uint32_t>(rank) : 0u;  // rank among the wave's keys of digit d
+        wave_sync();
+        if (valid && rank == 0) wrun[d] += static_cast<uint32_t>(__popcll(peers));
+        wave_sync();
     }
-    // block-local digit offsets: exclusive scan of the digit counts (one digit per thread)
+    __syncthreads();
+    // digit t (one per thread): its per-wave counts become per-wave offsets inside the digit,
+    // and its total enters the exclusive scan over digits
     {
-        const uint32_t c = s_run[threadIdx.x];
+        const int t = threadIdx.x;
+        uint32_t c = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t cw = s_wrun[w][t];
+            s_wrun[w][t] = c;
+            c += cw;
+        }
         uint32_t x = c;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
             const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(x), o, 64));
             if (lane >= o) x += y;
         }
-        __shared__ uint32_t s_wsum[kSortThreads / 64];
         if (lane == 63) s_wsum[wave] = x;
         __syncthreads();
         uint32_t before = 0;
         for (int w = 0; w < wave; ++w) before += s_wsum[w];
-        s_run[threadIdx.x] = before + x - c;
+        s_run[t] = before + x - c;
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
-        const int k = r * kSortThreads + threadIdx.x;
+        const int k = wave * kSlice + r * 64 + lane;
         if (k < cnt) {
             const int d = static_cast<int>((key[r] >> shift) & (kRadix - 1));
-            const uint32_t p = s_run[d] + loc[r];
+            const uint32_t p = s_run[d] + s_wrun[wave][d] + loc[r];
             s_key[p] = key[r];
             s_val[p] = val[r];
             if (EXTRA) s_xtra[p] = xtra[r];
@@ -645,15 +650,6 @@ __device__ __forceinline__ int cell_count(const uint64_t* __restrict__ kh, const
     }
     return lo - i;
 }
-// The same for a leaf above the depth cap, which holds at most kNcrit particles.
-template <int LEVELS>
-__device__ __forceinline__ int leaf_count(const uint64_t* __restrict__ kh, const uint64_t* __restrict__ kl, int i,
-                                          int s1, int d) {
-    int count = 1;
-    for (int k = 1; k <= kNcrit && i + k < s1; ++k) count += (cpl_at<LEVELS>(kh, kl, i, i + k) >= d);
-    return count;
-}
-
 // Leaf depth and common prefix with the predecessor, per sorted position; the number of
 // nodes and leaves each position starts, packed for one 64-bit scan.
 template <int LEVELS>
@@ -668,12 +664,25 @@ __global__ __launch_bounds__(256) void k_leaf_depth(const uint64_t* __restrict__
     int D = 0;  // positions past n report depth 0 to the block maximum
     if (i < n) {
         const int s = pos_sys[i];
+        // the +-8 neighbours' keys are loaded (clamped to the array) while the system's
+        // bounds are still in flight, which are only needed to mask the prefixes
+        uint64_t hk[2 * kNcrit + 1], lk[LEVELS <= kLevelsShort ? 1 : 2 * kNcrit + 1];
+#pragma unroll
+        for (int j = 0; j <= 2 * kNcrit; ++j) {
+            const int q = min(max(i + j - kNcrit, 0), n - 1);
+            hk[j] = kh[q];
+            if (LEVELS > kLevelsShort) lk[j] = kl[q];
+        }
+        auto cpl = [&](int a, int b) __attribute__((always_inline)) {
+            if (LEVELS <= kLevelsShort) return min(cpl_short(hk[a], hk[b]), LEVELS);
+            return cpl_deep(hk[a], lk[a], hk[b], lk[b]);
+        };
         const int s0 = sys_off[s], s1 = sys_off[s + 1];
         int L[kNcrit], R[kNcrit];
 #pragma unroll
         for (int k = 1; k <= kNcrit; ++k) {
-            L[k - 1] = (i - k >= s0) ? cpl_at<LEVELS>(kh, kl, i - k, i) : -1;
-            R[k - 1] = (i + k < s1) ? cpl_at<LEVELS>(kh, kl, i, i + k) : -1;
+            L[k - 1] = (i - k >= s0) ? cpl(kNcrit - k, kNcrit) : -1;
+            R[k - 1] = (i + k < s1) ? cpl(kNcrit, kNcrit + k) : -1;
         }
         // cell size at depth d: 1 + #{L >= d} + #{R >= d}; the leaf depth is the first d with
         // size <= 8, i.e. one past v8, the 8th largest of the 16 prefix lengths.  L and R are
@@ -754,12 +763,73 @@ __global__ __launch_bounds__(256) void k_nodes(const uint64_t* __restrict__ kh, 
     const int cp = cpl_prev[i];
     if (D <= cp) return;
     const int s = pos_sys[i];
+    // the keys of positions i-8 .. i+8 (clamped to the array), all loads in flight together
+    // while the system bounds are read: they give the leaf's count and, for 47% of the
+    // chains of a 1M Plummer tree, the start of the parent cell, without a search (as
+    // dependent loads behind the bounds tests they were 8 serial memory round trips)
+    uint64_t hk[2 * kNcrit + 1], lk[LEVELS <= kLevelsShort ? 1 : 2 * kNcrit + 1];
+#pragma unroll
+    for (int j = 0; j <= 2 * kNcrit; ++j) {
+        const int q = min(max(i + j - kNcrit, 0), n - 1);
+        hk[j] = kh[q];
+        if (LEVELS > kLevelsShort) lk[j] = kl[q];
+    }
+    auto cpl = [&](int a, int b) __attribute__((always_inline)) {
+        if (LEVELS <= kLevelsShort) return min(cpl_short(hk[a], hk[b]), LEVELS);
+        return cpl_deep(hk[a], lk[a], hk[b], lk[b]);
+    };
     const int s0 = sys_off[s], s1 = sys_off[s + 1];
     const uint64_t sc_i = node_scan[i];
     const int64_t base = static_cast<int64_t>(sc_i >> 32);
     const int lbase = static_cast<int>(sc_i & 0xffffffffu);
-    const uint64_t kh_i = kh[i];
-    const uint64_t kl_i = (LEVELS <= kLevelsShort) ? 0u : kl[i];
+    const uint64_t kl_i = (LEVELS <= kLevelsShort) ? 0u : lk[LEVELS <= kLevelsShort ? 0 : kNcrit];
+    // particle count of the leaf (depth D): i and the following positions in its cell; a
+    // leaf at the deepest level of the keys may hold more (two-word keys: the depth cap)
+    int leaf_cnt = 1;
+    if (D < LEVELS) {
+#pragma unroll
+        for (int k = 1; k <= kNcrit; ++k) leaf_cnt += (i + k < s1 && cpl(kNcrit, kNcrit + k) >= D);
+    } else {
+        leaf_cnt = cell_count<LEVELS>(kh, kl, i, s1, D);
+    }
+    // parent of the chain's first node (depth cp+1): the node of depth cp that holds i,
+    // started by the first position of that cell, the first j in [s0, i] with cpl(j, i) >= cp
+    // (non-decreasing in j)
+    int parent0 = -1;
+    if (cp >= 0) {
+        int lo;
+        if (i - kNcrit >= s0 && cpl(0, kNcrit) >= cp) {
+            // further back than 8 (the particles of i's earlier siblings): gallop back from
+            // i-8, then bisect the last interval (a bisection over the whole system range
+            // took 20 dependent memory round trips)
+            int in = i - kNcrit, out = s0 - 1;  // cpl(in, i) >= cp; out is outside (or before s0)
+            for (int step = 2 * kNcrit;; step <<= 1) {
+                const int j = i - step;
+                if (j < s0) break;
+                if (cpl_at<LEVELS>(kh, kl, j, i) < cp) {
+                    out = j;
+                    break;
+                }
+                in = j;
+            }
+            lo = out + 1;
+            int hi = in;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (cpl_at<LEVELS>(kh, kl, mid, i) >= cp)
+                    hi = mid;
+                else
+                    lo = mid + 1;
+            }
+        } else {
+            // within 8: i-1 .. i-k inside the cell, by the non-increasing prefixes
+            int k = 0;
+#pragma unroll
+            for (int q = 1; q <= kNcrit; ++q) k += (i - q >= s0 && cpl(kNcrit - q, kNcrit) >= cp);
+            lo = i - k;
+        }
+        parent0 = static_cast<int>(node_scan[lo] >> 32) + cp - (cpl_prev[lo] + 1);
+    }
     for (int d = cp + 1; d <= D; ++d) {
         const int64_t id64 = base + (d - (cp + 1));
         if (id64 >= cap_nodes) {
@@ -768,33 +838,19 @@ __global__ __launch_bounds__(256) void k_nodes(const uint64_t* __restrict__ kh, 
             return;
         }
         const int id = static_cast<int>(id64);
-        // particle range of the leaf: first j > i outside the depth-D cell.  Internal
-        // nodes (d < D) get theirs in k_node_finish, one search per node thread: searched
-        // here, a position that starts a long chain of cells ran one search per level
-        // serially and set the kernel's time.
-        if (d == D) nd_count[id] = (D < LEVELS) ? leaf_count<LEVELS>(kh, kl, i, s1, d) : cell_count<LEVELS>(kh, kl, i, s1, d);
+        // internal nodes' particle counts are summed by the upsweep
+        if (d == D) nd_count[id] = leaf_cnt;
         int parent;
         if (d == 0) {
             parent = -1;
             sys_root[s] = id;
-        } else if (d > cp + 1) {
-            parent = id - 1;
         } else {
-            // start of the depth-(d-1) cell that holds i
-            int lo = s0, hi = i;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (cpl_at<LEVELS>(kh, kl, mid, i) >= d - 1)
-                    hi = mid;
-                else
-                    lo = mid + 1;
-            }
-            parent = static_cast<int>(node_scan[lo] >> 32) + (d - 1) - (cpl_prev[lo] + 1);
+            parent = (d > cp + 1) ? id - 1 : parent0;
         }
         nd_start[id] = i;
         nd_depth[id] = d;
         nd_parent[id] = parent;
-        nd_oct[id] = (d > 0) ? key_digit(kh_i, kl_i, d) : 0;
+        nd_oct[id] = (d > 0) ? key_digit(hk[kNcrit], kl_i, d) : 0;
 #pragma unroll
         for (int c = 0; c < 8; ++c) nd_child[8 * id + c] = -1;
         if (d == D) leaf_list[lbase] = id;
@@ -980,9 +1036,14 @@ __global__ __launch_bounds__(256) void k_leaves(const int32_t* __restrict__ leaf
         double wr[kNcrit];
         double3 pd[F64 ? kNcrit : 1];
         if (small) {
+            // loads from clamped positions, all in flight together (loads behind the
+            // k < cnt tests were issued one round trip after another)
             uint32_t v[kNcrit];
 #pragma unroll
-            for (int k = 0; k < kNcrit; ++k) v[k] = (k < cnt) ? perm[st + k] : 0xffffffffu;
+            for (int k = 0; k < kNcrit; ++k) {
+                const uint32_t pv = perm[st + min(k, cnt - 1)];
+                v[k] = (k < cnt) ? pv : 0xffffffffu;
+            }
             // insertion network over <= 8 values (absent ones sort last)
 #pragma unroll
             for (int a = 1; a < kNcrit; ++a) {
@@ -1146,6 +1207,13 @@ __device__ __forceinline__ void up_combine(const int (&ch)[8], double G, Rec rec
     double4 cd[8];
     float4 cc[8], ca[8], cq[8];
     int cn[8];
+    // the learned moments of the first four children are fetched with the records (they
+    // are used last; fetched after the centre of mass they added two memory round trips)
+    float4 cm[2][16];
+#pragma unroll
+    for (int c = 0; c < (MOM ? 4 : 0); ++c)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cm[0][4 * c + j] = mom(c, j);
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const UpRec r = rec(c);
@@ -1202,27 +1270,28 @@ __device__ __forceinline__ void up_combine(const int (&ch)[8], double G, Rec rec
             qyz = heavy ? qyz + (ca[c].x + w * 3.f * dy * dz) : qyz;
         }
     }
-    // the learned moments: the children's 16 words are fetched after the centre of mass,
-    // four children at a time (all eight at once held ~300 registers, one wave per SIMD);
-    // each moment still sums the children in order
+    // the learned moments, four children at a time (all eight held at once took ~300
+    // registers): the last four children's words are fetched before the first four's are
+    // used; each moment still sums the children in order
 #pragma unroll
     for (int i = 0; i < 16; ++i) mo[i] = 0.f;
 #pragma unroll
-    for (int c0 = 0; c0 < (MOM ? 8 : 0); c0 += 4) {
-        float4 cm[16];
+    for (int c = 0; c < (MOM ? 4 : 0); ++c)
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
+        for (int j = 0; j < 4; ++j) cm[1][4 * c + j] = mom(4 + c, j);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) cm[4 * c + j] = mom(c0 + c, j);
+    for (int h = 0; h < (MOM ? 2 : 0); ++h) {
+        const int c0 = 4 * h;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) pin(cm[4 * c]), pin(cm[4 * c + 1]), pin(cm[4 * c + 2]), pin(cm[4 * c + 3]);
+        for (int c = 0; c < 4; ++c) pin(cm[h][4 * c]), pin(cm[h][4 * c + 1]), pin(cm[h][4 * c + 2]), pin(cm[h][4 * c + 3]);
 #pragma unroll
         for (int cc4 = 0; cc4 < 4; ++cc4) {
             const int c = c0 + cc4;
             const bool heavy = massive && ch[c] >= 0 && cc[c].w > 0.f;
             const float dx = cc[c].x - fx, dy = cc[c].y - fy, dz = cc[c].z - fz;
             // parallel-axis shift of the child's moments to this com
-            const float4 q0 = cm[4 * cc4 + 0], q1 = cm[4 * cc4 + 1], q2 = cm[4 * cc4 + 2], q3 = cm[4 * cc4 + 3];
+            const float4 q0 = cm[h][4 * cc4 + 0], q1 = cm[h][4 * cc4 + 1], q2 = cm[h][4 * cc4 + 2],
+                         q3 = cm[h][4 * cc4 + 3];
             const float m = cc[c].w;
             const float sxx = q0.x, syy = q0.y, szz = q0.z, sxy = q0.w, sxz = q1.x, syz = q1.y;
             float t[16];
@@ -1361,25 +1430,25 @@ __global__ __launch_bounds__(256) void k_up_pair(int depth, const int32_t* __res
         if (c >= 0) {
             const int4 c0 = reinterpret_cast<const int4*>(nd_child)[2 * c];
             const int4 c1 = reinterpret_cast<const int4*>(nd_child)[2 * c + 1];
-            const int gch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-            const bool internal = max(max(max(c0.x, c0.y), max(c0.z, c0.w)), max(max(c1.x, c1.y), max(c1.z, c1.w))) >= 0;
-            UpRec r;
+            // the child's own record, loaded with its child list: final for a leaf, unused
+            // (stale) for an internal child; loaded in the leaf branch it cost a round trip
+            // after the other lanes' combines in a wave that has both
+            UpRec r = up_load<QUAD>(c, nd_count, nd_quad, nd_aux, nd_com64);
             float mo[16];
-            if (internal) {
-                up_node_mem<QUAD, MOM, F64>(c, gch, G, nd_count, nd_com, nd_quad, nd_aux, nd_com64, nd_mom, r, mo);
-            } else {
-                r = up_load<QUAD>(c, nd_count, nd_quad, nd_aux, nd_com64);
-                if (MOM) {
+            if (MOM) {
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const float4 v = nd_mom[4 * c + j];
-                        mo[4 * j + 0] = v.x;
-                        mo[4 * j + 1] = v.y;
-                        mo[4 * j + 2] = v.z;
-                        mo[4 * j + 3] = v.w;
-                    }
+                for (int j = 0; j < 4; ++j) {
+                    const float4 v = nd_mom[4 * c + j];
+                    mo[4 * j + 0] = v.x;
+                    mo[4 * j + 1] = v.y;
+                    mo[4 * j + 2] = v.z;
+                    mo[4 * j + 3] = v.w;
                 }
             }
+            const int gch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+            const bool internal = max(max(max(c0.x, c0.y), max(c0.z, c0.w)), max(max(c1.x, c1.y), max(c1.z, c1.w))) >= 0;
+            if (internal)
+                up_node_mem<QUAD, MOM, F64>(c, gch, G, nd_count, nd_com, nd_quad, nd_aux, nd_com64, nd_mom, r, mo);
             s_rec[t] = r;
             if (MOM) {
 #pragma unroll
