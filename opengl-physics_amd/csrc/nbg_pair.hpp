@@ -160,13 +160,15 @@ __device__ __forceinline__ void m2p_cell(float4 cm, float4 q, float qyz, float t
 
 // m2p_cell's field of one cell added into (sx,sy,sz) when `use`, nothing otherwise.  The
 // terms are grouped per component as (f + c2) d - ir5 Qd, so a lane spends two fused
-// multiply-adds per component and two selects per cell; a lane that does not use the cell
-// may meet non-finite terms (its position on the cell's com), which the selects drop.
+// multiply-adds per component.  A lane that does not use the cell adds +inf to its r^2:
+// both reciprocal roots are then +0 and every term an exact zero, also where its position
+// is the cell's com (r^2 = 0), so no select is needed (one instruction less per cell than
+// two selects of the results).
 template <bool QUAD, bool SPLINE>
 __device__ __forceinline__ void m2p_add(float4 cm, float4 q, float qyz, float tx, float ty, float tz,
                                         const Soft& soft, bool use, float& sx, float& sy, float& sz) {
     const float dx = cm.x - tx, dy = cm.y - ty, dz = cm.z - tz;
-    const float dist2 = dx * dx + dy * dy + dz * dz;
+    const float dist2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, use ? 0.f : __builtin_inff())));
     float g = pair_coef<SPLINE>(dist2, cm.w, soft);
     if (QUAD) {
         const float qxx = q.x, qyy = q.y, qxy = q.z, qxz = q.w;
@@ -174,22 +176,20 @@ __device__ __forceinline__ void m2p_add(float4 cm, float4 q, float qyz, float tx
         const float qdy = qxy * dx + qyy * dy + qyz * dz;
         // Qzz = -(Qxx + Qyy) (traceless), applied term by term: both Q words are
         // wave-uniform, and their sum would first need one of them moved to a vector register
+        // (a separate Qzz word per node cost more in scalar-cache misses than it saved)
         const float qdz = qxz * dx + qyz * dy - qxx * dz - qyy * dz;
         const float qd2 = dx * qdx + dy * qdy + dz * qdz;
         const float ir = __builtin_amdgcn_rsqf(dist2);
         const float ir2 = ir * ir;
         const float ir5 = ir2 * ir2 * ir;
         g = fmaf(ir5, (2.5f * qd2) * ir2, g);
-        g = use ? g : 0.f;
-        const float w = use ? -ir5 : 0.f;
         sx = fmaf(g, dx, sx);
         sy = fmaf(g, dy, sy);
         sz = fmaf(g, dz, sz);
-        sx = fmaf(w, qdx, sx);
-        sy = fmaf(w, qdy, sy);
-        sz = fmaf(w, qdz, sz);
+        sx = fmaf(-ir5, qdx, sx);
+        sy = fmaf(-ir5, qdy, sy);
+        sz = fmaf(-ir5, qdz, sz);
     } else {
-        g = use ? g : 0.f;
         sx = fmaf(g, dx, sx);
         sy = fmaf(g, dy, sy);
         sz = fmaf(g, dz, sz);

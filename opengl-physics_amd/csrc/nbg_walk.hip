@@ -589,15 +589,20 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
         // deliveries).  Unused slots of the last group read the zero pad after the sorted
         // particles.
         // FAST: every pair term is finite (Plummer with eps > 0, or the spline), so a lane
-        // weighs a source by its use (1 or 0) instead of selecting, and its own particle
-        // adds an exact zero (d = 0); otherwise unused sources and the target itself are
-        // dropped by a select.
+        // drops an unused source arithmetically instead of selecting, and its own particle
+        // adds an exact zero (d = 0): Plummer adds +inf instead of eps^2 to its r^2 (the
+        // reciprocal root of +inf is +0, so the term is +0, as a zero weight gave, one
+        // multiply less per source), the spline weighs the mass by its use (1 or 0);
+        // otherwise unused sources and the target itself are dropped by a select.
         auto eval_near_t = [&](int h, int n, auto fast) __attribute__((always_inline)) {
             constexpr bool FAST = decltype(fast)::value;
             float sx = 0.f, sy = 0.f, sz = 0.f;
             int e = 0, cur = 0, end = 0;  // the cursor: leaf e, its next particle cur, its end
             uint64_t em = 0;
-            float wuse = 0.f;
+            // a source's per-lane use: its r^2 addend (Plummer FAST) or its mass weight
+            constexpr bool ADDEND = FAST && !SPL;
+            const float wyes = ADDEND ? soft.eps2 : 1.f, wno = ADDEND ? __builtin_inff() : 0.f;
+            float wuse = wno;
             const __amdgpu_buffer_rsrc_t srec = rec_rsrc(A.spos, A.n_pad + 1);
             // the n <= 64 leaves, one per lane (two LDS reads for all of them): the cursor takes
             // each leaf's words with v_readlane instead of waiting on an LDS read per leaf.  Past
@@ -614,7 +619,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 cur = live ? st_r : A.n_pad;
                 end = live ? st_r + c_r : -1;
                 em = live ? em_r : 0ull;
-                wuse = (em & mybit) ? 1.f : 0.f;
+                wuse = (em & mybit) ? wyes : wno;
             };
             load_entry(0);
             int groups = 0;
@@ -637,7 +642,10 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                     const float4 q = nstage[u];
                     const float dx = q.x - tp.x, dy = q.y - tp.y, dz = q.z - tp.z;
                     float f;
-                    if (FAST) {
+                    if (ADDEND) {
+                        const float inv = __builtin_amdgcn_rsqf(fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, us[u]))));
+                        f = q.w * (inv * inv * inv);
+                    } else if (FAST) {
                         f = pair_coef_d<SPL>(dx, dy, dz, q.w * us[u], soft);
                     } else {
                         f = pair_coef<SPL>(dx * dx + dy * dy + dz * dz, q.w, soft);
