@@ -540,6 +540,7 @@ nbg_ctx* nbg_create(int device, int64_t max_n) {
     r |= alloc_into(c, c->nd_geom, 4 * C);
     r |= alloc_into(c, c->nd_com, C);
     r |= alloc_into(c, c->nd_quad, C);
+    r |= alloc_into(c, c->nd_m2p, 4 * C);
     r |= alloc_into(c, c->nd_aux, C);
     r |= alloc_into(c, c->nd_com64, C);
     r |= alloc_into(c, c->nd_rmax64, C);
@@ -593,7 +594,7 @@ void nbg_destroy(nbg_ctx* c) {
     free_dev(c->stage4);
     free_dev(c->nd_start); free_dev(c->nd_count); free_dev(c->nd_depth); free_dev(c->nd_parent);
     free_dev(c->nd_oct); free_dev(c->nd_child); free_dev(c->nd_geom);
-    free_dev(c->nd_com); free_dev(c->nd_quad); free_dev(c->nd_aux); free_dev(c->nd_com64); free_dev(c->nd_rmax64);
+    free_dev(c->nd_com); free_dev(c->nd_quad); free_dev(c->nd_m2p); free_dev(c->nd_aux); free_dev(c->nd_com64); free_dev(c->nd_rmax64);
     free_dev(c->depth_nodes); free_dev(c->dep_hist); free_dev(c->dep_off); free_dev(c->dep_part); free_dev(c->cnt_far); free_dev(c->cnt_near);
     free_dev(c->dsc); free_dev(c->diag_part);
     free_dev(c->aold); free_dev(c->rung); free_dev(c->tlist); free_dev(c->list_off); free_dev(c->red_part);

@@ -173,6 +173,7 @@ struct nbg_ctx {
     double* nd_geom = nullptr;       // 4 per node: centre x,y,z, halfSize (FP64)
     float4* nd_com = nullptr;        // com x,y,z, G*mass
     float4* nd_quad = nullptr;       // G*(Qxx, Qyy, Qxy, Qxz)
+    float4* nd_m2p = nullptr;        // 4 per node (64 bytes): com, quad, (Qyz, Qzz, 0, 0), unused
     float4* nd_aux = nullptr;        // G*Qyz, (2 halfSize)^2, maxRadius, info bits
     double4* nd_com64 = nullptr;     // FP64 com x,y,z and mass (without G): exact MAC decisions
     double* nd_rmax64 = nullptr;     // FP64 maxRadius of leaves (the group radius of the MAC)

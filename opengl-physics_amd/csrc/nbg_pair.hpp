@@ -164,20 +164,18 @@ __device__ __forceinline__ void m2p_cell(float4 cm, float4 q, float qyz, float t
 // both reciprocal roots are then +0 and every term an exact zero, also where its position
 // is the cell's com (r^2 = 0), so no select is needed (one instruction less per cell than
 // two selects of the results).
+// q2 = (Qyz, Qzz), Qzz = -(Qxx + Qyy) stored with the cell's record.
 template <bool QUAD, bool SPLINE>
-__device__ __forceinline__ void m2p_add(float4 cm, float4 q, float qyz, float tx, float ty, float tz,
+__device__ __forceinline__ void m2p_add(float4 cm, float4 q, float2 q2, float tx, float ty, float tz,
                                         const Soft& soft, bool use, float& sx, float& sy, float& sz) {
     const float dx = cm.x - tx, dy = cm.y - ty, dz = cm.z - tz;
     const float dist2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, use ? 0.f : __builtin_inff())));
     float g = pair_coef<SPLINE>(dist2, cm.w, soft);
     if (QUAD) {
-        const float qxx = q.x, qyy = q.y, qxy = q.z, qxz = q.w;
+        const float qxx = q.x, qyy = q.y, qxy = q.z, qxz = q.w, qyz = q2.x, qzz = q2.y;
         const float qdx = qxx * dx + qxy * dy + qxz * dz;
         const float qdy = qxy * dx + qyy * dy + qyz * dz;
-        // Qzz = -(Qxx + Qyy) (traceless), applied term by term: both Q words are
-        // wave-uniform, and their sum would first need one of them moved to a vector register
-        // (a separate Qzz word per node cost more in scalar-cache misses than it saved)
-        const float qdz = qxz * dx + qyz * dy - qxx * dz - qyy * dz;
+        const float qdz = qxz * dx + qyz * dy + qzz * dz;
         const float qd2 = dx * qdx + dy * qdy + dz * qdz;
         const float ir = __builtin_amdgcn_rsqf(dist2);
         const float ir2 = ir * ir;

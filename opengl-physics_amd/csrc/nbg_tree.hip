@@ -1005,6 +1005,15 @@ void launch_build(nbg_ctx* c, int levels) {
 // incremental update of Tree.cpp:120-122 produces: the walk never uses it as a source, and
 // a massless group then opens every cell, as in the reference.
 
+// The far field's record of a node, one 64-byte line: (com, G*M), G*(Qxx, Qyy, Qxy, Qxz),
+// G*(Qyz, Qzz = -(Qxx + Qyy)); read together, they cost the walk one scalar-cache line per
+// cell instead of one from each of three arrays.
+__device__ __forceinline__ void m2p_store(float4* __restrict__ nd_m2p, int id, float4 com, float4 quad, float qyz) {
+    nd_m2p[4 * id + 0] = com;
+    nd_m2p[4 * id + 1] = quad;
+    nd_m2p[4 * id + 2] = make_float4(qyz, -(quad.x + quad.y), 0.f, 0.f);
+}
+
 // Per leaf, one lane each: its members put in ascending original index (the reference
 // bucket order, so `order` lists them as Finalize does), their packed records gathered into
 // sorted order as (x, y, z, G*m), the sorted-position -> leaf map, and the leaf's moments.
@@ -1019,10 +1028,10 @@ __global__ __launch_bounds__(256) void k_leaves(const int32_t* __restrict__ leaf
                                                 double G, const DevScalars* __restrict__ sc,
                                                 float4* __restrict__ spos, int32_t* __restrict__ leaf_of,
                                                 float4* __restrict__ nd_com, float4* __restrict__ nd_quad,
-                                                float4* __restrict__ nd_aux, double4* __restrict__ nd_com64,
-                                                double* __restrict__ nd_rmax64, float4* __restrict__ nd_mom,
-                                                const double* __restrict__ x64, const double* __restrict__ y64,
-                                                const double* __restrict__ z64) {
+                                                float4* __restrict__ nd_aux, float4* __restrict__ nd_m2p,
+                                                double4* __restrict__ nd_com64, double* __restrict__ nd_rmax64,
+                                                float4* __restrict__ nd_mom, const double* __restrict__ x64,
+                                                const double* __restrict__ y64, const double* __restrict__ z64) {
     const int nl = sc->cap_exceeded ? 0 : sc->n_leaves;  // a truncated pool: nothing to do
     for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nl; li += gridDim.x * blockDim.x) {
         const int id = leaf_list[li];
@@ -1168,8 +1177,11 @@ __global__ __launch_bounds__(256) void k_leaves(const int32_t* __restrict__ leaf
         }
         nd_com64[id] = make_double4(gx, gy, gz, M);
         nd_rmax64[id] = rmax;
-        nd_com[id] = make_float4(fx, fy, fz, static_cast<float>(G * M));
-        nd_quad[id] = make_float4(qxx, qyy, qxy, qxz);
+        const float4 com = make_float4(fx, fy, fz, static_cast<float>(G * M));
+        const float4 quad = make_float4(qxx, qyy, qxy, qxz);
+        nd_com[id] = com;
+        nd_quad[id] = quad;
+        m2p_store(nd_m2p, id, com, quad, qyz);
         // the aux word's x (qyz) and z (radius); y (size^2) and w (info) stay as built
         float* aux = reinterpret_cast<float*>(nd_aux + id);
         aux[0] = qyz;
@@ -1329,7 +1341,8 @@ template <bool MOM>
 __device__ __forceinline__ void up_store(int id, const UpRec& r, const float (&mo)[16], double G,
                                          int32_t* __restrict__ nd_count, float4* __restrict__ nd_com,
                                          float4* __restrict__ nd_quad, float4* __restrict__ nd_aux,
-                                         double4* __restrict__ nd_com64, float4* __restrict__ nd_mom) {
+                                         float4* __restrict__ nd_m2p, double4* __restrict__ nd_com64,
+                                         float4* __restrict__ nd_mom) {
     if (MOM) {
         nd_mom[4 * id + 0] = make_float4(mo[0], mo[1], mo[2], mo[3]);
         nd_mom[4 * id + 1] = make_float4(mo[4], mo[5], mo[6], mo[7]);
@@ -1338,9 +1351,11 @@ __device__ __forceinline__ void up_store(int id, const UpRec& r, const float (&m
     }
     nd_count[id] = r.cn;
     nd_com64[id] = r.cd;
-    nd_com[id] = make_float4(static_cast<float>(r.cd.x), static_cast<float>(r.cd.y), static_cast<float>(r.cd.z),
-                             static_cast<float>(G * r.cd.w));
+    const float4 com = make_float4(static_cast<float>(r.cd.x), static_cast<float>(r.cd.y), static_cast<float>(r.cd.z),
+                                   static_cast<float>(G * r.cd.w));
+    nd_com[id] = com;
     nd_quad[id] = r.cq;
+    m2p_store(nd_m2p, id, com, r.cq, r.ca.x);
     float* aux = reinterpret_cast<float*>(nd_aux + id);
     aux[0] = r.ca.x;
     aux[2] = r.ca.z;
@@ -1363,8 +1378,9 @@ __device__ __forceinline__ UpRec up_load(int i, const int32_t* __restrict__ nd_c
 template <bool QUAD, bool MOM, bool F64>
 __device__ __forceinline__ void up_node_mem(int id, const int (&ch)[8], double G, int32_t* __restrict__ nd_count,
                                             float4* __restrict__ nd_com, float4* __restrict__ nd_quad,
-                                            float4* __restrict__ nd_aux, double4* __restrict__ nd_com64,
-                                            float4* __restrict__ nd_mom, UpRec& out, float (&mo)[16]) {
+                                            float4* __restrict__ nd_aux, float4* __restrict__ nd_m2p,
+                                            double4* __restrict__ nd_com64, float4* __restrict__ nd_mom, UpRec& out,
+                                            float (&mo)[16]) {
     int src[8];  // an absent child reads the node's own slot (discarded)
 #pragma unroll
     for (int c = 0; c < 8; ++c) src[c] = ch[c] >= 0 ? ch[c] : id;
@@ -1372,7 +1388,7 @@ __device__ __forceinline__ void up_node_mem(int id, const int (&ch)[8], double G
         ch, G,
         [&](int c) __attribute__((always_inline)) { return up_load<QUAD>(src[c], nd_count, nd_quad, nd_aux, nd_com64); },
         [&](int c, int j) __attribute__((always_inline)) { return nd_mom[4 * src[c] + j]; }, out, mo);
-    up_store<MOM>(id, out, mo, G, nd_count, nd_com, nd_quad, nd_aux, nd_com64, nd_mom);
+    up_store<MOM>(id, out, mo, G, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64, nd_mom);
 }
 
 // One depth: a lane per internal node of that depth.
@@ -1383,8 +1399,8 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
                                                      const uint32_t* __restrict__ dep_part,
                                                      const uint32_t* __restrict__ dep_off,
                                                      float4* __restrict__ nd_com, float4* __restrict__ nd_quad,
-                                                     float4* __restrict__ nd_aux, double4* __restrict__ nd_com64,
-                                                     float4* __restrict__ nd_mom) {
+                                                     float4* __restrict__ nd_aux, float4* __restrict__ nd_m2p,
+                                                     double4* __restrict__ nd_com64, float4* __restrict__ nd_mom) {
     const int cnt = static_cast<int>(dep_part[depth]);
     const int off = static_cast<int>(dep_off[depth * kNodeBlocks]);
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
@@ -1394,7 +1410,7 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
         const int ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
         UpRec r;
         float mo[16];
-        up_node_mem<QUAD, MOM, F64>(id, ch, G, nd_count, nd_com, nd_quad, nd_aux, nd_com64, nd_mom, r, mo);
+        up_node_mem<QUAD, MOM, F64>(id, ch, G, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64, nd_mom, r, mo);
     }
 }
 
@@ -1413,7 +1429,8 @@ __global__ __launch_bounds__(256) void k_up_pair(int depth, const int32_t* __res
                                                  const uint32_t* __restrict__ dep_part,
                                                  const uint32_t* __restrict__ dep_off, float4* __restrict__ nd_com,
                                                  float4* __restrict__ nd_quad, float4* __restrict__ nd_aux,
-                                                 double4* __restrict__ nd_com64, float4* __restrict__ nd_mom) {
+                                                 float4* __restrict__ nd_m2p, double4* __restrict__ nd_com64,
+                                                 float4* __restrict__ nd_mom) {
     __shared__ UpRec s_rec[256];
     __shared__ float4 s_mom[MOM ? 4 * 256 : 1];
     __shared__ int s_ch[256];
@@ -1448,7 +1465,7 @@ __global__ __launch_bounds__(256) void k_up_pair(int depth, const int32_t* __res
             const int gch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
             const bool internal = max(max(max(c0.x, c0.y), max(c0.z, c0.w)), max(max(c1.x, c1.y), max(c1.z, c1.w))) >= 0;
             if (internal)
-                up_node_mem<QUAD, MOM, F64>(c, gch, G, nd_count, nd_com, nd_quad, nd_aux, nd_com64, nd_mom, r, mo);
+                up_node_mem<QUAD, MOM, F64>(c, gch, G, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64, nd_mom, r, mo);
             s_rec[t] = r;
             if (MOM) {
 #pragma unroll
@@ -1465,7 +1482,7 @@ __global__ __launch_bounds__(256) void k_up_pair(int depth, const int32_t* __res
             up_combine<QUAD, MOM, F64>(
                 ch, G, [&](int j) __attribute__((always_inline)) { return s_rec[g0 + j]; },
                 [&](int j, int w) __attribute__((always_inline)) { return s_mom[4 * (g0 + j) + w]; }, r, mo);
-            up_store<MOM>(id, r, mo, G, nd_count, nd_com, nd_quad, nd_aux, nd_com64, nd_mom);
+            up_store<MOM>(id, r, mo, G, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64, nd_mom);
         }
         wave_sync();
     }
@@ -1480,7 +1497,7 @@ void launch_upsweep(nbg_ctx* c, int levels, int multipole, double G, bool moment
 #define NBG_LEAVES(Q, M, F)                                                                                          \
     hipLaunchKernelGGL((k_leaves<Q, M, F>), dim3(gnodes), dim3(256), 0, st, c->leaf_list, c->nd_start, c->nd_count,   \
                        c->stage4, c->perm, Gf, G, c->dsc, c->spos, c->leaf_of, c->nd_com, c->nd_quad, c->nd_aux,    \
-                       c->nd_com64, c->nd_rmax64, c->nd_mom, c->x64, c->y64, c->z64)
+                       c->nd_m2p, c->nd_com64, c->nd_rmax64, c->nd_mom, c->x64, c->y64, c->z64)
     if (mom)
         NBG_LEAVES(true, true, false);
     else if (quad && c->f64_state)
@@ -1499,19 +1516,19 @@ void launch_upsweep(nbg_ctx* c, int levels, int multipole, double G, bool moment
     do {                                                                                                             \
         if (mom)                                                                                                     \
             hipLaunchKernelGGL((K<true, true, false>), dim3(GD), dim3(256), 0, st, D, c->depth_nodes, c->nd_child,    \
-                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64, c->nd_mom);   \
+                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_m2p, c->nd_com64, c->nd_mom);   \
         else if (quad && c->f64_state)                                                                               \
             hipLaunchKernelGGL((K<true, false, true>), dim3(GD), dim3(256), 0, st, D, c->depth_nodes, c->nd_child,    \
-                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64, c->nd_mom);   \
+                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_m2p, c->nd_com64, c->nd_mom);   \
         else if (quad)                                                                                               \
             hipLaunchKernelGGL((K<true, false, false>), dim3(GD), dim3(256), 0, st, D, c->depth_nodes, c->nd_child,   \
-                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64, c->nd_mom);   \
+                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_m2p, c->nd_com64, c->nd_mom);   \
         else if (c->f64_state)                                                                                       \
             hipLaunchKernelGGL((K<false, false, true>), dim3(GD), dim3(256), 0, st, D, c->depth_nodes, c->nd_child,   \
-                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64, c->nd_mom);   \
+                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_m2p, c->nd_com64, c->nd_mom);   \
         else                                                                                                         \
             hipLaunchKernelGGL((K<false, false, false>), dim3(GD), dim3(256), 0, st, D, c->depth_nodes, c->nd_child,  \
-                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_com64, c->nd_mom);   \
+                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_m2p, c->nd_com64, c->nd_mom);   \
     } while (0)
     const int top = (levels <= kLevelsShort) ? levels - 1 : kMaxDepth - 1;
     int d = top;

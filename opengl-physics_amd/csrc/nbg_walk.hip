@@ -349,6 +349,7 @@ struct BinArgs {
     const int32_t* nd_child;
     const float4* nd_com;
     const float4* nd_quad;
+    const float4* nd_m2p;  // 4 per node: com, quad, (Qyz, Qzz), unused
     const float4* nd_aux;
     const double4* nd_com64;
     const double* nd_rmax64;
@@ -546,7 +547,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 int nd[4];
                 bool um[4];
                 float4 cm[4], q[4];
-                float qyz[4];
+                float2 q2[4];
                 const int4 n4 = *reinterpret_cast<const int4*>(fnode + hb + e0);
                 const ulonglong2 m01 = *reinterpret_cast<const ulonglong2*>(fmask + hb + e0);
                 const ulonglong2 m23 = *reinterpret_cast<const ulonglong2*>(fmask + hb + e0 + 2);
@@ -560,7 +561,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    cm[u] = cld(A.nd_com, nd[u]);
+                    cm[u] = cld(A.nd_m2p, 4 * nd[u]);
                     if (REL) {
                         // the centre of mass relative to cb: an FP64 difference, rounded once
                         const double4 c64 = cld(A.nd_com64, nd[u]);
@@ -568,12 +569,15 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                         cm[u].y = static_cast<float>(c64.y - rby);
                         cm[u].z = static_cast<float>(c64.z - rbz);
                     }
-                    q[u] = cld(A.nd_quad, nd[u]);
-                    qyz[u] = __uint_as_float(cld_u32(A.nd_aux, 4 * nd[u]));
+                    q[u] = cld(A.nd_m2p, 4 * nd[u] + 1);
+                    {
+                        const float4 w = cld(A.nd_m2p, 4 * nd[u] + 2);
+                        q2[u] = make_float2(w.x, w.y);
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    m2p_add<QUAD, SPL>(cm[u], q[u], qyz[u], tp.x, tp.y, tp.z, soft, um[u], sx, sy, sz);
+                    m2p_add<QUAD, SPL>(cm[u], q[u], q2[u], tp.x, tp.y, tp.z, soft, um[u], sx, sy, sz);
                     if (COUNT) n_far += um[u];
                 }
             }
@@ -1161,6 +1165,7 @@ void launch_walk_bins(nbg_ctx* c, const nbg_params& p) {
     A.nd_child = c->nd_child;
     A.nd_com = c->nd_com;
     A.nd_quad = c->nd_quad;
+    A.nd_m2p = c->nd_m2p;
     A.nd_aux = c->nd_aux;
     A.nd_com64 = c->nd_com64;
     A.nd_rmax64 = c->nd_rmax64;

@@ -48,6 +48,7 @@ struct PPArgs {
     const int32_t* nd_child;
     const float4* nd_com;
     const float4* nd_quad;
+    const float4* nd_m2p;  // 4 per node: com, quad, (Qyz, Qzz), unused
     const float4* nd_aux;
     const double4* nd_com64;
     const double* nd_geom;
@@ -257,7 +258,7 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
                 int nd[4];
                 bool um[4];
                 float4 cm[4], q[4];
-                float qyz[4];
+                float2 q2[4];
                 const int4 n4 = *reinterpret_cast<const int4*>(fnode + hb + e0);
                 const ulonglong2 m01 = *reinterpret_cast<const ulonglong2*>(fmask + hb + e0);
                 const ulonglong2 m23 = *reinterpret_cast<const ulonglong2*>(fmask + hb + e0 + 2);
@@ -271,13 +272,16 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    cm[u] = cld(A.nd_com, nd[u]);
-                    q[u] = cld(A.nd_quad, nd[u]);
-                    qyz[u] = __uint_as_float(cld_u32(A.nd_aux, 4 * nd[u]));
+                    cm[u] = cld(A.nd_m2p, 4 * nd[u]);
+                    q[u] = cld(A.nd_m2p, 4 * nd[u] + 1);
+                    {
+                        const float4 w = cld(A.nd_m2p, 4 * nd[u] + 2);
+                        q2[u] = make_float2(w.x, w.y);
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    m2p_add<QUAD, SPL>(cm[u], q[u], qyz[u], tp.x, tp.y, tp.z, soft, um[u], sx, sy, sz);
+                    m2p_add<QUAD, SPL>(cm[u], q[u], q2[u], tp.x, tp.y, tp.z, soft, um[u], sx, sy, sz);
                     if (COUNT) n_m2p += um[u];
                 }
             }
@@ -650,6 +654,7 @@ void launch_walk_pp(nbg_ctx* c, const nbg_params& p, const int32_t* tlist, const
     A.nd_child = c->nd_child;
     A.nd_com = c->nd_com;
     A.nd_quad = c->nd_quad;
+    A.nd_m2p = c->nd_m2p;
     A.nd_aux = c->nd_aux;
     A.nd_com64 = c->nd_com64;
     A.nd_geom = c->nd_geom;
