@@ -94,21 +94,24 @@ def stage_hbm(st, steps, n, moments=False):
 
 # the kernels of each tree / integration stage and their launches per step (short names
 # as profiles/traffic_plummer_1m.json keys them) for keys of L levels; the upsweep's
-# internal nodes sit at depths 0..L-1, two depths per launch and depth 0 alone when L is odd
+# internal nodes sit at depths 0..L-1: two depths per launch from the deepest down to depth
+# 3 (depth 3 alone when left over), then depths 0..2 in one launch
 def stage_kernels(levels, moments):
     P = sort_passes(levels)
     m = "true" if moments else "false"
+    top = (levels if levels <= 21 else 40) - 1
+    pairs, single = (top - 2) // 2, (top - 2) % 2
     return {
-        "keys": [("k_reset", 1), ("void k_bbox<float>", 1), ("void k_root<float>", 1),
-                 (f"void k_keys<{levels}, float>", 1)],
+        "keys": [("void k_bbox<float>", 1), ("void k_root<float>", 1), (f"void k_keys<{levels}, float>", 1)],
         "sort": [("k_radix_hist", P), ("void k_scan_reduce<unsigned int>", P),
                  ("void k_scan_down_inline<unsigned int>", P), ("void k_radix_scatter<false>", P)],
         "build": [(f"void k_leaf_depth<{levels}>", 1), ("void k_scan_reduce<unsigned long>", 1),
-                  ("void k_scan_down_inline<unsigned long>", 1), ("k_node_totals", 1), (f"void k_nodes<{levels}>", 1),
+                  ("void k_scan_down_inline<unsigned long>", 1), (f"void k_nodes<{levels}>", 1),
                   ("k_link", 1), (f"void k_node_finish<{levels}>", 1), ("void k_scan_reduce<unsigned int>", 1),
                   ("void k_scan_down_inline<unsigned int>", 1), ("k_depth_scatter", 1), ("k_pack4", 1)],
-        "upsweep": [(f"void k_leaves<true, {m}, false>", 1), (f"void k_up_pair<true, {m}, false>", levels // 2)]
-                   + ([(f"void k_up_internal<true, {m}, false>", 1)] if levels % 2 else []),
+        "upsweep": [(f"void k_leaves<true, {m}, false>", 1), (f"void k_up_pair<true, {m}, false>", pairs),
+                    (f"void k_up_top<true, {m}, false>", 1)]
+                   + ([(f"void k_up_internal<true, {m}, false>", 1)] if single else []),
         "integrate": [("k_kick_drift", 1), ("k_kick", 1)],
     }
 

@@ -170,8 +170,8 @@ void excl_scan(hipStream_t st, const T* in, T* out, int64_t n, T* part, T* total
 
 // ============================================================================ bbox + keys
 
-// Resets the per-call device scalars.
-__global__ void k_reset(DevScalars* __restrict__ sc) {
+// Resets the per-call device scalars (threads 0..63 of one block).
+__device__ __forceinline__ void reset_scalars(DevScalars* __restrict__ sc) {
     const int t = threadIdx.x;
     if (t < 64) {
         sc->depth_count[t] = 0;
@@ -290,14 +290,17 @@ __global__ __launch_bounds__(kSortThreads) void k_bbox(const T* __restrict__ x, 
 }
 
 // Root geometry per system, FP64, as Tree.cpp:54-88: one block per system reduces the
-// bounds of its sort blocks.
+// bounds of its sort blocks.  Block 0 also resets the evaluation's device scalars (nothing
+// before this kernel reads them; a launch of its own cost a few microseconds).
 template <typename T>
 __global__ __launch_bounds__(256) void k_root(const typename PosOrd<T>::U* __restrict__ blk_bbox,
                                               const int32_t* __restrict__ sys_blk0,
-                                              const int32_t* __restrict__ sys_off, double* __restrict__ geom) {
+                                              const int32_t* __restrict__ sys_off, double* __restrict__ geom,
+                                              DevScalars* __restrict__ sc) {
     typedef typename PosOrd<T>::U U;
     __shared__ U s_red[6][4];
     const int s = blockIdx.x;
+    if (s == 0) reset_scalars(sc);
     const int b0 = sys_blk0[s], b1 = sys_blk0[s + 1];
     U lo[3] = {PosOrd<T>::lo0, PosOrd<T>::lo0, PosOrd<T>::lo0}, hi[3] = {PosOrd<T>::hi0, PosOrd<T>::hi0, PosOrd<T>::hi0};
     for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
@@ -380,7 +383,6 @@ __global__ __launch_bounds__(256) void k_keys(const T* __restrict__ x, const T* 
 void launch_bbox_and_keys(nbg_ctx* c, int levels) {
     hipStream_t st = c->stream;
     const bool f64 = c->f64_state;
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, st, c->dsc);
     // the root box and the keys from the positions of the state: FP32, or FP64 (Tree.cpp's
     // own precision) for an FP64 state
     if (f64) {
@@ -388,7 +390,7 @@ void launch_bbox_and_keys(nbg_ctx* c, int levels) {
         hipLaunchKernelGGL(k_bbox<double>, dim3(c->n_blocks), dim3(kSortThreads), 0, st, c->x64, c->y64, c->z64,
                            c->blk_start, c->blk_count, bb);
         hipLaunchKernelGGL(k_root<double>, dim3(c->nsys), dim3(256), 0, st, bb, c->sys_blk0, c->sys_off,
-                           c->sys_geom);
+                           c->sys_geom, c->dsc);
         if (levels == kLevelsTiny)
             hipLaunchKernelGGL((k_keys<kLevelsTiny, double>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x64,
                                c->y64, c->z64, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
@@ -406,7 +408,7 @@ void launch_bbox_and_keys(nbg_ctx* c, int levels) {
     hipLaunchKernelGGL(k_bbox<float>, dim3(c->n_blocks), dim3(kSortThreads), 0, st, c->x, c->y, c->z,
                        c->blk_start, c->blk_count, c->blk_bbox);
     hipLaunchKernelGGL(k_root<float>, dim3(c->nsys), dim3(256), 0, st, c->blk_bbox, c->sys_blk0, c->sys_off,
-                       c->sys_geom);
+                       c->sys_geom, c->dsc);
     if (levels == kLevelsTiny)
         hipLaunchKernelGGL((k_keys<kLevelsTiny, float>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x, c->y,
                            c->z, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
@@ -718,30 +720,38 @@ __global__ __launch_bounds__(256) void k_leaf_depth(const uint64_t* __restrict__
     if (threadIdx.x == 0) blk_depth[blockIdx.x] = max(max(s_dmax[0], s_dmax[1]), max(s_dmax[2], s_dmax[3]));
 }
 
-// Grand totals of the node scan -> device scalars.
-__global__ __launch_bounds__(256) void k_node_totals(const uint64_t* __restrict__ total, DevScalars* __restrict__ sc,
-                                                     int64_t cap_nodes, const int32_t* __restrict__ blk_depth,
-                                                     int nblk) {
+// Grand totals of the node scan and the deepest leaf -> device scalars (all threads of one
+// block).
+__device__ __forceinline__ void node_totals(const uint64_t* __restrict__ total, DevScalars* __restrict__ sc,
+                                            int64_t cap_nodes, const int32_t* __restrict__ blk_depth, int nblk) {
     __shared__ int s_m[256];
     int m = 0;
-    for (int b = threadIdx.x; b < nblk; b += 256) m = max(m, blk_depth[b]);
+    // eight loads in flight per thread and round (clamped indices repeat the last block)
+    for (int b0 = 0; b0 < nblk; b0 += 8 * 256) {
+        int v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = blk_depth[min(b0 + u * 256 + static_cast<int>(threadIdx.x), nblk - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) m = max(m, v[u]);
+    }
     s_m[threadIdx.x] = m;
     __syncthreads();
     for (int o = 128; o >= 1; o >>= 1) {
         if (threadIdx.x < o) s_m[threadIdx.x] = max(s_m[threadIdx.x], s_m[threadIdx.x + o]);
         __syncthreads();
     }
-    if (threadIdx.x != 0) return;
-    sc->max_depth = s_m[0];
-    const uint64_t t = *total;
-    const int64_t nodes = static_cast<int64_t>(t >> 32);
-    sc->n_leaves = static_cast<int32_t>(t & 0xffffffffu);
-    if (nodes > cap_nodes) {
-        sc->cap_exceeded = 1;
-        sc->sticky_cap = 1;
-        sc->n_nodes = static_cast<int32_t>(cap_nodes);
-    } else {
-        sc->n_nodes = static_cast<int32_t>(nodes);
+    if (threadIdx.x == 0) {
+        sc->max_depth = s_m[0];
+        const uint64_t t = *total;
+        const int64_t nodes = static_cast<int64_t>(t >> 32);
+        sc->n_leaves = static_cast<int32_t>(t & 0xffffffffu);
+        if (nodes > cap_nodes) {
+            sc->cap_exceeded = 1;
+            sc->sticky_cap = 1;
+            sc->n_nodes = static_cast<int32_t>(cap_nodes);
+        } else {
+            sc->n_nodes = static_cast<int32_t>(nodes);
+        }
     }
 }
 
@@ -756,7 +766,11 @@ __global__ __launch_bounds__(256) void k_nodes(const uint64_t* __restrict__ kh, 
                                                int32_t* __restrict__ nd_depth, int32_t* __restrict__ nd_parent,
                                                int32_t* __restrict__ nd_oct, int32_t* __restrict__ nd_child,
                                                int32_t* __restrict__ leaf_list, int32_t* __restrict__ sys_root,
-                                               DevScalars* __restrict__ sc) {
+                                               DevScalars* __restrict__ sc, const int32_t* __restrict__ blk_depth,
+                                               int nblk) {
+    // block 0 also publishes the totals (a launch of its own cost a few microseconds; no
+    // block of this kernel reads them)
+    if (blockIdx.x == 0) node_totals(node_scan + n, sc, cap_nodes, blk_depth, nblk);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int D = leaf_depth[i];
@@ -970,11 +984,10 @@ void launch_build_t(nbg_ctx* c) {
                        c->sys_off, n, c->leaf_depth, c->cpl_prev, c->node_scan, c->dsc, c->blk_depth);
     // node_scan[n] receives the grand total (the array holds n+1 entries)
     excl_scan<uint64_t>(st, c->node_scan, c->node_scan, n, c->scan_part, c->node_scan + n);
-    hipLaunchKernelGGL(k_node_totals, dim3(1), dim3(256), 0, st, c->node_scan + n, c->dsc, c->cap_nodes, c->blk_depth,
-                       g);
     hipLaunchKernelGGL(k_nodes<LEVELS>, dim3(g), dim3(256), 0, st, c->key_sorted, c->klo_sorted, c->pos_sys,
                        c->sys_off, c->leaf_depth, c->cpl_prev, c->node_scan, n, c->cap_nodes, c->nd_start, c->nd_count,
-                       c->nd_depth, c->nd_parent, c->nd_oct, c->nd_child, c->leaf_list, c->sys_root, c->dsc);
+                       c->nd_depth, c->nd_parent, c->nd_oct, c->nd_child, c->leaf_list, c->sys_root, c->dsc,
+                       c->blk_depth, g);
     hipLaunchKernelGGL(k_link, dim3(gnodes), dim3(256), 0, st, c->nd_parent, c->nd_oct, c->dsc, c->nd_child);
     hipLaunchKernelGGL(k_node_finish<LEVELS>, dim3(gnodes), dim3(256), 0, st, c->key_sorted, c->klo_sorted,
                        c->pos_sys, c->sys_off, c->sys_geom, c->nd_start, c->nd_count, c->nd_depth, c->nd_child,
@@ -1414,6 +1427,65 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
     }
 }
 
+// A node's record into LDS slot t for its parent's combine: its own record, loaded with its
+// child list (final for a leaf, unused for an internal node; loaded in a leaf branch it cost a
+// round trip after the other lanes' combines in a wave that has both), or, for an internal
+// node, combined from its children's records in memory (and written).
+template <bool QUAD, bool MOM, bool F64>
+__device__ __forceinline__ void up_child_to_lds(int c, int t, double G, const int32_t* __restrict__ nd_child,
+                                                int32_t* __restrict__ nd_count, float4* __restrict__ nd_com,
+                                                float4* __restrict__ nd_quad, float4* __restrict__ nd_aux,
+                                                float4* __restrict__ nd_m2p, double4* __restrict__ nd_com64,
+                                                float4* __restrict__ nd_mom, UpRec* s_rec, float4* s_mom) {
+    const int4 c0 = reinterpret_cast<const int4*>(nd_child)[2 * c];
+    const int4 c1 = reinterpret_cast<const int4*>(nd_child)[2 * c + 1];
+    UpRec r = up_load<QUAD>(c, nd_count, nd_quad, nd_aux, nd_com64);
+    float mo[16];
+    if (MOM) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float4 v = nd_mom[4 * c + j];
+            mo[4 * j + 0] = v.x;
+            mo[4 * j + 1] = v.y;
+            mo[4 * j + 2] = v.z;
+            mo[4 * j + 3] = v.w;
+        }
+    }
+    const int gch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+    const bool internal = max(max(max(c0.x, c0.y), max(c0.z, c0.w)), max(max(c1.x, c1.y), max(c1.z, c1.w))) >= 0;
+    if (internal)
+        up_node_mem<QUAD, MOM, F64>(c, gch, G, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64, nd_mom, r, mo);
+    s_rec[t] = r;
+    if (MOM) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) s_mom[4 * t + j] = make_float4(mo[4 * j], mo[4 * j + 1], mo[4 * j + 2], mo[4 * j + 3]);
+    }
+}
+
+// Combines node `id` from the eight LDS slots g0..g0+7 (its children, ch[j] < 0: absent) and
+// writes it (and, if t >= 0, its record into LDS slot t).
+template <bool QUAD, bool MOM, bool F64>
+__device__ __forceinline__ void up_node_lds(int id, const int (&ch)[8], int g0, int t, double G,
+                                            int32_t* __restrict__ nd_count, float4* __restrict__ nd_com,
+                                            float4* __restrict__ nd_quad, float4* __restrict__ nd_aux,
+                                            float4* __restrict__ nd_m2p, double4* __restrict__ nd_com64,
+                                            float4* __restrict__ nd_mom, UpRec* s_rec, float4* s_mom) {
+    UpRec r;
+    float mo[16];
+    up_combine<QUAD, MOM, F64>(
+        ch, G, [&](int j) __attribute__((always_inline)) { return s_rec[g0 + j]; },
+        [&](int j, int w) __attribute__((always_inline)) { return s_mom[4 * (g0 + j) + w]; }, r, mo);
+    up_store<MOM>(id, r, mo, G, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64, nd_mom);
+    if (t >= 0) {
+        s_rec[t] = r;
+        if (MOM) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                s_mom[4 * t + j] = make_float4(mo[4 * j], mo[4 * j + 1], mo[4 * j + 2], mo[4 * j + 3]);
+        }
+    }
+}
+
 // Two depths per launch, which halves the launches and the memory round trips per depth
 // (index list, child lists) that bound this small, latency-bound work: 8 lanes per internal
 // node of `depth`, lane a on its child a (at depth + 1).  An internal child is combined from
@@ -1444,44 +1516,84 @@ __global__ __launch_bounds__(256) void k_up_pair(int depth, const int32_t* __res
         const int id = (k < cnt) ? depth_nodes[off + k] : -1;
         const int c = (id >= 0) ? nd_child[8 * id + a] : -1;
         s_ch[t] = c;
-        if (c >= 0) {
-            const int4 c0 = reinterpret_cast<const int4*>(nd_child)[2 * c];
-            const int4 c1 = reinterpret_cast<const int4*>(nd_child)[2 * c + 1];
-            // the child's own record, loaded with its child list: final for a leaf, unused
-            // (stale) for an internal child; loaded in the leaf branch it cost a round trip
-            // after the other lanes' combines in a wave that has both
-            UpRec r = up_load<QUAD>(c, nd_count, nd_quad, nd_aux, nd_com64);
-            float mo[16];
-            if (MOM) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const float4 v = nd_mom[4 * c + j];
-                    mo[4 * j + 0] = v.x;
-                    mo[4 * j + 1] = v.y;
-                    mo[4 * j + 2] = v.z;
-                    mo[4 * j + 3] = v.w;
-                }
-            }
-            const int gch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-            const bool internal = max(max(max(c0.x, c0.y), max(c0.z, c0.w)), max(max(c1.x, c1.y), max(c1.z, c1.w))) >= 0;
-            if (internal)
-                up_node_mem<QUAD, MOM, F64>(c, gch, G, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64, nd_mom, r, mo);
-            s_rec[t] = r;
-            if (MOM) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) s_mom[4 * t + j] = make_float4(mo[4 * j], mo[4 * j + 1], mo[4 * j + 2], mo[4 * j + 3]);
-            }
-        }
+        if (c >= 0)
+            up_child_to_lds<QUAD, MOM, F64>(c, t, G, nd_child, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64,
+                                            nd_mom, s_rec, s_mom);
         wave_sync();
         if (a == 0 && id >= 0) {
             int ch[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) ch[j] = s_ch[g0 + j];
+            up_node_lds<QUAD, MOM, F64>(id, ch, g0, -1, G, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64, nd_mom,
+                                        s_rec, s_mom);
+        }
+        wave_sync();
+    }
+}
+
+// The top three depths (0, 1, 2) in one launch: one wave per internal node of depth 0 (a
+// system root), lane 8a + b on grandchild b of child a (depth 2; an internal one is combined
+// from its children, done by the previous launch); lane 8a then combines child a from its
+// eight grandchildren's LDS records (or reads it, a leaf), and lane 0 the root from its
+// children's.  `depth` is 0.
+template <bool QUAD, bool MOM, bool F64>
+__global__ __launch_bounds__(256) void k_up_top(int depth, const int32_t* __restrict__ depth_nodes,
+                                                const int32_t* __restrict__ nd_child,
+                                                int32_t* __restrict__ nd_count, double G,
+                                                const uint32_t* __restrict__ dep_part,
+                                                const uint32_t* __restrict__ dep_off, float4* __restrict__ nd_com,
+                                                float4* __restrict__ nd_quad, float4* __restrict__ nd_aux,
+                                                float4* __restrict__ nd_m2p, double4* __restrict__ nd_com64,
+                                                float4* __restrict__ nd_mom) {
+    __shared__ UpRec s_rec[256];
+    __shared__ float4 s_mom[MOM ? 4 * 256 : 1];
+    __shared__ int s_ch[256];
+    const int cnt = static_cast<int>(dep_part[depth]);
+    const int off = static_cast<int>(dep_off[depth * kNodeBlocks]);
+    const int t = threadIdx.x;
+    const int lane = lane_id(), w0 = t - lane;  // the wave's first slot
+    const int a = lane >> 3, b = lane & 7;
+    // block-uniform loop: every lane of a wave reaches each wave_sync
+    for (int base = blockIdx.x * (256 / 64); base < cnt; base += gridDim.x * (256 / 64)) {
+        const int k = base + (t >> 6);
+        const int id = (k < cnt) ? depth_nodes[off + k] : -1;
+        const int ca = (id >= 0) ? nd_child[8 * id + a] : -1;
+        const int g = (ca >= 0) ? nd_child[8 * ca + b] : -1;
+        s_ch[t] = g;
+        if (g >= 0)
+            up_child_to_lds<QUAD, MOM, F64>(g, t, G, nd_child, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64,
+                                            nd_mom, s_rec, s_mom);
+        wave_sync();
+        int ch[8];
+        if (b == 0 && ca >= 0) {
+            bool internal = false;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                ch[j] = s_ch[t + j];
+                internal |= ch[j] >= 0;
+            }
+            if (internal) {
+                up_node_lds<QUAD, MOM, F64>(ca, ch, t, t, G, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64,
+                                            nd_mom, s_rec, s_mom);
+            } else {
+                s_rec[t] = up_load<QUAD>(ca, nd_count, nd_quad, nd_aux, nd_com64);
+                if (MOM) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) s_mom[4 * t + j] = nd_mom[4 * ca + j];
+                }
+            }
+        }
+        wave_sync();
+        if (b == 0) s_ch[t] = ca;  // (every lane read its group's slots before the sync)
+        wave_sync();
+        if (lane == 0 && id >= 0) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) ch[j] = s_ch[w0 + 8 * j];
             UpRec r;
             float mo[16];
             up_combine<QUAD, MOM, F64>(
-                ch, G, [&](int j) __attribute__((always_inline)) { return s_rec[g0 + j]; },
-                [&](int j, int w) __attribute__((always_inline)) { return s_mom[4 * (g0 + j) + w]; }, r, mo);
+                ch, G, [&](int j) __attribute__((always_inline)) { return s_rec[w0 + 8 * j]; },
+                [&](int j, int wd) __attribute__((always_inline)) { return s_mom[4 * (w0 + 8 * j) + wd]; }, r, mo);
             up_store<MOM>(id, r, mo, G, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64, nd_mom);
         }
         wave_sync();
@@ -1510,8 +1622,7 @@ void launch_upsweep(nbg_ctx* c, int levels, int multipole, double G, bool moment
         NBG_LEAVES(false, false, false);
 #undef NBG_LEAVES
     // internal nodes sit at depths 0..top (depth-40 cells are always leaves): two depths
-    // per launch from the deepest, (top-1, top), (top-3, top-2), ..., then depth 0 alone
-    // when it is left over
+    // per launch from the deepest, (top-1, top), (top-3, top-2), ..., and the top three in one
 #define NBG_UP(K, D, GD)                                                                                             \
     do {                                                                                                             \
         if (mom)                                                                                                     \
@@ -1532,8 +1643,16 @@ void launch_upsweep(nbg_ctx* c, int levels, int multipole, double G, bool moment
     } while (0)
     const int top = (levels <= kLevelsShort) ? levels - 1 : kMaxDepth - 1;
     int d = top;
-    for (; d >= 1; d -= 2) NBG_UP(k_up_pair, d - 1, d - 1 < 5 ? 64 : 1024);
-    if (d == 0) NBG_UP(k_up_internal, 0, 64);
+    if (top >= 2) {
+        // pairs from the deepest down to depth 3, depth 3 alone if left over, then the top
+        // three depths in one launch
+        for (; d >= 4; d -= 2) NBG_UP(k_up_pair, d - 1, d - 1 < 5 ? 64 : 1024);
+        if (d == 3) NBG_UP(k_up_internal, 3, 64);
+        NBG_UP(k_up_top, 0, 64);
+    } else {
+        for (; d >= 1; d -= 2) NBG_UP(k_up_pair, d - 1, 64);
+        if (d == 0) NBG_UP(k_up_internal, 0, 64);
+    }
 #undef NBG_UP
 }
 
