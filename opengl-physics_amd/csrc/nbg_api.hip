@@ -33,6 +33,13 @@ int alloc_into(nbg_ctx* c, T*& p, size_t count) {
     return NBG_OK;
 }
 
+// One slot (0: com, 1: quad, 3: aux) of the first n node records, to host memory.
+hipError_t copy_rec_slot(const nbg_ctx* c, float4* dst, int slot, int64_t n) {
+    if (n <= 0) return hipSuccess;
+    return hipMemcpy2D(dst, sizeof(float4), c->nd_rec + slot, 4 * sizeof(float4), sizeof(float4),
+                       static_cast<size_t>(n), hipMemcpyDeviceToHost);
+}
+
 template <typename T>
 void free_dev(T*& p) {
     if (p) (void)hipFree(p);
@@ -278,7 +285,7 @@ int accel_eager(nbg_ctx* c, const nbg_params& p, nbg_stats* stats, bool list_onl
             (void)hipMemcpy(f.data(), c->cnt_far, nn * sizeof(int32_t), hipMemcpyDeviceToHost);
             (void)hipMemcpy(q.data(), c->cnt_near, nn * sizeof(int32_t), hipMemcpyDeviceToHost);
             (void)hipMemcpy(cnt.data(), c->nd_count, nn * sizeof(int32_t), hipMemcpyDeviceToHost);
-            (void)hipMemcpy(aux.data(), c->nd_aux, nn * sizeof(float4), hipMemcpyDeviceToHost);
+            (void)copy_rec_slot(c, aux.data(), 3, nn);
             int64_t m2p = 0, p2p = 0;
             for (int i = 0; i < nn; ++i) {
                 uint32_t info;
@@ -538,10 +545,7 @@ nbg_ctx* nbg_create(int device, int64_t max_n) {
     r |= alloc_into(c, c->nd_oct, C);
     r |= alloc_into(c, c->nd_child, 8 * C);
     r |= alloc_into(c, c->nd_geom, 4 * C);
-    r |= alloc_into(c, c->nd_com, C);
-    r |= alloc_into(c, c->nd_quad, C);
-    r |= alloc_into(c, c->nd_m2p, 4 * C);
-    r |= alloc_into(c, c->nd_aux, C);
+    r |= alloc_into(c, c->nd_rec, 4 * C);
     r |= alloc_into(c, c->nd_com64, C);
     r |= alloc_into(c, c->nd_rmax64, C);
     r |= alloc_into(c, c->depth_nodes, C);
@@ -594,7 +598,7 @@ void nbg_destroy(nbg_ctx* c) {
     free_dev(c->stage4);
     free_dev(c->nd_start); free_dev(c->nd_count); free_dev(c->nd_depth); free_dev(c->nd_parent);
     free_dev(c->nd_oct); free_dev(c->nd_child); free_dev(c->nd_geom);
-    free_dev(c->nd_com); free_dev(c->nd_quad); free_dev(c->nd_m2p); free_dev(c->nd_aux); free_dev(c->nd_com64); free_dev(c->nd_rmax64);
+    free_dev(c->nd_rec); free_dev(c->nd_com64); free_dev(c->nd_rmax64);
     free_dev(c->depth_nodes); free_dev(c->dep_hist); free_dev(c->dep_off); free_dev(c->dep_part); free_dev(c->cnt_far); free_dev(c->cnt_near);
     free_dev(c->dsc); free_dev(c->diag_part);
     free_dev(c->aold); free_dev(c->rung); free_dev(c->tlist); free_dev(c->list_off); free_dev(c->red_part);
@@ -946,7 +950,7 @@ int nbg_tree_export(nbg_ctx* c, int64_t* n_nodes, uint64_t* keys, int32_t* order
     if (center) NBG_HIP_CHECK(c, hipMemcpy(center, c->nd_geom, un * 4 * sizeof(double), hipMemcpyDeviceToHost));
     if (children) NBG_HIP_CHECK(c, hipMemcpy(children, c->nd_child, un * 8 * sizeof(int32_t), hipMemcpyDeviceToHost));
     std::vector<float4> aux(un);
-    NBG_HIP_CHECK(c, hipMemcpy(aux.data(), c->nd_aux, un * sizeof(float4), hipMemcpyDeviceToHost));
+    NBG_HIP_CHECK(c, copy_rec_slot(c, aux.data(), 3, nn));
     if (topo) {
         std::vector<int32_t> st(un), cnt(un), dep(un), par(un);
         NBG_HIP_CHECK(c, hipMemcpy(st.data(), c->nd_start, un * 4, hipMemcpyDeviceToHost));
@@ -968,8 +972,8 @@ int nbg_tree_export(nbg_ctx* c, int64_t* n_nodes, uint64_t* keys, int32_t* order
         std::vector<float4> com(un), quad(un);
         std::vector<double4> com64(un);
         NBG_HIP_CHECK(c, hipMemcpy(com64.data(), c->nd_com64, un * sizeof(double4), hipMemcpyDeviceToHost));
-        NBG_HIP_CHECK(c, hipMemcpy(com.data(), c->nd_com, un * sizeof(float4), hipMemcpyDeviceToHost));
-        NBG_HIP_CHECK(c, hipMemcpy(quad.data(), c->nd_quad, un * sizeof(float4), hipMemcpyDeviceToHost));
+        NBG_HIP_CHECK(c, copy_rec_slot(c, com.data(), 0, nn));
+        NBG_HIP_CHECK(c, copy_rec_slot(c, quad.data(), 1, nn));
         // device masses and quadrupoles carry G; report them without it
         const float ig = (c->last_G != 0.0) ? static_cast<float>(1.0 / c->last_G) : 1.0f;
         for (size_t i = 0; i < un; ++i) {

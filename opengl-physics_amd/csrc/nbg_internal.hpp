@@ -171,10 +171,8 @@ struct nbg_ctx {
     int32_t* nd_oct = nullptr;
     int32_t* nd_child = nullptr;     // 8 per node
     double* nd_geom = nullptr;       // 4 per node: centre x,y,z, halfSize (FP64)
-    float4* nd_com = nullptr;        // com x,y,z, G*mass
-    float4* nd_quad = nullptr;       // G*(Qxx, Qyy, Qxy, Qxz)
-    float4* nd_m2p = nullptr;        // 4 per node (64 bytes): com, quad, (Qyz, Qzz, 0, 0), unused
-    float4* nd_aux = nullptr;        // G*Qyz, (2 halfSize)^2, maxRadius, info bits
+    float4* nd_rec = nullptr;        // 4 per node (one 64-byte line): (com, G*M), G*(Qxx, Qyy, Qxy, Qxz),
+                                     // G*(Qyz, Qzz, first sorted particle, -), aux (G*Qyz, size^2, radius, info)
     double4* nd_com64 = nullptr;     // FP64 com x,y,z and mass (without G): exact MAC decisions
     double* nd_rmax64 = nullptr;     // FP64 maxRadius of leaves (the group radius of the MAC)
     int32_t* leaf_list = nullptr;    // leaf node ids, in pre-order

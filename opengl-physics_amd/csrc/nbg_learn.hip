@@ -459,8 +459,7 @@ struct InferArgs {
     const DevScalars* sc;
     DevScalars* scw;
     uint32_t cap;
-    const float4* nd_com;
-    const float4* nd_aux;
+    const float4* nd_rec;  // 4 per node: com, quad, (Qyz, Qzz, first particle), aux
     const float4* nd_mom;
     float eps;
     const uint16_t* w16;
@@ -515,10 +514,10 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_learn_infer(InferArgs A) {
     };
     auto load_raw = [&](int2 pr, PairRaw& r) {
         if (pr.x >= 0) {
-            r.lc = A.nd_com[pr.x];
-            r.lr = A.nd_aux[pr.x].z;
-            r.cm = A.nd_com[pr.y];
-            r.size2 = A.nd_aux[pr.y].y;
+            r.lc = A.nd_rec[4 * pr.x];
+            r.lr = A.nd_rec[4 * pr.x + 3].z;
+            r.cm = A.nd_rec[4 * pr.y];
+            r.size2 = A.nd_rec[4 * pr.y + 3].y;
             r.m0 = A.nd_mom[4 * pr.y + 0];
             r.m1 = A.nd_mom[4 * pr.y + 1];
             r.m2 = A.nd_mom[4 * pr.y + 2];
@@ -711,14 +710,14 @@ __global__ __launch_bounds__(256) void k_learn_sum(const int2* __restrict__ pair
 __global__ __launch_bounds__(256) void k_learn_zero(long long* __restrict__ acc, uint32_t* __restrict__ unit,
                                                     DevScalars* __restrict__ sc, const int32_t* __restrict__ sys_root,
                                                     const double* __restrict__ sys_geom,
-                                                    const float4* __restrict__ nd_com, int nsys) {
+                                                    const float4* __restrict__ nd_rec, int nsys) {
     if (blockIdx.x == 0) {
         // the one-pass unit: the largest G M / h^2 over the systems
         __shared__ float s_a[256];
         float a = 0.f;
         for (int k = threadIdx.x; k < nsys; k += blockDim.x) {
             const double h = sys_geom[4 * k + 3];
-            const float gm = nd_com[sys_root[k]].w;
+            const float gm = nd_rec[4 * sys_root[k]].w;
             const float ak = static_cast<float>(static_cast<double>(gm) / (h * h));
             if (ak > a) a = ak;  // a NaN (no mass, no extent) is skipped
         }
@@ -779,9 +778,7 @@ struct SampleArgs {
     int64_t nnew;
     const int32_t* nd_start;
     const int32_t* nd_count;
-    const float4* nd_com;
-    const float4* nd_quad;
-    const float4* nd_aux;
+    const float4* nd_rec;  // 4 per node: com, quad, (Qyz, Qzz, first particle), aux
     const float4* nd_mom;
     const float4* spos;
     const uint32_t* perm;
@@ -804,9 +801,9 @@ __global__ __launch_bounds__(256) void k_learn_samples(SampleArgs A) {
         const int2 pr = A.pairs[q];
         const int gst = A.nd_start[pr.x], gcn = A.nd_count[pr.x];
         const int vst = A.nd_start[pr.y], vcn = A.nd_count[pr.y];
-        const float4 cm = A.nd_com[pr.y];
-        const float4 qd = A.nd_quad[pr.y];
-        const float4 au = A.nd_aux[pr.y];
+        const float4 cm = A.nd_rec[4 * pr.y];
+        const float4 qd = A.nd_rec[4 * pr.y + 1];
+        const float4 au = A.nd_rec[4 * pr.y + 3];
         double rx = 0.0, ry = 0.0, rz = 0.0, wa = 0.0;
         for (int mI = 0; mI < gcn; ++mI) {
             const float4 pm = A.spos[gst + mI];
@@ -848,8 +845,8 @@ __global__ __launch_bounds__(256) void k_learn_samples(SampleArgs A) {
             const double a2 = A.amag2[A.perm[gst + mI]];
             wa += (a2 > 0.0) ? 1.0 / a2 : 0.0;
         }
-        const float4 lc = A.nd_com[pr.x];
-        const float4 tg = make_float4(lc.x, lc.y, lc.z, A.nd_aux[pr.x].z);
+        const float4 lc = A.nd_rec[4 * pr.x];
+        const float4 tg = make_float4(lc.x, lc.y, lc.z, A.nd_rec[4 * pr.x + 3].z);
         float f[kF], scale;
         pair_features(tg, cm, au.y, A.nd_mom + 4 * pr.y, static_cast<float>(sqrt(A.eps2)), f, scale);
         const double inv_n = gcn > 0 ? 1.0 / gcn : 0.0;
@@ -1246,7 +1243,7 @@ int learn_grow_pairs(nbg_ctx* c, int64_t need) {
 
 void launch_learn_zero(nbg_ctx* c) {
     hipLaunchKernelGGL(k_learn_zero, dim3(1024), dim3(256), 0, c->stream, c->lacc, c->lunit, c->dsc, c->sys_root,
-                       c->sys_geom, c->nd_com, c->nsys);
+                       c->sys_geom, c->nd_rec, c->nsys);
 }
 
 void launch_learn_correct(nbg_ctx* c, const nbg_params& p) {
@@ -1255,8 +1252,7 @@ void launch_learn_correct(nbg_ctx* c, const nbg_params& p) {
     A.sc = c->dsc;
     A.scw = c->dsc;
     A.cap = static_cast<uint32_t>(c->lpairs_cap);
-    A.nd_com = c->nd_com;
-    A.nd_aux = c->nd_aux;
+    A.nd_rec = c->nd_rec;
     A.nd_mom = c->nd_mom;
     A.eps = static_cast<float>(std::sqrt(p.eps2));
     A.w16 = c->lw16;
@@ -1303,9 +1299,7 @@ void launch_learn_samples(nbg_ctx* c, const nbg_params& p, int64_t npairs, int64
     A.nnew = nnew;
     A.nd_start = c->nd_start;
     A.nd_count = c->nd_count;
-    A.nd_com = c->nd_com;
-    A.nd_quad = c->nd_quad;
-    A.nd_aux = c->nd_aux;
+    A.nd_rec = c->nd_rec;
     A.nd_mom = c->nd_mom;
     A.spos = c->spos;
     A.perm = c->perm;

@@ -892,7 +892,7 @@ __global__ __launch_bounds__(256) void k_node_finish(const uint64_t* __restrict_
                                                      int32_t* __restrict__ nd_count,
                                                      const int32_t* __restrict__ nd_depth,
                                                      const int32_t* __restrict__ nd_child, double* __restrict__ nd_geom,
-                                                     float4* __restrict__ nd_aux, const DevScalars* __restrict__ sc,
+                                                     float4* __restrict__ nd_rec, const DevScalars* __restrict__ sc,
                                                      uint32_t* __restrict__ dep_hist) {
     __shared__ uint32_t s_hist[64];
     if (threadIdx.x < 64) s_hist[threadIdx.x] = 0;
@@ -927,7 +927,10 @@ __global__ __launch_bounds__(256) void k_node_finish(const uint64_t* __restrict_
         // an internal node's particle count is its children's sum, added by the upsweep
         const int cnt = (nk == 0) ? nd_count[id] : 0;
         const uint32_t info = (nk == 0) ? (kLeafBit | static_cast<uint32_t>(cnt)) : static_cast<uint32_t>(nk);
-        nd_aux[id] = make_float4(0.0f, static_cast<float>(size * size), 0.0f, __uint_as_float(info));
+        const float4 aux = make_float4(0.0f, static_cast<float>(size * size), 0.0f, __uint_as_float(info));
+        // the walk's record: the first particle and the aux word (the upsweep adds the rest)
+        reinterpret_cast<int*>(nd_rec + 4 * id + 2)[2] = i;
+        nd_rec[4 * id + 3] = aux;
         if (nk > 0) atomicAdd(&s_hist[d], 1u);
     }
     __syncthreads();
@@ -940,7 +943,7 @@ __global__ __launch_bounds__(256) void k_node_finish(const uint64_t* __restrict_
 // nodes per depth in LDS and places them from those slots; it covers the same node ids as
 // in k_node_finish (same grid, same stride).
 __global__ __launch_bounds__(256) void k_depth_scatter(const int32_t* __restrict__ nd_depth,
-                                                       const float4* __restrict__ nd_aux,
+                                                       const float4* __restrict__ nd_rec,
                                                        const DevScalars* __restrict__ sc,
                                                        const uint32_t* __restrict__ dep_off, int ndepth,
                                                        int32_t* __restrict__ depth_nodes) {
@@ -954,7 +957,7 @@ __global__ __launch_bounds__(256) void k_depth_scatter(const int32_t* __restrict
         const int id = base + threadIdx.x;
         int d = -1;
         uint32_t rank = 0;
-        if (id < nn && !(__float_as_uint(nd_aux[id].w) & kLeafBit)) {
+        if (id < nn && !(__float_as_uint(nd_rec[4 * id + 3].w) & kLeafBit)) {
             d = nd_depth[id];
             rank = atomicAdd(&s_cnt[d], 1u);
         }
@@ -991,10 +994,10 @@ void launch_build_t(nbg_ctx* c) {
     hipLaunchKernelGGL(k_link, dim3(gnodes), dim3(256), 0, st, c->nd_parent, c->nd_oct, c->dsc, c->nd_child);
     hipLaunchKernelGGL(k_node_finish<LEVELS>, dim3(gnodes), dim3(256), 0, st, c->key_sorted, c->klo_sorted,
                        c->pos_sys, c->sys_off, c->sys_geom, c->nd_start, c->nd_count, c->nd_depth, c->nd_child,
-                       c->nd_geom, c->nd_aux, c->dsc, c->dep_hist);
+                       c->nd_geom, c->nd_rec, c->dsc, c->dep_hist);
     constexpr int nd = kInternalDepths<LEVELS>;
     excl_scan<uint32_t>(st, c->dep_hist, c->dep_off, static_cast<int64_t>(nd) * kNodeBlocks, c->dep_part, nullptr);
-    hipLaunchKernelGGL(k_depth_scatter, dim3(gnodes), dim3(256), 0, st, c->nd_depth, c->nd_aux, c->dsc, c->dep_off,
+    hipLaunchKernelGGL(k_depth_scatter, dim3(gnodes), dim3(256), 0, st, c->nd_depth, c->nd_rec, c->dsc, c->dep_off,
                        nd, c->depth_nodes);
 }
 
@@ -1018,13 +1021,19 @@ void launch_build(nbg_ctx* c, int levels) {
 // incremental update of Tree.cpp:120-122 produces: the walk never uses it as a source, and
 // a massless group then opens every cell, as in the reference.
 
-// The far field's record of a node, one 64-byte line: (com, G*M), G*(Qxx, Qyy, Qxy, Qxz),
-// G*(Qyz, Qzz = -(Qxx + Qyy)); read together, they cost the walk one scalar-cache line per
-// cell instead of one from each of three arrays.
-__device__ __forceinline__ void m2p_store(float4* __restrict__ nd_m2p, int id, float4 com, float4 quad, float qyz) {
-    nd_m2p[4 * id + 0] = com;
-    nd_m2p[4 * id + 1] = quad;
-    nd_m2p[4 * id + 2] = make_float4(qyz, -(quad.x + quad.y), 0.f, 0.f);
+// A node's FP32 record, one 64-byte line: [0] (com, G*M), [1] G*(Qxx, Qyy, Qxy, Qxz),
+// [2] G*(Qyz, Qzz = -(Qxx + Qyy)), first sorted particle, -, [3] the aux word (G*Qyz, size^2,
+// radius, info).  The walk's far field reads [0..2] (one scalar-cache line per cell instead
+// of one from each of three arrays), its traversal [0], [3] and a leaf's first particle (one
+// line per node).  k_node_finish writes the particle and [3].y, [3].w; the moments the rest.
+__device__ __forceinline__ void rec_store(float4* __restrict__ nd_rec, int id, float4 com, float4 quad, float qyz,
+                                          float rmax) {
+    nd_rec[4 * id + 0] = com;
+    nd_rec[4 * id + 1] = quad;
+    reinterpret_cast<float2*>(nd_rec + 4 * id + 2)[0] = make_float2(qyz, -(quad.x + quad.y));
+    float* aux = reinterpret_cast<float*>(nd_rec + 4 * id + 3);
+    aux[0] = qyz;
+    aux[2] = rmax;
 }
 
 // Per leaf, one lane each: its members put in ascending original index (the reference
@@ -1040,8 +1049,7 @@ __global__ __launch_bounds__(256) void k_leaves(const int32_t* __restrict__ leaf
                                                 const float4* __restrict__ pos4, uint32_t* __restrict__ perm, float Gf,
                                                 double G, const DevScalars* __restrict__ sc,
                                                 float4* __restrict__ spos, int32_t* __restrict__ leaf_of,
-                                                float4* __restrict__ nd_com, float4* __restrict__ nd_quad,
-                                                float4* __restrict__ nd_aux, float4* __restrict__ nd_m2p,
+                                                float4* __restrict__ nd_rec,
                                                 double4* __restrict__ nd_com64, double* __restrict__ nd_rmax64,
                                                 float4* __restrict__ nd_mom, const double* __restrict__ x64,
                                                 const double* __restrict__ y64, const double* __restrict__ z64) {
@@ -1192,13 +1200,7 @@ __global__ __launch_bounds__(256) void k_leaves(const int32_t* __restrict__ leaf
         nd_rmax64[id] = rmax;
         const float4 com = make_float4(fx, fy, fz, static_cast<float>(G * M));
         const float4 quad = make_float4(qxx, qyy, qxy, qxz);
-        nd_com[id] = com;
-        nd_quad[id] = quad;
-        m2p_store(nd_m2p, id, com, quad, qyz);
-        // the aux word's x (qyz) and z (radius); y (size^2) and w (info) stay as built
-        float* aux = reinterpret_cast<float*>(nd_aux + id);
-        aux[0] = qyz;
-        aux[2] = static_cast<float>(rmax);
+        rec_store(nd_rec, id, com, quad, qyz, static_cast<float>(rmax));
     }
 }
 
@@ -1352,9 +1354,7 @@ __device__ __forceinline__ void up_combine(const int (&ch)[8], double G, Rec rec
 // An internal node's combined record to its arrays (the aux word's y and w stay as built).
 template <bool MOM>
 __device__ __forceinline__ void up_store(int id, const UpRec& r, const float (&mo)[16], double G,
-                                         int32_t* __restrict__ nd_count, float4* __restrict__ nd_com,
-                                         float4* __restrict__ nd_quad, float4* __restrict__ nd_aux,
-                                         float4* __restrict__ nd_m2p, double4* __restrict__ nd_com64,
+                                         int32_t* __restrict__ nd_count, float4* __restrict__ nd_rec, double4* __restrict__ nd_com64,
                                          float4* __restrict__ nd_mom) {
     if (MOM) {
         nd_mom[4 * id + 0] = make_float4(mo[0], mo[1], mo[2], mo[3]);
@@ -1366,23 +1366,18 @@ __device__ __forceinline__ void up_store(int id, const UpRec& r, const float (&m
     nd_com64[id] = r.cd;
     const float4 com = make_float4(static_cast<float>(r.cd.x), static_cast<float>(r.cd.y), static_cast<float>(r.cd.z),
                                    static_cast<float>(G * r.cd.w));
-    nd_com[id] = com;
-    nd_quad[id] = r.cq;
-    m2p_store(nd_m2p, id, com, r.cq, r.ca.x);
-    float* aux = reinterpret_cast<float*>(nd_aux + id);
-    aux[0] = r.ca.x;
-    aux[2] = r.ca.z;
+    rec_store(nd_rec, id, com, r.cq, r.ca.x, r.ca.z);
 }
 
 // A node's record as its arrays hold it (a leaf's, or an internal node combined earlier).
 template <bool QUAD>
 __device__ __forceinline__ UpRec up_load(int i, const int32_t* __restrict__ nd_count,
-                                         const float4* __restrict__ nd_quad, const float4* __restrict__ nd_aux,
+                                         const float4* __restrict__ nd_rec,
                                          const double4* __restrict__ nd_com64) {
     UpRec r;
     r.cd = nd_com64[i];
-    r.ca = nd_aux[i];
-    if (QUAD) r.cq = nd_quad[i];
+    r.ca = nd_rec[4 * i + 3];
+    if (QUAD) r.cq = nd_rec[4 * i + 1];
     r.cn = nd_count[i];
     return r;
 }
@@ -1390,8 +1385,7 @@ __device__ __forceinline__ UpRec up_load(int i, const int32_t* __restrict__ nd_c
 // Combines internal node `id` from its children's records in memory.
 template <bool QUAD, bool MOM, bool F64>
 __device__ __forceinline__ void up_node_mem(int id, const int (&ch)[8], double G, int32_t* __restrict__ nd_count,
-                                            float4* __restrict__ nd_com, float4* __restrict__ nd_quad,
-                                            float4* __restrict__ nd_aux, float4* __restrict__ nd_m2p,
+                                            float4* __restrict__ nd_rec,
                                             double4* __restrict__ nd_com64, float4* __restrict__ nd_mom, UpRec& out,
                                             float (&mo)[16]) {
     int src[8];  // an absent child reads the node's own slot (discarded)
@@ -1399,9 +1393,9 @@ __device__ __forceinline__ void up_node_mem(int id, const int (&ch)[8], double G
     for (int c = 0; c < 8; ++c) src[c] = ch[c] >= 0 ? ch[c] : id;
     up_combine<QUAD, MOM, F64>(
         ch, G,
-        [&](int c) __attribute__((always_inline)) { return up_load<QUAD>(src[c], nd_count, nd_quad, nd_aux, nd_com64); },
+        [&](int c) __attribute__((always_inline)) { return up_load<QUAD>(src[c], nd_count, nd_rec, nd_com64); },
         [&](int c, int j) __attribute__((always_inline)) { return nd_mom[4 * src[c] + j]; }, out, mo);
-    up_store<MOM>(id, out, mo, G, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64, nd_mom);
+    up_store<MOM>(id, out, mo, G, nd_count, nd_rec, nd_com64, nd_mom);
 }
 
 // One depth: a lane per internal node of that depth.
@@ -1411,8 +1405,7 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
                                                      int32_t* __restrict__ nd_count, double G,
                                                      const uint32_t* __restrict__ dep_part,
                                                      const uint32_t* __restrict__ dep_off,
-                                                     float4* __restrict__ nd_com, float4* __restrict__ nd_quad,
-                                                     float4* __restrict__ nd_aux, float4* __restrict__ nd_m2p,
+                                                     float4* __restrict__ nd_rec,
                                                      double4* __restrict__ nd_com64, float4* __restrict__ nd_mom) {
     const int cnt = static_cast<int>(dep_part[depth]);
     const int off = static_cast<int>(dep_off[depth * kNodeBlocks]);
@@ -1423,7 +1416,7 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
         const int ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
         UpRec r;
         float mo[16];
-        up_node_mem<QUAD, MOM, F64>(id, ch, G, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64, nd_mom, r, mo);
+        up_node_mem<QUAD, MOM, F64>(id, ch, G, nd_count, nd_rec, nd_com64, nd_mom, r, mo);
     }
 }
 
@@ -1433,13 +1426,11 @@ __global__ __launch_bounds__(256) void k_up_internal(int depth, const int32_t* _
 // node, combined from its children's records in memory (and written).
 template <bool QUAD, bool MOM, bool F64>
 __device__ __forceinline__ void up_child_to_lds(int c, int t, double G, const int32_t* __restrict__ nd_child,
-                                                int32_t* __restrict__ nd_count, float4* __restrict__ nd_com,
-                                                float4* __restrict__ nd_quad, float4* __restrict__ nd_aux,
-                                                float4* __restrict__ nd_m2p, double4* __restrict__ nd_com64,
+                                                int32_t* __restrict__ nd_count, float4* __restrict__ nd_rec, double4* __restrict__ nd_com64,
                                                 float4* __restrict__ nd_mom, UpRec* s_rec, float4* s_mom) {
     const int4 c0 = reinterpret_cast<const int4*>(nd_child)[2 * c];
     const int4 c1 = reinterpret_cast<const int4*>(nd_child)[2 * c + 1];
-    UpRec r = up_load<QUAD>(c, nd_count, nd_quad, nd_aux, nd_com64);
+    UpRec r = up_load<QUAD>(c, nd_count, nd_rec, nd_com64);
     float mo[16];
     if (MOM) {
 #pragma unroll
@@ -1454,7 +1445,7 @@ __device__ __forceinline__ void up_child_to_lds(int c, int t, double G, const in
     const int gch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
     const bool internal = max(max(max(c0.x, c0.y), max(c0.z, c0.w)), max(max(c1.x, c1.y), max(c1.z, c1.w))) >= 0;
     if (internal)
-        up_node_mem<QUAD, MOM, F64>(c, gch, G, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64, nd_mom, r, mo);
+        up_node_mem<QUAD, MOM, F64>(c, gch, G, nd_count, nd_rec, nd_com64, nd_mom, r, mo);
     s_rec[t] = r;
     if (MOM) {
 #pragma unroll
@@ -1466,16 +1457,14 @@ __device__ __forceinline__ void up_child_to_lds(int c, int t, double G, const in
 // writes it (and, if t >= 0, its record into LDS slot t).
 template <bool QUAD, bool MOM, bool F64>
 __device__ __forceinline__ void up_node_lds(int id, const int (&ch)[8], int g0, int t, double G,
-                                            int32_t* __restrict__ nd_count, float4* __restrict__ nd_com,
-                                            float4* __restrict__ nd_quad, float4* __restrict__ nd_aux,
-                                            float4* __restrict__ nd_m2p, double4* __restrict__ nd_com64,
+                                            int32_t* __restrict__ nd_count, float4* __restrict__ nd_rec, double4* __restrict__ nd_com64,
                                             float4* __restrict__ nd_mom, UpRec* s_rec, float4* s_mom) {
     UpRec r;
     float mo[16];
     up_combine<QUAD, MOM, F64>(
         ch, G, [&](int j) __attribute__((always_inline)) { return s_rec[g0 + j]; },
         [&](int j, int w) __attribute__((always_inline)) { return s_mom[4 * (g0 + j) + w]; }, r, mo);
-    up_store<MOM>(id, r, mo, G, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64, nd_mom);
+    up_store<MOM>(id, r, mo, G, nd_count, nd_rec, nd_com64, nd_mom);
     if (t >= 0) {
         s_rec[t] = r;
         if (MOM) {
@@ -1499,9 +1488,7 @@ __global__ __launch_bounds__(256) void k_up_pair(int depth, const int32_t* __res
                                                  const int32_t* __restrict__ nd_child,
                                                  int32_t* __restrict__ nd_count, double G,
                                                  const uint32_t* __restrict__ dep_part,
-                                                 const uint32_t* __restrict__ dep_off, float4* __restrict__ nd_com,
-                                                 float4* __restrict__ nd_quad, float4* __restrict__ nd_aux,
-                                                 float4* __restrict__ nd_m2p, double4* __restrict__ nd_com64,
+                                                 const uint32_t* __restrict__ dep_off, float4* __restrict__ nd_rec, double4* __restrict__ nd_com64,
                                                  float4* __restrict__ nd_mom) {
     __shared__ UpRec s_rec[256];
     __shared__ float4 s_mom[MOM ? 4 * 256 : 1];
@@ -1517,14 +1504,14 @@ __global__ __launch_bounds__(256) void k_up_pair(int depth, const int32_t* __res
         const int c = (id >= 0) ? nd_child[8 * id + a] : -1;
         s_ch[t] = c;
         if (c >= 0)
-            up_child_to_lds<QUAD, MOM, F64>(c, t, G, nd_child, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64,
+            up_child_to_lds<QUAD, MOM, F64>(c, t, G, nd_child, nd_count, nd_rec, nd_com64,
                                             nd_mom, s_rec, s_mom);
         wave_sync();
         if (a == 0 && id >= 0) {
             int ch[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) ch[j] = s_ch[g0 + j];
-            up_node_lds<QUAD, MOM, F64>(id, ch, g0, -1, G, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64, nd_mom,
+            up_node_lds<QUAD, MOM, F64>(id, ch, g0, -1, G, nd_count, nd_rec, nd_com64, nd_mom,
                                         s_rec, s_mom);
         }
         wave_sync();
@@ -1541,9 +1528,7 @@ __global__ __launch_bounds__(256) void k_up_top(int depth, const int32_t* __rest
                                                 const int32_t* __restrict__ nd_child,
                                                 int32_t* __restrict__ nd_count, double G,
                                                 const uint32_t* __restrict__ dep_part,
-                                                const uint32_t* __restrict__ dep_off, float4* __restrict__ nd_com,
-                                                float4* __restrict__ nd_quad, float4* __restrict__ nd_aux,
-                                                float4* __restrict__ nd_m2p, double4* __restrict__ nd_com64,
+                                                const uint32_t* __restrict__ dep_off, float4* __restrict__ nd_rec, double4* __restrict__ nd_com64,
                                                 float4* __restrict__ nd_mom) {
     __shared__ UpRec s_rec[256];
     __shared__ float4 s_mom[MOM ? 4 * 256 : 1];
@@ -1561,7 +1546,7 @@ __global__ __launch_bounds__(256) void k_up_top(int depth, const int32_t* __rest
         const int g = (ca >= 0) ? nd_child[8 * ca + b] : -1;
         s_ch[t] = g;
         if (g >= 0)
-            up_child_to_lds<QUAD, MOM, F64>(g, t, G, nd_child, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64,
+            up_child_to_lds<QUAD, MOM, F64>(g, t, G, nd_child, nd_count, nd_rec, nd_com64,
                                             nd_mom, s_rec, s_mom);
         wave_sync();
         int ch[8];
@@ -1573,10 +1558,10 @@ __global__ __launch_bounds__(256) void k_up_top(int depth, const int32_t* __rest
                 internal |= ch[j] >= 0;
             }
             if (internal) {
-                up_node_lds<QUAD, MOM, F64>(ca, ch, t, t, G, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64,
+                up_node_lds<QUAD, MOM, F64>(ca, ch, t, t, G, nd_count, nd_rec, nd_com64,
                                             nd_mom, s_rec, s_mom);
             } else {
-                s_rec[t] = up_load<QUAD>(ca, nd_count, nd_quad, nd_aux, nd_com64);
+                s_rec[t] = up_load<QUAD>(ca, nd_count, nd_rec, nd_com64);
                 if (MOM) {
 #pragma unroll
                     for (int j = 0; j < 4; ++j) s_mom[4 * t + j] = nd_mom[4 * ca + j];
@@ -1594,7 +1579,7 @@ __global__ __launch_bounds__(256) void k_up_top(int depth, const int32_t* __rest
             up_combine<QUAD, MOM, F64>(
                 ch, G, [&](int j) __attribute__((always_inline)) { return s_rec[w0 + 8 * j]; },
                 [&](int j, int wd) __attribute__((always_inline)) { return s_mom[4 * (w0 + 8 * j) + wd]; }, r, mo);
-            up_store<MOM>(id, r, mo, G, nd_count, nd_com, nd_quad, nd_aux, nd_m2p, nd_com64, nd_mom);
+            up_store<MOM>(id, r, mo, G, nd_count, nd_rec, nd_com64, nd_mom);
         }
         wave_sync();
     }
@@ -1608,8 +1593,8 @@ void launch_upsweep(nbg_ctx* c, int levels, int multipole, double G, bool moment
     const float Gf = static_cast<float>(G);
 #define NBG_LEAVES(Q, M, F)                                                                                          \
     hipLaunchKernelGGL((k_leaves<Q, M, F>), dim3(gnodes), dim3(256), 0, st, c->leaf_list, c->nd_start, c->nd_count,   \
-                       c->stage4, c->perm, Gf, G, c->dsc, c->spos, c->leaf_of, c->nd_com, c->nd_quad, c->nd_aux,    \
-                       c->nd_m2p, c->nd_com64, c->nd_rmax64, c->nd_mom, c->x64, c->y64, c->z64)
+                       c->stage4, c->perm, Gf, G, c->dsc, c->spos, c->leaf_of, c->nd_rec, c->nd_com64,       \
+                       c->nd_rmax64, c->nd_mom, c->x64, c->y64, c->z64)
     if (mom)
         NBG_LEAVES(true, true, false);
     else if (quad && c->f64_state)
@@ -1627,19 +1612,19 @@ void launch_upsweep(nbg_ctx* c, int levels, int multipole, double G, bool moment
     do {                                                                                                             \
         if (mom)                                                                                                     \
             hipLaunchKernelGGL((K<true, true, false>), dim3(GD), dim3(256), 0, st, D, c->depth_nodes, c->nd_child,    \
-                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_m2p, c->nd_com64, c->nd_mom);   \
+                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_rec, c->nd_com64, c->nd_mom);   \
         else if (quad && c->f64_state)                                                                               \
             hipLaunchKernelGGL((K<true, false, true>), dim3(GD), dim3(256), 0, st, D, c->depth_nodes, c->nd_child,    \
-                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_m2p, c->nd_com64, c->nd_mom);   \
+                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_rec, c->nd_com64, c->nd_mom);   \
         else if (quad)                                                                                               \
             hipLaunchKernelGGL((K<true, false, false>), dim3(GD), dim3(256), 0, st, D, c->depth_nodes, c->nd_child,   \
-                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_m2p, c->nd_com64, c->nd_mom);   \
+                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_rec, c->nd_com64, c->nd_mom);   \
         else if (c->f64_state)                                                                                       \
             hipLaunchKernelGGL((K<false, false, true>), dim3(GD), dim3(256), 0, st, D, c->depth_nodes, c->nd_child,   \
-                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_m2p, c->nd_com64, c->nd_mom);   \
+                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_rec, c->nd_com64, c->nd_mom);   \
         else                                                                                                         \
             hipLaunchKernelGGL((K<false, false, false>), dim3(GD), dim3(256), 0, st, D, c->depth_nodes, c->nd_child,  \
-                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_com, c->nd_quad, c->nd_aux, c->nd_m2p, c->nd_com64, c->nd_mom);   \
+                               c->nd_count, G, c->dep_part, c->dep_off, c->nd_rec, c->nd_com64, c->nd_mom);   \
     } while (0)
     const int top = (levels <= kLevelsShort) ? levels - 1 : kMaxDepth - 1;
     int d = top;

@@ -99,9 +99,7 @@ struct WalkArgs {
     const int32_t* nd_start;
     const int32_t* nd_count;
     const int32_t* nd_child;
-    const float4* nd_com;
-    const float4* nd_quad;
-    const float4* nd_aux;
+    const float4* nd_rec;  // 4 per node: com, quad, (Qyz, Qzz, first particle), aux
     const double4* nd_com64;
     const double* nd_rmax64;
     const double* nd_geom;
@@ -138,8 +136,8 @@ __global__ __launch_bounds__(64 * kWalkWaves) void k_walk(WalkArgs A) {
         const int g = A.leaf_list[li];
         const int gst = A.nd_start[g];
         const int gcnt = A.nd_count[g];
-        const float4 gc = A.nd_com[g];
-        const float gR = A.nd_aux[g].z;
+        const float4 gc = A.nd_rec[4 * g];
+        const float gR = A.nd_rec[4 * g + 3].z;
         const double4 g64 = A.nd_com64[g];
         const double gR64 = A.nd_rmax64[g];
         const float gscale = fabsf(gc.x) + fabsf(gc.y) + fabsf(gc.z) + gR;
@@ -172,9 +170,9 @@ __global__ __launch_bounds__(64 * kWalkWaves) void k_walk(WalkArgs A) {
                 bool is_far = false, is_near = false, is_open = false;
                 int lstart = 0, lcount = 0;
                 if (node >= 0 && node != g) {
-                    const float4 cm = A.nd_com[node];
+                    const float4 cm = A.nd_rec[4 * node];
                     if (cm.w > 0.f) {
-                        const float4 ax = A.nd_aux[node];
+                        const float4 ax = A.nd_rec[4 * node + 3];
                         const uint32_t info = __float_as_uint(ax.w);
                         if (info & kLeafBit) {
                             is_near = true;
@@ -258,7 +256,8 @@ __global__ __launch_bounds__(64 * kWalkWaves) void k_walk(WalkArgs A) {
                 }
                 if (far_tail - far_head >= 64) {
                     const int nd = far[(far_head + lane) & (kFarRing - 1)];
-                    m2p<QUAD, KAHAN>(acc, tx, ty, tz, nt, A.nd_com[nd], A.nd_quad[nd], A.nd_aux[nd].x, eps2);
+                    m2p<QUAD, KAHAN>(acc, tx, ty, tz, nt, A.nd_rec[4 * nd], A.nd_rec[4 * nd + 1],
+                                     A.nd_rec[4 * nd + 3].x, eps2);
                     far_head += 64;
                     if (COUNT) n_far += 64;
                 }
@@ -275,7 +274,8 @@ __global__ __launch_bounds__(64 * kWalkWaves) void k_walk(WalkArgs A) {
                 const int rem = far_tail - far_head;
                 if (lane < rem) {
                     const int nd = far[(far_head + lane) & (kFarRing - 1)];
-                    m2p<QUAD, KAHAN>(acc, tx, ty, tz, nt, A.nd_com[nd], A.nd_quad[nd], A.nd_aux[nd].x, eps2);
+                    m2p<QUAD, KAHAN>(acc, tx, ty, tz, nt, A.nd_rec[4 * nd], A.nd_rec[4 * nd + 1],
+                                     A.nd_rec[4 * nd + 3].x, eps2);
                 }
                 if (COUNT) n_far += rem;
             }
@@ -347,10 +347,7 @@ struct BinArgs {
     const int32_t* leaf_of;
     const int32_t* nd_start;
     const int32_t* nd_child;
-    const float4* nd_com;
-    const float4* nd_quad;
-    const float4* nd_m2p;  // 4 per node: com, quad, (Qyz, Qzz), unused
-    const float4* nd_aux;
+    const float4* nd_rec;  // 4 per node: com, quad, (Qyz, Qzz, first particle), aux
     const double4* nd_com64;
     const double* nd_rmax64;
     const double* nd_geom;
@@ -470,11 +467,12 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
         const uint64_t mybit = tv ? (1ull << j) : 0ull;
         const uint64_t all = (nl >= 64) ? ~0ull : ((1ull << nl) - 1ull);
         if (tv && mylf != prevlf) {
-            const float4 c = A.nd_com[mylf];
-            lcom[j] = make_float4(c.x, c.y, c.z, A.nd_aux[mylf].z);
+            const float4 c = A.nd_rec[4 * mylf];
+            const float4 la = A.nd_rec[4 * mylf + 3];
+            lcom[j] = make_float4(c.x, c.y, c.z, la.z);
             lid[j] = mylf;
             // the leaf's own members (self excluded when evaluated)
-            qsc[j] = make_int2(A.nd_start[mylf], static_cast<int>(__float_as_uint(A.nd_aux[mylf].w) & ~kLeafBit));
+            qsc[j] = make_int2(__float_as_int(A.nd_rec[4 * mylf + 2].z), static_cast<int>(__float_as_uint(la.w) & ~kLeafBit));
             qmask[j] = 1ull << j;
             if (REL) qnd[j] = mylf;
         }
@@ -561,7 +559,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    cm[u] = cld(A.nd_m2p, 4 * nd[u]);
+                    cm[u] = cld(A.nd_rec, 4 * nd[u]);
                     if (REL) {
                         // the centre of mass relative to cb: an FP64 difference, rounded once
                         const double4 c64 = cld(A.nd_com64, nd[u]);
@@ -569,9 +567,9 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                         cm[u].y = static_cast<float>(c64.y - rby);
                         cm[u].z = static_cast<float>(c64.z - rbz);
                     }
-                    q[u] = cld(A.nd_m2p, 4 * nd[u] + 1);
+                    q[u] = cld(A.nd_rec, 4 * nd[u] + 1);
                     {
-                        const float4 w = cld(A.nd_m2p, 4 * nd[u] + 2);
+                        const float4 w = cld(A.nd_rec, 4 * nd[u] + 2);
                         q2[u] = make_float2(w.x, w.y);
                     }
                 }
@@ -784,11 +782,13 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             float4 cm = make_float4(0.f, 0.f, 0.f, 0.f), au = cm;
             uint32_t info = 0;
             if (node >= 0) {
-                cm = A.nd_com[node];
-                au = A.nd_aux[node];
+                // the node's record line: com, aux word and (a leaf's) first particle
+                cm = A.nd_rec[4 * node];
+                au = A.nd_rec[4 * node + 3];
+                const int st_node = __float_as_int(A.nd_rec[4 * node + 2].z);
                 info = __float_as_uint(au.w);
                 if (info & kLeafBit) {
-                    lstart = A.nd_start[node];
+                    lstart = st_node;
                     lcount = static_cast<int>(info & ~kLeafBit);
                 }
             }
@@ -1054,9 +1054,7 @@ void launch_walk_leaf(nbg_ctx* c, const nbg_params& p) {
     A.nd_start = c->nd_start;
     A.nd_count = c->nd_count;
     A.nd_child = c->nd_child;
-    A.nd_com = c->nd_com;
-    A.nd_quad = c->nd_quad;
-    A.nd_aux = c->nd_aux;
+    A.nd_rec = c->nd_rec;
     A.sys_root = c->sys_root;
     A.pos_sys = c->pos_sys;
     A.spos = c->spos;
@@ -1163,10 +1161,7 @@ void launch_walk_bins(nbg_ctx* c, const nbg_params& p) {
     A.leaf_of = c->leaf_of;
     A.nd_start = c->nd_start;
     A.nd_child = c->nd_child;
-    A.nd_com = c->nd_com;
-    A.nd_quad = c->nd_quad;
-    A.nd_m2p = c->nd_m2p;
-    A.nd_aux = c->nd_aux;
+    A.nd_rec = c->nd_rec;
     A.nd_com64 = c->nd_com64;
     A.nd_rmax64 = c->nd_rmax64;
     A.nd_geom = c->nd_geom;

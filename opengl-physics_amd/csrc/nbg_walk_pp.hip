@@ -46,10 +46,7 @@ struct PPArgs {
     int n_pad;              // index of the first zero pad entry after the sorted particles
     const int32_t* nd_start;
     const int32_t* nd_child;
-    const float4* nd_com;
-    const float4* nd_quad;
-    const float4* nd_m2p;  // 4 per node: com, quad, (Qyz, Qzz), unused
-    const float4* nd_aux;
+    const float4* nd_rec;  // 4 per node: com, quad, (Qyz, Qzz, first particle), aux
     const double4* nd_com64;
     const double* nd_geom;
     const int32_t* sys_root;
@@ -272,10 +269,10 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    cm[u] = cld(A.nd_m2p, 4 * nd[u]);
-                    q[u] = cld(A.nd_m2p, 4 * nd[u] + 1);
+                    cm[u] = cld(A.nd_rec, 4 * nd[u]);
+                    q[u] = cld(A.nd_rec, 4 * nd[u] + 1);
                     {
-                        const float4 w = cld(A.nd_m2p, 4 * nd[u] + 2);
+                        const float4 w = cld(A.nd_rec, 4 * nd[u] + 2);
                         q2[u] = make_float2(w.x, w.y);
                     }
                 }
@@ -388,8 +385,8 @@ __global__ __launch_bounds__(64 * kPPWaves) void k_walk_pp(PPArgs A) {
             float4 cmv[kPPBatch], auv[kPPBatch];
 #pragma unroll
             for (int u = 0; u < kPPBatch; ++u) {
-                cmv[u] = cld(A.nd_com, nd[u]);
-                auv[u] = cld(A.nd_aux, nd[u]);
+                cmv[u] = cld(A.nd_rec, 4 * nd[u]);
+                auv[u] = cld(A.nd_rec, 4 * nd[u] + 3);
             }
             wave_sync();  // the popped entries are read before any push overwrites them
 #pragma unroll
@@ -526,8 +523,8 @@ __global__ __launch_bounds__(64 * kOneWaves) void k_walk_one(PPArgs A) {
             int kids[8];
             int nk = 0;
             if (node >= 0) {
-                const float4 cm = A.nd_com[node];
-                const float4 au = A.nd_aux[node];
+                const float4 cm = A.nd_rec[4 * node];
+                const float4 au = A.nd_rec[4 * node + 3];
                 const uint32_t info = __float_as_uint(au.w);
                 if (cm.w > 0.f) {
                     if (info & kLeafBit) {
@@ -546,7 +543,7 @@ __global__ __launch_bounds__(64 * kOneWaves) void k_walk_one(PPArgs A) {
                         }
                     } else if (pp_accept(A, node, cm, au, tp, tscale, rel_lane, aold, aold_f, true)) {
                         float ex, ey, ez;
-                        m2p_cell<QUAD, SPL>(cm, A.nd_quad[node], A.nd_aux[node].x, tp.x, tp.y, tp.z, soft, ex, ey,
+                        m2p_cell<QUAD, SPL>(cm, A.nd_rec[4 * node + 1], A.nd_rec[4 * node + 3].x, tp.x, tp.y, tp.z, soft, ex, ey,
                                             ez);
                         kadd<KAHAN>(ax, cx, ex);
                         kadd<KAHAN>(ay, cy, ey);
@@ -652,10 +649,7 @@ void launch_walk_pp(nbg_ctx* c, const nbg_params& p, const int32_t* tlist, const
     A.n_pad = c->n;
     A.nd_start = c->nd_start;
     A.nd_child = c->nd_child;
-    A.nd_com = c->nd_com;
-    A.nd_quad = c->nd_quad;
-    A.nd_m2p = c->nd_m2p;
-    A.nd_aux = c->nd_aux;
+    A.nd_rec = c->nd_rec;
     A.nd_com64 = c->nd_com64;
     A.nd_geom = c->nd_geom;
     A.sys_root = c->sys_root;
