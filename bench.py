@@ -105,10 +105,10 @@ def stage_kernels(levels, moments):
         "keys": [("void k_bbox<float>", 1), ("void k_root<float>", 1), (f"void k_keys<{levels}, float>", 1)],
         "sort": [("k_radix_hist", P), ("void k_scan_reduce<unsigned int>", P),
                  ("void k_scan_down_inline<unsigned int>", P), ("void k_radix_scatter<false>", P)],
-        "build": [(f"void k_leaf_depth<{levels}>", 1), ("void k_scan_reduce<unsigned long>", 1),
-                  ("void k_scan_down_inline<unsigned long>", 1), (f"void k_nodes<{levels}>", 1),
+        "build": [(f"void k_leaf_depth<{levels}>", 1), ("void k_scan_down_inline<unsigned long>", 1),
+                  (f"void k_nodes<{levels}>", 1),
                   ("k_link", 1), (f"void k_node_finish<{levels}>", 1), ("void k_scan_reduce<unsigned int>", 1),
-                  ("void k_scan_down_inline<unsigned int>", 1), ("k_depth_scatter", 1), ("k_pack4", 1)],
+                  ("void k_scan_down_inline<unsigned int>", 1), ("k_depth_scatter", 1)],
         "upsweep": [(f"void k_leaves<true, {m}, false>", 1), (f"void k_up_pair<true, {m}, false>", pairs),
                     (f"void k_up_top<true, {m}, false>", 1)]
                    + ([(f"void k_up_internal<true, {m}, false>", 1)] if single else []),

@@ -680,7 +680,9 @@ int nbg_upload_batch(nbg_ctx* c, int32_t nsys, const int64_t* offsets, const flo
     r |= alloc_into(c, c->bin_start, std::max<size_t>(1, wstart.size()));
     r |= alloc_into(c, c->bin_count, std::max<size_t>(1, wcnt.size()));
     r |= alloc_into(c, c->bin_cost, std::max<size_t>(1, wcnt.size()));
-    const size_t parts = std::max<size_t>(grid_for(n + 1, 2048), grid_for(static_cast<int64_t>(kRadix) * nb, 2048)) + 8;
+    // (also k_leaf_depth's per-block sums, one per 256 positions)
+    const size_t parts = std::max<size_t>(std::max<size_t>(grid_for(n + 1, 2048), grid_for(n, 256)),
+                                          grid_for(static_cast<int64_t>(kRadix) * nb, 2048)) + 8;
     r |= alloc_into(c, c->scan_part, parts);
     if (r != NBG_OK) return r;
     if (!bsys.empty()) {

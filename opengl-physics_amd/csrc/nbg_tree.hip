@@ -128,13 +128,26 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const T* __restrict_
 // last block stores the grand total.
 constexpr int kScanInline = 4096;
 
+// `per`: partials per tile (1: k_scan_reduce's tile sums; kScanTile / 256: the sums of
+// 256-element blocks a producer kernel wrote, saving the reduce launch).
 template <typename T>
 __global__ __launch_bounds__(kScanThreads) void k_scan_down_inline(const T* __restrict__ in, T* __restrict__ out,
                                                                    int64_t n, const T* __restrict__ part,
-                                                                   T* __restrict__ total_out) {
+                                                                   T* __restrict__ total_out, int per) {
     __shared__ T s_wave[kScanThreads / 64];
     T pre = 0;
-    for (int i = threadIdx.x; i < static_cast<int>(blockIdx.x); i += kScanThreads) pre += part[i];
+    // eight loads in flight per thread and round
+    const int np = per * static_cast<int>(blockIdx.x);
+    for (int i0 = 0; i0 < np; i0 += 8 * kScanThreads) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = i0 + u * kScanThreads + static_cast<int>(threadIdx.x);
+            v[u] = (i < np) ? part[min(i, np - 1)] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) pre += v[u];
+    }
     T before;
     (void)block_excl_scan(pre, s_wave, &before);
     const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
@@ -161,7 +174,7 @@ void excl_scan(hipStream_t st, const T* in, T* out, int64_t n, T* part, T* total
     const int nb = grid_for(n, kScanTile);
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(kScanThreads), 0, st, in, n, part);
     if (nb <= kScanInline) {
-        hipLaunchKernelGGL(k_scan_down_inline<T>, dim3(nb), dim3(kScanThreads), 0, st, in, out, n, part, total_out);
+        hipLaunchKernelGGL(k_scan_down_inline<T>, dim3(nb), dim3(kScanThreads), 0, st, in, out, n, part, total_out, 1);
         return;
     }
     hipLaunchKernelGGL(k_scan_partials<T>, dim3(1), dim3(kScanThreads), 0, st, part, static_cast<int64_t>(nb), total_out);
@@ -350,13 +363,17 @@ __global__ __launch_bounds__(256) void k_root(const typename PosOrd<T>::U* __res
 // Morton key of every particle from the FP64 centre recurrence (see file header).
 // LEVELS = 21 fills the short key; LEVELS = 40 also fills the low word; LEVELS = 16 fills
 // levels 1..16 of the short key (at their places in it, levels 17..21 zero).
+// pos4 (FP32 states; null otherwise): the input-order records (x, y, z, m) the per-leaf
+// gather reads, written here from the positions this kernel reads anyway.
 template <int LEVELS, typename T>
 __global__ __launch_bounds__(256) void k_keys(const T* __restrict__ x, const T* __restrict__ y,
                                               const T* __restrict__ z, const int32_t* __restrict__ pos_sys,
                                               const double* __restrict__ geom, int n, uint64_t* __restrict__ key,
-                                              uint64_t* __restrict__ klo, uint32_t* __restrict__ val) {
+                                              uint64_t* __restrict__ klo, uint32_t* __restrict__ val,
+                                              const float* __restrict__ m, float4* __restrict__ pos4) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (pos4) pos4[i] = make_float4(static_cast<float>(x[i]), static_cast<float>(y[i]), static_cast<float>(z[i]), m[i]);
     const int s = pos_sys[i];
     double cx = geom[4 * s + 0], cy = geom[4 * s + 1], cz = geom[4 * s + 2], h = geom[4 * s + 3];
     const double px = static_cast<double>(x[i]), py = static_cast<double>(y[i]), pz = static_cast<double>(z[i]);
@@ -393,16 +410,16 @@ void launch_bbox_and_keys(nbg_ctx* c, int levels) {
                            c->sys_geom, c->dsc);
         if (levels == kLevelsTiny)
             hipLaunchKernelGGL((k_keys<kLevelsTiny, double>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x64,
-                               c->y64, c->z64, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
+                               c->y64, c->z64, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a, nullptr, nullptr);
         else if (levels == kLevelsFast)
             hipLaunchKernelGGL((k_keys<kLevelsFast, double>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x64,
-                               c->y64, c->z64, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
+                               c->y64, c->z64, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a, nullptr, nullptr);
         else if (levels == kLevelsShort)
             hipLaunchKernelGGL((k_keys<kLevelsShort, double>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x64,
-                               c->y64, c->z64, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
+                               c->y64, c->z64, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a, nullptr, nullptr);
         else
             hipLaunchKernelGGL((k_keys<kLevelsDeep, double>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x64,
-                               c->y64, c->z64, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
+                               c->y64, c->z64, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a, nullptr, nullptr);
         return;
     }
     hipLaunchKernelGGL(k_bbox<float>, dim3(c->n_blocks), dim3(kSortThreads), 0, st, c->x, c->y, c->z,
@@ -411,16 +428,16 @@ void launch_bbox_and_keys(nbg_ctx* c, int levels) {
                        c->sys_geom, c->dsc);
     if (levels == kLevelsTiny)
         hipLaunchKernelGGL((k_keys<kLevelsTiny, float>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x, c->y,
-                           c->z, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
+                           c->z, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a, c->m, c->stage4);
     else if (levels == kLevelsFast)
         hipLaunchKernelGGL((k_keys<kLevelsFast, float>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x, c->y,
-                           c->z, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
+                           c->z, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a, c->m, c->stage4);
     else if (levels == kLevelsShort)
         hipLaunchKernelGGL((k_keys<kLevelsShort, float>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x, c->y,
-                           c->z, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
+                           c->z, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a, c->m, c->stage4);
     else
         hipLaunchKernelGGL((k_keys<kLevelsDeep, float>), dim3(grid_for(c->n, 256)), dim3(256), 0, st, c->x, c->y,
-                           c->z, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a);
+                           c->z, c->pos_sys, c->sys_geom, c->n, c->key_a, c->klo_a, c->val_a, c->m, c->stage4);
 }
 
 // ============================================================================ radix sort
@@ -660,8 +677,10 @@ __global__ __launch_bounds__(256) void k_leaf_depth(const uint64_t* __restrict__
                                                     const int32_t* __restrict__ sys_off, int n,
                                                     int32_t* __restrict__ leaf_depth, int32_t* __restrict__ cpl_prev,
                                                     uint64_t* __restrict__ node_scan, DevScalars* __restrict__ sc,
-                                                    int32_t* __restrict__ blk_depth) {
+                                                    int32_t* __restrict__ blk_depth, uint64_t* __restrict__ blk_sum) {
     __shared__ int s_dmax[4];
+    __shared__ uint64_t s_sum[4];
+    uint64_t nsv = 0;  // this position's node_scan word (0 past n)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int D = 0;  // positions past n report depth 0 to the block maximum
     if (i < n) {
@@ -708,16 +727,27 @@ __global__ __launch_bounds__(256) void k_leaf_depth(const uint64_t* __restrict__
         cpl_prev[i] = cp;
         const uint64_t ns = (D > cp) ? static_cast<uint64_t>(D - cp) : 0u;
         const uint64_t ls = (D > cp) ? 1u : 0u;
-        node_scan[i] = (ns << 32) | ls;
+        nsv = (ns << 32) | ls;
+        node_scan[i] = nsv;
     }
     // deepest leaf: one value per block (the totals kernel takes the maximum), no
     // same-address atomics; every thread reaches the one barrier
+    // and the block's sum of the node_scan words (the scan's per-block partial)
     int dmax = D;
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) dmax = max(dmax, __shfl_xor(dmax, o, 64));
-    if (lane_id() == 0) s_dmax[threadIdx.x >> 6] = dmax;
+    for (int o = 32; o >= 1; o >>= 1) {
+        dmax = max(dmax, __shfl_xor(dmax, o, 64));
+        nsv += static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(nsv), o, 64));
+    }
+    if (lane_id() == 0) {
+        s_dmax[threadIdx.x >> 6] = dmax;
+        s_sum[threadIdx.x >> 6] = nsv;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) blk_depth[blockIdx.x] = max(max(s_dmax[0], s_dmax[1]), max(s_dmax[2], s_dmax[3]));
+    if (threadIdx.x == 0) {
+        blk_depth[blockIdx.x] = max(max(s_dmax[0], s_dmax[1]), max(s_dmax[2], s_dmax[3]));
+        blk_sum[blockIdx.x] = (s_sum[0] + s_sum[1]) + (s_sum[2] + s_sum[3]);
+    }
 }
 
 // Grand totals of the node scan and the deepest leaf -> device scalars (all threads of one
@@ -984,9 +1014,16 @@ void launch_build_t(nbg_ctx* c) {
     const int g = grid_for(n, 256);
     const int gnodes = kNodeBlocks;  // grid-stride kernels over the device-side node count
     hipLaunchKernelGGL(k_leaf_depth<LEVELS>, dim3(g), dim3(256), 0, st, c->key_sorted, c->klo_sorted, c->pos_sys,
-                       c->sys_off, n, c->leaf_depth, c->cpl_prev, c->node_scan, c->dsc, c->blk_depth);
-    // node_scan[n] receives the grand total (the array holds n+1 entries)
-    excl_scan<uint64_t>(st, c->node_scan, c->node_scan, n, c->scan_part, c->node_scan + n);
+                       c->sys_off, n, c->leaf_depth, c->cpl_prev, c->node_scan, c->dsc, c->blk_depth, c->scan_part);
+    // node_scan[n] receives the grand total (the array holds n+1 entries); the block sums of
+    // k_leaf_depth (256 positions each) serve as the scan's partials
+    static_assert(kScanTile % 256 == 0, "whole leaf-depth blocks per scan tile");
+    const int tiles = grid_for(n, kScanTile);
+    if (tiles <= kScanInline)
+        hipLaunchKernelGGL(k_scan_down_inline<uint64_t>, dim3(tiles), dim3(kScanThreads), 0, st, c->node_scan,
+                           c->node_scan, static_cast<int64_t>(n), c->scan_part, c->node_scan + n, kScanTile / 256);
+    else
+        excl_scan<uint64_t>(st, c->node_scan, c->node_scan, n, c->scan_part, c->node_scan + n);
     hipLaunchKernelGGL(k_nodes<LEVELS>, dim3(g), dim3(256), 0, st, c->key_sorted, c->klo_sorted, c->pos_sys,
                        c->sys_off, c->leaf_depth, c->cpl_prev, c->node_scan, n, c->cap_nodes, c->nd_start, c->nd_count,
                        c->nd_depth, c->nd_parent, c->nd_oct, c->nd_child, c->leaf_list, c->sys_root, c->dsc,
@@ -1662,6 +1699,8 @@ void launch_rel(nbg_ctx* c) {
 }
 
 void launch_pack4(nbg_ctx* c) {
+    // (an FP32 state's records are written by the key kernel)
+    if (!c->f64_state) return;
     hipLaunchKernelGGL(k_pack4, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->x, c->y, c->z, c->m, c->n,
                        c->stage4);
 }
