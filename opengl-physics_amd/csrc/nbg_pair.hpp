@@ -165,11 +165,12 @@ __device__ __forceinline__ void m2p_cell(float4 cm, float4 q, float qyz, float t
 // is the cell's com (r^2 = 0), so no select is needed (one instruction less per cell than
 // two selects of the results).
 // q2 = (Qyz, Qzz), Qzz = -(Qxx + Qyy) stored with the cell's record.
+// r2add: the addend of r^2, 0 (the cell is used) or +inf (not used).
 template <bool QUAD, bool SPLINE>
-__device__ __forceinline__ void m2p_add(float4 cm, float4 q, float2 q2, float tx, float ty, float tz,
-                                        const Soft& soft, bool use, float& sx, float& sy, float& sz) {
+__device__ __forceinline__ void m2p_add_r2(float4 cm, float4 q, float2 q2, float tx, float ty, float tz,
+                                           const Soft& soft, float r2add, float& sx, float& sy, float& sz) {
     const float dx = cm.x - tx, dy = cm.y - ty, dz = cm.z - tz;
-    const float dist2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, use ? 0.f : __builtin_inff())));
+    const float dist2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, r2add)));
     float g = pair_coef<SPLINE>(dist2, cm.w, soft);
     if (QUAD) {
         const float qxx = q.x, qyy = q.y, qxy = q.z, qxz = q.w, qyz = q2.x, qzz = q2.y;
@@ -192,6 +193,18 @@ __device__ __forceinline__ void m2p_add(float4 cm, float4 q, float2 q2, float tx
         sy = fmaf(g, dy, sy);
         sz = fmaf(g, dz, sz);
     }
+}
+template <bool QUAD, bool SPLINE>
+__device__ __forceinline__ void m2p_add(float4 cm, float4 q, float2 q2, float tx, float ty, float tz,
+                                        const Soft& soft, bool use, float& sx, float& sy, float& sz) {
+    m2p_add_r2<QUAD, SPLINE>(cm, q, q2, tx, ty, tz, soft, use ? 0.f : __builtin_inff(), sx, sy, sz);
+}
+
+// `yes` where bit `sh` of the mask word w is set, `no` where it is clear: a signed one-bit
+// field extract (0 or all ones) and a bitfield select, two instructions.
+__device__ __forceinline__ float mask_pick(uint32_t w, int sh, float yes, float no) {
+    const uint32_t b = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(w), sh, 1));
+    return __uint_as_float((b & __float_as_uint(yes)) | (~b & __float_as_uint(no)));
 }
 
 }  // namespace nbg

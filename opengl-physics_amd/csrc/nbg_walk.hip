@@ -398,9 +398,13 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     __shared__ int s_snode[kBinWaves][kBinStack];
     __shared__ uint64_t s_smask[kBinWaves][kBinStack];
     __shared__ __attribute__((aligned(16))) int s_fnode[kBinWaves][kBinRing];
-    __shared__ __attribute__((aligned(16))) uint64_t s_fmask[kBinWaves][kBinRing];
+    // the groups' masks of the staged far cells and near leaves, as their low and high
+    // 32-bit words in two arrays: a lane reads only the word holding its own group's bit
+    __shared__ __attribute__((aligned(16))) uint32_t s_fmlo[kBinWaves][kBinRing];
+    __shared__ __attribute__((aligned(16))) uint32_t s_fmhi[kBinWaves][kBinRing];
     __shared__ int2 s_qsc[kBinWaves][kBinRing];  // near leaf: first sorted particle, count
-    __shared__ uint64_t s_qmask[kBinWaves][kBinRing];
+    __shared__ uint32_t s_qmlo[kBinWaves][kBinRing];
+    __shared__ uint32_t s_qmhi[kBinWaves][kBinRing];
     __shared__ int s_qnd[kBinWaves][REL ? kBinRing : 1];  // REL: node of each staged near leaf
     // the bin's leaf spheres at set-up; during the traversal, the records of the nodes whose
     // groups are tested one node at a time (centre of mass, size / theta)
@@ -415,9 +419,11 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     int* snode = s_snode[wave];
     uint64_t* smask = s_smask[wave];
     int* fnode = s_fnode[wave];
-    uint64_t* fmask = s_fmask[wave];
+    uint32_t* fmlo = s_fmlo[wave];
+    uint32_t* fmhi = s_fmhi[wave];
     int2* qsc = s_qsc[wave];
-    uint64_t* qmask = s_qmask[wave];
+    uint32_t* qmlo = s_qmlo[wave];
+    uint32_t* qmhi = s_qmhi[wave];
     int* qnd = s_qnd[wave];
     float4* lcom = s_lcom[wave];
     int* lid = s_lid[wave];
@@ -464,7 +470,10 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
         const uint64_t first = ballot(tv && mylf != prevlf);
         const int nl = __popcll(first);
         const int j = tv ? (__popcll(first & ((2ull << lane) - 1ull)) - 1) : 63;
-        const uint64_t mybit = tv ? (1ull << j) : 0ull;
+        // the mask word and bit of this lane's group (lanes past the bin's targets read
+        // group 63's, and their sums are dropped)
+        const bool jhi = j >= 32;
+        const int jsh = j & 31;
         const uint64_t all = (nl >= 64) ? ~0ull : ((1ull << nl) - 1ull);
         if (tv && mylf != prevlf) {
             const float4 c = A.nd_rec[4 * mylf];
@@ -473,7 +482,8 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             lid[j] = mylf;
             // the leaf's own members (self excluded when evaluated)
             qsc[j] = make_int2(__float_as_int(A.nd_rec[4 * mylf + 2].z), static_cast<int>(__float_as_uint(la.w) & ~kLeafBit));
-            qmask[j] = 1ull << j;
+            qmlo[j] = (j < 32) ? (1u << jsh) : 0u;
+            qmhi[j] = (j < 32) ? 0u : (1u << jsh);
             if (REL) qnd[j] = mylf;
         }
         const int root = cld(A.sys_root, cld(A.pos_sys, start));
@@ -537,25 +547,32 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
         // (their scalar loads issued before any arithmetic); every lane computes every
         // cell and keeps it only if its leaf's bit is in the cell's mask
         // (the ring is consumed from multiples of 64, so a group of four slots is one aligned
-        // 16-byte node read and two 16-byte mask reads; slots past n repeat the first node)
+        // 16-byte node read and one 16-byte read of the lane's mask words; slots past n repeat
+        // the first node with zero masks, written here into the free slots after the last)
+        const uint32_t* fmw = jhi ? fmhi : fmlo;
         auto eval_far = [&](int h, int n) __attribute__((always_inline)) {
             float sx = 0.f, sy = 0.f, sz = 0.f;
             const int hb = h & (kBinRing - 1);
+            if (n & 3) {
+                if (lane < 4 - (n & 3)) {
+                    fmlo[hb + n + lane] = 0u;
+                    fmhi[hb + n + lane] = 0u;
+                }
+            }
             for (int e0 = 0; e0 < n; e0 += 4) {
                 int nd[4];
-                bool um[4];
+                float add[4];
                 float4 cm[4], q[4];
                 float2 q2[4];
                 const int4 n4 = *reinterpret_cast<const int4*>(fnode + hb + e0);
-                const ulonglong2 m01 = *reinterpret_cast<const ulonglong2*>(fmask + hb + e0);
-                const ulonglong2 m23 = *reinterpret_cast<const ulonglong2*>(fmask + hb + e0 + 2);
+                const uint4 w4 = *reinterpret_cast<const uint4*>(fmw + hb + e0);
                 const int nv[4] = {n4.x, n4.y, n4.z, n4.w};
-                const uint64_t mv[4] = {m01.x, m01.y, m23.x, m23.y};
+                const uint32_t wv[4] = {w4.x, w4.y, w4.z, w4.w};
                 const int nd0 = __builtin_amdgcn_readfirstlane(nv[0]);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     nd[u] = (u == 0 || e0 + u < n) ? __builtin_amdgcn_readfirstlane(nv[u]) : nd0;
-                    um[u] = e0 + u < n && (mv[u] & mybit);
+                    add[u] = mask_pick(wv[u], jsh, 0.f, __builtin_inff());
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -575,8 +592,8 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    m2p_add<QUAD, SPL>(cm[u], q[u], q2[u], tp.x, tp.y, tp.z, soft, um[u], sx, sy, sz);
-                    if (COUNT) n_far += um[u];
+                    m2p_add_r2<QUAD, SPL>(cm[u], q[u], q2[u], tp.x, tp.y, tp.z, soft, add[u], sx, sy, sz);
+                    if (COUNT) n_far += add[u] == 0.f;
                 }
             }
             kadd<KAHAN>(ax, cx, sx);
@@ -600,7 +617,6 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             constexpr bool FAST = decltype(fast)::value;
             float sx = 0.f, sy = 0.f, sz = 0.f;
             int e = 0, cur = 0, end = 0;  // the cursor: leaf e, its next particle cur, its end
-            uint64_t em = 0;
             // a source's per-lane use: its r^2 addend (Plummer FAST) or its mass weight
             constexpr bool ADDEND = FAST && !SPL;
             const float wyes = ADDEND ? soft.eps2 : 1.f, wno = ADDEND ? __builtin_inff() : 0.f;
@@ -612,16 +628,18 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             // slot costs one test of the current leaf's end.
             const int lslot = (h + lane) & (kBinRing - 1);
             const int2 my_q = lane < n ? qsc[lslot] : make_int2(0, 0);
-            const uint64_t my_qm = lane < n ? qmask[lslot] : 0ull;
+            const uint32_t my_qlo = lane < n ? qmlo[lslot] : 0u, my_qhi = lane < n ? qmhi[lslot] : 0u;
             auto load_entry = [&](int ee) __attribute__((always_inline)) {
                 // (lane ee & 63 is read unconditionally and dropped past the last leaf)
                 const int st_r = rdlane(my_q.x, ee & 63), c_r = rdlane(my_q.y, ee & 63);
-                const uint64_t em_r = rdlane(my_qm, ee & 63);
+                const uint32_t lo_r = rdlane(my_qlo, ee & 63), hi_r = rdlane(my_qhi, ee & 63);
                 const bool live = ee < n;
                 cur = live ? st_r : A.n_pad;
                 end = live ? st_r + c_r : -1;
-                em = live ? em_r : 0ull;
-                wuse = (em & mybit) ? wyes : wno;
+                // the word of the leaf's mask (wave-uniform) holding the lane's group bit, by
+                // one 64-bit shift of 0 or 32; that bit picks the weight
+                const uint64_t em = live ? ((static_cast<uint64_t>(hi_r) << 32) | lo_r) : 0ull;
+                wuse = mask_pick(static_cast<uint32_t>(em >> (j & 32)), jsh, wyes, wno);
             };
             load_entry(0);
             int groups = 0;
@@ -634,7 +652,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                     idx[u] = min(cur, A.n_pad);
                     us[u] = wuse;
                     vidx = wrlane(vidx, idx[u], u);
-                    if (COUNT) n_near += (em & mybit) && (cur != tpos);
+                    if (COUNT) n_near += (wuse == wyes) && (cur != tpos);
                     if (__builtin_expect(++cur == end, 0)) load_entry(++e);
                 }
                 if (lane < NT) nstage[lane] = bld_lane(srec, vidx);
@@ -682,7 +700,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 const int st = __builtin_amdgcn_readfirstlane(e2.x);
                 const int c = __builtin_amdgcn_readfirstlane(e2.y);
                 const int nd = __builtin_amdgcn_readfirstlane(qnd[slot]);
-                const bool use = (qmask[slot] & mybit) != 0;
+                const bool use = (((jhi ? qmhi[slot] : qmlo[slot]) >> jsh) & 1u) != 0;
                 const double4 g = cld(reinterpret_cast<const double4*>(A.nd_geom), nd);
                 const float txr = tp.x - static_cast<float>(g.x - rbx);
                 const float tyr = tp.y - static_cast<float>(g.y - rby);
@@ -864,6 +882,14 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                     }
                 }
             }
+            // the children of a node that some group may open, loaded before the group tests
+            // decide (their latency hides behind the tests; a node every group accepts loads
+            // none, and one the tests accept in the end loads them for nothing)
+            int4 ch0 = make_int4(-1, -1, -1, -1), ch1 = ch0;
+            if ((open_m | todo) != 0) {
+                ch0 = reinterpret_cast<const int4*>(A.nd_child)[2 * node];
+                ch1 = reinterpret_cast<const int4*>(A.nd_child)[2 * node + 1];
+            }
             // Leaf-major group tests: one iteration per node left undecided by the sphere
             // shortcuts, each lane testing its own leaf group against that node, so that an
             // iteration serves every undecided group of the node (node-major, the wave looped
@@ -940,40 +966,28 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 if (has) {
                     const int slot = (f_tail + mbcnt(bm)) & (kBinRing - 1);
                     fnode[slot] = node;
-                    fmask[slot] = far_m;
+                    fmlo[slot] = static_cast<uint32_t>(far_m);
+                    fmhi[slot] = static_cast<uint32_t>(far_m >> 32);
                 }
                 f_tail += __popcll(bm);
             }
             // learned correction: the (cell, mask of owned leaves with a band pair) entries of
             // the step join the wave's pending entries (compacted through LDS: lcom is free
-            // between the group tests and the next step), which are listed leaf-major when 128
-            // are pending and at the end of the bin
+            // between the group tests and the next step; written here and read back after the
+            // step's one LDS sync below), which are listed leaf-major when 128 are pending and
+            // at the end of the bin
+            int lk = 0;
             if (LEARN) {
                 const uint64_t em = band_m & own_m;
                 const bool has = em != 0;
                 const uint64_t bm = ballot(has);
-                const int k = __popcll(bm);
-                if (k) {
-                    if (npend + k > 2 * 64) flush_pairs();
-                    int4* scr = reinterpret_cast<int4*>(lcom);
+                lk = __popcll(bm);
+                if (lk) {
+                    if (npend + lk > 2 * 64) flush_pairs();
                     if (has)
-                        scr[mbcnt(bm)] = make_int4(node, static_cast<int>(static_cast<uint32_t>(em)),
-                                                   static_cast<int>(static_cast<uint32_t>(em >> 32)), 0);
-                    wave_sync();
-                    // entry e of the pending list sits in lane e % 64 of row e / 64
-                    const int e0 = lane - npend, e1 = lane + 64 - npend;
-                    if (e0 >= 0 && e0 < k) {
-                        const int4 v = scr[e0];
-                        pnode0 = v.x;
-                        pm0 = (static_cast<uint64_t>(static_cast<uint32_t>(v.z)) << 32) | static_cast<uint32_t>(v.y);
-                    }
-                    if (e1 >= 0 && e1 < k) {
-                        const int4 v = scr[e1];
-                        pnode1 = v.x;
-                        pm1 = (static_cast<uint64_t>(static_cast<uint32_t>(v.z)) << 32) | static_cast<uint32_t>(v.y);
-                    }
-                    wave_sync();
-                    npend += k;
+                        reinterpret_cast<int4*>(lcom)[mbcnt(bm)] =
+                            make_int4(node, static_cast<int>(static_cast<uint32_t>(em)),
+                                      static_cast<int>(static_cast<uint32_t>(em >> 32)), 0);
                 }
             }
             // near leaves -> ring
@@ -983,20 +997,17 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 if (has) {
                     const int slot = (q_tail + mbcnt(bm)) & (kBinRing - 1);
                     qsc[slot] = make_int2(lstart, lcount);
-                    qmask[slot] = near_m;
+                    qmlo[slot] = static_cast<uint32_t>(near_m);
+                    qmhi[slot] = static_cast<uint32_t>(near_m >> 32);
                     if (REL) qnd[slot] = node;
                 }
                 q_tail += __popcll(bm);
             }
             // opened cells -> push children with the opening leaves' mask
             {
-                int kids[8];
+                const int kids[8] = {ch0.x, ch0.y, ch0.z, ch0.w, ch1.x, ch1.y, ch1.z, ch1.w};
                 int nk = 0;
                 if (open_m) {
-                    const int4 c0 = reinterpret_cast<const int4*>(A.nd_child)[2 * node];
-                    const int4 c1 = reinterpret_cast<const int4*>(A.nd_child)[2 * node + 1];
-                    kids[0] = c0.x; kids[1] = c0.y; kids[2] = c0.z; kids[3] = c0.w;
-                    kids[4] = c1.x; kids[5] = c1.y; kids[6] = c1.z; kids[7] = c1.w;
 #pragma unroll
                     for (int c = 0; c < 8; ++c) nk += kids[c] >= 0;
                 }
@@ -1016,6 +1027,24 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 sp += ktot;
             }
             wave_sync();
+            if (LEARN && lk) {
+                // entry e of the pending list sits in lane e % 64 of row e / 64 (the entries in
+                // lcom are read before the next step writes it: one wave's LDS accesses complete
+                // in order)
+                const int4* scr = reinterpret_cast<const int4*>(lcom);
+                const int e0 = lane - npend, e1 = lane + 64 - npend;
+                if (e0 >= 0 && e0 < lk) {
+                    const int4 v = scr[e0];
+                    pnode0 = v.x;
+                    pm0 = (static_cast<uint64_t>(static_cast<uint32_t>(v.z)) << 32) | static_cast<uint32_t>(v.y);
+                }
+                if (e1 >= 0 && e1 < lk) {
+                    const int4 v = scr[e1];
+                    pnode1 = v.x;
+                    pm1 = (static_cast<uint64_t>(static_cast<uint32_t>(v.z)) << 32) | static_cast<uint32_t>(v.y);
+                }
+                npend += lk;
+            }
             while (f_tail - f_head >= 64) {
                 eval_far(f_head, 64);
                 f_head += 64;
