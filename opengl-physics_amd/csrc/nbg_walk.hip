@@ -410,7 +410,6 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     // groups are tested one node at a time (centre of mass, size / theta)
     __shared__ float4 s_lcom[kBinWaves][64];
     __shared__ int s_lid[kBinWaves][64];
-    __shared__ float2 s_lrec[kBinWaves][LEARN ? 64 : 1];  // LEARN: (band distance, r2max) of those nodes
     __shared__ float4 s_nstage[kBinWaves][NT];             // the near field's current group of sources
     __shared__ int s_next[kBinWaves];
 
@@ -427,7 +426,6 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
     int* qnd = s_qnd[wave];
     float4* lcom = s_lcom[wave];
     int* lid = s_lid[wave];
-    float2* lrec = s_lrec[wave];
     float4* nstage = s_nstage[wave];
     const Soft soft = A.soft;
     const float inv_theta = A.inv_theta;
@@ -496,8 +494,8 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
         // leaf-major group tests (lanes past the bin's leaves hold leaf 0 and are never asked)
         const float4 my_lc = lcom[lane < nl ? lane : 0];
         const int my_lid = lid[lane < nl ? lane : 0];
-        // the group's share of the FP32 test's rounding bound (position magnitudes + radius)
-        const float my_abs = fabsf(my_lc.x) + fabsf(my_lc.y) + fabsf(my_lc.z) + my_lc.w;
+        // the group's share of the FP32 test's rounding bound below: 4e-6 (2 |c_group|_1 + R)
+        const float my_ub = 4e-6f * (2.f * (fabsf(my_lc.x) + fabsf(my_lc.y) + fabsf(my_lc.z)) + my_lc.w);
         // REL: the bin's reference point cb (its first leaf's FP64 centre) and each target
         // relative to it, from its offset in its own leaf plus the FP64 offset of the leaves
         double rbx = 0.0, rby = 0.0, rbz = 0.0;
@@ -898,7 +896,6 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             // registers, and node L's results are written into lane L with v_writelane.
             if (ballot(todo != 0 || (LEARN && (bneed != 0 || aneed)))) {
                 lcom[lane] = make_float4(cm.x, cm.y, cm.z, n_sth);
-                if (LEARN) lrec[lane] = make_float2(n_sbd, n_r2max);
                 wave_sync();
             }
             for (uint64_t und = ballot(todo != 0); und; und &= und - 1) {
@@ -911,10 +908,12 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 const float q = my_lc.w + nr.w;
                 const float q2 = q * q;
                 uint64_t acc = ballot(dist2 > q2) & tL;
-                // within the FP32 test's rounding bound, 2q (1e-4 q + 2e-6 (|c_node| + |c_group| + R)),
-                // the FP64 decision exactly as CollectGroupInteractions writes it (Solvers.cpp:261-269)
-                const float u = fmaf(q, 2e-4f, 4e-6f * ((fabsf(nr.x) + fabsf(nr.y)) + (fabsf(nr.z) + my_abs)));
-                const uint64_t closem = ballot(fabsf(dist2 - q2) <= q * u) & tL;
+                // within the FP32 test's rounding bound, 2q (1e-4 q + 2e-6 (|c_node|_1 + |c_group|_1 + R)),
+                // the FP64 decision exactly as CollectGroupInteractions writes it (Solvers.cpp:261-269).
+                // Only a pair with dist within 1e-4 relative of q can be decided wrongly, and for it
+                // |c_node|_1 <= |c_group|_1 + sqrt(3) dist <= |c_group|_1 + 1.7338 q, so the bound
+                // is taken as q (2.07e-4 q + 4e-6 (2 |c_group|_1 + R)): per-lane constants and q only
+                const uint64_t closem = ballot(fabsf(dist2 - q2) <= q * fmaf(q, 2.07e-4f, my_ub)) & tL;
                 if (closem) {
                     const int nodeL = rdlane(node, L);
                     bool a64 = false;
@@ -935,10 +934,10 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 // (an undecided node has no far or band groups before its tests)
                 far_m = wrlane(far_m, acc, L);
                 if (LEARN) {
-                    // band and correction scale, both by distance
-                    const float2 lr = lrec[L];
-                    const float qb = my_lc.w + lr.x;
-                    band_m = wrlane(band_m, acc & ballot(dist2 <= (band_on ? fminf(qb * qb, lr.y) : lr.y)), L);
+                    // band and correction scale, both by distance (the node's bounds read from
+                    // its lane; without a band its band distance is +inf and the scale decides)
+                    const float qb = my_lc.w + rdlane(n_sbd, L);
+                    band_m = wrlane(band_m, acc & ballot(dist2 <= fminf(qb * qb, rdlane(n_r2max, L))), L);
                 }
             }
             open_m |= todo & ~far_m;
@@ -950,7 +949,7 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                     const uint64_t bnL = rdlane(bneed, L), bmL = rdlane(band_m, L);
                     const bool anL = rdlane(static_cast<int>(aneed), L) != 0;
                     const float4 nr = lcom[L];
-                    const float2 lr = lrec[L];
+                    const float2 lr = make_float2(rdlane(n_sbd, L), rdlane(n_r2max, L));
                     const float dx = nr.x - my_lc.x, dy = nr.y - my_lc.y, dz = nr.z - my_lc.z;
                     const float r2 = dx * dx + dy * dy + dz * dz;
                     const float qb = my_lc.w + lr.x;
