@@ -776,17 +776,33 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 eval_near_t(h, n, std::false_type{});
         };
 
-        while (sp > 0) {
+        // pops the next batch of up to 64 stacked nodes (one per lane) and issues the loads of
+        // their record words, which land while the step before evaluates its staged sources
+        int node = -1;
+        uint64_t act = 0;
+        float4 rcm = make_float4(0.f, 0.f, 0.f, 0.f), rau = rcm;
+        int rst = 0;
+        auto pop = [&]() __attribute__((always_inline)) {
             const int room = kBinStack - sp - reserve;
             const int k = min(min(64, sp), max(1, room / 7));
-            int node = -1;
-            uint64_t act = 0;
+            node = -1;
+            act = 0;
             if (lane < k) {
                 node = snode[sp - k + lane];
                 act = smask[sp - k + lane];
             }
             sp -= k;
             wave_sync();
+            if (node >= 0) {
+                // the node's record line: com, aux word and (a leaf's) first particle
+                rcm = A.nd_rec[4 * node];
+                rau = A.nd_rec[4 * node + 3];
+                rst = __float_as_int(A.nd_rec[4 * node + 2].z);
+            }
+        };
+        bool more = sp > 0;
+        if (more) pop();
+        while (more) {
             uint64_t far_m = 0, near_m = 0, open_m = 0, band_m = 0;
             uint64_t todo = 0;  // groups the sphere shortcuts left undecided for this lane's node
             float n_sth = 0.f, n_sbd = 0.f, n_r2max = 0.f;
@@ -798,10 +814,9 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
             float4 cm = make_float4(0.f, 0.f, 0.f, 0.f), au = cm;
             uint32_t info = 0;
             if (node >= 0) {
-                // the node's record line: com, aux word and (a leaf's) first particle
-                cm = A.nd_rec[4 * node];
-                au = A.nd_rec[4 * node + 3];
-                const int st_node = __float_as_int(A.nd_rec[4 * node + 2].z);
+                cm = rcm;
+                au = rau;
+                const int st_node = rst;
                 info = __float_as_uint(au.w);
                 if (info & kLeafBit) {
                     lstart = st_node;
@@ -1044,6 +1059,8 @@ __global__ __launch_bounds__(64 * kBinWaves) void k_walk_bins(BinArgs A) {
                 }
                 npend += lk;
             }
+            more = sp > 0;
+            if (more) pop();
             while (f_tail - f_head >= 64) {
                 eval_far(f_head, 64);
                 f_head += 64;
